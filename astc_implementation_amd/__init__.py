@@ -1,0 +1,219 @@
+"""astc_implementation_amd: ASTC LDR block encoder for AMD Instinct MI355X.
+
+Python view of the C ABI in include/astcg.h (libastcg.so, built in-tree by
+``astc_implementation_amd.build``).  The encoder mirrors the reference's
+``ASTCEncoder`` (code/astc.h:724-748): construct on a device, ``configure``
+(= secondaryInit) for an image size and footprint, then ``encode``.
+
+The library is required: importing succeeds without it, but every entry
+point raises ``RuntimeError`` when libastcg.so is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libastcg.so")
+
+OK = 0
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "invalid footprint", 3: "buffer too small",
+    4: "not configured", 5: "device error", 6: "out of memory", 7: "i/o error", 8: "unsupported",
+}
+
+FOOTPRINTS = [(4, 4), (5, 4), (5, 5), (6, 5), (6, 6), (8, 5), (8, 6),
+              (10, 5), (10, 6), (8, 8), (10, 8), (10, 10), (12, 10), (12, 12)]
+
+
+class AstcError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__("astcg: %s%s" % (STATUS.get(status, "status %d" % status), (": " + msg) if msg else ""))
+
+
+class SymbolicBlock(ctypes.Structure):
+    """SymbolicBlock (code/astc.h:639-655), 432 bytes."""
+    _fields_ = [
+        ("errorval", ctypes.c_float),
+        ("block_mode_index", ctypes.c_uint32),
+        ("partition_count", ctypes.c_uint32),
+        ("partition_index", ctypes.c_uint32),
+        ("partition_formats_matched", ctypes.c_uint32),
+        ("quant_mode", ctypes.c_uint32),
+        ("_padding1", ctypes.c_uint32),
+        ("_padding2", ctypes.c_uint32),
+        ("partition_formats", ctypes.c_uint32 * 4),
+        ("packed_color_values", ctypes.c_uint32 * 32),
+        ("quantized_weights", ctypes.c_uint32 * 64),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("max_partitions", ctypes.c_uint32),
+        ("policy", ctypes.c_uint32),
+        ("chunk_blocks", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 5),
+    ]
+
+
+_lib = None
+
+
+def library():
+    """Loads libastcg.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libastcg.so not built: run astc_implementation_amd.build.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u32, sz, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_size_t, ctypes.c_int
+    sig = {
+        "astcg_default_params": (None, [vp]),
+        "astcg_create": (i32, [i32, ctypes.POINTER(vp)]),
+        "astcg_destroy": (i32, [vp]),
+        "astcg_configure": (i32, [vp, u32, u32, u32, u32, vp]),
+        "astcg_encode": (i32, [vp, vp, sz, vp, sz]),
+        "astcg_encode_device": (i32, [vp, vp, vp, vp]),
+        "astcg_get_symbolic": (i32, [vp, vp, sz]),
+        "astcg_block_counts": (i32, [vp, vp, vp, vp]),
+        "astcg_symbolic_to_physical": (i32, [u32, u32, vp, vp]),
+        "astcg_encode_ise": (i32, [u32, u32, vp, vp, u32]),
+        "astcg_ise_bitcount": (u32, [u32, u32]),
+        "astcg_store_image": (i32, [u32, u32, u32, u32, vp, sz, ctypes.c_char_p]),
+        "astcg_astc_file_in_memory": (i32, [u32, u32, u32, u32, vp, sz, vp, sz, ctypes.POINTER(sz)]),
+        "astcg_decode_image": (i32, [u32, u32, u32, u32, vp, sz, vp, sz]),
+        "astcg_physical_to_symbolic": (i32, [u32, u32, vp, vp]),
+        "astcg_last_error": (ctypes.c_char_p, [vp]),
+        "astcg_status_string": (ctypes.c_char_p, [i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status, enc=None):
+    if status != OK:
+        msg = ""
+        if enc is not None:
+            msg = (library().astcg_last_error(enc) or b"").decode()
+        raise AstcError(status, msg)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Encoder:
+    """ASTCEncoder on one HIP device (code/compression_pipeline.cpp:200-535)."""
+
+    def __init__(self, device=0):
+        lib = library()
+        h = ctypes.c_void_p()
+        _check(lib.astcg_create(device, ctypes.byref(h)))
+        self._h = h
+        self.width = self.height = self.bx = self.by = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().astcg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0):
+        """secondaryInit (compression_pipeline.cpp:263-321)."""
+        p = Params()
+        library().astcg_default_params(ctypes.byref(p))
+        p.max_partitions = max_partitions
+        p.chunk_blocks = chunk_blocks
+        _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
+        self.width, self.height, self.bx, self.by = width, height, bx, by
+
+    @property
+    def num_blocks(self):
+        n = ctypes.c_uint32()
+        _check(library().astcg_block_counts(self._h, None, None, ctypes.byref(n)), self._h)
+        return n.value
+
+    def encode(self, rgba):
+        """encode (compression_pipeline.cpp:323-535): host RGBA8 -> blocks."""
+        rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+        out = np.zeros(self.num_blocks * 16, np.uint8)
+        _check(library().astcg_encode(self._h, _ptr(rgba), rgba.nbytes, _ptr(out), out.nbytes), self._h)
+        return out
+
+    def encode_device(self, d_rgba, d_out, stream=None):
+        """Enqueues an encode of device buffers (integer addresses) on a HIP stream."""
+        _check(library().astcg_encode_device(self._h, ctypes.c_void_p(d_rgba), ctypes.c_void_p(d_out),
+                                             ctypes.c_void_p(stream or 0)), self._h)
+
+    def symbolic(self):
+        n = self.num_blocks
+        arr = (SymbolicBlock * n)()
+        _check(library().astcg_get_symbolic(self._h, arr, n), self._h)
+        return arr
+
+
+def symbolic_to_physical(bx, by, sb):
+    out = (ctypes.c_uint8 * 16)()
+    _check(library().astcg_symbolic_to_physical(bx, by, ctypes.byref(sb), out))
+    return bytes(out)
+
+
+def physical_to_symbolic(bx, by, block):
+    buf = (ctypes.c_uint8 * 16).from_buffer_copy(bytes(block))
+    sb = SymbolicBlock()
+    _check(library().astcg_physical_to_symbolic(bx, by, buf, ctypes.byref(sb)))
+    return sb
+
+
+def encode_ise(quant, values, bit_offset=0, nbytes=16):
+    vals = np.ascontiguousarray(values, dtype=np.uint8)
+    out = np.zeros(nbytes + 1, np.uint8)
+    _check(library().astcg_encode_ise(quant, len(vals), _ptr(vals), _ptr(out), bit_offset))
+    return bytes(out[:nbytes])
+
+
+def ise_bitcount(count, quant):
+    return library().astcg_ise_bitcount(count, quant)
+
+
+def decode_image(bx, by, width, height, blocks):
+    """ASTC LDR decode to RGBA8 (Khronos specification)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    out = np.zeros((height, width, 4), np.uint8)
+    st = library().astcg_decode_image(bx, by, width, height, _ptr(blocks), blocks.nbytes, _ptr(out), out.nbytes)
+    if st not in (OK, 8):
+        _check(st)
+    return out
+
+
+def astc_file_bytes(bx, by, width, height, blocks):
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    out = np.zeros(blocks.nbytes + 16, np.uint8)
+    n = ctypes.c_size_t()
+    _check(library().astcg_astc_file_in_memory(bx, by, width, height, _ptr(blocks), blocks.nbytes, _ptr(out),
+                                               out.nbytes, ctypes.byref(n)))
+    return bytes(out[:n.value])
+
+
+def store_image(bx, by, width, height, blocks, path):
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+    _check(library().astcg_store_image(bx, by, width, height, _ptr(blocks), blocks.nbytes, path.encode()))
+
+
+def psnr(a, b, channels=4):
+    a = a[..., :channels].astype(np.float64)
+    b = b[..., :channels].astype(np.float64)
+    mse = np.mean((a - b) ** 2)
+    return float("inf") if mse == 0 else 10.0 * np.log10(255.0 ** 2 / mse)

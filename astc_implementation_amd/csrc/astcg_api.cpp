@@ -1,0 +1,457 @@
+// C ABI of the gfx950 ASTC encoder (declared in include/astcg.h).
+//
+// The host side mirrors ASTCEncoder (code/compression_pipeline.cpp:200-535):
+// configure() builds the per-footprint metadata and uploads it once; encode()
+// runs the search on the device in chunks of blocks and packs the 128-bit
+// blocks on the device.  Unlike the reference there is no per-batch host
+// round trip: the only device->host copy is the finished block stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/astcg.h"
+#include "astcg_codec.h"
+#include "astcg_device.h"
+#include "astcg_tables.h"
+
+using namespace astcg;
+
+struct astcg_encoder {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    bool configured = false;
+    uint32_t width = 0, height = 0, bx = 0, by = 0, blocks_x = 0, blocks_y = 0, nblocks = 0;
+    astcg_params params{};
+    const Footprint* fp = nullptr;
+    void* d_tables = nullptr;
+    DevTables T{};
+    void* d_chunk = nullptr;
+    ChunkBuffers B{};
+    int chunk_blocks = 0;
+    SymBest* d_best = nullptr;
+    uint8_t* d_rgba = nullptr;
+    uint8_t* d_out = nullptr;
+};
+
+namespace {
+
+astcg_status fail(astcg_encoder* e, astcg_status s, const std::string& msg) {
+    if (e) e->last_error = msg;
+    return s;
+}
+
+astcg_status hip_fail(astcg_encoder* e, hipError_t err, const char* what) {
+    return fail(e, err == hipErrorOutOfMemory ? ASTCG_ERR_OUT_OF_MEMORY : ASTCG_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(err));
+}
+
+void release_image(astcg_encoder* e) {
+    if (e->d_tables) (void)hipFree(e->d_tables);
+    if (e->d_chunk) (void)hipFree(e->d_chunk);
+    if (e->d_best) (void)hipFree(e->d_best);
+    if (e->d_rgba) (void)hipFree(e->d_rgba);
+    if (e->d_out) (void)hipFree(e->d_out);
+    e->d_tables = e->d_chunk = nullptr;
+    e->d_best = nullptr;
+    e->d_rgba = e->d_out = nullptr;
+    e->configured = false;
+}
+
+// Packs all footprint tables into one device allocation.
+struct Layout {
+    size_t size = 0;
+    template <typename T>
+    size_t add(size_t count) {
+        size = (size + 255) & ~size_t(255);
+        const size_t off = size;
+        size += count * sizeof(T);
+        return off;
+    }
+};
+
+hipError_t upload_tables(astcg_encoder* e) {
+    const Footprint& f = *e->fp;
+    const int tc = f.texel_count;
+    // CSR of block modes grouped by decimation mode
+    std::vector<uint16_t> off(f.ndm + 1, 0), list;
+    for (int m = 0; m < f.ndm; ++m) {
+        off[m] = (uint16_t)list.size();
+        for (int k = 0; k < f.nbm; ++k)
+            if (f.bm[k].dm == m) list.push_back((uint16_t)k);
+    }
+    off[f.ndm] = (uint16_t)list.size();
+    std::vector<uint16_t> ordering(3 * 2 * 128);
+    for (int p = 0; p < 3; ++p)
+        for (int v = 0; v < 2; ++v)
+            for (int i = 0; i < 128; ++i) ordering[(p * 2 + v) * 128 + i] = f.ordering[p][v][i];
+    Layout L;
+    const size_t o_dm = L.add<DmInfo>(f.dm.size());
+    const size_t o_bm = L.add<BmInfo>(f.bm.size());
+    const size_t o_off = L.add<uint16_t>(off.size());
+    const size_t o_list = L.add<uint16_t>(std::max<size_t>(list.size(), 1));
+    const size_t o_t2wc = L.add<uint8_t>(f.t2w_count.size());
+    const size_t o_t2wi = L.add<uint8_t>(f.t2w_idx.size());
+    const size_t o_t2ww = L.add<float>(f.t2w_w.size());
+    const size_t o_w2tc = L.add<uint8_t>(f.w2t_count.size());
+    const size_t o_w2to = L.add<uint16_t>(f.w2t_off.size());
+    const size_t o_w2ti = L.add<uint8_t>(std::max<size_t>(f.w2t_idx.size(), 1));
+    const size_t o_w2tw = L.add<float>(std::max<size_t>(f.w2t_w.size(), 1));
+    const size_t o_pot = L.add<uint8_t>(f.part_of_texel.size());
+    const size_t o_seed = L.add<uint16_t>(f.part_seed.size());
+    const size_t o_cnt = L.add<uint8_t>(f.part_counts.size());
+    const size_t o_cov = L.add<uint64_t>(f.coverage.size());
+    const size_t o_ord = L.add<uint16_t>(ordering.size());
+    const size_t o_km = L.add<uint32_t>(64);
+    const size_t o_sin = L.add<float>(f.sin_table.size());
+    const size_t o_cos = L.add<float>(f.cos_table.size());
+    const size_t o_r2b = L.add<int16_t>(f.raw_to_bm.size());
+    const size_t o_rwc = L.add<uint8_t>(f.raw_wc.size());
+    const size_t o_rq = L.add<uint8_t>(f.raw_quant.size());
+    std::vector<uint8_t> host(L.size, 0);
+    auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
+    put(o_dm, f.dm.data(), f.dm.size() * sizeof(DmInfo));
+    put(o_bm, f.bm.data(), f.bm.size() * sizeof(BmInfo));
+    put(o_off, off.data(), off.size() * 2);
+    put(o_list, list.data(), list.size() * 2);
+    put(o_t2wc, f.t2w_count.data(), f.t2w_count.size());
+    put(o_t2wi, f.t2w_idx.data(), f.t2w_idx.size());
+    put(o_t2ww, f.t2w_w.data(), f.t2w_w.size() * 4);
+    put(o_w2tc, f.w2t_count.data(), f.w2t_count.size());
+    put(o_w2to, f.w2t_off.data(), f.w2t_off.size() * 2);
+    put(o_w2ti, f.w2t_idx.data(), f.w2t_idx.size());
+    put(o_w2tw, f.w2t_w.data(), f.w2t_w.size() * 4);
+    put(o_pot, f.part_of_texel.data(), f.part_of_texel.size());
+    put(o_seed, f.part_seed.data(), f.part_seed.size() * 2);
+    put(o_cnt, f.part_counts.data(), f.part_counts.size());
+    put(o_cov, f.coverage.data(), f.coverage.size() * 8);
+    put(o_ord, ordering.data(), ordering.size() * 2);
+    put(o_km, f.kmeans_texels, 64 * 4);
+    put(o_sin, f.sin_table.data(), f.sin_table.size() * 4);
+    put(o_cos, f.cos_table.data(), f.cos_table.size() * 4);
+    put(o_r2b, f.raw_to_bm.data(), f.raw_to_bm.size() * 2);
+    put(o_rwc, f.raw_wc.data(), f.raw_wc.size());
+    put(o_rq, f.raw_quant.data(), f.raw_quant.size());
+    hipError_t err = hipMalloc(&e->d_tables, L.size);
+    if (err != hipSuccess) return err;
+    err = hipMemcpy(e->d_tables, host.data(), L.size, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    uint8_t* b = static_cast<uint8_t*>(e->d_tables);
+    DevTables& T = e->T;
+    T.dm = reinterpret_cast<const DmInfo*>(b + o_dm);
+    T.bm = reinterpret_cast<const BmInfo*>(b + o_bm);
+    T.dm_bm_off = reinterpret_cast<const uint16_t*>(b + o_off);
+    T.dm_bm_list = reinterpret_cast<const uint16_t*>(b + o_list);
+    T.t2w_count = b + o_t2wc;
+    T.t2w_idx = b + o_t2wi;
+    T.t2w_w = reinterpret_cast<const float*>(b + o_t2ww);
+    T.w2t_count = b + o_w2tc;
+    T.w2t_off = reinterpret_cast<const uint16_t*>(b + o_w2to);
+    T.w2t_idx = b + o_w2ti;
+    T.w2t_w = reinterpret_cast<const float*>(b + o_w2tw);
+    T.part_of_texel = b + o_pot;
+    T.part_seed = reinterpret_cast<const uint16_t*>(b + o_seed);
+    T.part_counts = b + o_cnt;
+    T.coverage = reinterpret_cast<const uint64_t*>(b + o_cov);
+    T.ordering = reinterpret_cast<const uint16_t*>(b + o_ord);
+    T.kmeans_texels = reinterpret_cast<const uint32_t*>(b + o_km);
+    T.sin_t = reinterpret_cast<const float*>(b + o_sin);
+    T.cos_t = reinterpret_cast<const float*>(b + o_cos);
+    T.raw_to_bm = reinterpret_cast<const int16_t*>(b + o_r2b);
+    T.raw_wc = b + o_rwc;
+    T.raw_quant = b + o_rq;
+    T.bx = f.bx;
+    T.by = f.by;
+    T.tc = tc;
+    T.ndm = f.ndm;
+    T.nbm = f.nbm;
+    for (int i = 0; i < 4; ++i) T.count_selected[i] = f.count_selected[i];
+    return hipSuccess;
+}
+
+size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
+    Layout L;
+    const size_t npb = (size_t)nb * 4;
+    offs[0] = L.add<int16_t>(npb);
+    offs[1] = L.add<Ideal>(npb);
+    offs[2] = L.add<float>(npb * f.texel_count);
+    offs[3] = L.add<ColorTab>(npb);
+    offs[4] = L.add<Ccr>(npb * f.nbm);
+    offs[5] = L.add<uint8_t>(npb * f.nbm * 64);
+    offs[6] = L.add<int16_t>(npb * 8);
+    offs[7] = L.add<CandOut>(npb * 8);
+    return L.size;
+}
+
+}  // namespace
+
+extern "C" {
+
+void astcg_default_params(astcg_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->max_partitions = 4;
+    p->policy = ASTCG_POLICY_COMPAT;
+    p->chunk_blocks = 0;
+}
+
+astcg_status astcg_create(int hip_device, astcg_encoder** out) {
+    if (!out) return ASTCG_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    hipError_t err = hipGetDeviceCount(&n);
+    if (err != hipSuccess || hip_device < 0 || hip_device >= n) return ASTCG_ERR_DEVICE;
+    auto* e = new astcg_encoder();
+    e->device = hip_device;
+    err = hipSetDevice(hip_device);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err != hipSuccess) {
+        delete e;
+        return ASTCG_ERR_DEVICE;
+    }
+    upload_quant_tables(e->stream);
+    err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess) {
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return ASTCG_ERR_DEVICE;
+    }
+    *out = e;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_destroy(astcg_encoder* e) {
+    if (!e) return ASTCG_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(e->device);
+    release_image(e);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, uint32_t bx, uint32_t by,
+                             const astcg_params* params) {
+    if (!e) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (width == 0 || height == 0 || width >= (1u << 24) || height >= (1u << 24))
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "image dimensions must be in 1..2^24-1");
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    astcg_params p;
+    astcg_default_params(&p);
+    if (params) p = *params;
+    if (p.max_partitions < 1 || p.max_partitions > 4) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "max_partitions must be 1..4");
+    if (p.policy != ASTCG_POLICY_COMPAT) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "unknown partition policy");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    release_image(e);
+    e->width = width;
+    e->height = height;
+    e->bx = bx;
+    e->by = by;
+    e->blocks_x = (width + bx - 1) / bx;
+    e->blocks_y = (height + by - 1) / by;
+    const uint64_t nb = (uint64_t)e->blocks_x * e->blocks_y;
+    if (nb > 0x7FFFFFFFull) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "too many blocks");
+    e->nblocks = (uint32_t)nb;
+    e->params = p;
+    e->fp = footprint((int)bx, (int)by);
+    err = upload_tables(e);
+    if (err != hipSuccess) return hip_fail(e, err, "table upload");
+    // chunk size: keep the per-chunk working set near 2 GiB
+    size_t offs[8];
+    const size_t per_block = chunk_bytes_per_block(*e->fp, offs, 1);
+    int chunk = (int)std::max<size_t>(1024, (size_t(2) << 30) / per_block);
+    if (p.chunk_blocks) chunk = (int)p.chunk_blocks;
+    chunk = std::min<int>(chunk, (int)e->nblocks);
+    e->chunk_blocks = chunk;
+    const size_t total = chunk_bytes_per_block(*e->fp, offs, chunk);
+    err = hipMalloc(&e->d_chunk, total);
+    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
+    uint8_t* b = static_cast<uint8_t*>(e->d_chunk);
+    e->B.pb_part = reinterpret_cast<int16_t*>(b + offs[0]);
+    e->B.ideal = reinterpret_cast<Ideal*>(b + offs[1]);
+    e->B.wts = reinterpret_cast<float*>(b + offs[2]);
+    e->B.ctab = reinterpret_cast<ColorTab*>(b + offs[3]);
+    e->B.ccr = reinterpret_cast<Ccr*>(b + offs[4]);
+    e->B.qw = b + offs[5];
+    e->B.top = reinterpret_cast<int16_t*>(b + offs[6]);
+    e->B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
+    err = hipMalloc(&e->d_best, sizeof(SymBest) * e->nblocks);
+    if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
+    e->configured = true;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_t* d_out, void* stream) {
+    if (!e || !d_rgba || !d_out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "encode before configure");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    ImageArgs A{};
+    A.rgba = d_rgba;
+    A.width = (int)e->width;
+    A.height = (int)e->height;
+    A.blocks_x = (int)e->blocks_x;
+    A.num_blocks = (int)e->nblocks;
+    for (int blk0 = 0; blk0 < (int)e->nblocks; blk0 += e->chunk_blocks) {
+        A.blk0 = blk0;
+        A.nb = std::min(e->chunk_blocks, (int)e->nblocks - blk0);
+        err = launch_chunk(e->T, A, e->B, e->d_best, (int)e->params.max_partitions, s);
+        if (err != hipSuccess) return hip_fail(e, err, "search kernels");
+    }
+    err = launch_pack(e->T, e->d_best, (int)e->nblocks, d_out, s);
+    if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
+    return ASTCG_OK;
+}
+
+astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len, uint8_t* out, size_t out_len) {
+    if (!e || !rgba || !out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "encode before configure");
+    const size_t need_in = (size_t)e->width * e->height * 4, need_out = (size_t)e->nblocks * 16;
+    if (rgba_len < need_in || out_len < need_out) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "buffer too small");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    if (!e->d_rgba) {
+        err = hipMalloc(&e->d_rgba, need_in);
+        if (err != hipSuccess) return hip_fail(e, err, "input buffer");
+        err = hipMalloc(&e->d_out, need_out);
+        if (err != hipSuccess) return hip_fail(e, err, "output buffer");
+    }
+    err = hipMemcpyAsync(e->d_rgba, rgba, need_in, hipMemcpyHostToDevice, e->stream);
+    if (err != hipSuccess) return hip_fail(e, err, "upload");
+    astcg_status st = astcg_encode_device(e, e->d_rgba, e->d_out, e->stream);
+    if (st != ASTCG_OK) return st;
+    err = hipMemcpyAsync(out, e->d_out, need_out, hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess) return hip_fail(e, err, "download");
+    return ASTCG_OK;
+}
+
+astcg_status astcg_get_symbolic(astcg_encoder* e, astcg_symbolic_block* out, size_t count) {
+    if (!e || !out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "not configured");
+    if (count < e->nblocks) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "count below number of blocks");
+    std::vector<SymBest> h(e->nblocks);
+    hipError_t err = hipSetDevice(e->device);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(h.data(), e->d_best, sizeof(SymBest) * h.size(), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(e, err, "symbolic download");
+    for (size_t i = 0; i < h.size(); ++i) {
+        astcg_symbolic_block& s = out[i];
+        std::memset(&s, 0, sizeof(s));
+        s.errorval = h[i].error;
+        s.block_mode_index = h[i].raw_mode;
+        s.partition_count = h[i].pc;
+        s.partition_index = h[i].pc > 1 ? h[i].seed : 0;
+        s.partition_formats_matched = h[i].matched;
+        s.quant_mode = h[i].quant;
+        for (int p = 0; p < 4; ++p) s.partition_formats[p] = (h[i].formats >> (4 * p)) & 0xF;
+        for (int j = 0; j < 32; ++j) s.packed_color_values[j] = h[i].packed[j];
+        for (int j = 0; j < 64; ++j) s.quantized_weights[j] = h[i].qw[j];
+    }
+    return ASTCG_OK;
+}
+
+astcg_status astcg_block_counts(const astcg_encoder* e, uint32_t* bxn, uint32_t* byn, uint32_t* nb) {
+    if (!e) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!e->configured) return ASTCG_ERR_NOT_CONFIGURED;
+    if (bxn) *bxn = e->blocks_x;
+    if (byn) *byn = e->blocks_y;
+    if (nb) *nb = e->nblocks;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_symbolic_to_physical(uint32_t bx, uint32_t by, const astcg_symbolic_block* sb, uint8_t out[16]) {
+    if (!sb || !out) return ASTCG_ERR_INVALID_ARGUMENT;
+    return (astcg_status)symbolic_to_physical_host((int)bx, (int)by, *sb, out);
+}
+
+astcg_status astcg_encode_ise(uint32_t quant, uint32_t count, const uint8_t* in, uint8_t* out, uint32_t bit_offset) {
+    if (!in || !out || quant > 20 || count > 64) return ASTCG_ERR_INVALID_ARGUMENT;
+    static const int levels[21] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256};
+    for (uint32_t i = 0; i < count; ++i)
+        if (in[i] >= levels[quant]) return ASTCG_ERR_INVALID_ARGUMENT;
+    encode_ise_host((int)quant, (int)count, in, out, (int)bit_offset);
+    return ASTCG_OK;
+}
+
+uint32_t astcg_ise_bitcount(uint32_t count, uint32_t quant) { return (uint32_t)ise_bitcount((int)count, (int)quant); }
+
+astcg_status astcg_astc_file_in_memory(uint32_t bx, uint32_t by, uint32_t w, uint32_t h, const uint8_t* data,
+                                       size_t data_len, uint8_t* out, size_t out_cap, size_t* out_len) {
+    if (!data || !out || !out_len) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (out_cap < data_len + 16) return ASTCG_ERR_BUFFER_TOO_SMALL;
+    const uint32_t magic = 0x5CA1AB13u;
+    uint8_t hdr[16] = {(uint8_t)(magic & 0xFF), (uint8_t)((magic >> 8) & 0xFF), (uint8_t)((magic >> 16) & 0xFF),
+                       (uint8_t)(magic >> 24), (uint8_t)bx, (uint8_t)by, 1,
+                       (uint8_t)(w & 0xFF), (uint8_t)((w >> 8) & 0xFF), (uint8_t)((w >> 16) & 0xFF),
+                       (uint8_t)(h & 0xFF), (uint8_t)((h >> 8) & 0xFF), (uint8_t)((h >> 16) & 0xFF), 1, 0, 0};
+    std::memcpy(out, hdr, 16);
+    std::memcpy(out + 16, data, data_len);
+    *out_len = data_len + 16;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_store_image(uint32_t bx, uint32_t by, uint32_t w, uint32_t h, const uint8_t* data, size_t data_len,
+                               const char* path) {
+    if (!data || !path) return ASTCG_ERR_INVALID_ARGUMENT;
+    std::vector<uint8_t> buf(data_len + 16);
+    size_t n = 0;
+    astcg_status s = astcg_astc_file_in_memory(bx, by, w, h, data, data_len, buf.data(), buf.size(), &n);
+    if (s != ASTCG_OK) return s;
+    std::ofstream f(path, std::ios::out | std::ios::binary);
+    if (!f) return ASTCG_ERR_IO;
+    f.write(reinterpret_cast<const char*>(buf.data()), (std::streamsize)n);
+    return f ? ASTCG_OK : ASTCG_ERR_IO;
+}
+
+astcg_status astcg_decode_image(uint32_t bx, uint32_t by, uint32_t w, uint32_t h, const uint8_t* blocks,
+                                size_t blocks_len, uint8_t* rgba, size_t rgba_len) {
+    if (!blocks || !rgba || w == 0 || h == 0) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!valid_footprint((int)bx, (int)by)) return ASTCG_ERR_INVALID_FOOTPRINT;
+    const uint32_t bxn = (w + bx - 1) / bx, byn = (h + by - 1) / by;
+    if (blocks_len < (size_t)bxn * byn * 16 || rgba_len < (size_t)w * h * 4) return ASTCG_ERR_BUFFER_TOO_SMALL;
+    std::vector<uint8_t> tex(bx * by * 4);
+    astcg_status worst = ASTCG_OK;
+    for (uint32_t j = 0; j < byn; ++j)
+        for (uint32_t i = 0; i < bxn; ++i) {
+            const int r = decode_block_rgba8((int)bx, (int)by, blocks + ((size_t)j * bxn + i) * 16, tex.data());
+            if (r != ASTCG_OK) worst = ASTCG_ERR_UNSUPPORTED;
+            for (uint32_t y = 0; y < by; ++y)
+                for (uint32_t x = 0; x < bx; ++x) {
+                    const uint32_t px = i * bx + x, py = j * by + y;
+                    if (px >= w || py >= h) continue;
+                    std::memcpy(rgba + ((size_t)py * w + px) * 4, &tex[(y * bx + x) * 4], 4);
+                }
+        }
+    return worst;
+}
+
+astcg_status astcg_physical_to_symbolic(uint32_t bx, uint32_t by, const uint8_t in[16], astcg_symbolic_block* out) {
+    if (!in || !out) return ASTCG_ERR_INVALID_ARGUMENT;
+    return (astcg_status)physical_to_symbolic_host((int)bx, (int)by, in, out);
+}
+
+const char* astcg_last_error(const astcg_encoder* e) { return e ? e->last_error.c_str() : "null encoder"; }
+
+const char* astcg_status_string(astcg_status s) {
+    switch (s) {
+        case ASTCG_OK: return "ok";
+        case ASTCG_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case ASTCG_ERR_INVALID_FOOTPRINT: return "invalid footprint";
+        case ASTCG_ERR_BUFFER_TOO_SMALL: return "buffer too small";
+        case ASTCG_ERR_NOT_CONFIGURED: return "not configured";
+        case ASTCG_ERR_DEVICE: return "device error";
+        case ASTCG_ERR_OUT_OF_MEMORY: return "out of memory";
+        case ASTCG_ERR_IO: return "i/o error";
+        case ASTCG_ERR_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown";
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Device-side data layout shared by the host orchestration and the gfx950
+// kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "astcg_tables.h"
+
+namespace astcg {
+
+// Read-only per-footprint tables in HBM (one allocation, see upload_tables).
+struct DevTables {
+    const DmInfo* dm;
+    const BmInfo* bm;
+    const uint16_t* dm_bm_off;    // [ndm + 1] CSR over block modes grouped by decimation mode
+    const uint16_t* dm_bm_list;   // [nbm] block-mode trial indices
+    const uint8_t* t2w_count;     // [ndm][tc]
+    const uint8_t* t2w_idx;       // [ndm][tc][4]
+    const float* t2w_w;           // [ndm][tc][4]
+    const uint8_t* w2t_count;     // [ndm][64]
+    const uint16_t* w2t_off;      // [ndm][64]
+    const uint8_t* w2t_idx;       // pool
+    const float* w2t_w;           // pool
+    const uint8_t* part_of_texel; // [3][1024][tc]
+    const uint16_t* part_seed;    // [3][1024]
+    const uint8_t* part_counts;   // [3][1024][4]
+    const uint64_t* coverage;     // [3][1024][4]
+    const uint16_t* ordering;     // [3][2][128]
+    const uint32_t* kmeans_texels;// [64]
+    const float* sin_t;           // [1024][16]
+    const float* cos_t;           // [1024][16]
+    const int16_t* raw_to_bm;     // [2048]
+    const uint8_t* raw_wc;        // [2048]
+    const uint8_t* raw_quant;     // [2048]
+    int bx, by, tc, ndm, nbm;
+    int count_selected[4];
+};
+
+// Image and chunk geometry.
+struct ImageArgs {
+    const uint8_t* rgba;
+    int width, height;
+    int blocks_x, num_blocks;
+    int blk0;  // first block of the chunk
+    int nb;    // blocks in the chunk
+};
+
+// pass01 output (ideal endpoints), per partitioned block; texel weights are
+// stored separately ([pb][tc]).
+struct Ideal {
+    float avg[4][4];
+    float ep0[4][4];
+    float ep1[4][4];
+    float wes[4];        // weight error scale per partition
+    float cutoff;        // min_weight_cutoff
+    float pad[3];
+};
+
+// pass09 (1 partition: [quant][int count]) or pass10 (2-4 partitions:
+// [quant][combined int count]) colour-format table, per partitioned block.
+struct ColorTab {
+    float err[21][13];
+    uint16_t fmt[21][13];  // 4 formats x 4 bits
+};
+
+// pass11 result per (partitioned block, trial block mode).
+struct Ccr {
+    float total_error;
+    int8_t ql, ql_mod;
+    uint16_t formats;  // 4 x 4 bits
+};
+
+// Best refined candidate per (partitioned block, candidate slot).
+struct CandOut {
+    float error;
+    uint16_t bm;       // trial block mode
+    uint8_t matched;
+    uint8_t quant;     // final colour quant level
+    uint16_t formats;  // 4 x 4 bits
+    uint16_t pad;
+    uint8_t packed[32];
+    uint8_t qw[64];
+};
+
+// Best symbolic block so far, per image block.
+struct SymBest {
+    float error;
+    uint16_t raw_mode;
+    uint16_t seed;
+    uint8_t pc, matched, quant, pad;
+    uint16_t formats;
+    uint16_t pad2;
+    uint8_t packed[32];
+    uint8_t qw[64];
+};
+
+struct ChunkBuffers {
+    int16_t* pb_part;   // [4 * max_nb]
+    Ideal* ideal;       // [4 * max_nb]
+    float* wts;         // [4 * max_nb][tc]
+    ColorTab* ctab;     // [4 * max_nb]
+    Ccr* ccr;           // [4 * max_nb][nbm]
+    uint8_t* qw;        // [4 * max_nb][nbm][64]
+    int16_t* top;       // [4 * max_nb][8]
+    CandOut* cand;      // [4 * max_nb][8]
+};
+
+void upload_quant_tables(hipStream_t s);
+hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
+                        hipStream_t s);
+hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s);
+
+}  // namespace astcg

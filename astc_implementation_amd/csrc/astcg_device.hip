@@ -1,0 +1,1671 @@
+// gfx950 kernels of the ASTC block search.
+//
+// Each kernel maps one work item of the reference's WGSL passes to one lane:
+//   k_partition   passes 001-007  (one lane per image block, partition counts 2..4)
+//   k_ideal_color passes 01, 08, 09, 10 (one lane per partitioned block)
+//   k_dm_bm       passes 02-05 then 06, 07, 11 for every block mode of the
+//                 decimation mode (one lane per partitioned block x decimation mode)
+//   k_top         pass 12 (one lane per partitioned block)
+//   k_refine      passes 13-17 x 6 (one lane per partitioned block x candidate)
+//   k_pick        pass 18 + the host-side argmin of compression_pipeline.cpp:499-519
+//   k_pack        symbolic_to_physical (physical_compression.cpp:843-1003)
+// Every reduction the reference performs with float atomics is evaluated here
+// sequentially in texel/weight order, which fixes the summation order.
+// The file must be compiled with -ffp-contract=off.
+#include "astcg_device.h"
+
+namespace astcg {
+
+__constant__ QuantTables c_qt;
+
+void upload_quant_tables(hipStream_t s) {
+    const QuantTables& qt = quant_tables();
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_qt), &qt, sizeof(QuantTables), 0, hipMemcpyHostToDevice, s);
+}
+
+namespace {
+
+constexpr float kErrDefault = 1e37f;
+constexpr int kFreeBits[4] = {111, 97, 94, 91};  // pass07:5
+constexpr int kSteps[12] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
+constexpr int kLevelsD[21] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256};
+
+enum : int {
+    FMT_LUMINANCE = 0, FMT_LUMINANCE_DELTA = 1, FMT_LUMINANCE_ALPHA = 4, FMT_LUMINANCE_ALPHA_DELTA = 5,
+    FMT_RGB_SCALE = 6, FMT_RGB = 8, FMT_RGB_DELTA = 9, FMT_RGB_SCALE_ALPHA = 10, FMT_RGBA = 12, FMT_RGBA_DELTA = 13
+};
+
+__device__ __constant__ float kCW[4] = {0.30f * 2.25f, 0.59f * 2.25f, 0.11f * 2.25f, 1.0f};  // astc.h:60-63
+
+// ---- WGSL operation semantics --------------------------------------------
+__device__ __forceinline__ float wmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float wmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float wclamp(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float wround(float x) { return rintf(x); }
+__device__ __forceinline__ int32_t wf2i(float v) {
+    if (!(v < 2147483520.0f)) return 2147483647;
+    if (v < -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)v;
+}
+__device__ __forceinline__ uint32_t wf2u(float v) {
+    if (!(v < 4294967040.0f)) return 0xFFFFFFFFu;
+    if (v < 0.0f) return 0u;
+    return (uint32_t)v;
+}
+__device__ __forceinline__ float dot4(const float* a, const float* b) {
+    float s = a[0] * b[0];
+    s = s + a[1] * b[1];
+    s = s + a[2] * b[2];
+    return s + a[3] * b[3];
+}
+__device__ __forceinline__ float dot3(const float* a, const float* b) {
+    float s = a[0] * b[0];
+    s = s + a[1] * b[1];
+    return s + a[2] * b[2];
+}
+__device__ __forceinline__ void normalize4(const float* v, float* o) {
+    const float l = sqrtf(dot4(v, v));
+    o[0] = v[0] / l; o[1] = v[1] / l; o[2] = v[2] / l; o[3] = v[3] / l;
+}
+__device__ __forceinline__ void normalize3(const float* v, float* o) {
+    const float l = sqrtf(dot3(v, v));
+    o[0] = v[0] / l; o[1] = v[1] / l; o[2] = v[2] / l;
+}
+
+// Single-precision atan2 (Cephes atanf range reduction + polynomial); the
+// same sequence of IEEE operations runs in the host checker.
+__device__ float atanf_fixed(float x) {
+    float sign = 1.0f;
+    if (x < 0.0f) { sign = -1.0f; x = -x; }
+    float y;
+    if (x > 2.414213562373095f) { y = 1.5707963267948966f; x = -1.0f / x; }
+    else if (x > 0.4142135623730950f) { y = 0.7853981633974483f; x = (x - 1.0f) / (x + 1.0f); }
+    else y = 0.0f;
+    const float z = x * x;
+    float p = 8.05374449538e-2f;
+    p = p * z - 1.38776856032e-1f;
+    p = p * z + 1.99777106478e-1f;
+    p = p * z - 3.33329491539e-1f;
+    y = y + (p * z * x + x);
+    return sign * y;
+}
+__device__ float atan2f_fixed(float y, float x) {
+    if (x != x || y != y) return x + y;
+    if (x == 0.0f) return y > 0.0f ? 1.5707963267948966f : (y < 0.0f ? -1.5707963267948966f : 0.0f);
+    float w = 0.0f;
+    if (x < 0.0f) w = (y < 0.0f) ? -3.141592653589793f : 3.141592653589793f;
+    return w + atanf_fixed(y / x);
+}
+
+// ---- block access ----------------------------------------------------------
+struct Pb {
+    int g;          // image block index
+    int pc;         // partition count
+    int entry;      // packed partition table entry (pc > 1)
+    const uint8_t* tp;  // texel partitions (nullptr: all 0)
+    int ppc[4];
+    int seed;
+    int pb_in_batch;
+};
+
+__device__ __forceinline__ void texel_px(const ImageArgs& A, const DevTables& T, int g, int i, float* o) {
+    const int bxi = g % A.blocks_x, byi = g / A.blocks_x;
+    const int dy = i / T.bx, dx = i - dy * T.bx;
+    const int x = min(bxi * T.bx + dx, A.width - 1), y = min(byi * T.by + dy, A.height - 1);
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(A.rgba + ((size_t)y * A.width + x) * 4);
+    o[0] = c_qt.pixel_scale[v & 0xFF];
+    o[1] = c_qt.pixel_scale[(v >> 8) & 0xFF];
+    o[2] = c_qt.pixel_scale[(v >> 16) & 0xFF];
+    o[3] = c_qt.pixel_scale[v >> 24];
+}
+
+__device__ __forceinline__ int texel_part(const Pb& b, int i) { return b.tp ? b.tp[i] : 0; }
+
+__device__ Pb make_pb(const ImageArgs& A, const DevTables& T, const int16_t* pb_part, int pc, int pb) {
+    Pb b;
+    const int npos = pc == 1 ? 1 : 4;
+    const int k = pb / npos, c = pb - k * npos;
+    b.g = A.blk0 + k;
+    b.pc = pc;
+    const int j = b.g % kRefBatch;
+    b.pb_in_batch = pc == 1 ? j : j * 4 + c;
+    if (pc == 1) {
+        b.entry = 0;
+        b.tp = nullptr;
+        b.ppc[0] = T.tc; b.ppc[1] = b.ppc[2] = b.ppc[3] = 0;
+        b.seed = 0;
+    } else {
+        const int e = pb_part[pb];
+        b.entry = e;
+        const size_t row = (size_t)(pc - 2) * kPartitionings + e;
+        b.tp = T.part_of_texel + row * T.tc;
+        for (int i = 0; i < 4; ++i) b.ppc[i] = T.part_counts[row * 4 + i];
+        b.seed = T.part_seed[row];
+    }
+    return b;
+}
+
+__device__ __forceinline__ uint16_t pack_fmt4(const int* f) {
+    return (uint16_t)((f[0] & 0xF) | ((f[1] & 0xF) << 4) | ((f[2] & 0xF) << 8) | ((f[3] & 0xF) << 12));
+}
+__device__ __forceinline__ int fmt_of(uint16_t v, int p) { return (v >> (4 * p)) & 0xF; }
+
+// ---- passes 001-007: compat partition candidates ---------------------------
+__device__ float dist_sq(const float* a, const float* b) {
+    float d[4];
+    for (int c = 0; c < 4; ++c) { const float t = a[c] - b[c]; d[c] = t * t; }
+    return dot4(d, kCW);
+}
+
+// pass001-003: texel -> cluster after the reference's 4 k-means iterations
+__device__ void kmeans_assign(const ImageArgs& A, const DevTables& T, int g, int pc, uint8_t* asg) {
+    const int tc = T.tc;
+    float centers[4][4] = {};
+    texel_px(A, T, g, 145897 % tc, centers[0]);
+    float dist[kMaxTexels];
+    for (int t = 0; t < tc; ++t) { float px[4]; texel_px(A, T, g, t, px); dist[t] = dist_sq(centers[0], px); }
+    for (int p = 1; p < pc; ++p) {
+        uint32_t mx = 0;
+        for (int t = 0; t < tc; ++t) mx = max(mx, __float_as_uint(dist[t]));
+        int idx = 0;
+        for (int t = 0; t < tc; ++t) if (__float_as_uint(dist[t]) == mx) idx = t;
+        texel_px(A, T, g, idx, centers[p]);
+        if (p < pc - 1)
+            for (int t = 0; t < tc; ++t) { float px[4]; texel_px(A, T, g, t, px); dist[t] = wmin(dist[t], dist_sq(centers[p], px)); }
+    }
+    for (int it = 0; it < 4; ++it) {
+        int counts[4] = {0, 0, 0, 0};
+        for (int t = 0; t < tc; ++t) {
+            float px[4];
+            texel_px(A, T, g, t, px);
+            float bd = 1e30f;
+            int bp = 0;
+            for (int p = 0; p < pc; ++p) {
+                const float d = dist_sq(px, centers[p]);
+                if (d < bd) { bd = d; bp = p; }
+            }
+            asg[t] = (uint8_t)bp;
+            counts[bp]++;
+        }
+        bool problem = false;
+        for (int p = 0; p < pc; ++p) problem |= counts[p] == 0;
+        if (problem) for (int p = 0; p < pc; ++p) asg[p] = (uint8_t)p;
+        uint32_t sums[4][4] = {}, cnt[4] = {};
+        for (int t = 0; t < tc; ++t) {
+            float px[4];
+            texel_px(A, T, g, t, px);
+            const int p = asg[t];
+            for (int c = 0; c < 4; ++c) sums[p][c] += wf2u(px[c]);
+            cnt[p]++;
+        }
+        for (int p = 0; p < pc; ++p)
+            for (int c = 0; c < 4; ++c) centers[p][c] = cnt[p] > 0 ? (float)sums[p][c] / (float)cnt[p] : 0.0f;
+    }
+}
+
+// pass004: minimum-over-permutations coverage mismatch for one partitioning
+__device__ uint32_t mismatch_count(const DevTables& T, const uint8_t* asg, int pc, int entry) {
+    uint64_t a[4] = {0, 0, 0, 0};
+    for (int l = 0; l < 64; ++l) a[asg[T.kmeans_texels[l]]] |= 1ull << l;
+    const uint64_t* bb = T.coverage + ((size_t)(pc - 2) * kPartitionings + entry) * 4;
+    auto P = [&](int i, int j) { return __popcll(a[i] ^ bb[j]); };
+    if (pc == 2) return (uint32_t)(min(P(0, 0) + P(1, 1), P(0, 1) + P(1, 0)) / 2);
+    if (pc == 3) {
+        const int v0 = min(P(1, 1) + P(2, 2), P(1, 2) + P(2, 1)) + P(0, 0);
+        const int v1 = min(P(1, 0) + P(2, 2), P(1, 2) + P(2, 0)) + P(0, 1);
+        const int v2 = min(P(1, 0) + P(2, 1), P(1, 1) + P(2, 0)) + P(0, 2);
+        return (uint32_t)(min(v0, min(v1, v2)) / 2);
+    }
+    const int mx23 = min(P(2, 2) + P(3, 3), P(2, 3) + P(3, 2));
+    const int mx13 = min(P(2, 1) + P(3, 3), P(2, 3) + P(3, 1));
+    const int mx12 = min(P(2, 1) + P(3, 2), P(2, 2) + P(3, 1));
+    const int mx03 = min(P(2, 0) + P(3, 3), P(2, 3) + P(3, 0));
+    const int mx02 = min(P(2, 0) + P(3, 2), P(2, 2) + P(3, 0));
+    const int mx01 = min(P(2, 1) + P(3, 0), P(2, 0) + P(3, 1));
+    const int v0 = P(0, 0) + min(P(1, 1) + mx23, min(P(1, 2) + mx13, P(1, 3) + mx12));
+    const int v1 = P(0, 1) + min(P(1, 0) + mx23, min(P(1, 2) + mx03, P(1, 3) + mx02));
+    const int v2 = P(0, 2) + min(P(1, 1) + mx03, min(P(1, 0) + mx13, P(1, 3) + mx01));
+    const int v3 = P(0, 3) + min(P(1, 1) + mx02, min(P(1, 2) + mx01, P(1, 0) + mx12));
+    return (uint32_t)(min(v0, min(v1, min(v2, v3))) / 2);
+}
+
+// pass006: line-fit errors of one candidate partitioning, with the
+// reference's per-invocation avg[]/dir[] arrays (only lane p holds
+// partition p's average and direction).
+__device__ void eval_candidate(const ImageArgs& A, const DevTables& T, int g, int pc, int entry, float* eu,
+                               float* es) {
+    const int tc = T.tc;
+    const size_t row = (size_t)(pc - 2) * kPartitionings + entry;
+    const uint8_t* pot = T.part_of_texel + row * tc;
+    uint32_t sums[4][4] = {}, cnt[4] = {};
+    for (int t = 0; t < tc; ++t) {
+        float px[4];
+        texel_px(A, T, g, t, px);
+        const int p = pot[t];
+        cnt[p]++;
+        for (int c = 0; c < 4; ++c) sums[p][c] += wf2u(px[c]);
+    }
+    float avg[4][4] = {};
+    bool has_avg[4] = {false, false, false, false};
+    for (int p = 0; p < pc; ++p)
+        if ((float)cnt[p] > 0.0f) {
+            const float n = (float)cnt[p];
+            for (int c = 0; c < 4; ++c) avg[p][c] = (float)sums[p][c] / n;
+            has_avg[p] = true;
+        }
+    int32_t axis[4][4][4] = {};
+    for (int t = 0; t < tc; ++t) {
+        float px[4];
+        texel_px(A, T, g, t, px);
+        const int p = pot[t];
+        const bool own = p == t && t < pc && has_avg[p];
+        float datum[4];
+        for (int c = 0; c < 4; ++c) datum[c] = px[c] - (own ? avg[p][c] : 0.0f);
+        int32_t di[4];
+        for (int c = 0; c < 4; ++c) di[c] = wf2i(datum[c]);
+        for (int ax = 0; ax < 4; ++ax)
+            if (datum[ax] > 0.0f)
+                for (int c = 0; c < 4; ++c) axis[p][ax][c] = (int32_t)((uint32_t)axis[p][ax][c] + (uint32_t)di[c]);
+    }
+    float dir[4][4] = {};
+    for (int p = 0; p < pc; ++p) {
+        float s[4][4];
+        for (int ax = 0; ax < 4; ++ax)
+            for (int c = 0; c < 4; ++c) s[ax][c] = (float)axis[p][ax][c];
+        int best = 0;
+        float mp = dot4(s[0], s[0]);
+        for (int ax = 1; ax < 4; ++ax) {
+            const float pr = dot4(s[ax], s[ax]);
+            if (pr > mp) { mp = pr; best = ax; }
+        }
+        normalize4(s[best], dir[p]);
+    }
+    uint32_t usum = 0, ssum = 0;
+    int32_t lmin[4] = {2147483647, 2147483647, 2147483647, 2147483647};
+    int32_t lmax[4] = {-2147483647, -2147483647, -2147483647, -2147483647};
+    for (int t = 0; t < tc; ++t) {
+        float px[4];
+        texel_px(A, T, g, t, px);
+        const int p = pot[t];
+        const bool own = p == t && t < pc;
+        float ud[4], av[4];
+        for (int c = 0; c < 4; ++c) {
+            ud[c] = own ? dir[p][c] : 0.0f;
+            av[c] = (own && has_avg[p]) ? avg[p][c] : 0.0f;
+        }
+        const float uparam = dot4(px, ud);
+        float dm[4];
+        for (int c = 0; c < 4; ++c) dm[c] = px[c] - av[c];
+        const float s = dot4(dm, ud);
+        float diff[4];
+        for (int c = 0; c < 4; ++c) { const float d = px[c] - (av[c] + ud[c] * s); diff[c] = d * d; }
+        const float udist = dot4(diff, kCW);
+        float sd[4];
+        normalize4(av, sd);
+        const float ss = dot4(px, sd);
+        for (int c = 0; c < 4; ++c) { const float d = px[c] - sd[c] * ss; diff[c] = d * d; }
+        const float sdist = dot4(diff, kCW);
+        usum += wf2u(udist);
+        ssum += wf2u(sdist);
+        const int32_t pi = wf2i(uparam * 4096.0f);
+        lmin[p] = min(lmin[p], pi);
+        lmax[p] = max(lmax[p], pi);
+    }
+    float tu = (float)usum, ts = (float)ssum;
+    float wie = 0.055f;
+    if (tc <= 20) wie = 0.03f;
+    else if (tc <= 31) wie = 0.04f;
+    else if (tc <= 41) wie = 0.05f;
+    wie = wie * wie;
+    const uint8_t* cnts = T.part_counts + row * 4;
+    for (int p = 0; p < pc; ++p) {
+        const float minp = (float)lmin[p] / 4096.0f, maxp = (float)lmax[p] / 4096.0f;
+        const float len = wmax(maxp - minp, 1e-7f);
+        const float ew = (float)cnts[p] * wie;
+        float a0[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (p == 0 && has_avg[0]) for (int c = 0; c < 4; ++c) a0[c] = avg[0][c];
+        tu = tu + ew * (len * len);
+        ts = ts + ew * dot4(a0, a0);
+    }
+    *eu = tu;
+    *es = ts;
+}
+
+__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.nb) return;
+    const int g = A.blk0 + k;
+    const int j = g % kRefBatch, base = g - j;
+    const int B = min(kRefBatch, A.num_blocks - base);
+    int pos[4] = {0, 1, 2, 3};
+    // partition_ordering and the partitioning errors are allocated 920*128
+    // entries but indexed with stride 512, so in-batch blocks >= 230 read
+    // zeros and keep table positions 0..3.
+    if (j * 512 + 127 < kRefBatch * 128) {
+        // pass004 writes slot block*1024*i: the score this block reads is
+        // mismatch(j, partitioning 1), or mismatch(B-1, partitioning 0) for j == 0
+        uint8_t asg[kMaxTexels];
+        const int src = j == 0 ? base + B - 1 : g;
+        kmeans_assign(A, T, src, pc, asg);
+        const uint32_t v = mismatch_count(T, asg, pc, j == 0 ? 0 : 1);
+        const uint16_t* ord = T.ordering + ((pc - 2) * 2 + (v > 0 ? 1 : 0)) * 128;
+        int bui[4] = {0, 0, 0, 0}, bsi[4] = {0, 0, 0, 0};
+        float bue[4] = {1e30f, 1e30f, 1e30f, 1e30f}, bse[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+        for (int i = 0; i < 128; ++i) {
+            float eu, es;
+            eval_candidate(A, T, g, pc, ord[i], &eu, &es);
+            if (eu < bue[3]) {
+                bue[3] = eu; bui[3] = i;
+                for (int q = 3; q > 0; --q)
+                    if (bue[q] < bue[q - 1]) { float t = bue[q]; bue[q] = bue[q - 1]; bue[q - 1] = t; int u = bui[q]; bui[q] = bui[q - 1]; bui[q - 1] = u; }
+            }
+            if (es < bse[3]) {
+                bse[3] = es; bsi[3] = i;
+                for (int q = 3; q > 0; --q)
+                    if (bse[q] < bse[q - 1]) { float t = bse[q]; bse[q] = bse[q - 1]; bse[q - 1] = t; int u = bsi[q]; bsi[q] = bsi[q - 1]; bsi[q - 1] = u; }
+            }
+        }
+        int emitted = 0;
+        for (int i = 0; i < 8 && emitted < 4; ++i) {
+            const int idx = (i & 1) ? bsi[i >> 1] : bui[i >> 1];
+            bool dup = false;
+            for (int q = 0; q < emitted; ++q) dup |= pos[q] == idx;
+            if (!dup) pos[emitted++] = idx;
+        }
+        // pass007 uses the candidate position itself as the packed table index
+    }
+    for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)pos[c];
+}
+
+// ---- pass01 + pass08 + pass09 + pass10 --------------------------------------
+__global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
+                                                    Ideal* ideal_out, float* wts_out, ColorTab* ctab) {
+    const int pb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pb >= npb) return;
+    const Pb b = make_pb(A, T, pb_part, pc, pb);
+    const int tc = T.tc;
+    // pass01_ideal_endpoints_and_weights.wgsl:63-207
+    const float error_weight = (((kCW[0] + kCW[1]) + kCW[2]) + kCW[3]) / 4.0f;
+    float sum[4][4] = {}, count[4] = {};
+    bool gray = true;
+    float amin = 1e38f, amax = 0.0f;
+    for (int i = 0; i < tc; ++i) {
+        float px[4];
+        texel_px(A, T, b.g, i, px);
+        const int p = texel_part(b, i);
+        for (int c = 0; c < 4; ++c) sum[p][c] += px[c];
+        count[p] += 1.0f;
+        gray = gray && (px[0] == px[1] && px[0] == px[2]);
+        amax = fmaxf(amax, px[3]);
+        amin = fminf(amin, px[3]);
+    }
+    float avg[4][4];
+    for (int p = 0; p < 4; ++p)
+        for (int c = 0; c < 4; ++c) avg[p][c] = count[p] > 0.0f ? sum[p][c] / count[p] : 0.0f;
+    float dv[4][4] = {};
+    for (int i = 0; i < tc; ++i) {
+        float px[4];
+        texel_px(A, T, b.g, i, px);
+        const int p = texel_part(b, i);
+        for (int c = 0; c < 4; ++c) { const float d = wmax(px[c] - avg[p][c], 0.0f); dv[p][c] += d * d; }
+    }
+    float dir[4][4], minp[4], maxp[4];
+    for (int p = 0; p < 4; ++p) { normalize4(dv[p], dir[p]); minp[p] = 1e10f; maxp[p] = -1e10f; }
+    for (int i = 0; i < tc; ++i) {
+        float px[4], df[4];
+        texel_px(A, T, b.g, i, px);
+        const int p = texel_part(b, i);
+        for (int c = 0; c < 4; ++c) df[c] = px[c] - avg[p][c];
+        const float proj = dot4(df, dir[p]);
+        minp[p] = wmin(minp[p], proj);
+        maxp[p] = wmax(maxp[p], proj);
+    }
+    Ideal I;
+    float lensq[4];
+    for (int p = 0; p < 4; ++p) {
+        if (minp[p] >= maxp[p]) { minp[p] = 0.0f; maxp[p] = 1e-7f; }
+        const float len = maxp[p] - minp[p];
+        lensq[p] = len * len;
+        const bool zero = dir[p][0] == 0.0f && dir[p][1] == 0.0f && dir[p][2] == 0.0f && dir[p][3] == 0.0f;
+        for (int c = 0; c < 4; ++c) {
+            I.avg[p][c] = avg[p][c];
+            I.ep0[p][c] = zero ? avg[p][c] : avg[p][c] + dir[p][c] * minp[p];
+            I.ep1[p][c] = zero ? avg[p][c] : avg[p][c] + dir[p][c] * maxp[p];
+        }
+        I.wes[p] = lensq[p] * error_weight;
+    }
+    float* wts = wts_out + (size_t)pb * tc;
+    for (int i = 0; i < tc; ++i) {
+        float px[4], df[4];
+        texel_px(A, T, b.g, i, px);
+        const int p = texel_part(b, i);
+        for (int c = 0; c < 4; ++c) df[c] = px[c] - avg[p][c];
+        const float proj = dot4(df, dir[p]);
+        const float span = wmax(maxp[p] - minp[p], 1e-6f);
+        wts[i] = wclamp((proj - minp[p]) / span, 0.0f, 1.0f);
+    }
+    float min_ep[4] = {10.0f, 10.0f, 10.0f, 10.0f};
+    for (int p = 0; p < 4; ++p)
+        for (int c = 0; c < 4; ++c) {
+            const float ep = (1.0f - I.ep0[p][c]) / ((I.ep1[p][c] - I.ep0[p][c]) + 1e-9f);
+            if (ep > 0.5f && ep < min_ep[c]) min_ep[c] = ep;
+        }
+    I.cutoff = wmin(min_ep[0], wmin(min_ep[1], wmin(min_ep[2], min_ep[3])));
+    ideal_out[pb] = I;
+
+    // pass08_compute_encoding_choice_errors.wgsl:115-326
+    float sxp[4][4] = {}, syp[4][4] = {}, szp[4][4] = {};
+    for (int i = 0; i < tc; ++i) {
+        float px[4], t[4];
+        texel_px(A, T, b.g, i, px);
+        const int p = texel_part(b, i);
+        for (int c = 0; c < 4; ++c) t[c] = px[c] - avg[p][c];
+        t[3] = 0.0f;
+        if (t[0] > 0.0f) for (int c = 0; c < 4; ++c) sxp[p][c] += t[c];
+        if (t[1] > 0.0f) for (int c = 0; c < 4; ++c) syp[p][c] += t[c];
+        if (t[2] > 0.0f) for (int c = 0; c < 4; ++c) szp[p][c] += t[c];
+    }
+    float ub[4][4], ua[4][4], sb[4][4], la[4][4], lb[4];
+    const float one[4] = {1.0f, 1.0f, 1.0f, 0.0f};
+    normalize4(one, lb);
+    for (int p = 0; p < pc; ++p) {
+        const float* best = sxp[p];
+        float bs = dot4(sxp[p], sxp[p]);
+        const float py = dot4(syp[p], syp[p]), pz = dot4(szp[p], szp[p]);
+        if (py > bs) { best = syp[p]; bs = py; }
+        if (pz > bs) { best = szp[p]; bs = pz; }
+        const float a3[4] = {avg[p][0], avg[p][1], avg[p][2], 0.0f};
+        normalize4(best, ub[p]);
+        const float s = dot4(a3, ub[p]);
+        for (int c = 0; c < 4; ++c) ua[p][c] = a3[c] - ub[p][c] * s;
+        normalize4(a3, sb[p]);
+        const float sl = dot4(a3, lb);
+        for (int c = 0; c < 4; ++c) la[p][c] = a3[c] - lb[c] * sl;
+    }
+    float acc[4][5] = {};
+    for (int i = 0; i < tc; ++i) {
+        float px[4];
+        texel_px(A, T, b.g, i, px);
+        const int p = texel_part(b, i);
+        const float ad = px[3] - 65536.0f;
+        acc[p][0] += ad * ad * kCW[3];
+        float d[3];
+        float s = dot3(px, ub[p]);
+        for (int c = 0; c < 3; ++c) { const float t = (ua[p][c] + s * ub[p][c]) - px[c]; d[c] = t * t; }
+        acc[p][1] += dot3(d, kCW);
+        s = dot3(px, sb[p]);
+        for (int c = 0; c < 3; ++c) { const float t = s * sb[p][c] - px[c]; d[c] = t * t; }
+        acc[p][2] += dot3(d, kCW);
+        s = dot3(px, lb);
+        for (int c = 0; c < 3; ++c) { const float t = (la[p][c] + s * lb[c]) - px[c]; d[c] = t * t; }
+        acc[p][3] += dot3(d, kCW);
+        for (int c = 0; c < 3; ++c) { const float t = s * lb[c] - px[c]; d[c] = t * t; }
+        acc[p][4] += dot3(d, kCW);
+    }
+    const bool constant_alpha = amax == amin;
+    const int can_bc = !(gray && constant_alpha);
+
+    // pass09_compute_color_error.wgsl:140-268
+    float err[4][21][4];
+    int fmt[4][21][4];
+    const float lim = 0.12f * 65536.0f;
+    for (int p = 0; p < pc; ++p) {
+        const float rse = (acc[p][2] - acc[p][1]) * 0.7f;
+        const float le = (acc[p][4] - acc[p][1]) * 3.0f;
+        const float ade = acc[p][0] * 3.0f;
+        bool can_oe = true;
+        for (int c = 0; c < 3; ++c) can_oe = can_oe && (fabsf(I.ep1[p][c] - I.ep0[p][c]) < lim);
+        const float size = (float)b.ppc[p];
+        float sre[4];
+        for (int c = 0; c < 4; ++c) {
+            const float h0 = wmax(I.ep0[p][c] - 65535.0f, 0.0f), h1 = wmax(I.ep1[p][c] - 65535.0f, 0.0f);
+            const float l0 = wmin(I.ep0[p][c], 0.0f), l1 = wmin(I.ep1[p][c], 0.0f);
+            sre[c] = (((l0 * l0) + (l1 * l1)) + (h0 * h0)) + (h1 * h1);
+        }
+        const float rgb_range = dot3(sre, kCW) * 0.5f * size;
+        const float alpha_range = sre[3] * kCW[3] * 0.5f * size;
+        const float base_rgb = ((kCW[0] + kCW[1]) + kCW[2]) * size;
+        const float base_a = kCW[3] * size;
+        const float bc_rgba = can_bc ? 0.625f : 1.0f, bc_rgb = can_bc ? 0.5f : 1.0f;
+        const float oe_rgba0 = can_oe ? 0.5f : 1.0f, oe_rgb0 = can_oe ? 0.25f : 1.0f;
+        for (int q = 0; q < 4; ++q) {
+            for (int k = 0; k < 4; ++k) err[p][q][k] = kErrDefault;
+            fmt[p][q][3] = FMT_RGBA; fmt[p][q][2] = FMT_RGB; fmt[p][q][1] = FMT_RGB_SCALE; fmt[p][q][0] = FMT_LUMINANCE;
+        }
+        const int K[17] = {5, 7, 9, 11, 15, 19, 23, 31, 39, 47, 63, 79, 95, 127, 159, 191, 255};
+        for (int i = 0; i < 17; ++i) {
+            const int q = i + 4;
+            const float oe_rgba = q >= 19 ? 1.0f : oe_rgba0, oe_rgb = q >= 19 ? 1.0f : oe_rgb0;
+            const float bqe = (65536.0f * 65536.0f / 18.0f) / (float)(K[i] * K[i]);
+            const float qe_rgb = base_rgb * bqe, qe_rgba = (base_rgb + base_a) * bqe;
+            err[p][q][3] = ((qe_rgba * bc_rgba * oe_rgba) + rgb_range) + alpha_range;
+            fmt[p][q][3] = FMT_RGBA;
+            const float full = ((qe_rgb * bc_rgb * oe_rgb) + rgb_range) + ade;
+            const float rgbs_a = ((qe_rgba + rse) + rgb_range) + alpha_range;
+            if (rgbs_a < full) { err[p][q][2] = rgbs_a; fmt[p][q][2] = FMT_RGB_SCALE_ALPHA; }
+            else { err[p][q][2] = full; fmt[p][q][2] = FMT_RGB; }
+            const float rgbs = ((qe_rgb + rgb_range) + ade) + rse;
+            const float lumal = ((qe_rgba + rgb_range) + alpha_range) + le;
+            if (rgbs < lumal) { err[p][q][1] = rgbs; fmt[p][q][1] = FMT_RGB_SCALE; }
+            else { err[p][q][1] = lumal; fmt[p][q][1] = FMT_LUMINANCE_ALPHA; }
+            err[p][q][0] = ((qe_rgb + rgb_range) + ade) + le;
+            fmt[p][q][0] = FMT_LUMINANCE;
+        }
+    }
+    ColorTab* ct = ctab + pb;
+    if (pc == 1) {
+        for (int q = 0; q < 21; ++q)
+            for (int k = 0; k < 4; ++k) {
+                ct->err[q][k] = err[0][q][k];
+                ct->fmt[q][k] = (uint16_t)fmt[0][q][k];
+            }
+        return;
+    }
+    // pass10_color_combinations_for_quant_{2,3,4}part.wgsl
+    for (int q = 0; q < 21; ++q)
+        for (int k = 0; k < 13; ++k) { ct->err[q][k] = kErrDefault; ct->fmt[q][k] = 0; }
+    for (int q = 4; q < 21; ++q) {
+        float ce[13];
+        uint16_t cf[13];
+        for (int k = 0; k < 13; ++k) { ce[k] = kErrDefault; cf[k] = 0; }
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                const int lo2 = min(i, j), hi2 = max(i, j);
+                if (hi2 - lo2 > 1) continue;
+                if (pc == 2) {
+                    const float te = wmin(err[0][q][i] + err[1][q][j], 1e10f);
+                    if (te < ce[i + j]) {
+                        const int f[4] = {fmt[0][q][i], fmt[1][q][j], 0, 0};
+                        ce[i + j] = te; cf[i + j] = pack_fmt4(f);
+                    }
+                    continue;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const int lo3 = min(k, lo2), hi3 = max(k, hi2);
+                    if (hi3 - lo3 > 1) continue;
+                    if (pc == 3) {
+                        const float te = wmin((err[0][q][i] + err[1][q][j]) + err[2][q][k], 1e10f);
+                        if (te < ce[i + j + k]) {
+                            const int f[4] = {fmt[0][q][i], fmt[1][q][j], fmt[2][q][k], 0};
+                            ce[i + j + k] = te; cf[i + j + k] = pack_fmt4(f);
+                        }
+                        continue;
+                    }
+                    for (int l = 0; l < 4; ++l) {
+                        const int lo4 = min(l, lo3), hi4 = max(l, hi3);
+                        if (hi4 - lo4 > 1) continue;
+                        const float te = wmin(((err[0][q][i] + err[1][q][j]) + err[2][q][k]) + err[3][q][l], 1e10f);
+                        if (te < ce[i + j + k + l]) {
+                            const int f[4] = {fmt[0][q][i], fmt[1][q][j], fmt[2][q][k], fmt[3][q][l]};
+                            ce[i + j + k + l] = te; cf[i + j + k + l] = pack_fmt4(f);
+                        }
+                    }
+                }
+            }
+        for (int k = 0; k < 13; ++k) { ct->err[q][k] = ce[k]; ct->fmt[q][k] = cf[k]; }
+    }
+}
+
+// ---- passes 02-05, 06, 07, 11 -----------------------------------------------
+__global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
+                                              const Ideal* ideal,
+                                              const float* wts_all, const ColorTab* ctab, Ccr* ccr, uint8_t* qw_out) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= npb * T.ndm) return;
+    const int pb = tid / T.ndm, m = tid - pb * T.ndm;
+    const Ideal& I = ideal[pb];
+    const float* wts = wts_all + (size_t)pb * T.tc;
+    const DmInfo di = T.dm[m];
+    const int tc = di.texel_count, wc = di.weight_count;
+    const uint8_t* t2wc = T.t2w_count + (size_t)m * tc;
+    const uint8_t* t2wi = T.t2w_idx + (size_t)m * tc * 4;
+    const float* t2ww = T.t2w_w + (size_t)m * tc * 4;
+    const uint8_t* w2tc = T.w2t_count + m * 64;
+    const uint16_t* w2to = T.w2t_off + m * 64;
+    const uint8_t* w2ti = T.w2t_idx + di.w2t_base;
+    const float* w2tw = T.w2t_w + di.w2t_base;
+    // weight error scale of a texel = that of its partition (pass01 stores
+    // lengthSqr[p] * error_weight for every texel of partition p)
+    const Pb b = make_pb(A, T, pb_part, pc, pb);
+    const int pb_in_batch = b.pb_in_batch;
+    auto wes_of = [&](int t) -> float { return I.wes[texel_part(b, t)]; };
+    float dw[kMaxWeights];
+    // pass02_decimated_weights.wgsl:132-270
+    if (tc == wc) {
+        for (int i = 0; i < wc; ++i) dw[i] = wts[i];
+    } else {
+        for (int w = 0; w < wc; ++w) {
+            const int n = w2tc[w];
+            if (n == 0) { dw[w] = 0.0f; continue; }
+            const int off = w2to[w];
+            float ws = 0.0f, tcon = 1e-10f;
+            for (int jx = 0; jx < n; ++jx) {
+                const int t = w2ti[off + jx];
+                const float cw = w2tw[off + jx] * wes_of(t);
+                ws += wts[t] * cw;
+                tcon += cw;
+            }
+            dw[w] = ws / tcon;
+        }
+        float infill[kMaxTexels];
+        for (int i = 0; i < tc; ++i) {
+            float v = 0.0f;
+            for (int jx = 0; jx < t2wc[i]; ++jx) v += dw[t2wi[i * 4 + jx]] * t2ww[i * 4 + jx];
+            infill[i] = v;
+        }
+        float e0[kMaxWeights], e1[kMaxWeights];
+        for (int w = 0; w < wc; ++w) {
+            float a = 1e-10f, bacc = 0.0f;
+            const int n = w2tc[w], off = w2to[w];
+            for (int jx = 0; jx < n; ++jx) {
+                const int t = w2ti[off + jx];
+                const float contrib = w2tw[off + jx];
+                const float scale = wes_of(t) * contrib;
+                a += contrib * scale;
+                bacc += (infill[t] - wts[t]) * scale;
+            }
+            e0[w] = a;
+            e1[w] = bacc;
+        }
+        for (int w = 0; w < wc; ++w) dw[w] += wclamp((e1[w] * -16.0f) / e0[w], -0.25f, 0.25f);
+    }
+    // pass03_compute_angular_offsets.wgsl:65-116
+    float ao[16];
+    {
+        uint16_t is[kMaxWeights];
+        for (int i = 0; i < wc; ++i) is[i] = (uint16_t)wf2u(wround(wclamp(dw[i], 0.0f, 1.0f) * (1024.0f - 1.0f)));
+        const float inv2pi = 1.0f / (2.0f * 3.14159265358979323846f);
+        for (int s = 0; s < 16; ++s) {
+            float ax = 0.0f, ay = 0.0f;
+            for (int jx = 0; jx < wc; ++jx) {
+                ax += T.cos_t[is[jx] * 16 + s];
+                ay += T.sin_t[is[jx] * 16 + s];
+            }
+            ao[s] = atan2f_fixed(ay, ax) * inv2pi;
+        }
+    }
+    // pass04_lowest_and_highest_weight.wgsl:116-227
+    float hl_low[16], hl_err[16], hl_cl[16], hl_ch[16];
+    int hl_span[16];
+    {
+        float minw = 3.4e38f, maxw = -3.4e38f;
+        for (int i = 0; i < wc; ++i) {
+            if (dw[i] < minw) minw = dw[i];
+            if (dw[i] > maxw) maxw = dw[i];
+        }
+        for (int sp = 0; sp < di.max_angular_steps; ++sp) {
+            const float rcp = (float)(sp + 1), off = ao[sp];
+            const float minidx = wround(minw * rcp - off), maxidx = wround(maxw * rcp - off);
+            float e = 0.0f, cl = 0.0f, ch = 0.0f;
+            for (int jx = 0; jx < wc; ++jx) {
+                const float sval = dw[jx] * rcp - off;
+                const float r = wround(sval);
+                const float diff = sval - r;
+                e += diff * diff;
+                cl += (r == minidx) ? 1.0f - (2.0f * diff) : 0.0f;
+                ch += (r == maxidx) ? 1.0f + (2.0f * diff) : 0.0f;
+            }
+            int span = wf2i(maxidx - minidx + 1.0f);
+            span = max(2, min(span, (int)di.max_quant_steps + 3));
+            const float ssize = 1.0f / rcp, es = ssize * ssize;
+            hl_low[sp] = minidx;
+            hl_span[sp] = span;
+            hl_err[sp] = e * es;
+            hl_cl[sp] = cl * es;
+            hl_ch[sp] = ch * es;
+        }
+    }
+    // pass05_best_values_for_quant_levels.wgsl:84-175, then the 13-float
+    // output stride against an 8-float allocation (shaders_init.cpp:1176):
+    // slots beyond 8*920*4*ndm floats read as 0
+    float low[13], high[13];
+    {
+        float re[36], ri[36], rc[36];
+        for (int i = 0; i < di.max_quant_steps + 4 && i < 36; ++i) { re[i] = 1e37f; ri[i] = -1.0f; rc[i] = 0.0f; }
+        for (int i = 0; i < di.max_angular_steps; ++i) {
+            const float fi = (float)i;
+            const int sp = hl_span[i];
+            const float e = hl_err[i], ecl = e + hl_cl[i], ech = e + hl_ch[i], eclh = (e + hl_cl[i]) + hl_ch[i];
+            if (e < re[sp]) { re[sp] = e; ri[sp] = fi; rc[sp] = 0.0f; }
+            if (ecl < re[sp - 1]) { re[sp - 1] = ecl; ri[sp - 1] = fi; rc[sp - 1] = 1.0f; }
+            if (ech < re[sp - 1]) { re[sp - 1] = ech; ri[sp - 1] = fi; rc[sp - 1] = 0.0f; }
+            if (eclh < re[sp - 2]) { re[sp - 2] = eclh; ri[sp - 2] = fi; rc[sp - 2] = 1.0f; }
+        }
+        const long long alloc = 8ll * kRefBatch * 4 * T.ndm;
+        const long long t = (long long)pb_in_batch * T.ndm + m;
+        for (int q = 0; q < 13; ++q) { low[q] = 0.0f; high[q] = 0.0f; }
+        for (int q = 0; q <= di.max_quant_level; ++q) {
+            const int qs = kSteps[q];
+            const int bsi = max(wf2i(ri[qs]), 0);
+            const float lwi = hl_low[bsi] + rc[qs];
+            const float hwi = (lwi + (float)qs) - 1.0f;
+            const float stepsize = 1.0f / (1.0f + (float)bsi);
+            if (13 * t + q < alloc) {
+                low[q] = (ao[bsi] + lwi) * stepsize;
+                high[q] = (ao[bsi] + hwi) * stepsize;
+            }
+        }
+    }
+    // passes 06, 07 and 11 for every trial block mode of this decimation mode
+    const ColorTab& ct = ctab[pb];
+    for (int li = T.dm_bm_off[m]; li < T.dm_bm_off[m + 1]; ++li) {
+        const int k = T.dm_bm_list[li];
+        const BmInfo bmi = T.bm[k];
+        Ccr out;
+        out.ql = -1; out.ql_mod = -1; out.formats = 0;
+        uint8_t* qwo = qw_out + ((size_t)pb * T.nbm + k) * 64;
+        const int q = bmi.quant;
+        const int bc = kFreeBits[pc - 1] - bmi.weight_bits;
+        float we;
+        if (q > 11 || bc <= 0) {
+            we = 1e38f;
+        } else {
+            // pass06_remap_low_and_high_values.wgsl:72-98 (quant <= 12 here)
+            float lo = low[q], hi = high[q];
+            // pass07_weights_and_error_for_bm.wgsl:228-331
+            if (hi > 1.02f * I.cutoff) hi = 1.0f;
+            if (hi <= lo) { lo = 0.0f; hi = 1.0f; }
+            const float qlm1 = (float)(kLevelsD[q] - 1);
+            float rscale = hi - lo;
+            const float scale = 1.0f / rscale;
+            const float slb = lo * scale;
+            rscale = rscale / 64.0f;
+            float qwf[kMaxWeights];
+            uint8_t qwi[kMaxWeights];
+            const uint32_t qmax = wf2u(qlm1);
+            for (int i = 0; i < wc; ++i) {
+                const float ix = wclamp(dw[i] * scale - slb, 0.0f, 1.0f);
+                const float ix1 = ix * qlm1;
+                const uint32_t wl = wf2u(ix1);
+                const uint32_t wh = min(wl + 1u, qmax);
+                const int il = c_qt.quant_to_unquant[q][wl], ih = c_qt.quant_to_unquant[q][wh];
+                const float fl = (float)il, fh = (float)ih;
+                const bool use_high = (fl + fh) < (128.0f * ix);
+                qwi[i] = (uint8_t)(use_high ? ih : il);
+                qwf[i] = (use_high ? fh : fl) * rscale + lo;
+            }
+            float err = 0.0f;
+            for (int i = 0; i < tc; ++i) {
+                float v = 0.0f;
+                for (int jx = 0; jx < t2wc[i]; ++jx) v += qwf[t2wi[i * 4 + jx]] * t2ww[i * 4 + jx];
+                const float diff = v - wts[i];
+                err += diff * diff * wes_of(i);
+            }
+            we = err;
+            // only whole groups of four weights reach the output (pass07:322-331)
+            const int keep = (wc / 4) * 4;
+            for (int i = 0; i < 64; ++i) qwo[i] = i < keep ? qwi[i] : 0;
+        }
+        // pass11_best_color_combination_for_mode_{1,2,3,4}part.wgsl
+        if (!(we < kErrDefault)) {
+            out.total_error = kErrDefault;
+        } else if (pc == 1) {
+            int best_idx = 0;
+            float best = kErrDefault;
+            for (int ic = 1; ic <= 4; ++ic) {
+                const int ql = c_qt.quant_mode[ic][bc];
+                if (ql < 4) continue;
+                const float e = ct.err[ql][ic - 1];
+                if (e < best) { best = e; best_idx = ic; }
+            }
+            const int fql = c_qt.quant_mode[best_idx][bc];
+            const int f = fql >= 4 ? ct.fmt[fql][best_idx - 1] : FMT_LUMINANCE;
+            out.total_error = best + we;
+            out.ql = (int8_t)fql;
+            out.ql_mod = (int8_t)fql;
+            out.formats = (uint16_t)f;
+        } else {
+            const int start = pc, end = pc == 2 ? 8 : 9, modbits = pc == 2 ? 2 : (pc == 3 ? 5 : 8);
+            int best_idx = 0;
+            float best = kErrDefault;
+            for (int ic = start; ic <= end; ++ic) {
+                const int ql = c_qt.quant_mode[ic][bc];
+                if (ql < 4) break;
+                const float e = ct.err[ql][ic - start];
+                if (e < best) { best = e; best_idx = ic; }
+            }
+            const int fql = c_qt.quant_mode[best_idx][bc];
+            const int fqlm = c_qt.quant_mode[best_idx][bc + modbits];
+            out.formats = fql >= 4 ? ct.fmt[fql][best_idx - start] : 0;
+            out.total_error = best + we;
+            out.ql = (int8_t)fql;
+            out.ql_mod = (int8_t)fqlm;
+        }
+        ccr[(size_t)pb * T.nbm + k] = out;
+    }
+}
+
+// ---- pass12 -------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_top(int npb, int nbm, const Ccr* ccr, int16_t* top) {
+    const int pb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pb >= npb) return;
+    float te[8];
+    int ti[8];
+    for (int i = 0; i < 8; ++i) { te[i] = kErrDefault; ti[i] = 0; }
+    const Ccr* c = ccr + (size_t)pb * nbm;
+    for (int k = 0; k < nbm; ++k) {
+        const float e = c[k].total_error;
+        if (e < te[7]) {
+            te[7] = e; ti[7] = k;
+            for (int j = 7; j > 0; --j)
+                if (te[j] < te[j - 1]) {
+                    const float fe = te[j]; te[j] = te[j - 1]; te[j - 1] = fe;
+                    const int fi = ti[j]; ti[j] = ti[j - 1]; ti[j - 1] = fi;
+                }
+        }
+    }
+    for (int i = 0; i < 8; ++i) top[pb * 8 + i] = (int16_t)(te[i] < kErrDefault ? ti[i] : -1);
+}
+
+// ---- pass14 helpers (pass14_pack_color_endpoints.wgsl:391-1221) -------------
+__device__ __forceinline__ int u2q(int q, int index) { return c_qt.color_u2q[q - 4][index]; }
+
+__device__ void quant3_int(int q, const int* v, int* o) {
+    for (int c = 0; c < 3; ++c) o[c] = u2q(q, v[c] * 2 + 1);
+    o[3] = 0;
+}
+__device__ void quant3_flt(int q, const int* v, const float* vf, int* o) {
+    for (int c = 0; c < 3; ++c) {
+        int idx = v[c] * 2;
+        if (vf[c] - (float)v[c] >= -0.1f) idx += 1;
+        o[c] = u2q(q, idx);
+    }
+    o[3] = 0;
+}
+__device__ __forceinline__ int quant1_int(int q, int v) { return u2q(q, v * 2 + 1); }
+__device__ __forceinline__ int quant1_flt(int q, int v, float vf) {
+    int idx = v * 2;
+    if (vf - (float)v >= -0.1f) idx += 1;
+    return u2q(q, idx);
+}
+__device__ void bit_transfer_signed(int* in0, int* in1) {
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t a = (uint32_t)in0[c], b = (uint32_t)in1[c];
+        const int nb = (int)((b >> 1) | (a & 0x80u));
+        int na = (int)((a >> 1) & 0x3Fu);
+        if (na & 0x20) na -= 0x40;
+        in0[c] = na;
+        in1[c] = nb;
+    }
+}
+__device__ __forceinline__ void uncontract(int* v) {
+    const int b = v[2];
+    const int r = (v[0] + b) >> 1, g = (v[1] + b) >> 1;
+    v[0] = r; v[1] = g;
+}
+__device__ void rgba_delta_unpack(const int* in0, const int* in1, int* o0, int* o1) {
+    int i0[4], i1[4];
+    for (int c = 0; c < 4; ++c) { i0[c] = in0[c]; i1[c] = in1[c]; }
+    bit_transfer_signed(i1, i0);
+    const int rgb_sum = i1[0] + i1[1] + i1[2];
+    for (int c = 0; c < 4; ++c) i1[c] = i1[c] + i0[c];
+    if (rgb_sum < 0) {
+        uncontract(i0); uncontract(i1);
+        for (int c = 0; c < 4; ++c) { const int t = i0[c]; i0[c] = i1[c]; i1[c] = t; }
+    }
+    for (int c = 0; c < 4; ++c) { o0[c] = min(max(i0[c], 0), 255); o1[c] = min(max(i1[c], 0), 255); }
+}
+__device__ void rgba_unpack(const int* in0, const int* in1, int* o0, int* o1) {
+    int i0[4], i1[4];
+    for (int c = 0; c < 4; ++c) { i0[c] = in0[c]; i1[c] = in1[c]; }
+    if ((i0[0] + i0[1] + i0[2]) > (i1[0] + i1[1] + i1[2])) {
+        uncontract(i0); uncontract(i1);
+        for (int c = 0; c < 4; ++c) { const int t = i0[c]; i0[c] = i1[c]; i1[c] = t; }
+    }
+    for (int c = 0; c < 4; ++c) { o0[c] = i0[c]; o1[c] = i1[c]; }
+}
+__device__ float enc_error(const float* u0, const float* u1, const int* q0, const int* q1) {
+    float e[4];
+    for (int c = 0; c < 4; ++c) {
+        const float e0 = u0[c] - (float)q0[c], e1 = u1[c] - (float)q1[c];
+        e[c] = e0 * e0 + e1 * e1;
+    }
+    return ((e[0] + e[1]) + e[2]) + e[3];
+}
+__device__ __forceinline__ bool out_of_range(const float* c) {
+    return c[0] < 0.0f || c[0] > 255.0f || c[1] < 0.0f || c[1] > 255.0f || c[2] < 0.0f || c[2] > 255.0f ||
+           c[3] < 0.0f || c[3] > 255.0f;
+}
+__device__ bool try_rgb_delta_core(const float* c0, const float* c1, int* out0, int* out1, int q, bool blue) {
+    int c0a[4], c0b[4], c0be[4], c1d[4], c1de[4];
+    for (int i = 0; i < 4; ++i) c0a[i] = wf2i(wround(c0[i])) << 1;
+    for (int i = 0; i < 4; ++i) c0b[i] = c0a[i] & 0xFF;
+    quant3_int(q, c0b, c0be);
+    for (int i = 0; i < 4; ++i) c0b[i] = c0be[i] | (c0a[i] & 0x100);
+    for (int i = 0; i < 4; ++i) c1d[i] = (wf2i(wround(c1[i])) << 1) - c0b[i];
+    c1d[3] = 0;
+    for (int i = 0; i < 4; ++i) if (c1d[i] > 63 || c1d[i] < -64) return false;
+    for (int i = 0; i < 4; ++i) c1d[i] = (c1d[i] & 0x7F) | ((c0b[i] & 0x100) >> 1);
+    quant3_int(q, c1d, c1de);
+    for (int i = 0; i < 3; ++i) if (((c1d[i] ^ c1de[i]) & 0xC0) != 0) return false;
+    int ep0[4], ep1[4];
+    for (int i = 0; i < 4; ++i) { ep0[i] = c0be[i]; ep1[i] = c1de[i]; }
+    bit_transfer_signed(ep1, ep0);
+    const int s = ep1[0] + ep1[1] + ep1[2];
+    if (blue ? s >= 0 : s < 0) return false;
+    for (int i = 0; i < 4; ++i) {
+        ep0[i] = ep0[i] + ep1[i];
+        if (ep0[i] < 0 || ep0[i] > 0xFF) return false;
+    }
+    for (int i = 0; i < 4; ++i) { out0[i] = c0be[i]; out1[i] = c1de[i]; }
+    return true;
+}
+// try_quantize_rgb_delta_blue_contract, pass14:525-606
+__device__ bool try_rgb_delta_bc(const float* color0, const float* color1, int* out0, int* out1, int q) {
+    float c0[4], c1[4];
+    const float s0[4] = {color1[2], color1[2], color1[2], color1[3]};
+    const float s1[4] = {color0[2], color0[2], color0[2], color0[3]};
+    for (int i = 0; i < 4; ++i) { c0[i] = (color1[i] + color1[i]) - s0[i]; c1[i] = (color0[i] + color0[i]) - s1[i]; }
+    if (out_of_range(c0) || out_of_range(c1)) return false;
+    return try_rgb_delta_core(c0, c1, out0, out1, q, true);
+}
+// try_quantize_rgb_blue_contract, pass14:674-711
+__device__ bool try_rgb_bc(const float* color0, const float* color1, int* out0, int* out1, int q) {
+    float c0[4], c1[4];
+    const float s0[4] = {color0[2], color0[2], color0[2], color0[3]};
+    const float s1[4] = {color1[2], color1[2], color1[2], color1[3]};
+    for (int i = 0; i < 4; ++i) { c0[i] = (color0[i] + color0[i]) - s0[i]; c1[i] = (color1[i] + color1[i]) - s1[i]; }
+    if (out_of_range(c0) || out_of_range(c1)) return false;
+    int r0[4], r1[4], c0i[4], c1i[4];
+    for (int i = 0; i < 4; ++i) { r0[i] = wf2i(wround(c0[i])); r1[i] = wf2i(wround(c1[i])); }
+    quant3_flt(q, r0, c0, c0i);
+    quant3_flt(q, r1, c1, c1i);
+    if ((c1i[0] + c1i[1] + c1i[2]) <= (c0i[0] + c0i[1] + c0i[2])) return false;
+    for (int i = 0; i < 4; ++i) { out0[i] = c1i[i]; out1[i] = c0i[i]; }
+    return true;
+}
+// quantize_rgb, pass14:715-746
+__device__ void quantize_rgb(const float* color0, const float* color1, int* out0, int* out1, int q) {
+    float c0[4], c1[4];
+    for (int i = 0; i < 4; ++i) { c0[i] = color0[i]; c1[i] = color1[i]; }
+    int c0i[4], c1i[4];
+    for (;;) {
+        int c0q[4], c1q[4];
+        for (int i = 0; i < 4; ++i) c0q[i] = max(wf2i(wround(c0[i])), 0);
+        quant3_flt(q, c0q, c0, c0i);
+        for (int i = 0; i < 4; ++i) c0[i] = c0[i] - 0.2f;
+        for (int i = 0; i < 4; ++i) c1q[i] = min(wf2i(wround(c1[i])), 255);
+        quant3_flt(q, c1q, c1, c1i);
+        for (int i = 0; i < 4; ++i) c1[i] = c1[i] + 0.2f;
+        if ((c0i[0] + c0i[1] + c0i[2]) <= (c1i[0] + c1i[1] + c1i[2])) break;
+    }
+    for (int i = 0; i < 4; ++i) { out0[i] = c0i[i]; out1[i] = c1i[i]; }
+}
+// try_quantize_alpha_delta, pass14:748-797
+__device__ bool try_alpha_delta(const float* color0, const float* color1, int* out0, int* out1, int q) {
+    const int a0a = wf2i(wround(color0[3])) << 1;
+    int a0b = a0a & 0xFF;
+    const int a0be = quant1_int(q, a0b);
+    a0b = a0be | (a0a & 0x100);
+    int a1d = (wf2i(wround(color1[3])) << 1) - a0b;
+    if (a1d > 63 || a1d < -64) return false;
+    a1d = (a1d & 0x7F) | ((a0b & 0x100) >> 1);
+    const int a1de = quant1_int(q, a1d);
+    int a1du = a1de;
+    if (((a1d ^ a1du) & 0xC0) != 0) return false;
+    a1du &= 0x7F;
+    if (a1du & 0x40) a1du -= 0x80;
+    a1du += a0b;
+    if (a1du < 0 || a1du > 0x1FF) return false;
+    out0[3] = a0be;
+    out1[3] = a1de;
+    return true;
+}
+// quantize_rgbs, pass14:858-884
+__device__ void quantize_rgbs(const float* color, uint32_t* out, int q) {
+    const float scale = 1.0f / 257.0f;
+    const float r = wclamp(color[0] * scale, 0.0f, 255.0f);
+    const float g = wclamp(color[1] * scale, 0.0f, 255.0f);
+    const float b = wclamp(color[2] * scale, 0.0f, 255.0f);
+    const int ri = quant1_flt(q, wf2i(wround(r)), r);
+    const int gi = quant1_flt(q, wf2i(wround(g)), g);
+    const int bi = quant1_flt(q, wf2i(wround(b)), b);
+    const float oldsum = ((color[0] + color[1]) + color[2]) * scale;
+    const float newsum = (float)(uint32_t)(ri + gi + bi);
+    const float scalea = wclamp((color[3] * (oldsum + 1e-10f)) / (newsum + 1e-10f), 0.0f, 1.0f);
+    const int sidx = min(max(wf2i(wround(scalea * 256.0f)), 0), 255);
+    out[0] = (uint32_t)ri; out[1] = (uint32_t)gi; out[2] = (uint32_t)bi;
+    out[3] = (uint32_t)quant1_int(q, sidx);
+}
+__device__ __forceinline__ float lum3(const float* c) { return ((c[0] + c[1]) + c[2]) / 3.0f; }
+__device__ bool try_la_delta(const float* c0, const float* c1, uint32_t* out, int q) {
+    const float l0 = lum3(c0), l1 = lum3(c1);
+    const int l0a = wf2i(wround(l0)) << 1, a0a = wf2i(wround(c0[3])) << 1;
+    int l0b = l0a & 0xFF, a0b = a0a & 0xFF;
+    const int l0be = quant1_int(q, l0b), a0be = quant1_int(q, a0b);
+    l0b = l0be | (l0a & 0x100);
+    a0b = a0be | (a0a & 0x100);
+    int l1d = (wf2i(wround(l1)) << 1) - l0b;
+    int a1d = (wf2i(wround(c1[3])) << 1) - a0b;
+    if (l1d > 63 || l1d < -64 || a1d > 63 || a1d < -64) return false;
+    l1d = (l1d & 0x7F) | ((l0b & 0x100) >> 1);
+    a1d = (a1d & 0x7F) | ((a0b & 0x100) >> 1);
+    const int l1de = quant1_int(q, l1d), a1de = quant1_int(q, a1d);
+    int l1du = l1de, a1du = a1de;
+    if (((l1d ^ l1du) & 0xC0) != 0) return false;
+    if (((a1d ^ a1du) & 0xC0) != 0) return false;
+    l1du &= 0x7F; a1du &= 0x7F;
+    if (l1du & 0x40) l1du -= 0x80;
+    if (a1du & 0x40) a1du -= 0x80;
+    l1du += l0b; a1du += a0b;
+    if (l1du < 0 || l1du > 0x1FF || a1du < 0 || a1du > 0x1FF) return false;
+    out[0] = (uint32_t)l0be; out[1] = (uint32_t)l1de; out[2] = (uint32_t)a0be; out[3] = (uint32_t)a1de;
+    return true;
+}
+
+// pack_color_endpoints_helper, pass14:1018-1221
+__device__ int pack_color_endpoints(const float* color0, const float* color1, const float* rgbs, int format, int q,
+                                    uint32_t* packed) {
+    float c0[4], c1[4];
+    for (int i = 0; i < 4; ++i) {
+        c0[i] = wclamp(color0[i], 0.0f, 65535.0f) / 257.0f;
+        c1[i] = wclamp(color1[i], 0.0f, 65535.0f) / 257.0f;
+    }
+    float best_error = kErrDefault;
+    int best_format = FMT_LUMINANCE;
+    for (int i = 0; i < 8; ++i) packed[i] = 0;
+    int o0[4] = {0, 0, 0, 0}, o1[4] = {0, 0, 0, 0}, t0[4] = {0, 0, 0, 0}, t1[4] = {0, 0, 0, 0}, u0[4], u1[4];
+    auto take = [&](float e, int f) {
+        if (e < best_error) {
+            best_error = e; best_format = f;
+            for (int i = 0; i < 4; ++i) { o0[i] = t0[i]; o1[i] = t1[i]; }
+        }
+    };
+    switch (format) {
+        case FMT_RGB:
+        case FMT_RGBA: {
+            const bool alpha = format == FMT_RGBA;
+            if (q <= 18) {
+                bool ok = try_rgb_delta_bc(c0, c1, o0, o1, q);
+                if (ok && alpha) ok = try_alpha_delta(c1, c0, o0, o1, q);
+                if (ok) {
+                    rgba_delta_unpack(o0, o1, u0, u1);
+                    best_format = alpha ? FMT_RGBA_DELTA : FMT_RGB_DELTA;
+                    best_error = enc_error(c0, c1, u0, u1);
+                }
+                bool ok2 = try_rgb_delta_core(c0, c1, t0, t1, q, false);
+                if (ok2 && alpha) ok2 = try_alpha_delta(c0, c1, t0, t1, q);
+                if (ok2) {
+                    rgba_delta_unpack(t0, t1, u0, u1);
+                    take(enc_error(c0, c1, u0, u1), alpha ? FMT_RGBA_DELTA : FMT_RGB_DELTA);
+                }
+            }
+            if (q < 20 && try_rgb_bc(c0, c1, t0, t1, q)) {
+                if (alpha) {
+                    t0[3] = quant1_flt(q, wf2i(wround(c1[3])), c1[3]);
+                    t1[3] = quant1_flt(q, wf2i(wround(c0[3])), c0[3]);
+                }
+                rgba_unpack(t0, t1, u0, u1);
+                take(enc_error(c0, c1, u0, u1), alpha ? FMT_RGBA : FMT_RGB);
+            }
+            quantize_rgb(c0, c1, t0, t1, q);
+            if (alpha) {
+                t0[3] = quant1_flt(q, wf2i(wround(c0[3])), c0[3]);
+                t1[3] = quant1_flt(q, wf2i(wround(c1[3])), c1[3]);
+            }
+            rgba_unpack(t0, t1, u0, u1);
+            take(enc_error(c0, c1, u0, u1), alpha ? FMT_RGBA : FMT_RGB);
+            packed[0] = (uint32_t)o0[0]; packed[1] = (uint32_t)o1[0];
+            packed[2] = (uint32_t)o0[1]; packed[3] = (uint32_t)o1[1];
+            packed[4] = (uint32_t)o0[2]; packed[5] = (uint32_t)o1[2];
+            if (alpha) { packed[6] = (uint32_t)o0[3]; packed[7] = (uint32_t)o1[3]; }
+            break;
+        }
+        case FMT_RGB_SCALE:
+            quantize_rgbs(rgbs, packed, q);
+            best_format = FMT_RGB_SCALE;
+            break;
+        case FMT_RGB_SCALE_ALPHA:
+            packed[4] = (uint32_t)quant1_flt(q, wf2i(wround(c0[3])), c0[3]);
+            packed[5] = (uint32_t)quant1_flt(q, wf2i(wround(c1[3])), c1[3]);
+            quantize_rgbs(rgbs, packed, q);
+            best_format = FMT_RGB_SCALE_ALPHA;
+            break;
+        case FMT_LUMINANCE_ALPHA:
+            if (q <= 18 && try_la_delta(c0, c1, packed, q)) { best_format = FMT_LUMINANCE_ALPHA_DELTA; break; }
+            {
+                const float l0 = lum3(c0), l1 = lum3(c1);
+                packed[0] = (uint32_t)quant1_flt(q, wf2i(wround(l0)), l0);
+                packed[1] = (uint32_t)quant1_flt(q, wf2i(wround(l1)), l1);
+                packed[2] = (uint32_t)quant1_flt(q, wf2i(wround(c0[3])), c0[3]);
+                packed[3] = (uint32_t)quant1_flt(q, wf2i(wround(c1[3])), c1[3]);
+            }
+            best_format = FMT_LUMINANCE_ALPHA;
+            break;
+        default: {  // FMT_LUMINANCE and anything else
+            float l0 = lum3(c0), l1 = lum3(c1);
+            if (l0 > l1) { const float avg = (l0 + l1) * 0.5f; l0 = avg; l1 = avg; }
+            packed[0] = (uint32_t)quant1_flt(q, wf2i(wround(l0)), l0);
+            packed[1] = (uint32_t)quant1_flt(q, wf2i(wround(l1)), l1);
+            best_format = FMT_LUMINANCE;
+            break;
+        }
+    }
+    return best_format;
+}
+
+// LDR endpoint decode, pass15_unpack_color_endpoints.wgsl:307-391
+__device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, int* o1) {
+    int in0[4], in1[4];
+    switch (format) {
+        case FMT_LUMINANCE_DELTA: {
+            const int v0 = (int)v[0], v1 = (int)v[1];
+            const int l0 = (v0 >> 2) | (v1 & 0xC0);
+            const int l1 = min(l0 + (v1 & 0x3F), 255);
+            o0[0] = o0[1] = o0[2] = l0; o0[3] = 255;
+            o1[0] = o1[1] = o1[2] = l1; o1[3] = 255;
+            return;
+        }
+        case FMT_LUMINANCE_ALPHA:
+            o0[0] = o0[1] = o0[2] = (int)v[0]; o0[3] = (int)v[2];
+            o1[0] = o1[1] = o1[2] = (int)v[1]; o1[3] = (int)v[3];
+            return;
+        case FMT_LUMINANCE_ALPHA_DELTA: {
+            int l0 = (int)v[0], l1 = (int)v[1], a0 = (int)v[2], a1 = (int)v[3];
+            l0 |= (l1 & 0x80) << 1;
+            a0 |= (a1 & 0x80) << 1;
+            l1 &= 0x7F; a1 &= 0x7F;
+            if (l1 & 0x40) l1 -= 0x80;
+            if (a1 & 0x40) a1 -= 0x80;
+            l0 >>= 1; l1 >>= 1; a0 >>= 1; a1 >>= 1;
+            l1 = min(max(l0 + l1, 0), 255);
+            a1 = min(max(a0 + a1, 0), 255);
+            o0[0] = o0[1] = o0[2] = l0; o0[3] = a0;
+            o1[0] = o1[1] = o1[2] = l1; o1[3] = a1;
+            return;
+        }
+        case FMT_RGB_SCALE: {
+            const int s = (int)v[3];
+            const int i0[4] = {(int)v[0], (int)v[1], (int)v[2], 255};
+            for (int c = 0; c < 4; ++c) { o1[c] = i0[c]; o0[c] = (i0[c] * s) >> 8; }
+            o0[3] = 255;
+            return;
+        }
+        case FMT_RGB_SCALE_ALPHA: {
+            const int s = (int)v[3];
+            const int i0[4] = {(int)v[0], (int)v[1], (int)v[2], (int)v[4]};
+            o1[0] = i0[0]; o1[1] = i0[1]; o1[2] = i0[2]; o1[3] = (int)v[5];
+            for (int c = 0; c < 4; ++c) o0[c] = (i0[c] * s) >> 8;
+            o0[3] = (int)v[4];
+            return;
+        }
+        case FMT_RGB:
+        case FMT_RGB_DELTA:
+        case FMT_RGBA:
+        case FMT_RGBA_DELTA: {
+            const bool alpha = format == FMT_RGBA || format == FMT_RGBA_DELTA;
+            in0[0] = (int)v[0]; in0[1] = (int)v[2]; in0[2] = (int)v[4]; in0[3] = alpha ? (int)v[6] : 0;
+            in1[0] = (int)v[1]; in1[1] = (int)v[3]; in1[2] = (int)v[5]; in1[3] = alpha ? (int)v[7] : 0;
+            if (format == FMT_RGB || format == FMT_RGBA) rgba_unpack(in0, in1, o0, o1);
+            else rgba_delta_unpack(in0, in1, o0, o1);
+            if (!alpha) { o0[3] = 255; o1[3] = 255; }
+            return;
+        }
+        default:  // FMT_LUMINANCE and anything else
+            o0[0] = o0[1] = o0[2] = (int)v[0]; o0[3] = 255;
+            o1[0] = o1[1] = o1[2] = (int)v[1]; o1[3] = 255;
+            return;
+    }
+}
+
+// ---- passes 13-17, refinement loop of compression_pipeline.cpp:473-484 ---------
+struct Cand {
+    float ep0[4][4], ep1[4][4];
+    int quant_level, quant_level_mod;
+    int formats[4];
+    uint8_t qw[kMaxWeights];
+    int matched, final_quant;
+    int final_formats[4];
+    uint32_t packed[32];
+};
+
+__global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
+                                               const Ideal* ideal, const int16_t* top, const Ccr* ccr,
+                                               const uint8_t* qw_all, CandOut* cand_out) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= npb * 8) return;
+    const int pb = tid >> 3, slot = tid & 7;
+    CandOut* co = cand_out + tid;
+    const int k = top[tid];
+    if (k < 0) {  // pass12 produced no candidate for this slot
+        co->error = kErrDefault;
+        return;
+    }
+    const Pb b = make_pb(A, T, pb_part, pc, pb);
+    const BmInfo bmi = T.bm[k];
+    const DmInfo di = T.dm[bmi.dm];
+    const int tc = T.tc, wc = di.weight_count;
+    const uint8_t* t2wc = T.t2w_count + (size_t)bmi.dm * tc;
+    const uint8_t* t2wi = T.t2w_idx + (size_t)bmi.dm * tc * 4;
+    const float* t2ww = T.t2w_w + (size_t)bmi.dm * tc * 4;
+    const uint8_t* w2tc = T.w2t_count + bmi.dm * 64;
+    const uint16_t* w2to = T.w2t_off + bmi.dm * 64;
+    const uint8_t* w2ti = T.w2t_idx + di.w2t_base;
+    const float* w2tw = T.w2t_w + di.w2t_base;
+    const bool undecimated = tc == wc;
+    const Ideal& I = ideal[pb];
+    const Ccr cr = ccr[(size_t)pb * T.nbm + k];
+    Cand c;
+    for (int p = 0; p < 4; ++p)
+        for (int j = 0; j < 4; ++j) { c.ep0[p][j] = I.ep0[p][j]; c.ep1[p][j] = I.ep1[p][j]; }
+    c.quant_level = cr.ql;
+    c.quant_level_mod = cr.ql_mod;
+    for (int p = 0; p < 4; ++p) c.formats[p] = fmt_of(cr.formats, p);
+    const uint8_t* qsrc = qw_all + ((size_t)pb * T.nbm + k) * 64;
+    for (int i = 0; i < kMaxWeights; ++i) c.qw[i] = qsrc[i];
+    const float lsw = (kCW[0] + kCW[1]) + kCW[2];
+    float best_err = kErrDefault;
+    CandOut best;
+    best.error = kErrDefault;
+    int ue0[4][4], ue1[4][4];
+    for (int a = 0; a < 6; ++a) {
+        // pass13_recompute_ideal_endpoints.wgsl:182-393
+        float rgbs[4][4];
+        {
+            float wmin1[4], wmax1[4], ls[4], ms[4], rs[4], smin[4], smax[4], cx[4][4], cy[4][4], sv[4][2], sdir[4][3];
+            for (int p = 0; p < pc; ++p) {
+                wmin1[p] = 1.0f; wmax1[p] = 0.0f; ls[p] = ms[p] = rs[p] = 0.0f; smin[p] = 1e10f; smax[p] = 0.0f;
+                for (int j = 0; j < 4; ++j) { cx[p][j] = 0.0f; cy[p][j] = 0.0f; }
+                sv[p][0] = sv[p][1] = 0.0f;
+                const float size = (float)b.ppc[p];
+                float q3[3];
+                for (int j = 0; j < 3; ++j) {
+                    const float rsum = (I.avg[p][j] * size) * kCW[j];
+                    const float rws = wmax(kCW[j] * size, 1e-17f);
+                    q3[j] = rsum / rws;
+                }
+                normalize3(q3, sdir[p]);
+            }
+            for (int i = 0; i < tc; ++i) {
+                float w;
+                if (undecimated) w = (float)c.qw[i] / 64.0f;
+                else {
+                    w = 0.0f;
+                    for (int j = 0; j < t2wc[i]; ++j) w += ((float)c.qw[t2wi[i * 4 + j]] / 64.0f) * t2ww[i * 4 + j];
+                }
+                const int p = texel_part(b, i);
+                float rgba[4];
+                texel_px(A, T, b.g, i, rgba);
+                if (w < wmin1[p]) wmin1[p] = w;
+                if (w > wmax1[p]) wmax1[p] = w;
+                const float sc = dot3(rgba, sdir[p]);
+                if (sc < smin[p]) smin[p] = sc;
+                if (sc > smax[p]) smax[p] = sc;
+                const float om = 1.0f - w;
+                ls[p] += om * om;
+                ms[p] += om * w;
+                rs[p] += w * w;
+                for (int j = 0; j < 4; ++j) {
+                    const float cyk = rgba[j] * w;
+                    cy[p][j] += cyk;
+                    cx[p][j] += rgba[j] - cyk;
+                }
+                sv[p][0] += om * sc * lsw;
+                sv[p][1] += w * sc * lsw;
+            }
+            for (int p = 0; p < pc; ++p) {
+                float colx[4], coly[4], lv[4], mv[4], rv[4];
+                for (int j = 0; j < 4; ++j) {
+                    colx[j] = cx[p][j] * kCW[j]; coly[j] = cy[p][j] * kCW[j];
+                    lv[j] = ls[p] * kCW[j]; mv[j] = ms[p] * kCW[j]; rv[j] = rs[p] * kCW[j];
+                }
+                const float lx = ls[p] * lsw, ly = ms[p] * lsw, lz = rs[p] * lsw;
+                const float scalediv = wclamp(smin[p] / wmax(smax[p], 1e-10f), 0.0f, 1.0f);
+                rgbs[p][0] = sdir[p][0] * smax[p]; rgbs[p][1] = sdir[p][1] * smax[p]; rgbs[p][2] = sdir[p][2] * smax[p];
+                rgbs[p][3] = scalediv;
+                if (wmin1[p] >= wmax1[p] * 0.999f) {
+                    const float size = (float)b.ppc[p];
+                    for (int j = 0; j < 4; ++j) {
+                        const float ac = (colx[j] + coly[j]) / wmax(kCW[j] * size, 1e-17f);
+                        if (ac == ac) { c.ep0[p][j] = ac; c.ep1[p][j] = ac; }
+                    }
+                    rgbs[p][3] = 1.0f;
+                } else {
+                    for (int j = 0; j < 4; ++j) {
+                        const float det = (lv[j] * rv[j]) - (mv[j] * mv[j]);
+                        const float rdet = 1.0f / det;
+                        const float mss = ((lv[j] * lv[j]) + 2.0f * (mv[j] * mv[j])) + (rv[j] * rv[j]);
+                        const float e0 = (rv[j] * colx[j] - mv[j] * coly[j]) * rdet;
+                        const float e1 = (lv[j] * coly[j] - mv[j] * colx[j]) * rdet;
+                        if (fabsf(det) > (mss * 1e-4f) && e0 == e0 && e1 == e1) { c.ep0[p][j] = e0; c.ep1[p][j] = e1; }
+                    }
+                    const float lsdet = (lx * lz) - (ly * ly);
+                    const float lsrdet = 1.0f / lsdet;
+                    const float lsmss = ((lx * lx) + 2.0f * (ly * ly)) + (lz * lz);
+                    const float se0 = (lz * sv[p][0] - ly * sv[p][1]) * lsrdet;
+                    const float se1 = (lx * sv[p][1] - ly * sv[p][0]) * lsrdet;
+                    if ((fabsf(lsdet) > (lsmss * 1e-4f)) && se0 == se0 && se1 == se1 && se0 < se1) {
+                        rgbs[p][0] = sdir[p][0] * se1; rgbs[p][1] = sdir[p][1] * se1; rgbs[p][2] = sdir[p][2] * se1;
+                        rgbs[p][3] = se0 / se1;
+                    }
+                }
+            }
+        }
+        // pass14_pack_color_endpoints.wgsl:1226-1304
+        {
+            bool all_same = c.quant_level != c.quant_level_mod;
+            int fn[4] = {0, 0, 0, 0};
+            uint32_t vn[32];
+            for (int i = 0; i < 32; ++i) vn[i] = 0;
+            for (int p = 0; p < pc; ++p) {
+                const int f = pack_color_endpoints(c.ep0[p], c.ep1[p], rgbs[p], c.formats[p], c.quant_level, vn + p * 8);
+                fn[p] = f;
+                if (p > 0) all_same = all_same && (f == fn[0]);
+            }
+            bool matched = false;
+            if (pc >= 2 && all_same) {
+                int fm[4] = {0, 0, 0, 0};
+                uint32_t vm[32];
+                for (int i = 0; i < 32; ++i) vm[i] = 0;
+                bool same_mod = true;
+                for (int p = 0; p < pc; ++p) {
+                    const int f = pack_color_endpoints(c.ep0[p], c.ep1[p], rgbs[p], c.formats[p], c.quant_level_mod, vm + p * 8);
+                    fm[p] = f;
+                    if (p > 0 && f != fm[0]) { same_mod = false; break; }
+                }
+                if (same_mod) {
+                    matched = true;
+                    for (int p = 0; p < 4; ++p) fn[p] = fm[p];
+                    for (int i = 0; i < 32; ++i) vn[i] = vm[i];
+                }
+            }
+            c.matched = matched ? 1 : 0;
+            c.final_quant = matched ? c.quant_level_mod : c.quant_level;
+            for (int p = 0; p < 4; ++p) c.final_formats[p] = fn[p];
+            for (int i = 0; i < 32; ++i) c.packed[i] = vn[i];
+        }
+        // pass15_unpack_color_endpoints.wgsl:307-391
+        for (int p = 0; p < pc; ++p) {
+            int o0[4], o1[4];
+            unpack_color_endpoints(c.final_formats[p], c.packed + p * 8, o0, o1);
+            for (int j = 0; j < 4; ++j) { ue0[p][j] = o0[j] * 257; ue1[p][j] = o1[j] * 257; }
+        }
+        // pass17 (before pass16 on the first iteration) and pass16 + pass17
+        for (int phase = (a == 0 ? 0 : 1); phase < 2; ++phase) {
+            if (phase == 1) {
+                // pass16_realign_weights.wgsl:211-367
+                float base[4][4], offs[4][4];
+                for (int p = 0; p < pc; ++p)
+                    for (int j = 0; j < 4; ++j) {
+                        base[p][j] = (float)ue0[p][j];
+                        offs[p][j] = (float)(ue1[p][j] - ue0[p][j]) * (1.0f / 64.0f);
+                    }
+                const int q = bmi.quant;
+                if (undecimated) {
+                    for (int t = 0; t < tc; ++t) {
+                        const int uqw = c.qw[t];
+                        const uint32_t pn = c_qt.prev_next[q][uqw];
+                        const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
+                        const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
+                        const int p = texel_part(b, t);
+                        float px[4];
+                        texel_px(A, T, b.g, t, px);
+                        float s0[4], s1[4], s2[4];
+                        for (int j = 0; j < 4; ++j) {
+                            const float diff = (base[p][j] + offs[p][j] * wb) - px[j];
+                            const float dfd = diff + offs[p][j] * dd, dfu = diff + offs[p][j] * udf;
+                            s0[j] = diff * diff; s1[j] = dfd * dfd; s2[j] = dfu * dfu;
+                        }
+                        const float eb = dot4(s0, kCW), ed = dot4(s1, kCW), eu = dot4(s2, kCW);
+                        if ((eu < eb) && (eu < ed) && (uqw < 64)) c.qw[t] = (uint8_t)up;
+                        else if ((ed < eb) && (uqw > 0)) c.qw[t] = (uint8_t)down;
+                    }
+                } else {
+                    float uq[kMaxWeights];
+                    for (int i = 0; i < wc; ++i) uq[i] = (float)c.qw[i];
+                    for (int we = 0; we < wc; ++we) {
+                        const int uqw = c.qw[we];
+                        const uint32_t pn = c_qt.prev_next[q][uqw];
+                        const float down = (float)(pn & 0xFF), up = (float)((pn >> 8) & 0xFF), wbase = (float)uqw;
+                        const float ddn = down - wbase, dup = up - wbase;
+                        float ebv[4] = {0, 0, 0, 0}, edv[4] = {0, 0, 0, 0}, euv[4] = {0, 0, 0, 0};
+                        const int n = w2tc[we], woff = w2to[we];
+                        for (int te = 0; te < n; ++te) {
+                            const int t = w2ti[woff + te];
+                            float wbt = 0.0f, twb = 0.0f;
+                            bool found = false;
+                            for (int j = 0; j < t2wc[t]; ++j) {
+                                const int wi = t2wi[t * 4 + j];
+                                wbt += uq[wi] * t2ww[t * 4 + j];
+                                if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
+                            }
+                            const float wdd = ddn * twb, wud = dup * twb;
+                            const int p = texel_part(b, t);
+                            float px[4];
+                            texel_px(A, T, b.g, t, px);
+                            for (int j = 0; j < 4; ++j) {
+                                const float diff = (base[p][j] + offs[p][j] * wbt) - px[j];
+                                const float dd = diff + offs[p][j] * wdd, du = diff + offs[p][j] * wud;
+                                ebv[j] += diff * diff;
+                                edv[j] += dd * dd;
+                                euv[j] += du * du;
+                            }
+                        }
+                        const float eb = dot4(ebv, kCW), ed = dot4(edv, kCW), eu = dot4(euv, kCW);
+                        if ((eu < eb) && (eu < ed) && (uqw < 64)) { c.qw[we] = (uint8_t)up; uq[we] = up; }
+                        else if ((ed < eb) && (uqw > 0)) { c.qw[we] = (uint8_t)down; uq[we] = down; }
+                    }
+                }
+            }
+            // pass17_compute_final_error.wgsl:149-231
+            float total = 0.0f;
+            for (int i = 0; i < tc; ++i) {
+                float w;
+                if (undecimated) w = (float)c.qw[i] / 64.0f;
+                else {
+                    w = 0.0f;
+                    for (int j = 0; j < t2wc[i]; ++j) w += ((float)c.qw[t2wi[i * 4 + j]] / 64.0f) * t2ww[i * 4 + j];
+                }
+                const int p = texel_part(b, i);
+                float px[4];
+                texel_px(A, T, b.g, i, px);
+                const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
+                float dd[4];
+                for (int j = 0; j < 4; ++j) {
+                    const int col = (ue0[p][j] * w0 + ue1[p][j] * w1 + 32) >> 6;
+                    const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
+                    dd[j] = df * df;
+                }
+                total += wmin(dot4(dd, kCW), kErrDefault);
+            }
+            if (total < best_err) {
+                best_err = total;
+                best.error = total;
+                best.bm = (uint16_t)k;
+                best.matched = (uint8_t)c.matched;
+                best.quant = (uint8_t)c.final_quant;
+                best.formats = pack_fmt4(c.final_formats);
+                for (int i = 0; i < 32; ++i) best.packed[i] = (uint8_t)c.packed[i];
+                for (int i = 0; i < 64; ++i) best.qw[i] = c.qw[i];
+            }
+        }
+    }
+    (void)slot;
+    (void)w2tw;
+    *co = best;
+}
+
+// ---- pass18 + host argmin (compression_pipeline.cpp:499-519) --------------------
+__global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, const int16_t* pb_part,
+                                             const CandOut* cand, SymBest* best) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.nb) return;
+    const int npos = pc == 1 ? 1 : 4;
+    float cur_err = 0.0f;
+    int cur_pb = -1, cur_slot = -1;
+    for (int c = 0; c < npos; ++c) {
+        const int pb = k * npos + c;
+        float be = kErrDefault;
+        int bs = -1;
+        for (int s = 0; s < 8; ++s) {
+            const float e = cand[pb * 8 + s].error;
+            if (e < be) { be = e; bs = s; }
+        }
+        if (cur_pb < 0 || be < cur_err) { cur_err = be; cur_pb = pb; cur_slot = bs; }
+    }
+    SymBest& sb = best[A.blk0 + k];
+    if (pc == 1) sb.error = 1e30f;
+    if (cur_slot >= 0 && cur_err < sb.error) {
+        const CandOut& co = cand[cur_pb * 8 + cur_slot];
+        sb.error = cur_err;
+        sb.raw_mode = T.bm[co.bm].raw_mode;
+        sb.pc = (uint8_t)pc;
+        sb.seed = pc == 1 ? 0 : T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[cur_pb]];
+        sb.matched = co.matched;
+        sb.quant = co.quant;
+        sb.formats = co.formats;
+        for (int i = 0; i < 32; ++i) sb.packed[i] = co.packed[i];
+        for (int i = 0; i < 64; ++i) sb.qw[i] = co.qw[i];
+    }
+}
+
+// ---- symbolic_to_physical (physical_compression.cpp:843-1003) ---------------------
+__device__ __forceinline__ void write_bits(uint32_t value, int bitcount, int bitoffset, uint32_t* w) {
+    // 128-bit block as four little-endian words; a field never spans more than two words
+    const uint32_t mask = bitcount >= 32 ? 0xFFFFFFFFu : ((1u << bitcount) - 1u);
+    value &= mask;
+    const int wi = bitoffset >> 5, sh = bitoffset & 31;
+    w[wi] = (w[wi] & ~(mask << sh)) | (value << sh);
+    if (sh + bitcount > 32) {
+        const int s2 = 32 - sh;
+        w[wi + 1] = (w[wi + 1] & ~(mask >> s2)) | (value >> s2);
+    }
+}
+
+__device__ void encode_ise(int q, int count, const uint8_t* in, uint32_t* w, int off) {
+    int bits, trits = 0, quints = 0;
+    {
+        const int n = kLevelsD[q];
+        trits = n % 3 == 0;
+        quints = n % 5 == 0;
+        const int base = trits ? n / 3 : (quints ? n / 5 : n);
+        bits = 31 - __clz(base);
+    }
+    const uint32_t mask = (1u << bits) - 1u;
+    if (trits) {
+        const int tb[5] = {2, 2, 1, 2, 1}, ts[5] = {0, 2, 4, 5, 7};
+        for (int i = 0; i < count; i += 5) {
+            int t[5] = {0, 0, 0, 0, 0};
+            for (int k = 0; k < 5; ++k) t[k] = (i + k < count) ? (in[i + k] >> bits) : 0;
+            const int T = c_qt.int_of_trits[(((t[4] * 3 + t[3]) * 3 + t[2]) * 3 + t[1]) * 3 + t[0]];
+            for (int k = 0; k < 5 && i + k < count; ++k) {
+                const uint32_t pack = (in[i + k] & mask) | (((uint32_t)(T >> ts[k]) & ((1u << tb[k]) - 1u)) << bits);
+                write_bits(pack, bits + tb[k], off, w);
+                off += bits + tb[k];
+            }
+        }
+    } else if (quints) {
+        const int qb[3] = {3, 2, 2}, qs[3] = {0, 3, 5};
+        for (int i = 0; i < count; i += 3) {
+            int t[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k) t[k] = (i + k < count) ? (in[i + k] >> bits) : 0;
+            const int Q = c_qt.int_of_quints[(t[2] * 5 + t[1]) * 5 + t[0]];
+            for (int k = 0; k < 3 && i + k < count; ++k) {
+                const uint32_t pack = (in[i + k] & mask) | (((uint32_t)(Q >> qs[k]) & ((1u << qb[k]) - 1u)) << bits);
+                write_bits(pack, bits + qb[k], off, w);
+                off += bits + qb[k];
+            }
+        }
+    } else {
+        for (int i = 0; i < count; ++i) {
+            write_bits(in[i], bits, off, w);
+            off += bits;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t bitrev32(uint32_t v) { return __brev(v); }
+
+__global__ void __launch_bounds__(64) k_pack(DevTables T, const SymBest* best, int nblocks, uint8_t* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblocks) return;
+    const SymBest& sb = best[k];
+    const int raw = sb.raw_mode;
+    const int wc = T.raw_wc[raw], q = T.raw_quant[raw];
+    const int pc = sb.pc;
+    uint32_t wb[4] = {0, 0, 0, 0};
+    uint8_t wv[kMaxWeights];
+    const float levels = (float)kLevelsD[q];
+    for (int i = 0; i < wc; ++i) {
+        const float qw = ((float)sb.qw[i] / 64.0f) * (levels - 1.0f);
+        wv[i] = c_qt.weight_scramble[q][(int)(qw + 0.5f)];
+    }
+    encode_ise(q, wc, wv, wb, 0);
+    // bit-reverse the whole 128-bit weight string into the top of the block
+    uint32_t w[4];
+    for (int i = 0; i < 4; ++i) w[i] = bitrev32(wb[3 - i]);
+    int wbits;
+    {
+        const int n = kLevelsD[q];
+        const int t = n % 3 == 0, u = n % 5 == 0;
+        const int base = t ? n / 3 : (u ? n / 5 : n);
+        const int b = 31 - __clz(base);
+        wbits = b * wc + (t ? (8 * wc + 4) / 5 : 0) + (u ? (7 * wc + 2) / 3 : 0);
+    }
+    write_bits((uint32_t)raw, 11, 0, w);
+    write_bits((uint32_t)(pc - 1), 2, 11, w);
+    if (pc > 1) {
+        write_bits(sb.seed & 0x3F, 6, 13, w);
+        write_bits(sb.seed >> 6, 4, 19, w);
+        if (sb.matched) {
+            write_bits((uint32_t)fmt_of(sb.formats, 0) << 2, 6, 23, w);
+        } else {
+            int low_class = 4;
+            for (int i = 0; i < pc; ++i) low_class = min(low_class, fmt_of(sb.formats, i) >> 2);
+            if (low_class == 3) low_class = 2;
+            int enc = low_class + 1, bitpos = 2;
+            for (int i = 0; i < pc; ++i) { enc |= ((fmt_of(sb.formats, i) >> 2) - low_class) << bitpos; ++bitpos; }
+            for (int i = 0; i < pc; ++i) { enc |= (fmt_of(sb.formats, i) & 3) << bitpos; bitpos += 2; }
+            const int hsize = 3 * pc - 4;
+            write_bits((uint32_t)(enc & 0x3F), 6, 23, w);
+            write_bits((uint32_t)(enc >> 6), hsize, 128 - wbits - hsize, w);
+        }
+    } else {
+        write_bits((uint32_t)fmt_of(sb.formats, 0), 4, 13, w);
+    }
+    uint8_t vals[32];
+    int nv = 0;
+    const int cq = sb.quant;
+    for (int i = 0; i < pc; ++i) {
+        const int n = 2 * (fmt_of(sb.formats, i) >> 2) + 2;
+        for (int j = 0; j < n; ++j) vals[nv + j] = c_qt.color_pquant[cq - 4][sb.packed[i * 8 + j]];
+        nv += n;
+    }
+    encode_ise(cq, nv, vals, w, pc == 1 ? 17 : 29);
+    uint4* o = reinterpret_cast<uint4*>(out + (size_t)k * 16);
+    *o = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+}  // namespace
+
+hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
+                        hipStream_t s) {
+    const int threads = 64;
+    for (int pc = 1; pc <= max_pc; ++pc) {
+        const int npos = pc == 1 ? 1 : 4;
+        const int npb = A.nb * npos;
+        if (pc > 1) {
+            hipLaunchKernelGGL(k_partition, dim3((A.nb + threads - 1) / threads), dim3(threads), 0, s, A, T, pc,
+                               B.pb_part);
+        }
+        hipLaunchKernelGGL(k_ideal_color, dim3((npb + threads - 1) / threads), dim3(threads), 0, s, A, T, pc,
+                           B.pb_part, npb, B.ideal, B.wts, B.ctab);
+        const int ndmw = npb * T.ndm;
+        hipLaunchKernelGGL(k_dm_bm, dim3((ndmw + threads - 1) / threads), dim3(threads), 0, s, A, T, pc, B.pb_part, npb,
+                           B.ideal, B.wts, B.ctab, B.ccr, B.qw);
+        hipLaunchKernelGGL(k_top, dim3((npb + threads - 1) / threads), dim3(threads), 0, s, npb, T.nbm, B.ccr,
+                           B.top);
+        hipLaunchKernelGGL(k_refine, dim3((npb * 8 + threads - 1) / threads), dim3(threads), 0, s, A, T, pc,
+                           B.pb_part, npb, B.ideal, B.top, B.ccr, B.qw, B.cand);
+        hipLaunchKernelGGL(k_pick, dim3((A.nb + threads - 1) / threads), dim3(threads), 0, s, A, T, pc, B.pb_part,
+                           B.cand, best);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack, dim3((nblocks + 63) / 64), dim3(64), 0, s, T, best, nblocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace astcg

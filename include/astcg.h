@@ -1,0 +1,139 @@
+/*
+ * astcg: ASTC LDR 2D block encoder for AMD Instinct MI355X (gfx950).
+ *
+ * C ABI that replaces the reference's C++ entry points
+ * (Jure-Z/ASTC_implementation):
+ *   ASTCEncoder::ASTCEncoder / init      code/astc.h:724-730,  compression_pipeline.cpp:200-233
+ *   ASTCEncoder::secondaryInit           code/astc.h:731,      compression_pipeline.cpp:263-321
+ *   ASTCEncoder::encode                  code/astc.h:733,      compression_pipeline.cpp:323-535
+ *   symbolic_to_physical                 code/astc.h:718-722,  physical_compression.cpp:843-1003
+ *   encode_ise                           code/astc.h:710-716,  physical_compression.cpp:672-826
+ *   get_ise_sequence_bitcount            code/astc.h:676,      physical_compression.cpp:625-639
+ *   store_image                          code/astc_store.h:8-16, astc_store.cpp:18-61
+ *   create_astc_file_in_memory           code/astc_store.h:18-30, astc_store.cpp:65-104
+ *
+ * Conventions: every function returns an astcg_status; nothing throws across
+ * the ABI.  The caller owns all host buffers.  One encoder per thread; calls
+ * on distinct encoders are independent.
+ */
+#ifndef ASTCG_H
+#define ASTCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ASTCG_OK = 0,
+    ASTCG_ERR_INVALID_ARGUMENT = 1,   /* null pointer, zero size, bad enum */
+    ASTCG_ERR_INVALID_FOOTPRINT = 2,  /* not one of the 14 2D ASTC footprints */
+    ASTCG_ERR_BUFFER_TOO_SMALL = 3,   /* rgba_len or out_len below what w x h needs */
+    ASTCG_ERR_NOT_CONFIGURED = 4,     /* encode before configure */
+    ASTCG_ERR_DEVICE = 5,             /* HIP runtime failure, see astcg_last_error */
+    ASTCG_ERR_OUT_OF_MEMORY = 6,
+    ASTCG_ERR_IO = 7,                 /* file could not be written */
+    ASTCG_ERR_UNSUPPORTED = 8         /* e.g. a symbolic block the packer cannot encode */
+} astcg_status;
+
+typedef struct astcg_encoder astcg_encoder;
+
+/* Candidate policy for partition counts 2..4.  COMPAT reproduces the
+ * reference's effective selection (packed table positions chosen by passes
+ * 001-007, including their indexing quirks). */
+typedef enum { ASTCG_POLICY_COMPAT = 0 } astcg_policy;
+
+typedef struct {
+    uint32_t max_partitions;  /* 1..4 (reference: 4) */
+    uint32_t policy;          /* astcg_policy */
+    uint32_t chunk_blocks;    /* blocks per device chunk, 0 = automatic */
+    uint32_t reserved[5];
+} astcg_params;
+
+/* SymbolicBlock (code/astc.h:639-655), byte-identical: 432 bytes. */
+typedef struct {
+    float errorval;
+    uint32_t block_mode_index;          /* raw 11-bit block mode */
+    uint32_t partition_count;
+    uint32_t partition_index;           /* 10-bit partition seed */
+    uint32_t partition_formats_matched;
+    uint32_t quant_mode;                /* colour endpoint quant level (4..20) */
+    uint32_t _padding1;
+    uint32_t _padding2;
+    uint32_t partition_formats[4];
+    uint32_t packed_color_values[32];   /* unquantised 0..255 */
+    uint32_t quantized_weights[64];     /* unquantised 0..64 */
+} astcg_symbolic_block;
+
+void astcg_default_params(astcg_params* p);
+
+/* ASTCEncoder(device) + init(): binds a HIP device. */
+astcg_status astcg_create(int hip_device, astcg_encoder** out);
+astcg_status astcg_destroy(astcg_encoder* enc);
+
+/* secondaryInit(w, h, bx, by): per image size and footprint. */
+astcg_status astcg_configure(astcg_encoder* enc, uint32_t width, uint32_t height, uint32_t block_x,
+                             uint32_t block_y, const astcg_params* params);
+
+/* encode(rgba, out, out_len): synchronous, host buffers.  rgba is row-major
+ * RGBA8 (width*height*4 bytes); out receives blocks_x*blocks_y 16-byte
+ * blocks in row-major block order. */
+astcg_status astcg_encode(astcg_encoder* enc, const uint8_t* rgba, size_t rgba_len, uint8_t* out,
+                          size_t out_len);
+
+/* Same on device memory; work is enqueued on `stream` (a hipStream_t, NULL =
+ * the null stream) and the call returns without synchronising.  The encoder
+ * keeps its scratch buffers until the next configure/destroy. */
+astcg_status astcg_encode_device(astcg_encoder* enc, const uint8_t* d_rgba, uint8_t* d_out, void* stream);
+
+/* Optional: copy the chosen symbolic blocks of the last encode (host memory,
+ * blocks_x*blocks_y entries). */
+astcg_status astcg_get_symbolic(astcg_encoder* enc, astcg_symbolic_block* out, size_t count);
+
+/* blocks_x, blocks_y and the number of blocks of the configured image. */
+astcg_status astcg_block_counts(const astcg_encoder* enc, uint32_t* blocks_x, uint32_t* blocks_y,
+                                uint32_t* num_blocks);
+
+/* symbolic_to_physical (host implementation of the device packer). */
+astcg_status astcg_symbolic_to_physical(uint32_t block_x, uint32_t block_y, const astcg_symbolic_block* sb,
+                                        uint8_t out[16]);
+
+/* encode_ise: bounded integer sequence encoding of count symbols (each below
+ * the quant level's alphabet size) at bit_offset of out. */
+astcg_status astcg_encode_ise(uint32_t quant, uint32_t count, const uint8_t* in, uint8_t* out,
+                              uint32_t bit_offset);
+
+/* get_ise_sequence_bitcount (1024 for an out-of-range quant level). */
+uint32_t astcg_ise_bitcount(uint32_t count, uint32_t quant);
+
+/* store_image: 16-byte .astc header (magic 0x5CA1AB13) followed by the blocks. */
+astcg_status astcg_store_image(uint32_t block_x, uint32_t block_y, uint32_t width, uint32_t height,
+                               const uint8_t* data, size_t data_len, const char* path);
+
+/* create_astc_file_in_memory: writes header + blocks into out (capacity
+ * out_cap); *out_len receives 16 + data_len. */
+astcg_status astcg_astc_file_in_memory(uint32_t block_x, uint32_t block_y, uint32_t width, uint32_t height,
+                                       const uint8_t* data, size_t data_len, uint8_t* out, size_t out_cap,
+                                       size_t* out_len);
+
+/* LDR decoder (ASTC specification), host: decodes num_blocks row-major
+ * blocks of a width x height image into RGBA8.  Not part of the reference;
+ * used to measure quality. */
+astcg_status astcg_decode_image(uint32_t block_x, uint32_t block_y, uint32_t width, uint32_t height,
+                                const uint8_t* blocks, size_t blocks_len, uint8_t* rgba, size_t rgba_len);
+
+/* Decodes one block to its symbolic form (inverse of symbolic_to_physical for
+ * the block modes the encoder emits).  Returns ASTCG_ERR_UNSUPPORTED for
+ * void-extent, dual-plane or HDR blocks. */
+astcg_status astcg_physical_to_symbolic(uint32_t block_x, uint32_t block_y, const uint8_t in[16],
+                                        astcg_symbolic_block* out);
+
+const char* astcg_last_error(const astcg_encoder* enc);
+const char* astcg_status_string(astcg_status s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
