@@ -54,8 +54,16 @@ class Params(ctypes.Structure):
         ("max_partitions", ctypes.c_uint32),
         ("policy", ctypes.c_uint32),
         ("chunk_blocks", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 5),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
+
+
+FLAG_PROFILE = 1
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("total_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
 
 
 _lib = None
@@ -86,6 +94,8 @@ def library():
         "astcg_astc_file_in_memory": (i32, [u32, u32, u32, u32, vp, sz, vp, sz, ctypes.POINTER(sz)]),
         "astcg_decode_image": (i32, [u32, u32, u32, u32, vp, sz, vp, sz]),
         "astcg_physical_to_symbolic": (i32, [u32, u32, vp, vp]),
+        "astcg_kernel_times": (i32, [vp, vp, sz, ctypes.POINTER(sz)]),
+        "astcg_reset_kernel_times": (i32, [vp]),
         "astcg_last_error": (ctypes.c_char_p, [vp]),
         "astcg_status_string": (ctypes.c_char_p, [i32]),
     }
@@ -130,12 +140,13 @@ class Encoder:
         except Exception:
             pass
 
-    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0):
+    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False):
         """secondaryInit (compression_pipeline.cpp:263-321)."""
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
+        p.flags = FLAG_PROFILE if profile else 0
         _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
         self.width, self.height, self.bx, self.by = width, height, bx, by
 
@@ -156,6 +167,17 @@ class Encoder:
         """Enqueues an encode of device buffers (integer addresses) on a HIP stream."""
         _check(library().astcg_encode_device(self._h, ctypes.c_void_p(d_rgba), ctypes.c_void_p(d_out),
                                              ctypes.c_void_p(stream or 0)), self._h)
+
+    def kernel_times(self):
+        """{kernel: {"total_ms", "launches"}} since the last reset (needs profile=True)."""
+        arr = (KernelTime * 16)()
+        n = ctypes.c_size_t()
+        _check(library().astcg_kernel_times(self._h, arr, 16, ctypes.byref(n)), self._h)
+        return {arr[i].name.decode(): {"total_ms": arr[i].total_ms, "launches": arr[i].launches}
+                for i in range(n.value)}
+
+    def reset_kernel_times(self):
+        _check(library().astcg_reset_kernel_times(self._h), self._h)
 
     def symbolic(self):
         n = self.num_blocks

@@ -37,6 +37,53 @@ struct astcg_encoder {
     SymBest* d_best = nullptr;
     uint8_t* d_rgba = nullptr;
     uint8_t* d_out = nullptr;
+    struct EventProfiler* prof = nullptr;
+};
+
+// Records a start/stop event pair around every kernel launch; the pairs are
+// resolved (and the pool recycled) when the times are queried.
+struct EventProfiler : Profiler {
+    struct Rec { int kernel; hipEvent_t a, b; };
+    std::vector<hipEvent_t> pool;
+    std::vector<Rec> recs;
+    double total_ms[K_COUNT] = {};
+    long long launches[K_COUNT] = {};
+    hipEvent_t open = nullptr;
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void begin(int, hipStream_t s) override {
+        open = take();
+        (void)hipEventRecord(open, s);
+    }
+    void end(int kernel, hipStream_t s) override {
+        hipEvent_t b = take();
+        (void)hipEventRecord(b, s);
+        recs.push_back({kernel, open, b});
+    }
+    void resolve() {
+        for (Rec& r : recs) {
+            float ms = 0.0f;
+            (void)hipEventSynchronize(r.b);
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            total_ms[r.kernel] += ms;
+            launches[r.kernel] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    ~EventProfiler() override {
+        resolve();
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
@@ -60,6 +107,8 @@ void release_image(astcg_encoder* e) {
     e->d_tables = e->d_chunk = nullptr;
     e->d_best = nullptr;
     e->d_rgba = e->d_out = nullptr;
+    delete e->prof;
+    e->prof = nullptr;
     e->configured = false;
 }
 
@@ -282,6 +331,7 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
     err = hipMalloc(&e->d_best, sizeof(SymBest) * e->nblocks);
     if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
+    if (p.flags & ASTCG_FLAG_PROFILE) e->prof = new EventProfiler();
     e->configured = true;
     return ASTCG_OK;
 }
@@ -301,10 +351,10 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
     for (int blk0 = 0; blk0 < (int)e->nblocks; blk0 += e->chunk_blocks) {
         A.blk0 = blk0;
         A.nb = std::min(e->chunk_blocks, (int)e->nblocks - blk0);
-        err = launch_chunk(e->T, A, e->B, e->d_best, (int)e->params.max_partitions, s);
+        err = launch_chunk(e->T, A, e->B, e->d_best, (int)e->params.max_partitions, s, e->prof);
         if (err != hipSuccess) return hip_fail(e, err, "search kernels");
     }
-    err = launch_pack(e->T, e->d_best, (int)e->nblocks, d_out, s);
+    err = launch_pack(e->T, e->d_best, (int)e->nblocks, d_out, s, e->prof);
     if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
     return ASTCG_OK;
 }
@@ -435,6 +485,34 @@ astcg_status astcg_decode_image(uint32_t bx, uint32_t by, uint32_t w, uint32_t h
 astcg_status astcg_physical_to_symbolic(uint32_t bx, uint32_t by, const uint8_t in[16], astcg_symbolic_block* out) {
     if (!in || !out) return ASTCG_ERR_INVALID_ARGUMENT;
     return (astcg_status)physical_to_symbolic_host((int)bx, (int)by, in, out);
+}
+
+astcg_status astcg_kernel_times(astcg_encoder* e, astcg_kernel_time* out, size_t cap, size_t* count) {
+    if (!e || !count) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!e->prof) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "profiling not enabled");
+    (void)hipSetDevice(e->device);
+    e->prof->resolve();
+    size_t n = 0;
+    for (int k = 0; k < K_COUNT; ++k) {
+        if (!e->prof->launches[k]) continue;
+        if (out && n < cap) {
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", kernel_name(k));
+            out[n].total_ms = e->prof->total_ms[k];
+            out[n].launches = (uint64_t)e->prof->launches[k];
+        }
+        ++n;
+    }
+    *count = n;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_reset_kernel_times(astcg_encoder* e) {
+    if (!e) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!e->prof) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "profiling not enabled");
+    (void)hipSetDevice(e->device);
+    e->prof->resolve();
+    for (int k = 0; k < K_COUNT; ++k) { e->prof->total_ms[k] = 0.0; e->prof->launches[k] = 0; }
+    return ASTCG_OK;
 }
 
 const char* astcg_last_error(const astcg_encoder* e) { return e ? e->last_error.c_str() : "null encoder"; }

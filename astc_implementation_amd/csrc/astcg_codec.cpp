@@ -172,6 +172,10 @@ int symbolic_to_physical_host(int bx, int by, const astcg_symbolic_block& sb, ui
             int low_class = 4;
             for (int i = 0; i < pc; ++i) low_class = std::min(low_class, (int)(sb.partition_formats[i] >> 2));
             if (low_class == 3) low_class = 2;
+            for (int i = 0; i < pc; ++i) {
+                const int d = (int)(sb.partition_formats[i] >> 2) - low_class;
+                if (d < 0 || d > 1) return ASTCG_ERR_UNSUPPORTED;  // formats span more than two classes
+            }
             int enc = low_class + 1, bitpos = 2;
             for (int i = 0; i < pc; ++i) enc |= ((int)(sb.partition_formats[i] >> 2) - low_class) << bitpos++;
             for (int i = 0; i < pc; ++i) { enc |= (int)(sb.partition_formats[i] & 3) << bitpos; bitpos += 2; }
@@ -190,6 +194,9 @@ int symbolic_to_physical_host(int bx, int by, const astcg_symbolic_block& sb, ui
         for (int j = 0; j < n; ++j) vals[nv + j] = qt.color_pquant[sb.quant_mode - 4][sb.packed_color_values[i * 8 + j] & 0xFF];
         nv += n;
     }
+    // the colour integers must end below the weights (and the extra CEM bits)
+    const int cem_bits = (pc > 1 && !sb.partition_formats_matched) ? 3 * pc - 4 : 0;
+    if ((pc == 1 ? 17 : 29) + ise_bitcount(nv, (int)sb.quant_mode) > 128 - wbits - cem_bits) return ASTCG_ERR_UNSUPPORTED;
     encode_ise_host((int)sb.quant_mode, nv, vals, blk, pc == 1 ? 17 : 29);
     std::memcpy(out, blk, 16);
     return ASTCG_OK;

@@ -105,9 +105,20 @@ struct ChunkBuffers {
     CandOut* cand;      // [4 * max_nb][8]
 };
 
+// Optional per-kernel timing with HIP events recorded around each launch.
+enum KernelId { K_PARTITION = 0, K_IDEAL_COLOR, K_DM_BM, K_TOP, K_REFINE, K_PICK, K_PACK, K_COUNT };
+const char* kernel_name(int id);
+
+struct Profiler {
+    virtual void begin(int kernel, hipStream_t s) = 0;
+    virtual void end(int kernel, hipStream_t s) = 0;
+    virtual ~Profiler() = default;
+};
+
 void upload_quant_tables(hipStream_t s);
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s);
-hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s);
+                        hipStream_t s, Profiler* prof);
+hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s,
+                       Profiler* prof);
 
 }  // namespace astcg

@@ -1504,7 +1504,11 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         if (cur_pb < 0 || be < cur_err) { cur_err = be; cur_pb = pb; cur_slot = bs; }
     }
     SymBest& sb = best[A.blk0 + k];
-    if (pc == 1) sb.error = 1e30f;
+    if (pc == 1) {
+        sb.error = 1e30f;
+        sb.raw_mode = 0xFFFF;
+        sb.pc = 0;
+    }
     if (cur_slot >= 0 && cur_err < sb.error) {
         const CandOut& co = cand[cur_pb * 8 + cur_slot];
         sb.error = cur_err;
@@ -1580,14 +1584,36 @@ __global__ void __launch_bounds__(64) k_pack(DevTables T, const SymBest* best, i
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nblocks) return;
     const SymBest& sb = best[k];
-    const int raw = sb.raw_mode;
+    uint4* o = reinterpret_cast<uint4*>(out + (size_t)k * 16);
+    const int raw = sb.raw_mode & 0x7FF;
     const int wc = T.raw_wc[raw], q = T.raw_quant[raw];
     const int pc = sb.pc;
+    // a block without any candidate (cannot happen for valid footprints) is
+    // emitted as an all-zero block (reserved mode, decodes to the error colour)
+    bool valid = sb.error < 1e30f && sb.raw_mode < 2048 && wc > 0 && pc >= 1 && pc <= 4 && sb.quant >= 4 &&
+                 sb.quant <= 20;
+    if (valid) {
+        int nv = 0;
+        for (int i = 0; i < pc; ++i) nv += 2 * (fmt_of(sb.formats, i) >> 2) + 2;
+        const int n = kLevelsD[sb.quant];
+        const int t = n % 3 == 0, u = n % 5 == 0;
+        const int cb = 31 - __clz(t ? n / 3 : (u ? n / 5 : n));
+        const int cbits = cb * nv + (t ? (8 * nv + 4) / 5 : 0) + (u ? (7 * nv + 2) / 3 : 0);
+        const int wq = kLevelsD[q];
+        const int wt = wq % 3 == 0, wu = wq % 5 == 0;
+        const int wb = 31 - __clz(wt ? wq / 3 : (wu ? wq / 5 : wq));
+        const int wbits_chk = wb * wc + (wt ? (8 * wc + 4) / 5 : 0) + (wu ? (7 * wc + 2) / 3 : 0);
+        valid = nv <= 32 && (pc == 1 ? 17 : 29) + cbits + ((pc > 1 && !sb.matched) ? 3 * pc - 4 : 0) <= 128 - wbits_chk;
+    }
+    if (!valid) {
+        *o = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
     uint32_t wb[4] = {0, 0, 0, 0};
     uint8_t wv[kMaxWeights];
     const float levels = (float)kLevelsD[q];
     for (int i = 0; i < wc; ++i) {
-        const float qw = ((float)sb.qw[i] / 64.0f) * (levels - 1.0f);
+        const float qw = ((float)min((int)sb.qw[i], 64) / 64.0f) * (levels - 1.0f);
         wv[i] = c_qt.weight_scramble[q][(int)(qw + 0.5f)];
     }
     encode_ise(q, wc, wv, wb, 0);
@@ -1632,39 +1658,47 @@ __global__ void __launch_bounds__(64) k_pack(DevTables T, const SymBest* best, i
         nv += n;
     }
     encode_ise(cq, nv, vals, w, pc == 1 ? 17 : 29);
-    uint4* o = reinterpret_cast<uint4*>(out + (size_t)k * 16);
     *o = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 }  // namespace
 
+const char* kernel_name(int id) {
+    static const char* names[K_COUNT] = {"k_partition", "k_ideal_color", "k_dm_bm", "k_top",
+                                         "k_refine", "k_pick", "k_pack"};
+    return id >= 0 && id < K_COUNT ? names[id] : "?";
+}
+
+#define ASTCG_LAUNCH(id, kernel, grid, ...)                                        \
+    do {                                                                           \
+        if (prof) prof->begin(id, s);                                              \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, s, __VA_ARGS__);  \
+        if (prof) prof->end(id, s);                                                \
+    } while (0)
+
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s) {
+                        hipStream_t s, Profiler* prof) {
     const int threads = 64;
+    auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
     for (int pc = 1; pc <= max_pc; ++pc) {
         const int npos = pc == 1 ? 1 : 4;
         const int npb = A.nb * npos;
-        if (pc > 1) {
-            hipLaunchKernelGGL(k_partition, dim3((A.nb + threads - 1) / threads), dim3(threads), 0, s, A, T, pc,
-                               B.pb_part);
-        }
-        hipLaunchKernelGGL(k_ideal_color, dim3((npb + threads - 1) / threads), dim3(threads), 0, s, A, T, pc,
-                           B.pb_part, npb, B.ideal, B.wts, B.ctab);
-        const int ndmw = npb * T.ndm;
-        hipLaunchKernelGGL(k_dm_bm, dim3((ndmw + threads - 1) / threads), dim3(threads), 0, s, A, T, pc, B.pb_part, npb,
-                           B.ideal, B.wts, B.ctab, B.ccr, B.qw);
-        hipLaunchKernelGGL(k_top, dim3((npb + threads - 1) / threads), dim3(threads), 0, s, npb, T.nbm, B.ccr,
-                           B.top);
-        hipLaunchKernelGGL(k_refine, dim3((npb * 8 + threads - 1) / threads), dim3(threads), 0, s, A, T, pc,
-                           B.pb_part, npb, B.ideal, B.top, B.ccr, B.qw, B.cand);
-        hipLaunchKernelGGL(k_pick, dim3((A.nb + threads - 1) / threads), dim3(threads), 0, s, A, T, pc, B.pb_part,
-                           B.cand, best);
+        if (pc > 1) ASTCG_LAUNCH(K_PARTITION, k_partition, blocks_for(A.nb), A, T, pc, B.pb_part);
+        ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab);
+        ASTCG_LAUNCH(K_DM_BM, k_dm_bm, blocks_for((long long)npb * T.ndm), A, T, pc, B.pb_part, npb, B.ideal, B.wts,
+                     B.ctab, B.ccr, B.qw);
+        ASTCG_LAUNCH(K_TOP, k_top, blocks_for(npb), npb, T.nbm, B.ccr, B.top);
+        ASTCG_LAUNCH(K_REFINE, k_refine, blocks_for((long long)npb * 8), A, T, pc, B.pb_part, npb, B.ideal, B.top,
+                     B.ccr, B.qw, B.cand);
+        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack, dim3((nblocks + 63) / 64), dim3(64), 0, s, T, best, nblocks, out);
+hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s,
+                       Profiler* prof) {
+    const int threads = 64;
+    ASTCG_LAUNCH(K_PACK, k_pack, (nblocks + threads - 1) / threads, T, best, nblocks, out);
     return hipGetLastError();
 }
 

@@ -1,0 +1,131 @@
+"""Throughput benchmark: Mtexel/s of the plain (compat) ASTC search on one
+MI355X, 4096x4096 RGBA8 synthetic image (gradients + value noise +
+rectangles, seed 1), 4x4 blocks, partition counts 1..4.
+
+One step = one full encode of the image from RGBA8 resident in HBM to the
+128-bit block stream in HBM.  Prints one JSON line (see the round driver's
+contract); the CPU baseline is the C oracle timed on a bounded sample of the
+same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+WIDTH = HEIGHT = 4096
+SEED = 1
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--block", default="4x4")
+    ap.add_argument("--size", type=int, default=WIDTH)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(img, bx, by, target_s):
+    """Times the oracle on the first blocks of the same image, growing the
+    sample until it takes about target_s seconds."""
+    import oracle_lib as O
+    threads = min(os.cpu_count() or 1, 16)
+    n = 256
+    while True:
+        t0 = time.perf_counter()
+        O.encode(img, bx, by, 4, threads=threads, first_block=0, block_count=n, want_symbolic=False)
+        dt = time.perf_counter() - t0
+        if dt >= target_s / 4 or n >= 1 << 20:
+            break
+        n *= 4
+    texels = n * bx * by
+    return {"value": texels / dt / 1e6, "unit": "Mtexel/s", "cores": threads, "kind": "port",
+            "sample": "first %d blocks (%d texels) of the same %dx%d image, C oracle, %.1f s" %
+                      (n, texels, img.shape[1], img.shape[0], dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (the encoder shards by block range with no "
+              "collectives; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    bx, by = (int(v) for v in a.block.split("x"))
+    import torch
+    import astc_implementation_amd as A
+    from astc_implementation_amd import synth
+
+    w = h = a.size
+    img = synth.gradient_image(w, h, SEED)
+    enc = A.Encoder(0)
+    enc.configure(w, h, bx, by, max_partitions=4, profile=True)
+    nblocks = enc.num_blocks
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(img.reshape(-1)).to(dev)
+    d_out = torch.zeros(nblocks * 16, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(a.warmup):
+        enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    enc.reset_kernel_times()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(a.steps):
+        enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / a.steps
+    texels = w * h
+    value = texels / (ms / 1e3) / 1e6
+    ktimes = enc.kernel_times()
+    dom = max(ktimes, key=lambda k: ktimes[k]["total_ms"])
+    launches = ktimes[dom]["launches"]
+    avg_ms = ktimes[dom]["total_ms"] / max(launches, 1)
+    # algorithmic HBM bytes per texel: RGBA8 read + 16-byte block write
+    bytes_per_texel = 4.0 + 16.0 / (bx * by)
+    texels_per_launch = texels * a.steps / max(launches, 1)
+    achieved = bytes_per_texel * texels_per_launch / (avg_ms / 1e3) / 1e9
+    blocks = d_out.cpu().numpy()
+    dec = A.decode_image(bx, by, w, h, blocks)
+    line = {
+        "metric": "Mtexel/s at 4096^2 RGBA8 %dx%d (plain search)" % (bx, by),
+        "value": value,
+        "unit": "Mtexel/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain compat search, %dx%d RGBA8 gradient+noise+rectangles seed %d, %dx%d blocks, "
+                               "partitions 1-4" % (w, h, SEED, bx, by),
+                   "blocks": nblocks, "psnr_rgba_db": round(A.psnr(dec, img), 3)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches},
+        "kernels_ms_per_step": {k: v["total_ms"] / a.steps for k, v in ktimes.items()},
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(img, bx, by, a.cpu_seconds)
+    enc.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

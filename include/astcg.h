@@ -45,12 +45,21 @@ typedef struct astcg_encoder astcg_encoder;
  * 001-007, including their indexing quirks). */
 typedef enum { ASTCG_POLICY_COMPAT = 0 } astcg_policy;
 
+#define ASTCG_FLAG_PROFILE 1u  /* record HIP events around every kernel launch */
+
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */
     uint32_t policy;          /* astcg_policy */
     uint32_t chunk_blocks;    /* blocks per device chunk, 0 = automatic */
-    uint32_t reserved[5];
+    uint32_t flags;           /* ASTCG_FLAG_* */
+    uint32_t reserved[4];
 } astcg_params;
+
+typedef struct {
+    char name[32];
+    double total_ms;
+    uint64_t launches;
+} astcg_kernel_time;
 
 /* SymbolicBlock (code/astc.h:639-655), byte-identical: 432 bytes. */
 typedef struct {
@@ -129,6 +138,13 @@ astcg_status astcg_decode_image(uint32_t block_x, uint32_t block_y, uint32_t wid
  * void-extent, dual-plane or HDR blocks. */
 astcg_status astcg_physical_to_symbolic(uint32_t block_x, uint32_t block_y, const uint8_t in[16],
                                         astcg_symbolic_block* out);
+
+/* Measurement hook with no counterpart in the reference (it has no timers):
+ * with ASTCG_FLAG_PROFILE set at configure, per-kernel device time summed
+ * over the launches since the last reset.  *count receives the number of
+ * kernels with launches; up to cap entries are written. */
+astcg_status astcg_kernel_times(astcg_encoder* enc, astcg_kernel_time* out, size_t cap, size_t* count);
+astcg_status astcg_reset_kernel_times(astcg_encoder* enc);
 
 const char* astcg_last_error(const astcg_encoder* enc);
 const char* astcg_status_string(astcg_status s);

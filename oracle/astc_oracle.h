@@ -100,6 +100,9 @@ int oracle_footprint_info(int bx, int by, int32_t* counts, int cap);
 int oracle_partition_entry(int bx, int by, int pc, int idx, uint8_t* part_of_texel,
                            int32_t* seed_and_counts);
 
+/* Decimation table of trial decimation mode m (see or_tables.c). */
+int oracle_decimation_entry(int bx, int by, int m, int32_t* out, int cap);
+
 /* Stage-level hooks used by unit tests (single partitioned block). */
 int oracle_debug_kmeans_candidates(const uint8_t* rgba, int width, int height, int bx, int by,
                                    int block_index, int partition_count, int batch_size,

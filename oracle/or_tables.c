@@ -610,3 +610,25 @@ int oracle_partition_entry(int bx, int by, int pc, int idx, uint8_t* pot, int32_
     sc[5] = pi->partition_count;
     return d->texel_count;
 }
+
+/* Decimation table of trial decimation mode `m`: out = {texel_count,
+ * weight_count, wx, wy, then per texel: count, 4 x (weight index, 16*contribution)} */
+int oracle_decimation_entry(int bx, int by, int m, int32_t* out, int cap) {
+    or_desc* d = or_desc_get(bx, by);
+    if (!d || m < 0 || m >= d->ndm) return -1;
+    const or_dm* di = &d->dm[d->valid_dm[m]];
+    int n = 0;
+#define PUT(v) do { if (n < cap) out[n] = (int32_t)(v); n++; } while (0)
+    PUT(di->texel_count); PUT(di->weight_count); PUT(di->wx); PUT(di->wy);
+    for (int t = 0; t < di->texel_count; t++) {
+        int c = di->texel_weight_count[t];
+        int off = di->t2w_off + di->texel_weights_offset[t];
+        PUT(c);
+        for (int j = 0; j < 4; j++) {
+            if (j < c) { PUT(d->t2w_idx[off + j]); PUT((int)(d->t2w_contrib[off + j] * 16.0f + 0.5f)); }
+            else { PUT(-1); PUT(0); }
+        }
+    }
+#undef PUT
+    return n;
+}

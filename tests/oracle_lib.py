@@ -70,6 +70,8 @@ def lib():
         L.oracle_partition_entry.restype = i32
         L.oracle_debug_kmeans_candidates.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, vp]
         L.oracle_debug_kmeans_candidates.restype = i32
+        L.oracle_decimation_entry.argtypes = [i32, i32, i32, vp, i32]
+        L.oracle_decimation_entry.restype = i32
         _lib = L
     return _lib
 
@@ -138,3 +140,20 @@ def compat_positions(rgba, bx, by, block_index, partition_count, batch_size=920)
     if r != 0:
         raise RuntimeError("oracle_debug_kmeans_candidates failed")
     return list(pos)
+
+
+def decimation_entry(bx, by, m):
+    """(texel_count, weight_count, wx, wy, [(weight index, 16*contribution) per texel])"""
+    cap = 4 + 144 * 9
+    buf = (ctypes.c_int32 * cap)()
+    n = lib().oracle_decimation_entry(bx, by, m, buf, cap)
+    if n < 0:
+        return None
+    v = list(buf)[:n]
+    tc, wc, wx, wy = v[:4]
+    texels = []
+    for t in range(tc):
+        base = 4 + t * 9
+        c = v[base]
+        texels.append([(v[base + 1 + 2 * j], v[base + 2 + 2 * j]) for j in range(c)])
+    return tc, wc, wx, wy, texels
