@@ -14,6 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libastcg.so")
+CLI = os.path.join(HERE, "astcg_cli")
 
 SOURCES = ["astcg_tables.cpp", "astcg_codec.cpp", "astcg_api.cpp", "astcg_device.hip"]
 
@@ -56,6 +57,14 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    # command-line front end (same arguments as the reference's native CLI)
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + INCLUDE, os.path.join(CSRC, "astcg_cli.cpp"), "-o", CLI + ".tmp",
+           "-L" + HERE, "-lastcg", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("cli build failed")
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 
