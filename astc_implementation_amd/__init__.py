@@ -76,6 +76,13 @@ def library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libastcg.so not built: run astc_implementation_amd.build.build()")
+    # PyTorch-ROCm ships its own HIP runtime; loading it first makes this
+    # library bind to the same runtime, so both can share device memory and
+    # streams in one process.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     vp, u32, sz, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_size_t, ctypes.c_int
     sig = {

@@ -653,7 +653,8 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int jx = 0; jx < t2wc[i]; ++jx) v += dw[t2wi[i * 4 + jx]] * t2ww[i * 4 + jx];
             infill[i] = v;
         }
-        float e0[kMaxWeights], e1[kMaxWeights];
+        // one refinement step per weight; each step only reads the infill of
+        // the initial estimate, so the weights can be updated in place
         for (int w = 0; w < wc; ++w) {
             float a = 1e-10f, bacc = 0.0f;
             const int n = w2tc[w], off = w2to[w];
@@ -664,10 +665,8 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 a += contrib * scale;
                 bacc += (infill[t] - wts[t]) * scale;
             }
-            e0[w] = a;
-            e1[w] = bacc;
+            dw[w] += wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
         }
-        for (int w = 0; w < wc; ++w) dw[w] += wclamp((e1[w] * -16.0f) / e0[w], -0.25f, 0.25f);
     }
     // pass03_compute_angular_offsets.wgsl:65-116
     float ao[16];
