@@ -329,6 +329,9 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->B.qw = b + offs[5];
     e->B.top = reinterpret_cast<int16_t*>(b + offs[6]);
     e->B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
+    int maxw = 4;
+    for (const DmInfo& d : e->fp->dm) maxw = std::max<int>(maxw, d.weight_count);
+    e->B.wstride = (maxw + 3) & ~3;
     err = hipMalloc(&e->d_best, sizeof(SymBest) * e->nblocks);
     if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
     if (p.flags & ASTCG_FLAG_PROFILE) e->prof = new EventProfiler();

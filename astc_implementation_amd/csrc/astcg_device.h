@@ -103,6 +103,7 @@ struct ChunkBuffers {
     uint8_t* qw;        // [4 * max_nb][nbm][64]
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]
+    int wstride;        // LDS row of quantised weights: max weight count of the trial grids, multiple of 4
 };
 
 // Optional per-kernel timing with HIP events recorded around each launch.

@@ -1208,212 +1208,270 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 }
 
 // ---- passes 13-17, refinement loop of compression_pipeline.cpp:473-484 ---------
-struct Cand {
-    float ep0[4][4], ep1[4][4];
-    int quant_level, quant_level_mod;
-    int formats[4];
-    uint8_t qw[kMaxWeights];
-    int matched, final_quant;
-    int final_formats[4];
-    uint32_t packed[32];
+// One lane per (partitioned block, candidate slot); a 64-lane workgroup holds
+// 8 partitioned blocks x 8 slots.  The blocks' texels and partition maps, and
+// each lane's quantised weights and unpacked endpoints, live in LDS; the
+// per-partition sums of pass13 are accumulated partition by partition (each
+// partition still sees its texels in index order, so the float results are
+// those of the interleaved loop).
+constexpr int kRefinePbs = 8;
+
+struct PackOut {
+    int format;
+    uint2 v;  // eight 0..255 values, little-endian bytes
 };
+
+__device__ __noinline__ PackOut pack_partition(float4 e0, float4 e1, float4 rgbs, int format, int q) {
+    const float a[4] = {e0.x, e0.y, e0.z, e0.w}, b[4] = {e1.x, e1.y, e1.z, e1.w};
+    const float r[4] = {rgbs.x, rgbs.y, rgbs.z, rgbs.w};
+    uint32_t v[8];
+    PackOut o;
+    o.format = pack_color_endpoints(a, b, r, format, q, v);
+    o.v.x = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
+    o.v.y = v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24);
+    return o;
+}
+
+__device__ __forceinline__ uint32_t pk_byte(uint2 v, int i) { return ((i < 4 ? v.x : v.y) >> (8 * (i & 3))) & 0xFF; }
+
+__device__ __forceinline__ int4 load_ue(const uchar4 u) { return make_int4(u.x * 257, u.y * 257, u.z * 257, u.w * 257); }
+
+size_t refine_lds_bytes(const DevTables& T, int wstride) {
+    const size_t px = (size_t)kRefinePbs * T.tc * sizeof(float4);
+    const size_t tp = ((size_t)kRefinePbs * T.tc + 15) & ~size_t(15);
+    return px + tp + 2 * 64 * (size_t)wstride + 2 * 4 * 64 * sizeof(uchar4);
+}
 
 __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
                                                const Ideal* ideal, const int16_t* top, const Ccr* ccr,
-                                               const uint8_t* qw_all, CandOut* cand_out) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= npb * 8) return;
-    const int pb = tid >> 3, slot = tid & 7;
+                                               const uint8_t* qw_all, CandOut* cand_out, int wstride) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tc = T.tc;
+    float4* s_px = reinterpret_cast<float4*>(smem);
+    uint8_t* s_tp = smem + (size_t)kRefinePbs * tc * sizeof(float4);
+    uint8_t* s_qw = s_tp + (((size_t)kRefinePbs * tc + 15) & ~size_t(15));
+    uint8_t* s_bqw = s_qw + 64 * wstride;
+    uchar4* s_ue0 = reinterpret_cast<uchar4*>(s_bqw + 64 * wstride);
+    uchar4* s_ue1 = s_ue0 + 4 * 64;
+    const int lane = threadIdx.x;
+    const int lpb = lane >> 3, slot = lane & 7;
+    const int pb = blockIdx.x * kRefinePbs + lpb;
+    // stage the texels (as search values) and partition maps of the 8 blocks
+    if (pb < npb) {
+        const Pb b0 = make_pb(A, T, pb_part, pc, pb);
+        for (int i = slot; i < tc; i += 8) {
+            float px[4];
+            texel_px(A, T, b0.g, i, px);
+            s_px[lpb * tc + i] = make_float4(px[0], px[1], px[2], px[3]);
+            s_tp[lpb * tc + i] = (uint8_t)texel_part(b0, i);
+        }
+    }
+    __syncthreads();
+    const int tid = pb * 8 + slot;
+    if (pb >= npb) return;
     CandOut* co = cand_out + tid;
     const int k = top[tid];
     if (k < 0) {  // pass12 produced no candidate for this slot
         co->error = kErrDefault;
         return;
     }
-    const Pb b = make_pb(A, T, pb_part, pc, pb);
+    const float4* bpx = s_px + lpb * tc;
+    const uint8_t* btp = s_tp + lpb * tc;
+    uint8_t* qw = s_qw + lane * wstride;
+    uint8_t* bqw = s_bqw + lane * wstride;
+    int ppc[4];
+    if (pc == 1) {
+        ppc[0] = tc; ppc[1] = ppc[2] = ppc[3] = 0;
+    } else {
+        const size_t row = (size_t)(pc - 2) * kPartitionings + pb_part[pb];
+        for (int i = 0; i < 4; ++i) ppc[i] = T.part_counts[row * 4 + i];
+    }
     const BmInfo bmi = T.bm[k];
     const DmInfo di = T.dm[bmi.dm];
-    const int tc = T.tc, wc = di.weight_count;
+    const int wc = di.weight_count;
     const uint8_t* t2wc = T.t2w_count + (size_t)bmi.dm * tc;
     const uint8_t* t2wi = T.t2w_idx + (size_t)bmi.dm * tc * 4;
     const float* t2ww = T.t2w_w + (size_t)bmi.dm * tc * 4;
     const uint8_t* w2tc = T.w2t_count + bmi.dm * 64;
     const uint16_t* w2to = T.w2t_off + bmi.dm * 64;
     const uint8_t* w2ti = T.w2t_idx + di.w2t_base;
-    const float* w2tw = T.w2t_w + di.w2t_base;
     const bool undecimated = tc == wc;
     const Ideal& I = ideal[pb];
     const Ccr cr = ccr[(size_t)pb * T.nbm + k];
-    Cand c;
-    for (int p = 0; p < 4; ++p)
-        for (int j = 0; j < 4; ++j) { c.ep0[p][j] = I.ep0[p][j]; c.ep1[p][j] = I.ep1[p][j]; }
-    c.quant_level = cr.ql;
-    c.quant_level_mod = cr.ql_mod;
-    for (int p = 0; p < 4; ++p) c.formats[p] = fmt_of(cr.formats, p);
-    const uint8_t* qsrc = qw_all + ((size_t)pb * T.nbm + k) * 64;
-    for (int i = 0; i < kMaxWeights; ++i) c.qw[i] = qsrc[i];
+    float4 ep0[4], ep1[4], avg[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        ep0[p] = make_float4(I.ep0[p][0], I.ep0[p][1], I.ep0[p][2], I.ep0[p][3]);
+        ep1[p] = make_float4(I.ep1[p][0], I.ep1[p][1], I.ep1[p][2], I.ep1[p][3]);
+        avg[p] = make_float4(I.avg[p][0], I.avg[p][1], I.avg[p][2], I.avg[p][3]);
+    }
+    const int ql = cr.ql, qlm = cr.ql_mod;
+    const uint16_t in_formats = cr.formats;
+    {
+        const uint8_t* qsrc = qw_all + ((size_t)pb * T.nbm + k) * 64;
+        for (int i = 0; i < wc; ++i) qw[i] = qsrc[i];
+    }
     const float lsw = (kCW[0] + kCW[1]) + kCW[2];
+    const float cw[4] = {kCW[0], kCW[1], kCW[2], kCW[3]};
     float best_err = kErrDefault;
-    CandOut best;
-    best.error = kErrDefault;
-    int ue0[4][4], ue1[4][4];
+    int best_matched = 0, best_quant = 0;
+    uint16_t best_formats = 0;
+    uint2 best_pk[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
+    auto weight_of = [&](int i) -> float {
+        if (undecimated) return (float)qw[i] / 64.0f;
+        float w = 0.0f;
+        const int n = t2wc[i];
+        for (int j = 0; j < n; ++j) w += ((float)qw[t2wi[i * 4 + j]] / 64.0f) * t2ww[i * 4 + j];
+        return w;
+    };
     for (int a = 0; a < 6; ++a) {
         // pass13_recompute_ideal_endpoints.wgsl:182-393
-        float rgbs[4][4];
-        {
-            float wmin1[4], wmax1[4], ls[4], ms[4], rs[4], smin[4], smax[4], cx[4][4], cy[4][4], sv[4][2], sdir[4][3];
-            for (int p = 0; p < pc; ++p) {
-                wmin1[p] = 1.0f; wmax1[p] = 0.0f; ls[p] = ms[p] = rs[p] = 0.0f; smin[p] = 1e10f; smax[p] = 0.0f;
-                for (int j = 0; j < 4; ++j) { cx[p][j] = 0.0f; cy[p][j] = 0.0f; }
-                sv[p][0] = sv[p][1] = 0.0f;
-                const float size = (float)b.ppc[p];
+        float4 rgbs[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            rgbs[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (p >= pc) continue;
+            const float size = (float)ppc[p];
+            float sdir[3];
+            {
+                const float av[3] = {avg[p].x, avg[p].y, avg[p].z};
                 float q3[3];
-                for (int j = 0; j < 3; ++j) {
-                    const float rsum = (I.avg[p][j] * size) * kCW[j];
-                    const float rws = wmax(kCW[j] * size, 1e-17f);
-                    q3[j] = rsum / rws;
-                }
-                normalize3(q3, sdir[p]);
+                for (int j = 0; j < 3; ++j) q3[j] = ((av[j] * size) * cw[j]) / wmax(cw[j] * size, 1e-17f);
+                normalize3(q3, sdir);
             }
+            float wmin1 = 1.0f, wmax1 = 0.0f, ls = 0.0f, ms = 0.0f, rs = 0.0f, smin = 1e10f, smax = 0.0f;
+            float cx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, cy[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sv0 = 0.0f, sv1 = 0.0f;
             for (int i = 0; i < tc; ++i) {
-                float w;
-                if (undecimated) w = (float)c.qw[i] / 64.0f;
-                else {
-                    w = 0.0f;
-                    for (int j = 0; j < t2wc[i]; ++j) w += ((float)c.qw[t2wi[i * 4 + j]] / 64.0f) * t2ww[i * 4 + j];
-                }
-                const int p = texel_part(b, i);
-                float rgba[4];
-                texel_px(A, T, b.g, i, rgba);
-                if (w < wmin1[p]) wmin1[p] = w;
-                if (w > wmax1[p]) wmax1[p] = w;
-                const float sc = dot3(rgba, sdir[p]);
-                if (sc < smin[p]) smin[p] = sc;
-                if (sc > smax[p]) smax[p] = sc;
+                if (pc > 1 && btp[i] != p) continue;
+                const float w = weight_of(i);
+                const float4 px4 = bpx[i];
+                const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
+                if (w < wmin1) wmin1 = w;
+                if (w > wmax1) wmax1 = w;
+                const float sc = dot3(rgba, sdir);
+                if (sc < smin) smin = sc;
+                if (sc > smax) smax = sc;
                 const float om = 1.0f - w;
-                ls[p] += om * om;
-                ms[p] += om * w;
-                rs[p] += w * w;
+                ls += om * om;
+                ms += om * w;
+                rs += w * w;
                 for (int j = 0; j < 4; ++j) {
                     const float cyk = rgba[j] * w;
-                    cy[p][j] += cyk;
-                    cx[p][j] += rgba[j] - cyk;
+                    cy[j] += cyk;
+                    cx[j] += rgba[j] - cyk;
                 }
-                sv[p][0] += om * sc * lsw;
-                sv[p][1] += w * sc * lsw;
+                sv0 += om * sc * lsw;
+                sv1 += w * sc * lsw;
             }
-            for (int p = 0; p < pc; ++p) {
-                float colx[4], coly[4], lv[4], mv[4], rv[4];
+            float colx[4], coly[4], lv[4], mv[4], rv[4];
+            for (int j = 0; j < 4; ++j) {
+                colx[j] = cx[j] * cw[j]; coly[j] = cy[j] * cw[j];
+                lv[j] = ls * cw[j]; mv[j] = ms * cw[j]; rv[j] = rs * cw[j];
+            }
+            const float lx = ls * lsw, ly = ms * lsw, lz = rs * lsw;
+            const float scalediv = wclamp(smin / wmax(smax, 1e-10f), 0.0f, 1.0f);
+            rgbs[p] = make_float4(sdir[0] * smax, sdir[1] * smax, sdir[2] * smax, scalediv);
+            float e0v[4] = {ep0[p].x, ep0[p].y, ep0[p].z, ep0[p].w};
+            float e1v[4] = {ep1[p].x, ep1[p].y, ep1[p].z, ep1[p].w};
+            if (wmin1 >= wmax1 * 0.999f) {
                 for (int j = 0; j < 4; ++j) {
-                    colx[j] = cx[p][j] * kCW[j]; coly[j] = cy[p][j] * kCW[j];
-                    lv[j] = ls[p] * kCW[j]; mv[j] = ms[p] * kCW[j]; rv[j] = rs[p] * kCW[j];
+                    const float ac = (colx[j] + coly[j]) / wmax(cw[j] * size, 1e-17f);
+                    if (ac == ac) { e0v[j] = ac; e1v[j] = ac; }
                 }
-                const float lx = ls[p] * lsw, ly = ms[p] * lsw, lz = rs[p] * lsw;
-                const float scalediv = wclamp(smin[p] / wmax(smax[p], 1e-10f), 0.0f, 1.0f);
-                rgbs[p][0] = sdir[p][0] * smax[p]; rgbs[p][1] = sdir[p][1] * smax[p]; rgbs[p][2] = sdir[p][2] * smax[p];
-                rgbs[p][3] = scalediv;
-                if (wmin1[p] >= wmax1[p] * 0.999f) {
-                    const float size = (float)b.ppc[p];
-                    for (int j = 0; j < 4; ++j) {
-                        const float ac = (colx[j] + coly[j]) / wmax(kCW[j] * size, 1e-17f);
-                        if (ac == ac) { c.ep0[p][j] = ac; c.ep1[p][j] = ac; }
-                    }
-                    rgbs[p][3] = 1.0f;
-                } else {
-                    for (int j = 0; j < 4; ++j) {
-                        const float det = (lv[j] * rv[j]) - (mv[j] * mv[j]);
-                        const float rdet = 1.0f / det;
-                        const float mss = ((lv[j] * lv[j]) + 2.0f * (mv[j] * mv[j])) + (rv[j] * rv[j]);
-                        const float e0 = (rv[j] * colx[j] - mv[j] * coly[j]) * rdet;
-                        const float e1 = (lv[j] * coly[j] - mv[j] * colx[j]) * rdet;
-                        if (fabsf(det) > (mss * 1e-4f) && e0 == e0 && e1 == e1) { c.ep0[p][j] = e0; c.ep1[p][j] = e1; }
-                    }
-                    const float lsdet = (lx * lz) - (ly * ly);
-                    const float lsrdet = 1.0f / lsdet;
-                    const float lsmss = ((lx * lx) + 2.0f * (ly * ly)) + (lz * lz);
-                    const float se0 = (lz * sv[p][0] - ly * sv[p][1]) * lsrdet;
-                    const float se1 = (lx * sv[p][1] - ly * sv[p][0]) * lsrdet;
-                    if ((fabsf(lsdet) > (lsmss * 1e-4f)) && se0 == se0 && se1 == se1 && se0 < se1) {
-                        rgbs[p][0] = sdir[p][0] * se1; rgbs[p][1] = sdir[p][1] * se1; rgbs[p][2] = sdir[p][2] * se1;
-                        rgbs[p][3] = se0 / se1;
-                    }
+                rgbs[p].w = 1.0f;
+            } else {
+                for (int j = 0; j < 4; ++j) {
+                    const float det = (lv[j] * rv[j]) - (mv[j] * mv[j]);
+                    const float rdet = 1.0f / det;
+                    const float mss = ((lv[j] * lv[j]) + 2.0f * (mv[j] * mv[j])) + (rv[j] * rv[j]);
+                    const float e0 = (rv[j] * colx[j] - mv[j] * coly[j]) * rdet;
+                    const float e1 = (lv[j] * coly[j] - mv[j] * colx[j]) * rdet;
+                    if (fabsf(det) > (mss * 1e-4f) && e0 == e0 && e1 == e1) { e0v[j] = e0; e1v[j] = e1; }
                 }
+                const float lsdet = (lx * lz) - (ly * ly);
+                const float lsrdet = 1.0f / lsdet;
+                const float lsmss = ((lx * lx) + 2.0f * (ly * ly)) + (lz * lz);
+                const float se0 = (lz * sv0 - ly * sv1) * lsrdet;
+                const float se1 = (lx * sv1 - ly * sv0) * lsrdet;
+                if ((fabsf(lsdet) > (lsmss * 1e-4f)) && se0 == se0 && se1 == se1 && se0 < se1)
+                    rgbs[p] = make_float4(sdir[0] * se1, sdir[1] * se1, sdir[2] * se1, se0 / se1);
             }
+            ep0[p] = make_float4(e0v[0], e0v[1], e0v[2], e0v[3]);
+            ep1[p] = make_float4(e1v[0], e1v[1], e1v[2], e1v[3]);
         }
         // pass14_pack_color_endpoints.wgsl:1226-1304
-        {
-            bool all_same = c.quant_level != c.quant_level_mod;
-            int fn[4] = {0, 0, 0, 0};
-            uint32_t vn[32];
-            for (int i = 0; i < 32; ++i) vn[i] = 0;
-            for (int p = 0; p < pc; ++p) {
-                const int f = pack_color_endpoints(c.ep0[p], c.ep1[p], rgbs[p], c.formats[p], c.quant_level, vn + p * 8);
-                fn[p] = f;
-                if (p > 0) all_same = all_same && (f == fn[0]);
-            }
-            bool matched = false;
-            if (pc >= 2 && all_same) {
-                int fm[4] = {0, 0, 0, 0};
-                uint32_t vm[32];
-                for (int i = 0; i < 32; ++i) vm[i] = 0;
-                bool same_mod = true;
-                for (int p = 0; p < pc; ++p) {
-                    const int f = pack_color_endpoints(c.ep0[p], c.ep1[p], rgbs[p], c.formats[p], c.quant_level_mod, vm + p * 8);
-                    fm[p] = f;
-                    if (p > 0 && f != fm[0]) { same_mod = false; break; }
-                }
-                if (same_mod) {
-                    matched = true;
-                    for (int p = 0; p < 4; ++p) fn[p] = fm[p];
-                    for (int i = 0; i < 32; ++i) vn[i] = vm[i];
-                }
-            }
-            c.matched = matched ? 1 : 0;
-            c.final_quant = matched ? c.quant_level_mod : c.quant_level;
-            for (int p = 0; p < 4; ++p) c.final_formats[p] = fn[p];
-            for (int i = 0; i < 32; ++i) c.packed[i] = vn[i];
+        int fn[4] = {0, 0, 0, 0};
+        uint2 pk[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
+        bool all_same = ql != qlm;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (p >= pc) continue;
+            const PackOut o = pack_partition(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), ql);
+            fn[p] = o.format;
+            pk[p] = o.v;
+            if (p > 0) all_same = all_same && (o.format == fn[0]);
         }
+        bool matched = false;
+        if (pc >= 2 && all_same) {
+            int fm[4] = {0, 0, 0, 0};
+            uint2 pm[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
+            bool same_mod = true;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                if (p >= pc || !same_mod) continue;
+                const PackOut o = pack_partition(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), qlm);
+                fm[p] = o.format;
+                pm[p] = o.v;
+                if (p > 0 && o.format != fm[0]) same_mod = false;
+            }
+            if (same_mod) {
+                matched = true;
+#pragma unroll
+                for (int p = 0; p < 4; ++p) { fn[p] = fm[p]; pk[p] = pm[p]; }
+            }
+        }
+        const int final_quant = matched ? qlm : ql;
         // pass15_unpack_color_endpoints.wgsl:307-391
-        for (int p = 0; p < pc; ++p) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (p >= pc) continue;
+            uint32_t v[8];
+            for (int i = 0; i < 8; ++i) v[i] = pk_byte(pk[p], i);
             int o0[4], o1[4];
-            unpack_color_endpoints(c.final_formats[p], c.packed + p * 8, o0, o1);
-            for (int j = 0; j < 4; ++j) { ue0[p][j] = o0[j] * 257; ue1[p][j] = o1[j] * 257; }
+            unpack_color_endpoints(fn[p], v, o0, o1);
+            s_ue0[p * 64 + lane] = make_uchar4(o0[0], o0[1], o0[2], o0[3]);
+            s_ue1[p * 64 + lane] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
         }
-        // pass17 (before pass16 on the first iteration) and pass16 + pass17
+        // pass17 (before pass16 on the first iteration), then pass16 + pass17
         for (int phase = (a == 0 ? 0 : 1); phase < 2; ++phase) {
             if (phase == 1) {
                 // pass16_realign_weights.wgsl:211-367
-                float base[4][4], offs[4][4];
-                for (int p = 0; p < pc; ++p)
-                    for (int j = 0; j < 4; ++j) {
-                        base[p][j] = (float)ue0[p][j];
-                        offs[p][j] = (float)(ue1[p][j] - ue0[p][j]) * (1.0f / 64.0f);
-                    }
                 const int q = bmi.quant;
                 if (undecimated) {
                     for (int t = 0; t < tc; ++t) {
-                        const int uqw = c.qw[t];
+                        const int uqw = qw[t];
                         const uint32_t pn = c_qt.prev_next[q][uqw];
                         const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
                         const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
-                        const int p = texel_part(b, t);
-                        float px[4];
-                        texel_px(A, T, b.g, t, px);
+                        const int p = btp[t];
+                        const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
+                        const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
+                        const float4 px4 = bpx[t];
+                        const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                         float s0[4], s1[4], s2[4];
                         for (int j = 0; j < 4; ++j) {
-                            const float diff = (base[p][j] + offs[p][j] * wb) - px[j];
-                            const float dfd = diff + offs[p][j] * dd, dfu = diff + offs[p][j] * udf;
+                            const float base = (float)e0[j], offs = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+                            const float diff = (base + offs * wb) - px[j];
+                            const float dfd = diff + offs * dd, dfu = diff + offs * udf;
                             s0[j] = diff * diff; s1[j] = dfd * dfd; s2[j] = dfu * dfu;
                         }
-                        const float eb = dot4(s0, kCW), ed = dot4(s1, kCW), eu = dot4(s2, kCW);
-                        if ((eu < eb) && (eu < ed) && (uqw < 64)) c.qw[t] = (uint8_t)up;
-                        else if ((ed < eb) && (uqw > 0)) c.qw[t] = (uint8_t)down;
+                        const float eb = dot4(s0, cw), ed = dot4(s1, cw), eu = dot4(s2, cw);
+                        if ((eu < eb) && (eu < ed) && (uqw < 64)) qw[t] = (uint8_t)up;
+                        else if ((ed < eb) && (uqw > 0)) qw[t] = (uint8_t)down;
                     }
                 } else {
-                    float uq[kMaxWeights];
-                    for (int i = 0; i < wc; ++i) uq[i] = (float)c.qw[i];
                     for (int we = 0; we < wc; ++we) {
-                        const int uqw = c.qw[we];
+                        const int uqw = qw[we];
                         const uint32_t pn = c_qt.prev_next[q][uqw];
                         const float down = (float)(pn & 0xFF), up = (float)((pn >> 8) & 0xFF), wbase = (float)uqw;
                         const float ddn = down - wbase, dup = up - wbase;
@@ -1425,62 +1483,70 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
                             bool found = false;
                             for (int j = 0; j < t2wc[t]; ++j) {
                                 const int wi = t2wi[t * 4 + j];
-                                wbt += uq[wi] * t2ww[t * 4 + j];
+                                wbt += (float)qw[wi] * t2ww[t * 4 + j];
                                 if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
                             }
                             const float wdd = ddn * twb, wud = dup * twb;
-                            const int p = texel_part(b, t);
-                            float px[4];
-                            texel_px(A, T, b.g, t, px);
+                            const int p = btp[t];
+                            const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
+                            const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
+                            const float4 px4 = bpx[t];
+                            const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                             for (int j = 0; j < 4; ++j) {
-                                const float diff = (base[p][j] + offs[p][j] * wbt) - px[j];
-                                const float dd = diff + offs[p][j] * wdd, du = diff + offs[p][j] * wud;
+                                const float base = (float)e0[j], offs = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+                                const float diff = (base + offs * wbt) - px[j];
+                                const float dd = diff + offs * wdd, du = diff + offs * wud;
                                 ebv[j] += diff * diff;
                                 edv[j] += dd * dd;
                                 euv[j] += du * du;
                             }
                         }
-                        const float eb = dot4(ebv, kCW), ed = dot4(edv, kCW), eu = dot4(euv, kCW);
-                        if ((eu < eb) && (eu < ed) && (uqw < 64)) { c.qw[we] = (uint8_t)up; uq[we] = up; }
-                        else if ((ed < eb) && (uqw > 0)) { c.qw[we] = (uint8_t)down; uq[we] = down; }
+                        const float eb = dot4(ebv, cw), ed = dot4(edv, cw), eu = dot4(euv, cw);
+                        if ((eu < eb) && (eu < ed) && (uqw < 64)) qw[we] = (uint8_t)up;
+                        else if ((ed < eb) && (uqw > 0)) qw[we] = (uint8_t)down;
                     }
                 }
             }
             // pass17_compute_final_error.wgsl:149-231
             float total = 0.0f;
             for (int i = 0; i < tc; ++i) {
-                float w;
-                if (undecimated) w = (float)c.qw[i] / 64.0f;
-                else {
-                    w = 0.0f;
-                    for (int j = 0; j < t2wc[i]; ++j) w += ((float)c.qw[t2wi[i * 4 + j]] / 64.0f) * t2ww[i * 4 + j];
-                }
-                const int p = texel_part(b, i);
-                float px[4];
-                texel_px(A, T, b.g, i, px);
+                const float w = weight_of(i);
+                const int p = btp[i];
+                const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
+                const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
+                const float4 px4 = bpx[i];
+                const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                 const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
                 float dd[4];
                 for (int j = 0; j < 4; ++j) {
-                    const int col = (ue0[p][j] * w0 + ue1[p][j] * w1 + 32) >> 6;
+                    const int col = (e0[j] * w0 + e1[j] * w1 + 32) >> 6;
                     const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
                     dd[j] = df * df;
                 }
-                total += wmin(dot4(dd, kCW), kErrDefault);
+                total += wmin(dot4(dd, cw), kErrDefault);
             }
             if (total < best_err) {
                 best_err = total;
-                best.error = total;
-                best.bm = (uint16_t)k;
-                best.matched = (uint8_t)c.matched;
-                best.quant = (uint8_t)c.final_quant;
-                best.formats = pack_fmt4(c.final_formats);
-                for (int i = 0; i < 32; ++i) best.packed[i] = (uint8_t)c.packed[i];
-                for (int i = 0; i < 64; ++i) best.qw[i] = c.qw[i];
+                best_matched = matched ? 1 : 0;
+                best_quant = final_quant;
+                best_formats = pack_fmt4(fn);
+#pragma unroll
+                for (int p = 0; p < 4; ++p) best_pk[p] = pk[p];
+                for (int i = 0; i < wc; ++i) bqw[i] = qw[i];
             }
         }
     }
-    (void)slot;
-    (void)w2tw;
+    CandOut best;
+    best.error = best_err;
+    best.bm = (uint16_t)k;
+    best.matched = (uint8_t)best_matched;
+    best.quant = (uint8_t)best_quant;
+    best.formats = best_formats;
+    best.pad = 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+        for (int i = 0; i < 8; ++i) best.packed[p * 8 + i] = (uint8_t)pk_byte(best_pk[p], i);
+    for (int i = 0; i < 64; ++i) best.qw[i] = i < wc ? bqw[i] : 0;
     *co = best;
 }
 
@@ -1687,8 +1753,13 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
         ASTCG_LAUNCH(K_DM_BM, k_dm_bm, blocks_for((long long)npb * T.ndm), A, T, pc, B.pb_part, npb, B.ideal, B.wts,
                      B.ctab, B.ccr, B.qw);
         ASTCG_LAUNCH(K_TOP, k_top, blocks_for(npb), npb, T.nbm, B.ccr, B.top);
-        ASTCG_LAUNCH(K_REFINE, k_refine, blocks_for((long long)npb * 8), A, T, pc, B.pb_part, npb, B.ideal, B.top,
-                     B.ccr, B.qw, B.cand);
+        {
+            if (prof) prof->begin(K_REFINE, s);
+            const size_t lds = refine_lds_bytes(T, B.wstride);
+            hipLaunchKernelGGL(k_refine, dim3((npb + kRefinePbs - 1) / kRefinePbs), dim3(64), lds, s, A, T, pc,
+                               B.pb_part, npb, B.ideal, B.top, B.ccr, B.qw, B.cand, B.wstride);
+            if (prof) prof->end(K_REFINE, s);
+        }
         ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best);
     }
     return hipGetLastError();

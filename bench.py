@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--size", type=int, default=WIDTH)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain device-to-device copy of this many MiB before the encode "
+                         "(a known-size reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -75,6 +78,13 @@ def main():
     d_in = torch.from_numpy(img.reshape(-1)).to(dev)
     d_out = torch.zeros(nblocks * 16, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
+    if a.copy_probe_mib:
+        src = torch.ones(a.copy_probe_mib << 18, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(a.warmup):
         enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
     torch.cuda.synchronize()
