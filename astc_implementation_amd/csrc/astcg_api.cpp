@@ -135,6 +135,14 @@ hipError_t upload_tables(astcg_encoder* e) {
             if (f.bm[k].dm == m) list.push_back((uint16_t)k);
     }
     off[f.ndm] = (uint16_t)list.size();
+    // packed per-block row of quantised weights: each trial mode keeps only
+    // its own weight count
+    std::vector<uint16_t> qw_off(std::max(f.nbm, 1), 0);
+    int qw_total = 0;
+    for (int k = 0; k < f.nbm; ++k) {
+        qw_off[k] = (uint16_t)qw_total;
+        qw_total += f.dm[f.bm[k].dm].weight_count;
+    }
     std::vector<uint16_t> ordering(3 * 2 * 128);
     for (int p = 0; p < 3; ++p)
         for (int v = 0; v < 2; ++v)
@@ -162,6 +170,7 @@ hipError_t upload_tables(astcg_encoder* e) {
     const size_t o_r2b = L.add<int16_t>(f.raw_to_bm.size());
     const size_t o_rwc = L.add<uint8_t>(f.raw_wc.size());
     const size_t o_rq = L.add<uint8_t>(f.raw_quant.size());
+    const size_t o_qwo = L.add<uint16_t>(qw_off.size());
     std::vector<uint8_t> host(L.size, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
     put(o_dm, f.dm.data(), f.dm.size() * sizeof(DmInfo));
@@ -186,6 +195,7 @@ hipError_t upload_tables(astcg_encoder* e) {
     put(o_r2b, f.raw_to_bm.data(), f.raw_to_bm.size() * 2);
     put(o_rwc, f.raw_wc.data(), f.raw_wc.size());
     put(o_rq, f.raw_quant.data(), f.raw_quant.size());
+    put(o_qwo, qw_off.data(), qw_off.size() * 2);
     hipError_t err = hipMalloc(&e->d_tables, L.size);
     if (err != hipSuccess) return err;
     err = hipMemcpy(e->d_tables, host.data(), L.size, hipMemcpyHostToDevice);
@@ -214,6 +224,8 @@ hipError_t upload_tables(astcg_encoder* e) {
     T.raw_to_bm = reinterpret_cast<const int16_t*>(b + o_r2b);
     T.raw_wc = b + o_rwc;
     T.raw_quant = b + o_rq;
+    T.qw_off = reinterpret_cast<const uint16_t*>(b + o_qwo);
+    T.qw_stride = (qw_total + 15) & ~15;
     T.bx = f.bx;
     T.by = f.by;
     T.tc = tc;
@@ -231,7 +243,9 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[2] = L.add<float>(npb * f.texel_count);
     offs[3] = L.add<ColorTab>(npb);
     offs[4] = L.add<Ccr>(npb * f.nbm);
-    offs[5] = L.add<uint8_t>(npb * f.nbm * 64);
+    int qw_total = 0;
+    for (int k = 0; k < f.nbm; ++k) qw_total += f.dm[f.bm[k].dm].weight_count;
+    offs[5] = L.add<uint8_t>(npb * (size_t)((qw_total + 15) & ~15));
     offs[6] = L.add<int16_t>(npb * 8);
     offs[7] = L.add<CandOut>(npb * 8);
     return L.size;

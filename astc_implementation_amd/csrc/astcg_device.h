@@ -32,6 +32,8 @@ struct DevTables {
     const int16_t* raw_to_bm;     // [2048]
     const uint8_t* raw_wc;        // [2048]
     const uint8_t* raw_quant;     // [2048]
+    const uint16_t* qw_off;       // [nbm] offset of a trial mode's weights in a block's packed weight row
+    int qw_stride;                // bytes of packed quantised weights per partitioned block
     int bx, by, tc, ndm, nbm;
     int count_selected[4];
 };
@@ -100,7 +102,7 @@ struct ChunkBuffers {
     float* wts;         // [4 * max_nb][tc]
     ColorTab* ctab;     // [4 * max_nb]
     Ccr* ccr;           // [4 * max_nb][nbm]
-    uint8_t* qw;        // [4 * max_nb][nbm][64]
+    uint8_t* qw;        // [4 * max_nb][qw_stride]
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]
     int wstride;        // LDS row of quantised weights: max weight count of the trial grids, multiple of 4

@@ -151,33 +151,89 @@ __device__ __forceinline__ uint16_t pack_fmt4(const int* f) {
 __device__ __forceinline__ int fmt_of(uint16_t v, int p) { return (v >> (4 * p)) & 0xF; }
 
 // ---- passes 001-007: compat partition candidates ---------------------------
-__device__ float dist_sq(const float* a, const float* b) {
+// A 64-lane workgroup handles 4 blocks with 16 lanes each.  The blocks'
+// texels sit in LDS.  k-means runs texel-parallel: every cross-texel
+// reduction in passes 001-003 is an integer sum, an integer max or a
+// highest-index pick, so splitting it over lanes gives the same result as
+// the sequential loop.  The 128 ordered candidates of pass006 are spread over
+// the 16 lanes, and the in-order top-4 insertion of pass007 runs on the
+// group's first lane.
+constexpr int kPartBlocks = 4;
+constexpr int kPartLanes = 16;
+constexpr int kPartMaxR = (kMaxTexels + kPartLanes - 1) / kPartLanes;
+
+__device__ __forceinline__ float dist_sq(const float* a, const float* b) {
     float d[4];
     for (int c = 0; c < 4; ++c) { const float t = a[c] - b[c]; d[c] = t * t; }
     return dot4(d, kCW);
 }
+__device__ __forceinline__ void ld4(const float4* p, int i, float* o) {
+    const float4 v = p[i];
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ uint32_t grp_max_u32(uint32_t v) {
+    for (int m = 1; m < kPartLanes; m <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, m, kPartLanes));
+    return v;
+}
+__device__ __forceinline__ int grp_max_i32(int v) {
+    for (int m = 1; m < kPartLanes; m <<= 1) v = max(v, __shfl_xor(v, m, kPartLanes));
+    return v;
+}
+__device__ __forceinline__ uint32_t grp_sum_u32(uint32_t v) {
+    for (int m = 1; m < kPartLanes; m <<= 1) v += (uint32_t)__shfl_xor((int)v, m, kPartLanes);
+    return v;
+}
+__device__ __forceinline__ uint64_t grp_or_u64(uint64_t v) {
+    for (int m = 1; m < kPartLanes; m <<= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, kPartLanes);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, kPartLanes);
+        v |= ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
 
-// pass001-003: texel -> cluster after the reference's 4 k-means iterations
-__device__ void kmeans_assign(const ImageArgs& A, const DevTables& T, int g, int pc, uint8_t* asg) {
-    const int tc = T.tc;
+// pass001-003: texel -> cluster after the reference's 4 k-means iterations;
+// lane `sub` of the group owns texels sub, sub + 16, ...
+__device__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, uint8_t* asg) {
     float centers[4][4] = {};
-    texel_px(A, T, g, 145897 % tc, centers[0]);
-    float dist[kMaxTexels];
-    for (int t = 0; t < tc; ++t) { float px[4]; texel_px(A, T, g, t, px); dist[t] = dist_sq(centers[0], px); }
+    ld4(kpx, 145897 % tc, centers[0]);
+    float dist[kPartMaxR];
+#pragma unroll
+    for (int r = 0; r < kPartMaxR; ++r) {
+        const int t = sub + r * kPartLanes;
+        dist[r] = 0.0f;
+        if (t < tc) { float px[4]; ld4(kpx, t, px); dist[r] = dist_sq(centers[0], px); }
+    }
     for (int p = 1; p < pc; ++p) {
         uint32_t mx = 0;
-        for (int t = 0; t < tc; ++t) mx = max(mx, __float_as_uint(dist[t]));
-        int idx = 0;
-        for (int t = 0; t < tc; ++t) if (__float_as_uint(dist[t]) == mx) idx = t;
-        texel_px(A, T, g, idx, centers[p]);
-        if (p < pc - 1)
-            for (int t = 0; t < tc; ++t) { float px[4]; texel_px(A, T, g, t, px); dist[t] = wmin(dist[t], dist_sq(centers[p], px)); }
+#pragma unroll
+        for (int r = 0; r < kPartMaxR; ++r)
+            if (sub + r * kPartLanes < tc) mx = max(mx, __float_as_uint(dist[r]));
+        mx = grp_max_u32(mx);
+        int idx = 0;  // the highest texel holding the maximum
+#pragma unroll
+        for (int r = 0; r < kPartMaxR; ++r) {
+            const int t = sub + r * kPartLanes;
+            if (t < tc && __float_as_uint(dist[r]) == mx) idx = t;
+        }
+        idx = grp_max_i32(idx);
+        ld4(kpx, idx, centers[p]);
+        if (p < pc - 1) {
+#pragma unroll
+            for (int r = 0; r < kPartMaxR; ++r) {
+                const int t = sub + r * kPartLanes;
+                if (t < tc) { float px[4]; ld4(kpx, t, px); dist[r] = wmin(dist[r], dist_sq(centers[p], px)); }
+            }
+        }
     }
     for (int it = 0; it < 4; ++it) {
-        int counts[4] = {0, 0, 0, 0};
-        for (int t = 0; t < tc; ++t) {
+        uint32_t counts[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < kPartMaxR; ++r) {
+            const int t = sub + r * kPartLanes;
+            if (t >= tc) continue;
             float px[4];
-            texel_px(A, T, g, t, px);
+            ld4(kpx, t, px);
             float bd = 1e30f;
             int bp = 0;
             for (int p = 0; p < pc; ++p) {
@@ -185,28 +241,44 @@ __device__ void kmeans_assign(const ImageArgs& A, const DevTables& T, int g, int
                 if (d < bd) { bd = d; bp = p; }
             }
             asg[t] = (uint8_t)bp;
-            counts[bp]++;
+            for (int p = 0; p < 4; ++p) counts[p] += bp == p ? 1u : 0u;
         }
         bool problem = false;
-        for (int p = 0; p < pc; ++p) problem |= counts[p] == 0;
-        if (problem) for (int p = 0; p < pc; ++p) asg[p] = (uint8_t)p;
+        for (int p = 0; p < pc; ++p) problem |= grp_sum_u32(counts[p]) == 0;
+        if (problem && sub < pc) asg[sub] = (uint8_t)sub;  // texels 0..pc-1 (pc <= 4 < 16)
         uint32_t sums[4][4] = {}, cnt[4] = {};
-        for (int t = 0; t < tc; ++t) {
+#pragma unroll
+        for (int r = 0; r < kPartMaxR; ++r) {
+            const int t = sub + r * kPartLanes;
+            if (t >= tc) continue;
             float px[4];
-            texel_px(A, T, g, t, px);
+            ld4(kpx, t, px);
             const int p = asg[t];
-            for (int c = 0; c < 4; ++c) sums[p][c] += wf2u(px[c]);
-            cnt[p]++;
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t on = p == q ? 1u : 0u;
+                for (int c = 0; c < 4; ++c) sums[q][c] += on * wf2u(px[c]);
+                cnt[q] += on;
+            }
         }
-        for (int p = 0; p < pc; ++p)
-            for (int c = 0; c < 4; ++c) centers[p][c] = cnt[p] > 0 ? (float)sums[p][c] / (float)cnt[p] : 0.0f;
+        for (int p = 0; p < pc; ++p) {
+            const uint32_t n = grp_sum_u32(cnt[p]);
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t s = grp_sum_u32(sums[p][c]);
+                centers[p][c] = n > 0 ? (float)s / (float)n : 0.0f;
+            }
+        }
     }
 }
 
-// pass004: minimum-over-permutations coverage mismatch for one partitioning
-__device__ uint32_t mismatch_count(const DevTables& T, const uint8_t* asg, int pc, int entry) {
+// pass004: minimum-over-permutations coverage mismatch for one partitioning;
+// lanes OR their share of the 64 k-means texels into the group's bitmaps
+__device__ uint32_t mismatch_count_grp(const DevTables& T, const uint8_t* asg, int pc, int entry, int sub) {
     uint64_t a[4] = {0, 0, 0, 0};
-    for (int l = 0; l < 64; ++l) a[asg[T.kmeans_texels[l]]] |= 1ull << l;
+    for (int l = sub; l < 64; l += kPartLanes) {
+        const int p = asg[T.kmeans_texels[l]];
+        for (int q = 0; q < 4; ++q) a[q] |= (p == q) ? (1ull << l) : 0ull;
+    }
+    for (int q = 0; q < 4; ++q) a[q] = grp_or_u64(a[q]);
     const uint64_t* bb = T.coverage + ((size_t)(pc - 2) * kPartitionings + entry) * 4;
     auto P = [&](int i, int j) { return __popcll(a[i] ^ bb[j]); };
     if (pc == 2) return (uint32_t)(min(P(0, 0) + P(1, 1), P(0, 1) + P(1, 0)) / 2);
@@ -231,16 +303,15 @@ __device__ uint32_t mismatch_count(const DevTables& T, const uint8_t* asg, int p
 
 // pass006: line-fit errors of one candidate partitioning, with the
 // reference's per-invocation avg[]/dir[] arrays (only lane p holds
-// partition p's average and direction).
-__device__ void eval_candidate(const ImageArgs& A, const DevTables& T, int g, int pc, int entry, float* eu,
-                               float* es) {
+// partition p's average and direction).  Its cross-texel sums are integer.
+__device__ void eval_candidate(const float4* bpx, const DevTables& T, int pc, int entry, float* eu, float* es) {
     const int tc = T.tc;
     const size_t row = (size_t)(pc - 2) * kPartitionings + entry;
     const uint8_t* pot = T.part_of_texel + row * tc;
     uint32_t sums[4][4] = {}, cnt[4] = {};
     for (int t = 0; t < tc; ++t) {
         float px[4];
-        texel_px(A, T, g, t, px);
+        ld4(bpx, t, px);
         const int p = pot[t];
         cnt[p]++;
         for (int c = 0; c < 4; ++c) sums[p][c] += wf2u(px[c]);
@@ -256,7 +327,7 @@ __device__ void eval_candidate(const ImageArgs& A, const DevTables& T, int g, in
     int32_t axis[4][4][4] = {};
     for (int t = 0; t < tc; ++t) {
         float px[4];
-        texel_px(A, T, g, t, px);
+        ld4(bpx, t, px);
         const int p = pot[t];
         const bool own = p == t && t < pc && has_avg[p];
         float datum[4];
@@ -285,7 +356,7 @@ __device__ void eval_candidate(const ImageArgs& A, const DevTables& T, int g, in
     int32_t lmax[4] = {-2147483647, -2147483647, -2147483647, -2147483647};
     for (int t = 0; t < tc; ++t) {
         float px[4];
-        texel_px(A, T, g, t, px);
+        ld4(bpx, t, px);
         const int p = pot[t];
         const bool own = p == t && t < pc;
         float ud[4], av[4];
@@ -331,29 +402,63 @@ __device__ void eval_candidate(const ImageArgs& A, const DevTables& T, int g, in
     *es = ts;
 }
 
+size_t partition_lds_bytes(const DevTables& T) {
+    const size_t stride = T.tc + 1;
+    return 2 * kPartBlocks * stride * sizeof(float4) + kPartBlocks * 128 * sizeof(float2) +
+           kPartBlocks * (((size_t)T.tc + 3) & ~size_t(3));
+}
+
 __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.nb) return;
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tc = T.tc, lane = threadIdx.x, grp = lane / kPartLanes, sub = lane % kPartLanes;
+    const int stride = tc + 1;  // float4 units; the pad spreads the 4 blocks over LDS banks
+    float4* s_px = reinterpret_cast<float4*>(smem);        // blocks being partitioned
+    float4* s_kpx = s_px + kPartBlocks * stride;           // blocks whose k-means clusters they read
+    float2* s_err = reinterpret_cast<float2*>(s_kpx + kPartBlocks * stride);
+    uint8_t* s_asg = reinterpret_cast<uint8_t*>(s_err + kPartBlocks * 128);
+    const int tcp = (tc + 3) & ~3;
+    const int k = blockIdx.x * kPartBlocks + grp;
     const int g = A.blk0 + k;
     const int j = g % kRefBatch, base = g - j;
     const int B = min(kRefBatch, A.num_blocks - base);
-    int pos[4] = {0, 1, 2, 3};
     // partition_ordering and the partitioning errors are allocated 920*128
     // entries but indexed with stride 512, so in-batch blocks >= 230 read
     // zeros and keep table positions 0..3.
-    if (j * 512 + 127 < kRefBatch * 128) {
-        // pass004 writes slot block*1024*i: the score this block reads is
-        // mismatch(j, partitioning 1), or mismatch(B-1, partitioning 0) for j == 0
-        uint8_t asg[kMaxTexels];
-        const int src = j == 0 ? base + B - 1 : g;
-        kmeans_assign(A, T, src, pc, asg);
-        const uint32_t v = mismatch_count(T, asg, pc, j == 0 ? 0 : 1);
+    const bool work = k < A.nb && j * 512 + 127 < kRefBatch * 128;
+    // pass004 writes slot block*1024*i: the score this block reads is
+    // mismatch(j, partitioning 1), or mismatch(B-1, partitioning 0) for j == 0
+    const int src = j == 0 ? base + B - 1 : g;
+    if (work) {
+        for (int t = sub; t < tc; t += kPartLanes) {
+            float px[4];
+            texel_px(A, T, g, t, px);
+            s_px[grp * stride + t] = make_float4(px[0], px[1], px[2], px[3]);
+            if (src != g) texel_px(A, T, src, t, px);
+            s_kpx[grp * stride + t] = make_float4(px[0], px[1], px[2], px[3]);
+        }
+    }
+    __syncthreads();
+    uint8_t* asg = s_asg + grp * tcp;
+    if (work) kmeans_assign_grp(s_kpx + grp * stride, tc, pc, sub, asg);
+    __syncthreads();
+    if (work) {
+        const uint32_t v = mismatch_count_grp(T, asg, pc, j == 0 ? 0 : 1, sub);
         const uint16_t* ord = T.ordering + ((pc - 2) * 2 + (v > 0 ? 1 : 0)) * 128;
+        for (int i = sub; i < 128; i += kPartLanes) {
+            float eu, es;
+            eval_candidate(s_px + grp * stride, T, pc, ord[i], &eu, &es);
+            s_err[grp * 128 + i] = make_float2(eu, es);
+        }
+    }
+    __syncthreads();
+    if (k >= A.nb || sub != 0) return;
+    int pos[4] = {0, 1, 2, 3};
+    if (work) {
         int bui[4] = {0, 0, 0, 0}, bsi[4] = {0, 0, 0, 0};
         float bue[4] = {1e30f, 1e30f, 1e30f, 1e30f}, bse[4] = {1e30f, 1e30f, 1e30f, 1e30f};
         for (int i = 0; i < 128; ++i) {
-            float eu, es;
-            eval_candidate(A, T, g, pc, ord[i], &eu, &es);
+            const float2 e = s_err[grp * 128 + i];
+            const float eu = e.x, es = e.y;
             if (eu < bue[3]) {
                 bue[3] = eu; bui[3] = i;
                 for (int q = 3; q > 0; --q)
@@ -607,16 +712,52 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
 }
 
 // ---- passes 02-05, 06, 07, 11 -----------------------------------------------
+// One lane per (partitioned block, decimation mode).  The ideal weights and
+// per-texel error scales of the workgroup's blocks are staged in LDS, and so
+// are each lane's decimated weights ([weight][lane], conflict-free).  Passes
+// 03-05 run step by step: pass05 only needs its running minimum at the step
+// counts of the quant levels the mode can use, so each level tracks its own
+// (error, step, cut) triple, updated by the four rules in the order pass05
+// applies them; that is the same sequence of comparisons on the same values.
+size_t dm_bm_lds_bytes(const DevTables& T, int wstride) {
+    const size_t maxpb = 63 / T.ndm + 2;
+    return maxpb * T.tc * 2 * sizeof(float) + 2 * 64 * (size_t)wstride * sizeof(float);
+}
+
 __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
-                                              const Ideal* ideal,
-                                              const float* wts_all, const ColorTab* ctab, Ccr* ccr, uint8_t* qw_out) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= npb * T.ndm) return;
-    const int pb = tid / T.ndm, m = tid - pb * T.ndm;
-    const Ideal& I = ideal[pb];
-    const float* wts = wts_all + (size_t)pb * T.tc;
+                                              const Ideal* ideal, const float* wts_all, const ColorTab* ctab, Ccr* ccr,
+                                              uint8_t* qw_out, int wstride) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int ndm = T.ndm, tc = T.tc, lane = threadIdx.x;
+    const long long t0 = (long long)blockIdx.x * 64;
+    const int pb0 = (int)(t0 / ndm);
+    const int maxpb = 63 / ndm + 2;
+    float* s_wts = reinterpret_cast<float*>(smem);
+    float* s_wes = s_wts + maxpb * tc;
+    float* s_dw = s_wes + maxpb * tc;
+    float* s_aux = s_dw + 64 * wstride;
+    {
+        const int pb_end = min((int)((t0 + 63) / ndm) + 1, npb);
+        const int n = (pb_end - pb0) * tc;
+        for (int e = lane; e < n; e += 64) {
+            const int l = e / tc, t = e - l * tc, pb = pb0 + l;
+            s_wts[e] = wts_all[(size_t)pb * tc + t];
+            int p = 0;
+            if (pc > 1) p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[pb]) * tc + t];
+            // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
+            s_wes[e] = ideal[pb].wes[p];
+        }
+    }
+    __syncthreads();
+    const long long tid = t0 + lane;
+    if (tid >= (long long)npb * ndm) return;
+    const int pb = (int)(tid / ndm), m = (int)(tid - (long long)pb * ndm);
+    const float* wts = s_wts + (pb - pb0) * tc;
+    const float* wes = s_wes + (pb - pb0) * tc;
+    float* dw = s_dw + lane;    // dw[i * 64]
+    float* aux = s_aux + lane;  // aux[i * 64]
     const DmInfo di = T.dm[m];
-    const int tc = di.texel_count, wc = di.weight_count;
+    const int wc = di.weight_count;
     const uint8_t* t2wc = T.t2w_count + (size_t)m * tc;
     const uint8_t* t2wi = T.t2w_idx + (size_t)m * tc * 4;
     const float* t2ww = T.t2w_w + (size_t)m * tc * 4;
@@ -624,135 +765,136 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     const uint16_t* w2to = T.w2t_off + m * 64;
     const uint8_t* w2ti = T.w2t_idx + di.w2t_base;
     const float* w2tw = T.w2t_w + di.w2t_base;
-    // weight error scale of a texel = that of its partition (pass01 stores
-    // lengthSqr[p] * error_weight for every texel of partition p)
-    const Pb b = make_pb(A, T, pb_part, pc, pb);
-    const int pb_in_batch = b.pb_in_batch;
-    auto wes_of = [&](int t) -> float { return I.wes[texel_part(b, t)]; };
-    float dw[kMaxWeights];
+    int pb_in_batch;
+    {
+        const int npos = pc == 1 ? 1 : 4;
+        const int kb = pb / npos, c = pb - kb * npos;
+        const int j = (A.blk0 + kb) % kRefBatch;
+        pb_in_batch = pc == 1 ? j : j * 4 + c;
+    }
     // pass02_decimated_weights.wgsl:132-270
     if (tc == wc) {
-        for (int i = 0; i < wc; ++i) dw[i] = wts[i];
+        for (int i = 0; i < wc; ++i) dw[i * 64] = wts[i];
     } else {
         for (int w = 0; w < wc; ++w) {
             const int n = w2tc[w];
-            if (n == 0) { dw[w] = 0.0f; continue; }
-            const int off = w2to[w];
-            float ws = 0.0f, tcon = 1e-10f;
-            for (int jx = 0; jx < n; ++jx) {
-                const int t = w2ti[off + jx];
-                const float cw = w2tw[off + jx] * wes_of(t);
-                ws += wts[t] * cw;
-                tcon += cw;
-            }
-            dw[w] = ws / tcon;
-        }
-        float infill[kMaxTexels];
-        for (int i = 0; i < tc; ++i) {
             float v = 0.0f;
-            for (int jx = 0; jx < t2wc[i]; ++jx) v += dw[t2wi[i * 4 + jx]] * t2ww[i * 4 + jx];
-            infill[i] = v;
+            if (n != 0) {
+                const int off = w2to[w];
+                float ws = 0.0f, tcon = 1e-10f;
+                for (int jx = 0; jx < n; ++jx) {
+                    const int t = w2ti[off + jx];
+                    const float cw = w2tw[off + jx] * wes[t];
+                    ws += wts[t] * cw;
+                    tcon += cw;
+                }
+                v = ws / tcon;
+            }
+            aux[w * 64] = v;  // initial estimate
         }
-        // one refinement step per weight; each step only reads the infill of
-        // the initial estimate, so the weights can be updated in place
+        // one refinement step per weight against the infill of the initial estimate
         for (int w = 0; w < wc; ++w) {
             float a = 1e-10f, bacc = 0.0f;
             const int n = w2tc[w], off = w2to[w];
             for (int jx = 0; jx < n; ++jx) {
                 const int t = w2ti[off + jx];
                 const float contrib = w2tw[off + jx];
-                const float scale = wes_of(t) * contrib;
+                const float scale = wes[t] * contrib;
+                float infill = 0.0f;
+                const int nt = t2wc[t];
+                for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
                 a += contrib * scale;
-                bacc += (infill[t] - wts[t]) * scale;
+                bacc += (infill - wts[t]) * scale;
             }
-            dw[w] += wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
+            dw[w * 64] = aux[w * 64] + wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
         }
     }
-    // pass03_compute_angular_offsets.wgsl:65-116
-    float ao[16];
-    {
-        uint16_t is[kMaxWeights];
-        for (int i = 0; i < wc; ++i) is[i] = (uint16_t)wf2u(wround(wclamp(dw[i], 0.0f, 1.0f) * (1024.0f - 1.0f)));
-        const float inv2pi = 1.0f / (2.0f * 3.14159265358979323846f);
-        for (int s = 0; s < 16; ++s) {
-            float ax = 0.0f, ay = 0.0f;
-            for (int jx = 0; jx < wc; ++jx) {
-                ax += T.cos_t[is[jx] * 16 + s];
-                ay += T.sin_t[is[jx] * 16 + s];
-            }
-            ao[s] = atan2f_fixed(ay, ax) * inv2pi;
+    // passes 03-05 (pass03_compute_angular_offsets.wgsl:65-116,
+    // pass04_lowest_and_highest_weight.wgsl:116-227,
+    // pass05_best_values_for_quant_levels.wgsl:84-175)
+    float minw = 3.4e38f, maxw = -3.4e38f;
+    for (int i = 0; i < wc; ++i) {
+        const float v = dw[i * 64];
+        if (v < minw) minw = v;
+        if (v > maxw) maxw = v;
+    }
+    const int nq = di.max_quant_level + 1;  // <= 8 (QUANT_12)
+    float be[8], bstep[8], brc[8], blow[8], bao[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { be[q] = 1e37f; bstep[q] = 0.0f; brc[q] = 0.0f; blow[q] = 0.0f; bao[q] = 0.0f; }
+    const float inv2pi = 1.0f / (2.0f * 3.14159265358979323846f);
+    const int mqs = di.max_quant_steps;
+    for (int sp = 0; sp < di.max_angular_steps; ++sp) {
+        float ax = 0.0f, ay = 0.0f;
+        for (int jx = 0; jx < wc; ++jx) {
+            const uint32_t is = wf2u(wround(wclamp(dw[jx * 64], 0.0f, 1.0f) * (1024.0f - 1.0f)));
+            ax += T.cos_t[is * 16 + sp];
+            ay += T.sin_t[is * 16 + sp];
+        }
+        const float off = atan2f_fixed(ay, ax) * inv2pi;
+        const float rcp = (float)(sp + 1);
+        const float minidx = wround(minw * rcp - off), maxidx = wround(maxw * rcp - off);
+        float e = 0.0f, cl = 0.0f, ch = 0.0f;
+        for (int jx = 0; jx < wc; ++jx) {
+            const float sval = dw[jx * 64] * rcp - off;
+            const float r = wround(sval);
+            const float diff = sval - r;
+            e += diff * diff;
+            cl += (r == minidx) ? 1.0f - (2.0f * diff) : 0.0f;
+            ch += (r == maxidx) ? 1.0f + (2.0f * diff) : 0.0f;
+        }
+        int span = wf2i(maxidx - minidx + 1.0f);
+        span = max(2, min(span, mqs + 3));
+        const float ssize = 1.0f / rcp, es = ssize * ssize;
+        const float he = e * es, hcl = cl * es, hch = ch * es;
+        const float ecl = he + hcl, ech = he + hch, eclh = (he + hcl) + hch;
+        const float fi = (float)sp;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q >= nq) continue;
+            if (sp == 0) { blow[q] = minidx; bao[q] = off; }  // step 0 stands in for an unset slot
+            const int x = kSteps[q];
+            bool upd = false;
+            if (span == x && he < be[q]) { be[q] = he; bstep[q] = fi; brc[q] = 0.0f; upd = true; }
+            if (span - 1 == x && ecl < be[q]) { be[q] = ecl; bstep[q] = fi; brc[q] = 1.0f; upd = true; }
+            if (span - 1 == x && ech < be[q]) { be[q] = ech; bstep[q] = fi; brc[q] = 0.0f; upd = true; }
+            if (span - 2 == x && eclh < be[q]) { be[q] = eclh; bstep[q] = fi; brc[q] = 1.0f; upd = true; }
+            if (upd) { blow[q] = minidx; bao[q] = off; }
         }
     }
-    // pass04_lowest_and_highest_weight.wgsl:116-227
-    float hl_low[16], hl_err[16], hl_cl[16], hl_ch[16];
-    int hl_span[16];
+    // the 13-float output stride against an 8-float allocation
+    // (shaders_init.cpp:1176): slots beyond 8*920*4*ndm floats read as 0
+    float low[8], high[8];
     {
-        float minw = 3.4e38f, maxw = -3.4e38f;
-        for (int i = 0; i < wc; ++i) {
-            if (dw[i] < minw) minw = dw[i];
-            if (dw[i] > maxw) maxw = dw[i];
-        }
-        for (int sp = 0; sp < di.max_angular_steps; ++sp) {
-            const float rcp = (float)(sp + 1), off = ao[sp];
-            const float minidx = wround(minw * rcp - off), maxidx = wround(maxw * rcp - off);
-            float e = 0.0f, cl = 0.0f, ch = 0.0f;
-            for (int jx = 0; jx < wc; ++jx) {
-                const float sval = dw[jx] * rcp - off;
-                const float r = wround(sval);
-                const float diff = sval - r;
-                e += diff * diff;
-                cl += (r == minidx) ? 1.0f - (2.0f * diff) : 0.0f;
-                ch += (r == maxidx) ? 1.0f + (2.0f * diff) : 0.0f;
-            }
-            int span = wf2i(maxidx - minidx + 1.0f);
-            span = max(2, min(span, (int)di.max_quant_steps + 3));
-            const float ssize = 1.0f / rcp, es = ssize * ssize;
-            hl_low[sp] = minidx;
-            hl_span[sp] = span;
-            hl_err[sp] = e * es;
-            hl_cl[sp] = cl * es;
-            hl_ch[sp] = ch * es;
-        }
-    }
-    // pass05_best_values_for_quant_levels.wgsl:84-175, then the 13-float
-    // output stride against an 8-float allocation (shaders_init.cpp:1176):
-    // slots beyond 8*920*4*ndm floats read as 0
-    float low[13], high[13];
-    {
-        float re[36], ri[36], rc[36];
-        for (int i = 0; i < di.max_quant_steps + 4 && i < 36; ++i) { re[i] = 1e37f; ri[i] = -1.0f; rc[i] = 0.0f; }
-        for (int i = 0; i < di.max_angular_steps; ++i) {
-            const float fi = (float)i;
-            const int sp = hl_span[i];
-            const float e = hl_err[i], ecl = e + hl_cl[i], ech = e + hl_ch[i], eclh = (e + hl_cl[i]) + hl_ch[i];
-            if (e < re[sp]) { re[sp] = e; ri[sp] = fi; rc[sp] = 0.0f; }
-            if (ecl < re[sp - 1]) { re[sp - 1] = ecl; ri[sp - 1] = fi; rc[sp - 1] = 1.0f; }
-            if (ech < re[sp - 1]) { re[sp - 1] = ech; ri[sp - 1] = fi; rc[sp - 1] = 0.0f; }
-            if (eclh < re[sp - 2]) { re[sp - 2] = eclh; ri[sp - 2] = fi; rc[sp - 2] = 1.0f; }
-        }
-        const long long alloc = 8ll * kRefBatch * 4 * T.ndm;
-        const long long t = (long long)pb_in_batch * T.ndm + m;
-        for (int q = 0; q < 13; ++q) { low[q] = 0.0f; high[q] = 0.0f; }
-        for (int q = 0; q <= di.max_quant_level; ++q) {
+        const long long alloc = 8ll * kRefBatch * 4 * ndm;
+        const long long t = (long long)pb_in_batch * ndm + m;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            low[q] = 0.0f;
+            high[q] = 0.0f;
+            if (q >= nq) continue;
             const int qs = kSteps[q];
-            const int bsi = max(wf2i(ri[qs]), 0);
-            const float lwi = hl_low[bsi] + rc[qs];
+            const int bsi = max(wf2i(bstep[q]), 0);
+            const float lwi = blow[q] + brc[q];
             const float hwi = (lwi + (float)qs) - 1.0f;
             const float stepsize = 1.0f / (1.0f + (float)bsi);
             if (13 * t + q < alloc) {
-                low[q] = (ao[bsi] + lwi) * stepsize;
-                high[q] = (ao[bsi] + hwi) * stepsize;
+                low[q] = (bao[q] + lwi) * stepsize;
+                high[q] = (bao[q] + hwi) * stepsize;
             }
         }
     }
     // passes 06, 07 and 11 for every trial block mode of this decimation mode
+    const Ideal& I = ideal[pb];
+    const float cutoff = I.cutoff;
     const ColorTab& ct = ctab[pb];
+    uint8_t* qwo_pb = qw_out + (size_t)pb * T.qw_stride;
     for (int li = T.dm_bm_off[m]; li < T.dm_bm_off[m + 1]; ++li) {
         const int k = T.dm_bm_list[li];
         const BmInfo bmi = T.bm[k];
         Ccr out;
         out.ql = -1; out.ql_mod = -1; out.formats = 0;
-        uint8_t* qwo = qw_out + ((size_t)pb * T.nbm + k) * 64;
+        uint8_t* qwo = qwo_pb + T.qw_off[k];
         const int q = bmi.quant;
         const int bc = kFreeBits[pc - 1] - bmi.weight_bits;
         float we;
@@ -760,40 +902,40 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             we = 1e38f;
         } else {
             // pass06_remap_low_and_high_values.wgsl:72-98 (quant <= 12 here)
-            float lo = low[q], hi = high[q];
+            float lo = 0.0f, hi = 0.0f;
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq)
+                if (qq == q) { lo = low[qq]; hi = high[qq]; }
             // pass07_weights_and_error_for_bm.wgsl:228-331
-            if (hi > 1.02f * I.cutoff) hi = 1.0f;
+            if (hi > 1.02f * cutoff) hi = 1.0f;
             if (hi <= lo) { lo = 0.0f; hi = 1.0f; }
             const float qlm1 = (float)(kLevelsD[q] - 1);
             float rscale = hi - lo;
             const float scale = 1.0f / rscale;
             const float slb = lo * scale;
             rscale = rscale / 64.0f;
-            float qwf[kMaxWeights];
-            uint8_t qwi[kMaxWeights];
             const uint32_t qmax = wf2u(qlm1);
+            // only whole groups of four weights reach the output (pass07:322-331)
+            const int keep = (wc / 4) * 4;
             for (int i = 0; i < wc; ++i) {
-                const float ix = wclamp(dw[i] * scale - slb, 0.0f, 1.0f);
+                const float ix = wclamp(dw[i * 64] * scale - slb, 0.0f, 1.0f);
                 const float ix1 = ix * qlm1;
                 const uint32_t wl = wf2u(ix1);
                 const uint32_t wh = min(wl + 1u, qmax);
                 const int il = c_qt.quant_to_unquant[q][wl], ih = c_qt.quant_to_unquant[q][wh];
                 const float fl = (float)il, fh = (float)ih;
                 const bool use_high = (fl + fh) < (128.0f * ix);
-                qwi[i] = (uint8_t)(use_high ? ih : il);
-                qwf[i] = (use_high ? fh : fl) * rscale + lo;
+                qwo[i] = i < keep ? (uint8_t)(use_high ? ih : il) : 0;
+                aux[i * 64] = (use_high ? fh : fl) * rscale + lo;
             }
             float err = 0.0f;
             for (int i = 0; i < tc; ++i) {
                 float v = 0.0f;
-                for (int jx = 0; jx < t2wc[i]; ++jx) v += qwf[t2wi[i * 4 + jx]] * t2ww[i * 4 + jx];
+                for (int jx = 0; jx < t2wc[i]; ++jx) v += aux[t2wi[i * 4 + jx] * 64] * t2ww[i * 4 + jx];
                 const float diff = v - wts[i];
-                err += diff * diff * wes_of(i);
+                err += diff * diff * wes[i];
             }
             we = err;
-            // only whole groups of four weights reach the output (pass07:322-331)
-            const int keep = (wc / 4) * 4;
-            for (int i = 0; i < 64; ++i) qwo[i] = i < keep ? qwi[i] : 0;
         }
         // pass11_best_color_combination_for_mode_{1,2,3,4}part.wgsl
         if (!(we < kErrDefault)) {
@@ -1308,7 +1450,7 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     const int ql = cr.ql, qlm = cr.ql_mod;
     const uint16_t in_formats = cr.formats;
     {
-        const uint8_t* qsrc = qw_all + ((size_t)pb * T.nbm + k) * 64;
+        const uint8_t* qsrc = qw_all + (size_t)pb * T.qw_stride + T.qw_off[k];
         for (int i = 0; i < wc; ++i) qw[i] = qsrc[i];
     }
     const float lsw = (kCW[0] + kCW[1]) + kCW[2];
@@ -1748,10 +1890,19 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
     for (int pc = 1; pc <= max_pc; ++pc) {
         const int npos = pc == 1 ? 1 : 4;
         const int npb = A.nb * npos;
-        if (pc > 1) ASTCG_LAUNCH(K_PARTITION, k_partition, blocks_for(A.nb), A, T, pc, B.pb_part);
+        if (pc > 1) {
+            if (prof) prof->begin(K_PARTITION, s);
+            hipLaunchKernelGGL(k_partition, dim3((A.nb + kPartBlocks - 1) / kPartBlocks), dim3(64),
+                               partition_lds_bytes(T), s, A, T, pc, B.pb_part);
+            if (prof) prof->end(K_PARTITION, s);
+        }
         ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab);
-        ASTCG_LAUNCH(K_DM_BM, k_dm_bm, blocks_for((long long)npb * T.ndm), A, T, pc, B.pb_part, npb, B.ideal, B.wts,
-                     B.ctab, B.ccr, B.qw);
+        {
+            if (prof) prof->begin(K_DM_BM, s);
+            hipLaunchKernelGGL(k_dm_bm, dim3(blocks_for((long long)npb * T.ndm)), dim3(64), dm_bm_lds_bytes(T, B.wstride),
+                               s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab, B.ccr, B.qw, B.wstride);
+            if (prof) prof->end(K_DM_BM, s);
+        }
         ASTCG_LAUNCH(K_TOP, k_top, blocks_for(npb), npb, T.nbm, B.ccr, B.top);
         {
             if (prof) prof->begin(K_REFINE, s);
