@@ -101,7 +101,7 @@ struct ChunkBuffers {
     Ideal* ideal;       // [4 * max_nb]
     float* wts;         // [4 * max_nb][tc]
     ColorTab* ctab;     // [4 * max_nb]
-    Ccr* ccr;           // [4 * max_nb][nbm]
+    Ccr* ccr;           // [nbm][npb of the launch]
     uint8_t* qw;        // [4 * max_nb][qw_stride]
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]

@@ -204,7 +204,9 @@ __device__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, ui
         dist[r] = 0.0f;
         if (t < tc) { float px[4]; ld4(kpx, t, px); dist[r] = dist_sq(centers[0], px); }
     }
-    for (int p = 1; p < pc; ++p) {
+#pragma unroll
+    for (int p = 1; p < 4; ++p) {
+        if (p >= pc) continue;
         uint32_t mx = 0;
 #pragma unroll
         for (int r = 0; r < kPartMaxR; ++r)
@@ -236,15 +238,20 @@ __device__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, ui
             ld4(kpx, t, px);
             float bd = 1e30f;
             int bp = 0;
-            for (int p = 0; p < pc; ++p) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                if (p >= pc) continue;
                 const float d = dist_sq(px, centers[p]);
                 if (d < bd) { bd = d; bp = p; }
             }
             asg[t] = (uint8_t)bp;
+#pragma unroll
             for (int p = 0; p < 4; ++p) counts[p] += bp == p ? 1u : 0u;
         }
         bool problem = false;
-        for (int p = 0; p < pc; ++p) problem |= grp_sum_u32(counts[p]) == 0;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            if (p < pc) problem |= grp_sum_u32(counts[p]) == 0;
         if (problem && sub < pc) asg[sub] = (uint8_t)sub;  // texels 0..pc-1 (pc <= 4 < 16)
         uint32_t sums[4][4] = {}, cnt[4] = {};
 #pragma unroll
@@ -254,13 +261,16 @@ __device__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, ui
             float px[4];
             ld4(kpx, t, px);
             const int p = asg[t];
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t on = p == q ? 1u : 0u;
                 for (int c = 0; c < 4; ++c) sums[q][c] += on * wf2u(px[c]);
                 cnt[q] += on;
             }
         }
-        for (int p = 0; p < pc; ++p) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (p >= pc) continue;
             const uint32_t n = grp_sum_u32(cnt[p]);
             for (int c = 0; c < 4; ++c) {
                 const uint32_t s = grp_sum_u32(sums[p][c]);
@@ -304,7 +314,10 @@ __device__ uint32_t mismatch_count_grp(const DevTables& T, const uint8_t* asg, i
 // pass006: line-fit errors of one candidate partitioning, with the
 // reference's per-invocation avg[]/dir[] arrays (only lane p holds
 // partition p's average and direction).  Its cross-texel sums are integer.
-__device__ void eval_candidate(const float4* bpx, const DevTables& T, int pc, int entry, float* eu, float* es) {
+__device__ __forceinline__ void eval_candidate(const float4* bpx, const DevTables& T, int pc, int entry, float* eu, float* es) {
+    // Per-partition state is indexed through compile-time partition numbers
+    // (selects), so it stays in registers.  Only texels 0..pc-1 can be
+    // "own" texels (own: partition == texel index), so those are peeled.
     const int tc = T.tc;
     const size_t row = (size_t)(pc - 2) * kPartitionings + entry;
     const uint8_t* pot = T.part_of_texel + row * tc;
@@ -313,56 +326,71 @@ __device__ void eval_candidate(const float4* bpx, const DevTables& T, int pc, in
         float px[4];
         ld4(bpx, t, px);
         const int p = pot[t];
-        cnt[p]++;
-        for (int c = 0; c < 4; ++c) sums[p][c] += wf2u(px[c]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < pc && p == q) {
+                cnt[q]++;
+                for (int c = 0; c < 4; ++c) sums[q][c] += wf2u(px[c]);
+            }
     }
     float avg[4][4] = {};
     bool has_avg[4] = {false, false, false, false};
-    for (int p = 0; p < pc; ++p)
-        if ((float)cnt[p] > 0.0f) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+        if (p < pc && (float)cnt[p] > 0.0f) {
             const float n = (float)cnt[p];
             for (int c = 0; c < 4; ++c) avg[p][c] = (float)sums[p][c] / n;
             has_avg[p] = true;
         }
     int32_t axis[4][4][4] = {};
-    for (int t = 0; t < tc; ++t) {
+    auto axis_texel = [&](int t, bool own_ok, const float* own_avg) __attribute__((always_inline)) {
         float px[4];
         ld4(bpx, t, px);
         const int p = pot[t];
-        const bool own = p == t && t < pc && has_avg[p];
+        const bool own = own_ok && p == t;
         float datum[4];
-        for (int c = 0; c < 4; ++c) datum[c] = px[c] - (own ? avg[p][c] : 0.0f);
+        for (int c = 0; c < 4; ++c) datum[c] = px[c] - (own ? own_avg[c] : 0.0f);
         int32_t di[4];
         for (int c = 0; c < 4; ++c) di[c] = wf2i(datum[c]);
-        for (int ax = 0; ax < 4; ++ax)
-            if (datum[ax] > 0.0f)
-                for (int c = 0; c < 4; ++c) axis[p][ax][c] = (int32_t)((uint32_t)axis[p][ax][c] + (uint32_t)di[c]);
-    }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < pc && p == q)
+                for (int ax = 0; ax < 4; ++ax)
+                    if (datum[ax] > 0.0f)
+                        for (int c = 0; c < 4; ++c) axis[q][ax][c] = (int32_t)((uint32_t)axis[q][ax][c] + (uint32_t)di[c]);
+    };
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (t < tc) axis_texel(t, t < pc && has_avg[t], avg[t]);
+    for (int t = 4; t < tc; ++t) axis_texel(t, false, avg[0]);
     float dir[4][4] = {};
-    for (int p = 0; p < pc; ++p) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= pc) continue;
         float s[4][4];
         for (int ax = 0; ax < 4; ++ax)
             for (int c = 0; c < 4; ++c) s[ax][c] = (float)axis[p][ax][c];
-        int best = 0;
+        float best[4];
+        for (int c = 0; c < 4; ++c) best[c] = s[0][c];
         float mp = dot4(s[0], s[0]);
         for (int ax = 1; ax < 4; ++ax) {
             const float pr = dot4(s[ax], s[ax]);
-            if (pr > mp) { mp = pr; best = ax; }
+            if (pr > mp) { mp = pr; for (int c = 0; c < 4; ++c) best[c] = s[ax][c]; }
         }
-        normalize4(s[best], dir[p]);
+        normalize4(best, dir[p]);
     }
     uint32_t usum = 0, ssum = 0;
     int32_t lmin[4] = {2147483647, 2147483647, 2147483647, 2147483647};
     int32_t lmax[4] = {-2147483647, -2147483647, -2147483647, -2147483647};
-    for (int t = 0; t < tc; ++t) {
+    auto line_texel = [&](int t, bool own_ok, const float* own_dir, const float* own_avg) __attribute__((always_inline)) {
         float px[4];
         ld4(bpx, t, px);
         const int p = pot[t];
-        const bool own = p == t && t < pc;
+        const bool own = own_ok && p == t;
         float ud[4], av[4];
         for (int c = 0; c < 4; ++c) {
-            ud[c] = own ? dir[p][c] : 0.0f;
-            av[c] = (own && has_avg[p]) ? avg[p][c] : 0.0f;
+            ud[c] = own ? own_dir[c] : 0.0f;
+            av[c] = own ? own_avg[c] : 0.0f;
         }
         const float uparam = dot4(px, ud);
         float dm[4];
@@ -379,9 +407,21 @@ __device__ void eval_candidate(const float4* bpx, const DevTables& T, int pc, in
         usum += wf2u(udist);
         ssum += wf2u(sdist);
         const int32_t pi = wf2i(uparam * 4096.0f);
-        lmin[p] = min(lmin[p], pi);
-        lmax[p] = max(lmax[p], pi);
-    }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < pc && p == q) { lmin[q] = min(lmin[q], pi); lmax[q] = max(lmax[q], pi); }
+    };
+    // own texel: avg is used only when the partition has texels (it always
+    // does when p == t), dir always
+    float zero4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (t < tc) {
+            float oa[4];
+            for (int c = 0; c < 4; ++c) oa[c] = has_avg[t] ? avg[t][c] : 0.0f;
+            line_texel(t, t < pc, dir[t], oa);
+        }
+    for (int t = 4; t < tc; ++t) line_texel(t, false, zero4, zero4);
     float tu = (float)usum, ts = (float)ssum;
     float wie = 0.055f;
     if (tc <= 20) wie = 0.03f;
@@ -389,7 +429,9 @@ __device__ void eval_candidate(const float4* bpx, const DevTables& T, int pc, in
     else if (tc <= 41) wie = 0.05f;
     wie = wie * wie;
     const uint8_t* cnts = T.part_counts + row * 4;
-    for (int p = 0; p < pc; ++p) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= pc) continue;
         const float minp = (float)lmin[p] / 4096.0f, maxp = (float)lmax[p] / 4096.0f;
         const float len = wmax(maxp - minp, 1e-7f);
         const float ew = (float)cnts[p] * wie;
@@ -471,11 +513,18 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int 
             }
         }
         int emitted = 0;
-        for (int i = 0; i < 8 && emitted < 4; ++i) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
             const int idx = (i & 1) ? bsi[i >> 1] : bui[i >> 1];
-            bool dup = false;
-            for (int q = 0; q < emitted; ++q) dup |= pos[q] == idx;
-            if (!dup) pos[emitted++] = idx;
+            bool dup = emitted >= 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dup |= q < emitted && pos[q] == idx;
+            if (!dup) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q == emitted) pos[q] = idx;
+                ++emitted;
+            }
         }
         // pass007 uses the candidate position itself as the packed table index
     }
@@ -483,6 +532,12 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int 
 }
 
 // ---- pass01 + pass08 + pass09 + pass10 --------------------------------------
+// One lane per partitioned block.  Per-partition accumulators are updated
+// through a compile-time partition index (texel partition compared against
+// each of the four), so they stay in registers; pass09 and pass10 run
+// quant level by quant level.
+__device__ __constant__ int kColorLevels[17] = {5, 7, 9, 11, 15, 19, 23, 31, 39, 47, 63, 79, 95, 127, 159, 191, 255};
+
 __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
                                                     Ideal* ideal_out, float* wts_out, ColorTab* ctab) {
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
@@ -498,13 +553,18 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
         float px[4];
         texel_px(A, T, b.g, i, px);
         const int p = texel_part(b, i);
-        for (int c = 0; c < 4; ++c) sum[p][c] += px[c];
-        count[p] += 1.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p == q) {
+                for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
+                count[q] += 1.0f;
+            }
         gray = gray && (px[0] == px[1] && px[0] == px[2]);
         amax = fmaxf(amax, px[3]);
         amin = fminf(amin, px[3]);
     }
     float avg[4][4];
+#pragma unroll
     for (int p = 0; p < 4; ++p)
         for (int c = 0; c < 4; ++c) avg[p][c] = count[p] > 0.0f ? sum[p][c] / count[p] : 0.0f;
     float dv[4][4] = {};
@@ -512,73 +572,100 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
         float px[4];
         texel_px(A, T, b.g, i, px);
         const int p = texel_part(b, i);
-        for (int c = 0; c < 4; ++c) { const float d = wmax(px[c] - avg[p][c], 0.0f); dv[p][c] += d * d; }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p == q)
+                for (int c = 0; c < 4; ++c) { const float d = wmax(px[c] - avg[q][c], 0.0f); dv[q][c] += d * d; }
     }
     float dir[4][4], minp[4], maxp[4];
+#pragma unroll
     for (int p = 0; p < 4; ++p) { normalize4(dv[p], dir[p]); minp[p] = 1e10f; maxp[p] = -1e10f; }
     for (int i = 0; i < tc; ++i) {
-        float px[4], df[4];
+        float px[4];
         texel_px(A, T, b.g, i, px);
         const int p = texel_part(b, i);
-        for (int c = 0; c < 4; ++c) df[c] = px[c] - avg[p][c];
-        const float proj = dot4(df, dir[p]);
-        minp[p] = wmin(minp[p], proj);
-        maxp[p] = wmax(maxp[p], proj);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p == q) {
+                float df[4];
+                for (int c = 0; c < 4; ++c) df[c] = px[c] - avg[q][c];
+                const float proj = dot4(df, dir[q]);
+                minp[q] = wmin(minp[q], proj);
+                maxp[q] = wmax(maxp[q], proj);
+            }
     }
     Ideal I;
-    float lensq[4];
+#pragma unroll
     for (int p = 0; p < 4; ++p) {
         if (minp[p] >= maxp[p]) { minp[p] = 0.0f; maxp[p] = 1e-7f; }
         const float len = maxp[p] - minp[p];
-        lensq[p] = len * len;
+        const float lensq = len * len;
         const bool zero = dir[p][0] == 0.0f && dir[p][1] == 0.0f && dir[p][2] == 0.0f && dir[p][3] == 0.0f;
         for (int c = 0; c < 4; ++c) {
             I.avg[p][c] = avg[p][c];
             I.ep0[p][c] = zero ? avg[p][c] : avg[p][c] + dir[p][c] * minp[p];
             I.ep1[p][c] = zero ? avg[p][c] : avg[p][c] + dir[p][c] * maxp[p];
         }
-        I.wes[p] = lensq[p] * error_weight;
+        I.wes[p] = lensq * error_weight;
     }
     float* wts = wts_out + (size_t)pb * tc;
     for (int i = 0; i < tc; ++i) {
-        float px[4], df[4];
+        float px[4];
         texel_px(A, T, b.g, i, px);
         const int p = texel_part(b, i);
-        for (int c = 0; c < 4; ++c) df[c] = px[c] - avg[p][c];
-        const float proj = dot4(df, dir[p]);
-        const float span = wmax(maxp[p] - minp[p], 1e-6f);
-        wts[i] = wclamp((proj - minp[p]) / span, 0.0f, 1.0f);
+        float w = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p == q) {
+                float df[4];
+                for (int c = 0; c < 4; ++c) df[c] = px[c] - avg[q][c];
+                const float proj = dot4(df, dir[q]);
+                const float span = wmax(maxp[q] - minp[q], 1e-6f);
+                w = wclamp((proj - minp[q]) / span, 0.0f, 1.0f);
+            }
+        wts[i] = w;
     }
     float min_ep[4] = {10.0f, 10.0f, 10.0f, 10.0f};
+#pragma unroll
     for (int p = 0; p < 4; ++p)
         for (int c = 0; c < 4; ++c) {
             const float ep = (1.0f - I.ep0[p][c]) / ((I.ep1[p][c] - I.ep0[p][c]) + 1e-9f);
             if (ep > 0.5f && ep < min_ep[c]) min_ep[c] = ep;
         }
     I.cutoff = wmin(min_ep[0], wmin(min_ep[1], wmin(min_ep[2], min_ep[3])));
+    I.pad[0] = I.pad[1] = I.pad[2] = 0.0f;
     ideal_out[pb] = I;
 
     // pass08_compute_encoding_choice_errors.wgsl:115-326
     float sxp[4][4] = {}, syp[4][4] = {}, szp[4][4] = {};
     for (int i = 0; i < tc; ++i) {
-        float px[4], t[4];
+        float px[4];
         texel_px(A, T, b.g, i, px);
         const int p = texel_part(b, i);
-        for (int c = 0; c < 4; ++c) t[c] = px[c] - avg[p][c];
-        t[3] = 0.0f;
-        if (t[0] > 0.0f) for (int c = 0; c < 4; ++c) sxp[p][c] += t[c];
-        if (t[1] > 0.0f) for (int c = 0; c < 4; ++c) syp[p][c] += t[c];
-        if (t[2] > 0.0f) for (int c = 0; c < 4; ++c) szp[p][c] += t[c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p == q) {
+                float t[4];
+                for (int c = 0; c < 4; ++c) t[c] = px[c] - avg[q][c];
+                t[3] = 0.0f;
+                if (t[0] > 0.0f) for (int c = 0; c < 4; ++c) sxp[q][c] += t[c];
+                if (t[1] > 0.0f) for (int c = 0; c < 4; ++c) syp[q][c] += t[c];
+                if (t[2] > 0.0f) for (int c = 0; c < 4; ++c) szp[q][c] += t[c];
+            }
     }
     float ub[4][4], ua[4][4], sb[4][4], la[4][4], lb[4];
     const float one[4] = {1.0f, 1.0f, 1.0f, 0.0f};
     normalize4(one, lb);
-    for (int p = 0; p < pc; ++p) {
-        const float* best = sxp[p];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        for (int c = 0; c < 4; ++c) { ub[p][c] = 0.0f; ua[p][c] = 0.0f; sb[p][c] = 0.0f; la[p][c] = 0.0f; }
+        if (p >= pc) continue;
+        float best[4];
         float bs = dot4(sxp[p], sxp[p]);
+        for (int c = 0; c < 4; ++c) best[c] = sxp[p][c];
         const float py = dot4(syp[p], syp[p]), pz = dot4(szp[p], szp[p]);
-        if (py > bs) { best = syp[p]; bs = py; }
-        if (pz > bs) { best = szp[p]; bs = pz; }
+        if (py > bs) { for (int c = 0; c < 4; ++c) best[c] = syp[p][c]; bs = py; }
+        if (pz > bs) { for (int c = 0; c < 4; ++c) best[c] = szp[p][c]; bs = pz; }
         const float a3[4] = {avg[p][0], avg[p][1], avg[p][2], 0.0f};
         normalize4(best, ub[p]);
         const float s = dot4(a3, ub[p]);
@@ -592,32 +679,36 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
         float px[4];
         texel_px(A, T, b.g, i, px);
         const int p = texel_part(b, i);
-        const float ad = px[3] - 65536.0f;
-        acc[p][0] += ad * ad * kCW[3];
-        float d[3];
-        float s = dot3(px, ub[p]);
-        for (int c = 0; c < 3; ++c) { const float t = (ua[p][c] + s * ub[p][c]) - px[c]; d[c] = t * t; }
-        acc[p][1] += dot3(d, kCW);
-        s = dot3(px, sb[p]);
-        for (int c = 0; c < 3; ++c) { const float t = s * sb[p][c] - px[c]; d[c] = t * t; }
-        acc[p][2] += dot3(d, kCW);
-        s = dot3(px, lb);
-        for (int c = 0; c < 3; ++c) { const float t = (la[p][c] + s * lb[c]) - px[c]; d[c] = t * t; }
-        acc[p][3] += dot3(d, kCW);
-        for (int c = 0; c < 3; ++c) { const float t = s * lb[c] - px[c]; d[c] = t * t; }
-        acc[p][4] += dot3(d, kCW);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (p == q) {
+                const float ad = px[3] - 65536.0f;
+                acc[q][0] += ad * ad * kCW[3];
+                float d[3];
+                float s = dot3(px, ub[q]);
+                for (int c = 0; c < 3; ++c) { const float t = (ua[q][c] + s * ub[q][c]) - px[c]; d[c] = t * t; }
+                acc[q][1] += dot3(d, kCW);
+                s = dot3(px, sb[q]);
+                for (int c = 0; c < 3; ++c) { const float t = s * sb[q][c] - px[c]; d[c] = t * t; }
+                acc[q][2] += dot3(d, kCW);
+                s = dot3(px, lb);
+                for (int c = 0; c < 3; ++c) { const float t = (la[q][c] + s * lb[c]) - px[c]; d[c] = t * t; }
+                acc[q][3] += dot3(d, kCW);
+                for (int c = 0; c < 3; ++c) { const float t = s * lb[c] - px[c]; d[c] = t * t; }
+                acc[q][4] += dot3(d, kCW);
+            }
     }
     const bool constant_alpha = amax == amin;
     const int can_bc = !(gray && constant_alpha);
 
-    // pass09_compute_color_error.wgsl:140-268
-    float err[4][21][4];
-    int fmt[4][21][4];
+    // pass09_compute_color_error.wgsl:140-268: quant-independent terms
     const float lim = 0.12f * 65536.0f;
-    for (int p = 0; p < pc; ++p) {
-        const float rse = (acc[p][2] - acc[p][1]) * 0.7f;
-        const float le = (acc[p][4] - acc[p][1]) * 3.0f;
-        const float ade = acc[p][0] * 3.0f;
+    float rse[4], le[4], ade[4], rgb_range[4], alpha_range[4], base_rgb[4], base_a[4], oe_rgba0[4], oe_rgb0[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        rse[p] = (acc[p][2] - acc[p][1]) * 0.7f;
+        le[p] = (acc[p][4] - acc[p][1]) * 3.0f;
+        ade[p] = acc[p][0] * 3.0f;
         bool can_oe = true;
         for (int c = 0; c < 3; ++c) can_oe = can_oe && (fabsf(I.ep1[p][c] - I.ep0[p][c]) < lim);
         const float size = (float)b.ppc[p];
@@ -627,82 +718,88 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
             const float l0 = wmin(I.ep0[p][c], 0.0f), l1 = wmin(I.ep1[p][c], 0.0f);
             sre[c] = (((l0 * l0) + (l1 * l1)) + (h0 * h0)) + (h1 * h1);
         }
-        const float rgb_range = dot3(sre, kCW) * 0.5f * size;
-        const float alpha_range = sre[3] * kCW[3] * 0.5f * size;
-        const float base_rgb = ((kCW[0] + kCW[1]) + kCW[2]) * size;
-        const float base_a = kCW[3] * size;
-        const float bc_rgba = can_bc ? 0.625f : 1.0f, bc_rgb = can_bc ? 0.5f : 1.0f;
-        const float oe_rgba0 = can_oe ? 0.5f : 1.0f, oe_rgb0 = can_oe ? 0.25f : 1.0f;
-        for (int q = 0; q < 4; ++q) {
-            for (int k = 0; k < 4; ++k) err[p][q][k] = kErrDefault;
-            fmt[p][q][3] = FMT_RGBA; fmt[p][q][2] = FMT_RGB; fmt[p][q][1] = FMT_RGB_SCALE; fmt[p][q][0] = FMT_LUMINANCE;
-        }
-        const int K[17] = {5, 7, 9, 11, 15, 19, 23, 31, 39, 47, 63, 79, 95, 127, 159, 191, 255};
-        for (int i = 0; i < 17; ++i) {
-            const int q = i + 4;
-            const float oe_rgba = q >= 19 ? 1.0f : oe_rgba0, oe_rgb = q >= 19 ? 1.0f : oe_rgb0;
-            const float bqe = (65536.0f * 65536.0f / 18.0f) / (float)(K[i] * K[i]);
-            const float qe_rgb = base_rgb * bqe, qe_rgba = (base_rgb + base_a) * bqe;
-            err[p][q][3] = ((qe_rgba * bc_rgba * oe_rgba) + rgb_range) + alpha_range;
-            fmt[p][q][3] = FMT_RGBA;
-            const float full = ((qe_rgb * bc_rgb * oe_rgb) + rgb_range) + ade;
-            const float rgbs_a = ((qe_rgba + rse) + rgb_range) + alpha_range;
-            if (rgbs_a < full) { err[p][q][2] = rgbs_a; fmt[p][q][2] = FMT_RGB_SCALE_ALPHA; }
-            else { err[p][q][2] = full; fmt[p][q][2] = FMT_RGB; }
-            const float rgbs = ((qe_rgb + rgb_range) + ade) + rse;
-            const float lumal = ((qe_rgba + rgb_range) + alpha_range) + le;
-            if (rgbs < lumal) { err[p][q][1] = rgbs; fmt[p][q][1] = FMT_RGB_SCALE; }
-            else { err[p][q][1] = lumal; fmt[p][q][1] = FMT_LUMINANCE_ALPHA; }
-            err[p][q][0] = ((qe_rgb + rgb_range) + ade) + le;
-            fmt[p][q][0] = FMT_LUMINANCE;
-        }
+        rgb_range[p] = dot3(sre, kCW) * 0.5f * size;
+        alpha_range[p] = sre[3] * kCW[3] * 0.5f * size;
+        base_rgb[p] = ((kCW[0] + kCW[1]) + kCW[2]) * size;
+        base_a[p] = kCW[3] * size;
+        oe_rgba0[p] = can_oe ? 0.5f : 1.0f;
+        oe_rgb0[p] = can_oe ? 0.25f : 1.0f;
     }
+    const float bc_rgba = can_bc ? 0.625f : 1.0f, bc_rgb = can_bc ? 0.5f : 1.0f;
     ColorTab* ct = ctab + pb;
-    if (pc == 1) {
-        for (int q = 0; q < 21; ++q)
-            for (int k = 0; k < 4; ++k) {
-                ct->err[q][k] = err[0][q][k];
-                ct->fmt[q][k] = (uint16_t)fmt[0][q][k];
-            }
-        return;
-    }
-    // pass10_color_combinations_for_quant_{2,3,4}part.wgsl
-    for (int q = 0; q < 21; ++q)
-        for (int k = 0; k < 13; ++k) { ct->err[q][k] = kErrDefault; ct->fmt[q][k] = 0; }
-    for (int q = 4; q < 21; ++q) {
+    const int ncols = pc == 1 ? 4 : 13;
+    for (int q = 0; q < 4; ++q)
+        for (int k = 0; k < ncols; ++k) {
+            ct->err[q][k] = kErrDefault;
+            // pass09 leaves the unused quant rows with its default formats
+            ct->fmt[q][k] = pc == 1 ? (uint16_t)(k == 3 ? FMT_RGBA : k == 2 ? FMT_RGB : k == 1 ? FMT_RGB_SCALE : FMT_LUMINANCE) : 0;
+        }
+    for (int i = 0; i < 17; ++i) {
+        const int q = i + 4;
+        const int K = kColorLevels[i];
+        const float bqe = (65536.0f * 65536.0f / 18.0f) / (float)(K * K);
+        float err[4][4];
+        int fmt[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const float oe_rgba = q >= 19 ? 1.0f : oe_rgba0[p], oe_rgb = q >= 19 ? 1.0f : oe_rgb0[p];
+            const float qe_rgb = base_rgb[p] * bqe, qe_rgba = (base_rgb[p] + base_a[p]) * bqe;
+            err[p][3] = ((qe_rgba * bc_rgba * oe_rgba) + rgb_range[p]) + alpha_range[p];
+            fmt[p][3] = FMT_RGBA;
+            const float full = ((qe_rgb * bc_rgb * oe_rgb) + rgb_range[p]) + ade[p];
+            const float rgbs_a = ((qe_rgba + rse[p]) + rgb_range[p]) + alpha_range[p];
+            if (rgbs_a < full) { err[p][2] = rgbs_a; fmt[p][2] = FMT_RGB_SCALE_ALPHA; }
+            else { err[p][2] = full; fmt[p][2] = FMT_RGB; }
+            const float rgbs = ((qe_rgb + rgb_range[p]) + ade[p]) + rse[p];
+            const float lumal = ((qe_rgba + rgb_range[p]) + alpha_range[p]) + le[p];
+            if (rgbs < lumal) { err[p][1] = rgbs; fmt[p][1] = FMT_RGB_SCALE; }
+            else { err[p][1] = lumal; fmt[p][1] = FMT_LUMINANCE_ALPHA; }
+            err[p][0] = ((qe_rgb + rgb_range[p]) + ade[p]) + le[p];
+            fmt[p][0] = FMT_LUMINANCE;
+        }
+        if (pc == 1) {
+            for (int k = 0; k < 4; ++k) { ct->err[q][k] = err[0][k]; ct->fmt[q][k] = (uint16_t)fmt[0][k]; }
+            continue;
+        }
+        // pass10_color_combinations_for_quant_{2,3,4}part.wgsl
         float ce[13];
         uint16_t cf[13];
+#pragma unroll
         for (int k = 0; k < 13; ++k) { ce[k] = kErrDefault; cf[k] = 0; }
-        for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) {
-                const int lo2 = min(i, j), hi2 = max(i, j);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb) {
+                const int lo2 = min(a, bb), hi2 = max(a, bb);
                 if (hi2 - lo2 > 1) continue;
                 if (pc == 2) {
-                    const float te = wmin(err[0][q][i] + err[1][q][j], 1e10f);
-                    if (te < ce[i + j]) {
-                        const int f[4] = {fmt[0][q][i], fmt[1][q][j], 0, 0};
-                        ce[i + j] = te; cf[i + j] = pack_fmt4(f);
+                    const float te = wmin(err[0][a] + err[1][bb], 1e10f);
+                    if (te < ce[a + bb]) {
+                        const int f[4] = {fmt[0][a], fmt[1][bb], 0, 0};
+                        ce[a + bb] = te; cf[a + bb] = pack_fmt4(f);
                     }
                     continue;
                 }
-                for (int k = 0; k < 4; ++k) {
-                    const int lo3 = min(k, lo2), hi3 = max(k, hi2);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int lo3 = min(c, lo2), hi3 = max(c, hi2);
                     if (hi3 - lo3 > 1) continue;
                     if (pc == 3) {
-                        const float te = wmin((err[0][q][i] + err[1][q][j]) + err[2][q][k], 1e10f);
-                        if (te < ce[i + j + k]) {
-                            const int f[4] = {fmt[0][q][i], fmt[1][q][j], fmt[2][q][k], 0};
-                            ce[i + j + k] = te; cf[i + j + k] = pack_fmt4(f);
+                        const float te = wmin((err[0][a] + err[1][bb]) + err[2][c], 1e10f);
+                        if (te < ce[a + bb + c]) {
+                            const int f[4] = {fmt[0][a], fmt[1][bb], fmt[2][c], 0};
+                            ce[a + bb + c] = te; cf[a + bb + c] = pack_fmt4(f);
                         }
                         continue;
                     }
-                    for (int l = 0; l < 4; ++l) {
-                        const int lo4 = min(l, lo3), hi4 = max(l, hi3);
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        const int lo4 = min(d, lo3), hi4 = max(d, hi3);
                         if (hi4 - lo4 > 1) continue;
-                        const float te = wmin(((err[0][q][i] + err[1][q][j]) + err[2][q][k]) + err[3][q][l], 1e10f);
-                        if (te < ce[i + j + k + l]) {
-                            const int f[4] = {fmt[0][q][i], fmt[1][q][j], fmt[2][q][k], fmt[3][q][l]};
-                            ce[i + j + k + l] = te; cf[i + j + k + l] = pack_fmt4(f);
+                        const float te = wmin(((err[0][a] + err[1][bb]) + err[2][c]) + err[3][d], 1e10f);
+                        if (te < ce[a + bb + c + d]) {
+                            const int f[4] = {fmt[0][a], fmt[1][bb], fmt[2][c], fmt[3][d]};
+                            ce[a + bb + c + d] = te; cf[a + bb + c + d] = pack_fmt4(f);
                         }
                     }
                 }
@@ -712,16 +809,28 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
 }
 
 // ---- passes 02-05, 06, 07, 11 -----------------------------------------------
-// One lane per (partitioned block, decimation mode).  The ideal weights and
-// per-texel error scales of the workgroup's blocks are staged in LDS, and so
-// are each lane's decimated weights ([weight][lane], conflict-free).  Passes
-// 03-05 run step by step: pass05 only needs its running minimum at the step
-// counts of the quant levels the mode can use, so each level tracks its own
-// (error, step, cut) triple, updated by the four rules in the order pass05
-// applies them; that is the same sequence of comparisons on the same values.
+// A 64-lane workgroup runs one decimation mode for 64 partitioned blocks, one
+// block per lane, so loop bounds and table reads are uniform across the
+// wave.  The blocks' ideal weights and per-texel error scales are staged in
+// LDS, and so are each lane's decimated weights; all three are laid out
+// [index][lane] (conflict-free).  Workgroups are dispatched round-robin over
+// the 8 XCDs, so the grid is ordered to send every decimation mode of one
+// block group to the same XCD, whose L2 then holds that group's data.
+// Passes 03-05 run step by step: pass05 only needs its running minimum at
+// the step counts of the quant levels the mode can use, so each level tracks
+// its own (error, step, cut) triple, updated by the four rules in the order
+// pass05 applies them; that is the same sequence of comparisons on the same
+// values.
+constexpr int kDmPbs = 64;
+constexpr int kXcds = 8;
+
 size_t dm_bm_lds_bytes(const DevTables& T, int wstride) {
-    const size_t maxpb = 63 / T.ndm + 2;
-    return maxpb * T.tc * 2 * sizeof(float) + 2 * 64 * (size_t)wstride * sizeof(float);
+    return (size_t)kDmPbs * T.tc * 2 * sizeof(float) + 2 * 64 * (size_t)wstride * sizeof(float);
+}
+
+unsigned dm_bm_grid(const DevTables& T, int npb) {
+    const int groups = (npb + kDmPbs - 1) / kDmPbs;
+    return (unsigned)(((groups + kXcds - 1) / kXcds) * kXcds * T.ndm);
 }
 
 __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
@@ -729,31 +838,32 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                                               uint8_t* qw_out, int wstride) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int ndm = T.ndm, tc = T.tc, lane = threadIdx.x;
-    const long long t0 = (long long)blockIdx.x * 64;
-    const int pb0 = (int)(t0 / ndm);
-    const int maxpb = 63 / ndm + 2;
+    const int wg = blockIdx.x;
+    const int slot = wg / kXcds, xcd = wg - slot * kXcds;
+    const int grp = xcd + kXcds * (slot / ndm);
+    const int m = slot - (slot / ndm) * ndm;
+    const int pb0 = grp * kDmPbs;
+    if (pb0 >= npb) return;
     float* s_wts = reinterpret_cast<float*>(smem);
-    float* s_wes = s_wts + maxpb * tc;
-    float* s_dw = s_wes + maxpb * tc;
+    float* s_wes = s_wts + kDmPbs * tc;
+    float* s_dw = s_wes + kDmPbs * tc;
     float* s_aux = s_dw + 64 * wstride;
     {
-        const int pb_end = min((int)((t0 + 63) / ndm) + 1, npb);
-        const int n = (pb_end - pb0) * tc;
-        for (int e = lane; e < n; e += 64) {
+        const int nl = min(kDmPbs, npb - pb0);
+        for (int e = lane; e < nl * tc; e += 64) {
             const int l = e / tc, t = e - l * tc, pb = pb0 + l;
-            s_wts[e] = wts_all[(size_t)pb * tc + t];
+            s_wts[t * kDmPbs + l] = wts_all[(size_t)pb * tc + t];
             int p = 0;
             if (pc > 1) p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[pb]) * tc + t];
             // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
-            s_wes[e] = ideal[pb].wes[p];
+            s_wes[t * kDmPbs + l] = ideal[pb].wes[p];
         }
     }
     __syncthreads();
-    const long long tid = t0 + lane;
-    if (tid >= (long long)npb * ndm) return;
-    const int pb = (int)(tid / ndm), m = (int)(tid - (long long)pb * ndm);
-    const float* wts = s_wts + (pb - pb0) * tc;
-    const float* wes = s_wes + (pb - pb0) * tc;
+    const int pb = pb0 + lane;
+    if (pb >= npb) return;
+    const float* wts = s_wts + lane;  // wts[i * 64]
+    const float* wes = s_wes + lane;
     float* dw = s_dw + lane;    // dw[i * 64]
     float* aux = s_aux + lane;  // aux[i * 64]
     const DmInfo di = T.dm[m];
@@ -774,7 +884,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     }
     // pass02_decimated_weights.wgsl:132-270
     if (tc == wc) {
-        for (int i = 0; i < wc; ++i) dw[i * 64] = wts[i];
+        for (int i = 0; i < wc; ++i) dw[i * 64] = wts[i * 64];
     } else {
         for (int w = 0; w < wc; ++w) {
             const int n = w2tc[w];
@@ -784,8 +894,8 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 float ws = 0.0f, tcon = 1e-10f;
                 for (int jx = 0; jx < n; ++jx) {
                     const int t = w2ti[off + jx];
-                    const float cw = w2tw[off + jx] * wes[t];
-                    ws += wts[t] * cw;
+                    const float cw = w2tw[off + jx] * wes[t * 64];
+                    ws += wts[t * 64] * cw;
                     tcon += cw;
                 }
                 v = ws / tcon;
@@ -799,12 +909,12 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int jx = 0; jx < n; ++jx) {
                 const int t = w2ti[off + jx];
                 const float contrib = w2tw[off + jx];
-                const float scale = wes[t] * contrib;
+                const float scale = wes[t * 64] * contrib;
                 float infill = 0.0f;
                 const int nt = t2wc[t];
                 for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
                 a += contrib * scale;
-                bacc += (infill - wts[t]) * scale;
+                bacc += (infill - wts[t * 64]) * scale;
             }
             dw[w * 64] = aux[w * 64] + wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
         }
@@ -932,8 +1042,8 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int i = 0; i < tc; ++i) {
                 float v = 0.0f;
                 for (int jx = 0; jx < t2wc[i]; ++jx) v += aux[t2wi[i * 4 + jx] * 64] * t2ww[i * 4 + jx];
-                const float diff = v - wts[i];
-                err += diff * diff * wes[i];
+                const float diff = v - wts[i * 64];
+                err += diff * diff * wes[i * 64];
             }
             we = err;
         }
@@ -972,7 +1082,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             out.ql = (int8_t)fql;
             out.ql_mod = (int8_t)fqlm;
         }
-        ccr[(size_t)pb * T.nbm + k] = out;
+        ccr[(size_t)k * npb + pb] = out;  // [mode][block]: coalesced across lanes
     }
 }
 
@@ -983,9 +1093,8 @@ __global__ void __launch_bounds__(64) k_top(int npb, int nbm, const Ccr* ccr, in
     float te[8];
     int ti[8];
     for (int i = 0; i < 8; ++i) { te[i] = kErrDefault; ti[i] = 0; }
-    const Ccr* c = ccr + (size_t)pb * nbm;
     for (int k = 0; k < nbm; ++k) {
-        const float e = c[k].total_error;
+        const float e = ccr[(size_t)k * npb + pb].total_error;
         if (e < te[7]) {
             te[7] = e; ti[7] = k;
             for (int j = 7; j > 0; --j)
@@ -1439,7 +1548,7 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     const uint8_t* w2ti = T.w2t_idx + di.w2t_base;
     const bool undecimated = tc == wc;
     const Ideal& I = ideal[pb];
-    const Ccr cr = ccr[(size_t)pb * T.nbm + k];
+    const Ccr cr = ccr[(size_t)k * npb + pb];
     float4 ep0[4], ep1[4], avg[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
@@ -1899,7 +2008,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
         ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab);
         {
             if (prof) prof->begin(K_DM_BM, s);
-            hipLaunchKernelGGL(k_dm_bm, dim3(blocks_for((long long)npb * T.ndm)), dim3(64), dm_bm_lds_bytes(T, B.wstride),
+            hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(T, npb)), dim3(64), dm_bm_lds_bytes(T, B.wstride),
                                s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab, B.ccr, B.qw, B.wstride);
             if (prof) prof->end(K_DM_BM, s);
         }
