@@ -248,6 +248,8 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[5] = L.add<uint8_t>(npb * (size_t)((qw_total + 15) & ~15));
     offs[6] = L.add<int16_t>(npb * 8);
     offs[7] = L.add<CandOut>(npb * 8);
+    offs[8] = L.add<int>(2 * (size_t)f.nbm + 1);
+    offs[9] = L.add<int>(npb * 8 + (size_t)f.nbm * 64);
     return L.size;
 }
 
@@ -324,10 +326,12 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->fp = footprint((int)bx, (int)by);
     err = upload_tables(e);
     if (err != hipSuccess) return hip_fail(e, err, "table upload");
-    // chunk size: keep the per-chunk working set near 2 GiB
-    size_t offs[8];
-    const size_t per_block = chunk_bytes_per_block(*e->fp, offs, 1);
-    int chunk = (int)std::max<size_t>(1024, (size_t(2) << 30) / per_block);
+    // chunk size: keep the per-chunk working set near 8 GiB (a 4096^2 image
+    // at 4x4 takes two chunks)
+    size_t offs[10];
+    const size_t fixed = chunk_bytes_per_block(*e->fp, offs, 0);
+    const size_t per_block = (chunk_bytes_per_block(*e->fp, offs, 1 << 16) - fixed) >> 16;
+    int chunk = (int)std::max<size_t>(1024, ((size_t(8) << 30) - fixed) / per_block);
     if (p.chunk_blocks) chunk = (int)p.chunk_blocks;
     chunk = std::min<int>(chunk, (int)e->nblocks);
     e->chunk_blocks = chunk;
@@ -343,6 +347,13 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->B.qw = b + offs[5];
     e->B.top = reinterpret_cast<int16_t*>(b + offs[6]);
     e->B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
+    e->B.bin_count = reinterpret_cast<int*>(b + offs[8]);
+    e->B.bin_cur = e->B.bin_count + e->fp->nbm;
+    e->B.work_total = e->B.bin_cur + e->fp->nbm;
+    e->B.work = reinterpret_cast<int*>(b + offs[9]);
+    // the per-mode counts start at zero; k_bins clears them after each use
+    err = hipMemset(e->B.bin_count, 0, sizeof(int) * (2 * (size_t)e->fp->nbm + 1));
+    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
     int maxw = 4;
     for (const DmInfo& d : e->fp->dm) maxw = std::max<int>(maxw, d.weight_count);
     e->B.wstride = (maxw + 3) & ~3;

@@ -105,6 +105,10 @@ struct ChunkBuffers {
     uint8_t* qw;        // [4 * max_nb][qw_stride]
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]
+    int* bin_count;     // [nbm] candidates per trial mode (zero between launches)
+    int* bin_cur;       // [nbm] next free work-list position per mode
+    int* work_total;    // [1] work-list length including padding
+    int* work;          // [4 * max_nb * 8 + nbm * 64] (partitioned block * 8 + slot), bucketed by mode
     int wstride;        // LDS row of quantised weights: max weight count of the trial grids, multiple of 4
 };
 

@@ -1087,24 +1087,86 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
 }
 
 // ---- pass12 -------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_top(int npb, int nbm, const Ccr* ccr, int16_t* top) {
+// Top-8 trial modes per partitioned block, plus the per-mode candidate counts
+// that size the refinement work list (LDS histogram, one global add per
+// workgroup and mode).  Slots without a candidate get the default error.
+__global__ void __launch_bounds__(64) k_top(int npb, int nbm, const Ccr* ccr, int16_t* top, int* bin_count,
+                                            CandOut* cand) {
+    extern __shared__ int s_hist[];  // [nbm]
+    for (int i = threadIdx.x; i < nbm; i += 64) s_hist[i] = 0;
+    __syncthreads();
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pb >= npb) return;
-    float te[8];
-    int ti[8];
-    for (int i = 0; i < 8; ++i) { te[i] = kErrDefault; ti[i] = 0; }
-    for (int k = 0; k < nbm; ++k) {
-        const float e = ccr[(size_t)k * npb + pb].total_error;
-        if (e < te[7]) {
-            te[7] = e; ti[7] = k;
-            for (int j = 7; j > 0; --j)
-                if (te[j] < te[j - 1]) {
-                    const float fe = te[j]; te[j] = te[j - 1]; te[j - 1] = fe;
-                    const int fi = ti[j]; ti[j] = ti[j - 1]; ti[j - 1] = fi;
-                }
+    if (pb < npb) {
+        float te[8];
+        int ti[8];
+        for (int i = 0; i < 8; ++i) { te[i] = kErrDefault; ti[i] = 0; }
+        for (int k = 0; k < nbm; ++k) {
+            const float e = ccr[(size_t)k * npb + pb].total_error;
+            if (e < te[7]) {
+                te[7] = e; ti[7] = k;
+                for (int j = 7; j > 0; --j)
+                    if (te[j] < te[j - 1]) {
+                        const float fe = te[j]; te[j] = te[j - 1]; te[j - 1] = fe;
+                        const int fi = ti[j]; ti[j] = ti[j - 1]; ti[j - 1] = fi;
+                    }
+            }
+        }
+        for (int i = 0; i < 8; ++i) {
+            const int v = te[i] < kErrDefault ? ti[i] : -1;
+            top[pb * 8 + i] = (int16_t)v;
+            if (v >= 0) atomicAdd(&s_hist[v], 1);
+            else cand[pb * 8 + i].error = kErrDefault;  // pass12 produced no candidate for this slot
         }
     }
-    for (int i = 0; i < 8; ++i) top[pb * 8 + i] = (int16_t)(te[i] < kErrDefault ? ti[i] : -1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbm; i += 64)
+        if (s_hist[i]) atomicAdd(&bin_count[i], s_hist[i]);
+}
+
+// Work-list layout: bucket k starts at a multiple of 64; padding entries are
+// -1.  Resets the counts for the next launch.
+__global__ void __launch_bounds__(64) k_bins(int nbm, int* bin_count, int* bin_cur, int* work, int* work_total) {
+    extern __shared__ int s_off[];  // [nbm + 1]
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < nbm; ++k) {
+            s_off[k] = run;
+            run += (bin_count[k] + 63) & ~63;
+        }
+        s_off[nbm] = run;
+        *work_total = run;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbm; k += 64) {
+        const int c = bin_count[k], o = s_off[k];
+        bin_cur[k] = o;
+        for (int i = o + c; i < s_off[k + 1]; ++i) work[i] = -1;
+        bin_count[k] = 0;
+    }
+}
+
+// Fills the buckets: positions are reserved per workgroup and mode, so the
+// order inside a bucket is arbitrary; every task writes its own output slot.
+__global__ void __launch_bounds__(64) k_scatter(int npb, int nbm, const int16_t* top, int* bin_cur, int* work) {
+    extern __shared__ int s_sc[];  // [2][nbm]
+    int* s_cnt = s_sc;
+    int* s_base = s_sc + nbm;
+    for (int i = threadIdx.x; i < nbm; i += 64) s_cnt[i] = 0;
+    __syncthreads();
+    const int pb = blockIdx.x * blockDim.x + threadIdx.x;
+    int kk[8], loc[8];
+    for (int s = 0; s < 8; ++s) {
+        kk[s] = pb < npb ? (int)top[pb * 8 + s] : -1;
+        loc[s] = kk[s] >= 0 ? atomicAdd(&s_cnt[kk[s]], 1) : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbm; i += 64) {
+        const int c = s_cnt[i];
+        s_base[i] = c ? atomicAdd(&bin_cur[i], c) : 0;
+    }
+    __syncthreads();
+    for (int s = 0; s < 8; ++s)
+        if (kk[s] >= 0) work[s_base[kk[s]] + loc[s]] = pb * 8 + s;
 }
 
 // ---- pass14 helpers (pass14_pack_color_endpoints.wgsl:391-1221) -------------
@@ -1459,13 +1521,22 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 }
 
 // ---- passes 13-17, refinement loop of compression_pipeline.cpp:473-484 ---------
-// One lane per (partitioned block, candidate slot); a 64-lane workgroup holds
-// 8 partitioned blocks x 8 slots.  The blocks' texels and partition maps, and
-// each lane's quantised weights and unpacked endpoints, live in LDS; the
-// per-partition sums of pass13 are accumulated partition by partition (each
-// partition still sees its texels in index order, so the float results are
-// those of the interleaved loop).
-constexpr int kRefinePbs = 8;
+// One lane per (partitioned block, candidate slot).  The candidates are
+// first bucketed by trial block mode (k_top counts, k_bins lays out buckets
+// padded to whole workgroups, k_scatter fills them), so a workgroup's lanes
+// share the mode: loop bounds and decimation-table reads are uniform.  Each
+// lane keeps its block's texels (small footprints), partition map, quantised
+// weights and unpacked endpoints in its own LDS rows; the per-partition sums
+// of pass13 are accumulated partition by partition (each partition still
+// sees its texels in index order, so the float results are those of the
+// interleaved loop).
+constexpr int kRefineStageMaxTc = 36;  // stage texels in LDS up to 6x6
+
+size_t refine_lds_bytes(const DevTables& T, int wstride) {
+    size_t bytes = 2 * 4 * 64 * sizeof(uchar4) + (size_t)T.tc * 64 + 2 * (size_t)wstride * 64;
+    if (T.tc <= kRefineStageMaxTc) bytes += (size_t)T.tc * 64 * sizeof(float4);
+    return bytes;
+}
 
 struct PackOut {
     int format;
@@ -1487,56 +1558,49 @@ __device__ __forceinline__ uint32_t pk_byte(uint2 v, int i) { return ((i < 4 ? v
 
 __device__ __forceinline__ int4 load_ue(const uchar4 u) { return make_int4(u.x * 257, u.y * 257, u.z * 257, u.w * 257); }
 
-size_t refine_lds_bytes(const DevTables& T, int wstride) {
-    const size_t px = (size_t)kRefinePbs * T.tc * sizeof(float4);
-    const size_t tp = ((size_t)kRefinePbs * T.tc + 15) & ~size_t(15);
-    return px + tp + 2 * 64 * (size_t)wstride + 2 * 4 * 64 * sizeof(uchar4);
-}
-
+template <bool kStage>
 __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
                                                const Ideal* ideal, const int16_t* top, const Ccr* ccr,
-                                               const uint8_t* qw_all, CandOut* cand_out, int wstride) {
+                                               const uint8_t* qw_all, const int* work, const int* work_total,
+                                               CandOut* cand_out, int wstride) {
     extern __shared__ __align__(16) uint8_t smem[];
-    const int tc = T.tc;
+    const int tc = T.tc, lane = threadIdx.x;
+    const int base = blockIdx.x * 64;
+    if (base >= *work_total) return;
+    // every lane of the workgroup refines a candidate of the same trial block
+    // mode (the work list is bucketed by mode, buckets padded to 64 lanes)
+    const int task = work[base + lane];
+    const int k = __builtin_amdgcn_readfirstlane((int)top[__builtin_amdgcn_readfirstlane(work[base])]);
+    if (task < 0) return;
+    // per-lane LDS rows, laid out [index][lane]
     float4* s_px = reinterpret_cast<float4*>(smem);
-    uint8_t* s_tp = smem + (size_t)kRefinePbs * tc * sizeof(float4);
-    uint8_t* s_qw = s_tp + (((size_t)kRefinePbs * tc + 15) & ~size_t(15));
-    uint8_t* s_bqw = s_qw + 64 * wstride;
-    uchar4* s_ue0 = reinterpret_cast<uchar4*>(s_bqw + 64 * wstride);
+    uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(float4) : 0));
     uchar4* s_ue1 = s_ue0 + 4 * 64;
-    const int lane = threadIdx.x;
-    const int lpb = lane >> 3, slot = lane & 7;
-    const int pb = blockIdx.x * kRefinePbs + lpb;
-    // stage the texels (as search values) and partition maps of the 8 blocks
-    if (pb < npb) {
-        const Pb b0 = make_pb(A, T, pb_part, pc, pb);
-        for (int i = slot; i < tc; i += 8) {
+    uint8_t* s_tp = reinterpret_cast<uint8_t*>(s_ue1 + 4 * 64);
+    uint8_t* s_qw = s_tp + (size_t)tc * 64;
+    uint8_t* s_bqw = s_qw + (size_t)wstride * 64;
+    const int pb = task >> 3;
+    const Pb b = make_pb(A, T, pb_part, pc, pb);
+    for (int i = 0; i < tc; ++i) {
+        if (kStage) {
             float px[4];
-            texel_px(A, T, b0.g, i, px);
-            s_px[lpb * tc + i] = make_float4(px[0], px[1], px[2], px[3]);
-            s_tp[lpb * tc + i] = (uint8_t)texel_part(b0, i);
+            texel_px(A, T, b.g, i, px);
+            s_px[i * 64 + lane] = make_float4(px[0], px[1], px[2], px[3]);
         }
+        s_tp[i * 64 + lane] = (uint8_t)texel_part(b, i);
     }
-    __syncthreads();
-    const int tid = pb * 8 + slot;
-    if (pb >= npb) return;
-    CandOut* co = cand_out + tid;
-    const int k = top[tid];
-    if (k < 0) {  // pass12 produced no candidate for this slot
-        co->error = kErrDefault;
-        return;
-    }
-    const float4* bpx = s_px + lpb * tc;
-    const uint8_t* btp = s_tp + lpb * tc;
-    uint8_t* qw = s_qw + lane * wstride;
-    uint8_t* bqw = s_bqw + lane * wstride;
+    auto px_of = [&](int i) -> float4 {
+        if (kStage) return s_px[i * 64 + lane];
+        float px[4];
+        texel_px(A, T, b.g, i, px);
+        return make_float4(px[0], px[1], px[2], px[3]);
+    };
+    const uint8_t* btp = s_tp + lane;  // btp[i * 64]
+    uint8_t* qw = s_qw + lane;
+    uint8_t* bqw = s_bqw + lane;
     int ppc[4];
-    if (pc == 1) {
-        ppc[0] = tc; ppc[1] = ppc[2] = ppc[3] = 0;
-    } else {
-        const size_t row = (size_t)(pc - 2) * kPartitionings + pb_part[pb];
-        for (int i = 0; i < 4; ++i) ppc[i] = T.part_counts[row * 4 + i];
-    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ppc[i] = b.ppc[i];
     const BmInfo bmi = T.bm[k];
     const DmInfo di = T.dm[bmi.dm];
     const int wc = di.weight_count;
@@ -1560,7 +1624,7 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     const uint16_t in_formats = cr.formats;
     {
         const uint8_t* qsrc = qw_all + (size_t)pb * T.qw_stride + T.qw_off[k];
-        for (int i = 0; i < wc; ++i) qw[i] = qsrc[i];
+        for (int i = 0; i < wc; ++i) qw[i * 64] = qsrc[i];
     }
     const float lsw = (kCW[0] + kCW[1]) + kCW[2];
     const float cw[4] = {kCW[0], kCW[1], kCW[2], kCW[3]};
@@ -1569,10 +1633,10 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     uint16_t best_formats = 0;
     uint2 best_pk[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
     auto weight_of = [&](int i) -> float {
-        if (undecimated) return (float)qw[i] / 64.0f;
+        if (undecimated) return (float)qw[i * 64] / 64.0f;
         float w = 0.0f;
         const int n = t2wc[i];
-        for (int j = 0; j < n; ++j) w += ((float)qw[t2wi[i * 4 + j]] / 64.0f) * t2ww[i * 4 + j];
+        for (int j = 0; j < n; ++j) w += ((float)qw[t2wi[i * 4 + j] * 64] / 64.0f) * t2ww[i * 4 + j];
         return w;
     };
     for (int a = 0; a < 6; ++a) {
@@ -1593,9 +1657,9 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
             float wmin1 = 1.0f, wmax1 = 0.0f, ls = 0.0f, ms = 0.0f, rs = 0.0f, smin = 1e10f, smax = 0.0f;
             float cx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, cy[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sv0 = 0.0f, sv1 = 0.0f;
             for (int i = 0; i < tc; ++i) {
-                if (pc > 1 && btp[i] != p) continue;
+                if (pc > 1 && btp[i * 64] != p) continue;
                 const float w = weight_of(i);
-                const float4 px4 = bpx[i];
+                const float4 px4 = px_of(i);
                 const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
                 if (w < wmin1) wmin1 = w;
                 if (w > wmax1) wmax1 = w;
@@ -1694,20 +1758,26 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
             s_ue1[p * 64 + lane] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
         }
         // pass17 (before pass16 on the first iteration), then pass16 + pass17
+        // Once pass16 leaves every weight where it was, the next iteration
+        // recomputes the same endpoints from the same weights (pass13 keeps a
+        // previous endpoint only where it kept it this time), so it repeats
+        // this iteration's error and cannot replace the best candidate: the
+        // remaining iterations are skipped.
+        bool moved = false;
         for (int phase = (a == 0 ? 0 : 1); phase < 2; ++phase) {
             if (phase == 1) {
                 // pass16_realign_weights.wgsl:211-367
                 const int q = bmi.quant;
                 if (undecimated) {
                     for (int t = 0; t < tc; ++t) {
-                        const int uqw = qw[t];
+                        const int uqw = qw[t * 64];
                         const uint32_t pn = c_qt.prev_next[q][uqw];
                         const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
                         const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
-                        const int p = btp[t];
+                        const int p = btp[t * 64];
                         const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
                         const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
-                        const float4 px4 = bpx[t];
+                        const float4 px4 = px_of(t);
                         const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                         float s0[4], s1[4], s2[4];
                         for (int j = 0; j < 4; ++j) {
@@ -1717,12 +1787,15 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
                             s0[j] = diff * diff; s1[j] = dfd * dfd; s2[j] = dfu * dfu;
                         }
                         const float eb = dot4(s0, cw), ed = dot4(s1, cw), eu = dot4(s2, cw);
-                        if ((eu < eb) && (eu < ed) && (uqw < 64)) qw[t] = (uint8_t)up;
-                        else if ((ed < eb) && (uqw > 0)) qw[t] = (uint8_t)down;
+                        int nq = uqw;
+                        if ((eu < eb) && (eu < ed) && (uqw < 64)) nq = up;
+                        else if ((ed < eb) && (uqw > 0)) nq = down;
+                        moved |= nq != uqw;
+                        qw[t * 64] = (uint8_t)nq;
                     }
                 } else {
                     for (int we = 0; we < wc; ++we) {
-                        const int uqw = qw[we];
+                        const int uqw = qw[we * 64];
                         const uint32_t pn = c_qt.prev_next[q][uqw];
                         const float down = (float)(pn & 0xFF), up = (float)((pn >> 8) & 0xFF), wbase = (float)uqw;
                         const float ddn = down - wbase, dup = up - wbase;
@@ -1734,14 +1807,14 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
                             bool found = false;
                             for (int j = 0; j < t2wc[t]; ++j) {
                                 const int wi = t2wi[t * 4 + j];
-                                wbt += (float)qw[wi] * t2ww[t * 4 + j];
+                                wbt += (float)qw[wi * 64] * t2ww[t * 4 + j];
                                 if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
                             }
                             const float wdd = ddn * twb, wud = dup * twb;
-                            const int p = btp[t];
+                            const int p = btp[t * 64];
                             const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
                             const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
-                            const float4 px4 = bpx[t];
+                            const float4 px4 = px_of(t);
                             const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                             for (int j = 0; j < 4; ++j) {
                                 const float base = (float)e0[j], offs = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
@@ -1753,8 +1826,11 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
                             }
                         }
                         const float eb = dot4(ebv, cw), ed = dot4(edv, cw), eu = dot4(euv, cw);
-                        if ((eu < eb) && (eu < ed) && (uqw < 64)) qw[we] = (uint8_t)up;
-                        else if ((ed < eb) && (uqw > 0)) qw[we] = (uint8_t)down;
+                        int nq = uqw;
+                        if ((eu < eb) && (eu < ed) && (uqw < 64)) nq = (int)up;
+                        else if ((ed < eb) && (uqw > 0)) nq = (int)down;
+                        moved |= nq != uqw;
+                        qw[we * 64] = (uint8_t)nq;
                     }
                 }
             }
@@ -1762,10 +1838,10 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
             float total = 0.0f;
             for (int i = 0; i < tc; ++i) {
                 const float w = weight_of(i);
-                const int p = btp[i];
+                const int p = btp[i * 64];
                 const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
                 const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
-                const float4 px4 = bpx[i];
+                const float4 px4 = px_of(i);
                 const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                 const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
                 float dd[4];
@@ -1783,9 +1859,10 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
                 best_formats = pack_fmt4(fn);
 #pragma unroll
                 for (int p = 0; p < 4; ++p) best_pk[p] = pk[p];
-                for (int i = 0; i < wc; ++i) bqw[i] = qw[i];
+                for (int i = 0; i < wc; ++i) bqw[i * 64] = qw[i * 64];
             }
         }
+        if (!moved) break;
     }
     CandOut best;
     best.error = best_err;
@@ -1797,8 +1874,8 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
 #pragma unroll
     for (int p = 0; p < 4; ++p)
         for (int i = 0; i < 8; ++i) best.packed[p * 8 + i] = (uint8_t)pk_byte(best_pk[p], i);
-    for (int i = 0; i < 64; ++i) best.qw[i] = i < wc ? bqw[i] : 0;
-    *co = best;
+    for (int i = 0; i < 64; ++i) best.qw[i] = i < wc ? bqw[i * 64] : 0;
+    cand_out[task] = best;
 }
 
 // ---- pass18 + host argmin (compression_pipeline.cpp:499-519) --------------------
@@ -2012,12 +2089,27 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
                                s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab, B.ccr, B.qw, B.wstride);
             if (prof) prof->end(K_DM_BM, s);
         }
-        ASTCG_LAUNCH(K_TOP, k_top, blocks_for(npb), npb, T.nbm, B.ccr, B.top);
+        {
+            if (prof) prof->begin(K_TOP, s);
+            const size_t hist = (size_t)T.nbm * sizeof(int);
+            hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, npb, T.nbm, B.ccr, B.top,
+                               B.bin_count, B.cand);
+            hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), hist + sizeof(int), s, T.nbm, B.bin_count, B.bin_cur,
+                               B.work, B.work_total);
+            hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, npb, T.nbm, B.top,
+                               B.bin_cur, B.work);
+            if (prof) prof->end(K_TOP, s);
+        }
         {
             if (prof) prof->begin(K_REFINE, s);
             const size_t lds = refine_lds_bytes(T, B.wstride);
-            hipLaunchKernelGGL(k_refine, dim3((npb + kRefinePbs - 1) / kRefinePbs), dim3(64), lds, s, A, T, pc,
-                               B.pb_part, npb, B.ideal, B.top, B.ccr, B.qw, B.cand, B.wstride);
+            const unsigned grid = blocks_for((long long)npb * 8 + (long long)T.nbm * 63);
+            if (T.tc <= kRefineStageMaxTc)
+                hipLaunchKernelGGL(k_refine<true>, dim3(grid), dim3(64), lds, s, A, T, pc, B.pb_part, npb, B.ideal,
+                                   B.top, B.ccr, B.qw, B.work, B.work_total, B.cand, B.wstride);
+            else
+                hipLaunchKernelGGL(k_refine<false>, dim3(grid), dim3(64), lds, s, A, T, pc, B.pb_part, npb, B.ideal,
+                                   B.top, B.ccr, B.qw, B.work, B.work_total, B.cand, B.wstride);
             if (prof) prof->end(K_REFINE, s);
         }
         ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best);
