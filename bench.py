@@ -22,6 +22,21 @@ import numpy as np  # noqa: E402
 WIDTH = HEIGHT = 4096
 SEED = 1
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch of a kernel, measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs of this script (scaled by a plain copy kernel of known
+# size); the summary it comes from is named in the file.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(workload, kernel):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            entry = json.load(f).get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None, None
+    if not entry:
+        return None, None
+    return entry["bytes_per_launch"], entry["source"]
 
 
 def parse():
@@ -40,18 +55,23 @@ def parse():
 
 
 def cpu_baseline(img, bx, by, target_s):
-    """Times the oracle on the first blocks of the same image, growing the
-    sample until it takes about target_s seconds."""
+    """Times the oracle on the first blocks of the same image: a growing
+    calibration sample, then one run sized to take about target_s seconds."""
     import oracle_lib as O
     threads = min(os.cpu_count() or 1, 16)
+    total = ((img.shape[1] + bx - 1) // bx) * ((img.shape[0] + by - 1) // by)
     n = 256
     while True:
         t0 = time.perf_counter()
         O.encode(img, bx, by, 4, threads=threads, first_block=0, block_count=n, want_symbolic=False)
         dt = time.perf_counter() - t0
-        if dt >= target_s / 4 or n >= 1 << 20:
+        if dt >= 1.0 or n >= total:
             break
-        n *= 4
+        n = min(n * 4, total)
+    n = int(min(total, max(n, n * target_s / dt)))
+    t0 = time.perf_counter()
+    O.encode(img, bx, by, 4, threads=threads, first_block=0, block_count=n, want_symbolic=False)
+    dt = time.perf_counter() - t0
     texels = n * bx * by
     return {"value": texels / dt / 1e6, "unit": "Mtexel/s", "cores": threads, "kind": "port",
             "sample": "first %d blocks (%d texels) of the same %dx%d image, C oracle, %.1f s" %
@@ -108,6 +128,8 @@ def main():
     bytes_per_texel = 4.0 + 16.0 / (bx * by)
     texels_per_launch = texels * a.steps / max(launches, 1)
     achieved = bytes_per_texel * texels_per_launch / (avg_ms / 1e3) / 1e9
+    workload_key = "%dx%d seed %d %dx%d" % (w, h, SEED, bx, by)
+    traffic, traffic_src = measured_traffic(workload_key, dom)
     blocks = d_out.cpu().numpy()
     dec = A.decode_image(bx, by, w, h, blocks)
     line = {
@@ -127,8 +149,10 @@ def main():
                                "partitions 1-4" % (w, h, SEED, bx, by),
                    "blocks": nblocks, "psnr_rgba_db": round(A.psnr(dec, img), 3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches,
+                     "algorithmic_bytes_per_launch": bytes_per_texel * texels_per_launch,
+                     "traffic_source": traffic_src},
         "kernels_ms_per_step": {k: v["total_ms"] / a.steps for k, v in ktimes.items()},
     }
     if not a.no_cpu:
