@@ -241,7 +241,8 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[0] = L.add<int16_t>(npb);
     offs[1] = L.add<Ideal>(npb);
     offs[2] = L.add<float>(npb * f.texel_count);
-    offs[3] = L.add<ColorTab>(npb);
+    offs[3] = L.add<float>(npb * kColorTabRows);
+    offs[10] = L.add<uint16_t>(npb * kColorTabRows);
     offs[4] = L.add<Ccr>(npb * f.nbm);
     int qw_total = 0;
     for (int k = 0; k < f.nbm; ++k) qw_total += f.dm[f.bm[k].dm].weight_count;
@@ -328,7 +329,7 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     if (err != hipSuccess) return hip_fail(e, err, "table upload");
     // chunk size: keep the per-chunk working set near 8 GiB (a 4096^2 image
     // at 4x4 takes two chunks)
-    size_t offs[10];
+    size_t offs[11];
     const size_t fixed = chunk_bytes_per_block(*e->fp, offs, 0);
     const size_t per_block = (chunk_bytes_per_block(*e->fp, offs, 1 << 16) - fixed) >> 16;
     int chunk = (int)std::max<size_t>(1024, ((size_t(8) << 30) - fixed) / per_block);
@@ -342,7 +343,8 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->B.pb_part = reinterpret_cast<int16_t*>(b + offs[0]);
     e->B.ideal = reinterpret_cast<Ideal*>(b + offs[1]);
     e->B.wts = reinterpret_cast<float*>(b + offs[2]);
-    e->B.ctab = reinterpret_cast<ColorTab*>(b + offs[3]);
+    e->B.ct_err = reinterpret_cast<float*>(b + offs[3]);
+    e->B.ct_fmt = reinterpret_cast<uint16_t*>(b + offs[10]);
     e->B.ccr = reinterpret_cast<Ccr*>(b + offs[4]);
     e->B.qw = b + offs[5];
     e->B.top = reinterpret_cast<int16_t*>(b + offs[6]);

@@ -59,11 +59,10 @@ struct Ideal {
 };
 
 // pass09 (1 partition: [quant][int count]) or pass10 (2-4 partitions:
-// [quant][combined int count]) colour-format table, per partitioned block.
-struct ColorTab {
-    float err[21][13];
-    uint16_t fmt[21][13];  // 4 formats x 4 bits
-};
+// [quant][combined int count]) colour-format table: error and 4 x 4-bit
+// formats per (quant 4..20, column 0..12, partitioned block), stored
+// [quant - 4][column][PB].
+constexpr int kColorTabRows = 17 * 13;
 
 // pass11 result per (partitioned block, trial block mode).
 struct Ccr {
@@ -99,10 +98,11 @@ struct SymBest {
 struct ChunkBuffers {
     int16_t* pb_part;   // [4 * max_nb]
     Ideal* ideal;       // [4 * max_nb]
-    float* wts;         // [4 * max_nb][tc]
-    ColorTab* ctab;     // [4 * max_nb]
+    float* wts;         // [tc][npb of the launch]
+    float* ct_err;      // [17][13][npb of the launch]
+    uint16_t* ct_fmt;   // [17][13][npb of the launch]
     Ccr* ccr;           // [nbm][npb of the launch]
-    uint8_t* qw;        // [4 * max_nb][qw_stride]
+    uint8_t* qw;        // [qw_stride][npb of the launch]: row qw_off[mode] + weight
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]
     int* bin_count;     // [nbm] candidates per trial mode (zero between launches)

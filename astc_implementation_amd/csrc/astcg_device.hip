@@ -539,7 +539,7 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int 
 __device__ __constant__ int kColorLevels[17] = {5, 7, 9, 11, 15, 19, 23, 31, 39, 47, 63, 79, 95, 127, 159, 191, 255};
 
 __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
-                                                    Ideal* ideal_out, float* wts_out, ColorTab* ctab) {
+                                                    Ideal* ideal_out, float* wts_out, float* ct_err, uint16_t* ct_fmt) {
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     if (pb >= npb) return;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
@@ -608,7 +608,6 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
         }
         I.wes[p] = lensq * error_weight;
     }
-    float* wts = wts_out + (size_t)pb * tc;
     for (int i = 0; i < tc; ++i) {
         float px[4];
         texel_px(A, T, b.g, i, px);
@@ -623,7 +622,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
                 const float span = wmax(maxp[q] - minp[q], 1e-6f);
                 w = wclamp((proj - minp[q]) / span, 0.0f, 1.0f);
             }
-        wts[i] = w;
+        wts_out[(size_t)i * npb + pb] = w;  // [texel][PB]: coalesced
     }
     float min_ep[4] = {10.0f, 10.0f, 10.0f, 10.0f};
 #pragma unroll
@@ -726,14 +725,8 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
         oe_rgb0[p] = can_oe ? 0.25f : 1.0f;
     }
     const float bc_rgba = can_bc ? 0.625f : 1.0f, bc_rgb = can_bc ? 0.5f : 1.0f;
-    ColorTab* ct = ctab + pb;
-    const int ncols = pc == 1 ? 4 : 13;
-    for (int q = 0; q < 4; ++q)
-        for (int k = 0; k < ncols; ++k) {
-            ct->err[q][k] = kErrDefault;
-            // pass09 leaves the unused quant rows with its default formats
-            ct->fmt[q][k] = pc == 1 ? (uint16_t)(k == 3 ? FMT_RGBA : k == 2 ? FMT_RGB : k == 1 ? FMT_RGB_SCALE : FMT_LUMINANCE) : 0;
-        }
+    // [quant - 4][column][PB]; rows for quant levels below 4 are never read
+    auto ct_at = [&](int q, int col) { return ((size_t)(q - 4) * 13 + col) * npb + pb; };
     for (int i = 0; i < 17; ++i) {
         const int q = i + 4;
         const int K = kColorLevels[i];
@@ -758,7 +751,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
             fmt[p][0] = FMT_LUMINANCE;
         }
         if (pc == 1) {
-            for (int k = 0; k < 4; ++k) { ct->err[q][k] = err[0][k]; ct->fmt[q][k] = (uint16_t)fmt[0][k]; }
+            for (int k = 0; k < 4; ++k) { ct_err[ct_at(q, k)] = err[0][k]; ct_fmt[ct_at(q, k)] = (uint16_t)fmt[0][k]; }
             continue;
         }
         // pass10_color_combinations_for_quant_{2,3,4}part.wgsl
@@ -804,7 +797,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
                     }
                 }
             }
-        for (int k = 0; k < 13; ++k) { ct->err[q][k] = ce[k]; ct->fmt[q][k] = cf[k]; }
+        for (int k = 0; k < 13; ++k) { ct_err[ct_at(q, k)] = ce[k]; ct_fmt[ct_at(q, k)] = cf[k]; }
     }
 }
 
@@ -834,7 +827,8 @@ unsigned dm_bm_grid(const DevTables& T, int npb) {
 }
 
 __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
-                                              const Ideal* ideal, const float* wts_all, const ColorTab* ctab, Ccr* ccr,
+                                              const Ideal* ideal, const float* wts_all, const float* ct_err,
+                                              const uint16_t* ct_fmt, Ccr* ccr,
                                               uint8_t* qw_out, int wstride) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int ndm = T.ndm, tc = T.tc, lane = threadIdx.x;
@@ -850,9 +844,10 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     float* s_aux = s_dw + 64 * wstride;
     {
         const int nl = min(kDmPbs, npb - pb0);
-        for (int e = lane; e < nl * tc; e += 64) {
-            const int l = e / tc, t = e - l * tc, pb = pb0 + l;
-            s_wts[t * kDmPbs + l] = wts_all[(size_t)pb * tc + t];
+        for (int e = lane; e < kDmPbs * tc; e += 64) {
+            const int t = e / kDmPbs, l = e - t * kDmPbs, pb = pb0 + l;
+            if (l >= nl) continue;
+            s_wts[t * kDmPbs + l] = wts_all[(size_t)t * npb + pb];
             int p = 0;
             if (pc > 1) p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[pb]) * tc + t];
             // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
@@ -997,14 +992,14 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     // passes 06, 07 and 11 for every trial block mode of this decimation mode
     const Ideal& I = ideal[pb];
     const float cutoff = I.cutoff;
-    const ColorTab& ct = ctab[pb];
-    uint8_t* qwo_pb = qw_out + (size_t)pb * T.qw_stride;
+    // colour-format table entries of this PB: [quant - 4][column][PB]
+    auto ct_at = [&](int q, int col) { return ((size_t)(q - 4) * 13 + col) * npb + pb; };
     for (int li = T.dm_bm_off[m]; li < T.dm_bm_off[m + 1]; ++li) {
         const int k = T.dm_bm_list[li];
         const BmInfo bmi = T.bm[k];
         Ccr out;
         out.ql = -1; out.ql_mod = -1; out.formats = 0;
-        uint8_t* qwo = qwo_pb + T.qw_off[k];
+        uint8_t* qwo = qw_out + (size_t)T.qw_off[k] * npb + pb;  // [weight row][PB]: coalesced
         const int q = bmi.quant;
         const int bc = kFreeBits[pc - 1] - bmi.weight_bits;
         float we;
@@ -1035,7 +1030,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 const int il = c_qt.quant_to_unquant[q][wl], ih = c_qt.quant_to_unquant[q][wh];
                 const float fl = (float)il, fh = (float)ih;
                 const bool use_high = (fl + fh) < (128.0f * ix);
-                qwo[i] = i < keep ? (uint8_t)(use_high ? ih : il) : 0;
+                qwo[(size_t)i * npb] = i < keep ? (uint8_t)(use_high ? ih : il) : 0;
                 aux[i * 64] = (use_high ? fh : fl) * rscale + lo;
             }
             float err = 0.0f;
@@ -1056,11 +1051,11 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int ic = 1; ic <= 4; ++ic) {
                 const int ql = c_qt.quant_mode[ic][bc];
                 if (ql < 4) continue;
-                const float e = ct.err[ql][ic - 1];
+                const float e = ct_err[ct_at(ql, ic - 1)];
                 if (e < best) { best = e; best_idx = ic; }
             }
             const int fql = c_qt.quant_mode[best_idx][bc];
-            const int f = fql >= 4 ? ct.fmt[fql][best_idx - 1] : FMT_LUMINANCE;
+            const int f = fql >= 4 ? ct_fmt[ct_at(fql, best_idx - 1)] : FMT_LUMINANCE;
             out.total_error = best + we;
             out.ql = (int8_t)fql;
             out.ql_mod = (int8_t)fql;
@@ -1072,12 +1067,12 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int ic = start; ic <= end; ++ic) {
                 const int ql = c_qt.quant_mode[ic][bc];
                 if (ql < 4) break;
-                const float e = ct.err[ql][ic - start];
+                const float e = ct_err[ct_at(ql, ic - start)];
                 if (e < best) { best = e; best_idx = ic; }
             }
             const int fql = c_qt.quant_mode[best_idx][bc];
             const int fqlm = c_qt.quant_mode[best_idx][bc + modbits];
-            out.formats = fql >= 4 ? ct.fmt[fql][best_idx - start] : 0;
+            out.formats = fql >= 4 ? ct_fmt[ct_at(fql, best_idx - start)] : 0;
             out.total_error = best + we;
             out.ql = (int8_t)fql;
             out.ql_mod = (int8_t)fqlm;
@@ -1623,8 +1618,8 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     const int ql = cr.ql, qlm = cr.ql_mod;
     const uint16_t in_formats = cr.formats;
     {
-        const uint8_t* qsrc = qw_all + (size_t)pb * T.qw_stride + T.qw_off[k];
-        for (int i = 0; i < wc; ++i) qw[i * 64] = qsrc[i];
+        const uint8_t* qsrc = qw_all + (size_t)T.qw_off[k] * npb + pb;
+        for (int i = 0; i < wc; ++i) qw[i * 64] = qsrc[(size_t)i * npb];
     }
     const float lsw = (kCW[0] + kCW[1]) + kCW[2];
     const float cw[4] = {kCW[0], kCW[1], kCW[2], kCW[3]};
@@ -2082,11 +2077,13 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
                                partition_lds_bytes(T), s, A, T, pc, B.pb_part);
             if (prof) prof->end(K_PARTITION, s);
         }
-        ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab);
+        ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err,
+                     B.ct_fmt);
         {
             if (prof) prof->begin(K_DM_BM, s);
             hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(T, npb)), dim3(64), dm_bm_lds_bytes(T, B.wstride),
-                               s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ctab, B.ccr, B.qw, B.wstride);
+                               s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw,
+                               B.wstride);
             if (prof) prof->end(K_DM_BM, s);
         }
         {

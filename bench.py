@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--block", default="4x4")
     ap.add_argument("--size", type=int, default=WIDTH)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device-to-device copy of this many MiB before the encode "
