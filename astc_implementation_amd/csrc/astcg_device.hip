@@ -818,7 +818,8 @@ constexpr int kDmPbs = 64;
 constexpr int kXcds = 8;
 
 size_t dm_bm_lds_bytes(const DevTables& T, int wstride) {
-    return (size_t)kDmPbs * T.tc * 2 * sizeof(float) + 2 * 64 * (size_t)wstride * sizeof(float);
+    return (size_t)kDmPbs * T.tc * sizeof(float) + 2 * 64 * (size_t)wstride * sizeof(float) +
+           (((size_t)kDmPbs * T.tc + 15) & ~size_t(15));
 }
 
 unsigned dm_bm_grid(const DevTables& T, int npb) {
@@ -839,9 +840,9 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     const int pb0 = grp * kDmPbs;
     if (pb0 >= npb) return;
     float* s_wts = reinterpret_cast<float*>(smem);
-    float* s_wes = s_wts + kDmPbs * tc;
-    float* s_dw = s_wes + kDmPbs * tc;
+    float* s_dw = s_wts + kDmPbs * tc;
     float* s_aux = s_dw + 64 * wstride;
+    uint8_t* s_tpart = reinterpret_cast<uint8_t*>(s_aux + 64 * wstride);
     {
         const int nl = min(kDmPbs, npb - pb0);
         for (int e = lane; e < kDmPbs * tc; e += 64) {
@@ -850,15 +851,20 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             s_wts[t * kDmPbs + l] = wts_all[(size_t)t * npb + pb];
             int p = 0;
             if (pc > 1) p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[pb]) * tc + t];
-            // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
-            s_wes[t * kDmPbs + l] = ideal[pb].wes[p];
+            s_tpart[t * kDmPbs + l] = (uint8_t)p;
         }
     }
     __syncthreads();
     const int pb = pb0 + lane;
     if (pb >= npb) return;
     const float* wts = s_wts + lane;  // wts[i * 64]
-    const float* wes = s_wes + lane;
+    // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
+    const float4 wes4 = *reinterpret_cast<const float4*>(ideal[pb].wes);
+    const uint8_t* tpart = s_tpart + lane;
+    auto wes_of = [&](int t) {
+        const int p = tpart[t * 64];
+        return p == 0 ? wes4.x : p == 1 ? wes4.y : p == 2 ? wes4.z : wes4.w;
+    };
     float* dw = s_dw + lane;    // dw[i * 64]
     float* aux = s_aux + lane;  // aux[i * 64]
     const DmInfo di = T.dm[m];
@@ -889,7 +895,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 float ws = 0.0f, tcon = 1e-10f;
                 for (int jx = 0; jx < n; ++jx) {
                     const int t = w2ti[off + jx];
-                    const float cw = w2tw[off + jx] * wes[t * 64];
+                    const float cw = w2tw[off + jx] * wes_of(t);
                     ws += wts[t * 64] * cw;
                     tcon += cw;
                 }
@@ -904,7 +910,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int jx = 0; jx < n; ++jx) {
                 const int t = w2ti[off + jx];
                 const float contrib = w2tw[off + jx];
-                const float scale = wes[t * 64] * contrib;
+                const float scale = wes_of(t) * contrib;
                 float infill = 0.0f;
                 const int nt = t2wc[t];
                 for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
@@ -929,14 +935,32 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     for (int q = 0; q < 8; ++q) { be[q] = 1e37f; bstep[q] = 0.0f; brc[q] = 0.0f; blow[q] = 0.0f; bao[q] = 0.0f; }
     const float inv2pi = 1.0f / (2.0f * 3.14159265358979323846f);
     const int mqs = di.max_quant_steps;
-    for (int sp = 0; sp < di.max_angular_steps; ++sp) {
-        float ax = 0.0f, ay = 0.0f;
+    // pass03 for all 16 step counts at once: one 64-byte table row per weight
+    // and table, each step's sums still taken in weight order
+    float ao[16];
+    {
+        float ax[16], ay[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) { ax[s] = 0.0f; ay[s] = 0.0f; }
         for (int jx = 0; jx < wc; ++jx) {
             const uint32_t is = wf2u(wround(wclamp(dw[jx * 64], 0.0f, 1.0f) * (1024.0f - 1.0f)));
-            ax += T.cos_t[is * 16 + sp];
-            ay += T.sin_t[is * 16 + sp];
+            const float4* cr = reinterpret_cast<const float4*>(T.cos_t + is * 16);
+            const float4* sr = reinterpret_cast<const float4*>(T.sin_t + is * 16);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float4 c = cr[v], sn = sr[v];
+                ax[4 * v + 0] += c.x; ax[4 * v + 1] += c.y; ax[4 * v + 2] += c.z; ax[4 * v + 3] += c.w;
+                ay[4 * v + 0] += sn.x; ay[4 * v + 1] += sn.y; ay[4 * v + 2] += sn.z; ay[4 * v + 3] += sn.w;
+            }
         }
-        const float off = atan2f_fixed(ay, ax) * inv2pi;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) ao[s] = atan2f_fixed(ay[s], ax[s]) * inv2pi;
+    }
+    for (int sp = 0; sp < di.max_angular_steps; ++sp) {
+        float off = ao[0];
+#pragma unroll
+        for (int s = 1; s < 16; ++s)
+            if (sp == s) off = ao[s];
         const float rcp = (float)(sp + 1);
         const float minidx = wround(minw * rcp - off), maxidx = wround(maxw * rcp - off);
         float e = 0.0f, cl = 0.0f, ch = 0.0f;
@@ -1038,7 +1062,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 float v = 0.0f;
                 for (int jx = 0; jx < t2wc[i]; ++jx) v += aux[t2wi[i * 4 + jx] * 64] * t2ww[i * 4 + jx];
                 const float diff = v - wts[i * 64];
-                err += diff * diff * wes[i * 64];
+                err += diff * diff * wes_of(i);
             }
             we = err;
         }
@@ -1525,12 +1549,24 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 // of pass13 are accumulated partition by partition (each partition still
 // sees its texels in index order, so the float results are those of the
 // interleaved loop).
-constexpr int kRefineStageMaxTc = 36;  // stage texels in LDS up to 6x6
+constexpr int kRefineStageMaxTc = 64;  // stage RGBA8 texels in LDS up to 8x8
 
 size_t refine_lds_bytes(const DevTables& T, int wstride) {
     size_t bytes = 2 * 4 * 64 * sizeof(uchar4) + (size_t)T.tc * 64 + 2 * (size_t)wstride * 64;
-    if (T.tc <= kRefineStageMaxTc) bytes += (size_t)T.tc * 64 * sizeof(float4);
+    if (T.tc <= kRefineStageMaxTc) bytes += (size_t)T.tc * 64 * sizeof(uint32_t);
     return bytes;
+}
+
+// (u8 / 255) * 65536 with IEEE float division semantics, without a divide or
+// a table: the product with the rounded reciprocal is corrected by one fused
+// residual step (Markstein), which is exact for all 256 inputs, and the
+// scaling by 2^16 is exact.
+__device__ __forceinline__ float u8_search_value(uint32_t v) {
+    const float x = (float)v;
+    const float r = 1.0f / 255.0f;
+    const float q = x * r;
+    const float e = __builtin_fmaf(-q, 255.0f, x);
+    return __builtin_fmaf(e, r * 65536.0f, q * 65536.0f);
 }
 
 struct PackOut {
@@ -1553,12 +1589,16 @@ __device__ __forceinline__ uint32_t pk_byte(uint2 v, int i) { return ((i < 4 ? v
 
 __device__ __forceinline__ int4 load_ue(const uchar4 u) { return make_int4(u.x * 257, u.y * 257, u.z * 257, u.w * 257); }
 
-template <bool kStage>
-__global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
+// Specialised per partition count, so per-partition state only occupies
+// registers for the partitions that exist.
+template <bool kStage, int PC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                                const Ideal* ideal, const int16_t* top, const Ccr* ccr,
                                                const uint8_t* qw_all, const int* work, const int* work_total,
                                                CandOut* cand_out, int wstride) {
     extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int pc = PC;
     const int tc = T.tc, lane = threadIdx.x;
     const int base = blockIdx.x * 64;
     if (base >= *work_total) return;
@@ -1568,24 +1608,29 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     const int k = __builtin_amdgcn_readfirstlane((int)top[__builtin_amdgcn_readfirstlane(work[base])]);
     if (task < 0) return;
     // per-lane LDS rows, laid out [index][lane]
-    float4* s_px = reinterpret_cast<float4*>(smem);
-    uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(float4) : 0));
+    uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);
+    uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(uint32_t) : 0));
     uchar4* s_ue1 = s_ue0 + 4 * 64;
     uint8_t* s_tp = reinterpret_cast<uint8_t*>(s_ue1 + 4 * 64);
     uint8_t* s_qw = s_tp + (size_t)tc * 64;
     uint8_t* s_bqw = s_qw + (size_t)wstride * 64;
     const int pb = task >> 3;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
+    const int bxi = b.g % A.blocks_x, byi = b.g / A.blocks_x;
     for (int i = 0; i < tc; ++i) {
         if (kStage) {
-            float px[4];
-            texel_px(A, T, b.g, i, px);
-            s_px[i * 64 + lane] = make_float4(px[0], px[1], px[2], px[3]);
+            const int dy = i / T.bx, dx = i - dy * T.bx;
+            const int x = min(bxi * T.bx + dx, A.width - 1), y = min(byi * T.by + dy, A.height - 1);
+            s_tex[i * 64 + lane] = *reinterpret_cast<const uint32_t*>(A.rgba + ((size_t)y * A.width + x) * 4);
         }
         s_tp[i * 64 + lane] = (uint8_t)texel_part(b, i);
     }
     auto px_of = [&](int i) -> float4 {
-        if (kStage) return s_px[i * 64 + lane];
+        if (kStage) {
+            const uint32_t v = s_tex[i * 64 + lane];
+            return make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
+                               u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
+        }
         float px[4];
         texel_px(A, T, b.g, i, px);
         return make_float4(px[0], px[1], px[2], px[3]);
@@ -1608,12 +1653,16 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
     const bool undecimated = tc == wc;
     const Ideal& I = ideal[pb];
     const Ccr cr = ccr[(size_t)k * npb + pb];
-    float4 ep0[4], ep1[4], avg[4];
+    float4 ep0[4], ep1[4];
+    float sdirs[4][3];  // pass13's scaled direction: depends on the average only
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         ep0[p] = make_float4(I.ep0[p][0], I.ep0[p][1], I.ep0[p][2], I.ep0[p][3]);
         ep1[p] = make_float4(I.ep1[p][0], I.ep1[p][1], I.ep1[p][2], I.ep1[p][3]);
-        avg[p] = make_float4(I.avg[p][0], I.avg[p][1], I.avg[p][2], I.avg[p][3]);
+        const float size = (float)ppc[p];
+        float q3[3];
+        for (int j = 0; j < 3; ++j) q3[j] = ((I.avg[p][j] * size) * kCW[j]) / wmax(kCW[j] * size, 1e-17f);
+        normalize3(q3, sdirs[p]);
     }
     const int ql = cr.ql, qlm = cr.ql_mod;
     const uint16_t in_formats = cr.formats;
@@ -1642,13 +1691,7 @@ __global__ void __launch_bounds__(64) k_refine(ImageArgs A, DevTables T, int pc,
             rgbs[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (p >= pc) continue;
             const float size = (float)ppc[p];
-            float sdir[3];
-            {
-                const float av[3] = {avg[p].x, avg[p].y, avg[p].z};
-                float q3[3];
-                for (int j = 0; j < 3; ++j) q3[j] = ((av[j] * size) * cw[j]) / wmax(cw[j] * size, 1e-17f);
-                normalize3(q3, sdir);
-            }
+            const float* sdir = sdirs[p];
             float wmin1 = 1.0f, wmax1 = 0.0f, ls = 0.0f, ms = 0.0f, rs = 0.0f, smin = 1e10f, smax = 0.0f;
             float cx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, cy[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sv0 = 0.0f, sv1 = 0.0f;
             for (int i = 0; i < tc; ++i) {
@@ -2064,6 +2107,27 @@ const char* kernel_name(int id) {
         if (prof) prof->end(id, s);                                                \
     } while (0)
 
+template <bool kStage>
+void launch_refine_pc(int pc, dim3 grid, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
+                      const ChunkBuffers& B, int npb) {
+#define ASTCG_REFINE(PCV)                                                                                         \
+    hipLaunchKernelGGL((k_refine<kStage, PCV>), grid, dim3(64), lds, s, A, T, B.pb_part, npb, B.ideal, B.top, B.ccr, \
+                       B.qw, B.work, B.work_total, B.cand, B.wstride)
+    switch (pc) {
+        case 1: ASTCG_REFINE(1); break;
+        case 2: ASTCG_REFINE(2); break;
+        case 3: ASTCG_REFINE(3); break;
+        default: ASTCG_REFINE(4); break;
+    }
+#undef ASTCG_REFINE
+}
+
+void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
+                   const ChunkBuffers& B, int npb) {
+    if (stage) launch_refine_pc<true>(pc, grid, lds, s, A, T, B, npb);
+    else launch_refine_pc<false>(pc, grid, lds, s, A, T, B, npb);
+}
+
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof) {
     const int threads = 64;
@@ -2101,12 +2165,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
             if (prof) prof->begin(K_REFINE, s);
             const size_t lds = refine_lds_bytes(T, B.wstride);
             const unsigned grid = blocks_for((long long)npb * 8 + (long long)T.nbm * 63);
-            if (T.tc <= kRefineStageMaxTc)
-                hipLaunchKernelGGL(k_refine<true>, dim3(grid), dim3(64), lds, s, A, T, pc, B.pb_part, npb, B.ideal,
-                                   B.top, B.ccr, B.qw, B.work, B.work_total, B.cand, B.wstride);
-            else
-                hipLaunchKernelGGL(k_refine<false>, dim3(grid), dim3(64), lds, s, A, T, pc, B.pb_part, npb, B.ideal,
-                                   B.top, B.ccr, B.qw, B.work, B.work_total, B.cand, B.wstride);
+            launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb);
             if (prof) prof->end(K_REFINE, s);
         }
         ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best);
