@@ -62,6 +62,19 @@ class Params(ctypes.Structure):
 FLAG_PROFILE = 1
 
 
+class Texture(ctypes.Structure):
+    """astcg_texture: one texture of a batch."""
+    _fields_ = [
+        ("rgba", ctypes.c_void_p),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("block_x", ctypes.c_uint32),
+        ("block_y", ctypes.c_uint32),
+        ("out", ctypes.c_void_p),
+        ("out_len", ctypes.c_size_t),
+    ]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("total_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
 
@@ -92,6 +105,8 @@ def library():
         "astcg_configure": (i32, [vp, u32, u32, u32, u32, vp]),
         "astcg_encode": (i32, [vp, vp, sz, vp, sz]),
         "astcg_encode_device": (i32, [vp, vp, vp, vp]),
+        "astcg_encode_batch": (i32, [vp, vp, u32, vp]),
+        "astcg_encode_batch_device": (i32, [vp, vp, u32, vp, vp]),
         "astcg_get_symbolic": (i32, [vp, vp, sz]),
         "astcg_block_counts": (i32, [vp, vp, vp, vp]),
         "astcg_symbolic_to_physical": (i32, [u32, u32, vp, vp]),
@@ -174,6 +189,37 @@ class Encoder:
         """Enqueues an encode of device buffers (integer addresses) on a HIP stream."""
         _check(library().astcg_encode_device(self._h, ctypes.c_void_p(d_rgba), ctypes.c_void_p(d_out),
                                              ctypes.c_void_p(stream or 0)), self._h)
+
+    def encode_batch(self, images, footprints, max_partitions=4, chunk_blocks=0, profile=False):
+        """Encodes several RGBA8 images (H x W x 4 uint8 arrays) with their own
+        footprints [(bx, by), ...] in one batched call; returns the block
+        arrays in input order."""
+        imgs = [np.ascontiguousarray(im, dtype=np.uint8) for im in images]
+        outs, tex = [], (Texture * max(len(imgs), 1))()
+        for i, (im, (bx, by)) in enumerate(zip(imgs, footprints)):
+            h, w = im.shape[:2]
+            out = np.zeros(((w + bx - 1) // bx) * ((h + by - 1) // by) * 16, np.uint8)
+            outs.append(out)
+            tex[i] = Texture(im.ctypes.data, w, h, bx, by, out.ctypes.data, out.nbytes)
+        p = Params()
+        library().astcg_default_params(ctypes.byref(p))
+        p.max_partitions = max_partitions
+        p.chunk_blocks = chunk_blocks
+        p.flags = FLAG_PROFILE if profile else 0
+        _check(library().astcg_encode_batch(self._h, tex, len(imgs), ctypes.byref(p)), self._h)
+        return outs
+
+    def encode_batch_device(self, textures, max_partitions=4, stream=None):
+        """textures: [(d_rgba, width, height, bx, by, d_out, out_len)] with
+        device addresses; enqueued on `stream`."""
+        tex = (Texture * max(len(textures), 1))()
+        for i, t in enumerate(textures):
+            tex[i] = Texture(*t)
+        p = Params()
+        library().astcg_default_params(ctypes.byref(p))
+        p.max_partitions = max_partitions
+        _check(library().astcg_encode_batch_device(self._h, tex, len(textures), ctypes.byref(p),
+                                                   ctypes.c_void_p(stream or 0)), self._h)
 
     def kernel_times(self):
         """{kernel: {"total_ms", "launches"}} since the last reset (needs profile=True)."""

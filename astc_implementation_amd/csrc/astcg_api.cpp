@@ -5,10 +5,13 @@
 // runs the search on the device in chunks of blocks and packs the 128-bit
 // blocks on the device.  Unlike the reference there is no per-batch host
 // round trip: the only device->host copy is the finished block stream.
+// Footprint tables are cached per encoder, so batches of textures with
+// several footprints reuse them; the working buffers grow on demand.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
 #include <cstring>
 #include <fstream>
 #include <string>
@@ -21,22 +24,40 @@
 
 using namespace astcg;
 
+// Device tables of one footprint.
+struct FpState {
+    const Footprint* fp = nullptr;
+    void* d_tables = nullptr;
+    DevTables T{};
+    int wstride = 0;  // LDS row of quantised weights: max trial weight count, multiple of 4
+};
+
 struct astcg_encoder {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
+    std::map<int, FpState> fps;  // keyed by bx * 16 + by
+    // working buffers, grown on demand
+    void* d_chunk = nullptr;
+    size_t chunk_cap = 0;
+    ChunkBuffers B{};
+    const FpState* layout_fp = nullptr;  // footprint and chunk size B is laid out for
+    int layout_chunk = 0;
+    SymBest* d_best = nullptr;
+    size_t best_cap = 0;
+    // single-image configuration
     bool configured = false;
     uint32_t width = 0, height = 0, bx = 0, by = 0, blocks_x = 0, blocks_y = 0, nblocks = 0;
     astcg_params params{};
-    const Footprint* fp = nullptr;
-    void* d_tables = nullptr;
-    DevTables T{};
-    void* d_chunk = nullptr;
-    ChunkBuffers B{};
+    const FpState* cur = nullptr;
     int chunk_blocks = 0;
-    SymBest* d_best = nullptr;
-    uint8_t* d_rgba = nullptr;
+    uint8_t* d_rgba = nullptr;  // host-buffer staging
     uint8_t* d_out = nullptr;
+    size_t rgba_cap = 0, out_cap = 0;
+    // batch texture tables
+    TexDesc* d_tex = nullptr;
+    size_t tex_cap = 0;
+    std::vector<TexDesc> h_tex;
     struct EventProfiler* prof = nullptr;
 };
 
@@ -98,18 +119,42 @@ astcg_status hip_fail(astcg_encoder* e, hipError_t err, const char* what) {
                 std::string(what) + ": " + hipGetErrorString(err));
 }
 
-void release_image(astcg_encoder* e) {
-    if (e->d_tables) (void)hipFree(e->d_tables);
+void release_all(astcg_encoder* e) {
+    for (auto& kv : e->fps)
+        if (kv.second.d_tables) (void)hipFree(kv.second.d_tables);
+    e->fps.clear();
     if (e->d_chunk) (void)hipFree(e->d_chunk);
     if (e->d_best) (void)hipFree(e->d_best);
     if (e->d_rgba) (void)hipFree(e->d_rgba);
     if (e->d_out) (void)hipFree(e->d_out);
-    e->d_tables = e->d_chunk = nullptr;
+    if (e->d_tex) (void)hipFree(e->d_tex);
+    e->d_chunk = nullptr;
     e->d_best = nullptr;
     e->d_rgba = e->d_out = nullptr;
+    e->d_tex = nullptr;
+    e->chunk_cap = e->best_cap = e->rgba_cap = e->out_cap = e->tex_cap = 0;
+    e->layout_fp = nullptr;
+    e->cur = nullptr;
     delete e->prof;
     e->prof = nullptr;
     e->configured = false;
+}
+
+// Grows a device buffer to at least `bytes` (contents are not kept).  Work
+// already queued may still use the old buffer, so the device is drained
+// before it is freed.
+hipError_t grow(void** p, size_t* cap, size_t bytes) {
+    if (*cap >= bytes && *p) return hipSuccess;
+    if (*p) {
+        hipError_t err = hipDeviceSynchronize();
+        if (err != hipSuccess) return err;
+        (void)hipFree(*p);
+    }
+    *p = nullptr;
+    *cap = 0;
+    hipError_t err = hipMalloc(p, std::max<size_t>(bytes, 256));
+    if (err == hipSuccess) *cap = std::max<size_t>(bytes, 256);
+    return err;
 }
 
 // Packs all footprint tables into one device allocation.
@@ -124,8 +169,8 @@ struct Layout {
     }
 };
 
-hipError_t upload_tables(astcg_encoder* e) {
-    const Footprint& f = *e->fp;
+hipError_t upload_tables(FpState& F) {
+    const Footprint& f = *F.fp;
     const int tc = f.texel_count;
     // CSR of block modes grouped by decimation mode
     std::vector<uint16_t> off(f.ndm + 1, 0), list;
@@ -196,12 +241,12 @@ hipError_t upload_tables(astcg_encoder* e) {
     put(o_rwc, f.raw_wc.data(), f.raw_wc.size());
     put(o_rq, f.raw_quant.data(), f.raw_quant.size());
     put(o_qwo, qw_off.data(), qw_off.size() * 2);
-    hipError_t err = hipMalloc(&e->d_tables, L.size);
+    hipError_t err = hipMalloc(&F.d_tables, L.size);
     if (err != hipSuccess) return err;
-    err = hipMemcpy(e->d_tables, host.data(), L.size, hipMemcpyHostToDevice);
+    err = hipMemcpy(F.d_tables, host.data(), L.size, hipMemcpyHostToDevice);
     if (err != hipSuccess) return err;
-    uint8_t* b = static_cast<uint8_t*>(e->d_tables);
-    DevTables& T = e->T;
+    uint8_t* b = static_cast<uint8_t*>(F.d_tables);
+    DevTables& T = F.T;
     T.dm = reinterpret_cast<const DmInfo*>(b + o_dm);
     T.bm = reinterpret_cast<const BmInfo*>(b + o_bm);
     T.dm_bm_off = reinterpret_cast<const uint16_t*>(b + o_off);
@@ -232,6 +277,9 @@ hipError_t upload_tables(astcg_encoder* e) {
     T.ndm = f.ndm;
     T.nbm = f.nbm;
     for (int i = 0; i < 4; ++i) T.count_selected[i] = f.count_selected[i];
+    int maxw = 4;
+    for (const DmInfo& d : f.dm) maxw = std::max<int>(maxw, d.weight_count);
+    F.wstride = (maxw + 3) & ~3;
     return hipSuccess;
 }
 
@@ -252,6 +300,97 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[8] = L.add<int>(2 * (size_t)f.nbm + 1);
     offs[9] = L.add<int>(npb * 8 + (size_t)f.nbm * 64);
     return L.size;
+}
+
+// Tables of footprint bx x by, uploaded on first use.
+astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpState** out) {
+    const int key = (int)(bx * 16 + by);
+    auto it = e->fps.find(key);
+    if (it == e->fps.end()) {
+        FpState F;
+        F.fp = footprint((int)bx, (int)by);
+        hipError_t err = upload_tables(F);
+        if (err != hipSuccess) {
+            if (F.d_tables) (void)hipFree(F.d_tables);
+            return hip_fail(e, err, "table upload");
+        }
+        it = e->fps.emplace(key, F).first;
+    }
+    *out = &it->second;
+    return ASTCG_OK;
+}
+
+// Blocks per chunk: keeps the per-chunk working set near 8 GiB (a 4096^2
+// image at 4x4 takes two chunks).
+int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested) {
+    size_t offs[11];
+    const size_t fixed = chunk_bytes_per_block(f, offs, 0);
+    const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) - fixed) >> 16;
+    int chunk = (int)std::max<size_t>(1024, ((size_t(8) << 30) - fixed) / per_block);
+    if (requested) chunk = (int)requested;
+    return std::max(1, std::min<int>(chunk, (int)nblocks));
+}
+
+// Lays the chunk buffers out for footprint F and `chunk` blocks, growing the
+// allocation if needed; the per-mode counters are cleared whenever the
+// layout changes (k_bins clears them after every use otherwise).
+astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStream_t s) {
+    if (e->layout_fp == &F && e->layout_chunk == chunk) return ASTCG_OK;
+    size_t offs[11];
+    const size_t total = chunk_bytes_per_block(*F.fp, offs, chunk);
+    hipError_t err = grow(&e->d_chunk, &e->chunk_cap, total);
+    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
+    uint8_t* b = static_cast<uint8_t*>(e->d_chunk);
+    ChunkBuffers& B = e->B;
+    B.pb_part = reinterpret_cast<int16_t*>(b + offs[0]);
+    B.ideal = reinterpret_cast<Ideal*>(b + offs[1]);
+    B.wts = reinterpret_cast<float*>(b + offs[2]);
+    B.ct_err = reinterpret_cast<float*>(b + offs[3]);
+    B.ct_fmt = reinterpret_cast<uint16_t*>(b + offs[10]);
+    B.ccr = reinterpret_cast<Ccr*>(b + offs[4]);
+    B.qw = b + offs[5];
+    B.top = reinterpret_cast<int16_t*>(b + offs[6]);
+    B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
+    B.bin_count = reinterpret_cast<int*>(b + offs[8]);
+    B.bin_cur = B.bin_count + F.fp->nbm;
+    B.work_total = B.bin_cur + F.fp->nbm;
+    B.work = reinterpret_cast<int*>(b + offs[9]);
+    B.wstride = F.wstride;
+    err = hipMemsetAsync(B.bin_count, 0, sizeof(int) * (2 * (size_t)F.fp->nbm + 1), s);
+    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
+    e->layout_fp = &F;
+    e->layout_chunk = chunk;
+    return ASTCG_OK;
+}
+
+// Runs the whole search + pack over the blocks described by A (A.num_blocks
+// blocks of one footprint), chunk by chunk, on stream s.
+astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
+                        hipStream_t s) {
+    const int chunk = chunk_size(*F.fp, (uint32_t)A.num_blocks, chunk_req);
+    astcg_status st = layout_chunk(e, F, chunk, s);
+    if (st != ASTCG_OK) return st;
+    hipError_t err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)A.num_blocks);
+    if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
+    for (int blk0 = 0; blk0 < A.num_blocks; blk0 += chunk) {
+        A.blk0 = blk0;
+        A.nb = std::min(chunk, A.num_blocks - blk0);
+        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof);
+        if (err != hipSuccess) return hip_fail(e, err, "search kernels");
+    }
+    err = launch_pack(F.T, A, e->d_best, s, e->prof);
+    if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
+    return ASTCG_OK;
+}
+
+astcg_status check_params(astcg_encoder* e, const astcg_params* params, astcg_params* p) {
+    astcg_default_params(p);
+    if (params) *p = *params;
+    if (p->max_partitions < 1 || p->max_partitions > 4)
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "max_partitions must be 1..4");
+    if (p->policy != ASTCG_POLICY_COMPAT) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "unknown partition policy");
+    if ((p->flags & ASTCG_FLAG_PROFILE) && !e->prof) e->prof = new EventProfiler();
+    return ASTCG_OK;
 }
 
 }  // namespace
@@ -294,7 +433,8 @@ astcg_status astcg_create(int hip_device, astcg_encoder** out) {
 astcg_status astcg_destroy(astcg_encoder* e) {
     if (!e) return ASTCG_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(e->device);
-    release_image(e);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    release_all(e);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return ASTCG_OK;
@@ -307,61 +447,33 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "image dimensions must be in 1..2^24-1");
     if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
     astcg_params p;
-    astcg_default_params(&p);
-    if (params) p = *params;
-    if (p.max_partitions < 1 || p.max_partitions > 4) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "max_partitions must be 1..4");
-    if (p.policy != ASTCG_POLICY_COMPAT) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "unknown partition policy");
+    astcg_status st = check_params(e, params, &p);
+    if (st != ASTCG_OK) return st;
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
-    release_image(e);
+    e->configured = false;
+    const uint32_t bxn = (width + bx - 1) / bx, byn = (height + by - 1) / by;
+    const uint64_t nb = (uint64_t)bxn * byn;
+    if (nb > 0x7FFFFFFFull) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "too many blocks");
+    const FpState* F = nullptr;
+    st = get_footprint(e, bx, by, &F);
+    if (st != ASTCG_OK) return st;
     e->width = width;
     e->height = height;
     e->bx = bx;
     e->by = by;
-    e->blocks_x = (width + bx - 1) / bx;
-    e->blocks_y = (height + by - 1) / by;
-    const uint64_t nb = (uint64_t)e->blocks_x * e->blocks_y;
-    if (nb > 0x7FFFFFFFull) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "too many blocks");
+    e->blocks_x = bxn;
+    e->blocks_y = byn;
     e->nblocks = (uint32_t)nb;
     e->params = p;
-    e->fp = footprint((int)bx, (int)by);
-    err = upload_tables(e);
-    if (err != hipSuccess) return hip_fail(e, err, "table upload");
-    // chunk size: keep the per-chunk working set near 8 GiB (a 4096^2 image
-    // at 4x4 takes two chunks)
-    size_t offs[11];
-    const size_t fixed = chunk_bytes_per_block(*e->fp, offs, 0);
-    const size_t per_block = (chunk_bytes_per_block(*e->fp, offs, 1 << 16) - fixed) >> 16;
-    int chunk = (int)std::max<size_t>(1024, ((size_t(8) << 30) - fixed) / per_block);
-    if (p.chunk_blocks) chunk = (int)p.chunk_blocks;
-    chunk = std::min<int>(chunk, (int)e->nblocks);
-    e->chunk_blocks = chunk;
-    const size_t total = chunk_bytes_per_block(*e->fp, offs, chunk);
-    err = hipMalloc(&e->d_chunk, total);
-    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
-    uint8_t* b = static_cast<uint8_t*>(e->d_chunk);
-    e->B.pb_part = reinterpret_cast<int16_t*>(b + offs[0]);
-    e->B.ideal = reinterpret_cast<Ideal*>(b + offs[1]);
-    e->B.wts = reinterpret_cast<float*>(b + offs[2]);
-    e->B.ct_err = reinterpret_cast<float*>(b + offs[3]);
-    e->B.ct_fmt = reinterpret_cast<uint16_t*>(b + offs[10]);
-    e->B.ccr = reinterpret_cast<Ccr*>(b + offs[4]);
-    e->B.qw = b + offs[5];
-    e->B.top = reinterpret_cast<int16_t*>(b + offs[6]);
-    e->B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
-    e->B.bin_count = reinterpret_cast<int*>(b + offs[8]);
-    e->B.bin_cur = e->B.bin_count + e->fp->nbm;
-    e->B.work_total = e->B.bin_cur + e->fp->nbm;
-    e->B.work = reinterpret_cast<int*>(b + offs[9]);
-    // the per-mode counts start at zero; k_bins clears them after each use
-    err = hipMemset(e->B.bin_count, 0, sizeof(int) * (2 * (size_t)e->fp->nbm + 1));
-    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
-    int maxw = 4;
-    for (const DmInfo& d : e->fp->dm) maxw = std::max<int>(maxw, d.weight_count);
-    e->B.wstride = (maxw + 3) & ~3;
-    err = hipMalloc(&e->d_best, sizeof(SymBest) * e->nblocks);
+    e->cur = F;
+    // allocate the working set now so that encode_device allocates nothing
+    e->chunk_blocks = chunk_size(*F->fp, e->nblocks, p.chunk_blocks);
+    st = layout_chunk(e, *F, e->chunk_blocks, e->stream);
+    if (st != ASTCG_OK) return st;
+    err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)e->nblocks);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
-    if (p.flags & ASTCG_FLAG_PROFILE) e->prof = new EventProfiler();
     e->configured = true;
     return ASTCG_OK;
 }
@@ -373,20 +485,16 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
     ImageArgs A{};
-    A.rgba = d_rgba;
-    A.width = (int)e->width;
-    A.height = (int)e->height;
-    A.blocks_x = (int)e->blocks_x;
+    A.t0.rgba = d_rgba;
+    A.t0.out = d_out;
+    A.t0.width = (int)e->width;
+    A.t0.height = (int)e->height;
+    A.t0.blocks_x = (int)e->blocks_x;
+    A.t0.first_block = 0;
+    A.t0.num_blocks = (int)e->nblocks;
+    A.ntex = 1;
     A.num_blocks = (int)e->nblocks;
-    for (int blk0 = 0; blk0 < (int)e->nblocks; blk0 += e->chunk_blocks) {
-        A.blk0 = blk0;
-        A.nb = std::min(e->chunk_blocks, (int)e->nblocks - blk0);
-        err = launch_chunk(e->T, A, e->B, e->d_best, (int)e->params.max_partitions, s, e->prof);
-        if (err != hipSuccess) return hip_fail(e, err, "search kernels");
-    }
-    err = launch_pack(e->T, e->d_best, (int)e->nblocks, d_out, s, e->prof);
-    if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
-    return ASTCG_OK;
+    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, s);
 }
 
 astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len, uint8_t* out, size_t out_len) {
@@ -396,18 +504,163 @@ astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len
     if (rgba_len < need_in || out_len < need_out) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "buffer too small");
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
-    if (!e->d_rgba) {
-        err = hipMalloc(&e->d_rgba, need_in);
-        if (err != hipSuccess) return hip_fail(e, err, "input buffer");
-        err = hipMalloc(&e->d_out, need_out);
-        if (err != hipSuccess) return hip_fail(e, err, "output buffer");
-    }
+    err = grow(reinterpret_cast<void**>(&e->d_rgba), &e->rgba_cap, need_in);
+    if (err == hipSuccess) err = grow(reinterpret_cast<void**>(&e->d_out), &e->out_cap, need_out);
+    if (err != hipSuccess) return hip_fail(e, err, "staging buffers");
     err = hipMemcpyAsync(e->d_rgba, rgba, need_in, hipMemcpyHostToDevice, e->stream);
     if (err != hipSuccess) return hip_fail(e, err, "upload");
     astcg_status st = astcg_encode_device(e, e->d_rgba, e->d_out, e->stream);
     if (st != ASTCG_OK) return st;
     err = hipMemcpyAsync(out, e->d_out, need_out, hipMemcpyDeviceToHost, e->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess) return hip_fail(e, err, "download");
+    return ASTCG_OK;
+}
+
+namespace {
+
+// Checks one texture of a batch; returns its block count in *nb.
+astcg_status check_texture(astcg_encoder* e, const astcg_texture& t, uint32_t i, uint64_t* nb) {
+    char msg[96];
+    if (!t.rgba || !t.out) {
+        std::snprintf(msg, sizeof(msg), "texture %u: null pointer", i);
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, msg);
+    }
+    if (t.width == 0 || t.height == 0 || t.width >= (1u << 24) || t.height >= (1u << 24)) {
+        std::snprintf(msg, sizeof(msg), "texture %u: dimensions must be in 1..2^24-1", i);
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, msg);
+    }
+    if (!valid_footprint((int)t.block_x, (int)t.block_y)) {
+        std::snprintf(msg, sizeof(msg), "texture %u: not a 2D ASTC footprint", i);
+        return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, msg);
+    }
+    *nb = (uint64_t)((t.width + t.block_x - 1) / t.block_x) * ((t.height + t.block_y - 1) / t.block_y);
+    if (t.out_len < *nb * 16) {
+        std::snprintf(msg, sizeof(msg), "texture %u: out_len below %llu", i, (unsigned long long)(*nb * 16));
+        return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, msg);
+    }
+    return ASTCG_OK;
+}
+
+// Encodes textures whose rgba/out are device pointers: one launch chain per
+// footprint, over the textures of that footprint in their batch order.
+astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t count, const astcg_params& p,
+                          hipStream_t s) {
+    std::vector<int> keys;
+    for (uint32_t i = 0; i < count; ++i) {
+        const int key = (int)(tex[i].block_x * 16 + tex[i].block_y);
+        if (std::find(keys.begin(), keys.end(), key) == keys.end()) keys.push_back(key);
+    }
+    // texture tables of all groups, uploaded once
+    e->h_tex.clear();
+    std::vector<size_t> group_start(keys.size() + 1, 0);
+    std::vector<uint64_t> group_blocks(keys.size(), 0);
+    for (size_t gi = 0; gi < keys.size(); ++gi) {
+        const uint32_t bx = (uint32_t)keys[gi] / 16, by = (uint32_t)keys[gi] % 16;
+        group_start[gi] = e->h_tex.size();
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (tex[i].block_x != bx || tex[i].block_y != by) continue;
+            TexDesc d{};
+            d.rgba = tex[i].rgba;
+            d.out = tex[i].out;
+            d.width = (int)tex[i].width;
+            d.height = (int)tex[i].height;
+            d.blocks_x = (int)((tex[i].width + bx - 1) / bx);
+            d.num_blocks = d.blocks_x * (int)((tex[i].height + by - 1) / by);
+            d.first_block = (int)total;
+            total += (uint64_t)d.num_blocks;
+            e->h_tex.push_back(d);
+        }
+        if (total > 0x7FFFFFFFull) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "too many blocks in one footprint group");
+        group_blocks[gi] = total;
+    }
+    group_start[keys.size()] = e->h_tex.size();
+    if (!e->h_tex.empty()) {
+        hipError_t err = grow(reinterpret_cast<void**>(&e->d_tex), &e->tex_cap, sizeof(TexDesc) * e->h_tex.size());
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(e->d_tex, e->h_tex.data(), sizeof(TexDesc) * e->h_tex.size(), hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        if (err != hipSuccess) return hip_fail(e, err, "texture table");
+    }
+    for (size_t gi = 0; gi < keys.size(); ++gi) {
+        const FpState* F = nullptr;
+        astcg_status st = get_footprint(e, (uint32_t)keys[gi] / 16, (uint32_t)keys[gi] % 16, &F);
+        if (st != ASTCG_OK) return st;
+        ImageArgs A{};
+        A.ntex = (int)(group_start[gi + 1] - group_start[gi]);
+        A.num_blocks = (int)group_blocks[gi];
+        A.t0 = e->h_tex[group_start[gi]];
+        A.tex = e->d_tex + group_start[gi];
+        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, s);
+        if (st != ASTCG_OK) return st;
+    }
+    return ASTCG_OK;
+}
+
+}  // namespace
+
+astcg_status astcg_encode_batch_device(astcg_encoder* e, const astcg_texture* textures, uint32_t count,
+                                       const astcg_params* params, void* stream) {
+    if (!e || (!textures && count)) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    astcg_params p;
+    astcg_status st = check_params(e, params, &p);
+    if (st != ASTCG_OK) return st;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint64_t nb = 0;
+        st = check_texture(e, textures[i], i, &nb);
+        if (st != ASTCG_OK) return st;
+    }
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    // the search buffers may be laid out for another footprint afterwards
+    e->layout_fp = nullptr;
+    st = batch_device(e, textures, count, p, static_cast<hipStream_t>(stream));
+    e->layout_fp = nullptr;
+    return st;
+}
+
+astcg_status astcg_encode_batch(astcg_encoder* e, const astcg_texture* textures, uint32_t count,
+                                const astcg_params* params) {
+    if (!e || (!textures && count)) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    astcg_params p;
+    astcg_status st = check_params(e, params, &p);
+    if (st != ASTCG_OK) return st;
+    size_t in_bytes = 0, out_bytes = 0;
+    std::vector<uint64_t> nbs(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        st = check_texture(e, textures[i], i, &nbs[i]);
+        if (st != ASTCG_OK) return st;
+        in_bytes += ((size_t)textures[i].width * textures[i].height * 4 + 255) & ~size_t(255);
+        out_bytes += ((size_t)nbs[i] * 16 + 255) & ~size_t(255);
+    }
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    err = grow(reinterpret_cast<void**>(&e->d_rgba), &e->rgba_cap, in_bytes);
+    if (err == hipSuccess) err = grow(reinterpret_cast<void**>(&e->d_out), &e->out_cap, out_bytes);
+    if (err != hipSuccess) return hip_fail(e, err, "staging buffers");
+    // stage every texture in device memory, then run the device path
+    std::vector<astcg_texture> dev(textures, textures + count);
+    size_t io = 0, oo = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const size_t n = (size_t)textures[i].width * textures[i].height * 4;
+        err = hipMemcpyAsync(e->d_rgba + io, textures[i].rgba, n, hipMemcpyHostToDevice, e->stream);
+        if (err != hipSuccess) return hip_fail(e, err, "upload");
+        dev[i].rgba = e->d_rgba + io;
+        dev[i].out = e->d_out + oo;
+        dev[i].out_len = (size_t)nbs[i] * 16;
+        io += (n + 255) & ~size_t(255);
+        oo += ((size_t)nbs[i] * 16 + 255) & ~size_t(255);
+    }
+    e->layout_fp = nullptr;
+    st = batch_device(e, dev.data(), count, p, e->stream);
+    e->layout_fp = nullptr;
+    if (st != ASTCG_OK) return st;
+    for (uint32_t i = 0; i < count; ++i) {
+        err = hipMemcpyAsync(textures[i].out, dev[i].out, (size_t)nbs[i] * 16, hipMemcpyDeviceToHost, e->stream);
+        if (err != hipSuccess) return hip_fail(e, err, "download");
+    }
+    err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return hip_fail(e, err, "download");
     return ASTCG_OK;
 }

@@ -38,13 +38,26 @@ struct DevTables {
     int count_selected[4];
 };
 
-// Image and chunk geometry.
-struct ImageArgs {
+// One texture of a launch: its blocks occupy [first_block, first_block +
+// num_blocks) of the launch's block index space; `out` receives its 16-byte
+// blocks in row-major block order.
+struct TexDesc {
     const uint8_t* rgba;
-    int width, height;
-    int blocks_x, num_blocks;
-    int blk0;  // first block of the chunk
-    int nb;    // blocks in the chunk
+    uint8_t* out;
+    int width, height, blocks_x;
+    int first_block, num_blocks;
+    int pad;
+};
+
+// Image and chunk geometry.  A single image is one texture held by value;
+// a batch keeps its texture table (sorted by first_block) in HBM.
+struct ImageArgs {
+    TexDesc t0;               // the texture when ntex == 1
+    const TexDesc* tex;       // [ntex] when ntex > 1
+    int ntex;
+    int num_blocks;           // blocks of all textures
+    int blk0;                 // first block of the chunk
+    int nb;                   // blocks in the chunk
 };
 
 // pass01 output (ideal endpoints), per partitioned block; texel weights are
@@ -125,7 +138,7 @@ struct Profiler {
 void upload_quant_tables(hipStream_t s);
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof);
-hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s,
+hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s,
                        Profiler* prof);
 
 }  // namespace astcg

@@ -98,8 +98,58 @@ __device__ float atan2f_fixed(float y, float x) {
 }
 
 // ---- block access ----------------------------------------------------------
+__device__ __forceinline__ TexDesc tex_of(const ImageArgs& A, int g) {
+    if (A.ntex == 1) return A.t0;
+    int lo = 0, hi = A.ntex - 1;  // last texture starting at or before block g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.tex[mid].first_block <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return A.tex[lo];
+}
+
+// Texel origin of one block of a texture (edge blocks clamp to the last
+// row/column, compression_pipeline.cpp:33,38).
+struct BlockLoc {
+    const uint8_t* rgba;
+    int width, height;
+    int x0, y0;
+    int local;  // block index within its texture
+    int tex_blocks;
+};
+
+__device__ __forceinline__ BlockLoc block_loc(const ImageArgs& A, const DevTables& T, int g) {
+    const TexDesc t = tex_of(A, g);
+    BlockLoc l;
+    l.rgba = t.rgba;
+    l.width = t.width;
+    l.height = t.height;
+    l.local = g - t.first_block;
+    l.tex_blocks = t.num_blocks;
+    const int byi = l.local / t.blocks_x, bxi = l.local - byi * t.blocks_x;
+    l.x0 = bxi * T.bx;
+    l.y0 = byi * T.by;
+    return l;
+}
+
+__device__ __forceinline__ uint32_t texel_rgba8(const BlockLoc& l, const DevTables& T, int i) {
+    const int dy = i / T.bx, dx = i - dy * T.bx;
+    const int x = min(l.x0 + dx, l.width - 1), y = min(l.y0 + dy, l.height - 1);
+    return *reinterpret_cast<const uint32_t*>(l.rgba + ((size_t)y * l.width + x) * 4);
+}
+
+__device__ __forceinline__ void texel_px(const BlockLoc& l, const DevTables& T, int i, float* o) {
+    const uint32_t v = texel_rgba8(l, T, i);
+    o[0] = c_qt.pixel_scale[v & 0xFF];
+    o[1] = c_qt.pixel_scale[(v >> 8) & 0xFF];
+    o[2] = c_qt.pixel_scale[(v >> 16) & 0xFF];
+    o[3] = c_qt.pixel_scale[v >> 24];
+}
+
 struct Pb {
-    int g;          // image block index
+    int g;          // block index of the launch
+    BlockLoc loc;
     int pc;         // partition count
     int entry;      // packed partition table entry (pc > 1)
     const uint8_t* tp;  // texel partitions (nullptr: all 0)
@@ -108,17 +158,6 @@ struct Pb {
     int pb_in_batch;
 };
 
-__device__ __forceinline__ void texel_px(const ImageArgs& A, const DevTables& T, int g, int i, float* o) {
-    const int bxi = g % A.blocks_x, byi = g / A.blocks_x;
-    const int dy = i / T.bx, dx = i - dy * T.bx;
-    const int x = min(bxi * T.bx + dx, A.width - 1), y = min(byi * T.by + dy, A.height - 1);
-    const uint32_t v = *reinterpret_cast<const uint32_t*>(A.rgba + ((size_t)y * A.width + x) * 4);
-    o[0] = c_qt.pixel_scale[v & 0xFF];
-    o[1] = c_qt.pixel_scale[(v >> 8) & 0xFF];
-    o[2] = c_qt.pixel_scale[(v >> 16) & 0xFF];
-    o[3] = c_qt.pixel_scale[v >> 24];
-}
-
 __device__ __forceinline__ int texel_part(const Pb& b, int i) { return b.tp ? b.tp[i] : 0; }
 
 __device__ Pb make_pb(const ImageArgs& A, const DevTables& T, const int16_t* pb_part, int pc, int pb) {
@@ -126,8 +165,10 @@ __device__ Pb make_pb(const ImageArgs& A, const DevTables& T, const int16_t* pb_
     const int npos = pc == 1 ? 1 : 4;
     const int k = pb / npos, c = pb - k * npos;
     b.g = A.blk0 + k;
+    b.loc = block_loc(A, T, b.g);
     b.pc = pc;
-    const int j = b.g % kRefBatch;
+    // the reference encodes each texture in batches of 920 blocks
+    const int j = b.loc.local % kRefBatch;
     b.pb_in_batch = pc == 1 ? j : j * 4 + c;
     if (pc == 1) {
         b.entry = 0;
@@ -461,21 +502,22 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int 
     const int tcp = (tc + 3) & ~3;
     const int k = blockIdx.x * kPartBlocks + grp;
     const int g = A.blk0 + k;
-    const int j = g % kRefBatch, base = g - j;
-    const int B = min(kRefBatch, A.num_blocks - base);
+    const BlockLoc loc = block_loc(A, T, k < A.nb ? g : A.blk0);
+    const int j = loc.local % kRefBatch, base = loc.local - j;  // within the texture
+    const int B = min(kRefBatch, loc.tex_blocks - base);
     // partition_ordering and the partitioning errors are allocated 920*128
     // entries but indexed with stride 512, so in-batch blocks >= 230 read
     // zeros and keep table positions 0..3.
     const bool work = k < A.nb && j * 512 + 127 < kRefBatch * 128;
     // pass004 writes slot block*1024*i: the score this block reads is
     // mismatch(j, partitioning 1), or mismatch(B-1, partitioning 0) for j == 0
-    const int src = j == 0 ? base + B - 1 : g;
+    const int src = j == 0 ? g + B - 1 : g;
     if (work) {
         for (int t = sub; t < tc; t += kPartLanes) {
             float px[4];
-            texel_px(A, T, g, t, px);
+            texel_px(loc, T, t, px);
             s_px[grp * stride + t] = make_float4(px[0], px[1], px[2], px[3]);
-            if (src != g) texel_px(A, T, src, t, px);
+            if (src != g) texel_px(block_loc(A, T, src), T, t, px);
             s_kpx[grp * stride + t] = make_float4(px[0], px[1], px[2], px[3]);
         }
     }
@@ -551,7 +593,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
     float amin = 1e38f, amax = 0.0f;
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -570,7 +612,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
     float dv[4][4] = {};
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -582,7 +624,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
     for (int p = 0; p < 4; ++p) { normalize4(dv[p], dir[p]); minp[p] = 1e10f; maxp[p] = -1e10f; }
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -610,7 +652,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
     }
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         const int p = texel_part(b, i);
         float w = 0.0f;
 #pragma unroll
@@ -639,7 +681,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
     float sxp[4][4] = {}, syp[4][4] = {}, szp[4][4] = {};
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -676,7 +718,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
     float acc[4][5] = {};
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -880,7 +922,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     {
         const int npos = pc == 1 ? 1 : 4;
         const int kb = pb / npos, c = pb - kb * npos;
-        const int j = (A.blk0 + kb) % kRefBatch;
+        const int j = block_loc(A, T, A.blk0 + kb).local % kRefBatch;
         pb_in_batch = pc == 1 ? j : j * 4 + c;
     }
     // pass02_decimated_weights.wgsl:132-270
@@ -1616,13 +1658,8 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
     uint8_t* s_bqw = s_qw + (size_t)wstride * 64;
     const int pb = task >> 3;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
-    const int bxi = b.g % A.blocks_x, byi = b.g / A.blocks_x;
     for (int i = 0; i < tc; ++i) {
-        if (kStage) {
-            const int dy = i / T.bx, dx = i - dy * T.bx;
-            const int x = min(bxi * T.bx + dx, A.width - 1), y = min(byi * T.by + dy, A.height - 1);
-            s_tex[i * 64 + lane] = *reinterpret_cast<const uint32_t*>(A.rgba + ((size_t)y * A.width + x) * 4);
-        }
+        if (kStage) s_tex[i * 64 + lane] = texel_rgba8(b.loc, T, i);
         s_tp[i * 64 + lane] = (uint8_t)texel_part(b, i);
     }
     auto px_of = [&](int i) -> float4 {
@@ -1632,7 +1669,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
         }
         float px[4];
-        texel_px(A, T, b.g, i, px);
+        texel_px(b.loc, T, i, px);
         return make_float4(px[0], px[1], px[2], px[3]);
     };
     const uint8_t* btp = s_tp + lane;  // btp[i * 64]
@@ -2011,11 +2048,12 @@ __device__ void encode_ise(int q, int count, const uint8_t* in, uint32_t* w, int
 
 __device__ __forceinline__ uint32_t bitrev32(uint32_t v) { return __brev(v); }
 
-__global__ void __launch_bounds__(64) k_pack(DevTables T, const SymBest* best, int nblocks, uint8_t* out) {
+__global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const SymBest* best) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nblocks) return;
+    if (k >= A.num_blocks) return;
     const SymBest& sb = best[k];
-    uint4* o = reinterpret_cast<uint4*>(out + (size_t)k * 16);
+    const TexDesc t = tex_of(A, k);
+    uint4* o = reinterpret_cast<uint4*>(t.out + (size_t)(k - t.first_block) * 16);
     const int raw = sb.raw_mode & 0x7FF;
     const int wc = T.raw_wc[raw], q = T.raw_quant[raw];
     const int pc = sb.pc;
@@ -2173,10 +2211,9 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const DevTables& T, const SymBest* best, int nblocks, uint8_t* out, hipStream_t s,
-                       Profiler* prof) {
+hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof) {
     const int threads = 64;
-    ASTCG_LAUNCH(K_PACK, k_pack, (nblocks + threads - 1) / threads, T, best, nblocks, out);
+    ASTCG_LAUNCH(K_PACK, k_pack, (A.num_blocks + threads - 1) / threads, T, A, best);
     return hipGetLastError();
 }
 

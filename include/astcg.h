@@ -97,6 +97,32 @@ astcg_status astcg_encode(astcg_encoder* enc, const uint8_t* rgba, size_t rgba_l
  * keeps its scratch buffers until the next configure/destroy. */
 astcg_status astcg_encode_device(astcg_encoder* enc, const uint8_t* d_rgba, uint8_t* d_out, void* stream);
 
+/* One texture of a batch.  `rgba` is row-major RGBA8 (width*height*4 bytes),
+ * `out` receives ceil(width/block_x)*ceil(height/block_y) 16-byte blocks in
+ * row-major block order; out_len is checked against that size. */
+typedef struct astcg_texture {
+    const uint8_t* rgba;
+    uint32_t width, height;
+    uint32_t block_x, block_y;
+    uint8_t* out;
+    size_t out_len;
+} astcg_texture;
+
+/* Batched encode of `count` textures of any sizes and footprints.  Each
+ * texture's blocks equal what configure(width, height, block_x, block_y) +
+ * encode() produce for it alone, i.e. what the reference's
+ * ASTCEncoder::secondaryInit + encode (code/compression_pipeline.cpp:263-535)
+ * produce per image; textures of one footprint share each kernel launch.
+ * Host buffers, synchronous.  `params` may be NULL (defaults); its flags may
+ * enable kernel timing.  Does not change the single-image configuration. */
+astcg_status astcg_encode_batch(astcg_encoder* enc, const astcg_texture* textures, uint32_t count,
+                                const astcg_params* params);
+
+/* Same with `rgba` / `out` in device memory of the encoder's device; work is
+ * enqueued on `stream` and the call returns without synchronising. */
+astcg_status astcg_encode_batch_device(astcg_encoder* enc, const astcg_texture* textures, uint32_t count,
+                                       const astcg_params* params, void* stream);
+
 /* Optional: copy the chosen symbolic blocks of the last encode (host memory,
  * blocks_x*blocks_y entries). */
 astcg_status astcg_get_symbolic(astcg_encoder* enc, astcg_symbolic_block* out, size_t count);
