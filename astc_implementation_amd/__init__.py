@@ -106,6 +106,8 @@ def library():
         "astcg_encode": (i32, [vp, vp, sz, vp, sz]),
         "astcg_encode_device": (i32, [vp, vp, vp, vp]),
         "astcg_encode_batch": (i32, [vp, vp, u32, vp]),
+        "astcg_decode_device": (i32, [vp, u32, u32, u32, u32, vp, vp, vp]),
+        "astcg_decode": (i32, [vp, u32, u32, u32, u32, vp, sz, vp, sz]),
         "astcg_encode_batch_device": (i32, [vp, vp, u32, vp, vp]),
         "astcg_get_symbolic": (i32, [vp, vp, sz]),
         "astcg_block_counts": (i32, [vp, vp, vp, vp]),
@@ -220,6 +222,18 @@ class Encoder:
         p.max_partitions = max_partitions
         _check(library().astcg_encode_batch_device(self._h, tex, len(textures), ctypes.byref(p),
                                                    ctypes.c_void_p(stream or 0)), self._h)
+
+    def decode(self, bx, by, width, height, blocks):
+        """GPU ASTC LDR decode of host blocks -> (height, width, 4) uint8."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint8)
+        out = np.zeros((height, width, 4), np.uint8)
+        _check(library().astcg_decode(self._h, bx, by, width, height, _ptr(blocks), blocks.nbytes, _ptr(out),
+                                      out.nbytes), self._h)
+        return out
+
+    def decode_device(self, bx, by, width, height, d_blocks, d_rgba, stream=None):
+        _check(library().astcg_decode_device(self._h, bx, by, width, height, ctypes.c_void_p(d_blocks),
+                                             ctypes.c_void_p(d_rgba), ctypes.c_void_p(stream or 0)), self._h)
 
     def kernel_times(self):
         """{kernel: {"total_ms", "launches"}} since the last reset (needs profile=True)."""

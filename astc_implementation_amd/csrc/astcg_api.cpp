@@ -216,6 +216,7 @@ hipError_t upload_tables(FpState& F) {
     const size_t o_rwc = L.add<uint8_t>(f.raw_wc.size());
     const size_t o_rq = L.add<uint8_t>(f.raw_quant.size());
     const size_t o_qwo = L.add<uint16_t>(qw_off.size());
+    const size_t o_s2p = L.add<uint16_t>(f.seed_to_packed.size());
     std::vector<uint8_t> host(L.size, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
     put(o_dm, f.dm.data(), f.dm.size() * sizeof(DmInfo));
@@ -241,6 +242,7 @@ hipError_t upload_tables(FpState& F) {
     put(o_rwc, f.raw_wc.data(), f.raw_wc.size());
     put(o_rq, f.raw_quant.data(), f.raw_quant.size());
     put(o_qwo, qw_off.data(), qw_off.size() * 2);
+    put(o_s2p, f.seed_to_packed.data(), f.seed_to_packed.size() * 2);
     hipError_t err = hipMalloc(&F.d_tables, L.size);
     if (err != hipSuccess) return err;
     err = hipMemcpy(F.d_tables, host.data(), L.size, hipMemcpyHostToDevice);
@@ -270,6 +272,7 @@ hipError_t upload_tables(FpState& F) {
     T.raw_wc = b + o_rwc;
     T.raw_quant = b + o_rq;
     T.qw_off = reinterpret_cast<const uint16_t*>(b + o_qwo);
+    T.seed_to_packed = reinterpret_cast<const uint16_t*>(b + o_s2p);
     T.qw_stride = (qw_total + 15) & ~15;
     T.bx = f.bx;
     T.by = f.by;
@@ -763,6 +766,42 @@ astcg_status astcg_decode_image(uint32_t bx, uint32_t by, uint32_t w, uint32_t h
                 }
         }
     return worst;
+}
+
+astcg_status astcg_decode_device(astcg_encoder* e, uint32_t bx, uint32_t by, uint32_t w, uint32_t h,
+                                 const uint8_t* d_blocks, uint8_t* d_rgba, void* stream) {
+    if (!e || !d_blocks || !d_rgba || w == 0 || h == 0 || w >= (1u << 24) || h >= (1u << 24))
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "bad decode arguments");
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    const uint64_t nb = (uint64_t)((w + bx - 1) / bx) * ((h + by - 1) / by);
+    if (nb > 0x7FFFFFFFull) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "too many blocks");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    const FpState* F = nullptr;
+    astcg_status st = get_footprint(e, bx, by, &F);
+    if (st != ASTCG_OK) return st;
+    err = launch_decode(F->T, d_blocks, (int)w, (int)h, d_rgba, static_cast<hipStream_t>(stream));
+    return err == hipSuccess ? ASTCG_OK : hip_fail(e, err, "decode kernel");
+}
+
+astcg_status astcg_decode(astcg_encoder* e, uint32_t bx, uint32_t by, uint32_t w, uint32_t h, const uint8_t* blocks,
+                          size_t blocks_len, uint8_t* rgba, size_t rgba_len) {
+    if (!e || !blocks || !rgba || w == 0 || h == 0) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "bad decode arguments");
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    const size_t nb = (size_t)((w + bx - 1) / bx) * ((h + by - 1) / by);
+    if (blocks_len < nb * 16 || rgba_len < (size_t)w * h * 4) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "buffer too small");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    // the staging buffers double as decode input (blocks) and output (texels)
+    err = grow(reinterpret_cast<void**>(&e->d_out), &e->out_cap, nb * 16);
+    if (err == hipSuccess) err = grow(reinterpret_cast<void**>(&e->d_rgba), &e->rgba_cap, (size_t)w * h * 4);
+    if (err == hipSuccess) err = hipMemcpyAsync(e->d_out, blocks, nb * 16, hipMemcpyHostToDevice, e->stream);
+    if (err != hipSuccess) return hip_fail(e, err, "decode staging");
+    astcg_status st = astcg_decode_device(e, bx, by, w, h, e->d_out, e->d_rgba, e->stream);
+    if (st != ASTCG_OK) return st;
+    err = hipMemcpyAsync(rgba, e->d_rgba, (size_t)w * h * 4, hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    return err == hipSuccess ? ASTCG_OK : hip_fail(e, err, "decode download");
 }
 
 astcg_status astcg_physical_to_symbolic(uint32_t bx, uint32_t by, const uint8_t in[16], astcg_symbolic_block* out) {

@@ -99,16 +99,10 @@ void decode_ise_host(int quant, int count, const uint8_t* in, int bit_offset, ui
                 T |= (int)read_bits(in, off, tb[k]) << ts[k];
                 off += tb[k];
             }
-            // inverse trit table lookup: the encoder table is a bijection onto used codes
-            const QuantTables& qt = quant_tables();
-            int digits[5] = {0, 0, 0, 0, 0};
-            for (int idx = 0; idx < 243; ++idx)
-                if (qt.int_of_trits[idx] == T) {
-                    int v = idx;
-                    for (int k = 0; k < 5; ++k) { digits[k] = v % 3; v /= 3; }
-                    break;
-                }
-            for (int k = 0; k < 5 && i + k < count; ++k) out[i + k] = (uint8_t)((digits[k] << bits) | m[k]);
+            // spec trit decode of the 8-bit block (all 256 codes, including
+            // the ones an encoder never emits)
+            const int dg = quant_tables().trit_digits[T];
+            for (int k = 0; k < 5 && i + k < count; ++k) out[i + k] = (uint8_t)((((dg >> (2 * k)) & 3) << bits) | m[k]);
         }
     } else if (quints) {
         static const int qb[3] = {3, 2, 2}, qs[3] = {0, 3, 5};
@@ -120,15 +114,8 @@ void decode_ise_host(int quant, int count, const uint8_t* in, int bit_offset, ui
                 Q |= (int)read_bits(in, off, qb[k]) << qs[k];
                 off += qb[k];
             }
-            const QuantTables& qt = quant_tables();
-            int digits[3] = {0, 0, 0};
-            for (int idx = 0; idx < 125; ++idx)
-                if (qt.int_of_quints[idx] == Q) {
-                    int v = idx;
-                    for (int k = 0; k < 3; ++k) { digits[k] = v % 5; v /= 5; }
-                    break;
-                }
-            for (int k = 0; k < 3 && i + k < count; ++k) out[i + k] = (uint8_t)((digits[k] << bits) | m[k]);
+            const int dg = quant_tables().quint_digits[Q];
+            for (int k = 0; k < 3 && i + k < count; ++k) out[i + k] = (uint8_t)((((dg >> (3 * k)) & 7) << bits) | m[k]);
         }
     } else {
         for (int i = 0; i < count; ++i) {

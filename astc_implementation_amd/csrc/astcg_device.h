@@ -32,6 +32,7 @@ struct DevTables {
     const int16_t* raw_to_bm;     // [2048]
     const uint8_t* raw_wc;        // [2048]
     const uint8_t* raw_quant;     // [2048]
+    const uint16_t* seed_to_packed; // [3][1024] partition seed -> packed table entry
     const uint16_t* qw_off;       // [nbm] offset of a trial mode's weights in a block's packed weight row
     int qw_stride;                // bytes of packed quantised weights per partitioned block
     int bx, by, tc, ndm, nbm;
@@ -138,6 +139,9 @@ struct Profiler {
 void upload_quant_tables(hipStream_t s);
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof);
+// ASTC LDR decode of a width x height image's blocks (row-major) to RGBA8.
+hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
+                         hipStream_t s);
 hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s,
                        Profiler* prof);
 

@@ -2130,6 +2130,293 @@ __global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const Sym
     *o = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// ---- ASTC LDR decoder (Khronos ASTC specification, 2D LDR profile) ---------
+// Independent of the search: one lane per block, writing the decoded RGBA8
+// texels that fall inside the image.  Error blocks and HDR endpoint modes
+// decode to the error colour (magenta), LDR void-extent blocks to their
+// constant colour.
+__device__ __forceinline__ uint32_t rd_bits(const uint32_t* w, int off, int n) {
+    if (n <= 0) return 0u;
+    const int wi = off >> 5, sh = off & 31;
+    uint64_t v = w[wi];
+    if (wi < 3) v |= (uint64_t)w[wi + 1] << 32;
+    return (uint32_t)(v >> sh) & (uint32_t)((1ull << n) - 1ull);
+}
+
+__device__ __forceinline__ void ise_shape(int q, int* bits, int* trits, int* quints) {
+    const int n = kLevelsD[q];
+    *trits = n % 3 == 0;
+    *quints = n % 5 == 0;
+    const int base = *trits ? n / 3 : (*quints ? n / 5 : n);
+    *bits = 31 - __clz(base);
+}
+
+__device__ int ise_bits(int count, int q) {
+    int b, t, u;
+    ise_shape(q, &b, &t, &u);
+    return b * count + (t ? (8 * count + 4) / 5 : 0) + (u ? (7 * count + 2) / 3 : 0);
+}
+
+__device__ void ise_decode(int q, int count, const uint32_t* w, int off, uint8_t* out) {
+    int b, t, u;
+    ise_shape(q, &b, &t, &u);
+    if (t) {
+        const int tb[5] = {2, 2, 1, 2, 1}, ts[5] = {0, 2, 4, 5, 7};
+        for (int i = 0; i < count; i += 5) {
+            int m[5] = {0, 0, 0, 0, 0}, T = 0;
+            for (int k = 0; k < 5 && i + k < count; ++k) {
+                m[k] = (int)rd_bits(w, off, b);
+                off += b;
+                T |= (int)rd_bits(w, off, tb[k]) << ts[k];
+                off += tb[k];
+            }
+            const int dg = c_qt.trit_digits[T];
+            for (int k = 0; k < 5 && i + k < count; ++k) out[i + k] = (uint8_t)((((dg >> (2 * k)) & 3) << b) | m[k]);
+        }
+    } else if (u) {
+        const int qb[3] = {3, 2, 2}, qs[3] = {0, 3, 5};
+        for (int i = 0; i < count; i += 3) {
+            int m[3] = {0, 0, 0}, Q = 0;
+            for (int k = 0; k < 3 && i + k < count; ++k) {
+                m[k] = (int)rd_bits(w, off, b);
+                off += b;
+                Q |= (int)rd_bits(w, off, qb[k]) << qs[k];
+                off += qb[k];
+            }
+            const int dg = c_qt.quint_digits[Q];
+            for (int k = 0; k < 3 && i + k < count; ++k) out[i + k] = (uint8_t)((((dg >> (3 * k)) & 7) << b) | m[k]);
+        }
+    } else {
+        for (int i = 0; i < count; ++i) {
+            out[i] = (uint8_t)rd_bits(w, off, b);
+            off += b;
+        }
+    }
+}
+
+// Block-mode field decode (ASTC spec, 2D): weight grid, dual plane, weight quant.
+__device__ bool dec_mode_fields(int mode, int* xw, int* yw, int* dual, int* quant) {
+    int base_q = (mode >> 4) & 1;
+    int H = (mode >> 9) & 1, D = (mode >> 10) & 1;
+    const int A = (mode >> 5) & 3;
+    if (mode & 3) {
+        base_q |= (mode & 3) << 1;
+        int B = (mode >> 7) & 3;
+        switch ((mode >> 2) & 3) {
+            case 0: *xw = B + 4; *yw = A + 2; break;
+            case 1: *xw = B + 8; *yw = A + 2; break;
+            case 2: *xw = A + 2; *yw = B + 8; break;
+            default:
+                B &= 1;
+                if (mode & 0x100) { *xw = B + 2; *yw = A + 2; }
+                else { *xw = A + 2; *yw = B + 6; }
+                break;
+        }
+    } else {
+        base_q |= ((mode >> 2) & 3) << 1;
+        if (((mode >> 2) & 3) == 0) return false;
+        const int B = (mode >> 9) & 3;
+        switch ((mode >> 7) & 3) {
+            case 0: *xw = 12; *yw = A + 2; break;
+            case 1: *xw = A + 2; *yw = 12; break;
+            case 2: *xw = A + 6; *yw = B + 6; D = 0; H = 0; break;
+            default:
+                if (((mode >> 5) & 3) == 0) { *xw = 6; *yw = 10; }
+                else if (((mode >> 5) & 3) == 1) { *xw = 10; *yw = 6; }
+                else return false;
+                break;
+        }
+    }
+    *quant = (base_q - 2) + 6 * H;
+    *dual = D;
+    return true;
+}
+
+__device__ __forceinline__ void bit_transfer_signed(int& a, int& b) {
+    b >>= 1;
+    b |= a & 0x80;
+    a >>= 1;
+    a &= 0x3F;
+    if (a & 0x20) a -= 0x40;
+}
+__device__ __forceinline__ int clamp255(int x) { return x < 0 ? 0 : (x > 255 ? 255 : x); }
+
+// LDR colour endpoint decode (endpoint modes 0, 1, 4, 5, 6, 8, 9, 10, 12, 13).
+__device__ bool dec_endpoints(int cem, const int* v, int* e0, int* e1) {
+    switch (cem) {
+        case 0:
+            e0[0] = e0[1] = e0[2] = v[0]; e0[3] = 255;
+            e1[0] = e1[1] = e1[2] = v[1]; e1[3] = 255;
+            return true;
+        case 1: {
+            const int l0 = (v[0] >> 2) | (v[1] & 0xC0);
+            const int l1 = min(l0 + (v[1] & 0x3F), 255);
+            e0[0] = e0[1] = e0[2] = l0; e0[3] = 255;
+            e1[0] = e1[1] = e1[2] = l1; e1[3] = 255;
+            return true;
+        }
+        case 4:
+            e0[0] = e0[1] = e0[2] = v[0]; e0[3] = v[2];
+            e1[0] = e1[1] = e1[2] = v[1]; e1[3] = v[3];
+            return true;
+        case 5: {
+            int v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+            bit_transfer_signed(v1, v0);
+            bit_transfer_signed(v3, v2);
+            e0[0] = e0[1] = e0[2] = v0; e0[3] = v2;
+            e1[0] = e1[1] = e1[2] = clamp255(v0 + v1); e1[3] = clamp255(v2 + v3);
+            return true;
+        }
+        case 6:
+        case 10:
+            for (int c = 0; c < 3; ++c) { e1[c] = v[c]; e0[c] = (v[c] * v[3]) >> 8; }
+            e0[3] = cem == 10 ? v[4] : 255;
+            e1[3] = cem == 10 ? v[5] : 255;
+            return true;
+        case 8:
+        case 12: {
+            int c0[4] = {v[0], v[2], v[4], cem == 12 ? v[6] : 255};
+            int c1[4] = {v[1], v[3], v[5], cem == 12 ? v[7] : 255};
+            if (c1[0] + c1[1] + c1[2] < c0[0] + c0[1] + c0[2]) {  // blue contraction
+                for (int c = 0; c < 2; ++c) { c0[c] = (c0[c] + c0[2]) >> 1; c1[c] = (c1[c] + c1[2]) >> 1; }
+                for (int c = 0; c < 4; ++c) { e0[c] = c1[c]; e1[c] = c0[c]; }
+            } else {
+                for (int c = 0; c < 4; ++c) { e0[c] = c0[c]; e1[c] = c1[c]; }
+            }
+            return true;
+        }
+        case 9:
+        case 13: {
+            int a[4] = {v[0], v[2], v[4], cem == 13 ? v[6] : 255};
+            int b[4] = {v[1], v[3], v[5], cem == 13 ? v[7] : 0};
+            for (int c = 0; c < (cem == 13 ? 4 : 3); ++c) bit_transfer_signed(b[c], a[c]);
+            int c0[4], c1[4];
+            for (int c = 0; c < 4; ++c) { c0[c] = a[c]; c1[c] = a[c] + b[c]; }
+            if (cem == 9) { c0[3] = 255; c1[3] = 255; }
+            if (b[0] + b[1] + b[2] < 0) {
+                for (int c = 0; c < 2; ++c) { c0[c] = (c0[c] + c0[2]) >> 1; c1[c] = (c1[c] + c1[2]) >> 1; }
+                for (int c = 0; c < 4; ++c) { e0[c] = clamp255(c1[c]); e1[c] = clamp255(c0[c]); }
+            } else {
+                for (int c = 0; c < 4; ++c) { e0[c] = clamp255(c0[c]); e1[c] = clamp255(c1[c]); }
+            }
+            return true;
+        }
+        default:
+            return false;  // HDR endpoint modes
+    }
+}
+
+__global__ void __launch_bounds__(64) k_decode(DevTables T, const uint8_t* blocks, int width, int height, int blocks_x,
+                                               int nblocks, uint8_t* rgba) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblocks) return;
+    const int bx = T.bx, by = T.by, tc = T.tc;
+    const int x0 = (k % blocks_x) * bx, y0 = (k / blocks_x) * by;
+    const uint4 raw = reinterpret_cast<const uint4*>(blocks)[k];
+    const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+    auto put = [&](int t, uint32_t px) {
+        const int x = x0 + t % bx, y = y0 + t / bx;
+        if (x < width && y < height) *reinterpret_cast<uint32_t*>(rgba + ((size_t)y * width + x) * 4) = px;
+    };
+    const uint32_t magenta = 0xFFFF00FFu;
+    auto fill = [&](uint32_t px) { for (int t = 0; t < tc; ++t) put(t, px); };
+    const int mode = (int)rd_bits(w, 0, 11);
+    if ((mode & 0x1FF) == 0x1FC) {
+        if (mode & 0x200) { fill(magenta); return; }  // HDR void extent
+        uint32_t px = 0;
+        for (int c = 0; c < 4; ++c) px |= (uint32_t)((rd_bits(w, 64 + 16 * c, 16) + 128) / 257) << (8 * c);
+        fill(px);
+        return;
+    }
+    int xw = 0, yw = 0, dual = 0, q = 0;
+    if (!dec_mode_fields(mode, &xw, &yw, &dual, &q) || xw > bx || yw > by) { fill(magenta); return; }
+    const int wc = xw * yw * (dual + 1);
+    const int wbits = ise_bits(wc, q);
+    if (wc > 64 || wbits < 24 || wbits > 96) { fill(magenta); return; }
+    const int pc = (int)rd_bits(w, 11, 2) + 1;
+    if (dual && pc == 4) { fill(magenta); return; }
+    int below = 128 - wbits, color_start, seed = 0;
+    int cem[4] = {0, 0, 0, 0};
+    if (pc == 1) {
+        cem[0] = (int)rd_bits(w, 13, 4);
+        color_start = 17;
+    } else {
+        seed = (int)rd_bits(w, 13, 10);
+        color_start = 29;
+        const int cemlow = (int)rd_bits(w, 23, 6);
+        if ((cemlow & 3) == 0) {
+            for (int i = 0; i < pc; ++i) cem[i] = cemlow >> 2;
+        } else {
+            const int hsize = 3 * pc - 4;
+            below -= hsize;
+            const int enc = cemlow | ((int)rd_bits(w, below, hsize) << 6);
+            const int base_class = (enc & 3) - 1;
+            int bitpos = 2, cls[4];
+            for (int i = 0; i < pc; ++i) cls[i] = base_class + ((enc >> bitpos++) & 1);
+            for (int i = 0; i < pc; ++i) { cem[i] = (cls[i] << 2) | ((enc >> bitpos) & 3); bitpos += 2; }
+        }
+    }
+    int ccs = 0;
+    if (dual) { below -= 2; ccs = (int)rd_bits(w, below, 2); }
+    int nvals = 0;
+    for (int i = 0; i < pc; ++i) nvals += 2 * ((cem[i] >> 2) + 1);
+    if (nvals > 18) { fill(magenta); return; }
+    const int avail = below - color_start;
+    int cq = -1;
+    for (int qq = 4; qq <= 20; ++qq)
+        if (ise_bits(nvals, qq) <= avail) cq = qq;
+    if (cq < 0) { fill(magenta); return; }
+    uint8_t codes[18];
+    ise_decode(cq, nvals, w, color_start, codes);
+    int vals[18];
+    for (int i = 0; i < nvals; ++i) vals[i] = c_qt.color_unq[cq - 4][codes[i]];
+    int e0[4][4], e1[4][4];
+    int vi = 0;
+    for (int p = 0; p < pc; ++p) {
+        if (!dec_endpoints(cem[p], vals + vi, e0[p], e1[p])) { fill(magenta); return; }
+        vi += 2 * ((cem[p] >> 2) + 1);
+    }
+    // weights are stored bit-reversed from the top of the block
+    const uint32_t rw[4] = {__brev(w[3]), __brev(w[2]), __brev(w[1]), __brev(w[0])};
+    uint8_t wcodes[64];
+    ise_decode(q, wc, rw, 0, wcodes);
+    int uw[64];
+    for (int i = 0; i < wc; ++i) uw[i] = c_qt.weight_unq[q][wcodes[i]];
+    const int planes = dual ? 2 : 1;
+    const uint8_t* part = nullptr;
+    if (pc > 1) {
+        const int e = T.seed_to_packed[(pc - 2) * kPartitionings + seed];
+        part = T.part_of_texel + ((size_t)(pc - 2) * kPartitionings + e) * tc;
+    }
+    for (int t = 0; t < tc; ++t) {
+        const int y = t / bx, x = t - y * bx;
+        // weight infill
+        const int xwp = (((1024 + bx / 2) / (bx - 1)) * x * (xw - 1) + 32) >> 6;
+        const int ywp = (((1024 + by / 2) / (by - 1)) * y * (yw - 1) + 32) >> 6;
+        const int xf = xwp & 0xF, yf = ywp & 0xF, base = (xwp >> 4) + (ywp >> 4) * xw;
+        const int w11 = (xf * yf + 8) >> 4, w01 = xf - w11, w10 = yf - w11, w00 = 16 - xf - yf + w11;
+        int wt[2] = {0, 0};
+        for (int plane = 0; plane < planes; ++plane) {
+            auto W = [&](int idx) { return idx < xw * yw ? uw[idx * planes + plane] : 0; };
+            int s = 0;
+            if (w00) s += W(base) * w00;
+            if (w01) s += W(base + 1) * w01;
+            if (w10) s += W(base + xw) * w10;
+            if (w11) s += W(base + xw + 1) * w11;
+            wt[plane] = (s + 8) >> 4;
+        }
+        const int p = part ? part[t] : 0;
+        uint32_t px = 0;
+        for (int c = 0; c < 4; ++c) {
+            const int wv = (dual && c == ccs) ? wt[1] : wt[0];
+            const int c0 = e0[p][c] * 257, c1 = e1[p][c] * 257;
+            const int v16 = (c0 * (64 - wv) + c1 * wv + 32) >> 6;
+            px |= (uint32_t)((v16 * 255 + 32767) / 65535) << (8 * c);
+        }
+        put(t, px);
+    }
+}
+
 }  // namespace
 
 const char* kernel_name(int id) {
@@ -2208,6 +2495,15 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
         }
         ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
+                         hipStream_t s) {
+    const int blocks_x = (width + T.bx - 1) / T.bx;
+    const int nblocks = blocks_x * ((height + T.by - 1) / T.by);
+    hipLaunchKernelGGL(k_decode, dim3((nblocks + 63) / 64), dim3(64), 0, s, T, blocks, width, height, blocks_x, nblocks,
+                       rgba);
     return hipGetLastError();
 }
 

@@ -230,6 +230,20 @@ static QuantTables build_quant_tables() {
         t.int_of_quints[(d[2] * 5 + d[1]) * 5 + d[0]] = (uint8_t)Q;
     }
     for (int v = 0; v < 256; ++v) t.pixel_scale[v] = ((float)v / 255.0f) * 65536.0f;
+    for (int T = 0; T < 256; ++T) {
+        int d[5];
+        decode_trits(T, d);
+        t.trit_digits[T] = (uint16_t)(d[0] | (d[1] << 2) | (d[2] << 4) | (d[3] << 6) | (d[4] << 8));
+    }
+    for (int Q = 0; Q < 128; ++Q) {
+        int d[3];
+        decode_quints(Q, d);
+        t.quint_digits[Q] = (uint16_t)(d[0] | (d[1] << 3) | (d[2] << 6));
+    }
+    for (int q = 4; q < 21; ++q)
+        for (int c = 0; c < kLevels[q]; ++c) t.color_unq[q - 4][c] = (uint8_t)color_unquant(q, c);
+    for (int q = 0; q < 12; ++q)
+        for (int c = 0; c < kLevels[q]; ++c) t.weight_unq[q][c] = (uint8_t)weight_unquant(q, c);
     return t;
 }
 

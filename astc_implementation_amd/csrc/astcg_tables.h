@@ -24,6 +24,11 @@ struct QuantTables {
     uint8_t int_of_trits[243];
     uint8_t int_of_quints[125];
     float pixel_scale[256];            // (u8 / 255) * 65536
+    // decoder side
+    uint16_t trit_digits[256];         // 8-bit trit block -> 5 digits, 2 bits each
+    uint16_t quint_digits[128];        // 7-bit quint block -> 3 digits, 3 bits each
+    uint8_t color_unq[17][256];        // colour ISE code -> 0..255 (quant 4..20)
+    uint8_t weight_unq[12][32];        // weight ISE code -> 0..64
 };
 const QuantTables& quant_tables();
 

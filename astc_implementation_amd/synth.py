@@ -73,11 +73,10 @@ def value_noise(rng, h, w, octaves=4, base_cell=64):
         fy, fx = ys / cell, xs / cell
         iy, ix = np.floor(fy).astype(np.int64), np.floor(fx).astype(np.int64)
         ty, tx = _smoothstep(fy - iy)[:, None], _smoothstep(fx - ix)[None, :]
-        a = lat[iy][:, ix]
-        b = lat[iy][:, ix + 1]
-        c = lat[iy + 1][:, ix]
-        d = lat[iy + 1][:, ix + 1]
-        v = (a * (1 - tx) + b * tx) * (1 - ty) + (c * (1 - tx) + d * tx) * ty
+        # interpolate every lattice row along x once, then blend rows along y
+        # (the same operations, in the same order, as the bilinear formula)
+        rows = lat[:, ix] * (1 - tx) + lat[:, ix + 1] * tx
+        v = rows[iy] * (1 - ty) + rows[iy + 1] * ty
         wt = 0.5 ** (o + 1)
         total += wt * v
         wsum += wt

@@ -159,6 +159,16 @@ astcg_status astcg_astc_file_in_memory(uint32_t block_x, uint32_t block_y, uint3
 astcg_status astcg_decode_image(uint32_t block_x, uint32_t block_y, uint32_t width, uint32_t height,
                                 const uint8_t* blocks, size_t blocks_len, uint8_t* rgba, size_t rgba_len);
 
+/* The same decoder on the encoder's GPU (identical output; no counterpart in
+ * the reference, which has no decoder).  Device buffers, enqueued on
+ * `stream`; error blocks decode to magenta as on the host. */
+astcg_status astcg_decode_device(astcg_encoder* enc, uint32_t block_x, uint32_t block_y, uint32_t width,
+                                 uint32_t height, const uint8_t* d_blocks, uint8_t* d_rgba, void* stream);
+
+/* GPU decode of host buffers (synchronous). */
+astcg_status astcg_decode(astcg_encoder* enc, uint32_t block_x, uint32_t block_y, uint32_t width, uint32_t height,
+                          const uint8_t* blocks, size_t blocks_len, uint8_t* rgba, size_t rgba_len);
+
 /* Decodes one block to its symbolic form (inverse of symbolic_to_physical for
  * the block modes the encoder emits).  Returns ASTCG_ERR_UNSUPPORTED for
  * void-extent, dual-plane or HDR blocks. */

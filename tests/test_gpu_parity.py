@@ -1,6 +1,8 @@
 """GPU kernels vs the CPU oracle on the same seeded images: the 128-bit
 blocks must be identical (the kernels evaluate every reduction in the same
 fixed order as the oracle)."""
+import functools
+
 import numpy as np
 import pytest
 
@@ -43,3 +45,45 @@ def test_all_partitions_bit_exact(encoder, bx, by):
     same, g, r, sym = _compare(encoder, img, bx, by, 4)
     bad = np.nonzero(~same)[0]
     assert bad.size == 0, "blocks differ: %s" % bad[:10].tolist()
+
+
+@pytest.mark.parametrize("bx,by", [(4, 4), (5, 5), (6, 6), (8, 8), (10, 10), (12, 10)])
+def test_alpha_image_bit_exact(encoder, bx, by):
+    """Second throughput image of the benchmark set: radial alpha + noise."""
+    img = synth.alpha_image(96, 72, 2)
+    same, g, r, sym = _compare(encoder, img, bx, by, 4)
+    bad = np.nonzero(~same)[0]
+    assert bad.size == 0, "blocks differ: %s" % bad[:10].tolist()
+
+
+def test_small_config_512_bit_exact(encoder):
+    """The reference's CPU-runnable case: 512x512 seed 0, 4x4, partitions 1-4."""
+    img = synth.gradient_image(512, 512, 0)
+    same, g, r, sym = _compare(encoder, img, 4, 4, 4)
+    bad = np.nonzero(~same)[0]
+    assert bad.size == 0, "blocks differ: %s" % bad[:10].tolist()
+
+
+@functools.lru_cache(maxsize=2)
+def _throughput_image(alpha):
+    return synth.alpha_image(4096, 4096, 2) if alpha else synth.gradient_image(4096, 4096, 1)
+
+
+@pytest.mark.parametrize("bx,by,alpha", [(4, 4, False), (6, 6, False), (8, 8, False), (4, 4, True)])
+def test_full_size_sampled_batches(encoder, bx, by, alpha):
+    """4096x4096 at full size: whole reference batches of 920 blocks (first,
+    middle and the partial last one) equal the oracle's encoding of those
+    batches; the full encode is deterministic and decodes close to the input."""
+    img = _throughput_image(alpha)
+    encoder.configure(4096, 4096, bx, by)
+    gpu = encoder.encode(img)
+    assert np.array_equal(gpu, encoder.encode(img))
+    nb = gpu.size // 16
+    last = (nb - 1) // 920 * 920
+    for first in (0, (nb // 2) // 920 * 920, last):
+        count = min(920, nb - first)
+        ref, _, _ = O.encode(img, bx, by, 4, threads=8, first_block=first, block_count=count,
+                             want_symbolic=False)
+        assert np.array_equal(gpu[first * 16:(first + count) * 16], ref), "batch at block %d" % first
+    dec = A.decode_image(bx, by, 4096, 4096, gpu)
+    assert A.psnr(dec, img) > {4: 45.0, 6: 38.0, 8: 34.0}[bx]
