@@ -60,6 +60,7 @@ class Params(ctypes.Structure):
 
 
 FLAG_PROFILE = 1
+FLAG_LABELS = 2
 
 
 class Texture(ctypes.Structure):
@@ -107,6 +108,8 @@ def library():
         "astcg_encode_device": (i32, [vp, vp, vp, vp]),
         "astcg_encode_batch": (i32, [vp, vp, u32, vp]),
         "astcg_decode_device": (i32, [vp, u32, u32, u32, u32, vp, vp, vp]),
+        "astcg_get_labels": (i32, [vp, vp, sz]),
+        "astcg_trial_modes": (i32, [u32, u32, ctypes.POINTER(u32), vp, sz]),
         "astcg_decode": (i32, [vp, u32, u32, u32, u32, vp, sz, vp, sz]),
         "astcg_encode_batch_device": (i32, [vp, vp, u32, vp, vp]),
         "astcg_get_symbolic": (i32, [vp, vp, sz]),
@@ -164,13 +167,13 @@ class Encoder:
         except Exception:
             pass
 
-    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False):
+    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False, labels=False):
         """secondaryInit (compression_pipeline.cpp:263-321)."""
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
-        p.flags = FLAG_PROFILE if profile else 0
+        p.flags = (FLAG_PROFILE if profile else 0) | (FLAG_LABELS if labels else 0)
         _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
         self.width, self.height, self.bx, self.by = width, height, bx, by
 
@@ -245,6 +248,13 @@ class Encoder:
 
     def reset_kernel_times(self):
         _check(library().astcg_reset_kernel_times(self._h), self._h)
+
+    def labels(self):
+        """(num_blocks, 6) int32 search outcome of the last encode made with
+        labels=True: pc, seed, trial mode, best seed for 2 / 3 / 4 partitions."""
+        out = np.zeros((self.num_blocks, 6), np.int32)
+        _check(library().astcg_get_labels(self._h, _ptr(out), self.num_blocks), self._h)
+        return out
 
     def symbolic(self):
         n = self.num_blocks

@@ -16,7 +16,7 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libastcg.so")
 CLI = os.path.join(HERE, "astcg_cli")
 
-SOURCES = ["astcg_tables.cpp", "astcg_codec.cpp", "astcg_api.cpp", "astcg_device.hip"]
+SOURCES = ["astcg_tables.cpp", "astcg_codec.cpp", "astcg_api.cpp", "astcg_device.hip", "astcg_mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

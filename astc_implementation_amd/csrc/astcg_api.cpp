@@ -58,6 +58,8 @@ struct astcg_encoder {
     TexDesc* d_tex = nullptr;
     size_t tex_cap = 0;
     std::vector<TexDesc> h_tex;
+    int16_t* d_pcseed = nullptr;  // [3][nblocks] with ASTCG_FLAG_LABELS
+    size_t pcseed_cap = 0;
     struct EventProfiler* prof = nullptr;
 };
 
@@ -128,6 +130,9 @@ void release_all(astcg_encoder* e) {
     if (e->d_rgba) (void)hipFree(e->d_rgba);
     if (e->d_out) (void)hipFree(e->d_out);
     if (e->d_tex) (void)hipFree(e->d_tex);
+    if (e->d_pcseed) (void)hipFree(e->d_pcseed);
+    e->d_pcseed = nullptr;
+    e->pcseed_cap = 0;
     e->d_chunk = nullptr;
     e->d_best = nullptr;
     e->d_rgba = e->d_out = nullptr;
@@ -369,7 +374,7 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
 // Runs the whole search + pack over the blocks described by A (A.num_blocks
 // blocks of one footprint), chunk by chunk, on stream s.
 astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
-                        hipStream_t s) {
+                        hipStream_t s, int16_t* pc_seed = nullptr) {
     const int chunk = chunk_size(*F.fp, (uint32_t)A.num_blocks, chunk_req);
     astcg_status st = layout_chunk(e, F, chunk, s);
     if (st != ASTCG_OK) return st;
@@ -378,7 +383,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
     for (int blk0 = 0; blk0 < A.num_blocks; blk0 += chunk) {
         A.blk0 = blk0;
         A.nb = std::min(chunk, A.num_blocks - blk0);
-        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof);
+        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, pc_seed);
         if (err != hipSuccess) return hip_fail(e, err, "search kernels");
     }
     err = launch_pack(F.T, A, e->d_best, s, e->prof);
@@ -497,7 +502,14 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
     A.t0.num_blocks = (int)e->nblocks;
     A.ntex = 1;
     A.num_blocks = (int)e->nblocks;
-    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, s);
+    int16_t* pc_seed = nullptr;
+    if (e->params.flags & ASTCG_FLAG_LABELS) {
+        err = grow(reinterpret_cast<void**>(&e->d_pcseed), &e->pcseed_cap, sizeof(int16_t) * 3 * (size_t)e->nblocks);
+        if (err == hipSuccess) err = hipMemsetAsync(e->d_pcseed, 0xFF, sizeof(int16_t) * 3 * (size_t)e->nblocks, s);
+        if (err != hipSuccess) return hip_fail(e, err, "label buffer");
+        pc_seed = e->d_pcseed;
+    }
+    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, s, pc_seed);
 }
 
 astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len, uint8_t* out, size_t out_len) {
@@ -690,6 +702,40 @@ astcg_status astcg_get_symbolic(astcg_encoder* e, astcg_symbolic_block* out, siz
         for (int j = 0; j < 32; ++j) s.packed_color_values[j] = h[i].packed[j];
         for (int j = 0; j < 64; ++j) s.quantized_weights[j] = h[i].qw[j];
     }
+    return ASTCG_OK;
+}
+
+astcg_status astcg_get_labels(astcg_encoder* e, int32_t* out, size_t count) {
+    if (!e || !out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "not configured");
+    if (!(e->params.flags & ASTCG_FLAG_LABELS) || !e->d_pcseed)
+        return fail(e, ASTCG_ERR_NOT_CONFIGURED, "encode with ASTCG_FLAG_LABELS first");
+    if (count < e->nblocks) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "count below number of blocks");
+    std::vector<SymBest> h(e->nblocks);
+    std::vector<int16_t> ps(3 * (size_t)e->nblocks);
+    hipError_t err = hipSetDevice(e->device);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(h.data(), e->d_best, sizeof(SymBest) * h.size(), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(ps.data(), e->d_pcseed, sizeof(int16_t) * ps.size(), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(e, err, "label download");
+    const Footprint& f = *e->cur->fp;
+    for (size_t i = 0; i < h.size(); ++i) {
+        int32_t* o = out + 6 * i;
+        o[0] = h[i].pc;
+        o[1] = h[i].pc > 1 ? h[i].seed : 0;
+        o[2] = h[i].raw_mode < 2048 ? f.raw_to_bm[h[i].raw_mode] : -1;
+        for (int p = 0; p < 3; ++p) o[3 + p] = ps[(size_t)p * e->nblocks + i];
+    }
+    return ASTCG_OK;
+}
+
+astcg_status astcg_trial_modes(uint32_t bx, uint32_t by, uint32_t* count, uint16_t* raw_modes, size_t cap) {
+    if (!count) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!valid_footprint((int)bx, (int)by)) return ASTCG_ERR_INVALID_FOOTPRINT;
+    const Footprint* f = footprint((int)bx, (int)by);
+    *count = (uint32_t)f->nbm;
+    if (raw_modes)
+        for (size_t i = 0; i < (size_t)f->nbm && i < cap; ++i) raw_modes[i] = f->bm[i].raw_mode;
     return ASTCG_OK;
 }
 

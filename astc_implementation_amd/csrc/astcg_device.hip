@@ -1955,7 +1955,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
 
 // ---- pass18 + host argmin (compression_pipeline.cpp:499-519) --------------------
 __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, const int16_t* pb_part,
-                                             const CandOut* cand, SymBest* best) {
+                                             const CandOut* cand, SymBest* best, int16_t* pc_seed) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.nb) return;
     const int npos = pc == 1 ? 1 : 4;
@@ -1972,6 +1972,11 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         if (cur_pb < 0 || be < cur_err) { cur_err = be; cur_pb = pb; cur_slot = bs; }
     }
     SymBest& sb = best[A.blk0 + k];
+    // optional record of each partition count's best partitioning (labels
+    // for the candidate predictor)
+    if (pc_seed && pc > 1)
+        pc_seed[(size_t)(pc - 2) * A.num_blocks + A.blk0 + k] =
+            cur_slot >= 0 ? (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[cur_pb]] : (int16_t)-1;
     if (pc == 1) {
         sb.error = 1e30f;
         sb.raw_mode = 0xFFFF;
@@ -2454,7 +2459,7 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
 }
 
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s, Profiler* prof) {
+                        hipStream_t s, Profiler* prof, int16_t* pc_seed) {
     const int threads = 64;
     auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
     for (int pc = 1; pc <= max_pc; ++pc) {
@@ -2493,7 +2498,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
             launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb);
             if (prof) prof->end(K_REFINE, s);
         }
-        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best);
+        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best, pc_seed);
     }
     return hipGetLastError();
 }

@@ -1,0 +1,333 @@
+// Candidate predictor kernels for gfx950: a small MLP over the normalised
+// texels of a block (BF16 operands on the matrix cores, FP32 accumulation),
+// with the training kernels (backward, softmax cross-entropy, Adam) and the
+// inference-side top-K selection.  The reference has no learned component;
+// these kernels serve the learned candidate policy of the encoder.
+//
+// GEMM: C[M x N] (fp32) = op(A)[M x K] * op(B)[K x N] (+ bias[N]) (+ C),
+// A and B in bf16.  A 256-thread workgroup computes a 64 x 64 tile; each of
+// its four waves owns a 32 x 32 quarter as 2 x 2 v_mfma_f32_16x16x32_bf16
+// tiles.  Both operand tiles sit in LDS with K contiguous (row stride 40
+// elements to spread the 16-byte fragment reads over the banks).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/astcg_predictor.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBM = 64, kBN = 64, kBK = 32, kLdsStride = 40;
+
+__device__ __forceinline__ float bf2f(bf16 v) { return (float)v; }
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const bf16* __restrict__ A, int lda,
+                                               const bf16* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
+                                               const float* __restrict__ bias, int accumulate) {
+    __shared__ __align__(16) bf16 As[kBM * kLdsStride];
+    __shared__ __align__(16) bf16 Bt[kBN * kLdsStride];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int bm = blockIdx.y * kBM, bn = blockIdx.x * kBN;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const bf16 zero = (bf16)0.0f;
+    for (int bk = 0; bk < K; bk += kBK) {
+        // A tile: (m, k) -> As[m][k]
+        if (!TA) {
+            const int m = t >> 2, k0 = (t & 3) * 8;
+            const int gm = bm + m, gk = bk + k0;
+            bf16* dst = As + m * kLdsStride + k0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dst[j] = (gm < M && gk + j < K) ? A[(size_t)gm * lda + gk + j] : zero;
+        } else {
+            const int k = t >> 3, m0 = (t & 7) * 8;
+            const int gk = bk + k, gm = bm + m0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) As[(m0 + j) * kLdsStride + k] = (gk < K && gm + j < M) ? A[(size_t)gk * lda + gm + j] : zero;
+        }
+        // B tile: (k, n) -> Bt[n][k]
+        if (!TB) {
+            const int k = t >> 3, n0 = (t & 7) * 8;
+            const int gk = bk + k, gn = bn + n0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Bt[(n0 + j) * kLdsStride + k] = (gk < K && gn + j < N) ? B[(size_t)gk * ldb + gn + j] : zero;
+        } else {
+            const int n = t >> 2, k0 = (t & 3) * 8;
+            const int gn = bn + n, gk = bk + k0;
+            bf16* dst = Bt + n * kLdsStride + k0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dst[j] = (gn < N && gk + j < K) ? B[(size_t)gn * ldb + gk + j] : zero;
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = *reinterpret_cast<const bf16x8*>(As + (32 * wm + 16 * i + (lane & 15)) * kLdsStride + 8 * (lane >> 4));
+            b[i] = *reinterpret_cast<const bf16x8*>(Bt + (32 * wn + 16 * i + (lane & 15)) * kLdsStride + 8 * (lane >> 4));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // D fragment: lane holds rows 4 * (lane / 16) + r, column lane % 16
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = bn + 32 * wn + 16 * j + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm + 32 * wm + 16 * i + 4 * (lane >> 4) + r;
+                if (row >= M) continue;
+                float v = acc[i][j][r];
+                if (bias) v += bias[col];
+                float* c = C + (size_t)row * ldc + col;
+                *c = accumulate ? *c + v : v;
+            }
+        }
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+    return cdf + z * pdf;
+}
+
+__global__ void k_gelu_fwd(const float* z, bf16* h, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) h[i] = (bf16)gelu(z[i]);
+}
+
+__global__ void k_gelu_bwd(const float* dh, const float* z, bf16* dz, float* dz32, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = dh[i] * gelu_grad(z[i]);
+    dz[i] = (bf16)g;
+    if (dz32) dz32[i] = g;
+}
+
+__global__ void k_to_bf16(const float* x, bf16* y, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (bf16)x[i];
+}
+
+// Column sums of a rows x cols fp32 matrix (bias gradients).
+__global__ void k_colsum(const float* x, int rows, int cols, float* out, int accumulate) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; ++r) s += x[(size_t)r * cols + c];
+    out[c] = accumulate ? out[c] + s : s;
+}
+
+// Softmax cross-entropy over the heads of one row (one wave per row).  Heads
+// are consecutive logit ranges; a label < 0 masks its head.  Writes
+// dlogits = (softmax - onehot) * scale (bf16 and fp32) and the row loss.
+__global__ void __launch_bounds__(64) k_softmax_xent(const float* logits, int rows, int ld, int nheads,
+                                                      const int* head_off, const int* head_len, const int* labels,
+                                                      float scale, bf16* dlogits, float* dlogits32, float* loss) {
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= rows) return;
+    const float* x = logits + (size_t)r * ld;
+    float row_loss = 0.0f;
+    for (int h = 0; h < nheads; ++h) {
+        const int off = head_off[h], len = head_len[h], lab = labels[(size_t)r * nheads + h];
+        float mx = -3.4e38f;
+        for (int i = lane; i < len; i += 64) mx = fmaxf(mx, x[off + i]);
+        for (int m = 32; m > 0; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m));
+        float se = 0.0f;
+        for (int i = lane; i < len; i += 64) se += expf(x[off + i] - mx);
+        for (int m = 32; m > 0; m >>= 1) se += __shfl_xor(se, m);
+        const float lse = mx + logf(se);
+        for (int i = lane; i < len; i += 64) {
+            const size_t o = (size_t)r * ld + off + i;
+            float g = 0.0f;
+            if (lab >= 0) g = (expf(x[off + i] - lse) - (i == lab ? 1.0f : 0.0f)) * scale;
+            dlogits[o] = (bf16)g;
+            if (dlogits32) dlogits32[o] = g;
+        }
+        if (lab >= 0) row_loss += lse - x[off + lab];
+    }
+    if (lane == 0 && loss) loss[r] = row_loss;
+}
+
+// Adam (bias-corrected) on fp32 master weights; refreshes the bf16 copy.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, bf16* w16, long long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, float decay) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * ((mi / c1) / (sqrtf(vi / c2) + eps) + decay * w[i]);
+    w[i] = wi;
+    if (w16) w16[i] = (bf16)wi;
+}
+
+// Block texels -> normalised RGB features (u8 / 255), one row per block;
+// edge texels clamp to the image as the encoder does.
+__global__ void k_features(const uint8_t* rgba, int width, int height, int bx, int by, int blk0, int nblk,
+                           bf16* x, int ldx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tc = bx * by;
+    if (i >= nblk * tc) return;
+    const int b = i / tc, t = i - b * tc;
+    const int g = blk0 + b;
+    const int blocks_x = (width + bx - 1) / bx;
+    const int x0 = (g % blocks_x) * bx, y0 = (g / blocks_x) * by;
+    const int px = min(x0 + t % bx, width - 1), py = min(y0 + t / bx, height - 1);
+    const uint8_t* p = rgba + ((size_t)py * width + px) * 4;
+    bf16* row = x + (size_t)b * ldx + 3 * t;
+    for (int c = 0; c < 3; ++c) row[c] = (bf16)((float)p[c] * (1.0f / 255.0f));
+}
+
+// Top-k indices of each head of each row (one wave per row and head):
+// repeated arg-max with the lowest index winning ties.
+__global__ void __launch_bounds__(64) k_topk(const float* logits, int rows, int ld, int nheads, const int* head_off,
+                                             const int* head_len, int k, int* out_idx, float* out_prob) {
+    const int r = blockIdx.x, h = blockIdx.y, lane = threadIdx.x;
+    if (r >= rows || h >= nheads) return;
+    const float* x = logits + (size_t)r * ld + head_off[h];
+    const int len = head_len[h];
+    float mx = -3.4e38f;
+    for (int i = lane; i < len; i += 64) mx = fmaxf(mx, x[i]);
+    for (int m = 32; m > 0; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m));
+    float se = 0.0f;
+    for (int i = lane; i < len; i += 64) se += expf(x[i] - mx);
+    for (int m = 32; m > 0; m >>= 1) se += __shfl_xor(se, m);
+    float prev_v = 3.4e38f;
+    int prev_i = -1;
+    for (int j = 0; j < k; ++j) {
+        // best value below (prev_v, prev_i) in (value desc, index asc) order
+        float bv = -3.4e38f;
+        int bi = 0x7FFFFFFF;
+        for (int i = lane; i < len; i += 64) {
+            const float v = x[i];
+            const bool after = v < prev_v || (v == prev_v && i > prev_i);
+            if (after && (v > bv || (v == bv && i < bi))) { bv = v; bi = i; }
+        }
+        for (int m = 32; m > 0; m >>= 1) {
+            const float ov = __shfl_xor(bv, m);
+            const int oi = __shfl_xor(bi, m);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (lane == 0) {
+            out_idx[((size_t)r * nheads + h) * k + j] = bi < len ? bi : -1;
+            if (out_prob) out_prob[((size_t)r * nheads + h) * k + j] = bi < len ? expf(bv - mx) / se : 0.0f;
+        }
+        prev_v = bv;
+        prev_i = bi;
+    }
+}
+
+inline unsigned grid1(long long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace
+
+extern "C" {
+
+astcg_status astcg_mlp_gemm(int m, int n, int k, const void* a, int lda, int trans_a, const void* b, int ldb,
+                            int trans_b, float* c, int ldc, const float* bias, int accumulate, void* stream) {
+    if (m <= 0 || n <= 0 || k <= 0 || !a || !b || !c) return ASTCG_ERR_INVALID_ARGUMENT;
+    const dim3 grid((n + kBN - 1) / kBN, (m + kBM - 1) / kBM);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bf16* A = static_cast<const bf16*>(a);
+    const bf16* B = static_cast<const bf16*>(b);
+    if (!trans_a && !trans_b) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, c, ldc, bias, accumulate);
+    else if (!trans_a && trans_b) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, c, ldc, bias, accumulate);
+    else if (trans_a && !trans_b) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, c, ldc, bias, accumulate);
+    else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, s, m, n, k, A, lda, B, ldb, c, ldc, bias, accumulate);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_gelu(const float* z, void* h, long long n, void* stream) {
+    if (!z || !h || n < 0) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (n) hipLaunchKernelGGL(k_gelu_fwd, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), z,
+                              static_cast<bf16*>(h), n);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_gelu_backward(const float* dh, const float* z, void* dz, float* dz32, long long n,
+                                     void* stream) {
+    if (!dh || !z || !dz || n < 0) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (n) hipLaunchKernelGGL(k_gelu_bwd, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), dh, z,
+                              static_cast<bf16*>(dz), dz32, n);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_to_bf16(const float* x, void* y, long long n, void* stream) {
+    if (!x || !y || n < 0) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (n) hipLaunchKernelGGL(k_to_bf16, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x,
+                              static_cast<bf16*>(y), n);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_colsum(const float* x, int rows, int cols, float* out, int accumulate, void* stream) {
+    if (!x || !out || rows < 0 || cols <= 0) return ASTCG_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(k_colsum, dim3(grid1(cols, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x, rows, cols,
+                       out, accumulate);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_softmax_xent(const float* logits, int rows, int ld, int nheads, const int* head_off,
+                                    const int* head_len, const int* labels, float scale, void* dlogits,
+                                    float* dlogits32, float* loss, void* stream) {
+    if (!logits || rows < 0 || nheads <= 0 || !head_off || !head_len || !labels || !dlogits)
+        return ASTCG_ERR_INVALID_ARGUMENT;
+    if (rows)
+        hipLaunchKernelGGL(k_softmax_xent, dim3(rows), dim3(64), 0, static_cast<hipStream_t>(stream), logits, rows, ld,
+                           nheads, head_off, head_len, labels, scale, static_cast<bf16*>(dlogits), dlogits32, loss);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_adam(float* w, const float* g, float* m, float* v, void* w16, long long n, float lr,
+                            float beta1, float beta2, float eps, int step, float weight_decay, void* stream) {
+    if (!w || !g || !m || !v || n < 0 || step < 1) return ASTCG_ERR_INVALID_ARGUMENT;
+    float c1 = 1.0f, c2 = 1.0f;
+    for (int i = 0; i < step && i < 100000; ++i) { c1 *= beta1; c2 *= beta2; }
+    c1 = 1.0f - c1;
+    c2 = 1.0f - c2;
+    if (n)
+        hipLaunchKernelGGL(k_adam, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), w, g, m, v,
+                           static_cast<bf16*>(w16), n, lr, beta1, beta2, eps, c1, c2, weight_decay);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_features(const uint8_t* d_rgba, int width, int height, int block_x, int block_y,
+                                int first_block, int num_blocks, void* x, int ldx, void* stream) {
+    if (!d_rgba || !x || width <= 0 || height <= 0 || block_x < 4 || block_y < 4 || num_blocks < 0 ||
+        ldx < 3 * block_x * block_y)
+        return ASTCG_ERR_INVALID_ARGUMENT;
+    const long long n = (long long)num_blocks * block_x * block_y;
+    if (n)
+        hipLaunchKernelGGL(k_features, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), d_rgba,
+                           width, height, block_x, block_y, first_block, num_blocks, static_cast<bf16*>(x), ldx);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_topk(const float* logits, int rows, int ld, int nheads, const int* head_off,
+                            const int* head_len, int k, int* out_idx, float* out_prob, void* stream) {
+    if (!logits || rows < 0 || nheads <= 0 || !head_off || !head_len || k <= 0 || !out_idx)
+        return ASTCG_ERR_INVALID_ARGUMENT;
+    if (rows)
+        hipLaunchKernelGGL(k_topk, dim3(rows, nheads), dim3(64), 0, static_cast<hipStream_t>(stream), logits, rows, ld,
+                           nheads, head_off, head_len, k, out_idx, out_prob);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+}  // extern "C"

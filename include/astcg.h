@@ -46,6 +46,7 @@ typedef struct astcg_encoder astcg_encoder;
 typedef enum { ASTCG_POLICY_COMPAT = 0 } astcg_policy;
 
 #define ASTCG_FLAG_PROFILE 1u  /* record HIP events around every kernel launch */
+#define ASTCG_FLAG_LABELS 2u   /* keep each partition count's best partitioning (astcg_get_labels) */
 
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */
@@ -126,6 +127,18 @@ astcg_status astcg_encode_batch_device(astcg_encoder* enc, const astcg_texture* 
 /* Optional: copy the chosen symbolic blocks of the last encode (host memory,
  * blocks_x*blocks_y entries). */
 astcg_status astcg_get_symbolic(astcg_encoder* enc, astcg_symbolic_block* out, size_t count);
+
+/* Search outcome per block of the last single-image encode made with
+ * ASTCG_FLAG_LABELS: 6 int32 per block = {partition count, partition seed,
+ * trial block mode index (0..trial modes - 1), best seed for 2, 3 and 4
+ * partitions (-1 = none)}.  Training labels for the candidate predictor; no
+ * counterpart in the reference. */
+astcg_status astcg_get_labels(astcg_encoder* enc, int32_t* out, size_t count);
+
+/* Number of trial block modes (the reference's valid_block_modes) of a
+ * footprint, and the raw 11-bit mode of trial mode i. */
+astcg_status astcg_trial_modes(uint32_t block_x, uint32_t block_y, uint32_t* count, uint16_t* raw_modes,
+                               size_t cap);
 
 /* blocks_x, blocks_y and the number of blocks of the configured image. */
 astcg_status astcg_block_counts(const astcg_encoder* enc, uint32_t* blocks_x, uint32_t* blocks_y,

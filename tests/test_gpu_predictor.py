@@ -1,0 +1,177 @@
+"""Candidate-predictor kernels (include/astcg_predictor.h) against a plain
+PyTorch fp32 reference on the CPU that rounds to bfloat16 at the same points
+(GEMM operands, hidden activations, backward deltas); tolerance 2e-2
+relative to each tensor's scale, which covers fp32 summation order and
+single-ulp bf16 flips of intermediate activations."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import astc_implementation_amd as A
+from astc_implementation_amd import synth
+from astc_implementation_amd.predictor import Predictor, HEADS
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+TOL = 2e-2
+
+
+def _close(gpu, ref, tol=TOL):
+    gpu = gpu.detach().float().cpu()
+    ref = ref.detach().float().cpu()
+    scale = max(float(ref.abs().max()), 1e-6)
+    err = float((gpu - ref).abs().max())
+    assert err <= tol * scale, "max abs err %g vs scale %g" % (err, scale)
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).float()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 32), (37, 101, 70), (130, 3125, 256), (256, 48, 300)])
+def test_gemm_matches_reference(ta, tb, m, n, k):
+    g = torch.Generator().manual_seed(m * 7 + n * 3 + k)
+    a = torch.randn((k, m) if ta else (m, k), generator=g)
+    b = torch.randn((n, k) if tb else (k, n), generator=g)
+    bias = torch.randn(n, generator=g)
+    P = Predictor((4, 4))
+    da, db = a.to(torch.bfloat16).cuda(), b.to(torch.bfloat16).cuda()
+    c = torch.full((m, n), 3.0, device="cuda")
+    P.gemm(m, n, k, da, a.shape[1], ta, db, b.shape[1], tb, c, n, bias.cuda())
+    ref = (_bf(a).t() if ta else _bf(a)) @ (_bf(b).t() if tb else _bf(b)) + bias
+    torch.cuda.synchronize()
+    _close(c, ref, 1e-4 * k ** 0.5)
+    c2 = c.clone()
+    P.gemm(m, n, k, da, a.shape[1], ta, db, b.shape[1], tb, c2, n, None, accumulate=True)
+    torch.cuda.synchronize()
+    _close(c2, 2 * ref - bias, 2e-4 * k ** 0.5)
+
+
+def _gelu(z):
+    return 0.5 * z * (1.0 + torch.erf(z / 2 ** 0.5))
+
+
+def _gelu_grad(z):
+    return 0.5 * (1.0 + torch.erf(z / 2 ** 0.5)) + z * torch.exp(-0.5 * z * z) / (2 * np.pi) ** 0.5
+
+
+def _reference(P, x, labels):
+    W1, b1, W2, b2, W3, b3 = [w.detach().cpu() for w in P.params]
+    B = x.shape[0]
+    z1 = _bf(x) @ _bf(W1) + b1
+    h1 = _bf(_gelu(z1))
+    z2 = h1 @ _bf(W2) + b2
+    h2 = _bf(_gelu(z2))
+    logits = h2 @ _bf(W3) + b3
+    dlog = torch.zeros_like(logits)
+    loss = torch.zeros(B)
+    for hi, (off, ln) in enumerate(zip(P.head_off, P.head_len)):
+        lab = labels[:, hi]
+        sl = logits[:, off:off + ln]
+        lse = torch.logsumexp(sl, dim=1)
+        p = torch.softmax(sl, dim=1)
+        on = lab >= 0
+        oh = torch.zeros_like(p)
+        oh[on, lab[on].long()] = 1.0
+        dlog[:, off:off + ln] = torch.where(on[:, None], (p - oh) / B, torch.zeros_like(p))
+        loss += torch.where(on, lse - sl.gather(1, lab.clamp(min=0).long()[:, None])[:, 0], torch.zeros(B))
+    d16 = _bf(dlog)
+    gW3 = h2.t() @ d16
+    gb3 = dlog.sum(0)
+    dz2 = (d16 @ _bf(W3).t()) * _gelu_grad(z2)
+    gW2 = h1.t() @ _bf(dz2)
+    gb2 = dz2.sum(0)
+    dz1 = (_bf(dz2) @ _bf(W2).t()) * _gelu_grad(z1)
+    gW1 = _bf(x).t() @ _bf(dz1)
+    gb1 = dz1.sum(0)
+    return logits, loss, [gW1, gb1, gW2, gb2, gW3, gb3]
+
+
+@pytest.mark.parametrize("block,B", [((4, 4), 37), ((6, 6), 64)])
+def test_forward_backward_match_reference(block, B):
+    P = Predictor(block, seed=3)
+    g = torch.Generator().manual_seed(B)
+    x = torch.rand((B, P.nin), generator=g)
+    labels = torch.stack([torch.randint(0, ln, (B,), generator=g) for ln in P.head_len], 1).int()
+    labels[::3, 1] = -1  # masked heads
+    labels[1::4, 3] = -1
+    logits = P.forward(x.to(torch.bfloat16).cuda())
+    loss, grads = P.backward(logits, labels.cuda())
+    torch.cuda.synchronize()
+    r_logits, r_loss, r_grads = _reference(P, x, labels)
+    _close(logits, r_logits)
+    _close(loss, r_loss)
+    for g_gpu, g_ref in zip(grads, r_grads):
+        _close(g_gpu, g_ref)
+
+
+def test_adam_matches_reference():
+    P = Predictor((4, 4), seed=5)
+    g = torch.Generator().manual_seed(9)
+    grads = [torch.randn(w.shape, generator=g) for w in P.params]
+    w0 = [w.detach().cpu().clone() for w in P.params]
+    for _ in range(3):
+        P.adam([gg.cuda() for gg in grads], lr=1e-2, weight_decay=0.01)
+    torch.cuda.synchronize()
+    for w, w_init, gg in zip(P.params, w0, grads):
+        ref = w_init.clone()
+        m = torch.zeros_like(ref)
+        v = torch.zeros_like(ref)
+        for step in range(1, 4):
+            m = 0.9 * m + 0.1 * gg
+            v = 0.999 * v + 0.001 * gg * gg
+            ref = ref - 1e-2 * ((m / (1 - 0.9 ** step)) / ((v / (1 - 0.999 ** step)).sqrt() + 1e-8) + 0.01 * ref)
+        _close(w, ref, 1e-5)
+
+
+def test_topk_and_features():
+    P = Predictor((4, 4))
+    g = torch.Generator().manual_seed(2)
+    logits = torch.randn((50, P.nout), generator=g)
+    logits[0, 10:20] = 5.0  # ties: lowest index first
+    idx, prob = P.topk(logits.cuda(), 8)
+    torch.cuda.synchronize()
+    for r in range(50):
+        for h, (off, ln) in enumerate(zip(P.head_off, P.head_len)):
+            sl = logits[r, off:off + ln].numpy()
+            order = np.lexsort((np.arange(ln), -sl))[:8]
+            got = idx[r, h].cpu().numpy()
+            # heads shorter than k pad with -1
+            assert list(got[:len(order)]) == list(order) and all(got[len(order):] == -1)
+            p = np.exp(sl - sl.max()) / np.exp(sl - sl.max()).sum()
+            assert np.allclose(prob[r, h].cpu().numpy()[:len(order)], p[order], rtol=1e-4, atol=1e-6)
+    img = synth.gradient_image(19, 13, 4)
+    x = P.features(torch.from_numpy(img.reshape(-1).copy()).cuda(), 19, 13, 2, 7)
+    torch.cuda.synchronize()
+    for b in range(7):
+        gb = 2 + b
+        bx0, by0 = (gb % 5) * 4, (gb // 5) * 4
+        ref = []
+        for t in range(16):
+            px = img[min(by0 + t // 4, 12), min(bx0 + t % 4, 18), :3].astype(np.float32) / 255.0
+            ref.extend(px)
+        _close(x[b].float(), torch.tensor(ref), 4e-3)
+
+
+def test_search_labels_are_consistent():
+    enc = A.Encoder(0)
+    img = synth.gradient_image(64, 48, 12)
+    enc.configure(64, 48, 4, 4, labels=True)
+    enc.encode(img)
+    lab = enc.labels()
+    sym = enc.symbolic()
+    raws = (ctypes.c_uint16 * 64)()
+    n = ctypes.c_uint32()
+    A.library().astcg_trial_modes(4, 4, ctypes.byref(n), raws, 64)
+    for i in range(len(sym)):
+        pc = sym[i].partition_count
+        assert lab[i, 0] == pc
+        assert raws[lab[i, 2]] == sym[i].block_mode_index
+        if pc > 1:
+            assert lab[i, 1] == sym[i].partition_index
+            assert lab[i, 3 + pc - 2] == lab[i, 1]
+        assert all(0 <= lab[i, 3 + j] < 1024 for j in range(3))
+    enc.close()
