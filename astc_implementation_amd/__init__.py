@@ -76,6 +76,22 @@ class Texture(ctypes.Structure):
     ]
 
 
+class LearnedParams(ctypes.Structure):
+    """astcg_learned_params: pruning of the LEARNED candidate policy."""
+    _fields_ = [
+        ("seeds", ctypes.c_uint32),
+        ("modes", ctypes.c_uint32),
+        ("slots", ctypes.c_uint32),
+        ("fallback_below", ctypes.c_float),
+        ("count_below", ctypes.c_float),
+        ("reserved", ctypes.c_uint32 * 3),
+    ]
+
+
+POLICY_COMPAT = 0
+POLICY_LEARNED = 1
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("total_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
 
@@ -109,6 +125,9 @@ def library():
         "astcg_encode_batch": (i32, [vp, vp, u32, vp]),
         "astcg_decode_device": (i32, [vp, u32, u32, u32, u32, vp, vp, vp]),
         "astcg_get_labels": (i32, [vp, vp, sz]),
+        "astcg_default_learned_params": (None, [vp]),
+        "astcg_set_predictor": (i32, [vp, u32, u32, u32, vp, sz, vp]),
+        "astcg_fallback_rate": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
         "astcg_trial_modes": (i32, [u32, u32, ctypes.POINTER(u32), vp, sz]),
         "astcg_decode": (i32, [vp, u32, u32, u32, u32, vp, sz, vp, sz]),
         "astcg_encode_batch_device": (i32, [vp, vp, u32, vp, vp]),
@@ -167,12 +186,14 @@ class Encoder:
         except Exception:
             pass
 
-    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False, labels=False):
+    def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False, labels=False,
+                  policy=POLICY_COMPAT):
         """secondaryInit (compression_pipeline.cpp:263-321)."""
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
+        p.policy = policy
         p.flags = (FLAG_PROFILE if profile else 0) | (FLAG_LABELS if labels else 0)
         _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
         self.width, self.height, self.bx, self.by = width, height, bx, by
@@ -195,7 +216,8 @@ class Encoder:
         _check(library().astcg_encode_device(self._h, ctypes.c_void_p(d_rgba), ctypes.c_void_p(d_out),
                                              ctypes.c_void_p(stream or 0)), self._h)
 
-    def encode_batch(self, images, footprints, max_partitions=4, chunk_blocks=0, profile=False):
+    def encode_batch(self, images, footprints, max_partitions=4, chunk_blocks=0, profile=False,
+                     policy=POLICY_COMPAT):
         """Encodes several RGBA8 images (H x W x 4 uint8 arrays) with their own
         footprints [(bx, by), ...] in one batched call; returns the block
         arrays in input order."""
@@ -211,6 +233,7 @@ class Encoder:
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
         p.flags = FLAG_PROFILE if profile else 0
+        p.policy = policy
         _check(library().astcg_encode_batch(self._h, tex, len(imgs), ctypes.byref(p)), self._h)
         return outs
 
@@ -248,6 +271,23 @@ class Encoder:
 
     def reset_kernel_times(self):
         _check(library().astcg_reset_kernel_times(self._h), self._h)
+
+    def set_predictor(self, bx, by, params, hidden=256, **learned):
+        """Installs predictor weights (six fp32 arrays W1, b1, W2, b2, W3, b3)
+        for footprint bx x by; keyword arguments override the pruning
+        defaults (seeds, modes, slots, fallback_below, count_below)."""
+        flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
+        lp = LearnedParams()
+        library().astcg_default_learned_params(ctypes.byref(lp))
+        for k, v in learned.items():
+            setattr(lp, k, v)
+        _check(library().astcg_set_predictor(self._h, bx, by, hidden, _ptr(flat), flat.size, ctypes.byref(lp)),
+               self._h)
+
+    def fallback_rate(self):
+        r = ctypes.c_double()
+        _check(library().astcg_fallback_rate(self._h, ctypes.byref(r)), self._h)
+        return r.value
 
     def labels(self):
         """(num_blocks, 6) int32 search outcome of the last encode made with

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/astcg.h"
+#include "../../include/astcg_predictor.h"
 #include "astcg_codec.h"
 #include "astcg_device.h"
 #include "astcg_tables.h"
@@ -30,6 +31,15 @@ struct FpState {
     void* d_tables = nullptr;
     DevTables T{};
     int wstride = 0;  // LDS row of quantised weights: max trial weight count, multiple of 4
+};
+
+// Candidate predictor of one footprint (LEARNED policy).
+struct PredState {
+    int nin = 0, hidden = 0, nout = 0, nmodes = 0;
+    void* w16[3] = {nullptr, nullptr, nullptr};  // bf16 W1, W2, W3
+    float* bias[3] = {nullptr, nullptr, nullptr};
+    int* d_head = nullptr;  // [10]: head offsets then lengths
+    astcg_learned_params lp{};
 };
 
 struct astcg_encoder {
@@ -58,6 +68,12 @@ struct astcg_encoder {
     TexDesc* d_tex = nullptr;
     size_t tex_cap = 0;
     std::vector<TexDesc> h_tex;
+    std::map<int, PredState> preds;  // keyed like fps
+    void* d_pwork = nullptr;         // predictor inference workspace
+    size_t pwork_cap = 0;
+    void* d_plan = nullptr;          // search plan buffers
+    size_t plan_cap = 0;
+    long long fb_blocks = 0, fb_total = 0;
     int16_t* d_pcseed = nullptr;  // [3][nblocks] with ASTCG_FLAG_LABELS
     size_t pcseed_cap = 0;
     struct EventProfiler* prof = nullptr;
@@ -131,6 +147,18 @@ void release_all(astcg_encoder* e) {
     if (e->d_out) (void)hipFree(e->d_out);
     if (e->d_tex) (void)hipFree(e->d_tex);
     if (e->d_pcseed) (void)hipFree(e->d_pcseed);
+    for (auto& kv : e->preds) {
+        for (int i = 0; i < 3; ++i) {
+            if (kv.second.w16[i]) (void)hipFree(kv.second.w16[i]);
+            if (kv.second.bias[i]) (void)hipFree(kv.second.bias[i]);
+        }
+        if (kv.second.d_head) (void)hipFree(kv.second.d_head);
+    }
+    e->preds.clear();
+    if (e->d_pwork) (void)hipFree(e->d_pwork);
+    if (e->d_plan) (void)hipFree(e->d_plan);
+    e->d_pwork = e->d_plan = nullptr;
+    e->pwork_cap = e->plan_cap = 0;
     e->d_pcseed = nullptr;
     e->pcseed_cap = 0;
     e->d_chunk = nullptr;
@@ -371,20 +399,118 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
     return ASTCG_OK;
 }
 
+constexpr int kInferBlocks = 32768;  // predictor inference sub-chunk
+
+// LEARNED policy: runs the predictor over the chunk's blocks and lays out the
+// search plan (top-k of every head per block, then the plan kernels).
+astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, const ImageArgs& A,
+                        const std::vector<TexDesc>& texs, hipStream_t s, SearchPlan* plan) {
+    const astcg_learned_params& lp = P.lp;
+    const int kmax = (int)std::max<uint32_t>(std::max<uint32_t>(lp.seeds, lp.modes), 4);
+    const int nb = A.nb, words = (F.fp->nbm + 31) / 32;
+    // plan buffers
+    struct Lay { size_t size = 0; size_t add(size_t b) { size = (size + 255) & ~size_t(255); size_t o = size; size += b; return o; } } L;
+    const size_t o_idx = L.add(sizeof(int) * (size_t)nb * 5 * kmax), o_prob = L.add(sizeof(float) * (size_t)nb * 5 * kmax);
+    const size_t o_mask = L.add(sizeof(uint32_t) * (size_t)nb * words), o_slots = L.add((size_t)nb);
+    const size_t o_fb = L.add((size_t)nb), o_list = L.add(sizeof(int) * (size_t)nb * 4);
+    const size_t o_inv = L.add(sizeof(int) * (size_t)nb * 4), o_cnt = L.add(sizeof(int) * 4);
+    hipError_t err = grow(&e->d_plan, &e->plan_cap, L.size);
+    if (err != hipSuccess) return hip_fail(e, err, "plan buffers");
+    uint8_t* pb = static_cast<uint8_t*>(e->d_plan);
+    // inference workspace for one sub-chunk
+    const int H = P.hidden, NO = P.nout, sub = std::min(nb, kInferBlocks);
+    Lay W;
+    const size_t o_x = W.add(2 * (size_t)sub * P.nin), o_z = W.add(4 * (size_t)sub * H), o_h = W.add(2 * (size_t)sub * H);
+    const size_t o_z2 = W.add(4 * (size_t)sub * H), o_h2 = W.add(2 * (size_t)sub * H), o_l = W.add(4 * (size_t)sub * NO);
+    err = grow(&e->d_pwork, &e->pwork_cap, W.size);
+    if (err != hipSuccess) return hip_fail(e, err, "predictor workspace");
+    uint8_t* wb = static_cast<uint8_t*>(e->d_pwork);
+    void* X = wb + o_x;
+    float* Z1 = reinterpret_cast<float*>(wb + o_z);
+    void* H1 = wb + o_h;
+    float* Z2 = reinterpret_cast<float*>(wb + o_z2);
+    void* H2 = wb + o_h2;
+    float* LG = reinterpret_cast<float*>(wb + o_l);
+    int* tidx = reinterpret_cast<int*>(pb + o_idx);
+    float* tprob = reinterpret_cast<float*>(pb + o_prob);
+    for (int b0 = 0; b0 < nb; b0 += sub) {
+        const int n = std::min(sub, nb - b0);
+        const int g0 = A.blk0 + b0;
+        // features: one call per texture overlapping the blocks g0 .. g0 + n - 1
+        for (const TexDesc& t : texs) {
+            const int lo = std::max(g0, t.first_block), hi = std::min(g0 + n, t.first_block + t.num_blocks);
+            if (lo >= hi) continue;
+            astcg_status st = astcg_mlp_features(t.rgba, t.width, t.height, (int)F.fp->bx, (int)F.fp->by,
+                                                 lo - t.first_block, hi - lo,
+                                                 static_cast<uint8_t*>(X) + 2 * (size_t)(lo - g0) * P.nin, P.nin, s);
+            if (st != ASTCG_OK) return fail(e, st, "predictor features");
+        }
+        astcg_status st = astcg_mlp_gemm(n, H, P.nin, X, P.nin, 0, P.w16[0], H, 0, Z1, H, P.bias[0], 0, s);
+        if (st == ASTCG_OK) st = astcg_mlp_gelu(Z1, H1, (long long)n * H, s);
+        if (st == ASTCG_OK) st = astcg_mlp_gemm(n, H, H, H1, H, 0, P.w16[1], H, 0, Z2, H, P.bias[1], 0, s);
+        if (st == ASTCG_OK) st = astcg_mlp_gelu(Z2, H2, (long long)n * H, s);
+        if (st == ASTCG_OK) st = astcg_mlp_gemm(n, NO, H, H2, H, 0, P.w16[2], NO, 0, LG, NO, P.bias[2], 0, s);
+        if (st == ASTCG_OK)
+            st = astcg_mlp_topk(LG, n, NO, 5, P.d_head, P.d_head + 5, kmax, tidx + (size_t)b0 * 5 * kmax,
+                                tprob + (size_t)b0 * 5 * kmax, s);
+        if (st != ASTCG_OK) return fail(e, st, "predictor inference");
+    }
+    SearchPlan& pl = *plan;
+    pl = SearchPlan{};
+    pl.mode_mask = reinterpret_cast<uint32_t*>(pb + o_mask);
+    pl.mask_words = words;
+    pl.slots = pb + o_slots;
+    pl.pb_list = reinterpret_cast<int*>(pb + o_list);
+    pl.pb_inv = reinterpret_cast<int*>(pb + o_inv);
+    pl.pb_count = reinterpret_cast<int*>(pb + o_cnt);
+    pl.topk_idx = tidx;
+    pl.topk_prob = tprob;
+    pl.kmax = kmax;
+    pl.seeds = (int)lp.seeds;
+    pl.modes = (int)lp.modes;
+    pl.nslots = (int)lp.slots;
+    pl.fallback_below = lp.fallback_below;
+    pl.count_below = lp.count_below;
+    pl.fallback = pb + o_fb;
+    return ASTCG_OK;
+}
+
 // Runs the whole search + pack over the blocks described by A (A.num_blocks
-// blocks of one footprint), chunk by chunk, on stream s.
+// blocks of one footprint), chunk by chunk, on stream s.  `pred` selects the
+// LEARNED policy (texs: the textures of A, for the predictor's inputs).
 astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
-                        hipStream_t s, int16_t* pc_seed = nullptr) {
+                        hipStream_t s, int16_t* pc_seed = nullptr, const PredState* pred = nullptr,
+                        const std::vector<TexDesc>* texs = nullptr) {
     const int chunk = chunk_size(*F.fp, (uint32_t)A.num_blocks, chunk_req);
     astcg_status st = layout_chunk(e, F, chunk, s);
     if (st != ASTCG_OK) return st;
     hipError_t err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)A.num_blocks);
     if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
+    std::vector<TexDesc> one;
+    if (pred && !texs) {
+        one.push_back(A.t0);
+        texs = &one;
+    }
+    e->fb_blocks = e->fb_total = 0;
     for (int blk0 = 0; blk0 < A.num_blocks; blk0 += chunk) {
         A.blk0 = blk0;
         A.nb = std::min(chunk, A.num_blocks - blk0);
-        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, pc_seed);
+        SearchPlan plan;
+        if (pred) {
+            st = plan_chunk(e, F, *pred, A, *texs, s, &plan);
+            if (st != ASTCG_OK) return st;
+        }
+        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, pc_seed, pred ? &plan : nullptr);
         if (err != hipSuccess) return hip_fail(e, err, "search kernels");
+        if (pred) {
+            // fallback statistics of the chunk (small read-back)
+            std::vector<uint8_t> fb(A.nb);
+            err = hipMemcpyAsync(fb.data(), plan.fallback, A.nb, hipMemcpyDeviceToHost, s);
+            if (err == hipSuccess) err = hipStreamSynchronize(s);
+            if (err != hipSuccess) return hip_fail(e, err, "plan read-back");
+            for (uint8_t v : fb) e->fb_blocks += v;
+            e->fb_total += A.nb;
+        }
     }
     err = launch_pack(F.T, A, e->d_best, s, e->prof);
     if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
@@ -396,7 +522,8 @@ astcg_status check_params(astcg_encoder* e, const astcg_params* params, astcg_pa
     if (params) *p = *params;
     if (p->max_partitions < 1 || p->max_partitions > 4)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "max_partitions must be 1..4");
-    if (p->policy != ASTCG_POLICY_COMPAT) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "unknown partition policy");
+    if (p->policy != ASTCG_POLICY_COMPAT && p->policy != ASTCG_POLICY_LEARNED)
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "unknown partition policy");
     if ((p->flags & ASTCG_FLAG_PROFILE) && !e->prof) e->prof = new EventProfiler();
     return ASTCG_OK;
 }
@@ -466,6 +593,8 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     const FpState* F = nullptr;
     st = get_footprint(e, bx, by, &F);
     if (st != ASTCG_OK) return st;
+    if (p.policy == ASTCG_POLICY_LEARNED && !e->preds.count((int)(bx * 16 + by)))
+        return fail(e, ASTCG_ERR_NOT_CONFIGURED, "LEARNED policy without a predictor for this footprint");
     e->width = width;
     e->height = height;
     e->bx = bx;
@@ -509,7 +638,13 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
         if (err != hipSuccess) return hip_fail(e, err, "label buffer");
         pc_seed = e->d_pcseed;
     }
-    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, s, pc_seed);
+    const PredState* pred = nullptr;
+    if (e->params.policy == ASTCG_POLICY_LEARNED) {
+        auto it = e->preds.find((int)(e->bx * 16 + e->by));
+        if (it == e->preds.end()) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "no predictor for this footprint");
+        pred = &it->second;
+    }
+    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, s, pc_seed, pred);
 }
 
 astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len, uint8_t* out, size_t out_len) {
@@ -607,7 +742,14 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
         A.num_blocks = (int)group_blocks[gi];
         A.t0 = e->h_tex[group_start[gi]];
         A.tex = e->d_tex + group_start[gi];
-        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, s);
+        const PredState* pred = nullptr;
+        std::vector<TexDesc> texs(e->h_tex.begin() + group_start[gi], e->h_tex.begin() + group_start[gi + 1]);
+        if (p.policy == ASTCG_POLICY_LEARNED) {
+            auto it = e->preds.find(keys[gi]);
+            if (it == e->preds.end()) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "no predictor for a batch footprint");
+            pred = &it->second;
+        }
+        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, s, nullptr, pred, &texs);
         if (st != ASTCG_OK) return st;
     }
     return ASTCG_OK;
@@ -726,6 +868,85 @@ astcg_status astcg_get_labels(astcg_encoder* e, int32_t* out, size_t count) {
         o[2] = h[i].raw_mode < 2048 ? f.raw_to_bm[h[i].raw_mode] : -1;
         for (int p = 0; p < 3; ++p) o[3 + p] = ps[(size_t)p * e->nblocks + i];
     }
+    return ASTCG_OK;
+}
+
+void astcg_default_learned_params(astcg_learned_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->seeds = 2;
+    p->modes = 16;
+    p->slots = 4;
+    p->fallback_below = 0.3f;
+    p->count_below = 0.1f;
+}
+
+astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uint32_t hidden, const float* weights,
+                                 size_t count, const astcg_learned_params* lp) {
+    if (!e || !weights || hidden == 0 || hidden > 4096) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "bad predictor");
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    astcg_learned_params q;
+    astcg_default_learned_params(&q);
+    if (lp) q = *lp;
+    if (q.seeds < 1 || q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8)
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
+    const Footprint* f = footprint((int)bx, (int)by);
+    const int nin = 3 * (int)(bx * by), H = (int)hidden, nout = 4 + 3 * 1024 + f->nbm;
+    const size_t sizes[6] = {(size_t)nin * H, (size_t)H, (size_t)H * H, (size_t)H, (size_t)H * nout, (size_t)nout};
+    size_t need = 0;
+    for (size_t z : sizes) need += z;
+    if (count != need) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "predictor weight count mismatch");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    const int key = (int)(bx * 16 + by);
+    auto old = e->preds.find(key);
+    if (old != e->preds.end()) {
+        (void)hipDeviceSynchronize();
+        for (int i = 0; i < 3; ++i) {
+            if (old->second.w16[i]) (void)hipFree(old->second.w16[i]);
+            if (old->second.bias[i]) (void)hipFree(old->second.bias[i]);
+        }
+        if (old->second.d_head) (void)hipFree(old->second.d_head);
+        e->preds.erase(old);
+    }
+    PredState P;
+    P.nin = nin;
+    P.hidden = H;
+    P.nout = nout;
+    P.nmodes = f->nbm;
+    P.lp = q;
+    float* tmp = nullptr;
+    err = hipMalloc(&tmp, sizes[4] * sizeof(float));
+    size_t off = 0;
+    for (int i = 0; i < 3 && err == hipSuccess; ++i) {
+        const size_t wn = sizes[2 * i], bn = sizes[2 * i + 1];
+        err = hipMalloc(&P.w16[i], wn * 2);
+        if (err == hipSuccess) err = hipMalloc(&P.bias[i], bn * sizeof(float));
+        if (err == hipSuccess) err = hipMemcpy(tmp, weights + off, wn * sizeof(float), hipMemcpyHostToDevice);
+        if (err == hipSuccess && astcg_mlp_to_bf16(tmp, P.w16[i], (long long)wn, nullptr) != ASTCG_OK) err = hipErrorUnknown;
+        if (err == hipSuccess) err = hipDeviceSynchronize();
+        if (err == hipSuccess) err = hipMemcpy(P.bias[i], weights + off + wn, bn * sizeof(float), hipMemcpyHostToDevice);
+        off += wn + bn;
+    }
+    const int head[10] = {0, 4, 4 + 1024, 4 + 2048, 4 + 3072, 4, 1024, 1024, 1024, f->nbm};
+    if (err == hipSuccess) err = hipMalloc(&P.d_head, sizeof(head));
+    if (err == hipSuccess) err = hipMemcpy(P.d_head, head, sizeof(head), hipMemcpyHostToDevice);
+    if (tmp) (void)hipFree(tmp);
+    if (err != hipSuccess) {
+        for (int i = 0; i < 3; ++i) {
+            if (P.w16[i]) (void)hipFree(P.w16[i]);
+            if (P.bias[i]) (void)hipFree(P.bias[i]);
+        }
+        if (P.d_head) (void)hipFree(P.d_head);
+        return hip_fail(e, err, "predictor upload");
+    }
+    e->preds[key] = P;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_fallback_rate(astcg_encoder* e, double* rate) {
+    if (!e || !rate) return ASTCG_ERR_INVALID_ARGUMENT;
+    *rate = e->fb_total ? (double)e->fb_blocks / (double)e->fb_total : 0.0;
     return ASTCG_OK;
 }
 

@@ -59,6 +59,28 @@ struct ImageArgs {
     int num_blocks;           // blocks of all textures
     int blk0;                 // first block of the chunk
     int nb;                   // blocks in the chunk
+    // partitioned blocks of the current partition count (learned policy;
+    // null = all nb * 4 positions): entry = block * 4 + position
+    const int* pb_list;
+    const int* pb_inv;        // [nb * 4] list index of each position, -1 = not searched
+    const int* pb_count;      // entries in pb_list
+};
+
+// Candidate plan of the learned policy for one chunk (null: the plain
+// search, every block's 4 partition positions, all modes, 8 slots).
+struct SearchPlan {
+    const uint32_t* mode_mask;  // [nb][mask_words] trial modes each block may use
+    int mask_words;
+    const uint8_t* slots;       // [nb] candidates refined per partitioned block (1..8)
+    // builds the partition-count lists after k_partition (see launch_plan)
+    int* pb_list;               // [nb * 4]
+    int* pb_inv;                // [nb * 4]
+    int* pb_count;              // [3]
+    const int* topk_idx;
+    const float* topk_prob;
+    int kmax, seeds, modes, nslots;
+    float fallback_below, count_below;
+    uint8_t* fallback;          // [nb]
 };
 
 // pass01 output (ideal endpoints), per partitioned block; texel weights are
@@ -140,7 +162,7 @@ void upload_quant_tables(hipStream_t s);
 // pc_seed (optional, [3][A.num_blocks]): seed of each partition count's best
 // partitioning per block, -1 when that count produced no candidate.
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s, Profiler* prof, int16_t* pc_seed = nullptr);
+                        hipStream_t s, Profiler* prof, int16_t* pc_seed = nullptr, const SearchPlan* plan = nullptr);
 // ASTC LDR decode of a width x height image's blocks (row-major) to RGBA8.
 hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
                          hipStream_t s);

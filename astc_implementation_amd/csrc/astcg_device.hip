@@ -99,14 +99,29 @@ __device__ float atan2f_fixed(float y, float x) {
 
 // ---- block access ----------------------------------------------------------
 __device__ __forceinline__ TexDesc tex_of(const ImageArgs& A, int g) {
-    if (A.ntex == 1) return A.t0;
-    int lo = 0, hi = A.ntex - 1;  // last texture starting at or before block g
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (A.tex[mid].first_block <= g) lo = mid;
-        else hi = mid - 1;
+    // fields are selected one by one: a pointer that could point at either the
+    // kernel argument or the table would force the arguments into scratch
+    int lo = 0;
+    if (A.ntex > 1) {
+        int hi = A.ntex - 1;  // last texture starting at or before block g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (A.tex[mid].first_block <= g) lo = mid;
+            else hi = mid - 1;
+        }
     }
-    return A.tex[lo];
+    const bool one = A.ntex == 1;
+    const TexDesc* p = A.tex + lo;
+    TexDesc t;
+    t.rgba = one ? A.t0.rgba : p->rgba;
+    t.out = one ? A.t0.out : p->out;
+    t.width = one ? A.t0.width : p->width;
+    t.height = one ? A.t0.height : p->height;
+    t.blocks_x = one ? A.t0.blocks_x : p->blocks_x;
+    t.first_block = one ? A.t0.first_block : p->first_block;
+    t.num_blocks = one ? A.t0.num_blocks : p->num_blocks;
+    t.pad = 0;
+    return t;
 }
 
 // Texel origin of one block of a texture (edge blocks clamp to the last
@@ -160,10 +175,35 @@ struct Pb {
 
 __device__ __forceinline__ int texel_part(const Pb& b, int i) { return b.tp ? b.tp[i] : 0; }
 
+// Partitioned block pb of partition count pc -> chunk block k, candidate
+// position c (0..3) and its slot in pb_part ([block][4]).
+__device__ __forceinline__ void pb_decode(const ImageArgs& A, int pc, int pb, int* k, int* c, int* slot) {
+    if (pc == 1) {
+        *k = pb; *c = 0; *slot = pb;
+    } else if (A.pb_list) {
+        const int kc = A.pb_list[pb];
+        *k = kc >> 2; *c = kc & 3; *slot = kc;
+    } else {
+        *k = pb >> 2; *c = pb & 3; *slot = pb;
+    }
+}
+
+// Partitioned blocks in this launch (the learned policy's list length, or
+// the launch's full count).
+__device__ __forceinline__ int pb_limit(const ImageArgs& A, int pc, int npb) {
+    return (pc > 1 && A.pb_count) ? *A.pb_count : npb;
+}
+
+__device__ __forceinline__ int pb_slot_of(const ImageArgs& A, int pc, int pb) {
+    int k, c, slot;
+    pb_decode(A, pc, pb, &k, &c, &slot);
+    return slot;
+}
+
 __device__ Pb make_pb(const ImageArgs& A, const DevTables& T, const int16_t* pb_part, int pc, int pb) {
     Pb b;
-    const int npos = pc == 1 ? 1 : 4;
-    const int k = pb / npos, c = pb - k * npos;
+    int k, c, slot;
+    pb_decode(A, pc, pb, &k, &c, &slot);
     b.g = A.blk0 + k;
     b.loc = block_loc(A, T, b.g);
     b.pc = pc;
@@ -176,7 +216,7 @@ __device__ Pb make_pb(const ImageArgs& A, const DevTables& T, const int16_t* pb_
         b.ppc[0] = T.tc; b.ppc[1] = b.ppc[2] = b.ppc[3] = 0;
         b.seed = 0;
     } else {
-        const int e = pb_part[pb];
+        const int e = pb_part[slot];
         b.entry = e;
         const size_t row = (size_t)(pc - 2) * kPartitionings + e;
         b.tp = T.part_of_texel + row * T.tc;
@@ -583,7 +623,7 @@ __device__ __constant__ int kColorLevels[17] = {5, 7, 9, 11, 15, 19, 23, 31, 39,
 __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
                                                     Ideal* ideal_out, float* wts_out, float* ct_err, uint16_t* ct_fmt) {
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pb >= npb) return;
+    if (pb >= pb_limit(A, pc, npb)) return;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
     const int tc = T.tc;
     // pass01_ideal_endpoints_and_weights.wgsl:63-207
@@ -880,25 +920,30 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     const int grp = xcd + kXcds * (slot / ndm);
     const int m = slot - (slot / ndm) * ndm;
     const int pb0 = grp * kDmPbs;
-    if (pb0 >= npb) return;
+    const int npb_live = pb_limit(A, pc, npb);
+    if (pb0 >= npb_live) return;
     float* s_wts = reinterpret_cast<float*>(smem);
     float* s_dw = s_wts + kDmPbs * tc;
     float* s_aux = s_dw + 64 * wstride;
     uint8_t* s_tpart = reinterpret_cast<uint8_t*>(s_aux + 64 * wstride);
     {
-        const int nl = min(kDmPbs, npb - pb0);
+        const int nl = min(kDmPbs, npb_live - pb0);
         for (int e = lane; e < kDmPbs * tc; e += 64) {
             const int t = e / kDmPbs, l = e - t * kDmPbs, pb = pb0 + l;
             if (l >= nl) continue;
             s_wts[t * kDmPbs + l] = wts_all[(size_t)t * npb + pb];
             int p = 0;
-            if (pc > 1) p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[pb]) * tc + t];
+            if (pc > 1) {
+                int kb, cb, slot;
+                pb_decode(A, pc, pb, &kb, &cb, &slot);
+                p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[slot]) * tc + t];
+            }
             s_tpart[t * kDmPbs + l] = (uint8_t)p;
         }
     }
     __syncthreads();
     const int pb = pb0 + lane;
-    if (pb >= npb) return;
+    if (pb >= npb_live) return;
     const float* wts = s_wts + lane;  // wts[i * 64]
     // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
     const float4 wes4 = *reinterpret_cast<const float4*>(ideal[pb].wes);
@@ -920,8 +965,8 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     const float* w2tw = T.w2t_w + di.w2t_base;
     int pb_in_batch;
     {
-        const int npos = pc == 1 ? 1 : 4;
-        const int kb = pb / npos, c = pb - kb * npos;
+        int kb, c, slot;
+        pb_decode(A, pc, pb, &kb, &c, &slot);
         const int j = block_loc(A, T, A.blk0 + kb).local % kRefBatch;
         pb_in_batch = pc == 1 ? j : j * 4 + c;
     }
@@ -1151,17 +1196,28 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
 // Top-8 trial modes per partitioned block, plus the per-mode candidate counts
 // that size the refinement work list (LDS histogram, one global add per
 // workgroup and mode).  Slots without a candidate get the default error.
-__global__ void __launch_bounds__(64) k_top(int npb, int nbm, const Ccr* ccr, int16_t* top, int* bin_count,
-                                            CandOut* cand) {
+__global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr, int16_t* top,
+                                            int* bin_count, CandOut* cand, const uint32_t* mode_mask, int mask_words,
+                                            const uint8_t* slots) {
     extern __shared__ int s_hist[];  // [nbm]
     for (int i = threadIdx.x; i < nbm; i += 64) s_hist[i] = 0;
     __syncthreads();
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pb < npb) {
+    if (pb < pb_limit(A, pc, npb)) {
+        // learned policy: the block's allowed trial modes and slot count
+        const uint32_t* mask = nullptr;
+        int nslots = 8;
+        if (mode_mask || slots) {
+            int kb, cb, sl;
+            pb_decode(A, pc, pb, &kb, &cb, &sl);
+            if (mode_mask) mask = mode_mask + (size_t)kb * mask_words;
+            if (slots) nslots = slots[kb];
+        }
         float te[8];
         int ti[8];
         for (int i = 0; i < 8; ++i) { te[i] = kErrDefault; ti[i] = 0; }
         for (int k = 0; k < nbm; ++k) {
+            if (mask && !((mask[k >> 5] >> (k & 31)) & 1u)) continue;
             const float e = ccr[(size_t)k * npb + pb].total_error;
             if (e < te[7]) {
                 te[7] = e; ti[7] = k;
@@ -1173,7 +1229,7 @@ __global__ void __launch_bounds__(64) k_top(int npb, int nbm, const Ccr* ccr, in
             }
         }
         for (int i = 0; i < 8; ++i) {
-            const int v = te[i] < kErrDefault ? ti[i] : -1;
+            const int v = (te[i] < kErrDefault && i < nslots) ? ti[i] : -1;
             top[pb * 8 + i] = (int16_t)v;
             if (v >= 0) atomicAdd(&s_hist[v], 1);
             else cand[pb * 8 + i].error = kErrDefault;  // pass12 produced no candidate for this slot
@@ -1208,7 +1264,8 @@ __global__ void __launch_bounds__(64) k_bins(int nbm, int* bin_count, int* bin_c
 
 // Fills the buckets: positions are reserved per workgroup and mode, so the
 // order inside a bucket is arbitrary; every task writes its own output slot.
-__global__ void __launch_bounds__(64) k_scatter(int npb, int nbm, const int16_t* top, int* bin_cur, int* work) {
+__global__ void __launch_bounds__(64) k_scatter(ImageArgs A, int pc, int npb, int nbm, const int16_t* top, int* bin_cur,
+                                                int* work) {
     extern __shared__ int s_sc[];  // [2][nbm]
     int* s_cnt = s_sc;
     int* s_base = s_sc + nbm;
@@ -1217,7 +1274,7 @@ __global__ void __launch_bounds__(64) k_scatter(int npb, int nbm, const int16_t*
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     int kk[8], loc[8];
     for (int s = 0; s < 8; ++s) {
-        kk[s] = pb < npb ? (int)top[pb * 8 + s] : -1;
+        kk[s] = pb < pb_limit(A, pc, npb) ? (int)top[pb * 8 + s] : -1;
         loc[s] = kk[s] >= 0 ? atomicAdd(&s_cnt[kk[s]], 1) : 0;
     }
     __syncthreads();
@@ -1962,7 +2019,8 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
     float cur_err = 0.0f;
     int cur_pb = -1, cur_slot = -1;
     for (int c = 0; c < npos; ++c) {
-        const int pb = k * npos + c;
+        const int pb = (pc > 1 && A.pb_inv) ? A.pb_inv[k * 4 + c] : k * npos + c;
+        if (pb < 0) continue;  // position not searched (learned policy)
         float be = kErrDefault;
         int bs = -1;
         for (int s = 0; s < 8; ++s) {
@@ -1976,7 +2034,8 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
     // for the candidate predictor)
     if (pc_seed && pc > 1)
         pc_seed[(size_t)(pc - 2) * A.num_blocks + A.blk0 + k] =
-            cur_slot >= 0 ? (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[cur_pb]] : (int16_t)-1;
+            cur_slot >= 0 ? (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[pb_slot_of(A, pc, cur_pb)]]
+                          : (int16_t)-1;
     if (pc == 1) {
         sb.error = 1e30f;
         sb.raw_mode = 0xFFFF;
@@ -1987,7 +2046,9 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         sb.error = cur_err;
         sb.raw_mode = T.bm[co.bm].raw_mode;
         sb.pc = (uint8_t)pc;
-        sb.seed = pc == 1 ? 0 : T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[cur_pb]];
+        int kb, cb, slot;
+        pb_decode(A, pc, cur_pb, &kb, &cb, &slot);
+        sb.seed = pc == 1 ? 0 : T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[slot]];
         sb.matched = co.matched;
         sb.quant = co.quant;
         sb.formats = co.formats;
@@ -2133,6 +2194,83 @@ __global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const Sym
     }
     encode_ise(cq, nv, vals, w, pc == 1 ? 17 : 29);
     *o = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ---- learned candidate policy: per-chunk search plan ------------------------
+// Predictor outputs per block (top-k of its five heads): [nb][5][kmax]
+// indices and probabilities; heads are partition count (1..4 as 0..3),
+// seeds for 2 / 3 / 4 partitions, trial mode.
+__global__ void __launch_bounds__(64) k_plan_zero(int* counts) {
+    if (threadIdx.x < 3) counts[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, int nbm, uint32_t* mask, uint8_t* slots) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.nb) return;
+    const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
+    const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
+    const bool fb = tp[4 * P.kmax] < P.fallback_below;
+    P.fallback[k] = fb ? 1 : 0;
+    uint32_t* m = mask + (size_t)k * P.mask_words;
+    for (int w = 0; w < P.mask_words; ++w) {
+        const int lo = 32 * w;
+        m[w] = !fb ? 0u : (nbm - lo >= 32 ? 0xFFFFFFFFu : (nbm > lo ? (1u << (nbm - lo)) - 1u : 0u));
+    }
+    if (!fb)
+        for (int j = 0; j < P.kmax && j < P.modes; ++j) {
+            const int md = ti[4 * P.kmax + j];
+            if (md >= 0 && md < nbm) m[md >> 5] |= 1u << (md & 31);
+        }
+    slots[k] = fb ? 8 : (uint8_t)P.nslots;
+}
+
+// One lane per block: the partitioned blocks of partition count pc.  A
+// fallback block keeps the 4 compat positions k_partition wrote; another
+// block searches the predictor's top seeds of this count (when the count is
+// its most likely one or at least count_below probable), written into its
+// pb_part slots 0..n-1.  Entries are appended to the list through one
+// atomic per workgroup; their order does not affect any result.
+__global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, int16_t* pb_part, SearchPlan P,
+                                             int* count) {
+    __shared__ int s_n, s_base;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    int n = 0;
+    if (k < A.nb) {
+        const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
+        const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
+        if (P.fallback[k]) {
+            n = 4;
+        } else {
+            float pp = 0.0f;
+            bool first = false;
+            for (int j = 0; j < P.kmax && j < 4; ++j)
+                if (ti[j] == pc - 1) { pp = tp[j]; first = j == 0; }
+            if (first || pp >= P.count_below) {
+                for (int j = 0; j < P.kmax && n < P.seeds && n < 4; ++j) {
+                    const int seed = ti[(pc - 1) * P.kmax + j];
+                    if (seed < 0 || seed >= kPartitionings) break;
+                    const int e = T.seed_to_packed[(pc - 2) * kPartitionings + seed];
+                    if (e >= T.count_selected[pc - 1]) continue;  // not a selected partitioning
+                    pb_part[k * 4 + n] = (int16_t)e;
+                    ++n;
+                }
+            }
+        }
+    }
+    const int local = n ? atomicAdd(&s_n, n) : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_n ? atomicAdd(count, s_n) : 0;
+    __syncthreads();
+    if (k < A.nb) {
+        for (int c = 0; c < 4; ++c) P.pb_inv[k * 4 + c] = -1;
+        for (int c = 0; c < n; ++c) {
+            const int idx = s_base + local + c;
+            P.pb_list[idx] = k * 4 + c;
+            P.pb_inv[k * 4 + c] = idx;
+        }
+    }
 }
 
 // ---- ASTC LDR decoder (Khronos ASTC specification, 2D LDR profile) ---------
@@ -2458,17 +2596,36 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
     else launch_refine_pc<false>(pc, grid, lds, s, A, T, B, npb);
 }
 
-hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s, Profiler* prof, int16_t* pc_seed) {
+hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
+                        hipStream_t s, Profiler* prof, int16_t* pc_seed, const SearchPlan* plan) {
     const int threads = 64;
     auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
+    const uint32_t* mode_mask = nullptr;
+    const uint8_t* slots = nullptr;
+    int mask_words = 0;
+    if (plan) {
+        hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(64), 0, s, plan->pb_count);
+        hipLaunchKernelGGL(k_plan_modes, dim3(blocks_for(A0.nb)), dim3(64), 0, s, A0, *plan, T.nbm,
+                           const_cast<uint32_t*>(plan->mode_mask), const_cast<uint8_t*>(plan->slots));
+        mode_mask = plan->mode_mask;
+        slots = plan->slots;
+        mask_words = plan->mask_words;
+    }
     for (int pc = 1; pc <= max_pc; ++pc) {
         const int npos = pc == 1 ? 1 : 4;
-        const int npb = A.nb * npos;
+        const int npb = A0.nb * npos;
+        ImageArgs A = A0;
         if (pc > 1) {
             if (prof) prof->begin(K_PARTITION, s);
             hipLaunchKernelGGL(k_partition, dim3((A.nb + kPartBlocks - 1) / kPartBlocks), dim3(64),
                                partition_lds_bytes(T), s, A, T, pc, B.pb_part);
+            if (plan) {
+                hipLaunchKernelGGL(k_plan, dim3(blocks_for(A.nb)), dim3(64), 0, s, A, T, pc, B.pb_part, *plan,
+                                   plan->pb_count + (pc - 2));
+                A.pb_list = plan->pb_list;
+                A.pb_inv = plan->pb_inv;
+                A.pb_count = plan->pb_count + (pc - 2);
+            }
             if (prof) prof->end(K_PARTITION, s);
         }
         ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err,
@@ -2483,11 +2640,11 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
         {
             if (prof) prof->begin(K_TOP, s);
             const size_t hist = (size_t)T.nbm * sizeof(int);
-            hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, npb, T.nbm, B.ccr, B.top,
-                               B.bin_count, B.cand);
+            hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, A, pc, npb, T.nbm, B.ccr, B.top,
+                               B.bin_count, B.cand, mode_mask, mask_words, slots);
             hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), hist + sizeof(int), s, T.nbm, B.bin_count, B.bin_cur,
                                B.work, B.work_total);
-            hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, npb, T.nbm, B.top,
+            hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, A, pc, npb, T.nbm, B.top,
                                B.bin_cur, B.work);
             if (prof) prof->end(K_TOP, s);
         }

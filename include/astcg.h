@@ -40,10 +40,29 @@ typedef enum {
 
 typedef struct astcg_encoder astcg_encoder;
 
-/* Candidate policy for partition counts 2..4.  COMPAT reproduces the
- * reference's effective selection (packed table positions chosen by passes
- * 001-007, including their indexing quirks). */
-typedef enum { ASTCG_POLICY_COMPAT = 0 } astcg_policy;
+/* Candidate policy.  COMPAT reproduces the reference's search (packed table
+ * positions chosen by passes 001-007 including their indexing quirks, all
+ * trial modes, 8 refined candidates).  LEARNED prunes it with a trained
+ * candidate predictor (astcg_set_predictor): no counterpart in the
+ * reference. */
+typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1 } astcg_policy;
+
+/* Pruning of the LEARNED policy.  A block whose top trial-mode probability
+ * is below fallback_below gets the full COMPAT search; any other block
+ * searches the `seeds` most likely partitionings of every partition count
+ * that is its most likely count or at least count_below probable, only its
+ * `modes` most likely trial modes, and refines `slots` candidates per
+ * partitioned block instead of 8. */
+typedef struct {
+    uint32_t seeds;        /* 1..4 */
+    uint32_t modes;        /* 1..64 */
+    uint32_t slots;        /* 1..8 */
+    float fallback_below;  /* default 0.3 */
+    float count_below;     /* default 0.1 */
+    uint32_t reserved[3];
+} astcg_learned_params;
+
+void astcg_default_learned_params(astcg_learned_params* p);
 
 #define ASTCG_FLAG_PROFILE 1u  /* record HIP events around every kernel launch */
 #define ASTCG_FLAG_LABELS 2u   /* keep each partition count's best partitioning (astcg_get_labels) */
@@ -127,6 +146,19 @@ astcg_status astcg_encode_batch_device(astcg_encoder* enc, const astcg_texture* 
 /* Optional: copy the chosen symbolic blocks of the last encode (host memory,
  * blocks_x*blocks_y entries). */
 astcg_status astcg_get_symbolic(astcg_encoder* enc, astcg_symbolic_block* out, size_t count);
+
+/* Installs the candidate predictor of footprint block_x x block_y used by
+ * the LEARNED policy: an MLP nin -> hidden -> hidden -> nout with GELU
+ * (nin = 3 * block_x * block_y, nout = 4 + 3 * 1024 + trial modes), given as
+ * the fp32 concatenation W1 [nin x hidden], b1 [hidden], W2 [hidden x hidden],
+ * b2 [hidden], W3 [hidden x nout], b3 [nout] (row-major, `count` floats), and
+ * its pruning parameters (NULL = defaults). */
+astcg_status astcg_set_predictor(astcg_encoder* enc, uint32_t block_x, uint32_t block_y, uint32_t hidden,
+                                 const float* weights, size_t count, const astcg_learned_params* lp);
+
+/* Fraction of the blocks of the last LEARNED encode that fell back to the
+ * full search. */
+astcg_status astcg_fallback_rate(astcg_encoder* enc, double* rate);
 
 /* Search outcome per block of the last single-image encode made with
  * ASTCG_FLAG_LABELS: 6 int32 per block = {partition count, partition seed,

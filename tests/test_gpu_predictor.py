@@ -175,3 +175,39 @@ def test_search_labels_are_consistent():
             assert lab[i, 3 + pc - 2] == lab[i, 1]
         assert all(0 <= lab[i, 3 + j] < 1024 for j in range(3))
     enc.close()
+
+
+def test_learned_policy_with_full_fallback_equals_plain():
+    """Every block below the fallback threshold: the learned policy's plan
+    (predictor inference, partition lists built in arbitrary order, mode
+    masks, slot counts) must reproduce the plain search bit for bit."""
+    P = Predictor((4, 4), seed=1)
+    enc = A.Encoder(0)
+    img = synth.gradient_image(120, 92, 31)
+    enc.configure(120, 92, 4, 4)
+    plain = enc.encode(img)
+    enc.set_predictor(4, 4, [w.cpu().numpy() for w in P.params], fallback_below=2.0)
+    enc.configure(120, 92, 4, 4, policy=A.POLICY_LEARNED, chunk_blocks=300)
+    learned = enc.encode(img)
+    assert enc.fallback_rate() == 1.0
+    assert np.array_equal(learned, plain)
+    out = enc.encode_batch([img, img[:40, :50]], [(4, 4), (4, 4)], policy=A.POLICY_LEARNED)
+    assert np.array_equal(out[0], plain)
+    enc.close()
+
+
+def test_learned_policy_without_fallback_is_valid():
+    """No fallback, the predictor's own candidates only: every block is a
+    legal ASTC block (decodes without the error colour) of sane quality."""
+    P = Predictor((4, 4), seed=2)
+    enc = A.Encoder(0)
+    img = synth.gradient_image(96, 64, 8)
+    enc.set_predictor(4, 4, [w.cpu().numpy() for w in P.params], fallback_below=0.0, seeds=2, modes=16, slots=2)
+    enc.configure(96, 64, 4, 4, policy=A.POLICY_LEARNED)
+    blocks = enc.encode(img)
+    assert enc.fallback_rate() == 0.0
+    dec = enc.decode(4, 4, 96, 64, blocks)
+    magenta = np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1)
+    assert not magenta.any()
+    assert A.psnr(dec, img) > 30.0
+    enc.close()
