@@ -38,7 +38,7 @@ struct PredState {
     int nin = 0, hidden = 0, nout = 0, nmodes = 0;
     void* w16[3] = {nullptr, nullptr, nullptr};  // bf16 W1, W2, W3
     float* bias[3] = {nullptr, nullptr, nullptr};
-    int* d_head = nullptr;  // [10]: head offsets then lengths
+    int* d_head = nullptr;  // [15]: head offsets, lengths, per-head top-k counts
     astcg_learned_params lp{};
 };
 
@@ -400,6 +400,7 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
 }
 
 constexpr int kInferBlocks = 32768;  // predictor inference sub-chunk
+typedef const uint8_t* bfloat16_ptr;    // bf16 weights addressed bytewise
 
 // LEARNED policy: runs the predictor over the chunk's blocks and lays out the
 // search plan (top-k of every head per block, then the plan kernels).
@@ -449,10 +450,21 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
         if (st == ASTCG_OK) st = astcg_mlp_gelu(Z1, H1, (long long)n * H, s);
         if (st == ASTCG_OK) st = astcg_mlp_gemm(n, H, H, H1, H, 0, P.w16[1], H, 0, Z2, H, P.bias[1], 0, s);
         if (st == ASTCG_OK) st = astcg_mlp_gelu(Z2, H2, (long long)n * H, s);
-        if (st == ASTCG_OK) st = astcg_mlp_gemm(n, NO, H, H2, H, 0, P.w16[2], NO, 0, LG, NO, P.bias[2], 0, s);
+        // output layer: only the heads the plan reads (the seed heads are 3072
+        // of the columns and unused with the compat positions)
+        const int mode_off = 4 + 3 * 1024;
+        const bfloat16_ptr w3 = static_cast<const uint8_t*>(P.w16[2]);
+        if (lp.seeds == 0) {
+            if (st == ASTCG_OK) st = astcg_mlp_gemm(n, 4, H, H2, H, 0, w3, NO, 0, LG, NO, P.bias[2], 0, s);
+            if (st == ASTCG_OK)
+                st = astcg_mlp_gemm(n, P.nmodes, H, H2, H, 0, w3 + 2 * (size_t)mode_off, NO, 0, LG + mode_off, NO,
+                                    P.bias[2] + mode_off, 0, s);
+        } else if (st == ASTCG_OK) {
+            st = astcg_mlp_gemm(n, NO, H, H2, H, 0, w3, NO, 0, LG, NO, P.bias[2], 0, s);
+        }
         if (st == ASTCG_OK)
-            st = astcg_mlp_topk(LG, n, NO, 5, P.d_head, P.d_head + 5, kmax, tidx + (size_t)b0 * 5 * kmax,
-                                tprob + (size_t)b0 * 5 * kmax, s);
+            st = astcg_mlp_topk_heads(LG, n, NO, 5, P.d_head, P.d_head + 5, kmax, P.d_head + 10,
+                                      tidx + (size_t)b0 * 5 * kmax, tprob + (size_t)b0 * 5 * kmax, s);
         if (st != ASTCG_OK) return fail(e, st, "predictor inference");
     }
     SearchPlan& pl = *plan;
@@ -874,11 +886,11 @@ astcg_status astcg_get_labels(astcg_encoder* e, int32_t* out, size_t count) {
 void astcg_default_learned_params(astcg_learned_params* p) {
     if (!p) return;
     std::memset(p, 0, sizeof(*p));
-    p->seeds = 2;
+    p->seeds = 0;
     p->modes = 16;
-    p->slots = 4;
-    p->fallback_below = 0.3f;
-    p->count_below = 0.1f;
+    p->slots = 5;
+    p->fallback_below = 0.5f;
+    p->count_below = 0.0f;
 }
 
 astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uint32_t hidden, const float* weights,
@@ -888,7 +900,7 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     astcg_learned_params q;
     astcg_default_learned_params(&q);
     if (lp) q = *lp;
-    if (q.seeds < 1 || q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8)
+    if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     const Footprint* f = footprint((int)bx, (int)by);
     const int nin = 3 * (int)(bx * by), H = (int)hidden, nout = 4 + 3 * 1024 + f->nbm;
@@ -928,7 +940,9 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
         if (err == hipSuccess) err = hipMemcpy(P.bias[i], weights + off + wn, bn * sizeof(float), hipMemcpyHostToDevice);
         off += wn + bn;
     }
-    const int head[10] = {0, 4, 4 + 1024, 4 + 2048, 4 + 3072, 4, 1024, 1024, 1024, f->nbm};
+    const int sk = (int)q.seeds;
+    const int head[15] = {0, 4, 4 + 1024, 4 + 2048, 4 + 3072, 4, 1024, 1024, 1024, f->nbm,
+                          4, sk, sk, sk, (int)std::min<uint32_t>(q.modes, (uint32_t)f->nbm)};
     if (err == hipSuccess) err = hipMalloc(&P.d_head, sizeof(head));
     if (err == hipSuccess) err = hipMemcpy(P.d_head, head, sizeof(head), hipMemcpyHostToDevice);
     if (tmp) (void)hipFree(tmp);

@@ -912,7 +912,8 @@ unsigned dm_bm_grid(const DevTables& T, int npb) {
 __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
                                               const Ideal* ideal, const float* wts_all, const float* ct_err,
                                               const uint16_t* ct_fmt, Ccr* ccr,
-                                              uint8_t* qw_out, int wstride) {
+                                              uint8_t* qw_out, int wstride,
+                                              const uint32_t* mode_mask, int mask_words) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int ndm = T.ndm, tc = T.tc, lane = threadIdx.x;
     const int wg = blockIdx.x;
@@ -944,6 +945,20 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     __syncthreads();
     const int pb = pb0 + lane;
     if (pb >= npb_live) return;
+    // learned policy: only the block's allowed trial modes are evaluated (k_top
+    // never reads the others); a decimation mode none of them uses is skipped
+    const uint32_t* mask = nullptr;
+    if (mode_mask) {
+        int kb, cb, slot;
+        pb_decode(A, pc, pb, &kb, &cb, &slot);
+        mask = mode_mask + (size_t)kb * mask_words;
+        bool any = false;
+        for (int li = T.dm_bm_off[m]; li < T.dm_bm_off[m + 1]; ++li) {
+            const int k = T.dm_bm_list[li];
+            any |= ((mask[k >> 5] >> (k & 31)) & 1u) != 0u;
+        }
+        if (!any) return;
+    }
     const float* wts = s_wts + lane;  // wts[i * 64]
     // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
     const float4 wes4 = *reinterpret_cast<const float4*>(ideal[pb].wes);
@@ -1107,6 +1122,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     auto ct_at = [&](int q, int col) { return ((size_t)(q - 4) * 13 + col) * npb + pb; };
     for (int li = T.dm_bm_off[m]; li < T.dm_bm_off[m + 1]; ++li) {
         const int k = T.dm_bm_list[li];
+        if (mask && !((mask[k >> 5] >> (k & 31)) & 1u)) continue;
         const BmInfo bmi = T.bm[k];
         Ccr out;
         out.ql = -1; out.ql_mod = -1; out.formats = 0;
@@ -2209,7 +2225,13 @@ __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, in
     if (k >= A.nb) return;
     const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
     const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
-    const bool fb = tp[4 * P.kmax] < P.fallback_below;
+    // confidence: the probability mass of the trial modes the block keeps
+    // (the top-k kernel fills min(modes, nbm) entries of the mode head)
+    const int nkeep = min(min(P.kmax, P.modes), nbm);
+    float kept = 0.0f;
+    for (int j = 0; j < nkeep; ++j)
+        if (ti[4 * P.kmax + j] >= 0) kept += tp[4 * P.kmax + j];
+    const bool fb = kept < P.fallback_below;
     P.fallback[k] = fb ? 1 : 0;
     uint32_t* m = mask + (size_t)k * P.mask_words;
     for (int w = 0; w < P.mask_words; ++w) {
@@ -2217,7 +2239,7 @@ __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, in
         m[w] = !fb ? 0u : (nbm - lo >= 32 ? 0xFFFFFFFFu : (nbm > lo ? (1u << (nbm - lo)) - 1u : 0u));
     }
     if (!fb)
-        for (int j = 0; j < P.kmax && j < P.modes; ++j) {
+        for (int j = 0; j < nkeep; ++j) {
             const int md = ti[4 * P.kmax + j];
             if (md >= 0 && md < nbm) m[md >> 5] |= 1u << (md & 31);
         }
@@ -2247,8 +2269,10 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
             bool first = false;
             for (int j = 0; j < P.kmax && j < 4; ++j)
                 if (ti[j] == pc - 1) { pp = tp[j]; first = j == 0; }
-            if (first || pp >= P.count_below) {
-                for (int j = 0; j < P.kmax && n < P.seeds && n < 4; ++j) {
+            if ((first || pp >= P.count_below) && P.seeds == 0) {
+                n = 4;  // the compat positions k_partition wrote
+            } else if (first || pp >= P.count_below) {
+                for (int j = 0; j < P.kmax && j < P.seeds && n < 4; ++j) {
                     const int seed = ti[(pc - 1) * P.kmax + j];
                     if (seed < 0 || seed >= kPartitionings) break;
                     const int e = T.seed_to_packed[(pc - 2) * kPartitionings + seed];
@@ -2634,7 +2658,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             if (prof) prof->begin(K_DM_BM, s);
             hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(T, npb)), dim3(64), dm_bm_lds_bytes(T, B.wstride),
                                s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw,
-                               B.wstride);
+                               B.wstride, mode_mask, mask_words);
             if (prof) prof->end(K_DM_BM, s);
         }
         {

@@ -199,9 +199,12 @@ __global__ void k_features(const uint8_t* rgba, int width, int height, int bx, i
 // Top-k indices of each head of each row (one wave per row and head):
 // repeated arg-max with the lowest index winning ties.
 __global__ void __launch_bounds__(64) k_topk(const float* logits, int rows, int ld, int nheads, const int* head_off,
-                                             const int* head_len, int k, int* out_idx, float* out_prob) {
+                                             const int* head_len, int kmax, const int* head_k, int* out_idx,
+                                             float* out_prob) {
     const int r = blockIdx.x, h = blockIdx.y, lane = threadIdx.x;
     if (r >= rows || h >= nheads) return;
+    const int k = head_k ? min(head_k[h], kmax) : kmax;
+    if (k <= 0) return;
     const float* x = logits + (size_t)r * ld + head_off[h];
     const int len = head_len[h];
     float mx = -3.4e38f;
@@ -227,8 +230,8 @@ __global__ void __launch_bounds__(64) k_topk(const float* logits, int rows, int 
             if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
         }
         if (lane == 0) {
-            out_idx[((size_t)r * nheads + h) * k + j] = bi < len ? bi : -1;
-            if (out_prob) out_prob[((size_t)r * nheads + h) * k + j] = bi < len ? expf(bv - mx) / se : 0.0f;
+            out_idx[((size_t)r * nheads + h) * kmax + j] = bi < len ? bi : -1;
+            if (out_prob) out_prob[((size_t)r * nheads + h) * kmax + j] = bi < len ? expf(bv - mx) / se : 0.0f;
         }
         prev_v = bv;
         prev_i = bi;
@@ -322,11 +325,17 @@ astcg_status astcg_mlp_features(const uint8_t* d_rgba, int width, int height, in
 
 astcg_status astcg_mlp_topk(const float* logits, int rows, int ld, int nheads, const int* head_off,
                             const int* head_len, int k, int* out_idx, float* out_prob, void* stream) {
-    if (!logits || rows < 0 || nheads <= 0 || !head_off || !head_len || k <= 0 || !out_idx)
+    return astcg_mlp_topk_heads(logits, rows, ld, nheads, head_off, head_len, k, nullptr, out_idx, out_prob, stream);
+}
+
+astcg_status astcg_mlp_topk_heads(const float* logits, int rows, int ld, int nheads, const int* head_off,
+                                  const int* head_len, int kmax, const int* head_k, int* out_idx, float* out_prob,
+                                  void* stream) {
+    if (!logits || rows < 0 || nheads <= 0 || !head_off || !head_len || kmax <= 0 || !out_idx)
         return ASTCG_ERR_INVALID_ARGUMENT;
     if (rows)
         hipLaunchKernelGGL(k_topk, dim3(rows, nheads), dim3(64), 0, static_cast<hipStream_t>(stream), logits, rows, ld,
-                           nheads, head_off, head_len, k, out_idx, out_prob);
+                           nheads, head_off, head_len, kmax, head_k, out_idx, out_prob);
     return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
 }
 

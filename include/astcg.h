@@ -47,18 +47,20 @@ typedef struct astcg_encoder astcg_encoder;
  * reference. */
 typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1 } astcg_policy;
 
-/* Pruning of the LEARNED policy.  A block whose top trial-mode probability
- * is below fallback_below gets the full COMPAT search; any other block
- * searches the `seeds` most likely partitionings of every partition count
- * that is its most likely count or at least count_below probable, only its
+/* Pruning of the LEARNED policy.  A block whose kept trial modes (its
+ * `modes` most likely) hold less than fallback_below of the predicted
+ * probability gets the full COMPAT search; any other block searches only the
+ * partition counts that are its most likely count or
+ * at least count_below probable (with the COMPAT positions when seeds is 0,
+ * else its `seeds` most likely partitionings of each such count), only its
  * `modes` most likely trial modes, and refines `slots` candidates per
  * partitioned block instead of 8. */
 typedef struct {
-    uint32_t seeds;        /* 1..4 */
-    uint32_t modes;        /* 1..64 */
-    uint32_t slots;        /* 1..8 */
-    float fallback_below;  /* default 0.3 */
-    float count_below;     /* default 0.1 */
+    uint32_t seeds;        /* 0 (COMPAT positions, default) or 1..4 */
+    uint32_t modes;        /* 1..64, default 16 */
+    uint32_t slots;        /* 1..8, default 5 */
+    float fallback_below;  /* default 0.5 */
+    float count_below;     /* default 0 (every partition count) */
     uint32_t reserved[3];
 } astcg_learned_params;
 

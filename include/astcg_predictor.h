@@ -63,6 +63,12 @@ astcg_status astcg_mlp_features(const uint8_t* d_rgba, int width, int height, in
 astcg_status astcg_mlp_topk(const float* logits, int rows, int ld, int nheads, const int* head_off,
                             const int* head_len, int k, int* out_idx, float* out_prob, void* stream);
 
+/* astcg_mlp_topk with a per-head count: head h keeps head_k[h] (<= kmax,
+ * device array; 0 skips the head) entries of its kmax-wide output slot. */
+astcg_status astcg_mlp_topk_heads(const float* logits, int rows, int ld, int nheads, const int* head_off,
+                                  const int* head_len, int kmax, const int* head_k, int* out_idx, float* out_prob,
+                                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif
