@@ -61,6 +61,7 @@ class Params(ctypes.Structure):
 
 FLAG_PROFILE = 1
 FLAG_LABELS = 2
+FLAG_VOID_EXTENT = 4
 
 
 class Texture(ctypes.Structure):
@@ -84,7 +85,8 @@ class LearnedParams(ctypes.Structure):
         ("slots", ctypes.c_uint32),
         ("fallback_below", ctypes.c_float),
         ("count_below", ctypes.c_float),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("partitionings", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -125,6 +127,7 @@ def library():
         "astcg_encode_batch": (i32, [vp, vp, u32, vp]),
         "astcg_decode_device": (i32, [vp, u32, u32, u32, u32, vp, vp, vp]),
         "astcg_get_labels": (i32, [vp, vp, sz]),
+        "astcg_get_candidates": (i32, [vp, vp, vp, sz]),
         "astcg_default_learned_params": (None, [vp]),
         "astcg_set_predictor": (i32, [vp, u32, u32, u32, vp, sz, vp]),
         "astcg_fallback_rate": (i32, [vp, ctypes.POINTER(ctypes.c_double)]),
@@ -187,14 +190,15 @@ class Encoder:
             pass
 
     def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False, labels=False,
-                  policy=POLICY_COMPAT):
+                  policy=POLICY_COMPAT, void_extent=False):
         """secondaryInit (compression_pipeline.cpp:263-321)."""
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
         p.policy = policy
-        p.flags = (FLAG_PROFILE if profile else 0) | (FLAG_LABELS if labels else 0)
+        p.flags = ((FLAG_PROFILE if profile else 0) | (FLAG_LABELS if labels else 0) |
+                   (FLAG_VOID_EXTENT if void_extent else 0))
         _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
         self.width, self.height, self.bx, self.by = width, height, bx, by
 
@@ -217,7 +221,7 @@ class Encoder:
                                              ctypes.c_void_p(stream or 0)), self._h)
 
     def encode_batch(self, images, footprints, max_partitions=4, chunk_blocks=0, profile=False,
-                     policy=POLICY_COMPAT):
+                     policy=POLICY_COMPAT, void_extent=False):
         """Encodes several RGBA8 images (H x W x 4 uint8 arrays) with their own
         footprints [(bx, by), ...] in one batched call; returns the block
         arrays in input order."""
@@ -232,12 +236,12 @@ class Encoder:
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
-        p.flags = FLAG_PROFILE if profile else 0
+        p.flags = (FLAG_PROFILE if profile else 0) | (FLAG_VOID_EXTENT if void_extent else 0)
         p.policy = policy
         _check(library().astcg_encode_batch(self._h, tex, len(imgs), ctypes.byref(p)), self._h)
         return outs
 
-    def encode_batch_device(self, textures, max_partitions=4, stream=None):
+    def encode_batch_device(self, textures, max_partitions=4, stream=None, policy=POLICY_COMPAT, profile=False):
         """textures: [(d_rgba, width, height, bx, by, d_out, out_len)] with
         device addresses; enqueued on `stream`."""
         tex = (Texture * max(len(textures), 1))()
@@ -246,6 +250,8 @@ class Encoder:
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
+        p.flags = FLAG_PROFILE if profile else 0
+        p.policy = policy
         _check(library().astcg_encode_batch_device(self._h, tex, len(textures), ctypes.byref(p),
                                                    ctypes.c_void_p(stream or 0)), self._h)
 
@@ -275,7 +281,8 @@ class Encoder:
     def set_predictor(self, bx, by, params, hidden=256, **learned):
         """Installs predictor weights (six fp32 arrays W1, b1, W2, b2, W3, b3)
         for footprint bx x by; keyword arguments override the pruning
-        defaults (seeds, modes, slots, fallback_below, count_below)."""
+        defaults (seeds, modes, slots, fallback_below, count_below,
+        partitionings)."""
         flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
         lp = LearnedParams()
         library().astcg_default_learned_params(ctypes.byref(lp))
@@ -283,6 +290,14 @@ class Encoder:
             setattr(lp, k, v)
         _check(library().astcg_set_predictor(self._h, bx, by, hidden, _ptr(flat), flat.size, ctypes.byref(lp)),
                self._h)
+
+    def load_predictor(self, bx, by, path=None, **learned):
+        """Installs a saved predictor (default: the package's
+        predictor_<bx>x<by>.npz) with the pruning settings saved with it;
+        keyword arguments override them."""
+        params, hidden, prune = load_predictor_file(path or predictor_path(bx, by))
+        prune.update(learned)
+        self.set_predictor(bx, by, params, hidden=hidden, **prune)
 
     def fallback_rate(self):
         r = ctypes.c_double()
@@ -296,11 +311,44 @@ class Encoder:
         _check(library().astcg_get_labels(self._h, _ptr(out), self.num_blocks), self._h)
         return out
 
+    def candidates(self):
+        """(seeds, errors) of the 4 compat positions of every partition count
+        in the last encode made with labels=True: (num_blocks, 3, 4) int16
+        and float32 (-1 / 1e30 where a position was not searched)."""
+        n = self.num_blocks
+        seeds = np.zeros((n, 3, 4), np.int16)
+        errors = np.zeros((n, 3, 4), np.float32)
+        _check(library().astcg_get_candidates(self._h, _ptr(seeds), _ptr(errors), n), self._h)
+        return seeds, errors
+
     def symbolic(self):
         n = self.num_blocks
         arr = (SymbolicBlock * n)()
         _check(library().astcg_get_symbolic(self._h, arr, n), self._h)
         return arr
+
+
+def predictor_path(bx, by):
+    """The package's trained predictor for footprint bx x by."""
+    return os.path.join(_HERE, "predictor_%dx%d.npz" % (bx, by))
+
+
+def load_predictor_file(path):
+    """(params, hidden, pruning) of a saved predictor: the six fp32 arrays
+    W1, b1, W2, b2, W3, b3, the hidden width, and the LearnedParams fields
+    tuned for it (empty when the file carries none)."""
+    z = np.load(path, allow_pickle=False)
+    params = [z["p%d" % i].astype(np.float32) for i in range(6)]
+    prune = {}
+    if "prune_i" in z.files:
+        seeds, modes, slots = (int(v) for v in z["prune_i"])
+        prune.update(seeds=seeds, modes=modes, slots=slots)
+    if "prune_f" in z.files:
+        fb, cb = (float(v) for v in z["prune_f"])
+        prune.update(fallback_below=fb, count_below=cb)
+    if "prune_p" in z.files:
+        prune.update(partitionings=int(z["prune_p"]))
+    return params, int(z["hidden"]), prune
 
 
 def symbolic_to_physical(bx, by, sb):

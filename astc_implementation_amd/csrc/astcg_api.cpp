@@ -39,8 +39,22 @@ struct PredState {
     void* w16[3] = {nullptr, nullptr, nullptr};  // bf16 W1, W2, W3
     float* bias[3] = {nullptr, nullptr, nullptr};
     int* d_head = nullptr;  // [15]: head offsets, lengths, per-head top-k counts
+    void* w3_seed = nullptr;  // bf16 [3][1024][hidden]: seed-head columns of W3, transposed
     astcg_learned_params lp{};
 };
+
+void free_pred(PredState& P) {
+    for (int i = 0; i < 3; ++i) {
+        if (P.w16[i]) (void)hipFree(P.w16[i]);
+        if (P.bias[i]) (void)hipFree(P.bias[i]);
+        P.w16[i] = nullptr;
+        P.bias[i] = nullptr;
+    }
+    if (P.d_head) (void)hipFree(P.d_head);
+    if (P.w3_seed) (void)hipFree(P.w3_seed);
+    P.d_head = nullptr;
+    P.w3_seed = nullptr;
+}
 
 struct astcg_encoder {
     int device = 0;
@@ -74,7 +88,7 @@ struct astcg_encoder {
     void* d_plan = nullptr;          // search plan buffers
     size_t plan_cap = 0;
     long long fb_blocks = 0, fb_total = 0;
-    int16_t* d_pcseed = nullptr;  // [3][nblocks] with ASTCG_FLAG_LABELS
+    void* d_pcseed = nullptr;  // label buffers (LabelBufs) with ASTCG_FLAG_LABELS
     size_t pcseed_cap = 0;
     struct EventProfiler* prof = nullptr;
 };
@@ -147,13 +161,7 @@ void release_all(astcg_encoder* e) {
     if (e->d_out) (void)hipFree(e->d_out);
     if (e->d_tex) (void)hipFree(e->d_tex);
     if (e->d_pcseed) (void)hipFree(e->d_pcseed);
-    for (auto& kv : e->preds) {
-        for (int i = 0; i < 3; ++i) {
-            if (kv.second.w16[i]) (void)hipFree(kv.second.w16[i]);
-            if (kv.second.bias[i]) (void)hipFree(kv.second.bias[i]);
-        }
-        if (kv.second.d_head) (void)hipFree(kv.second.d_head);
-    }
+    for (auto& kv : e->preds) free_pred(kv.second);
     e->preds.clear();
     if (e->d_pwork) (void)hipFree(e->d_pwork);
     if (e->d_plan) (void)hipFree(e->d_plan);
@@ -399,6 +407,18 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
     return ASTCG_OK;
 }
 
+// Label buffers of ASTCG_FLAG_LABELS over nb blocks, in one allocation:
+// candidate errors (float [3][4][nb]), best seeds, candidate seeds (int16).
+size_t label_bytes(size_t nb) { return nb * (12 * sizeof(float) + 15 * sizeof(int16_t)); }
+
+LabelBufs label_bufs(void* base, size_t nb) {
+    LabelBufs l;
+    l.cand_err = static_cast<float*>(base);
+    l.best_seed = reinterpret_cast<int16_t*>(l.cand_err + 12 * nb);
+    l.cand_seed = l.best_seed + 3 * nb;
+    return l;
+}
+
 constexpr int kInferBlocks = 32768;  // predictor inference sub-chunk
 typedef const uint8_t* bfloat16_ptr;    // bf16 weights addressed bytewise
 
@@ -415,6 +435,9 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     const size_t o_mask = L.add(sizeof(uint32_t) * (size_t)nb * words), o_slots = L.add((size_t)nb);
     const size_t o_fb = L.add((size_t)nb), o_list = L.add(sizeof(int) * (size_t)nb * 4);
     const size_t o_inv = L.add(sizeof(int) * (size_t)nb * 4), o_cnt = L.add(sizeof(int) * 4);
+    // ranking the compat positions needs every block's second hidden layer
+    const bool rank = P.w3_seed != nullptr;
+    const size_t o_h2all = rank ? L.add(2 * (size_t)nb * P.hidden) : 0;
     hipError_t err = grow(&e->d_plan, &e->plan_cap, L.size);
     if (err != hipSuccess) return hip_fail(e, err, "plan buffers");
     uint8_t* pb = static_cast<uint8_t*>(e->d_plan);
@@ -449,6 +472,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
         astcg_status st = astcg_mlp_gemm(n, H, P.nin, X, P.nin, 0, P.w16[0], H, 0, Z1, H, P.bias[0], 0, s);
         if (st == ASTCG_OK) st = astcg_mlp_gelu(Z1, H1, (long long)n * H, s);
         if (st == ASTCG_OK) st = astcg_mlp_gemm(n, H, H, H1, H, 0, P.w16[1], H, 0, Z2, H, P.bias[1], 0, s);
+        if (rank) H2 = pb + o_h2all + 2 * (size_t)b0 * H;
         if (st == ASTCG_OK) st = astcg_mlp_gelu(Z2, H2, (long long)n * H, s);
         // output layer: only the heads the plan reads (the seed heads are 3072
         // of the columns and unused with the compat positions)
@@ -484,6 +508,11 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     pl.fallback_below = lp.fallback_below;
     pl.count_below = lp.count_below;
     pl.fallback = pb + o_fb;
+    pl.keep_pb = rank ? (int)lp.partitionings : 4;
+    pl.hidden = H;
+    pl.h2 = rank ? reinterpret_cast<const uint16_t*>(pb + o_h2all) : nullptr;
+    pl.w3_seed = static_cast<const uint16_t*>(P.w3_seed);
+    pl.b3_seed = P.bias[2] + 4;
     return ASTCG_OK;
 }
 
@@ -491,7 +520,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
 // blocks of one footprint), chunk by chunk, on stream s.  `pred` selects the
 // LEARNED policy (texs: the textures of A, for the predictor's inputs).
 astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
-                        hipStream_t s, int16_t* pc_seed = nullptr, const PredState* pred = nullptr,
+                        uint32_t flags, hipStream_t s, const LabelBufs* labels = nullptr, const PredState* pred = nullptr,
                         const std::vector<TexDesc>* texs = nullptr) {
     const int chunk = chunk_size(*F.fp, (uint32_t)A.num_blocks, chunk_req);
     astcg_status st = layout_chunk(e, F, chunk, s);
@@ -503,7 +532,6 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
         one.push_back(A.t0);
         texs = &one;
     }
-    e->fb_blocks = e->fb_total = 0;
     for (int blk0 = 0; blk0 < A.num_blocks; blk0 += chunk) {
         A.blk0 = blk0;
         A.nb = std::min(chunk, A.num_blocks - blk0);
@@ -512,7 +540,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
             st = plan_chunk(e, F, *pred, A, *texs, s, &plan);
             if (st != ASTCG_OK) return st;
         }
-        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, pc_seed, pred ? &plan : nullptr);
+        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, labels, pred ? &plan : nullptr);
         if (err != hipSuccess) return hip_fail(e, err, "search kernels");
         if (pred) {
             // fallback statistics of the chunk (small read-back)
@@ -524,7 +552,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
             e->fb_total += A.nb;
         }
     }
-    err = launch_pack(F.T, A, e->d_best, s, e->prof);
+    err = launch_pack(F.T, A, e->d_best, s, e->prof, (flags & ASTCG_FLAG_VOID_EXTENT) != 0);
     if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
     return ASTCG_OK;
 }
@@ -643,12 +671,15 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
     A.t0.num_blocks = (int)e->nblocks;
     A.ntex = 1;
     A.num_blocks = (int)e->nblocks;
-    int16_t* pc_seed = nullptr;
+    LabelBufs lab{};
+    const LabelBufs* labels = nullptr;
     if (e->params.flags & ASTCG_FLAG_LABELS) {
-        err = grow(reinterpret_cast<void**>(&e->d_pcseed), &e->pcseed_cap, sizeof(int16_t) * 3 * (size_t)e->nblocks);
-        if (err == hipSuccess) err = hipMemsetAsync(e->d_pcseed, 0xFF, sizeof(int16_t) * 3 * (size_t)e->nblocks, s);
+        const size_t bytes = label_bytes(e->nblocks);
+        err = grow(&e->d_pcseed, &e->pcseed_cap, bytes);
+        if (err == hipSuccess) err = hipMemsetAsync(e->d_pcseed, 0xFF, bytes, s);
         if (err != hipSuccess) return hip_fail(e, err, "label buffer");
-        pc_seed = e->d_pcseed;
+        lab = label_bufs(e->d_pcseed, e->nblocks);
+        labels = &lab;
     }
     const PredState* pred = nullptr;
     if (e->params.policy == ASTCG_POLICY_LEARNED) {
@@ -656,7 +687,9 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
         if (it == e->preds.end()) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "no predictor for this footprint");
         pred = &it->second;
     }
-    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, s, pc_seed, pred);
+    e->fb_blocks = e->fb_total = 0;
+    return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, e->params.flags, s,
+                      labels, pred);
 }
 
 astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len, uint8_t* out, size_t out_len) {
@@ -738,6 +771,7 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
         group_blocks[gi] = total;
     }
     group_start[keys.size()] = e->h_tex.size();
+    e->fb_blocks = e->fb_total = 0;  // fallback statistics over the whole batch
     if (!e->h_tex.empty()) {
         hipError_t err = grow(reinterpret_cast<void**>(&e->d_tex), &e->tex_cap, sizeof(TexDesc) * e->h_tex.size());
         if (err == hipSuccess)
@@ -761,7 +795,7 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
             if (it == e->preds.end()) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "no predictor for a batch footprint");
             pred = &it->second;
         }
-        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, s, nullptr, pred, &texs);
+        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, p.flags, s, nullptr, pred, &texs);
         if (st != ASTCG_OK) return st;
     }
     return ASTCG_OK;
@@ -867,10 +901,11 @@ astcg_status astcg_get_labels(astcg_encoder* e, int32_t* out, size_t count) {
     if (count < e->nblocks) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "count below number of blocks");
     std::vector<SymBest> h(e->nblocks);
     std::vector<int16_t> ps(3 * (size_t)e->nblocks);
+    const LabelBufs lab = label_bufs(e->d_pcseed, e->nblocks);
     hipError_t err = hipSetDevice(e->device);
     if (err == hipSuccess) err = hipDeviceSynchronize();
     if (err == hipSuccess) err = hipMemcpy(h.data(), e->d_best, sizeof(SymBest) * h.size(), hipMemcpyDeviceToHost);
-    if (err == hipSuccess) err = hipMemcpy(ps.data(), e->d_pcseed, sizeof(int16_t) * ps.size(), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(ps.data(), lab.best_seed, sizeof(int16_t) * ps.size(), hipMemcpyDeviceToHost);
     if (err != hipSuccess) return hip_fail(e, err, "label download");
     const Footprint& f = *e->cur->fp;
     for (size_t i = 0; i < h.size(); ++i) {
@@ -880,6 +915,29 @@ astcg_status astcg_get_labels(astcg_encoder* e, int32_t* out, size_t count) {
         o[2] = h[i].raw_mode < 2048 ? f.raw_to_bm[h[i].raw_mode] : -1;
         for (int p = 0; p < 3; ++p) o[3 + p] = ps[(size_t)p * e->nblocks + i];
     }
+    return ASTCG_OK;
+}
+
+astcg_status astcg_get_candidates(astcg_encoder* e, int16_t* seeds, float* errors, size_t count) {
+    if (!e || !seeds || !errors) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "not configured");
+    if (!(e->params.flags & ASTCG_FLAG_LABELS) || !e->d_pcseed)
+        return fail(e, ASTCG_ERR_NOT_CONFIGURED, "encode with ASTCG_FLAG_LABELS first");
+    if (count < e->nblocks) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "count below number of blocks");
+    const size_t nb = e->nblocks;
+    std::vector<int16_t> cs(12 * nb);
+    std::vector<float> ce(12 * nb);
+    const LabelBufs lab = label_bufs(e->d_pcseed, e->nblocks);
+    hipError_t err = hipSetDevice(e->device);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(cs.data(), lab.cand_seed, sizeof(int16_t) * cs.size(), hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(ce.data(), lab.cand_err, sizeof(float) * ce.size(), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(e, err, "candidate download");
+    for (size_t i = 0; i < nb; ++i)
+        for (int j = 0; j < 12; ++j) {
+            seeds[i * 12 + j] = cs[(size_t)j * nb + i];
+            errors[i * 12 + j] = ce[(size_t)j * nb + i];
+        }
     return ASTCG_OK;
 }
 
@@ -900,8 +958,10 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     astcg_learned_params q;
     astcg_default_learned_params(&q);
     if (lp) q = *lp;
-    if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8)
+    if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
+    if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");
     const Footprint* f = footprint((int)bx, (int)by);
     const int nin = 3 * (int)(bx * by), H = (int)hidden, nout = 4 + 3 * 1024 + f->nbm;
     const size_t sizes[6] = {(size_t)nin * H, (size_t)H, (size_t)H * H, (size_t)H, (size_t)H * nout, (size_t)nout};
@@ -914,11 +974,7 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     auto old = e->preds.find(key);
     if (old != e->preds.end()) {
         (void)hipDeviceSynchronize();
-        for (int i = 0; i < 3; ++i) {
-            if (old->second.w16[i]) (void)hipFree(old->second.w16[i]);
-            if (old->second.bias[i]) (void)hipFree(old->second.bias[i]);
-        }
-        if (old->second.d_head) (void)hipFree(old->second.d_head);
+        free_pred(old->second);
         e->preds.erase(old);
     }
     PredState P;
@@ -945,13 +1001,22 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
                           4, sk, sk, sk, (int)std::min<uint32_t>(q.modes, (uint32_t)f->nbm)};
     if (err == hipSuccess) err = hipMalloc(&P.d_head, sizeof(head));
     if (err == hipSuccess) err = hipMemcpy(P.d_head, head, sizeof(head), hipMemcpyHostToDevice);
+    if (err == hipSuccess && q.seeds == 0 && q.partitionings >= 1 && q.partitionings < 4) {
+        // seed-head columns of W3 transposed ([3 * 1024][H], one contiguous
+        // row per seed) for ranking the compat positions
+        const float* w3 = weights + sizes[0] + sizes[1] + sizes[2] + sizes[3];
+        std::vector<float> t((size_t)3 * 1024 * H);
+        for (int c = 0; c < 3 * 1024; ++c)
+            for (int j = 0; j < H; ++j) t[(size_t)c * H + j] = w3[(size_t)j * nout + 4 + c];
+        err = hipMalloc(&P.w3_seed, t.size() * 2);
+        if (err == hipSuccess) err = hipMemcpy(tmp, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (err == hipSuccess && astcg_mlp_to_bf16(tmp, P.w3_seed, (long long)t.size(), nullptr) != ASTCG_OK)
+            err = hipErrorUnknown;
+        if (err == hipSuccess) err = hipDeviceSynchronize();
+    }
     if (tmp) (void)hipFree(tmp);
     if (err != hipSuccess) {
-        for (int i = 0; i < 3; ++i) {
-            if (P.w16[i]) (void)hipFree(P.w16[i]);
-            if (P.bias[i]) (void)hipFree(P.bias[i]);
-        }
-        if (P.d_head) (void)hipFree(P.d_head);
+        free_pred(P);
         return hip_fail(e, err, "predictor upload");
     }
     e->preds[key] = P;

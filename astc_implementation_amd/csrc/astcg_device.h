@@ -66,6 +66,16 @@ struct ImageArgs {
     const int* pb_count;      // entries in pb_list
 };
 
+// Search outcome recorded for training the candidate predictor (all over
+// the blocks of one launch chain, num_blocks of ImageArgs): per partition
+// count 2..4 the seed of the best partitioned block, and the seed and best
+// error of each of the 4 compat positions (-1 / 1e30 where not searched).
+struct LabelBufs {
+    int16_t* best_seed;  // [3][num_blocks]
+    int16_t* cand_seed;  // [3][4][num_blocks]
+    float* cand_err;     // [3][4][num_blocks]
+};
+
 // Candidate plan of the learned policy for one chunk (null: the plain
 // search, every block's 4 partition positions, all modes, 8 slots).
 struct SearchPlan {
@@ -81,6 +91,13 @@ struct SearchPlan {
     int kmax, seeds, modes, nslots;
     float fallback_below, count_below;
     uint8_t* fallback;          // [nb]
+    // ranking of the compat positions (keep_pb < 4): the blocks' second
+    // hidden activations and the seed-head weights, both bf16
+    int keep_pb;                // positions kept per searched count (1..4)
+    int hidden;
+    const uint16_t* h2;         // [nb][hidden]
+    const uint16_t* w3_seed;    // [3][1024][hidden] (seed-head columns, transposed)
+    const float* b3_seed;       // [3][1024]
 };
 
 // pass01 output (ideal endpoints), per partitioned block; texel weights are
@@ -162,11 +179,12 @@ void upload_quant_tables(hipStream_t s);
 // pc_seed (optional, [3][A.num_blocks]): seed of each partition count's best
 // partitioning per block, -1 when that count produced no candidate.
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s, Profiler* prof, int16_t* pc_seed = nullptr, const SearchPlan* plan = nullptr);
+                        hipStream_t s, Profiler* prof, const LabelBufs* labels = nullptr,
+                        const SearchPlan* plan = nullptr);
 // ASTC LDR decode of a width x height image's blocks (row-major) to RGBA8.
 hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
                          hipStream_t s);
-hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s,
-                       Profiler* prof);
+hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof,
+                       bool void_extent);
 
 }  // namespace astcg

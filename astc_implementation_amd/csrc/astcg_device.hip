@@ -2028,7 +2028,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
 
 // ---- pass18 + host argmin (compression_pipeline.cpp:499-519) --------------------
 __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, const int16_t* pb_part,
-                                             const CandOut* cand, SymBest* best, int16_t* pc_seed) {
+                                             const CandOut* cand, SymBest* best, LabelBufs lab) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.nb) return;
     const int npos = pc == 1 ? 1 : 4;
@@ -2036,20 +2036,32 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
     int cur_pb = -1, cur_slot = -1;
     for (int c = 0; c < npos; ++c) {
         const int pb = (pc > 1 && A.pb_inv) ? A.pb_inv[k * 4 + c] : k * npos + c;
-        if (pb < 0) continue;  // position not searched (learned policy)
+        if (pb < 0) {  // position not searched (learned policy)
+            if (lab.cand_seed) {
+                const size_t li = ((size_t)(pc - 2) * 4 + c) * A.num_blocks + A.blk0 + k;
+                lab.cand_seed[li] = -1;
+                lab.cand_err[li] = 1e30f;
+            }
+            continue;
+        }
         float be = kErrDefault;
         int bs = -1;
         for (int s = 0; s < 8; ++s) {
             const float e = cand[pb * 8 + s].error;
             if (e < be) { be = e; bs = s; }
         }
+        if (lab.cand_seed && pc > 1) {
+            const size_t li = ((size_t)(pc - 2) * 4 + c) * A.num_blocks + A.blk0 + k;
+            lab.cand_seed[li] = (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[pb_slot_of(A, pc, pb)]];
+            lab.cand_err[li] = bs >= 0 ? be : 1e30f;
+        }
         if (cur_pb < 0 || be < cur_err) { cur_err = be; cur_pb = pb; cur_slot = bs; }
     }
     SymBest& sb = best[A.blk0 + k];
     // optional record of each partition count's best partitioning (labels
     // for the candidate predictor)
-    if (pc_seed && pc > 1)
-        pc_seed[(size_t)(pc - 2) * A.num_blocks + A.blk0 + k] =
+    if (lab.best_seed && pc > 1)
+        lab.best_seed[(size_t)(pc - 2) * A.num_blocks + A.blk0 + k] =
             cur_slot >= 0 ? (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[pb_slot_of(A, pc, cur_pb)]]
                           : (int16_t)-1;
     if (pc == 1) {
@@ -2130,12 +2142,27 @@ __device__ void encode_ise(int q, int count, const uint8_t* in, uint32_t* w, int
 
 __device__ __forceinline__ uint32_t bitrev32(uint32_t v) { return __brev(v); }
 
-__global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const SymBest* best) {
+__global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const SymBest* best, int void_extent) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.num_blocks) return;
     const SymBest& sb = best[k];
     const TexDesc t = tex_of(A, k);
     uint4* o = reinterpret_cast<uint4*>(t.out + (size_t)(k - t.first_block) * 16);
+    if (void_extent) {
+        // a block whose texels (edge-clamped) are all equal becomes an LDR
+        // void-extent block (ASTC specification): block mode 0x1FC, LDR,
+        // the two reserved bits set, all-ones (absent) extent coordinates,
+        // then R, G, B, A as UNORM16 (u8 * 257, which decodes back exactly)
+        const BlockLoc l = block_loc(A, T, k);
+        const uint32_t c0 = texel_rgba8(l, T, 0);
+        bool same = true;
+        for (int i = 1; i < T.tc && same; ++i) same = texel_rgba8(l, T, i) == c0;
+        if (same) {
+            const uint32_t r = c0 & 0xFFu, g = (c0 >> 8) & 0xFFu, b = (c0 >> 16) & 0xFFu, a = c0 >> 24;
+            *o = make_uint4(0xFFFFFDFCu, 0xFFFFFFFFu, (r * 257u) | ((g * 257u) << 16), (b * 257u) | ((a * 257u) << 16));
+            return;
+        }
+    }
     const int raw = sb.raw_mode & 0x7FF;
     const int wc = T.raw_wc[raw], q = T.raw_quant[raw];
     const int pc = sb.pc;
@@ -2246,12 +2273,58 @@ __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, in
     slots[k] = fb ? 8 : (uint8_t)P.nslots;
 }
 
+__device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xFFFF0000u); }
+
+__device__ __forceinline__ float dot8_bf16(uint4 a, uint4 b) {
+    float r = bf16_lo(a.x) * bf16_lo(b.x);
+    r += bf16_hi(a.x) * bf16_hi(b.x);
+    r += bf16_lo(a.y) * bf16_lo(b.y);
+    r += bf16_hi(a.y) * bf16_hi(b.y);
+    r += bf16_lo(a.z) * bf16_lo(b.z);
+    r += bf16_hi(a.z) * bf16_hi(b.z);
+    r += bf16_lo(a.w) * bf16_lo(b.w);
+    r += bf16_hi(a.w) * bf16_hi(b.w);
+    return r;
+}
+
+// Keeps the P.keep_pb of block k's 4 compat positions (pb_part[k * 4 + c],
+// packed table entries of partition count pc) whose seeds score highest in
+// the predictor's seed head (logit = b3 + h2 . W3 column, FP32 sums of BF16
+// products in index order; ties keep the lower position), compacted in
+// position order into slots 0..n-1; returns n.
+__device__ int rank_positions(const DevTables& T, const SearchPlan& P, int pc, int k, int16_t* pb_part) {
+    const int H = P.hidden;
+    int row[4];
+    float acc[4];
+    for (int c = 0; c < 4; ++c) {
+        const int e = pb_part[k * 4 + c];
+        row[c] = (e >= 0 && e < kPartitionings) ? (pc - 2) * kPartitionings + T.part_seed[(pc - 2) * kPartitionings + e]
+                                                : -1;
+        acc[c] = row[c] >= 0 ? P.b3_seed[row[c]] : -INFINITY;
+    }
+    const uint4* h = reinterpret_cast<const uint4*>(P.h2 + (size_t)k * H);
+    for (int j = 0; j < H / 8; ++j) {
+        const uint4 hv = h[j];
+        for (int c = 0; c < 4; ++c)
+            if (row[c] >= 0) acc[c] += dot8_bf16(hv, reinterpret_cast<const uint4*>(P.w3_seed + (size_t)row[c] * H)[j]);
+    }
+    int n = 0;
+    for (int c = 0; c < 4; ++c) {
+        int r = 0;
+        for (int d = 0; d < 4; ++d) r += (acc[d] > acc[c]) || (acc[d] == acc[c] && d < c);
+        if (r < P.keep_pb) pb_part[k * 4 + n++] = pb_part[k * 4 + c];
+    }
+    return n;
+}
+
 // One lane per block: the partitioned blocks of partition count pc.  A
 // fallback block keeps the 4 compat positions k_partition wrote; another
-// block searches the predictor's top seeds of this count (when the count is
-// its most likely one or at least count_below probable), written into its
-// pb_part slots 0..n-1.  Entries are appended to the list through one
-// atomic per workgroup; their order does not affect any result.
+// block searches, when the count is its most likely one or at least
+// count_below probable, the compat positions (all 4, or the keep_pb its
+// seed head ranks highest) or the predictor's top seeds of this count,
+// written into its pb_part slots 0..n-1.  Entries are appended to the list
+// through one atomic per workgroup; their order does not affect any result.
 __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, int16_t* pb_part, SearchPlan P,
                                              int* count) {
     __shared__ int s_n, s_base;
@@ -2270,7 +2343,8 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
             for (int j = 0; j < P.kmax && j < 4; ++j)
                 if (ti[j] == pc - 1) { pp = tp[j]; first = j == 0; }
             if ((first || pp >= P.count_below) && P.seeds == 0) {
-                n = 4;  // the compat positions k_partition wrote
+                // the compat positions k_partition wrote
+                n = P.keep_pb < 4 ? rank_positions(T, P, pc, k, pb_part) : 4;
             } else if (first || pp >= P.count_below) {
                 for (int j = 0; j < P.kmax && j < P.seeds && n < 4; ++j) {
                     const int seed = ti[(pc - 1) * P.kmax + j];
@@ -2621,7 +2695,8 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
 }
 
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s, Profiler* prof, int16_t* pc_seed, const SearchPlan* plan) {
+                        hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan) {
+    const LabelBufs lab = labels ? *labels : LabelBufs{nullptr, nullptr, nullptr};
     const int threads = 64;
     auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
     const uint32_t* mode_mask = nullptr;
@@ -2679,7 +2754,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb);
             if (prof) prof->end(K_REFINE, s);
         }
-        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best, pc_seed);
+        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best, lab);
     }
     return hipGetLastError();
 }
@@ -2693,9 +2768,10 @@ hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, i
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof) {
+hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof,
+                       bool void_extent) {
     const int threads = 64;
-    ASTCG_LAUNCH(K_PACK, k_pack, (A.num_blocks + threads - 1) / threads, T, A, best);
+    ASTCG_LAUNCH(K_PACK, k_pack, (A.num_blocks + threads - 1) / threads, T, A, best, void_extent ? 1 : 0);
     return hipGetLastError();
 }
 

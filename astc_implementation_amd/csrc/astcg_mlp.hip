@@ -164,6 +164,41 @@ __global__ void __launch_bounds__(64) k_softmax_xent(const float* logits, int ro
     if (lane == 0 && loss) loss[r] = row_loss;
 }
 
+// Softmax cross-entropy restricted to candidate columns, one lane per row:
+// group g of row r is the columns cols[(r * ngroups + g) * ncand + j]
+// (< 0 = absent) and target[r * ngroups + g] the correct j (< 0 = none).
+// Adds (softmax - onehot) * scale at those columns into dlogits (bf16) and
+// dlogits32 (fp32, when not null), and the groups' negative log-likelihoods
+// into loss[r] (when not null).  A column named twice gets both terms.
+__global__ void k_rank_xent(const float* logits, int rows, int ld, int ngroups, int ncand, const int* cols,
+                            const int* target, float scale, bf16* dlogits, float* dlogits32, float* loss) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float* x = logits + (size_t)r * ld;
+    float row_loss = 0.0f;
+    for (int g = 0; g < ngroups; ++g) {
+        const int t = target[(size_t)r * ngroups + g];
+        const int* c = cols + ((size_t)r * ngroups + g) * ncand;
+        if (t < 0 || t >= ncand || c[t] < 0) continue;
+        float mx = -3.4e38f;
+        for (int j = 0; j < ncand; ++j)
+            if (c[j] >= 0) mx = fmaxf(mx, x[c[j]]);
+        float se = 0.0f;
+        for (int j = 0; j < ncand; ++j)
+            if (c[j] >= 0) se += expf(x[c[j]] - mx);
+        const float lse = mx + logf(se);
+        for (int j = 0; j < ncand; ++j) {
+            if (c[j] < 0) continue;
+            const float gr = (expf(x[c[j]] - lse) - (j == t ? 1.0f : 0.0f)) * scale;
+            const size_t o = (size_t)r * ld + c[j];
+            dlogits[o] = (bf16)((float)dlogits[o] + gr);
+            if (dlogits32) dlogits32[o] += gr;
+        }
+        row_loss += lse - x[c[t]];
+    }
+    if (loss) loss[r] += row_loss;
+}
+
 // Adam (bias-corrected) on fp32 master weights; refreshes the bf16 copy.
 __global__ void k_adam(float* w, const float* g, float* m, float* v, bf16* w16, long long n, float lr, float b1,
                        float b2, float eps, float c1, float c2, float decay) {
@@ -295,6 +330,17 @@ astcg_status astcg_mlp_softmax_xent(const float* logits, int rows, int ld, int n
     if (rows)
         hipLaunchKernelGGL(k_softmax_xent, dim3(rows), dim3(64), 0, static_cast<hipStream_t>(stream), logits, rows, ld,
                            nheads, head_off, head_len, labels, scale, static_cast<bf16*>(dlogits), dlogits32, loss);
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_rank_xent(const float* logits, int rows, int ld, int ngroups, int ncand, const int* cols,
+                                 const int* target, float scale, void* dlogits, float* dlogits32, float* loss,
+                                 void* stream) {
+    if (!logits || rows < 0 || ngroups <= 0 || ncand <= 0 || ncand > 64 || !cols || !target || !dlogits)
+        return ASTCG_ERR_INVALID_ARGUMENT;
+    if (rows)
+        hipLaunchKernelGGL(k_rank_xent, dim3(grid1(rows, 64)), dim3(64), 0, static_cast<hipStream_t>(stream), logits,
+                           rows, ld, ngroups, ncand, cols, target, scale, static_cast<bf16*>(dlogits), dlogits32, loss);
     return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
 }
 

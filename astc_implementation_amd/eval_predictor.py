@@ -41,7 +41,10 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--weights", default=None)
+    ap.add_argument("--settings", default=None,
+                    help="JSON list of astcg_learned_params dicts to sweep (default: the built-in list)")
     a = ap.parse_args()
+    settings = json.loads(a.settings) if a.settings else SETTINGS
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
     params, hidden = load_params(a.weights or os.path.join(HERE, "predictor_%dx%d.npz" % (bx, by)))
@@ -68,7 +71,7 @@ def main():
 
     mt, ps, _ = run(A.POLICY_COMPAT)
     print(json.dumps({"policy": "plain", "mtexel_s": round(mt, 2), "psnr": round(ps, 4)}), flush=True)
-    for lp in SETTINGS:
+    for lp in settings:
         mt2, ps2, fb = run(A.POLICY_LEARNED, **lp)
         print(json.dumps({"policy": "learned", **lp, "mtexel_s": round(mt2, 2), "psnr": round(ps2, 4),
                           "dpsnr": round(ps - ps2, 4), "speedup": round(mt2 / mt, 2), "fallback": round(fb, 4)}),

@@ -34,6 +34,7 @@ def _sig(lib):
         "astcg_mlp_to_bf16": [vp, vp, i64, vp],
         "astcg_mlp_colsum": [vp, i32, i32, vp, i32, vp],
         "astcg_mlp_softmax_xent": [vp, i32, i32, i32, vp, vp, vp, f32, vp, vp, vp, vp],
+        "astcg_mlp_rank_xent": [vp, i32, i32, i32, i32, vp, vp, f32, vp, vp, vp, vp],
         "astcg_mlp_adam": [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, f32, vp],
         "astcg_mlp_features": [vp, i32, i32, i32, i32, i32, i32, vp, i32, vp],
         "astcg_mlp_topk": [vp, i32, i32, i32, vp, vp, i32, vp, vp, vp],
@@ -122,9 +123,12 @@ class Predictor:
         self.act = (x16, z1, h1, z2, h2)
         return logits
 
-    def backward(self, logits, labels):
+    def backward(self, logits, labels, rank=None, rank_weight=1.0):
         """Softmax cross-entropy of every head (labels (B, 5) int32, -1 =
-        none); returns the per-row losses and the gradients (fp32, like params)."""
+        none), plus, with rank = (cols (B, 12) int32, target (B, 3) int32),
+        a softmax over the 4 compat positions' seed logits of each partition
+        count (cols: absolute logit columns, target: the winning position);
+        returns the per-row losses and the gradients (fp32, like params)."""
         torch = self.torch
         x16, z1, h1, z2, h2 = self.act
         B, H, NO = x16.shape[0], self.hidden, self.nout
@@ -137,6 +141,10 @@ class Predictor:
         _check(self.lib.astcg_mlp_softmax_xent(_p(logits), B, NO, len(HEADS), _p(self.d_head_off),
                                                _p(self.d_head_len), _p(labels), 1.0 / B, _p(dlog), _p(dlog32),
                                                _p(loss), self._stream()))
+        if rank is not None:
+            cols, target = rank
+            _check(self.lib.astcg_mlp_rank_xent(_p(logits), B, NO, 3, 4, _p(cols), _p(target), rank_weight / B,
+                                                _p(dlog), _p(dlog32), _p(loss), self._stream()))
         grads = [torch.empty_like(w) for w in self.params]
         gW1, gb1, gW2, gb2, gW3, gb3 = grads
         self.gemm(H, NO, B, h2, H, 1, dlog, NO, 0, gW3, NO)
@@ -163,9 +171,9 @@ class Predictor:
             _check(self.lib.astcg_mlp_adam(_p(w), _p(g), _p(m), _p(v), _p(w16), w.numel(), lr, betas[0], betas[1],
                                            eps, self.step, weight_decay, self._stream()))
 
-    def train_step(self, x16, labels, lr=1e-3):
+    def train_step(self, x16, labels, lr=1e-3, rank=None):
         logits = self.forward(x16)
-        loss, grads = self.backward(logits, labels)
+        loss, grads = self.backward(logits, labels, rank)
         self.adam(grads, lr=lr)
         return loss
 

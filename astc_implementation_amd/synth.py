@@ -153,3 +153,45 @@ def flat_noise_image(width, height, seed, sigma=3.0):
     img[..., :3] = _to_u8(col[None, None, :] + noise)
     img[..., 3] = 255
     return img
+
+
+# ---- texture batches ----------------------------------------------------------
+def batch_spec(count, lo, hi, footprints, seed0, order_seed):
+    """(kind, width, height, (bx, by), seed) of a batch of `count` textures:
+    sides uniform in lo..hi (independently per side, not block multiples),
+    footprint uniform over `footprints`, content by index: two of every four
+    gradient + noise + rectangles, one normal map, one flat colour + noise;
+    texture i uses seed seed0 + i."""
+    rng = SplitMix64(order_seed)
+    spec = []
+    for i in range(count):
+        w = lo + int(rng.uniform() * (hi - lo + 1))
+        h = lo + int(rng.uniform() * (hi - lo + 1))
+        fp = footprints[int(rng.uniform() * len(footprints))]
+        kind = ("gradient", "gradient", "normal", "flat")[i % 4]
+        spec.append((kind, w, h, fp, seed0 + i))
+    return spec
+
+
+def make_texture(entry):
+    kind, w, h, _, seed = entry
+    if kind == "gradient":
+        return gradient_image(w, h, seed)
+    if kind == "normal":
+        return normal_map_image(w, h, seed)
+    return flat_noise_image(w, h, seed)
+
+
+def make_textures(spec, workers=1):
+    """The images of a batch_spec, generated by `workers` processes."""
+    if workers <= 1:
+        return [make_texture(e) for e in spec]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        # largest first, so the pool stays busy to the end
+        order = sorted(range(len(spec)), key=lambda i: -spec[i][1] * spec[i][2])
+        imgs = pool.map(make_texture, [spec[i] for i in order], chunksize=1)
+    out = [None] * len(spec)
+    for i, img in zip(order, imgs):
+        out[i] = img
+    return out

@@ -36,12 +36,25 @@ def training_image(i, size):
     return synth.flat_noise_image(size, size, seed)
 
 
+def rank_targets(seeds, errors):
+    """Logit columns of the 4 compat positions of each partition count
+    ((n, 12) int32) and the winning position ((n, 3) int32: the first
+    position with the lowest error, as the search picks it; -1 when every
+    position reached the same error, which says nothing about the ranking)."""
+    cols = (4 + np.arange(3)[None, :, None] * 1024 + seeds.astype(np.int32)).reshape(-1, 12)
+    cols[seeds.reshape(-1, 12) < 0] = -1
+    tgt = np.argmin(errors, axis=2).astype(np.int32)
+    tgt[(errors == errors[:, :, :1]).all(axis=2)] = -1
+    return cols.astype(np.int32), tgt
+
+
 def collect(P, enc, n_blocks, size=1024, log=print):
     """Encodes training images with the plain search until n_blocks blocks
-    are labelled; returns device tensors X (bf16 features) and Y (int32
-    head labels: pc - 1, best seed for 2 / 3 / 4 partitions, trial mode)."""
+    are labelled; returns device tensors X (bf16 features), Y (int32 head
+    labels: pc - 1, best seed for 2 / 3 / 4 partitions, trial mode) and the
+    position-ranking labels R (logit columns, (n, 12)) and T (winners, (n, 3))."""
     import torch
-    xs, ys, total, i = [], [], 0, 0
+    xs, ys, rs, ts, total, i = [], [], [], [], 0, 0
     while total < n_blocks:
         img = training_image(i, size)
         h, w = img.shape[:2]
@@ -51,31 +64,54 @@ def collect(P, enc, n_blocks, size=1024, log=print):
         enc.encode_device(d_img.data_ptr(), d_out.data_ptr(), torch.cuda.current_stream().cuda_stream)
         lab = enc.labels()
         y = np.stack([lab[:, 0] - 1, lab[:, 3], lab[:, 4], lab[:, 5], lab[:, 2]], axis=1).astype(np.int32)
+        cols, tgt = rank_targets(*enc.candidates())
         xs.append(P.features(d_img, w, h, 0, enc.num_blocks))
         ys.append(torch.from_numpy(y).cuda())
+        rs.append(torch.from_numpy(cols).cuda())
+        ts.append(torch.from_numpy(tgt).cuda())
         total += enc.num_blocks
         i += 1
         log("image %d: %d blocks labelled" % (i, total))
-    return torch.cat(xs)[:n_blocks], torch.cat(ys)[:n_blocks]
+    return (torch.cat(xs)[:n_blocks], torch.cat(ys)[:n_blocks], torch.cat(rs)[:n_blocks],
+            torch.cat(ts)[:n_blocks])
 
 
-def recall(P, X, Y, ks=(1, 2, 4, 16), batch=8192):
-    """Fraction of rows whose label is within the top-k of its head."""
-    import torch
+def recall(P, X, Y, R=None, T=None, ks=(1, 2, 4, 16), batch=8192):
+    """Fraction of rows whose label is within the top-k of its head, and
+    (with R, T) the fraction of partition counts whose winning compat
+    position is ranked first / within the first two by its seed logit."""
     hits = {k: np.zeros(5) for k in ks}
+    rank_hits, rank_n = np.zeros(2), 0
     n = 0
     kmax = max(ks)
     for b0 in range(0, X.shape[0], batch):
         x, y = X[b0:b0 + batch], Y[b0:b0 + batch]
-        idx, _ = P.topk(P.forward(x), kmax)
+        logits = P.forward(x)
+        idx, _ = P.topk(logits, kmax)
         idx, y = idx.cpu().numpy(), y.cpu().numpy()
         for k in ks:
             hits[k] += (idx[:, :, :k] == y[:, :, None]).any(axis=2).sum(axis=0)
+        if R is not None:
+            lg = logits.cpu().numpy()
+            cols = R[b0:b0 + batch].cpu().numpy().reshape(-1, 3, 4)
+            tgt = T[b0:b0 + batch].cpu().numpy()
+            sc = np.take_along_axis(lg[:, None, :].repeat(3, 1), np.maximum(cols, 0), axis=2)
+            sc[cols < 0] = -np.inf
+            # rank of the winner: positions scoring higher, or equal and earlier
+            win = np.take_along_axis(sc, np.maximum(tgt, 0)[:, :, None], axis=2)
+            pos = np.arange(4)[None, None, :]
+            ahead = ((sc > win) | ((sc == win) & (pos < np.maximum(tgt, 0)[:, :, None]))).sum(axis=2)
+            ok = tgt >= 0
+            rank_hits += [((ahead == 0) & ok).sum(), ((ahead <= 1) & ok).sum()]
+            rank_n += int(ok.sum())
         n += x.shape[0]
-    return {k: (hits[k] / n).round(4).tolist() for k in ks}
+    out = {k: (hits[k] / n).round(4).tolist() for k in ks}
+    if rank_n:
+        out["rank_top1_top2"] = (rank_hits / rank_n).round(4).tolist()
+    return out
 
 
-def train(P, X, Y, epochs=4, batch=4096, lr=2e-3, log=print):
+def train(P, X, Y, R=None, T=None, epochs=4, batch=4096, lr=2e-3, log=print):
     import torch
     n = X.shape[0]
     steps = 0
@@ -85,7 +121,8 @@ def train(P, X, Y, epochs=4, batch=4096, lr=2e-3, log=print):
         lr_ep = lr * (0.5 ** ep)
         for b0 in range(0, n - batch + 1, batch):
             sel = perm[b0:b0 + batch]
-            loss = P.train_step(X[sel].contiguous(), Y[sel].contiguous(), lr=lr_ep)
+            rank = (R[sel].contiguous(), T[sel].contiguous()) if R is not None else None
+            loss = P.train_step(X[sel].contiguous(), Y[sel].contiguous(), lr=lr_ep, rank=rank)
             steps += 1
             if steps % 50 == 0:
                 tot += float(loss.mean())
@@ -98,7 +135,16 @@ def main():
     ap.add_argument("--block", default="4x4")
     ap.add_argument("--blocks", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--rank-loss", type=int, default=1,
+                    help="also train the seed heads to rank the 4 compat positions of each partition count")
     ap.add_argument("--out", default=None)
+    # pruning saved with the weights (tuned with eval_predictor)
+    ap.add_argument("--seeds", type=int, default=0)
+    ap.add_argument("--modes", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=5)
+    ap.add_argument("--fallback-below", type=float, default=0.5)
+    ap.add_argument("--count-below", type=float, default=0.0)
+    ap.add_argument("--partitionings", type=int, default=0)
     a = ap.parse_args()
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
@@ -106,28 +152,41 @@ def main():
     P = Predictor((bx, by), seed=0)
     enc = A.Encoder(0)
     t0 = time.time()
-    X, Y = collect(P, enc, a.blocks)
+    X, Y, R, T = collect(P, enc, a.blocks)
     torch.cuda.synchronize()
     t1 = time.time()
     n_val = min(65536, X.shape[0] // 10)
-    Xv, Yv, Xt, Yt = X[:n_val], Y[:n_val], X[n_val:], Y[n_val:]
-    train(P, Xt, Yt, epochs=a.epochs)
+    if not a.rank_loss:
+        R = T = None
+    sl_v, sl_t = slice(0, n_val), slice(n_val, None)
+    train(P, X[sl_t], Y[sl_t], R[sl_t] if R is not None else None, T[sl_t] if T is not None else None,
+          epochs=a.epochs)
     torch.cuda.synchronize()
     t2 = time.time()
-    rec = recall(P, Xv, Yv)
+    rec = recall(P, X[sl_v], Y[sl_v], R[sl_v] if R is not None else None, T[sl_v] if T is not None else None)
     print(json.dumps({"block": a.block, "train_blocks": int(Xt.shape[0]), "label_s": round(t1 - t0, 1),
                       "train_s": round(t2 - t1, 1), "val_recall_topk": rec}))
     out = a.out or os.path.join(HERE, "predictor_%dx%d.npz" % (bx, by))
-    st = P.state()
-    np.savez_compressed(out, **{"p%d" % i: w.astype(np.float16) for i, w in enumerate(st["params"])},
-                        hidden=np.int32(P.hidden), block=np.array([bx, by], np.int32))
+    save(out, P, dict(seeds=a.seeds, modes=a.modes, slots=a.slots, fallback_below=a.fallback_below,
+                      count_below=a.count_below, partitionings=a.partitionings))
     enc.close()
 
 
+def save(path, P, prune):
+    """Weights as fp16 plus the pruning settings (LearnedParams fields) to
+    install with them."""
+    st = P.state()
+    np.savez_compressed(path, **{"p%d" % i: w.astype(np.float16) for i, w in enumerate(st["params"])},
+                        hidden=np.int32(P.hidden), block=np.array([P.bx, P.by], np.int32),
+                        prune_i=np.array([prune["seeds"], prune["modes"], prune["slots"]], np.int32),
+                        prune_f=np.array([prune["fallback_below"], prune["count_below"]], np.float32),
+                        prune_p=np.int32(prune.get("partitionings", 0)))
+
+
 def load_params(path):
-    """The six fp32 parameter arrays of a saved predictor."""
-    z = np.load(path, allow_pickle=False)
-    return [z["p%d" % i].astype(np.float32) for i in range(6)], int(z["hidden"])
+    """The six fp32 parameter arrays of a saved predictor and its hidden width."""
+    params, hidden, _ = A.load_predictor_file(path)
+    return params, hidden
 
 
 if __name__ == "__main__":

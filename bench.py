@@ -1,7 +1,7 @@
 """Throughput benchmark: Mtexel/s of the ASTC encoder on one MI355X, 4096x4096
 RGBA8 synthetic image (gradients + value noise + rectangles, seed 1), 4x4
-blocks, partition counts 1..4, for the plain (compat) search and the learned
-(predictor-pruned) search on the same image.
+blocks (the headline) and 6x6 blocks, partition counts 1..4, for the plain
+(compat) search and the learned (predictor-pruned) search on the same image.
 
 One step = one full encode of the image from RGBA8 resident in HBM to the
 128-bit block stream in HBM.  The learned encoder's rate is the headline when
@@ -44,17 +44,24 @@ def measured_traffic(workload, kernel):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3; 1 for the batch workloads)")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["image", "batch64", "batch256"], default="image",
+                    help="image: one 4096x4096 image (the headline); batch64: 64 textures of 128..2048 per side, "
+                         "footprints 4x4/6x6/8x8, plain search; batch256: 256 textures of 64..4096 per side, "
+                         "footprints 4x4/6x6, plain and learned search")
     ap.add_argument("--block", default="4x4")
     ap.add_argument("--size", type=int, default=WIDTH)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--policy", choices=["plain", "learned", "both"], default="both",
                     help="search policies to time; the learned one needs predictor_<bx>x<by>.npz")
+    ap.add_argument("--also", default="6x6",
+                    help="comma-separated further footprints timed for both policies and reported in config "
+                         "(1 warm-up, 2 steps each; empty = none)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="run one plain device-to-device copy of this many MiB before the encode "
-                         "(a known-size reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
+                    help="run one streaming kernel over this many MiB (read once, written once) before "
+                         "the encode: a known-size reference dispatch for the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -95,10 +102,13 @@ def measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, policy, copy_probe
     d_out = torch.zeros(enc.num_blocks * 16, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
     if copy_probe_mib:
+        # one plain streaming kernel of known size (reads and writes
+        # copy_probe_mib MiB): the reference dispatch for scaling the
+        # FETCH_SIZE / WRITE_SIZE counters
         src = torch.ones(copy_probe_mib << 18, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.mul(src, 1, out=dst)
         torch.cuda.synchronize()
         del src, dst
     for _ in range(warmup):
@@ -122,35 +132,128 @@ def measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, policy, copy_probe
     return {"ms": ms, "value": w * h / (ms / 1e3) / 1e6, "ktimes": ktimes, "psnr": A.psnr(dec, img), "fallback": fb}
 
 
+def bench_batch(a):
+    """One batched call over a synthetic texture batch per step (textures of
+    uneven sizes and mixed footprints, resident in HBM); per policy the
+    aggregate Mtexel/s and every texture's decoded PSNR."""
+    import torch
+    import astc_implementation_amd as A
+    from astc_implementation_amd import synth
+    if a.workload == "batch64":
+        spec = synth.batch_spec(64, 128, 2048, [(4, 4), (6, 6), (8, 8)], 1000, 2)
+        policies = ["plain"]
+    else:
+        spec = synth.batch_spec(256, 64, 4096, [(4, 4), (6, 6)], 20000, 4)
+        policies = ["plain", "learned"] if a.policy == "both" else [a.policy]
+    steps = a.steps or 1
+    t0 = time.perf_counter()
+    imgs = synth.make_textures(spec, workers=min(16, os.cpu_count() or 1))
+    gen_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    d_in = [torch.from_numpy(im.reshape(-1)).to(dev) for im in imgs]
+    texels = sum(w * h for _, w, h, _, _ in spec)
+    nbs = [((w + bx - 1) // bx) * ((h + by - 1) // by) for _, w, h, (bx, by), _ in spec]
+    d_out = [torch.zeros(n * 16, dtype=torch.uint8, device=dev) for n in nbs]
+    textures = [(d_in[i].data_ptr(), w, h, bx, by, d_out[i].data_ptr(), nbs[i] * 16)
+                for i, (_, w, h, (bx, by), _) in enumerate(spec)]
+    enc = A.Encoder(0)
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for pol in policies:
+        policy = A.POLICY_LEARNED if pol == "learned" else A.POLICY_COMPAT
+        if pol == "learned":
+            for fp in sorted({e[3] for e in spec}):
+                enc.load_predictor(*fp)
+        for _ in range(a.warmup):
+            enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(steps):
+            enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        psnrs = []
+        for i, (_, w, h, (bx, by), _) in enumerate(spec):
+            dec = torch.empty(w * h * 4, dtype=torch.uint8, device=dev)
+            enc.decode_device(bx, by, w, h, d_out[i].data_ptr(), dec.data_ptr(), stream.cuda_stream)
+            mse = float(((dec.float() - d_in[i].float()) ** 2).mean())
+            psnrs.append(99.0 if mse == 0 else 10.0 * np.log10(255.0 * 255.0 / mse))
+        res[pol] = {"ms": ms, "value": texels / (ms / 1e3) / 1e6, "psnr": np.array(psnrs),
+                    "fallback": enc.fallback_rate() if pol == "learned" else None}
+    head = "learned" if "learned" in res else policies[0]
+    r = res[head]
+    cfg = {"workload": "%d textures, sides %d..%d, footprints %s, content 2/4 gradient+noise+rectangles, "
+                       "1/4 normal map, 1/4 flat colour+noise, %s search, one batched call per step"
+                       % (len(spec), min(min(e[1], e[2]) for e in spec), max(max(e[1], e[2]) for e in spec),
+                          "/".join("%dx%d" % fp for fp in sorted({e[3] for e in spec})), head),
+           "textures": len(spec), "texels": texels, "blocks": sum(nbs), "generation_s": round(gen_s, 1),
+           "policy": head}
+    for pol, v in res.items():
+        cfg["%s_mtexel_s" % pol] = v["value"]
+        cfg["%s_mean_psnr_db" % pol] = round(float(v["psnr"].mean()), 4)
+    if "plain" in res and "learned" in res:
+        d = res["plain"]["psnr"] - res["learned"]["psnr"]
+        cfg.update({"dpsnr_db_mean": round(float(d.mean()), 4), "dpsnr_db_max": round(float(d.max()), 4),
+                    "textures_over_0.05_db": int((d > MAX_DPSNR).sum()),
+                    "learned_fallback_rate": res["learned"]["fallback"],
+                    "learned_pruning": {"%dx%d" % fp: A.load_predictor_file(A.predictor_path(*fp))[2]
+                                        for fp in sorted({e[3] for e in spec})}})
+    line = {"metric": METRIC, "value": r["value"], "unit": "Mtexel/s", "n_gpus": 1, "steps": steps,
+            "warmup": a.warmup, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": cfg}
+    enc.close()
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only (the encoder shards by block range with no "
-              "collectives; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: this encoder is measured on one GPU only (it would shard by block range or by "
+              "texture with no collectives); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.workload != "image":
+        bench_batch(a)
+        return
+    a.steps = a.steps or 3
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
     import astc_implementation_amd as A
     from astc_implementation_amd import synth
-    from astc_implementation_amd.train_predictor import load_params
 
     w = h = a.size
     img = synth.gradient_image(w, h, SEED)
     enc = A.Encoder(0)
     d_in = torch.from_numpy(img.reshape(-1)).to(torch.device("cuda:0"))
-    weights = os.path.join(ROOT, "astc_implementation_amd", "predictor_%dx%d.npz" % (bx, by))
-    runs = {}
-    if a.policy in ("plain", "both"):
-        runs["plain"] = measure(enc, A, torch, d_in, w, h, bx, by, a.steps, a.warmup, A.POLICY_COMPAT,
-                                a.copy_probe_mib)
-    if a.policy in ("learned", "both") and os.path.exists(weights):
-        params, hidden = load_params(weights)
-        enc.set_predictor(bx, by, params, hidden=hidden)
-        runs["learned"] = measure(enc, A, torch, d_in, w, h, bx, by, a.steps, a.warmup, A.POLICY_LEARNED,
-                                  a.copy_probe_mib if "plain" not in runs else 0)
-    dpsnr = None
-    if "plain" in runs and "learned" in runs:
-        dpsnr = runs["plain"]["psnr"] - runs["learned"]["psnr"]
+    pruning = {}
+
+    def run_policies(bx, by, steps, warmup, probe):
+        runs = {}
+        if a.policy in ("plain", "both"):
+            runs["plain"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_COMPAT, probe)
+        if a.policy in ("learned", "both") and os.path.exists(A.predictor_path(bx, by)):
+            enc.load_predictor(bx, by)
+            pruning["%dx%d" % (bx, by)] = A.load_predictor_file(A.predictor_path(bx, by))[2]
+            runs["learned"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_LEARNED,
+                                      probe if "plain" not in runs else 0)
+        dpsnr = None
+        if "plain" in runs and "learned" in runs:
+            dpsnr = runs["plain"]["psnr"] - runs["learned"]["psnr"]
+        return runs, dpsnr
+
+    runs, dpsnr = run_policies(bx, by, a.steps, a.warmup, a.copy_probe_mib)
+    others = {}
+    for fp in [f for f in a.also.split(",") if f and f != a.block]:
+        obx, oby = (int(v) for v in fp.split("x"))
+        oruns, od = run_policies(obx, oby, 2, 1, 0)
+        others[fp] = {"plain_mtexel_s": oruns["plain"]["value"] if "plain" in oruns else None,
+                      "learned_mtexel_s": oruns["learned"]["value"] if "learned" in oruns else None,
+                      "psnr_plain_db": round(oruns["plain"]["psnr"], 4) if "plain" in oruns else None,
+                      "psnr_learned_db": round(oruns["learned"]["psnr"], 4) if "learned" in oruns else None,
+                      "dpsnr_db": round(od, 4) if od is not None else None,
+                      "learned_fallback_rate": oruns["learned"]["fallback"] if "learned" in oruns else None}
     # the learned encoder is the headline when it stays within the quality
     # bound against the plain search on the same image
     head = "learned" if "learned" in runs and (dpsnr is None or dpsnr <= MAX_DPSNR) else "plain"
@@ -189,7 +292,8 @@ def main():
                    "psnr_learned_db": round(runs["learned"]["psnr"], 4) if "learned" in runs else None,
                    "dpsnr_db": round(dpsnr, 4) if dpsnr is not None else None,
                    "learned_fallback_rate": runs["learned"]["fallback"] if "learned" in runs else None,
-                   "learned_pruning": "compat partition positions, top-16 trial modes, 5 refined candidates"},
+                   "learned_pruning": pruning,
+                   "other_footprints": others},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches,

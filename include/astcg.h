@@ -54,20 +54,27 @@ typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1 } astcg_policy;
  * at least count_below probable (with the COMPAT positions when seeds is 0,
  * else its `seeds` most likely partitionings of each such count), only its
  * `modes` most likely trial modes, and refines `slots` candidates per
- * partitioned block instead of 8. */
+ * partitioned block instead of 8.  With the COMPAT positions, `partitionings`
+ * (1..3) keeps only that many of the 4 positions of each searched count: the
+ * ones whose seeds the predictor's seed head scores highest. */
 typedef struct {
-    uint32_t seeds;        /* 0 (COMPAT positions, default) or 1..4 */
-    uint32_t modes;        /* 1..64, default 16 */
-    uint32_t slots;        /* 1..8, default 5 */
-    float fallback_below;  /* default 0.5 */
-    float count_below;     /* default 0 (every partition count) */
-    uint32_t reserved[3];
+    uint32_t seeds;         /* 0 (COMPAT positions, default) or 1..4 */
+    uint32_t modes;         /* 1..64, default 16 */
+    uint32_t slots;         /* 1..8, default 5 */
+    float fallback_below;   /* default 0.5 */
+    float count_below;      /* default 0 (every partition count) */
+    uint32_t partitionings; /* COMPAT positions kept per count: 1..4, 0 = 4 (default) */
+    uint32_t reserved[2];
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);
 
 #define ASTCG_FLAG_PROFILE 1u  /* record HIP events around every kernel launch */
 #define ASTCG_FLAG_LABELS 2u   /* keep each partition count's best partitioning (astcg_get_labels) */
+/* Blocks whose texels are all equal are written as LDR void-extent blocks
+ * (the exact colour); every other block is unchanged.  No counterpart in the
+ * reference, which never emits them (code/physical_compression.cpp:849). */
+#define ASTCG_FLAG_VOID_EXTENT 4u
 
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */
@@ -168,6 +175,13 @@ astcg_status astcg_fallback_rate(astcg_encoder* enc, double* rate);
  * partitions (-1 = none)}.  Training labels for the candidate predictor; no
  * counterpart in the reference. */
 astcg_status astcg_get_labels(astcg_encoder* enc, int32_t* out, size_t count);
+
+/* The 4 compat positions of every partition count of the same encode: per
+ * block 3 x 4 entries (count 2..4, position 0..3) of the position's
+ * partition seed and the best error its search reached (-1 / 1e30 where the
+ * position was not searched).  Labels for ranking the positions; no
+ * counterpart in the reference. */
+astcg_status astcg_get_candidates(astcg_encoder* enc, int16_t* seeds, float* errors, size_t count);
 
 /* Number of trial block modes (the reference's valid_block_modes) of a
  * footprint, and the raw 11-bit mode of trial mode i. */

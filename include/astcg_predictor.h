@@ -46,6 +46,18 @@ astcg_status astcg_mlp_softmax_xent(const float* logits, int rows, int ld, int n
                                     const int* head_len, const int* labels, float scale, void* dlogits,
                                     float* dlogits32, float* loss, void* stream);
 
+/* Softmax cross-entropy restricted to candidate columns: group g of row r
+ * (ngroups per row) is the ncand (<= 64) logit columns
+ * cols[(r * ngroups + g) * ncand + j] (< 0 = absent), target[r * ngroups + g]
+ * the index j of the correct one (< 0 = no target).  ADDS
+ * (softmax - onehot) * scale at those columns into dlogits (bf16) and
+ * dlogits32 (fp32, if not NULL), and the negative log-likelihoods into loss[r]
+ * (if not NULL): a second loss on top of astcg_mlp_softmax_xent, used to rank
+ * the compat partition positions with the seed heads. */
+astcg_status astcg_mlp_rank_xent(const float* logits, int rows, int ld, int ngroups, int ncand, const int* cols,
+                                 const int* target, float scale, void* dlogits, float* dlogits32, float* loss,
+                                 void* stream);
+
 /* Adam step `step` (>= 1, bias-corrected) with decoupled weight decay on fp32
  * master weights w; refreshes the bf16 copy w16 when not NULL. */
 astcg_status astcg_mlp_adam(float* w, const float* g, float* m, float* v, void* w16, long long n, float lr,

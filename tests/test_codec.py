@@ -341,3 +341,16 @@ def test_container_header(tmp_path):
     p = tmp_path / "img.astc"
     A.store_image(6, 5, 700, 300000, blocks, str(p))
     assert p.read_bytes() == data
+
+
+def test_void_extent_layout_decodes_to_its_colour():
+    """The LDR void-extent block the encoder emits for constant blocks
+    (ASTCG_FLAG_VOID_EXTENT), built here from the specification's layout,
+    decodes to exactly its colour at every footprint."""
+    rgba = (10, 200, 30, 128)
+    blk = np.zeros(16, np.uint8)
+    blk[:8] = [0xFC, 0xFD, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF, 0xFF]
+    blk[8:] = np.array([c * 257 for c in rgba], "<u2").view(np.uint8)
+    for bx, by in [(4, 4), (6, 6), (12, 12)]:
+        dec = A.decode_image(bx, by, bx, by, blk)
+        assert (dec == np.array(rgba, np.uint8)).all()

@@ -104,3 +104,25 @@ def test_batch_argument_validation(encoder):
     p.max_partitions = 0
     tex[0] = A.Texture(img.ctypes.data, 20, 20, 4, 4, out.ctypes.data, out.nbytes)
     assert lib.astcg_encode_batch(h, tex, 1, ctypes.byref(p)) == 1          # bad params
+
+
+def test_learned_batch_matches_single_learned_encodes(encoder):
+    """LEARNED policy over a mixed 4x4 / 6x6 batch with the trained
+    predictors: the plan of a block depends only on that block (predictor
+    rows are independent), so each texture must come out exactly as a
+    single-texture learned encode of it; the batch's fallback rate is the
+    block-weighted rate of its textures."""
+    imgs, fps = make_batch(41, 8, lo=20, hi=90, footprints=((4, 4), (6, 6)))
+    for fp in sorted(set(fps)):
+        encoder.load_predictor(*fp, fallback_below=0.6)
+    outs = encoder.encode_batch(imgs, fps, policy=A.POLICY_LEARNED)
+    fb_batch = encoder.fallback_rate()
+    fb_blocks = 0.0
+    for i, (img, (bx, by)) in enumerate(zip(imgs, fps)):
+        h, w = img.shape[:2]
+        encoder.configure(w, h, bx, by, policy=A.POLICY_LEARNED)
+        single = encoder.encode(img)
+        assert np.array_equal(outs[i], single), "texture %d (%s, %s)" % (i, img.shape, fps[i])
+        fb_blocks += encoder.fallback_rate() * encoder.num_blocks
+    total = sum(((im.shape[1] + bx - 1) // bx) * ((im.shape[0] + by - 1) // by) for im, (bx, by) in zip(imgs, fps))
+    assert abs(fb_batch - fb_blocks / total) < 1e-9

@@ -54,3 +54,73 @@ def test_default_learned_policy_quality(setup):
     p_net, fb = _learned_psnr(enc, img, params, hidden)
     assert p_plain - p_net <= 0.1, (p_plain, p_net)
     assert fb < 0.05
+
+
+def _bf16(a):
+    """float32 -> nearest bfloat16 (ties to even), as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def _seed_logits(params, img, bx, by):
+    """Host forward pass of the network to its 3 x 1024 seed-head logits,
+    rounding to bfloat16 where the kernels do (inputs, weights, hidden
+    activations)."""
+    from scipy.special import erf
+    h, w = img.shape[:2]
+    bxn, byn = (w + bx - 1) // bx, (h + by - 1) // by
+    ys = np.minimum(np.arange(byn)[:, None] * by + np.arange(by)[None, :], h - 1)
+    xs = np.minimum(np.arange(bxn)[:, None] * bx + np.arange(bx)[None, :], w - 1)
+    blk = img[ys[:, None, :, None], xs[None, :, None, :], :3]  # (byn, bxn, by, bx, 3)
+    x = _bf16(blk.reshape(bxn * byn, -1).astype(np.float32) * np.float32(1.0 / 255.0))
+    W1, b1, W2, b2, W3, b3 = params
+
+    def gelu(z):
+        return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    h1 = _bf16(gelu(x @ _bf16(W1) + b1))
+    h2 = _bf16(gelu(h1 @ _bf16(W2) + b2))
+    return h2 @ _bf16(W3[:, 4:4 + 3072]) + b3[4:4 + 3072]
+
+
+def test_ranked_partitionings_keep_the_seed_heads_best(setup):
+    """partitionings=1 keeps, per block and partition count, only the compat
+    position whose seed the seed head scores highest.  The plain search's
+    best seed of a count is one of the 4 compat positions, so the seed the
+    pruned search used (its label) must score at least as high as it.  The
+    host forward pass differs from the kernels in summation order, hence the
+    0.05 tolerance on the logits."""
+    enc, img, params, hidden, _ = setup
+    enc.configure(512, 512, 4, 4, labels=True)
+    enc.encode(img)
+    lab_plain = enc.labels()
+    enc.set_predictor(4, 4, params, hidden=hidden, fallback_below=0.0, partitionings=1)
+    enc.configure(512, 512, 4, 4, labels=True, policy=A.POLICY_LEARNED)
+    blocks = enc.encode(img)
+    lab_one = enc.labels()
+    logits = _seed_logits(params, img, 4, 4)
+    rows = np.arange(len(lab_one))
+    same = 0
+    for c in range(3):
+        s_one, s_plain = lab_one[:, 3 + c], lab_plain[:, 3 + c]
+        assert (s_one >= 0).all() and (s_plain >= 0).all()
+        l_one = logits[rows, c * 1024 + s_one]
+        l_plain = logits[rows, c * 1024 + s_plain]
+        bad = np.nonzero(l_one < l_plain - 0.05)[0]
+        assert bad.size == 0, (c, bad[:8], l_one[bad[:8]], l_plain[bad[:8]])
+        same += int((s_one == s_plain).sum())
+    # the seed head ranks the plain search's winner first far more often
+    # than chance (1 in 4; measured 0.46 on this image)
+    assert same > 0.35 * 3 * len(rows), same
+    dec = enc.decode(4, 4, 512, 512, blocks)
+    assert A.psnr(dec, img) > 40.0
+
+
+def test_partitionings_four_is_the_default(setup):
+    enc, img, params, hidden, _ = setup
+    outs = []
+    for k in (0, 4):
+        enc.set_predictor(4, 4, params, hidden=hidden, partitionings=k)
+        enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+        outs.append(enc.encode(img))
+    assert np.array_equal(outs[0], outs[1])

@@ -174,7 +174,54 @@ def test_search_labels_are_consistent():
             assert lab[i, 1] == sym[i].partition_index
             assert lab[i, 3 + pc - 2] == lab[i, 1]
         assert all(0 <= lab[i, 3 + j] < 1024 for j in range(3))
+    # the 4 compat positions of each count: their best is the count's label
+    seeds, errors = enc.candidates()
+    assert ((seeds >= 0) & (seeds < 1024)).all() and (errors < 1e30).all()
+    best = np.take_along_axis(seeds, np.argmin(errors, axis=2)[:, :, None], axis=2)[:, :, 0]
+    assert np.array_equal(best, lab[:, 3:6])
     enc.close()
+
+
+def test_rank_xent_matches_reference():
+    """Softmax over candidate columns (absent ones, a repeated column, a row
+    without target) added onto existing gradients and losses."""
+    g = torch.Generator().manual_seed(5)
+    B, NO = 37, 3129
+    logits = torch.randn(B, NO, generator=g) * 2.0
+    cols = torch.randint(0, NO, (B, 3, 4), generator=g, dtype=torch.int32)
+    cols[0, 1, 2] = -1
+    cols[1, 0, :] = cols[1, 0, 0]
+    target = torch.randint(0, 4, (B, 3), generator=g, dtype=torch.int32)
+    target[2, 0] = -1
+    target[0, 1] = 3
+    base = torch.randn(B, NO, generator=g) * 0.01
+    scale = 0.25
+    ref, ref_loss = base.clone(), torch.full((B,), 0.5)
+    for r in range(B):
+        for h in range(3):
+            t = int(target[r, h])
+            c = cols[r, h].long()
+            if t < 0 or c[t] < 0:
+                continue
+            ok = c >= 0
+            z = logits[r, c[ok]]
+            p = torch.softmax(z, 0)
+            oh = torch.zeros_like(p)
+            oh[int(ok[:t].sum())] = 1.0
+            for j, col in enumerate(c[ok].tolist()):
+                ref[r, col] += (p[j] - oh[j]) * scale
+            ref_loss[r] += -torch.log_softmax(z, 0)[int(ok[:t].sum())]
+    P = Predictor((4, 4))
+    d32 = base.cuda().contiguous()
+    d16 = base.to(torch.bfloat16).cuda().contiguous()
+    loss = torch.full((B,), 0.5).cuda()
+    lg, cl, tg = logits.cuda().contiguous(), cols.reshape(B, 12).cuda().contiguous(), target.cuda().contiguous()
+    A._check(P.lib.astcg_mlp_rank_xent(lg.data_ptr(), B, NO, 3, 4, cl.data_ptr(), tg.data_ptr(), scale,
+                                       d16.data_ptr(), d32.data_ptr(), loss.data_ptr(), None))
+    torch.cuda.synchronize()
+    _close(d32, ref, 1e-5)
+    _close(d16, ref, 1e-2)
+    _close(loss, ref_loss, 1e-5)
 
 
 def test_learned_policy_with_full_fallback_equals_plain():
