@@ -62,6 +62,7 @@ class Params(ctypes.Structure):
 FLAG_PROFILE = 1
 FLAG_LABELS = 2
 FLAG_VOID_EXTENT = 4
+FLAG_GRAPH = 8
 
 
 class Texture(ctypes.Structure):
@@ -190,7 +191,7 @@ class Encoder:
             pass
 
     def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False, labels=False,
-                  policy=POLICY_COMPAT, void_extent=False):
+                  policy=POLICY_COMPAT, void_extent=False, graph=False):
         """secondaryInit (compression_pipeline.cpp:263-321)."""
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
@@ -198,7 +199,7 @@ class Encoder:
         p.chunk_blocks = chunk_blocks
         p.policy = policy
         p.flags = ((FLAG_PROFILE if profile else 0) | (FLAG_LABELS if labels else 0) |
-                   (FLAG_VOID_EXTENT if void_extent else 0))
+                   (FLAG_VOID_EXTENT if void_extent else 0) | (FLAG_GRAPH if graph else 0))
         _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
         self.width, self.height, self.bx, self.by = width, height, bx, by
 
@@ -241,7 +242,8 @@ class Encoder:
         _check(library().astcg_encode_batch(self._h, tex, len(imgs), ctypes.byref(p)), self._h)
         return outs
 
-    def encode_batch_device(self, textures, max_partitions=4, stream=None, policy=POLICY_COMPAT, profile=False):
+    def encode_batch_device(self, textures, max_partitions=4, stream=None, policy=POLICY_COMPAT, profile=False,
+                            graph=False, void_extent=False):
         """textures: [(d_rgba, width, height, bx, by, d_out, out_len)] with
         device addresses; enqueued on `stream`."""
         tex = (Texture * max(len(textures), 1))()
@@ -250,7 +252,8 @@ class Encoder:
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
-        p.flags = FLAG_PROFILE if profile else 0
+        p.flags = ((FLAG_PROFILE if profile else 0) | (FLAG_GRAPH if graph else 0) |
+                   (FLAG_VOID_EXTENT if void_extent else 0))
         p.policy = policy
         _check(library().astcg_encode_batch_device(self._h, tex, len(textures), ctypes.byref(p),
                                                    ctypes.c_void_p(stream or 0)), self._h)

@@ -87,11 +87,17 @@ struct astcg_encoder {
     size_t pwork_cap = 0;
     void* d_plan = nullptr;          // search plan buffers
     size_t plan_cap = 0;
-    long long fb_blocks = 0, fb_total = 0;
+    long long fb_total = 0;                 // blocks searched under the LEARNED policy since the last reset
+    unsigned long long* d_fb = nullptr;     // device count of their fallback blocks
     void* d_pcseed = nullptr;  // label buffers (LabelBufs) with ASTCG_FLAG_LABELS
     size_t pcseed_cap = 0;
     struct EventProfiler* prof = nullptr;
+    // ASTCG_FLAG_GRAPH: instantiated launch chains, keyed by everything the
+    // chain's launches depend on (most recently used last)
+    std::vector<std::pair<std::vector<uint8_t>, hipGraphExec_t>> graphs;
 };
+
+constexpr size_t kMaxGraphs = 16;
 
 // Records a start/stop event pair around every kernel launch; the pairs are
 // resolved (and the pool recycled) when the times are queried.
@@ -163,6 +169,10 @@ void release_all(astcg_encoder* e) {
     if (e->d_pcseed) (void)hipFree(e->d_pcseed);
     for (auto& kv : e->preds) free_pred(kv.second);
     e->preds.clear();
+    for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
+    e->graphs.clear();
+    if (e->d_fb) (void)hipFree(e->d_fb);
+    e->d_fb = nullptr;
     if (e->d_pwork) (void)hipFree(e->d_pwork);
     if (e->d_plan) (void)hipFree(e->d_plan);
     e->d_pwork = e->d_plan = nullptr;
@@ -425,7 +435,7 @@ typedef const uint8_t* bfloat16_ptr;    // bf16 weights addressed bytewise
 // LEARNED policy: runs the predictor over the chunk's blocks and lays out the
 // search plan (top-k of every head per block, then the plan kernels).
 astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, const ImageArgs& A,
-                        const std::vector<TexDesc>& texs, hipStream_t s, SearchPlan* plan) {
+                        const std::vector<TexDesc>& texs, hipStream_t s, SearchPlan* plan, bool reserve_only = false) {
     const astcg_learned_params& lp = P.lp;
     const int kmax = (int)std::max<uint32_t>(std::max<uint32_t>(lp.seeds, lp.modes), 4);
     const int nb = A.nb, words = (F.fp->nbm + 31) / 32;
@@ -448,6 +458,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     const size_t o_z2 = W.add(4 * (size_t)sub * H), o_h2 = W.add(2 * (size_t)sub * H), o_l = W.add(4 * (size_t)sub * NO);
     err = grow(&e->d_pwork, &e->pwork_cap, W.size);
     if (err != hipSuccess) return hip_fail(e, err, "predictor workspace");
+    if (reserve_only) return ASTCG_OK;  // buffers sized, nothing enqueued
     uint8_t* wb = static_cast<uint8_t*>(e->d_pwork);
     void* X = wb + o_x;
     float* Z1 = reinterpret_cast<float*>(wb + o_z);
@@ -508,6 +519,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     pl.fallback_below = lp.fallback_below;
     pl.count_below = lp.count_below;
     pl.fallback = pb + o_fb;
+    pl.fb_count = e->d_fb;
     pl.keep_pb = rank ? (int)lp.partitionings : 4;
     pl.hidden = H;
     pl.h2 = rank ? reinterpret_cast<const uint16_t*>(pb + o_h2all) : nullptr;
@@ -532,28 +544,80 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
         one.push_back(A.t0);
         texs = &one;
     }
-    for (int blk0 = 0; blk0 < A.num_blocks; blk0 += chunk) {
+    if (pred) e->fb_total += A.num_blocks;
+    // graph replay: capture needs a stream of the caller's (not the null
+    // stream) and no per-launch profiling events
+    const bool graph = (flags & ASTCG_FLAG_GRAPH) && s != nullptr && !(flags & ASTCG_FLAG_PROFILE);
+    std::vector<uint8_t> key;
+    if (graph) {
+        if (pred) {  // size the plan buffers now: no allocation may happen while capturing
+            ImageArgs A1 = A;
+            A1.blk0 = 0;
+            A1.nb = std::min(chunk, A.num_blocks);
+            SearchPlan tmp;
+            st = plan_chunk(e, F, *pred, A1, *texs, s, &tmp, true);
+            if (st != ASTCG_OK) return st;
+        }
+        auto put = [&key](const void* p, size_t n) {
+            const uint8_t* b = static_cast<const uint8_t*>(p);
+            key.insert(key.end(), b, b + n);
+        };
+        const void* ptrs[] = {&F, pred, e->d_chunk, e->d_best, e->d_plan, e->d_pwork, e->d_fb,
+                              labels ? labels->best_seed : nullptr, labels ? labels->cand_seed : nullptr,
+                              labels ? labels->cand_err : nullptr};
+        put(ptrs, sizeof(ptrs));
+        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk};
+        put(ints, sizeof(ints));
+        put(&A, sizeof(A));
+        if (pred) put(&pred->lp, sizeof(pred->lp));
+        if (texs) put(texs->data(), sizeof(TexDesc) * texs->size());
+        for (size_t i = 0; i < e->graphs.size(); ++i) {
+            if (e->graphs[i].first != key) continue;
+            auto hit = e->graphs[i];
+            e->graphs.erase(e->graphs.begin() + (long)i);
+            e->graphs.push_back(hit);
+            err = hipGraphLaunch(hit.second, s);
+            if (err != hipSuccess) return hip_fail(e, err, "graph launch");
+            return ASTCG_OK;
+        }
+        err = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        if (err != hipSuccess) return hip_fail(e, err, "graph capture");
+    }
+    // the launch chain: every chunk's plan and search, then the packer
+    st = ASTCG_OK;
+    for (int blk0 = 0; blk0 < A.num_blocks && st == ASTCG_OK; blk0 += chunk) {
         A.blk0 = blk0;
         A.nb = std::min(chunk, A.num_blocks - blk0);
         SearchPlan plan;
-        if (pred) {
-            st = plan_chunk(e, F, *pred, A, *texs, s, &plan);
-            if (st != ASTCG_OK) return st;
-        }
+        if (pred) st = plan_chunk(e, F, *pred, A, *texs, s, &plan);
+        if (st != ASTCG_OK) break;
         err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, labels, pred ? &plan : nullptr);
-        if (err != hipSuccess) return hip_fail(e, err, "search kernels");
-        if (pred) {
-            // fallback statistics of the chunk (small read-back)
-            std::vector<uint8_t> fb(A.nb);
-            err = hipMemcpyAsync(fb.data(), plan.fallback, A.nb, hipMemcpyDeviceToHost, s);
-            if (err == hipSuccess) err = hipStreamSynchronize(s);
-            if (err != hipSuccess) return hip_fail(e, err, "plan read-back");
-            for (uint8_t v : fb) e->fb_blocks += v;
-            e->fb_total += A.nb;
-        }
+        if (err != hipSuccess) st = hip_fail(e, err, "search kernels");
     }
-    err = launch_pack(F.T, A, e->d_best, s, e->prof, (flags & ASTCG_FLAG_VOID_EXTENT) != 0);
-    if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
+    if (st == ASTCG_OK) {
+        err = launch_pack(F.T, A, e->d_best, s, e->prof, (flags & ASTCG_FLAG_VOID_EXTENT) != 0);
+        if (err != hipSuccess) st = hip_fail(e, err, "pack kernel");
+    }
+    if (!graph) return st;
+    hipGraph_t g = nullptr;
+    err = hipStreamEndCapture(s, &g);
+    if (st != ASTCG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (err != hipSuccess) return hip_fail(e, err, "graph capture");
+    hipGraphExec_t x = nullptr;
+    err = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) return hip_fail(e, err, "graph instantiate");
+    if (e->graphs.size() >= kMaxGraphs) {
+        (void)hipStreamSynchronize(s);
+        (void)hipGraphExecDestroy(e->graphs.front().second);
+        e->graphs.erase(e->graphs.begin());
+    }
+    e->graphs.emplace_back(std::move(key), x);
+    err = hipGraphLaunch(x, s);
+    if (err != hipSuccess) return hip_fail(e, err, "graph launch");
     return ASTCG_OK;
 }
 
@@ -687,7 +751,11 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
         if (it == e->preds.end()) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "no predictor for this footprint");
         pred = &it->second;
     }
-    e->fb_blocks = e->fb_total = 0;
+    if (pred) {
+        e->fb_total = 0;
+        err = launch_fb_reset(e->d_fb, s);
+        if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
+    }
     return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, e->params.flags, s,
                       labels, pred);
 }
@@ -771,7 +839,12 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
         group_blocks[gi] = total;
     }
     group_start[keys.size()] = e->h_tex.size();
-    e->fb_blocks = e->fb_total = 0;  // fallback statistics over the whole batch
+    // fallback statistics over the whole batch
+    e->fb_total = 0;
+    if (p.policy == ASTCG_POLICY_LEARNED && e->d_fb) {
+        hipError_t err = launch_fb_reset(e->d_fb, s);
+        if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
+    }
     if (!e->h_tex.empty()) {
         hipError_t err = grow(reinterpret_cast<void**>(&e->d_tex), &e->tex_cap, sizeof(TexDesc) * e->h_tex.size());
         if (err == hipSuccess)
@@ -977,6 +1050,11 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
         free_pred(old->second);
         e->preds.erase(old);
     }
+    if (!e->d_fb) {
+        err = hipMalloc(reinterpret_cast<void**>(&e->d_fb), sizeof(unsigned long long));
+        if (err == hipSuccess) err = hipMemset(e->d_fb, 0, sizeof(unsigned long long));
+        if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
+    }
     PredState P;
     P.nin = nin;
     P.hidden = H;
@@ -1025,7 +1103,14 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
 
 astcg_status astcg_fallback_rate(astcg_encoder* e, double* rate) {
     if (!e || !rate) return ASTCG_ERR_INVALID_ARGUMENT;
-    *rate = e->fb_total ? (double)e->fb_blocks / (double)e->fb_total : 0.0;
+    *rate = 0.0;
+    if (!e->fb_total || !e->d_fb) return ASTCG_OK;
+    unsigned long long n = 0;
+    hipError_t err = hipSetDevice(e->device);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(&n, e->d_fb, sizeof(n), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
+    *rate = (double)n / (double)e->fb_total;
     return ASTCG_OK;
 }
 

@@ -91,6 +91,7 @@ struct SearchPlan {
     int kmax, seeds, modes, nslots;
     float fallback_below, count_below;
     uint8_t* fallback;          // [nb]
+    unsigned long long* fb_count;  // running count of fallback blocks (one counter per encode call)
     // ranking of the compat positions (keep_pb < 4): the blocks' second
     // hidden activations and the seed-head weights, both bf16
     int keep_pb;                // positions kept per searched count (1..4)
@@ -184,6 +185,10 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffe
 // ASTC LDR decode of a width x height image's blocks (row-major) to RGBA8.
 hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
                          hipStream_t s);
+// Zeroes the fallback counter of the learned policy (a kernel, so that it
+// can be part of a captured graph).
+hipError_t launch_fb_reset(unsigned long long* fb_count, hipStream_t s);
+
 hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof,
                        bool void_extent);
 

@@ -2247,6 +2247,10 @@ __global__ void __launch_bounds__(64) k_plan_zero(int* counts) {
     if (threadIdx.x < 3) counts[threadIdx.x] = 0;
 }
 
+__global__ void __launch_bounds__(64) k_fb_reset(unsigned long long* fb_count) {
+    if (threadIdx.x == 0) *fb_count = 0ull;
+}
+
 __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, int nbm, uint32_t* mask, uint8_t* slots) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.nb) return;
@@ -2260,6 +2264,10 @@ __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, in
         if (ti[4 * P.kmax + j] >= 0) kept += tp[4 * P.kmax + j];
     const bool fb = kept < P.fallback_below;
     P.fallback[k] = fb ? 1 : 0;
+    // one atomic per wave for the fallback statistics
+    const unsigned long long wave_fb = __ballot(fb);
+    if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && wave_fb)
+        atomicAdd(P.fb_count, (unsigned long long)__popcll(wave_fb));
     uint32_t* m = mask + (size_t)k * P.mask_words;
     for (int w = 0; w < P.mask_words; ++w) {
         const int lo = 32 * w;
@@ -2765,6 +2773,11 @@ hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, i
     const int nblocks = blocks_x * ((height + T.by - 1) / T.by);
     hipLaunchKernelGGL(k_decode, dim3((nblocks + 63) / 64), dim3(64), 0, s, T, blocks, width, height, blocks_x, nblocks,
                        rgba);
+    return hipGetLastError();
+}
+
+hipError_t launch_fb_reset(unsigned long long* fb_count, hipStream_t s) {
+    hipLaunchKernelGGL(k_fb_reset, dim3(1), dim3(64), 0, s, fb_count);
     return hipGetLastError();
 }
 

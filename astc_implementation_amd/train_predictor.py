@@ -164,7 +164,7 @@ def main():
     torch.cuda.synchronize()
     t2 = time.time()
     rec = recall(P, X[sl_v], Y[sl_v], R[sl_v] if R is not None else None, T[sl_v] if T is not None else None)
-    print(json.dumps({"block": a.block, "train_blocks": int(Xt.shape[0]), "label_s": round(t1 - t0, 1),
+    print(json.dumps({"block": a.block, "train_blocks": int(X.shape[0] - n_val), "label_s": round(t1 - t0, 1),
                       "train_s": round(t2 - t1, 1), "val_recall_topk": rec}))
     out = a.out or os.path.join(HERE, "predictor_%dx%d.npz" % (bx, by))
     save(out, P, dict(seeds=a.seeds, modes=a.modes, slots=a.slots, fallback_below=a.fallback_below,

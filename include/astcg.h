@@ -75,6 +75,12 @@ void astcg_default_learned_params(astcg_learned_params* p);
  * (the exact colour); every other block is unchanged.  No counterpart in the
  * reference, which never emits them (code/physical_compression.cpp:849). */
 #define ASTCG_FLAG_VOID_EXTENT 4u
+/* Capture an encode's whole launch chain into a HIP graph on the first call
+ * and replay it on later calls with the same arguments (same buffers, sizes,
+ * footprints and settings); up to 16 chains are kept per encoder.  Needs a
+ * caller stream (not the null stream); ignored together with
+ * ASTCG_FLAG_PROFILE.  No counterpart in the reference. */
+#define ASTCG_FLAG_GRAPH 8u
 
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */
