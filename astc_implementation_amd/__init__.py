@@ -87,7 +87,8 @@ class LearnedParams(ctypes.Structure):
         ("fallback_below", ctypes.c_float),
         ("count_below", ctypes.c_float),
         ("partitionings", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("refine_partitionings", ctypes.c_uint32),
+        ("early_out", ctypes.c_float),
     ]
 
 
@@ -285,7 +286,7 @@ class Encoder:
         """Installs predictor weights (six fp32 arrays W1, b1, W2, b2, W3, b3)
         for footprint bx x by; keyword arguments override the pruning
         defaults (seeds, modes, slots, fallback_below, count_below,
-        partitionings)."""
+        partitionings, refine_partitionings, early_out)."""
         flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
         lp = LearnedParams()
         library().astcg_default_learned_params(ctypes.byref(lp))
@@ -351,6 +352,10 @@ def load_predictor_file(path):
         prune.update(fallback_below=fb, count_below=cb)
     if "prune_p" in z.files:
         prune.update(partitionings=int(z["prune_p"]))
+    if "prune_r" in z.files:
+        prune.update(refine_partitionings=int(z["prune_r"]))
+    if "prune_e" in z.files:
+        prune.update(early_out=float(z["prune_e"]))
     return params, int(z["hidden"]), prune
 
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <cstring>
@@ -40,6 +41,10 @@ struct PredState {
     float* bias[3] = {nullptr, nullptr, nullptr};
     int* d_head = nullptr;  // [15]: head offsets, lengths, per-head top-k counts
     void* w3_seed = nullptr;  // bf16 [3][1024][hidden]: seed-head columns of W3, transposed
+    // fused inference (astcg_mlp_infer; hidden width 256): transposed bf16
+    // weights, the output layer reduced to the count and mode heads
+    void* fw[3] = {nullptr, nullptr, nullptr};
+    float* fb3 = nullptr;
     astcg_learned_params lp{};
 };
 
@@ -54,6 +59,12 @@ void free_pred(PredState& P) {
     if (P.w3_seed) (void)hipFree(P.w3_seed);
     P.d_head = nullptr;
     P.w3_seed = nullptr;
+    for (int i = 0; i < 3; ++i) {
+        if (P.fw[i]) (void)hipFree(P.fw[i]);
+        P.fw[i] = nullptr;
+    }
+    if (P.fb3) (void)hipFree(P.fb3);
+    P.fb3 = nullptr;
 }
 
 struct astcg_encoder {
@@ -448,6 +459,8 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     // ranking the compat positions needs every block's second hidden layer
     const bool rank = P.w3_seed != nullptr;
     const size_t o_h2all = rank ? L.add(2 * (size_t)nb * P.hidden) : 0;
+    const bool prune_refine = lp.refine_partitionings >= 1 && lp.refine_partitionings < 4;
+    const size_t o_est = prune_refine ? L.add(sizeof(float) * (size_t)nb * 4) : 0;
     hipError_t err = grow(&e->d_plan, &e->plan_cap, L.size);
     if (err != hipSuccess) return hip_fail(e, err, "plan buffers");
     uint8_t* pb = static_cast<uint8_t*>(e->d_plan);
@@ -479,6 +492,15 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
                                                  lo - t.first_block, hi - lo,
                                                  static_cast<uint8_t*>(X) + 2 * (size_t)(lo - g0) * P.nin, P.nin, s);
             if (st != ASTCG_OK) return fail(e, st, "predictor features");
+        }
+        if (P.fw[0] && lp.seeds == 0) {
+            // one fused kernel: hidden layers, count and mode heads, top-k
+            astcg_status st = astcg_mlp_infer(n, X, P.nin, P.nin, P.fw[0], P.bias[0], P.fw[1], P.bias[1], P.fw[2], P.fb3,
+                                              4 + P.nmodes, P.nmodes, kmax, 4, (int)std::min<uint32_t>(lp.modes, P.nmodes),
+                                              tidx + (size_t)b0 * 5 * kmax, tprob + (size_t)b0 * 5 * kmax,
+                                              rank ? pb + o_h2all + 2 * (size_t)b0 * H : nullptr, s);
+            if (st != ASTCG_OK) return fail(e, st, "predictor inference");
+            continue;
         }
         astcg_status st = astcg_mlp_gemm(n, H, P.nin, X, P.nin, 0, P.w16[0], H, 0, Z1, H, P.bias[0], 0, s);
         if (st == ASTCG_OK) st = astcg_mlp_gelu(Z1, H1, (long long)n * H, s);
@@ -525,6 +547,16 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     pl.h2 = rank ? reinterpret_cast<const uint16_t*>(pb + o_h2all) : nullptr;
     pl.w3_seed = static_cast<const uint16_t*>(P.w3_seed);
     pl.b3_seed = P.bias[2] + 4;
+    pl.keep_refine = prune_refine ? (int)lp.refine_partitionings : 4;
+    if (lp.early_out > 0.0f) {
+        // the reference's error limit (code/compression_pipeline.cpp:276-282, 330)
+        const float texels = (float)(F.fp->bx * F.fp->by);
+        const float db = 80.0f - 19.0f * (logf(texels) / logf(10.0f));
+        const float weights_sum = 0.30f * 2.25f + 0.59f * 2.25f + 0.11f * 2.25f + 1.0f;  // astc.h:60-63
+        pl.error_limit = lp.early_out * powf(0.1f, db * 0.1f) * 65535.0f * 65535.0f * texels * weights_sum;
+    }
+    pl.best = e->d_best;
+    pl.pb_est = prune_refine ? reinterpret_cast<float*>(pb + o_est) : nullptr;
     return ASTCG_OK;
 }
 
@@ -1031,7 +1063,8 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     astcg_learned_params q;
     astcg_default_learned_params(&q);
     if (lp) q = *lp;
-    if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4)
+    if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
+        q.refine_partitionings > 4 || !(q.early_out >= 0.0f))
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");
@@ -1091,6 +1124,35 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
         if (err == hipSuccess && astcg_mlp_to_bf16(tmp, P.w3_seed, (long long)t.size(), nullptr) != ASTCG_OK)
             err = hipErrorUnknown;
         if (err == hipSuccess) err = hipDeviceSynchronize();
+    }
+    if (err == hipSuccess && H == 256 && nin <= 256 && 4 + f->nbm <= 256) {
+        // fused-inference copies: W1^T [256][kin], W2^T, and the count and
+        // mode columns of W3 as rows of [n3p][256] (zero padding)
+        const int kin = (nin + 63) / 64 * 64, n3p = (4 + f->nbm + 15) / 16 * 16;
+        const float* w1 = weights;
+        const float* w2 = weights + sizes[0] + sizes[1];
+        const float* w3 = w2 + sizes[2] + sizes[3];
+        const float* b3 = w3 + sizes[4];
+        std::vector<float> t1((size_t)256 * kin, 0.0f), t2((size_t)256 * 256), t3((size_t)n3p * 256, 0.0f), c3(n3p, 0.0f);
+        for (int j = 0; j < 256; ++j)
+            for (int i = 0; i < nin; ++i) t1[(size_t)j * kin + i] = w1[(size_t)i * 256 + j];
+        for (int j = 0; j < 256; ++j)
+            for (int i = 0; i < 256; ++i) t2[(size_t)j * 256 + i] = w2[(size_t)i * 256 + j];
+        for (int c = 0; c < 4 + f->nbm; ++c) {
+            const int src = c < 4 ? c : 4 + 3 * 1024 + (c - 4);
+            for (int i = 0; i < 256; ++i) t3[(size_t)c * 256 + i] = w3[(size_t)i * nout + src];
+            c3[c] = b3[src];
+        }
+        const std::vector<float>* ts[3] = {&t1, &t2, &t3};
+        for (int i = 0; i < 3 && err == hipSuccess; ++i) {
+            err = hipMalloc(&P.fw[i], ts[i]->size() * 2);
+            if (err == hipSuccess) err = hipMemcpy(tmp, ts[i]->data(), ts[i]->size() * sizeof(float), hipMemcpyHostToDevice);
+            if (err == hipSuccess && astcg_mlp_to_bf16(tmp, P.fw[i], (long long)ts[i]->size(), nullptr) != ASTCG_OK)
+                err = hipErrorUnknown;
+            if (err == hipSuccess) err = hipDeviceSynchronize();
+        }
+        if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&P.fb3), c3.size() * sizeof(float));
+        if (err == hipSuccess) err = hipMemcpy(P.fb3, c3.data(), c3.size() * sizeof(float), hipMemcpyHostToDevice);
     }
     if (tmp) (void)hipFree(tmp);
     if (err != hipSuccess) {

@@ -78,6 +78,8 @@ struct LabelBufs {
 
 // Candidate plan of the learned policy for one chunk (null: the plain
 // search, every block's 4 partition positions, all modes, 8 slots).
+struct SymBest;
+
 struct SearchPlan {
     const uint32_t* mode_mask;  // [nb][mask_words] trial modes each block may use
     int mask_words;
@@ -99,6 +101,14 @@ struct SearchPlan {
     const uint16_t* h2;         // [nb][hidden]
     const uint16_t* w3_seed;    // [3][1024][hidden] (seed-head columns, transposed)
     const float* b3_seed;       // [3][1024]
+    // refinement pruning (keep_refine < 4): each partitioned block's best
+    // estimated candidate error
+    int keep_refine;
+    float* pb_est;              // [nb * 4]
+    // early out: blocks whose best error so far is below error_limit skip
+    // the remaining partition counts (0 = off)
+    float error_limit;
+    const SymBest* best;        // running best per block (all blocks of the launch chain)
 };
 
 // pass01 output (ideal endpoints), per partitioned block; texel weights are

@@ -531,7 +531,8 @@ size_t partition_lds_bytes(const DevTables& T) {
            kPartBlocks * (((size_t)T.tc + 3) & ~size_t(3));
 }
 
-__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part) {
+__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part,
+                                                  const SymBest* best, float error_limit) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int tc = T.tc, lane = threadIdx.x, grp = lane / kPartLanes, sub = lane % kPartLanes;
     const int stride = tc + 1;  // float4 units; the pad spreads the 4 blocks over LDS banks
@@ -548,7 +549,9 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int 
     // partition_ordering and the partitioning errors are allocated 920*128
     // entries but indexed with stride 512, so in-batch blocks >= 230 read
     // zeros and keep table positions 0..3.
-    const bool work = k < A.nb && j * 512 + 127 < kRefBatch * 128;
+    // (learned policy, early out: a block already below the error limit is
+    // not searched at this count, so its candidates are not needed)
+    const bool work = k < A.nb && j * 512 + 127 < kRefBatch * 128 && !(error_limit > 0.0f && best[g].error < error_limit);
     // pass004 writes slot block*1024*i: the score this block reads is
     // mismatch(j, partitioning 1), or mismatch(B-1, partitioning 0) for j == 0
     const int src = j == 0 ? g + B - 1 : g;
@@ -1209,12 +1212,47 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
 }
 
 // ---- pass12 -------------------------------------------------------------------
+// Learned policy, refinement pruning: each partitioned block's lowest
+// estimated candidate error over its allowed trial modes.
+__global__ void __launch_bounds__(64) k_pb_est(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr,
+                                               const uint32_t* mode_mask, int mask_words, float* est) {
+    const int pb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pb >= pb_limit(A, pc, npb)) return;
+    int kb, cb, sl;
+    pb_decode(A, pc, pb, &kb, &cb, &sl);
+    const uint32_t* mask = mode_mask ? mode_mask + (size_t)kb * mask_words : nullptr;
+    float best = kErrDefault;
+    for (int k = 0; k < nbm; ++k) {
+        if (mask && !((mask[k >> 5] >> (k & 31)) & 1u)) continue;
+        best = fminf(best, ccr[(size_t)k * npb + pb].total_error);
+    }
+    est[pb] = best;
+}
+
+// Rank of partitioned block pb among the searched positions of its block by
+// estimated error (lower first, ties to the lower position).
+__device__ __forceinline__ int pb_est_rank(const ImageArgs& A, int pc, int pb, const float* est) {
+    int kb, cb, sl;
+    pb_decode(A, pc, pb, &kb, &cb, &sl);
+    const float mine = est[pb];
+    int r = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (c == cb) continue;
+        const int q = A.pb_inv ? A.pb_inv[kb * 4 + c] : kb * 4 + c;
+        if (q < 0) continue;
+        const float e = est[q];
+        r += (e < mine) || (e == mine && c < cb);
+    }
+    return r;
+}
+
 // Top-8 trial modes per partitioned block, plus the per-mode candidate counts
 // that size the refinement work list (LDS histogram, one global add per
 // workgroup and mode).  Slots without a candidate get the default error.
 __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr, int16_t* top,
                                             int* bin_count, CandOut* cand, const uint32_t* mode_mask, int mask_words,
-                                            const uint8_t* slots) {
+                                            const uint8_t* slots, const float* pb_est, int keep_refine,
+                                            const uint8_t* fallback) {
     extern __shared__ int s_hist[];  // [nbm]
     for (int i = threadIdx.x; i < nbm; i += 64) s_hist[i] = 0;
     __syncthreads();
@@ -1228,6 +1266,11 @@ __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nb
             pb_decode(A, pc, pb, &kb, &cb, &sl);
             if (mode_mask) mask = mode_mask + (size_t)kb * mask_words;
             if (slots) nslots = slots[kb];
+        }
+        if (pb_est) {  // refinement pruning, except for blocks that fall back to the full search
+            int kb, cb, sl;
+            pb_decode(A, pc, pb, &kb, &cb, &sl);
+            if (!fallback[kb] && pb_est_rank(A, pc, pb, pb_est) >= keep_refine) nslots = 0;
         }
         float te[8];
         int ti[8];
@@ -2343,7 +2386,9 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
     if (k < A.nb) {
         const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
         const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
-        if (P.fallback[k]) {
+        if (P.error_limit > 0.0f && P.best[A.blk0 + k].error < P.error_limit) {
+            n = 0;  // early out: the block's best encoding so far is good enough
+        } else if (P.fallback[k]) {
             n = 4;
         } else {
             float pp = 0.0f;
@@ -2725,7 +2770,8 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         if (pc > 1) {
             if (prof) prof->begin(K_PARTITION, s);
             hipLaunchKernelGGL(k_partition, dim3((A.nb + kPartBlocks - 1) / kPartBlocks), dim3(64),
-                               partition_lds_bytes(T), s, A, T, pc, B.pb_part);
+                               partition_lds_bytes(T), s, A, T, pc, B.pb_part, plan ? plan->best : nullptr,
+                               plan ? plan->error_limit : 0.0f);
             if (plan) {
                 hipLaunchKernelGGL(k_plan, dim3(blocks_for(A.nb)), dim3(64), 0, s, A, T, pc, B.pb_part, *plan,
                                    plan->pb_count + (pc - 2));
@@ -2746,9 +2792,14 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         }
         {
             if (prof) prof->begin(K_TOP, s);
+            float* pb_est = (plan && pc > 1 && plan->keep_refine < 4) ? plan->pb_est : nullptr;
+            if (pb_est)
+                hipLaunchKernelGGL(k_pb_est, dim3(blocks_for(npb)), dim3(64), 0, s, A, pc, npb, T.nbm, B.ccr,
+                                   mode_mask, mask_words, pb_est);
             const size_t hist = (size_t)T.nbm * sizeof(int);
             hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, A, pc, npb, T.nbm, B.ccr, B.top,
-                               B.bin_count, B.cand, mode_mask, mask_words, slots);
+                               B.bin_count, B.cand, mode_mask, mask_words, slots, pb_est,
+                               plan ? plan->keep_refine : 4, plan ? plan->fallback : nullptr);
             hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), hist + sizeof(int), s, T.nbm, B.bin_count, B.bin_cur,
                                B.work, B.work_total);
             hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, A, pc, npb, T.nbm, B.top,

@@ -275,6 +275,159 @@ __global__ void __launch_bounds__(64) k_topk(const float* logits, int rows, int 
 
 inline unsigned grid1(long long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
+
+// ---- fused inference for the encoder's search plan -----------------------------
+// 32 rows (blocks) per 256-thread workgroup, hidden width 256: features ->
+// H1 = GELU(X W1 + b1) -> H2 = GELU(H1 W2 + b2) (bf16, in LDS) -> logits of
+// the partition-count and trial-mode heads (LDS) -> softmax top-k of both.
+// Weights are stored transposed (N x K, K contiguous) so that a lane's B
+// fragment of v_mfma_f32_16x16x32_bf16 is one 16-byte load.
+constexpr int kFR = 32, kFH = 256, kFS = kFH + 8;  // rows, hidden width, LDS row stride (bf16)
+
+// out[32][256] = GELU(in[32][K] W + b) in bf16; wave w computes hidden
+// columns 64w .. 64w + 63 (4 column tiles) for both 16-row tiles.
+template <int K>
+__device__ __forceinline__ void dense_gelu(const bf16* in, const bf16* __restrict__ Wt, const float* __restrict__ b,
+                                           bf16* out, int wave, int lane) {
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int lr = lane & 15, lk = 8 * (lane >> 4);
+#pragma unroll 2
+    for (int ks = 0; ks < K / 32; ++ks) {
+        bf16x8 a[2], w[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(in + (16 * i + lr) * kFS + 32 * ks + lk);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            w[j] = *reinterpret_cast<const bf16x8*>(Wt + (size_t)(64 * wave + 16 * j + lr) * K + 32 * ks + lk);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = 64 * wave + 16 * j + lr;
+        const float bias = b[col];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[(16 * i + 4 * (lane >> 4) + r) * kFS + col] = (bf16)gelu(acc[i][j][r] + bias);
+    }
+}
+
+template <int KIN>
+__global__ void __launch_bounds__(256) k_infer(int rows, const bf16* __restrict__ X, int ldx, int nin,
+                                                const bf16* __restrict__ W1t, const float* __restrict__ b1,
+                                                const bf16* __restrict__ W2t, const float* __restrict__ b2,
+                                                const bf16* __restrict__ W3t, const float* __restrict__ b3, int n3p,
+                                                int nmodes, int kmax, int kpc, int kmode, int* __restrict__ out_idx,
+                                                float* __restrict__ out_prob, bf16* __restrict__ h2_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    bf16* sA = reinterpret_cast<bf16*>(smem);              // [32][kFS]: X, then H2
+    bf16* sB = sA + kFR * kFS;                             // [32][kFS]: H1
+    float* sL = reinterpret_cast<float*>(sB + kFR * kFS);  // [32][n3p + 1]: logits
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int r0 = blockIdx.x * kFR;
+    for (int i = t; i < kFR * KIN; i += 256) {
+        const int r = i / KIN, k = i - r * KIN;
+        sA[r * kFS + k] = (r0 + r < rows && k < nin) ? X[(size_t)(r0 + r) * ldx + k] : (bf16)0.0f;
+    }
+    __syncthreads();
+    dense_gelu<KIN>(sA, W1t, b1, sB, wave, lane);
+    __syncthreads();
+    dense_gelu<kFH>(sB, W2t, b2, sA, wave, lane);
+    __syncthreads();
+    // output layer: column tiles wave, wave + 4, ... of n3p / 16
+    {
+        const int nt = n3p / 16, lr = lane & 15, lk = 8 * (lane >> 4), ls = n3p + 1;
+        f32x4 acc[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < kFH / 32; ++ks) {
+            bf16x8 a[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(sA + (16 * i + lr) * kFS + 32 * ks + lk);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int tile = wave + 4 * j;
+                if (tile >= nt) continue;
+                const bf16x8 w = *reinterpret_cast<const bf16x8*>(W3t + (size_t)(16 * tile + lr) * kFH + 32 * ks + lk);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w, acc[i][j], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int tile = wave + 4 * j;
+            if (tile >= nt) continue;
+            const int col = 16 * tile + lr;
+            const float bias = b3[col];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sL[(16 * i + 4 * (lane >> 4) + r) * ls + col] = acc[i][j][r] + bias;
+        }
+    }
+    if (h2_out)
+        for (int i = t; i < kFR * (kFH / 8); i += 256) {
+            const int r = i / (kFH / 8), c = 8 * (i % (kFH / 8));
+            if (r0 + r < rows)
+                *reinterpret_cast<bf16x8*>(h2_out + (size_t)(r0 + r) * kFH + c) =
+                    *reinterpret_cast<const bf16x8*>(sA + r * kFS + c);
+        }
+    __syncthreads();
+    // top-k, 8 lanes per row: rank of each column = columns with a larger
+    // logit, or an equal one at a lower index
+    const int r = t >> 3, sub = t & 7;
+    const float* l = sL + r * (n3p + 1);
+    const bool live = r0 + r < rows;
+    const size_t o = (size_t)(r0 + r) * 5 * kmax;
+    if (live && sub == 0) {
+        float mx = l[0];
+        for (int j = 1; j < 4; ++j) mx = fmaxf(mx, l[j]);
+        float se = 0.0f;
+        for (int j = 0; j < 4; ++j) se += expf(l[j] - mx);
+        for (int j = 0; j < 4; ++j) {
+            int rk = 0;
+            for (int i = 0; i < 4; ++i) rk += (l[i] > l[j]) || (l[i] == l[j] && i < j);
+            if (rk < kpc) {
+                out_idx[o + rk] = j;
+                if (out_prob) out_prob[o + rk] = expf(l[j] - mx) / se;
+            }
+        }
+    }
+    const float* m = l + 4;
+    float mx = -3.4e38f;
+    for (int j = sub; j < nmodes; j += 8) mx = fmaxf(mx, m[j]);
+    mx = fmaxf(mx, __shfl_xor(mx, 1));
+    mx = fmaxf(mx, __shfl_xor(mx, 2));
+    mx = fmaxf(mx, __shfl_xor(mx, 4));
+    float se = 0.0f;
+    for (int j = sub; j < nmodes; j += 8) se += expf(m[j] - mx);
+    se += __shfl_xor(se, 1);
+    se += __shfl_xor(se, 2);
+    se += __shfl_xor(se, 4);
+    if (!live) return;
+    for (int j = sub; j < nmodes; j += 8) {
+        const float v = m[j];
+        int rk = 0;
+        for (int i = 0; i < nmodes && rk < kmode; ++i) rk += (m[i] > v) || (m[i] == v && i < j);
+        if (rk < kmode) {
+            out_idx[o + 4 * kmax + rk] = j;
+            if (out_prob) out_prob[o + 4 * kmax + rk] = expf(v - mx) / se;
+        }
+    }
+    for (int j = nmodes + sub; j < kmode; j += 8) {
+        out_idx[o + 4 * kmax + j] = -1;
+        if (out_prob) out_prob[o + 4 * kmax + j] = 0.0f;
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -385,4 +538,32 @@ astcg_status astcg_mlp_topk_heads(const float* logits, int rows, int ld, int nhe
     return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
 }
 
+
+astcg_status astcg_mlp_infer(int rows, const void* x, int ldx, int nin, const void* w1t, const float* b1,
+                             const void* w2t, const float* b2, const void* w3t, const float* b3, int n3, int nmodes,
+                             int kmax, int kpc, int kmode, int* out_idx, float* out_prob, void* h2_out, void* stream) {
+    const int kin = (nin + 63) / 64 * 64, n3p = (n3 + 15) / 16 * 16;
+    if (!x || !w1t || !b1 || !w2t || !b2 || !w3t || !b3 || !out_idx || rows < 0 || nin <= 0 || kin > 256 ||
+        ldx < nin || n3 < 4 + nmodes || nmodes <= 0 || n3p > 256 || kpc < 0 || kpc > 4 || kpc > kmax || kmode < 0 ||
+        kmode > kmax)
+        return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!rows) return ASTCG_OK;
+    const size_t lds = 2 * sizeof(bf16) * kFR * kFS + sizeof(float) * kFR * (n3p + 1);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid(grid1(rows, kFR));
+    const bf16 *X = static_cast<const bf16*>(x), *W1 = static_cast<const bf16*>(w1t), *W2 = static_cast<const bf16*>(w2t),
+               *W3 = static_cast<const bf16*>(w3t);
+    bf16* H2 = static_cast<bf16*>(h2_out);
+#define ASTCG_INFER(KIN)                                                                                             \
+    hipLaunchKernelGGL(k_infer<KIN>, grid, dim3(256), lds, s, rows, X, ldx, nin, W1, b1, W2, b2, W3, b3, n3p, nmodes, \
+                       kmax, kpc, kmode, out_idx, out_prob, H2)
+    switch (kin) {
+        case 64: ASTCG_INFER(64); break;
+        case 128: ASTCG_INFER(128); break;
+        case 192: ASTCG_INFER(192); break;
+        default: ASTCG_INFER(256); break;
+    }
+#undef ASTCG_INFER
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
 }  // extern "C"

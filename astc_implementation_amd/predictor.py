@@ -38,6 +38,7 @@ def _sig(lib):
         "astcg_mlp_adam": [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, f32, vp],
         "astcg_mlp_features": [vp, i32, i32, i32, i32, i32, i32, vp, i32, vp],
         "astcg_mlp_topk": [vp, i32, i32, i32, vp, vp, i32, vp, vp, vp],
+        "astcg_mlp_infer": [i32, vp, i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

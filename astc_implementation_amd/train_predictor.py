@@ -145,6 +145,8 @@ def main():
     ap.add_argument("--fallback-below", type=float, default=0.5)
     ap.add_argument("--count-below", type=float, default=0.0)
     ap.add_argument("--partitionings", type=int, default=0)
+    ap.add_argument("--refine-partitionings", type=int, default=0)
+    ap.add_argument("--early-out", type=float, default=0.0)
     a = ap.parse_args()
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
@@ -168,7 +170,8 @@ def main():
                       "train_s": round(t2 - t1, 1), "val_recall_topk": rec}))
     out = a.out or os.path.join(HERE, "predictor_%dx%d.npz" % (bx, by))
     save(out, P, dict(seeds=a.seeds, modes=a.modes, slots=a.slots, fallback_below=a.fallback_below,
-                      count_below=a.count_below, partitionings=a.partitionings))
+                      count_below=a.count_below, partitionings=a.partitionings,
+                      refine_partitionings=a.refine_partitionings, early_out=a.early_out))
     enc.close()
 
 
@@ -180,7 +183,9 @@ def save(path, P, prune):
                         hidden=np.int32(P.hidden), block=np.array([P.bx, P.by], np.int32),
                         prune_i=np.array([prune["seeds"], prune["modes"], prune["slots"]], np.int32),
                         prune_f=np.array([prune["fallback_below"], prune["count_below"]], np.float32),
-                        prune_p=np.int32(prune.get("partitionings", 0)))
+                        prune_p=np.int32(prune.get("partitionings", 0)),
+                        prune_r=np.int32(prune.get("refine_partitionings", 0)),
+                        prune_e=np.float32(prune.get("early_out", 0.0)))
 
 
 def load_params(path):

@@ -56,7 +56,15 @@ typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1 } astcg_policy;
  * `modes` most likely trial modes, and refines `slots` candidates per
  * partitioned block instead of 8.  With the COMPAT positions, `partitionings`
  * (1..3) keeps only that many of the 4 positions of each searched count: the
- * ones whose seeds the predictor's seed head scores highest. */
+ * ones whose seeds the predictor's seed head scores highest.
+ * `refine_partitionings` (1..3) prunes later, after the per-mode error
+ * estimates: a count's partitionings are ranked by their best estimated
+ * candidate error and only the best ones are refined.  `early_out` applies
+ * the error limit the reference computes but never uses
+ * (code/compression_pipeline.cpp:276-282, 330: 10^(-(80 - 19 log10 T) / 10)
+ * * 65535^2 * T * sum of channel weights, T texels per block): once a block's
+ * best encoding so far is below early_out times that limit, its remaining
+ * partition counts are not searched. */
 typedef struct {
     uint32_t seeds;         /* 0 (COMPAT positions, default) or 1..4 */
     uint32_t modes;         /* 1..64, default 16 */
@@ -64,7 +72,10 @@ typedef struct {
     float fallback_below;   /* default 0.5 */
     float count_below;      /* default 0 (every partition count) */
     uint32_t partitionings; /* COMPAT positions kept per count: 1..4, 0 = 4 (default) */
-    uint32_t reserved[2];
+    uint32_t refine_partitionings; /* refine only this many (1..3) of each searched count's partitionings:
+                                      those whose best estimated candidate is lowest; 0 = all (default) */
+    float early_out;        /* > 0: a block whose best error after a partition count is below early_out
+                               times the reference's error limit skips the remaining counts; 0 = off */
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);

@@ -81,6 +81,22 @@ astcg_status astcg_mlp_topk_heads(const float* logits, int rows, int ld, int nhe
                                   const int* head_len, int kmax, const int* head_k, int* out_idx, float* out_prob,
                                   void* stream);
 
+/* Fused inference for the encoder's search plan, hidden width 256: per row
+ * of x (bf16, nin <= 256 inputs, row stride ldx) computes
+ * h1 = gelu(x W1 + b1), h2 = gelu(h1 W2 + b2) (each rounded to bf16) and the
+ * logits h2 W3 + b3 of n3 output columns (4 + nmodes <= n3 <= 256): columns
+ * 0..3 are the partition-count head, 4 .. 4 + nmodes - 1 the trial-mode head.
+ * Writes the softmax top-k of both heads (kpc <= min(4, kmax) and
+ * kmode <= kmax entries, value descending, lowest column first on ties, -1
+ * past the head's end) to out_idx / out_prob[(r * 5 + h) * kmax + j] for
+ * h = 0 and h = 4 (out_prob may be NULL), and h2 to h2_out (bf16, row stride
+ * 256) when not NULL.  Weights transposed and zero-padded, bf16: w1t
+ * [256][nin rounded up to 64], w2t [256][256], w3t [n3 rounded up to 16][256];
+ * b3 has n3 rounded up to 16 entries. */
+astcg_status astcg_mlp_infer(int rows, const void* x, int ldx, int nin, const void* w1t, const float* b1,
+                             const void* w2t, const float* b2, const void* w3t, const float* b3, int n3, int nmodes,
+                             int kmax, int kpc, int kmode, int* out_idx, float* out_prob, void* h2_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,3 +124,59 @@ def test_partitionings_four_is_the_default(setup):
         enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
         outs.append(enc.encode(img))
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("keep", [1, 2])
+def test_refine_partitionings_limits_refined_positions(setup, keep):
+    """refine_partitionings=k: per block and partition count at most k of
+    the 4 positions get refined candidates (the others report no error),
+    every block stays a valid encoding, and blocks that fall back keep the
+    full search."""
+    enc, img, params, hidden, p_plain = setup
+    enc.set_predictor(4, 4, params, hidden=hidden, fallback_below=0.0, refine_partitionings=keep)
+    enc.configure(512, 512, 4, 4, labels=True, policy=A.POLICY_LEARNED)
+    blocks = enc.encode(img)
+    _, errors = enc.candidates()
+    refined = (errors < 1e29).sum(axis=2)
+    assert refined.max() <= keep and refined.min() >= 1
+    assert p_plain - A.psnr(enc.decode(4, 4, 512, 512, blocks), img) < 1.0
+    enc.set_predictor(4, 4, params, hidden=hidden, fallback_below=2.0, refine_partitionings=keep)
+    enc.configure(512, 512, 4, 4, labels=True, policy=A.POLICY_LEARNED)
+    enc.encode(img)
+    assert enc.fallback_rate() == 1.0
+    _, errors = enc.candidates()
+    assert ((errors < 1e29).sum(axis=2) == 4).all()
+
+
+def _reference_error_limit(bx, by):
+    """code/compression_pipeline.cpp:276-282 and :330 (float32 steps)."""
+    t = np.float32(bx * by)
+    db = np.float32(80.0) - np.float32(19.0) * (np.log(t, dtype=np.float32) / np.log(np.float32(10.0)))
+    ws = np.float32(0.30 * 2.25) + np.float32(0.59 * 2.25) + np.float32(0.11 * 2.25) + np.float32(1.0)
+    return float(np.float32(0.1) ** (db * np.float32(0.1)) * np.float32(65535.0 * 65535.0) * t * ws)
+
+
+def test_early_out_skips_counts_below_the_error_limit(setup):
+    """early_out=1: a block whose best encoding after a partition count is
+    below the reference's error limit searches no further count (all 4
+    positions of the next count report no error); blocks that stopped after
+    one partition keep a single-partition encoding below the limit, and
+    blocks above the limit after one partition search every count."""
+    enc, img, params, hidden, p_plain = setup
+    limit = _reference_error_limit(4, 4)
+    enc.set_predictor(4, 4, params, hidden=hidden, early_out=1.0)
+    enc.configure(512, 512, 4, 4, labels=True, policy=A.POLICY_LEARNED)
+    blocks = enc.encode(img)
+    _, errors = enc.candidates()
+    lab = enc.labels()
+    sym = enc.symbolic()
+    err = np.array([s.errorval for s in sym], np.float32)
+    skipped2 = (errors[:, 0, :] >= 1e29).all(axis=1)
+    assert 0 < skipped2.sum() < len(err)
+    assert (lab[skipped2, 0] == 1).all()
+    assert (err[skipped2] < limit).all()
+    # a block searched at 2 partitions was above the limit after 1 partition,
+    # so its final error cannot be below the limit through a 1-partition block
+    searched = ~skipped2
+    assert not ((lab[searched, 0] == 1) & (err[searched] < limit) & (errors[searched, 0, :] >= 1e29).all(axis=1)).any()
+    assert p_plain - A.psnr(enc.decode(4, 4, 512, 512, blocks), img) < 0.5

@@ -258,3 +258,56 @@ def test_learned_policy_without_fallback_is_valid():
     assert not magenta.any()
     assert A.psnr(dec, img) > 30.0
     enc.close()
+
+
+@pytest.mark.parametrize("block", [(4, 4), (6, 6)])
+def test_fused_inference_matches_unfused(block):
+    """astcg_mlp_infer (one kernel: both hidden layers, the count and mode
+    heads, top-k) against the separate GEMM / GELU / top-k kernels: same
+    matrix instructions in the same K order, so the same hidden activations,
+    indices and probabilities."""
+    P = Predictor(block, seed=11)
+    g = torch.Generator().manual_seed(12)
+    for w in P.params:  # non-zero biases, wider logits
+        w.add_(torch.randn(w.shape, generator=g).cuda() * 0.05)
+    P.refresh_bf16()
+    rows, kmax = 1000, 24
+    kmode = min(20, P.nmodes)
+    x = torch.rand((rows, P.nin), generator=g).to(torch.bfloat16).cuda()
+    logits = P.forward(x)
+    h2_ref = P.act[4]
+    idx_ref, prob_ref = P.topk(logits, kmax)
+    W1, b1, W2, b2, W3, b3 = P.params
+    kin = (P.nin + 63) // 64 * 64
+    n3 = 4 + P.nmodes
+    n3p = (n3 + 15) // 16 * 16
+    w1t = torch.zeros((256, kin), device="cuda")
+    w1t[:, :P.nin] = W1.t()
+    cols = list(range(4)) + [4 + 3072 + j for j in range(P.nmodes)]
+    w3t = torch.zeros((n3p, 256), device="cuda")
+    w3t[:n3] = W3[:, cols].t()
+    b3c = torch.zeros(n3p, device="cuda")
+    b3c[:n3] = b3[cols]
+    w1t, w2t, w3t = (t.to(torch.bfloat16).contiguous() for t in (w1t, W2.t(), w3t))
+    out_idx = torch.full((rows, 5, kmax), -7, dtype=torch.int32, device="cuda")
+    out_prob = torch.zeros((rows, 5, kmax), device="cuda")
+    h2 = torch.zeros((rows, 256), dtype=torch.bfloat16, device="cuda")
+    A._check(P.lib.astcg_mlp_infer(rows, x.data_ptr(), P.nin, P.nin, w1t.data_ptr(), b1.data_ptr(), w2t.data_ptr(),
+                                   b2.data_ptr(), w3t.data_ptr(), b3c.data_ptr(), n3, P.nmodes, kmax, 4, kmode,
+                                   out_idx.data_ptr(), out_prob.data_ptr(), h2.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert torch.equal(h2.cpu(), h2_ref.cpu())
+    assert torch.equal(out_idx[:, 0, :4].cpu(), idx_ref[:, 0, :4].cpu())
+    assert torch.equal(out_idx[:, 4, :kmode].cpu(), idx_ref[:, 4, :kmode].cpu())
+    _close(out_prob[:, 0, :4], prob_ref[:, 0, :4], 1e-5)
+    _close(out_prob[:, 4, :kmode], prob_ref[:, 4, :kmode], 1e-5)
+    assert (out_idx[:, 1:4].cpu() == -7).all()  # the seed heads are not written
+
+
+def test_fused_inference_argument_limits():
+    P = Predictor((4, 4))
+    lib = P.lib
+    null = None
+    # 8x8 has 339 trial modes: more output columns than the fused kernel takes
+    assert lib.astcg_mlp_infer(1, 1, 192, 192, 1, 1, 1, 1, 1, 1, 4 + 339, 339, 16, 4, 16, 1, null, null, null) == 1
+    assert lib.astcg_mlp_infer(1, 1, 300, 300, 1, 1, 1, 1, 1, 1, 57, 53, 16, 4, 16, 1, null, null, null) == 1
