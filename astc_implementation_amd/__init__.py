@@ -63,6 +63,7 @@ FLAG_PROFILE = 1
 FLAG_LABELS = 2
 FLAG_VOID_EXTENT = 4
 FLAG_GRAPH = 8
+FLAG_DUAL_PLANE = 16
 
 
 class Texture(ctypes.Structure):
@@ -192,7 +193,7 @@ class Encoder:
             pass
 
     def configure(self, width, height, bx, by, max_partitions=4, chunk_blocks=0, profile=False, labels=False,
-                  policy=POLICY_COMPAT, void_extent=False, graph=False):
+                  policy=POLICY_COMPAT, void_extent=False, graph=False, dual_plane=False):
         """secondaryInit (compression_pipeline.cpp:263-321)."""
         p = Params()
         library().astcg_default_params(ctypes.byref(p))
@@ -200,7 +201,8 @@ class Encoder:
         p.chunk_blocks = chunk_blocks
         p.policy = policy
         p.flags = ((FLAG_PROFILE if profile else 0) | (FLAG_LABELS if labels else 0) |
-                   (FLAG_VOID_EXTENT if void_extent else 0) | (FLAG_GRAPH if graph else 0))
+                   (FLAG_VOID_EXTENT if void_extent else 0) | (FLAG_GRAPH if graph else 0) |
+                   (FLAG_DUAL_PLANE if dual_plane else 0))
         _check(library().astcg_configure(self._h, width, height, bx, by, ctypes.byref(p)), self._h)
         self.width, self.height, self.bx, self.by = width, height, bx, by
 
@@ -223,7 +225,7 @@ class Encoder:
                                              ctypes.c_void_p(stream or 0)), self._h)
 
     def encode_batch(self, images, footprints, max_partitions=4, chunk_blocks=0, profile=False,
-                     policy=POLICY_COMPAT, void_extent=False):
+                     policy=POLICY_COMPAT, void_extent=False, dual_plane=False):
         """Encodes several RGBA8 images (H x W x 4 uint8 arrays) with their own
         footprints [(bx, by), ...] in one batched call; returns the block
         arrays in input order."""
@@ -238,7 +240,8 @@ class Encoder:
         library().astcg_default_params(ctypes.byref(p))
         p.max_partitions = max_partitions
         p.chunk_blocks = chunk_blocks
-        p.flags = (FLAG_PROFILE if profile else 0) | (FLAG_VOID_EXTENT if void_extent else 0)
+        p.flags = ((FLAG_PROFILE if profile else 0) | (FLAG_VOID_EXTENT if void_extent else 0) |
+                   (FLAG_DUAL_PLANE if dual_plane else 0))
         p.policy = policy
         _check(library().astcg_encode_batch(self._h, tex, len(imgs), ctypes.byref(p)), self._h)
         return outs

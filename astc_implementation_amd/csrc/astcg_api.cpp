@@ -279,6 +279,7 @@ hipError_t upload_tables(FpState& F) {
     const size_t o_rq = L.add<uint8_t>(f.raw_quant.size());
     const size_t o_qwo = L.add<uint16_t>(qw_off.size());
     const size_t o_s2p = L.add<uint16_t>(f.seed_to_packed.size());
+    const size_t o_dual = L.add<DualMode>(std::max<size_t>(f.dual.size(), 1));
     std::vector<uint8_t> host(L.size, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
     put(o_dm, f.dm.data(), f.dm.size() * sizeof(DmInfo));
@@ -305,6 +306,7 @@ hipError_t upload_tables(FpState& F) {
     put(o_rq, f.raw_quant.data(), f.raw_quant.size());
     put(o_qwo, qw_off.data(), qw_off.size() * 2);
     put(o_s2p, f.seed_to_packed.data(), f.seed_to_packed.size() * 2);
+    put(o_dual, f.dual.data(), f.dual.size() * sizeof(DualMode));
     hipError_t err = hipMalloc(&F.d_tables, L.size);
     if (err != hipSuccess) return err;
     err = hipMemcpy(F.d_tables, host.data(), L.size, hipMemcpyHostToDevice);
@@ -335,6 +337,8 @@ hipError_t upload_tables(FpState& F) {
     T.raw_quant = b + o_rq;
     T.qw_off = reinterpret_cast<const uint16_t*>(b + o_qwo);
     T.seed_to_packed = reinterpret_cast<const uint16_t*>(b + o_s2p);
+    T.dual = reinterpret_cast<const DualMode*>(b + o_dual);
+    T.ndual = (int)f.dual.size();
     T.qw_stride = (qw_total + 15) & ~15;
     T.bx = f.bx;
     T.by = f.by;
@@ -623,7 +627,8 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
         SearchPlan plan;
         if (pred) st = plan_chunk(e, F, *pred, A, *texs, s, &plan);
         if (st != ASTCG_OK) break;
-        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, labels, pred ? &plan : nullptr);
+        err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, labels, pred ? &plan : nullptr,
+                           (flags & ASTCG_FLAG_DUAL_PLANE) != 0);
         if (err != hipSuccess) st = hip_fail(e, err, "search kernels");
     }
     if (st == ASTCG_OK) {
@@ -977,6 +982,8 @@ astcg_status astcg_get_symbolic(astcg_encoder* e, astcg_symbolic_block* out, siz
     if (!e || !out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
     if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "not configured");
     if (count < e->nblocks) return fail(e, ASTCG_ERR_BUFFER_TOO_SMALL, "count below number of blocks");
+    if (e->params.flags & ASTCG_FLAG_DUAL_PLANE)
+        return fail(e, ASTCG_ERR_UNSUPPORTED, "SymbolicBlock has no plane-2 channel (ASTCG_FLAG_DUAL_PLANE)");
     std::vector<SymBest> h(e->nblocks);
     hipError_t err = hipSetDevice(e->device);
     if (err == hipSuccess) err = hipDeviceSynchronize();

@@ -35,6 +35,8 @@ struct DevTables {
     const uint16_t* seed_to_packed; // [3][1024] partition seed -> packed table entry
     const uint16_t* qw_off;       // [nbm] offset of a trial mode's weights in a block's packed weight row
     int qw_stride;                // bytes of packed quantised weights per partitioned block
+    const DualMode* dual;         // [ndual] dual-plane modes of the optional dual-plane search
+    int ndual;
     int bx, by, tc, ndm, nbm;
     int count_selected[4];
 };
@@ -152,7 +154,8 @@ struct SymBest {
     float error;
     uint16_t raw_mode;
     uint16_t seed;
-    uint8_t pc, matched, quant, pad;
+    uint8_t pc, matched, quant;
+    uint8_t pad;  // dual plane (optional search): plane-2 channel + 1, 0 = single plane
     uint16_t formats;
     uint16_t pad2;
     uint8_t packed[32];
@@ -191,7 +194,7 @@ void upload_quant_tables(hipStream_t s);
 // partitioning per block, -1 when that count produced no candidate.
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels = nullptr,
-                        const SearchPlan* plan = nullptr);
+                        const SearchPlan* plan = nullptr, bool dual_plane = false);
 // ASTC LDR decode of a width x height image's blocks (row-major) to RGBA8.
 hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
                          hipStream_t s);

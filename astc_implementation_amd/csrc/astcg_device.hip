@@ -2111,6 +2111,7 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         sb.error = 1e30f;
         sb.raw_mode = 0xFFFF;
         sb.pc = 0;
+        sb.pad = 0;  // single plane
     }
     if (cur_slot >= 0 && cur_err < sb.error) {
         const CandOut& co = cand[cur_pb * 8 + cur_slot];
@@ -2269,6 +2270,8 @@ __global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const Sym
         }
     } else {
         write_bits((uint32_t)fmt_of(sb.formats, 0), 4, 13, w);
+        // dual plane (optional search): the plane-2 channel just below the weights
+        if (sb.pad) write_bits((uint32_t)(sb.pad - 1) & 3u, 2, 128 - wbits - 2, w);
     }
     uint8_t vals[32];
     int nv = 0;
@@ -2280,6 +2283,209 @@ __global__ void __launch_bounds__(64) k_pack(DevTables T, ImageArgs A, const Sym
     }
     encode_ise(cq, nv, vals, w, pc == 1 ? 17 : 29);
     *o = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+
+// ---- optional dual-plane search (ASTCG_FLAG_DUAL_PLANE) ----------------------
+// The reference never tries dual-plane block modes (compression_pipeline.cpp:
+// 183-185).  One lane per block, after the search: for every plane-2 channel
+// c and every dual-plane mode of the footprint (single partition, direct
+// endpoints: CEM 8, or CEM 12 when alpha varies), plane-1 weights come from
+// a line fit of the other channels and plane-2 weights from channel c's own
+// range; both are decimated to the mode's grid and quantised, the endpoints
+// are refitted by least squares to the quantised weights and quantised to the
+// colour level the mode leaves room for, and the candidate is scored with the
+// search's final-error metric (pass17).  A candidate below the block's best
+// error replaces it; SymBest::pad then holds c + 1.
+constexpr int kDualMaxTc = 64;
+
+size_t dual_lds_bytes(const DevTables& T) { return (size_t)2 * T.tc * 64 * sizeof(float) + 2 * 32 * 64; }
+
+__global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* best) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = threadIdx.x, k = blockIdx.x * 64 + lane, tc = T.tc;
+    float* s_w = reinterpret_cast<float*>(smem);                  // [2][tc][64] ideal weights
+    uint8_t* s_q = reinterpret_cast<uint8_t*>(s_w + 2 * tc * 64);  // [2][32][64] quantised grid weights
+    if (k >= A.nb) return;  // no barriers below
+    const int g = A.blk0 + k;
+    const BlockLoc loc = block_loc(A, T, g);
+    SymBest& sb = best[g];
+    float best_err = sb.error;
+    float* w1s = s_w + lane;
+    float* w2s = s_w + tc * 64 + lane;
+    uint8_t* q1s = s_q + lane;
+    uint8_t* q2s = s_q + 32 * 64 + lane;
+    bool alpha_opaque = true;
+    for (int t = 0; t < tc; ++t) alpha_opaque &= (texel_rgba8(loc, T, t) >> 24) == 255u;
+    const int cem = alpha_opaque ? 8 : 12, nv = alpha_opaque ? 6 : 8;
+    for (int c = 0; c < 4; ++c) {
+        // plane 2: channel c over its range
+        float lo = 255.0f, hi = 0.0f;
+        for (int t = 0; t < tc; ++t) {
+            const float v = (float)((texel_rgba8(loc, T, t) >> (8 * c)) & 0xFFu);
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+        if (hi <= lo) continue;  // a constant channel needs no plane of its own
+        // plane 1: principal direction of the other channels (power iteration)
+        float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < tc; ++t) {
+            const uint32_t v = texel_rgba8(loc, T, t);
+            for (int j = 0; j < 4; ++j) m[j] += (float)((v >> (8 * j)) & 0xFFu);
+        }
+        for (int j = 0; j < 4; ++j) m[j] = j == c ? 0.0f : m[j] / (float)tc;
+        float cov[4][4] = {};
+        for (int t = 0; t < tc; ++t) {
+            const uint32_t v = texel_rgba8(loc, T, t);
+            float d[4];
+            for (int j = 0; j < 4; ++j) d[j] = j == c ? 0.0f : (float)((v >> (8 * j)) & 0xFFu) - m[j];
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) cov[i][j] += d[i] * d[j];
+        }
+        float dir[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        dir[c] = 0.0f;
+        for (int it = 0; it < 6; ++it) {
+            float nd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) nd[i] += cov[i][j] * dir[j];
+            const float n2 = nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2] + nd[3] * nd[3];
+            if (!(n2 > 1e-20f)) break;
+            const float r = rsqrtf(n2);
+            for (int i = 0; i < 4; ++i) dir[i] = nd[i] * r;
+        }
+        float smin = 3.4e38f, smax = -3.4e38f;
+        for (int t = 0; t < tc; ++t) {
+            const uint32_t v = texel_rgba8(loc, T, t);
+            float sp = 0.0f;
+            for (int j = 0; j < 4; ++j)
+                if (j != c) sp += ((float)((v >> (8 * j)) & 0xFFu) - m[j]) * dir[j];
+            w1s[t * 64] = sp;
+            smin = fminf(smin, sp);
+            smax = fmaxf(smax, sp);
+        }
+        const float span = smax - smin, inv1 = span > 1e-6f ? 1.0f / span : 0.0f, inv2 = 1.0f / (hi - lo);
+        for (int t = 0; t < tc; ++t) {
+            w1s[t * 64] = (w1s[t * 64] - smin) * inv1;
+            w2s[t * 64] = ((float)((texel_rgba8(loc, T, t) >> (8 * c)) & 0xFFu) - lo) * inv2;
+        }
+        for (int di = 0; di < T.ndual; ++di) {
+            const DualMode md = T.dual[di];
+            const int cq = alpha_opaque ? md.cq_rgb : md.cq_rgba;
+            if (cq < 4) continue;
+            const DmInfo dinfo = T.dm[md.dm];
+            const int wc = dinfo.weight_count, q = md.quant, lv = kLevelsD[q];
+            const bool undecimated = wc == tc;
+            const uint8_t* t2wc = T.t2w_count + (size_t)md.dm * tc;
+            const uint8_t* t2wi = T.t2w_idx + (size_t)md.dm * tc * 4;
+            const float* t2ww = T.t2w_w + (size_t)md.dm * tc * 4;
+            const uint8_t* w2tc = T.w2t_count + md.dm * 64;
+            const uint16_t* w2to = T.w2t_off + md.dm * 64;
+            const uint8_t* w2ti = T.w2t_idx + dinfo.w2t_base;
+            const float* w2tw = T.w2t_w + dinfo.w2t_base;
+            // decimate and quantise both planes
+            for (int plane = 0; plane < 2; ++plane) {
+                const float* ws = plane ? w2s : w1s;
+                uint8_t* qs = plane ? q2s : q1s;
+                for (int w = 0; w < wc; ++w) {
+                    float v;
+                    if (undecimated) {
+                        v = ws[w * 64];
+                    } else {
+                        float acc = 0.0f, den = 1e-10f;
+                        const int n = w2tc[w], off = w2to[w];
+                        for (int j = 0; j < n; ++j) {
+                            acc += ws[w2ti[off + j] * 64] * w2tw[off + j];
+                            den += w2tw[off + j];
+                        }
+                        v = acc / den;
+                    }
+                    const int rank = min(max((int)(fminf(fmaxf(v, 0.0f), 1.0f) * (float)(lv - 1) + 0.5f), 0), lv - 1);
+                    qs[w * 64] = c_qt.quant_to_unquant[q][rank];
+                }
+            }
+            // infilled weight (0..64) of texel t on a plane, as pass17 forms it
+            auto infill = [&](const uint8_t* qs, int t) -> int {
+                if (undecimated) return qs[t * 64];
+                float w = 0.0f;
+                const int n = t2wc[t];
+                for (int j = 0; j < n; ++j) w += ((float)qs[t2wi[t * 4 + j] * 64] / 64.0f) * t2ww[t * 4 + j];
+                return wf2i(wround(w * 64.0f));
+            };
+            // least-squares endpoints per channel for the quantised weights
+            float a00[2] = {0.0f, 0.0f}, a01[2] = {0.0f, 0.0f}, a11[2] = {0.0f, 0.0f};
+            float b0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, b1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < tc; ++t) {
+                const float wp[2] = {(float)infill(q1s, t) / 64.0f, (float)infill(q2s, t) / 64.0f};
+                for (int p = 0; p < 2; ++p) {
+                    a00[p] += (1.0f - wp[p]) * (1.0f - wp[p]);
+                    a01[p] += (1.0f - wp[p]) * wp[p];
+                    a11[p] += wp[p] * wp[p];
+                }
+                const uint32_t v = texel_rgba8(loc, T, t);
+                for (int j = 0; j < 4; ++j) {
+                    const float x = (float)((v >> (8 * j)) & 0xFFu), wj = wp[j == c ? 1 : 0];
+                    b0[j] += (1.0f - wj) * x;
+                    b1[j] += wj * x;
+                }
+            }
+            int eq0[4], eq1[4];
+            for (int j = 0; j < 4; ++j) {
+                const int p = j == c ? 1 : 0;
+                float e0 = j == c ? lo : m[j] + dir[j] * smin, e1 = j == c ? hi : m[j] + dir[j] * smax;
+                const float det = a00[p] * a11[p] - a01[p] * a01[p];
+                if (fabsf(det) > 1e-4f * (a00[p] * a00[p] + a11[p] * a11[p] + 2.0f * a01[p] * a01[p])) {
+                    e0 = (a11[p] * b0[j] - a01[p] * b1[j]) / det;
+                    e1 = (a00[p] * b1[j] - a01[p] * b0[j]) / det;
+                }
+                if (j == 3 && alpha_opaque) { e0 = 255.0f; e1 = 255.0f; }
+                const int i0 = min(max((int)rintf(2.0f * e0), 0), 510), i1 = min(max((int)rintf(2.0f * e1), 0), 510);
+                eq0[j] = c_qt.color_u2q[cq - 4][i0];
+                eq1[j] = c_qt.color_u2q[cq - 4][i1];
+            }
+            // direct endpoints decode with blue contraction when the second
+            // endpoint's RGB sum is the smaller: swap the endpoints and
+            // mirror both planes' weights instead (weight levels are
+            // symmetric about 32)
+            const bool swap = eq1[0] + eq1[1] + eq1[2] < eq0[0] + eq0[1] + eq0[2];
+            if (swap)
+                for (int j = 0; j < 4; ++j) { const int x = eq0[j]; eq0[j] = eq1[j]; eq1[j] = x; }
+            float total = 0.0f;
+            for (int t = 0; t < tc && total < best_err; ++t) {
+                int wt[2] = {infill(q1s, t), infill(q2s, t)};
+                if (swap) { wt[0] = 64 - wt[0]; wt[1] = 64 - wt[1]; }
+                const uint32_t v = texel_rgba8(loc, T, t);
+                float dd[4];
+                for (int j = 0; j < 4; ++j) {
+                    const int wv = wt[j == c ? 1 : 0];
+                    const int col = (eq0[j] * 257 * (64 - wv) + eq1[j] * 257 * wv + 32) >> 6;
+                    const float df = wmin(fabsf(c_qt.pixel_scale[(v >> (8 * j)) & 0xFFu] - (float)col), 1e15f);
+                    dd[j] = df * df;
+                }
+                total += wmin(dot4(dd, kCW), kErrDefault);
+            }
+            if (!(total < best_err)) continue;
+            best_err = total;
+            sb.error = total;
+            sb.raw_mode = md.raw_mode;
+            sb.pc = 1;
+            sb.seed = 0;
+            sb.matched = 0;
+            sb.quant = (uint8_t)cq;
+            sb.formats = (uint16_t)cem;
+            sb.pad = (uint8_t)(c + 1);
+            // CEM 8 / 12 value order: R0 R1 G0 G1 B0 B1 (A0 A1)
+            for (int j = 0; j < nv / 2; ++j) {
+                sb.packed[2 * j] = (uint8_t)eq0[j];
+                sb.packed[2 * j + 1] = (uint8_t)eq1[j];
+            }
+            // weights interleaved plane 1, plane 2 per grid point
+            for (int w = 0; w < wc; ++w) {
+                const int u1 = q1s[w * 64], u2 = q2s[w * 64];
+                sb.qw[2 * w] = (uint8_t)(swap ? 64 - u1 : u1);
+                sb.qw[2 * w + 1] = (uint8_t)(swap ? 64 - u2 : u2);
+            }
+        }
+    }
 }
 
 // ---- learned candidate policy: per-chunk search plan ------------------------
@@ -2748,7 +2954,8 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
 }
 
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
-                        hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan) {
+                        hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan,
+                        bool dual_plane) {
     const LabelBufs lab = labels ? *labels : LabelBufs{nullptr, nullptr, nullptr};
     const int threads = 64;
     auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
@@ -2814,6 +3021,11 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             if (prof) prof->end(K_REFINE, s);
         }
         ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best, lab);
+    }
+    if (dual_plane && T.ndual > 0 && T.tc <= kDualMaxTc) {
+        if (prof) prof->begin(K_PICK, s);
+        hipLaunchKernelGGL(k_dual, dim3(blocks_for(A0.nb)), dim3(64), dual_lds_bytes(T), s, A0, T, best);
+        if (prof) prof->end(K_PICK, s);
     }
     return hipGetLastError();
 }

@@ -577,6 +577,26 @@ std::unique_ptr<Footprint> build_footprint(int bx, int by) {
         f->bm.push_back(b);
     }
     f->nbm = (int)f->bm.size();
+    // Dual-plane modes for the optional dual-plane search: a trial grid, a
+    // legal weight bit count and room for at least QUANT_6 colour values
+    // after the 17 mode / partition / CEM bits and the 2 plane-selector bits.
+    for (const Mode& m : modes) {
+        if (!m.dual || dm_trial[m.dm] < 0 || m.wbits < 24 || m.wbits > 96 || f->raw_wc[m.raw] > 64) continue;
+        const int avail = 128 - 17 - 2 - m.wbits;
+        int cq6 = -1, cq8 = -1;
+        for (int q = 4; q <= 20; ++q) {
+            if (ise_bitcount(6, q) <= avail) cq6 = q;
+            if (ise_bitcount(8, q) <= avail) cq8 = q;
+        }
+        if (cq6 < 0) continue;
+        DualMode d{};
+        d.raw_mode = (uint16_t)m.raw;
+        d.dm = (uint8_t)dm_trial[m.dm];
+        d.quant = (uint8_t)m.quant;
+        d.cq_rgb = (int8_t)cq6;
+        d.cq_rgba = (int8_t)cq8;
+        f->dual.push_back(d);
+    }
     build_kmeans_texels(*f);
     build_partitions(*f);
     build_orderings(*f);

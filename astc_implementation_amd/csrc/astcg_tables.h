@@ -53,6 +53,17 @@ struct BmInfo {
     uint8_t pad[3];
 };
 
+// Dual-plane block mode tried by the optional dual-plane search (one
+// partition): the decimation grid must be a trial decimation mode.
+struct DualMode {
+    uint16_t raw_mode;   // 11-bit block mode (dual-plane bit set)
+    uint8_t dm;          // trial decimation mode index of its weight grid
+    uint8_t quant;       // weight quant level
+    int8_t cq_rgb;       // colour quant level left for CEM 8 (6 values), -1 = none
+    int8_t cq_rgba;      // colour quant level left for CEM 12 (8 values), -1 = none
+    uint8_t pad[2];
+};
+
 struct Footprint {
     int bx = 0, by = 0, texel_count = 0;
     int ndm = 0, nbm = 0;
@@ -72,6 +83,7 @@ struct Footprint {
     // raw block mode -> (weight count, quant, weight bits, dual plane) for the packer;
     // wc == 0 marks modes invalid for this footprint
     std::vector<uint8_t> raw_wc, raw_quant, raw_wbits;
+    std::vector<DualMode> dual;       // dual-plane modes of the optional dual-plane search
     // partition tables, packed order, for partition counts 2..4 (+ the 1-partition entry)
     std::vector<uint8_t> part_of_texel;  // [3][1024][texel_count]
     std::vector<uint16_t> part_seed;     // [3][1024]

@@ -46,16 +46,19 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3; 1 for the batch workloads)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["image", "batch64", "batch256"], default="image",
+    ap.add_argument("--workload", choices=["image", "batch64", "batch256", "small"], default="image",
                     help="image: one 4096x4096 image (the headline); batch64: 64 textures of 128..2048 per side, "
                          "footprints 4x4/6x6/8x8, plain search; batch256: 256 textures of 64..4096 per side, "
-                         "footprints 4x4/6x6, plain and learned search")
+                         "footprints 4x4/6x6, plain and learned search; small: repeated encodes of one 256x256 "
+                         "texture (a serving loop), launched eagerly and as a replayed HIP graph")
     ap.add_argument("--block", default="4x4")
     ap.add_argument("--size", type=int, default=WIDTH)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--policy", choices=["plain", "learned", "both"], default="both",
                     help="search policies to time; the learned one needs predictor_<bx>x<by>.npz")
+    ap.add_argument("--graph", action="store_true",
+                    help="batch workloads: capture the batched call's launch chain in a HIP graph and replay it")
     ap.add_argument("--also", default="6x6",
                     help="comma-separated further footprints timed for both policies and reported in config "
                          "(1 warm-up, 2 steps each; empty = none)")
@@ -165,13 +168,13 @@ def bench_batch(a):
             for fp in sorted({e[3] for e in spec}):
                 enc.load_predictor(*fp)
         for _ in range(a.warmup):
-            enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy)
+            enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy, graph=a.graph)
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         start.record(stream)
         for _ in range(steps):
-            enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy)
+            enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy, graph=a.graph)
         stop.record(stream)
         torch.cuda.synchronize()
         ms = start.elapsed_time(stop) / steps
@@ -190,7 +193,7 @@ def bench_batch(a):
                        % (len(spec), min(min(e[1], e[2]) for e in spec), max(max(e[1], e[2]) for e in spec),
                           "/".join("%dx%d" % fp for fp in sorted({e[3] for e in spec})), head),
            "textures": len(spec), "texels": texels, "blocks": sum(nbs), "generation_s": round(gen_s, 1),
-           "policy": head}
+           "policy": head, "graph": bool(a.graph)}
     for pol, v in res.items():
         cfg["%s_mtexel_s" % pol] = v["value"]
         cfg["%s_mean_psnr_db" % pol] = round(float(v["psnr"].mean()), 4)
@@ -208,12 +211,59 @@ def bench_batch(a):
     print(json.dumps(line))
 
 
+def bench_small(a):
+    """Latency-bound case: `steps` encodes of one 256x256 RGBA8 texture
+    (4x4, learned policy) into the same buffers, eager launches vs the
+    replayed HIP graph of the same chain."""
+    import torch
+    import astc_implementation_amd as A
+    from astc_implementation_amd import synth
+    steps = a.steps or 200
+    w = h = 256
+    img = synth.gradient_image(w, h, SEED)
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(img.reshape(-1)).to(dev)
+    enc = A.Encoder(0)
+    enc.load_predictor(4, 4)
+    stream = torch.cuda.Stream(dev)
+    res, outs = {}, {}
+    for graph in (False, True):
+        enc.configure(w, h, 4, 4, policy=A.POLICY_LEARNED, graph=graph)
+        d_out = torch.zeros(enc.num_blocks * 16, dtype=torch.uint8, device=dev)
+        for _ in range(max(a.warmup, 1)):
+            enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        res["graph" if graph else "eager"] = {"ms": ms, "value": w * h / (ms / 1e3) / 1e6}
+        outs[graph] = d_out.cpu().numpy()
+    assert np.array_equal(outs[False], outs[True])
+    r = res["graph"]
+    line = {"metric": METRIC, "value": r["value"], "unit": "Mtexel/s", "n_gpus": 1, "steps": steps,
+            "warmup": a.warmup, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "repeated encodes of one 256x256 RGBA8 texture, 4x4, learned policy, same buffers",
+                       "eager_mtexel_s": res["eager"]["value"], "eager_ms": res["eager"]["ms"],
+                       "graph_mtexel_s": res["graph"]["value"], "graph_ms": res["graph"]["ms"]}}
+    enc.close()
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this encoder is measured on one GPU only (it would shard by block range or by "
               "texture with no collectives); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "small":
+        bench_small(a)
+        return
     if a.workload != "image":
         bench_batch(a)
         return

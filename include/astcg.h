@@ -92,6 +92,13 @@ void astcg_default_learned_params(astcg_learned_params* p);
  * caller stream (not the null stream); ignored together with
  * ASTCG_FLAG_PROFILE.  No counterpart in the reference. */
 #define ASTCG_FLAG_GRAPH 8u
+/* After the search, try single-partition dual-plane block modes (one
+ * channel on a second weight plane, direct RGB / RGBA endpoints) for every
+ * block and keep one where it lowers the block's error.  The reference never
+ * tries dual-plane modes (code/compression_pipeline.cpp:183-185); footprints
+ * up to 64 texels.  astcg_get_symbolic does not represent the plane-2
+ * channel of such blocks. */
+#define ASTCG_FLAG_DUAL_PLANE 16u
 
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */

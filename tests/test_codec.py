@@ -354,3 +354,28 @@ def test_void_extent_layout_decodes_to_its_colour():
     for bx, by in [(4, 4), (6, 6), (12, 12)]:
         dec = A.decode_image(bx, by, bx, by, blk)
         assert (dec == np.array(rgba, np.uint8)).all()
+
+
+def test_predictor_files_carry_their_pruning():
+    """The committed predictor files load without pickles and carry weights
+    of the right shapes plus the pruning they were tuned with."""
+    for bx, by, nmodes in [(4, 4, 53), (6, 6, 198)]:
+        params, hidden, prune = A.load_predictor_file(A.predictor_path(bx, by))
+        nin = 3 * bx * by
+        assert [p.shape for p in params] == [(nin, hidden), (hidden,), (hidden, hidden), (hidden,),
+                                             (hidden, 4 + 3 * 1024 + nmodes), (4 + 3 * 1024 + nmodes,)]
+        assert all(p.dtype == np.float32 and np.isfinite(p).all() for p in params)
+        assert prune["seeds"] == 0 and 1 <= prune["modes"] <= 64 and 1 <= prune["slots"] <= 8
+        assert prune["early_out"] > 0
+
+
+def test_batch_spec_is_reproducible():
+    s1 = synth.batch_spec(64, 128, 2048, [(4, 4), (6, 6), (8, 8)], 1000, 2)
+    s2 = synth.batch_spec(64, 128, 2048, [(4, 4), (6, 6), (8, 8)], 1000, 2)
+    assert s1 == s2 and len(s1) == 64
+    assert all(128 <= w <= 2048 and 128 <= h <= 2048 for _, w, h, _, _ in s1)
+    assert {fp for *_, fp, _ in s1} == {(4, 4), (6, 6), (8, 8)}
+    assert [k for k, *_ in s1[:4]] == ["gradient", "gradient", "normal", "flat"]
+    imgs = synth.make_textures([("flat", 9, 7, (4, 4), 5), ("normal", 8, 8, (4, 4), 6)], workers=2)
+    assert np.array_equal(imgs[0], synth.flat_noise_image(9, 7, 5))
+    assert np.array_equal(imgs[1], synth.normal_map_image(8, 8, 6))

@@ -1,0 +1,95 @@
+"""ASTCG_FLAG_DUAL_PLANE: after the search, single-partition dual-plane
+block modes are tried for every block and kept where they lower the
+block's error.  The reference never tries them
+(code/compression_pipeline.cpp:183-185), so this is checked against the
+plain encode and the host ASTC decoder: every block stays legal, blocks the
+dual-plane search did not improve are bit-identical to the plain output,
+replaced blocks decode to a lower error, and the image PSNR does not drop."""
+import numpy as np
+import pytest
+
+import astc_implementation_amd as A
+from astc_implementation_amd import synth
+
+pytestmark = pytest.mark.gpu
+CW = np.array([0.30 * 2.25, 0.59 * 2.25, 0.11 * 2.25, 1.0])
+
+
+def block_errors(img, dec, bx, by):
+    """Channel-weighted squared error per block (edge blocks: their texels
+    inside the image)."""
+    h, w = img.shape[:2]
+    d = ((img.astype(np.float64) - dec.astype(np.float64)) ** 2 * CW).sum(axis=2)
+    bxn, byn = (w + bx - 1) // bx, (h + by - 1) // by
+    pad = np.zeros((byn * by, bxn * bx))
+    pad[:h, :w] = d
+    return pad.reshape(byn, by, bxn, bx).sum(axis=(1, 3)).reshape(-1)
+
+
+def dual_bit(blocks):
+    """Dual-plane flag of each block's 11-bit mode (ASTC block mode layouts:
+    bit 10, except the layouts with bits 1..0 == 0 and bits 8..7 == 10,
+    which have no dual plane)."""
+    b = blocks.reshape(-1, 16)
+    mode = b[:, 0].astype(np.int32) | ((b[:, 1].astype(np.int32) & 0x7) << 8)
+    no_d = ((mode & 3) == 0) & (((mode >> 7) & 3) == 2)
+    return ((mode >> 10) & 1).astype(bool) & ~no_d & ((mode & 0x1FF) != 0x1FC)
+
+
+@pytest.fixture(scope="module")
+def encoder():
+    enc = A.Encoder(0)
+    yield enc
+    enc.close()
+
+
+@pytest.mark.parametrize("bx,by", [(4, 4), (6, 6), (8, 8), (5, 4)])
+def test_dual_plane_blocks(encoder, bx, by):
+    # alpha independent of the colour (radial gradient + noise) and a normal
+    # map (two uncorrelated channels) side by side
+    img = np.concatenate([synth.alpha_image(130, 97, 4), synth.normal_map_image(130, 97, 5)], axis=1)
+    w, h = img.shape[1], img.shape[0]
+    encoder.configure(w, h, bx, by)
+    plain = encoder.encode(img)
+    encoder.configure(w, h, bx, by, dual_plane=True)
+    dual = encoder.encode(img)
+    dec_p = A.decode_image(bx, by, w, h, plain)
+    dec_d = A.decode_image(bx, by, w, h, dual)
+    assert not np.all(dec_d == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
+    assert np.array_equal(encoder.decode(bx, by, w, h, dual), dec_d)
+    changed = (plain.reshape(-1, 16) != dual.reshape(-1, 16)).any(axis=1)
+    assert changed.sum() > 0
+    assert dual_bit(dual)[changed].all()
+    assert not dual_bit(plain).any()
+    ep, ed = block_errors(img, dec_p, bx, by), block_errors(img, dec_d, bx, by)
+    # the search scores candidates with the reference's float weight infill
+    # (pass17), the decoder infills in integers: a weight can land one step
+    # away on rounding ties, so single blocks may decode slightly worse than
+    # they scored; together the replaced blocks decode better, and no block
+    # is grossly worse (a wrong plane channel or weight order would be)
+    assert ed[changed].sum() < ep[changed].sum()
+    assert (ed[changed] <= 4.0 * ep[changed] + 1024.0).all()
+    assert A.psnr(dec_d, img) >= A.psnr(dec_p, img)
+
+
+def test_dual_plane_in_batches_and_learned(encoder):
+    imgs = [synth.alpha_image(64, 52, 8), synth.normal_map_image(40, 44, 9)]
+    fps = [(4, 4), (6, 6)]
+    outs = encoder.encode_batch(imgs, fps, dual_plane=True)
+    for img, (bx, by), out in zip(imgs, fps, outs):
+        h, w = img.shape[:2]
+        encoder.configure(w, h, bx, by, dual_plane=True)
+        assert np.array_equal(out, encoder.encode(img))
+    encoder.load_predictor(4, 4)
+    encoder.configure(64, 52, 4, 4, policy=A.POLICY_LEARNED, dual_plane=True)
+    learned = encoder.encode(imgs[0])
+    assert dual_bit(learned).any()
+    dec = A.decode_image(4, 4, 64, 52, learned)
+    assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
+
+
+def test_symbolic_output_refused_with_dual_plane(encoder):
+    encoder.configure(16, 16, 4, 4, dual_plane=True)
+    encoder.encode(synth.alpha_image(16, 16, 3))
+    with pytest.raises(A.AstcError):
+        encoder.symbolic()

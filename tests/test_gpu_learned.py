@@ -180,3 +180,27 @@ def test_early_out_skips_counts_below_the_error_limit(setup):
     searched = ~skipped2
     assert not ((lab[searched, 0] == 1) & (err[searched] < limit) & (errors[searched, 0, :] >= 1e29).all(axis=1)).any()
     assert p_plain - A.psnr(enc.decode(4, 4, 512, 512, blocks), img) < 0.5
+
+
+@pytest.mark.parametrize("bx,by", [(4, 4), (6, 6)])
+def test_shipped_predictor_settings(bx, by):
+    """The committed predictors with the pruning saved in their files
+    (Encoder.load_predictor), on an image outside their training set: the
+    decoded PSNR stays within 0.1 dB of the plain search, every block is
+    a legal encoding and the early out skips partitioned searches for part
+    of the blocks."""
+    img = synth.gradient_image(384, 320, 7)
+    enc = A.Encoder(0)
+    enc.configure(384, 320, bx, by)
+    plain = enc.encode(img)
+    p_plain = A.psnr(enc.decode(bx, by, 384, 320, plain), img)
+    enc.load_predictor(bx, by)
+    enc.configure(384, 320, bx, by, policy=A.POLICY_LEARNED, labels=True)
+    learned = enc.encode(img)
+    dec = enc.decode(bx, by, 384, 320, learned)
+    assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
+    assert p_plain - A.psnr(dec, img) <= 0.1, (p_plain, A.psnr(dec, img))
+    _, errors = enc.candidates()
+    skipped = (errors[:, 0, :] >= 1e29).all(axis=1)
+    assert 0 < skipped.mean() < 1
+    enc.close()
