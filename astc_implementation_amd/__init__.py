@@ -410,8 +410,18 @@ def store_image(bx, by, width, height, blocks, path):
     _check(library().astcg_store_image(bx, by, width, height, _ptr(blocks), blocks.nbytes, path.encode()))
 
 
-def psnr(a, b, channels=4):
+# channel weights the search minimises (code/astc.h:60-63)
+CHANNEL_WEIGHTS = (0.30 * 2.25, 0.59 * 2.25, 0.11 * 2.25, 1.0)
+
+
+def psnr(a, b, channels=4, weights=None):
+    """PSNR (peak 255) over the first `channels` channels; with `weights`,
+    of the weighted mean of the per-channel squared errors."""
     a = a[..., :channels].astype(np.float64)
     b = b[..., :channels].astype(np.float64)
-    mse = np.mean((a - b) ** 2)
+    if weights is None:
+        mse = np.mean((a - b) ** 2)
+    else:
+        wv = np.asarray(weights[:channels], np.float64)
+        mse = float((np.mean((a - b) ** 2, axis=tuple(range(a.ndim - 1))) * wv).sum() / wv.sum())
     return float("inf") if mse == 0 else 10.0 * np.log10(255.0 ** 2 / mse)

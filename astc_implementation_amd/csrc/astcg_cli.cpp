@@ -96,8 +96,22 @@ bool load_pnm(const std::vector<uint8_t>& d, int& w, int& h, std::vector<uint8_t
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 5) {
-        std::fprintf(stderr, "usage: %s <input.tga|ppm|pam> <output.astc> <block_x> <block_y>\n", argv[0]);
+    // options after the reference CLI's four arguments (code/main.cpp:225-340):
+    // --partitions N (1..4), --void-extent, --dual-plane
+    astcg_params p;
+    astcg_default_params(&p);
+    bool ok = argc >= 5;
+    for (int i = 5; ok && i < argc; ++i) {
+        if (!std::strcmp(argv[i], "--void-extent")) p.flags |= ASTCG_FLAG_VOID_EXTENT;
+        else if (!std::strcmp(argv[i], "--dual-plane")) p.flags |= ASTCG_FLAG_DUAL_PLANE;
+        else if (!std::strcmp(argv[i], "--partitions") && i + 1 < argc) p.max_partitions = (uint32_t)std::atoi(argv[++i]);
+        else ok = false;
+    }
+    if (!ok) {
+        std::fprintf(stderr,
+                     "usage: %s <input.tga|ppm|pam> <output.astc> <block_x> <block_y> "
+                     "[--partitions N] [--void-extent] [--dual-plane]\n",
+                     argv[0]);
         return 1;
     }
     const int bx = std::atoi(argv[3]), by = std::atoi(argv[4]);
@@ -110,8 +124,6 @@ int main(int argc, char** argv) {
     }
     astcg_encoder* enc = nullptr;
     if (astcg_create(0, &enc) != ASTCG_OK) { std::fprintf(stderr, "no HIP device\n"); return 1; }
-    astcg_params p;
-    astcg_default_params(&p);
     astcg_status st = astcg_configure(enc, (uint32_t)w, (uint32_t)h, (uint32_t)bx, (uint32_t)by, &p);
     if (st != ASTCG_OK) {
         std::fprintf(stderr, "configure: %s (%s)\n", astcg_status_string(st), astcg_last_error(enc));

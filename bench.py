@@ -118,21 +118,33 @@ def measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, policy, copy_probe
         enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
     torch.cuda.synchronize()
     enc.reset_kernel_times()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(steps):
+    ev[0].record(stream)
+    for i in range(steps):
         enc.encode_device(d_in.data_ptr(), d_out.data_ptr(), stream.cuda_stream)
-    stop.record(stream)
+        ev[i + 1].record(stream)
     torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / steps
+    ms = ev[0].elapsed_time(ev[steps]) / steps
+    step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
     ktimes = enc.kernel_times()
     blocks = d_out.cpu().numpy()
     img = d_in.cpu().numpy().reshape(h, w, 4)
     dec = enc.decode(bx, by, w, h, blocks)
     fb = enc.fallback_rate() if policy == A.POLICY_LEARNED else None
-    return {"ms": ms, "value": w * h / (ms / 1e3) / 1e6, "ktimes": ktimes, "psnr": A.psnr(dec, img), "fallback": fb}
+    return {"ms": ms, "median_ms": float(np.median(step_ms)), "value": w * h / (ms / 1e3) / 1e6, "ktimes": ktimes,
+            "psnr": A.psnr(dec, img), "psnr_weighted": A.psnr(dec, img, weights=A.CHANNEL_WEIGHTS), "fallback": fb}
+
+
+def end_to_end(enc, A, img, bx, by, policy):
+    """Host RGBA8 in, host blocks out (upload, encode, download; astcg_encode):
+    Mtexel/s of one call after a warm-up call."""
+    h, w = img.shape[:2]
+    enc.configure(w, h, bx, by, policy=policy)
+    enc.encode(img)
+    t0 = time.perf_counter()
+    enc.encode(img)
+    return w * h / (time.perf_counter() - t0) / 1e6
 
 
 def bench_batch(a):
@@ -341,6 +353,11 @@ def main():
                    "psnr_plain_db": round(runs["plain"]["psnr"], 4) if "plain" in runs else None,
                    "psnr_learned_db": round(runs["learned"]["psnr"], 4) if "learned" in runs else None,
                    "dpsnr_db": round(dpsnr, 4) if dpsnr is not None else None,
+                   "psnr_weighted_plain_db": round(runs["plain"]["psnr_weighted"], 4) if "plain" in runs else None,
+                   "psnr_weighted_learned_db": round(runs["learned"]["psnr_weighted"], 4) if "learned" in runs else None,
+                   "median_ms_per_step": r["median_ms"],
+                   "end_to_end_mtexel_s": end_to_end(enc, A, img, bx, by, A.POLICY_LEARNED if head == "learned"
+                                                     else A.POLICY_COMPAT),
                    "learned_fallback_rate": runs["learned"]["fallback"] if "learned" in runs else None,
                    "learned_pruning": pruning,
                    "other_footprints": others},

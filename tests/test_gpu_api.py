@@ -102,3 +102,25 @@ def test_cli_round_trip(tmp_path):
     ref, _, _ = O.encode(img, 6, 6, 4, threads=4)
     assert data[:16] == A.astc_file_bytes(6, 6, 40, 24, ref)[:16]
     assert data[16:] == ref.tobytes()
+
+
+def test_cli_options(tmp_path):
+    """--partitions, --void-extent and --dual-plane reach the encoder: the
+    file equals the Python API's encode with the same settings."""
+    img = synth.alpha_image(40, 24, 22)
+    img[:8, :8] = (9, 99, 199, 255)
+    tga = bytearray([0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 40, 0, 24, 0, 32, 0x28])
+    tga += img[..., [2, 1, 0, 3]].tobytes()
+    src = tmp_path / "in.tga"
+    src.write_bytes(bytes(tga))
+    out = tmp_path / "out.astc"
+    cli = os.path.join(ROOT, "astc_implementation_amd", "astcg_cli")
+    subprocess.run([cli, str(src), str(out), "4", "4", "--partitions", "2", "--void-extent", "--dual-plane"],
+                   check=True, timeout=120)
+    enc = A.Encoder(0)
+    enc.configure(40, 24, 4, 4, max_partitions=2, void_extent=True, dual_plane=True)
+    ref = enc.encode(img)
+    enc.close()
+    assert out.read_bytes()[16:] == ref.tobytes()
+    bad = subprocess.run([cli, str(src), str(out), "4", "4", "--bogus"], timeout=120)
+    assert bad.returncode == 1
