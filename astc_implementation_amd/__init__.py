@@ -90,6 +90,8 @@ class LearnedParams(ctypes.Structure):
         ("partitionings", ctypes.c_uint32),
         ("refine_partitionings", ctypes.c_uint32),
         ("early_out", ctypes.c_float),
+        ("full_search_above", ctypes.c_float),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 
@@ -289,7 +291,7 @@ class Encoder:
         """Installs predictor weights (six fp32 arrays W1, b1, W2, b2, W3, b3)
         for footprint bx x by; keyword arguments override the pruning
         defaults (seeds, modes, slots, fallback_below, count_below,
-        partitionings, refine_partitionings, early_out)."""
+        partitionings, refine_partitionings, early_out, full_search_above)."""
         flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
         lp = LearnedParams()
         library().astcg_default_learned_params(ctypes.byref(lp))
@@ -359,6 +361,8 @@ def load_predictor_file(path):
         prune.update(refine_partitionings=int(z["prune_r"]))
     if "prune_e" in z.files:
         prune.update(early_out=float(z["prune_e"]))
+    if "prune_h" in z.files:
+        prune.update(full_search_above=float(z["prune_h"]))
     return params, int(z["hidden"]), prune
 
 

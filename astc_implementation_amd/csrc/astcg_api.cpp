@@ -552,12 +552,14 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     pl.w3_seed = static_cast<const uint16_t*>(P.w3_seed);
     pl.b3_seed = P.bias[2] + 4;
     pl.keep_refine = prune_refine ? (int)lp.refine_partitionings : 4;
-    if (lp.early_out > 0.0f) {
+    {
         // the reference's error limit (code/compression_pipeline.cpp:276-282, 330)
         const float texels = (float)(F.fp->bx * F.fp->by);
         const float db = 80.0f - 19.0f * (logf(texels) / logf(10.0f));
         const float weights_sum = 0.30f * 2.25f + 0.59f * 2.25f + 0.11f * 2.25f + 1.0f;  // astc.h:60-63
-        pl.error_limit = lp.early_out * powf(0.1f, db * 0.1f) * 65535.0f * 65535.0f * texels * weights_sum;
+        const float limit = powf(0.1f, db * 0.1f) * 65535.0f * 65535.0f * texels * weights_sum;
+        pl.error_limit = lp.early_out > 0.0f ? lp.early_out * limit : 0.0f;
+        pl.full_limit = lp.full_search_above > 0.0f ? lp.full_search_above * limit : 0.0f;
     }
     pl.best = e->d_best;
     pl.pb_est = prune_refine ? reinterpret_cast<float*>(pb + o_est) : nullptr;
@@ -1071,7 +1073,7 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     astcg_default_learned_params(&q);
     if (lp) q = *lp;
     if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
-        q.refine_partitionings > 4 || !(q.early_out >= 0.0f))
+        q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f))
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");
@@ -1084,6 +1086,12 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
     const int key = (int)(bx * 16 + by);
+    // captured launch chains hold the old predictor's buffers: drop them all
+    if (!e->graphs.empty()) {
+        (void)hipDeviceSynchronize();
+        for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
+        e->graphs.clear();
+    }
     auto old = e->preds.find(key);
     if (old != e->preds.end()) {
         (void)hipDeviceSynchronize();

@@ -110,6 +110,9 @@ struct SearchPlan {
     // early out: blocks whose best error so far is below error_limit skip
     // the remaining partition counts (0 = off)
     float error_limit;
+    // blocks whose best error after one partition is above full_limit get
+    // the full search for 2..4 partitions (0 = off)
+    float full_limit;
     const SymBest* best;        // running best per block (all blocks of the launch chain)
 };
 

@@ -2594,6 +2594,18 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
         const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
         if (P.error_limit > 0.0f && P.best[A.blk0 + k].error < P.error_limit) {
             n = 0;  // early out: the block's best encoding so far is good enough
+        } else if (pc == 2 && P.full_limit > 0.0f && !P.fallback[k] && P.best[A.blk0 + k].error > P.full_limit) {
+            // a hard block: the full search from 2 partitions on (all trial
+            // modes, 8 slots; later counts see the fallback flag)
+            P.fallback[k] = 1;
+            atomicAdd(P.fb_count, 1ull);
+            uint32_t* m = const_cast<uint32_t*>(P.mode_mask) + (size_t)k * P.mask_words;
+            for (int w = 0; w < P.mask_words; ++w) {
+                const int lo = 32 * w;
+                m[w] = T.nbm - lo >= 32 ? 0xFFFFFFFFu : (T.nbm > lo ? (1u << (T.nbm - lo)) - 1u : 0u);
+            }
+            const_cast<uint8_t*>(P.slots)[k] = 8;
+            n = 4;
         } else if (P.fallback[k]) {
             n = 4;
         } else {

@@ -76,6 +76,10 @@ typedef struct {
                                       those whose best estimated candidate is lowest; 0 = all (default) */
     float early_out;        /* > 0: a block whose best error after a partition count is below early_out
                                times the reference's error limit skips the remaining counts; 0 = off */
+    float full_search_above; /* > 0: a block whose best error after one partition is above this multiple
+                                of the error limit searches 2..4 partitions with every trial mode and 8
+                                slots (the COMPAT search); 0 = off */
+    uint32_t reserved[3];
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);

@@ -94,3 +94,22 @@ def test_graph_replays_mixed_batch(encoder):
         for j, (o, r) in enumerate(zip(d_out, refs)):
             assert np.array_equal(o.cpu().numpy(), r), "replay %d texture %d" % (i, j)
         _eager_work()
+
+
+def test_new_predictor_drops_captured_chains(encoder):
+    """Installing another predictor invalidates captured chains: the next
+    encode runs (and captures) the new network, matching its eager encode."""
+    w, h = 96, 80
+    img = synth.gradient_image(w, h, 24)
+    params, hidden, prune = A.load_predictor_file(A.predictor_path(4, 4))
+    zero = [np.zeros_like(p) for p in params]
+    outs = {}
+    for graph in (False, True):
+        for name, ps in (("trained", params), ("zero", zero)):
+            encoder.set_predictor(4, 4, ps, hidden=hidden, **prune)
+            encoder.configure(w, h, 4, 4, policy=A.POLICY_LEARNED, graph=graph)
+            outs[(graph, name)] = [encoder.encode(img) for _ in range(2)]
+    for name in ("trained", "zero"):
+        for o in outs[(True, name)]:
+            assert np.array_equal(o, outs[(False, name)][0])
+    assert not np.array_equal(outs[(False, "trained")][0], outs[(False, "zero")][0])

@@ -204,3 +204,26 @@ def test_shipped_predictor_settings(bx, by):
     skipped = (errors[:, 0, :] >= 1e29).all(axis=1)
     assert 0 < skipped.mean() < 1
     enc.close()
+
+
+def test_full_search_above_marks_hard_blocks(setup):
+    """full_search_above: blocks whose one-partition error is above the
+    multiple of the error limit are searched like the plain search from two
+    partitions on (counted as fallback); a multiple no block reaches changes
+    nothing."""
+    enc, img, params, hidden, _ = setup
+    limit = _reference_error_limit(4, 4)
+    base = dict(fallback_below=0.0, modes=8, slots=4)
+    enc.set_predictor(4, 4, params, hidden=hidden, **base)
+    enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+    ref = enc.encode(img)
+    enc.set_predictor(4, 4, params, hidden=hidden, full_search_above=1e12, **base)
+    enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+    assert np.array_equal(enc.encode(img), ref)
+    assert enc.fallback_rate() == 0.0
+    enc.set_predictor(4, 4, params, hidden=hidden, full_search_above=20.0, **base)
+    enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+    out = enc.encode(img)
+    rate = enc.fallback_rate()
+    assert 0.0 < rate < 1.0
+    assert A.psnr(enc.decode(4, 4, 512, 512, out), img) >= A.psnr(enc.decode(4, 4, 512, 512, ref), img)
