@@ -91,7 +91,8 @@ class LearnedParams(ctypes.Structure):
         ("refine_partitionings", ctypes.c_uint32),
         ("early_out", ctypes.c_float),
         ("full_search_above", ctypes.c_float),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("rounds", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -291,7 +292,8 @@ class Encoder:
         """Installs predictor weights (six fp32 arrays W1, b1, W2, b2, W3, b3)
         for footprint bx x by; keyword arguments override the pruning
         defaults (seeds, modes, slots, fallback_below, count_below,
-        partitionings, refine_partitionings, early_out, full_search_above)."""
+        partitionings, refine_partitionings, early_out, full_search_above,
+        rounds)."""
         flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
         lp = LearnedParams()
         library().astcg_default_learned_params(ctypes.byref(lp))
@@ -363,6 +365,8 @@ def load_predictor_file(path):
         prune.update(early_out=float(z["prune_e"]))
     if "prune_h" in z.files:
         prune.update(full_search_above=float(z["prune_h"]))
+    if "prune_n" in z.files:
+        prune.update(rounds=int(z["prune_n"]))
     return params, int(z["hidden"]), prune
 
 

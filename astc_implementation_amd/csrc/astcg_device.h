@@ -113,6 +113,7 @@ struct SearchPlan {
     // blocks whose best error after one partition is above full_limit get
     // the full search for 2..4 partitions (0 = off)
     float full_limit;
+    int iterations;             // refinement rounds per candidate (1..6)
     const SymBest* best;        // running best per block (all blocks of the launch chain)
 };
 

@@ -1754,7 +1754,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                                const Ideal* ideal, const int16_t* top, const Ccr* ccr,
                                                const uint8_t* qw_all, const int* work, const int* work_total,
-                                               CandOut* cand_out, int wstride) {
+                                               CandOut* cand_out, int wstride, int iterations) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr int pc = PC;
     const int tc = T.tc, lane = threadIdx.x;
@@ -1836,7 +1836,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
         for (int j = 0; j < n; ++j) w += ((float)qw[t2wi[i * 4 + j] * 64] / 64.0f) * t2ww[i * 4 + j];
         return w;
     };
-    for (int a = 0; a < 6; ++a) {
+    for (int a = 0; a < iterations; ++a) {  // 6 in the reference (pass13-17 rounds)
         // pass13_recompute_ideal_endpoints.wgsl:182-393
         float4 rgbs[4];
 #pragma unroll
@@ -2946,10 +2946,10 @@ const char* kernel_name(int id) {
 
 template <bool kStage>
 void launch_refine_pc(int pc, dim3 grid, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
-                      const ChunkBuffers& B, int npb) {
+                      const ChunkBuffers& B, int npb, int iterations) {
 #define ASTCG_REFINE(PCV)                                                                                         \
     hipLaunchKernelGGL((k_refine<kStage, PCV>), grid, dim3(64), lds, s, A, T, B.pb_part, npb, B.ideal, B.top, B.ccr, \
-                       B.qw, B.work, B.work_total, B.cand, B.wstride)
+                       B.qw, B.work, B.work_total, B.cand, B.wstride, iterations)
     switch (pc) {
         case 1: ASTCG_REFINE(1); break;
         case 2: ASTCG_REFINE(2); break;
@@ -2960,9 +2960,9 @@ void launch_refine_pc(int pc, dim3 grid, size_t lds, hipStream_t s, const ImageA
 }
 
 void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
-                   const ChunkBuffers& B, int npb) {
-    if (stage) launch_refine_pc<true>(pc, grid, lds, s, A, T, B, npb);
-    else launch_refine_pc<false>(pc, grid, lds, s, A, T, B, npb);
+                   const ChunkBuffers& B, int npb, int iterations) {
+    if (stage) launch_refine_pc<true>(pc, grid, lds, s, A, T, B, npb, iterations);
+    else launch_refine_pc<false>(pc, grid, lds, s, A, T, B, npb, iterations);
 }
 
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
@@ -3029,7 +3029,8 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             if (prof) prof->begin(K_REFINE, s);
             const size_t lds = refine_lds_bytes(T, B.wstride);
             const unsigned grid = blocks_for((long long)npb * 8 + (long long)T.nbm * 63);
-            launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb);
+            launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb,
+                          plan ? plan->iterations : 6);
             if (prof) prof->end(K_REFINE, s);
         }
         ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best, lab);

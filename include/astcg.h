@@ -79,7 +79,8 @@ typedef struct {
     float full_search_above; /* > 0: a block whose best error after one partition is above this multiple
                                 of the error limit searches 2..4 partitions with every trial mode and 8
                                 slots (the COMPAT search); 0 = off */
-    uint32_t reserved[3];
+    uint32_t rounds;        /* refinement rounds per candidate (pass13-17), 1..6, 0 = 6 as the reference */
+    uint32_t reserved[2];
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);
