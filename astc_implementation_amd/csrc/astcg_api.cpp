@@ -368,6 +368,7 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[7] = L.add<CandOut>(npb * 8);
     offs[8] = L.add<int>(2 * (size_t)f.nbm + 1);
     offs[9] = L.add<int>(npb * 8 + (size_t)f.nbm * 64);
+    offs[11] = L.add<float>(npb * 8);
     return L.size;
 }
 
@@ -392,7 +393,7 @@ astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpS
 // Blocks per chunk: keeps the per-chunk working set near 8 GiB (a 4096^2
 // image at 4x4 takes two chunks).
 int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested) {
-    size_t offs[11];
+    size_t offs[12];
     const size_t fixed = chunk_bytes_per_block(f, offs, 0);
     const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) - fixed) >> 16;
     int chunk = (int)std::max<size_t>(1024, ((size_t(8) << 30) - fixed) / per_block);
@@ -405,7 +406,7 @@ int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested) {
 // layout changes (k_bins clears them after every use otherwise).
 astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStream_t s) {
     if (e->layout_fp == &F && e->layout_chunk == chunk) return ASTCG_OK;
-    size_t offs[11];
+    size_t offs[12];
     const size_t total = chunk_bytes_per_block(*F.fp, offs, chunk);
     hipError_t err = grow(&e->d_chunk, &e->chunk_cap, total);
     if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
@@ -420,6 +421,7 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
     B.qw = b + offs[5];
     B.top = reinterpret_cast<int16_t*>(b + offs[6]);
     B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
+    B.cand_err = reinterpret_cast<float*>(b + offs[11]);
     B.bin_count = reinterpret_cast<int*>(b + offs[8]);
     B.bin_cur = B.bin_count + F.fp->nbm;
     B.work_total = B.bin_cur + F.fp->nbm;

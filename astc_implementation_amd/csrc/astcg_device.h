@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include <cstddef>
+
 #include "astcg_tables.h"
 
 namespace astcg {
@@ -149,9 +151,11 @@ struct CandOut {
     uint8_t quant;     // final colour quant level
     uint16_t formats;  // 4 x 4 bits
     uint16_t pad;
-    uint8_t packed[32];
+    uint8_t packed[32];  // packed and qw are copied as one 96-byte run of dwords
     uint8_t qw[64];
 };
+static_assert(offsetof(CandOut, packed) % 4 == 0 && offsetof(CandOut, qw) == offsetof(CandOut, packed) + 32,
+              "CandOut packed/qw layout");
 
 // Best symbolic block so far, per image block.
 struct SymBest {
@@ -165,6 +169,8 @@ struct SymBest {
     uint8_t packed[32];
     uint8_t qw[64];
 };
+static_assert(offsetof(SymBest, packed) % 4 == 0 && offsetof(SymBest, qw) == offsetof(SymBest, packed) + 32,
+              "SymBest packed/qw layout");
 
 struct ChunkBuffers {
     int16_t* pb_part;   // [4 * max_nb]
@@ -176,6 +182,7 @@ struct ChunkBuffers {
     uint8_t* qw;        // [qw_stride][npb of the launch]: row qw_off[mode] + weight
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]
+    float* cand_err;    // [4 * max_nb][8] each slot's CandOut::error again, packed for k_pick
     int* bin_count;     // [nbm] candidates per trial mode (zero between launches)
     int* bin_cur;       // [nbm] next free work-list position per mode
     int* work_total;    // [1] work-list length including padding

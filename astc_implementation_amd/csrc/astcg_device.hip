@@ -1250,8 +1250,8 @@ __device__ __forceinline__ int pb_est_rank(const ImageArgs& A, int pc, int pb, c
 // that size the refinement work list (LDS histogram, one global add per
 // workgroup and mode).  Slots without a candidate get the default error.
 __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr, int16_t* top,
-                                            int* bin_count, CandOut* cand, const uint32_t* mode_mask, int mask_words,
-                                            const uint8_t* slots, const float* pb_est, int keep_refine,
+                                            int* bin_count, CandOut* cand, float* cand_err, const uint32_t* mode_mask,
+                                            int mask_words, const uint8_t* slots, const float* pb_est, int keep_refine,
                                             const uint8_t* fallback) {
     extern __shared__ int s_hist[];  // [nbm]
     for (int i = threadIdx.x; i < nbm; i += 64) s_hist[i] = 0;
@@ -1291,7 +1291,10 @@ __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nb
             const int v = (te[i] < kErrDefault && i < nslots) ? ti[i] : -1;
             top[pb * 8 + i] = (int16_t)v;
             if (v >= 0) atomicAdd(&s_hist[v], 1);
-            else cand[pb * 8 + i].error = kErrDefault;  // pass12 produced no candidate for this slot
+            else {  // pass12 produced no candidate for this slot
+                cand[pb * 8 + i].error = kErrDefault;
+                cand_err[pb * 8 + i] = kErrDefault;
+            }
         }
     }
     __syncthreads();
@@ -1754,7 +1757,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                                const Ideal* ideal, const int16_t* top, const Ccr* ccr,
                                                const uint8_t* qw_all, const int* work, const int* work_total,
-                                               CandOut* cand_out, int wstride, int iterations) {
+                                               CandOut* cand_out, float* cand_err, int wstride, int iterations) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr int pc = PC;
     const int tc = T.tc, lane = threadIdx.x;
@@ -1956,10 +1959,16 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
         // remaining iterations are skipped.
         bool moved = false;
         for (int phase = (a == 0 ? 0 : 1); phase < 2; ++phase) {
+            float total = 0.0f;  // pass17's error sum
+            bool fused = false;  // pass17 already summed inside pass16's texel loop
             if (phase == 1) {
                 // pass16_realign_weights.wgsl:211-367
                 const int q = bmi.quant;
                 if (undecimated) {
+                    // every texel owns its weight, so texel t's final error
+                    // (pass17) depends only on its own realigned weight: both
+                    // passes share one walk over the texels, in texel order
+                    fused = true;
                     for (int t = 0; t < tc; ++t) {
                         const int uqw = qw[t * 64];
                         const uint32_t pn = c_qt.prev_next[q][uqw];
@@ -1983,6 +1992,14 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                         else if ((ed < eb) && (uqw > 0)) nq = down;
                         moved |= nq != uqw;
                         qw[t * 64] = (uint8_t)nq;
+                        // pass17_compute_final_error.wgsl:149-231 for this texel
+                        float dfe[4];
+                        for (int j = 0; j < 4; ++j) {
+                            const int col = (e0[j] * (64 - nq) + e1[j] * nq + 32) >> 6;
+                            const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
+                            dfe[j] = df * df;
+                        }
+                        total += wmin(dot4(dfe, cw), kErrDefault);
                     }
                 } else {
                     for (int we = 0; we < wc; ++we) {
@@ -2026,8 +2043,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                 }
             }
             // pass17_compute_final_error.wgsl:149-231
-            float total = 0.0f;
-            for (int i = 0; i < tc; ++i) {
+            for (int i = 0; i < tc && !fused; ++i) {
                 const float w = weight_of(i);
                 const int p = btp[i * 64];
                 const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
@@ -2067,11 +2083,13 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
         for (int i = 0; i < 8; ++i) best.packed[p * 8 + i] = (uint8_t)pk_byte(best_pk[p], i);
     for (int i = 0; i < 64; ++i) best.qw[i] = i < wc ? bqw[i * 64] : 0;
     cand_out[task] = best;
+    cand_err[task] = best.error;
 }
 
 // ---- pass18 + host argmin (compression_pipeline.cpp:499-519) --------------------
 __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, const int16_t* pb_part,
-                                             const CandOut* cand, SymBest* best, LabelBufs lab) {
+                                             const CandOut* cand, const float* cand_err, SymBest* best,
+                                             LabelBufs lab) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.nb) return;
     const int npos = pc == 1 ? 1 : 4;
@@ -2090,7 +2108,7 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         float be = kErrDefault;
         int bs = -1;
         for (int s = 0; s < 8; ++s) {
-            const float e = cand[pb * 8 + s].error;
+            const float e = cand_err[pb * 8 + s];
             if (e < be) { be = e; bs = s; }
         }
         if (lab.cand_seed && pc > 1) {
@@ -2124,8 +2142,10 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         sb.matched = co.matched;
         sb.quant = co.quant;
         sb.formats = co.formats;
-        for (int i = 0; i < 32; ++i) sb.packed[i] = co.packed[i];
-        for (int i = 0; i < 64; ++i) sb.qw[i] = co.qw[i];
+        // 96 bytes as 24 dwords (both records keep the arrays 4-byte aligned)
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(co.packed);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(sb.packed);
+        for (int i = 0; i < 24; ++i) dst[i] = src[i];
     }
 }
 
@@ -2949,7 +2969,7 @@ void launch_refine_pc(int pc, dim3 grid, size_t lds, hipStream_t s, const ImageA
                       const ChunkBuffers& B, int npb, int iterations) {
 #define ASTCG_REFINE(PCV)                                                                                         \
     hipLaunchKernelGGL((k_refine<kStage, PCV>), grid, dim3(64), lds, s, A, T, B.pb_part, npb, B.ideal, B.top, B.ccr, \
-                       B.qw, B.work, B.work_total, B.cand, B.wstride, iterations)
+                       B.qw, B.work, B.work_total, B.cand, B.cand_err, B.wstride, iterations)
     switch (pc) {
         case 1: ASTCG_REFINE(1); break;
         case 2: ASTCG_REFINE(2); break;
@@ -3017,7 +3037,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                                    mode_mask, mask_words, pb_est);
             const size_t hist = (size_t)T.nbm * sizeof(int);
             hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, A, pc, npb, T.nbm, B.ccr, B.top,
-                               B.bin_count, B.cand, mode_mask, mask_words, slots, pb_est,
+                               B.bin_count, B.cand, B.cand_err, mode_mask, mask_words, slots, pb_est,
                                plan ? plan->keep_refine : 4, plan ? plan->fallback : nullptr);
             hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), hist + sizeof(int), s, T.nbm, B.bin_count, B.bin_cur,
                                B.work, B.work_total);
@@ -3033,7 +3053,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                           plan ? plan->iterations : 6);
             if (prof) prof->end(K_REFINE, s);
         }
-        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, best, lab);
+        ASTCG_LAUNCH(K_PICK, k_pick, blocks_for(A.nb), A, T, pc, B.pb_part, B.cand, B.cand_err, best, lab);
     }
     if (dual_plane && T.ndual > 0 && T.tc <= kDualMaxTc) {
         if (prof) prof->begin(K_PICK, s);

@@ -227,3 +227,16 @@ def test_full_search_above_marks_hard_blocks(setup):
     rate = enc.fallback_rate()
     assert 0.0 < rate < 1.0
     assert A.psnr(enc.decode(4, 4, 512, 512, out), img) >= A.psnr(enc.decode(4, 4, 512, 512, ref), img)
+
+
+def test_rounds_six_is_the_default_and_fewer_rounds_stay_valid(setup):
+    enc, img, params, hidden, p_plain = setup
+    outs = {}
+    for r in (0, 6, 2):
+        enc.set_predictor(4, 4, params, hidden=hidden, fallback_below=0.0, rounds=r)
+        enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+        outs[r] = enc.encode(img)
+    assert np.array_equal(outs[0], outs[6])
+    dec = enc.decode(4, 4, 512, 512, outs[2])
+    assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
+    assert p_plain - A.psnr(dec, img) < 0.5
