@@ -1712,8 +1712,11 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 // interleaved loop).
 constexpr int kRefineStageMaxTc = 64;  // stage RGBA8 texels in LDS up to 8x8
 
-size_t refine_lds_bytes(const DevTables& T, int wstride) {
-    size_t bytes = 2 * 4 * 64 * sizeof(uchar4) + (size_t)T.tc * 64 + 2 * (size_t)wstride * 64;
+// LDS of one k_refine workgroup: staged texels, unpacked endpoints of the
+// pc partitions, the texel partition map (pc > 1 only), candidate and best
+// quantised weights.  A smaller 1-partition layout fits more workgroups per CU.
+size_t refine_lds_bytes(const DevTables& T, int wstride, int pc) {
+    size_t bytes = 2 * (size_t)pc * 64 * sizeof(uchar4) + (pc > 1 ? (size_t)T.tc * 64 : 0) + 2 * (size_t)wstride * 64;
     if (T.tc <= kRefineStageMaxTc) bytes += (size_t)T.tc * 64 * sizeof(uint32_t);
     return bytes;
 }
@@ -1771,15 +1774,15 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
     // per-lane LDS rows, laid out [index][lane]
     uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);
     uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(uint32_t) : 0));
-    uchar4* s_ue1 = s_ue0 + 4 * 64;
-    uint8_t* s_tp = reinterpret_cast<uint8_t*>(s_ue1 + 4 * 64);
-    uint8_t* s_qw = s_tp + (size_t)tc * 64;
+    uchar4* s_ue1 = s_ue0 + pc * 64;
+    uint8_t* s_tp = reinterpret_cast<uint8_t*>(s_ue1 + pc * 64);
+    uint8_t* s_qw = s_tp + (pc > 1 ? (size_t)tc * 64 : 0);
     uint8_t* s_bqw = s_qw + (size_t)wstride * 64;
     const int pb = task >> 3;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
     for (int i = 0; i < tc; ++i) {
         if (kStage) s_tex[i * 64 + lane] = texel_rgba8(b.loc, T, i);
-        s_tp[i * 64 + lane] = (uint8_t)texel_part(b, i);
+        if (pc > 1) s_tp[i * 64 + lane] = (uint8_t)texel_part(b, i);
     }
     auto px_of = [&](int i) -> float4 {
         if (kStage) {
@@ -1791,7 +1794,8 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
         texel_px(b.loc, T, i, px);
         return make_float4(px[0], px[1], px[2], px[3]);
     };
-    const uint8_t* btp = s_tp + lane;  // btp[i * 64]
+    const uint8_t* btp = s_tp + lane;  // btp[i * 64], pc > 1 only
+    auto part_of = [&](int i) -> int { return pc > 1 ? (int)btp[i * 64] : 0; };
     uint8_t* qw = s_qw + lane;
     uint8_t* bqw = s_bqw + lane;
     int ppc[4];
@@ -1974,7 +1978,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                         const uint32_t pn = c_qt.prev_next[q][uqw];
                         const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
                         const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
-                        const int p = btp[t * 64];
+                        const int p = part_of(t);
                         const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
                         const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
                         const float4 px4 = px_of(t);
@@ -2019,7 +2023,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                 if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
                             }
                             const float wdd = ddn * twb, wud = dup * twb;
-                            const int p = btp[t * 64];
+                            const int p = part_of(t);
                             const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
                             const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
                             const float4 px4 = px_of(t);
@@ -2045,7 +2049,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
             // pass17_compute_final_error.wgsl:149-231
             for (int i = 0; i < tc && !fused; ++i) {
                 const float w = weight_of(i);
-                const int p = btp[i * 64];
+                const int p = part_of(i);
                 const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
                 const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
                 const float4 px4 = px_of(i);
@@ -3047,7 +3051,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         }
         {
             if (prof) prof->begin(K_REFINE, s);
-            const size_t lds = refine_lds_bytes(T, B.wstride);
+            const size_t lds = refine_lds_bytes(T, B.wstride, pc);
             const unsigned grid = blocks_for((long long)npb * 8 + (long long)T.nbm * 63);
             launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb,
                           plan ? plan->iterations : 6);
