@@ -902,9 +902,12 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, in
 constexpr int kDmPbs = 64;
 constexpr int kXcds = 8;
 
-size_t dm_bm_lds_bytes(const DevTables& T, int wstride) {
-    return (size_t)kDmPbs * T.tc * sizeof(float) + 2 * 64 * (size_t)wstride * sizeof(float) +
-           (((size_t)kDmPbs * T.tc + 15) & ~size_t(15));
+// LDS of one k_dm_bm workgroup: decimated weights and scratch per lane, plus
+// the texel partition map for partitioned blocks.  The ideal texel weights
+// are read from global memory (coalesced, cache-resident) instead, which
+// lets more workgroups share a CU at the large footprints.
+size_t dm_bm_lds_bytes(const DevTables& T, int wstride, int pc) {
+    return 2 * 64 * (size_t)wstride * sizeof(float) + (pc > 1 ? (((size_t)kDmPbs * T.tc + 15) & ~size_t(15)) : 0);
 }
 
 unsigned dm_bm_grid(const DevTables& T, int npb) {
@@ -926,23 +929,17 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     const int pb0 = grp * kDmPbs;
     const int npb_live = pb_limit(A, pc, npb);
     if (pb0 >= npb_live) return;
-    float* s_wts = reinterpret_cast<float*>(smem);
-    float* s_dw = s_wts + kDmPbs * tc;
+    float* s_dw = reinterpret_cast<float*>(smem);
     float* s_aux = s_dw + 64 * wstride;
-    uint8_t* s_tpart = reinterpret_cast<uint8_t*>(s_aux + 64 * wstride);
-    {
+    uint8_t* s_tpart = reinterpret_cast<uint8_t*>(s_aux + 64 * wstride);  // pc > 1 only
+    if (pc > 1) {
         const int nl = min(kDmPbs, npb_live - pb0);
         for (int e = lane; e < kDmPbs * tc; e += 64) {
             const int t = e / kDmPbs, l = e - t * kDmPbs, pb = pb0 + l;
             if (l >= nl) continue;
-            s_wts[t * kDmPbs + l] = wts_all[(size_t)t * npb + pb];
-            int p = 0;
-            if (pc > 1) {
-                int kb, cb, slot;
-                pb_decode(A, pc, pb, &kb, &cb, &slot);
-                p = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[slot]) * tc + t];
-            }
-            s_tpart[t * kDmPbs + l] = (uint8_t)p;
+            int kb, cb, slot;
+            pb_decode(A, pc, pb, &kb, &cb, &slot);
+            s_tpart[t * kDmPbs + l] = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[slot]) * tc + t];
         }
     }
     __syncthreads();
@@ -962,12 +959,13 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
         }
         if (!any) return;
     }
-    const float* wts = s_wts + lane;  // wts[i * 64]
+    // ideal texel weights of this PB ([texel][PB]: coalesced across lanes)
+    auto wts_of = [&](int t) { return wts_all[(size_t)t * npb + pb]; };
     // pass01 stores lengthSqr[p] * error_weight for every texel of partition p
     const float4 wes4 = *reinterpret_cast<const float4*>(ideal[pb].wes);
     const uint8_t* tpart = s_tpart + lane;
     auto wes_of = [&](int t) {
-        const int p = tpart[t * 64];
+        const int p = pc > 1 ? tpart[t * 64] : 0;
         return p == 0 ? wes4.x : p == 1 ? wes4.y : p == 2 ? wes4.z : wes4.w;
     };
     float* dw = s_dw + lane;    // dw[i * 64]
@@ -990,7 +988,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     }
     // pass02_decimated_weights.wgsl:132-270
     if (tc == wc) {
-        for (int i = 0; i < wc; ++i) dw[i * 64] = wts[i * 64];
+        for (int i = 0; i < wc; ++i) dw[i * 64] = wts_of(i);
     } else {
         for (int w = 0; w < wc; ++w) {
             const int n = w2tc[w];
@@ -1001,7 +999,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 for (int jx = 0; jx < n; ++jx) {
                     const int t = w2ti[off + jx];
                     const float cw = w2tw[off + jx] * wes_of(t);
-                    ws += wts[t * 64] * cw;
+                    ws += wts_of(t) * cw;
                     tcon += cw;
                 }
                 v = ws / tcon;
@@ -1020,7 +1018,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
                 const int nt = t2wc[t];
                 for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
                 a += contrib * scale;
-                bacc += (infill - wts[t * 64]) * scale;
+                bacc += (infill - wts_of(t)) * scale;
             }
             dw[w * 64] = aux[w * 64] + wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
         }
@@ -1167,7 +1165,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
             for (int i = 0; i < tc; ++i) {
                 float v = 0.0f;
                 for (int jx = 0; jx < t2wc[i]; ++jx) v += aux[t2wi[i * 4 + jx] * 64] * t2ww[i * 4 + jx];
-                const float diff = v - wts[i * 64];
+                const float diff = v - wts_of(i);
                 err += diff * diff * wes_of(i);
             }
             we = err;
@@ -3028,7 +3026,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                      B.ct_fmt);
         {
             if (prof) prof->begin(K_DM_BM, s);
-            hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(T, npb)), dim3(64), dm_bm_lds_bytes(T, B.wstride),
+            hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(T, npb)), dim3(64), dm_bm_lds_bytes(T, B.wstride, pc),
                                s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw,
                                B.wstride, mode_mask, mask_words);
             if (prof) prof->end(K_DM_BM, s);
