@@ -280,6 +280,24 @@ hipError_t upload_tables(FpState& F) {
     const size_t o_qwo = L.add<uint16_t>(qw_off.size());
     const size_t o_s2p = L.add<uint16_t>(f.seed_to_packed.size());
     const size_t o_dual = L.add<DualMode>(std::max<size_t>(f.dual.size(), 1));
+    // decimation modes grouped by weight count for k_dm_bm's launches
+    std::vector<uint8_t> dm_order;
+    int group_start[4] = {0, 0, 0, 0}, group_w[3] = {4, 4, 4};
+    {
+        const int limits[3] = {16, 32, 1 << 30};
+        for (int g = 0; g < 3; ++g) {
+            group_start[g] = (int)dm_order.size();
+            for (int d = 0; d < f.ndm; ++d) {
+                const int wc = f.dm[d].weight_count;
+                if (wc <= limits[g] && (g == 0 || wc > limits[g - 1])) {
+                    dm_order.push_back((uint8_t)d);
+                    group_w[g] = std::max(group_w[g], (wc + 3) & ~3);
+                }
+            }
+        }
+        group_start[3] = (int)dm_order.size();
+    }
+    const size_t o_dmo = L.add<uint8_t>(std::max<size_t>(dm_order.size(), 1));
     std::vector<uint8_t> host(L.size, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
     put(o_dm, f.dm.data(), f.dm.size() * sizeof(DmInfo));
@@ -307,6 +325,7 @@ hipError_t upload_tables(FpState& F) {
     put(o_qwo, qw_off.data(), qw_off.size() * 2);
     put(o_s2p, f.seed_to_packed.data(), f.seed_to_packed.size() * 2);
     put(o_dual, f.dual.data(), f.dual.size() * sizeof(DualMode));
+    put(o_dmo, dm_order.data(), dm_order.size());
     hipError_t err = hipMalloc(&F.d_tables, L.size);
     if (err != hipSuccess) return err;
     err = hipMemcpy(F.d_tables, host.data(), L.size, hipMemcpyHostToDevice);
@@ -338,6 +357,9 @@ hipError_t upload_tables(FpState& F) {
     T.qw_off = reinterpret_cast<const uint16_t*>(b + o_qwo);
     T.seed_to_packed = reinterpret_cast<const uint16_t*>(b + o_s2p);
     T.dual = reinterpret_cast<const DualMode*>(b + o_dual);
+    T.dm_order = b + o_dmo;
+    for (int g = 0; g < 4; ++g) T.dm_group_start[g] = group_start[g];
+    for (int g = 0; g < 3; ++g) T.dm_group_w[g] = group_w[g];
     T.ndual = (int)f.dual.size();
     T.qw_stride = (qw_total + 15) & ~15;
     T.bx = f.bx;

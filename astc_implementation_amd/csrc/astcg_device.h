@@ -41,6 +41,12 @@ struct DevTables {
     int ndual;
     int bx, by, tc, ndm, nbm;
     int count_selected[4];
+    // k_dm_bm launches one group of decimation modes at a time, grouped by
+    // weight count (<= 16, <= 32, more), each with LDS rows sized for its
+    // widest grid: dm_order lists the modes group by group
+    const uint8_t* dm_order;      // [ndm]
+    int dm_group_start[4];        // group g = dm_order[dm_group_start[g] .. dm_group_start[g + 1])
+    int dm_group_w[3];            // LDS row (weights, multiple of 4) of group g
 };
 
 // One texture of a launch: its blocks occupy [first_block, first_block +

@@ -910,22 +910,26 @@ size_t dm_bm_lds_bytes(const DevTables& T, int wstride, int pc) {
     return 2 * 64 * (size_t)wstride * sizeof(float) + (pc > 1 ? (((size_t)kDmPbs * T.tc + 15) & ~size_t(15)) : 0);
 }
 
-unsigned dm_bm_grid(const DevTables& T, int npb) {
+unsigned dm_bm_grid(int npb, int ndm_launch) {
     const int groups = (npb + kDmPbs - 1) / kDmPbs;
-    return (unsigned)(((groups + kXcds - 1) / kXcds) * kXcds * T.ndm);
+    return (unsigned)(((groups + kXcds - 1) / kXcds) * kXcds * ndm_launch);
 }
 
 __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
                                               const Ideal* ideal, const float* wts_all, const float* ct_err,
                                               const uint16_t* ct_fmt, Ccr* ccr,
                                               uint8_t* qw_out, int wstride,
-                                              const uint32_t* mode_mask, int mask_words) {
+                                              const uint32_t* mode_mask, int mask_words, int dm_first,
+                                              int dm_count) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int ndm = T.ndm, tc = T.tc, lane = threadIdx.x;
     const int wg = blockIdx.x;
+    // workgroup -> (group of 64 PBs, decimation mode of this launch's group);
+    // consecutive workgroups go to different XCDs, so one XCD's workgroups
+    // share PB groups (and their rows in L2)
     const int slot = wg / kXcds, xcd = wg - slot * kXcds;
-    const int grp = xcd + kXcds * (slot / ndm);
-    const int m = slot - (slot / ndm) * ndm;
+    const int grp = xcd + kXcds * (slot / dm_count);
+    const int m = T.dm_order[dm_first + slot - (slot / dm_count) * dm_count];
     const int pb0 = grp * kDmPbs;
     const int npb_live = pb_limit(A, pc, npb);
     if (pb0 >= npb_live) return;
@@ -1708,7 +1712,7 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 // of pass13 are accumulated partition by partition (each partition still
 // sees its texels in index order, so the float results are those of the
 // interleaved loop).
-constexpr int kRefineStageMaxTc = 64;  // stage RGBA8 texels in LDS up to 8x8
+constexpr int kRefineStageMaxTc = 25;  // stage RGBA8 texels in LDS up to 5x5 (measured: larger blocks run faster unstaged, with more workgroups per CU)
 
 // LDS of one k_refine workgroup: staged texels, unpacked endpoints of the
 // pc partitions, the texel partition map (pc > 1 only), candidate and best
@@ -3026,9 +3030,14 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                      B.ct_fmt);
         {
             if (prof) prof->begin(K_DM_BM, s);
-            hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(T, npb)), dim3(64), dm_bm_lds_bytes(T, B.wstride, pc),
-                               s, A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw,
-                               B.wstride, mode_mask, mask_words);
+            for (int g = 0; g < 3; ++g) {
+                const int first = T.dm_group_start[g], count = T.dm_group_start[g + 1] - first;
+                if (count <= 0) continue;
+                const int w = T.dm_group_w[g];
+                hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(npb, count)), dim3(64), dm_bm_lds_bytes(T, w, pc), s, A, T,
+                                   pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw, w, mode_mask,
+                                   mask_words, first, count);
+            }
             if (prof) prof->end(K_DM_BM, s);
         }
         {
