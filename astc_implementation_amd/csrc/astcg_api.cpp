@@ -282,10 +282,11 @@ hipError_t upload_tables(FpState& F) {
     const size_t o_dual = L.add<DualMode>(std::max<size_t>(f.dual.size(), 1));
     // decimation modes grouped by weight count for k_dm_bm's launches
     std::vector<uint8_t> dm_order;
-    int group_start[4] = {0, 0, 0, 0}, group_w[3] = {4, 4, 4};
+    int group_start[kDmGroups + 1] = {}, group_w[kDmGroups] = {};
     {
-        const int limits[3] = {16, 32, 1 << 30};
-        for (int g = 0; g < 3; ++g) {
+        const int limits[kDmGroups] = {16, 24, 32, 1 << 30};
+        for (int g = 0; g < kDmGroups; ++g) {
+            group_w[g] = 4;
             group_start[g] = (int)dm_order.size();
             for (int d = 0; d < f.ndm; ++d) {
                 const int wc = f.dm[d].weight_count;
@@ -295,7 +296,7 @@ hipError_t upload_tables(FpState& F) {
                 }
             }
         }
-        group_start[3] = (int)dm_order.size();
+        group_start[kDmGroups] = (int)dm_order.size();
     }
     const size_t o_dmo = L.add<uint8_t>(std::max<size_t>(dm_order.size(), 1));
     std::vector<uint8_t> host(L.size, 0);
@@ -358,8 +359,8 @@ hipError_t upload_tables(FpState& F) {
     T.seed_to_packed = reinterpret_cast<const uint16_t*>(b + o_s2p);
     T.dual = reinterpret_cast<const DualMode*>(b + o_dual);
     T.dm_order = b + o_dmo;
-    for (int g = 0; g < 4; ++g) T.dm_group_start[g] = group_start[g];
-    for (int g = 0; g < 3; ++g) T.dm_group_w[g] = group_w[g];
+    for (int g = 0; g <= kDmGroups; ++g) T.dm_group_start[g] = group_start[g];
+    for (int g = 0; g < kDmGroups; ++g) T.dm_group_w[g] = group_w[g];
     T.ndual = (int)f.dual.size();
     T.qw_stride = (qw_total + 15) & ~15;
     T.bx = f.bx;

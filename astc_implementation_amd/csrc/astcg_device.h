@@ -11,6 +11,8 @@
 namespace astcg {
 
 // Read-only per-footprint tables in HBM (one allocation, see upload_tables).
+constexpr int kDmGroups = 4;  // weight-count groups of k_dm_bm's launches
+
 struct DevTables {
     const DmInfo* dm;
     const BmInfo* bm;
@@ -42,11 +44,11 @@ struct DevTables {
     int bx, by, tc, ndm, nbm;
     int count_selected[4];
     // k_dm_bm launches one group of decimation modes at a time, grouped by
-    // weight count (<= 16, <= 32, more), each with LDS rows sized for its
-    // widest grid: dm_order lists the modes group by group
+    // weight count (<= 16, <= 24, <= 32, more), each with LDS rows sized for
+    // its widest grid: dm_order lists the modes group by group
     const uint8_t* dm_order;      // [ndm]
-    int dm_group_start[4];        // group g = dm_order[dm_group_start[g] .. dm_group_start[g + 1])
-    int dm_group_w[3];            // LDS row (weights, multiple of 4) of group g
+    int dm_group_start[kDmGroups + 1];  // group g = dm_order[dm_group_start[g] .. dm_group_start[g + 1])
+    int dm_group_w[kDmGroups];          // LDS row (weights, multiple of 4) of group g
 };
 
 // One texture of a launch: its blocks occupy [first_block, first_block +

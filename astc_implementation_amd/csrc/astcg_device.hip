@@ -3030,7 +3030,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                      B.ct_fmt);
         {
             if (prof) prof->begin(K_DM_BM, s);
-            for (int g = 0; g < 3; ++g) {
+            for (int g = 0; g < kDmGroups; ++g) {
                 const int first = T.dm_group_start[g], count = T.dm_group_start[g + 1] - first;
                 if (count <= 0) continue;
                 const int w = T.dm_group_w[g];
