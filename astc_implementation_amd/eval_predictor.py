@@ -22,15 +22,11 @@ from astc_implementation_amd import synth  # noqa: E402
 from astc_implementation_amd.train_predictor import load_params  # noqa: E402
 
 SETTINGS = [
-    dict(seeds=0, modes=53, slots=8, fallback_below=0.0),
-    dict(seeds=0, modes=16, slots=8, fallback_below=0.0),
-    dict(seeds=0, modes=12, slots=8, fallback_below=0.0),
-    dict(seeds=0, modes=8, slots=8, fallback_below=0.0),
-    dict(seeds=0, modes=16, slots=6, fallback_below=0.0),
-    dict(seeds=0, modes=16, slots=5, fallback_below=0.0),
-    dict(seeds=0, modes=16, slots=4, fallback_below=0.9),
-    dict(seeds=2, modes=16, slots=5, fallback_below=0.0),
-    dict(seeds=0, modes=16, slots=5, fallback_below=0.0, count_below=0.1),
+    dict(seeds=0, modes=16, slots=5, fallback_below=0.5),
+    dict(seeds=0, modes=16, slots=8, fallback_below=0.5, early_out=1.0),
+    dict(seeds=0, modes=16, slots=8, fallback_below=0.5, early_out=1.2, full_search_above=8.0),
+    dict(seeds=0, modes=16, slots=8, fallback_below=0.5, early_out=1.25, full_search_above=8.0),
+    dict(seeds=0, modes=12, slots=6, fallback_below=0.5, early_out=1.3, full_search_above=10.0),
 ]
 
 

@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--block", default="4x4")
     ap.add_argument("--blocks", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--rank-loss", type=int, default=1,
+    ap.add_argument("--rank-loss", type=int, default=0,
                     help="also train the seed heads to rank the 4 compat positions of each partition count")
     ap.add_argument("--out", default=None)
     # pruning saved with the weights (tuned with eval_predictor)
