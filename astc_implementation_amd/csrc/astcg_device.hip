@@ -1755,6 +1755,31 @@ __device__ __forceinline__ uint32_t pk_byte(uint2 v, int i) { return ((i < 4 ? v
 
 __device__ __forceinline__ int4 load_ue(const uchar4 u) { return make_int4(u.x * 257, u.y * 257, u.z * 257, u.w * 257); }
 
+// pass13's sums over one partition's texels (weights, their complements,
+// colour and scalar projections), taken in texel order.
+struct Pass13Sums {
+    float wmin1 = 1.0f, wmax1 = 0.0f, ls = 0.0f, ms = 0.0f, rs = 0.0f, smin = 1e10f, smax = 0.0f;
+    float cx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, cy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float sv0 = 0.0f, sv1 = 0.0f;
+    __device__ __forceinline__ void add(float w, const float* rgba, float sc, float lsw) {
+        if (w < wmin1) wmin1 = w;
+        if (w > wmax1) wmax1 = w;
+        if (sc < smin) smin = sc;
+        if (sc > smax) smax = sc;
+        const float om = 1.0f - w;
+        ls += om * om;
+        ms += om * w;
+        rs += w * w;
+        for (int j = 0; j < 4; ++j) {
+            const float cyk = rgba[j] * w;
+            cy[j] += cyk;
+            cx[j] += rgba[j] - cyk;
+        }
+        sv0 += om * sc * lsw;
+        sv1 += w * sc * lsw;
+    }
+};
+
 // Specialised per partition count, so per-partition state only occupies
 // registers for the partitions that exist.
 template <bool kStage, int PC>
@@ -1847,6 +1872,23 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
     };
     for (int a = 0; a < iterations; ++a) {  // 6 in the reference (pass13-17 rounds)
         // pass13_recompute_ideal_endpoints.wgsl:182-393
+        // Per-partition sums over the partition's texels in texel order.  With
+        // two partitions one walk over the texels fills both sets (each set
+        // still receives its texels in index order), instead of one divergent
+        // walk per partition.
+        Pass13Sums sums2[2];
+        if (pc == 2) {
+            for (int i = 0; i < tc; ++i) {
+                const int p = btp[i * 64];
+                const float w = weight_of(i);
+                const float4 px4 = px_of(i);
+                const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
+                const float* sd = p ? sdirs[1] : sdirs[0];
+                const float sc = dot3(rgba, sd);
+                if (p) sums2[1].add(w, rgba, sc, lsw);
+                else sums2[0].add(w, rgba, sc, lsw);
+            }
+        }
         float4 rgbs[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -1854,30 +1896,22 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
             if (p >= pc) continue;
             const float size = (float)ppc[p];
             const float* sdir = sdirs[p];
-            float wmin1 = 1.0f, wmax1 = 0.0f, ls = 0.0f, ms = 0.0f, rs = 0.0f, smin = 1e10f, smax = 0.0f;
-            float cx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, cy[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sv0 = 0.0f, sv1 = 0.0f;
-            for (int i = 0; i < tc; ++i) {
-                if (pc > 1 && btp[i * 64] != p) continue;
-                const float w = weight_of(i);
-                const float4 px4 = px_of(i);
-                const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
-                if (w < wmin1) wmin1 = w;
-                if (w > wmax1) wmax1 = w;
-                const float sc = dot3(rgba, sdir);
-                if (sc < smin) smin = sc;
-                if (sc > smax) smax = sc;
-                const float om = 1.0f - w;
-                ls += om * om;
-                ms += om * w;
-                rs += w * w;
-                for (int j = 0; j < 4; ++j) {
-                    const float cyk = rgba[j] * w;
-                    cy[j] += cyk;
-                    cx[j] += rgba[j] - cyk;
+            Pass13Sums acc;
+            if (pc == 2) {
+                acc = sums2[p & 1];
+            } else {
+                for (int i = 0; i < tc; ++i) {
+                    if (pc > 1 && btp[i * 64] != p) continue;
+                    const float w = weight_of(i);
+                    const float4 px4 = px_of(i);
+                    const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
+                    acc.add(w, rgba, dot3(rgba, sdir), lsw);
                 }
-                sv0 += om * sc * lsw;
-                sv1 += w * sc * lsw;
             }
+            const float wmin1 = acc.wmin1, wmax1 = acc.wmax1, ls = acc.ls, ms = acc.ms, rs = acc.rs;
+            const float smin = acc.smin, smax = acc.smax, sv0 = acc.sv0, sv1 = acc.sv1;
+            const float* cx = acc.cx;
+            const float* cy = acc.cy;
             float colx[4], coly[4], lv[4], mv[4], rv[4];
             for (int j = 0; j < 4; ++j) {
                 colx[j] = cx[j] * cw[j]; coly[j] = cy[j] * cw[j];
