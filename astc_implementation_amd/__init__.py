@@ -92,7 +92,8 @@ class LearnedParams(ctypes.Structure):
         ("early_out", ctypes.c_float),
         ("full_search_above", ctypes.c_float),
         ("rounds", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("partitioned_slots", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -293,7 +294,7 @@ class Encoder:
         for footprint bx x by; keyword arguments override the pruning
         defaults (seeds, modes, slots, fallback_below, count_below,
         partitionings, refine_partitionings, early_out, full_search_above,
-        rounds)."""
+        rounds, partitioned_slots)."""
         flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
         lp = LearnedParams()
         library().astcg_default_learned_params(ctypes.byref(lp))
@@ -367,6 +368,8 @@ def load_predictor_file(path):
         prune.update(full_search_above=float(z["prune_h"]))
     if "prune_n" in z.files:
         prune.update(rounds=int(z["prune_n"]))
+    if "prune_s" in z.files:
+        prune.update(partitioned_slots=int(z["prune_s"]))
     return params, int(z["hidden"]), prune
 
 

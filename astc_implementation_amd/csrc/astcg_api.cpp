@@ -588,6 +588,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     }
     pl.best = e->d_best;
     pl.iterations = lp.rounds ? (int)lp.rounds : 6;
+    pl.part_slots = lp.partitioned_slots ? (int)lp.partitioned_slots : (int)lp.slots;
     pl.pb_est = prune_refine ? reinterpret_cast<float*>(pb + o_est) : nullptr;
     return ASTCG_OK;
 }
@@ -1099,7 +1100,8 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     astcg_default_learned_params(&q);
     if (lp) q = *lp;
     if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
-        q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f) || q.rounds > 6)
+        q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f) || q.rounds > 6 ||
+        q.partitioned_slots > 8)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");

@@ -124,6 +124,7 @@ struct SearchPlan {
     // the full search for 2..4 partitions (0 = off)
     float full_limit;
     int iterations;             // refinement rounds per candidate (1..6)
+    int part_slots;             // slots of non-fallback blocks for 2..4 partitions (1..8)
     const SymBest* best;        // running best per block (all blocks of the launch chain)
 };
 

@@ -2652,6 +2652,8 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
     if (k < A.nb) {
         const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
         const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
+        if (pc == 2 && !P.fallback[k] && P.part_slots != P.nslots)
+            const_cast<uint8_t*>(P.slots)[k] = (uint8_t)P.part_slots;  // slots for 2..4 partitions
         if (P.error_limit > 0.0f && P.best[A.blk0 + k].error < P.error_limit) {
             n = 0;  // early out: the block's best encoding so far is good enough
         } else if (pc == 2 && P.full_limit > 0.0f && !P.fallback[k] && P.best[A.blk0 + k].error > P.full_limit) {

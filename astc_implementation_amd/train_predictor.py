@@ -149,6 +149,7 @@ def main():
     ap.add_argument("--early-out", type=float, default=0.0)
     ap.add_argument("--full-search-above", type=float, default=0.0)
     ap.add_argument("--rounds", type=int, default=0)
+    ap.add_argument("--partitioned-slots", type=int, default=0)
     a = ap.parse_args()
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
@@ -174,7 +175,8 @@ def main():
     save(out, P, dict(seeds=a.seeds, modes=a.modes, slots=a.slots, fallback_below=a.fallback_below,
                       count_below=a.count_below, partitionings=a.partitionings,
                       refine_partitionings=a.refine_partitionings, early_out=a.early_out,
-                      full_search_above=a.full_search_above, rounds=a.rounds))
+                      full_search_above=a.full_search_above, rounds=a.rounds,
+                      partitioned_slots=a.partitioned_slots))
     enc.close()
 
 
@@ -190,7 +192,8 @@ def save(path, P, prune):
                         prune_r=np.int32(prune.get("refine_partitionings", 0)),
                         prune_e=np.float32(prune.get("early_out", 0.0)),
                         prune_h=np.float32(prune.get("full_search_above", 0.0)),
-                        prune_n=np.int32(prune.get("rounds", 0)))
+                        prune_n=np.int32(prune.get("rounds", 0)),
+                        prune_s=np.int32(prune.get("partitioned_slots", 0)))
 
 
 def load_params(path):

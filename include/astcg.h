@@ -80,7 +80,9 @@ typedef struct {
                                 of the error limit searches 2..4 partitions with every trial mode and 8
                                 slots (the COMPAT search); 0 = off */
     uint32_t rounds;        /* refinement rounds per candidate (pass13-17), 1..6, 0 = 6 as the reference */
-    uint32_t reserved[2];
+    uint32_t partitioned_slots; /* candidates refined per partitioned block for 2..4 partitions
+                                   (1..8, 0 = slots); fallback blocks keep 8 */
+    uint32_t reserved;
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);

@@ -240,3 +240,17 @@ def test_rounds_six_is_the_default_and_fewer_rounds_stay_valid(setup):
     dec = enc.decode(4, 4, 512, 512, outs[2])
     assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
     assert p_plain - A.psnr(dec, img) < 0.5
+
+
+def test_partitioned_slots_default_and_valid(setup):
+    enc, img, params, hidden, p_plain = setup
+    outs = {}
+    for ps in (0, 5, 2):
+        enc.set_predictor(4, 4, params, hidden=hidden, fallback_below=0.0, slots=5, partitioned_slots=ps)
+        enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+        outs[ps] = enc.encode(img)
+    assert np.array_equal(outs[0], outs[5])
+    assert not np.array_equal(outs[2], outs[5])
+    dec = enc.decode(4, 4, 512, 512, outs[2])
+    assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
+    assert p_plain - A.psnr(dec, img) < 0.5
