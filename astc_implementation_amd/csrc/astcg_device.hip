@@ -531,8 +531,41 @@ size_t partition_lds_bytes(const DevTables& T) {
            kPartBlocks * (((size_t)T.tc + 3) & ~size_t(3));
 }
 
-__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part,
-                                                  const SymBest* best, float error_limit) {
+// The learned policy's per-block decisions for partition count pc, shared by
+// k_partition (which skips the blocks whose positions are not searched) and
+// k_plan (which builds the lists); k is the block within the chunk, g its
+// index in the launch chain.
+__device__ __forceinline__ bool plan_early_out(const SearchPlan& P, int g) {
+    // the block's best encoding so far is below the error limit
+    return P.error_limit > 0.0f && P.best[g].error < P.error_limit;
+}
+
+__device__ __forceinline__ bool plan_hard(const SearchPlan& P, int pc, int k, int g) {
+    // a block whose best 1-partition error is above full_limit gets the full
+    // search from 2 partitions on
+    return pc == 2 && P.full_limit > 0.0f && !P.fallback[k] && P.best[g].error > P.full_limit;
+}
+
+__device__ bool plan_keeps_count(const SearchPlan& P, int pc, int k) {
+    // pc is the count head's most likely count, or at least count_below probable
+    const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
+    const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
+    float pp = 0.0f;
+    bool first = false;
+    for (int j = 0; j < P.kmax && j < 4; ++j)
+        if (ti[j] == pc - 1) { pp = tp[j]; first = j == 0; }
+    return first || pp >= P.count_below;
+}
+
+// Whether k_plan searches block k at count pc from the compat positions.
+__device__ bool plan_uses_positions(const SearchPlan& P, int pc, int k, int g) {
+    if (plan_early_out(P, g)) return false;
+    if (P.fallback[k] || plan_hard(P, pc, k, g)) return true;
+    return P.seeds == 0 && plan_keeps_count(P, pc, k);
+}
+
+__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part, SearchPlan P,
+                                                  int use_plan) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int tc = T.tc, lane = threadIdx.x, grp = lane / kPartLanes, sub = lane % kPartLanes;
     const int stride = tc + 1;  // float4 units; the pad spreads the 4 blocks over LDS banks
@@ -549,9 +582,9 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int 
     // partition_ordering and the partitioning errors are allocated 920*128
     // entries but indexed with stride 512, so in-batch blocks >= 230 read
     // zeros and keep table positions 0..3.
-    // (learned policy, early out: a block already below the error limit is
-    // not searched at this count, so its candidates are not needed)
-    const bool work = k < A.nb && j * 512 + 127 < kRefBatch * 128 && !(error_limit > 0.0f && best[g].error < error_limit);
+    // (learned policy: a block that is not searched at this count from the
+    // compat positions needs no candidates)
+    const bool work = k < A.nb && j * 512 + 127 < kRefBatch * 128 && (!use_plan || plan_uses_positions(P, pc, k, g));
     // pass004 writes slot block*1024*i: the score this block reads is
     // mismatch(j, partitioning 1), or mismatch(B-1, partitioning 0) for j == 0
     const int src = j == 0 ? g + B - 1 : g;
@@ -2651,12 +2684,12 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
     int n = 0;
     if (k < A.nb) {
         const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
-        const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
+        const int g = A.blk0 + k;
         if (pc == 2 && !P.fallback[k] && P.part_slots != P.nslots)
             const_cast<uint8_t*>(P.slots)[k] = (uint8_t)P.part_slots;  // slots for 2..4 partitions
-        if (P.error_limit > 0.0f && P.best[A.blk0 + k].error < P.error_limit) {
+        if (plan_early_out(P, g)) {
             n = 0;  // early out: the block's best encoding so far is good enough
-        } else if (pc == 2 && P.full_limit > 0.0f && !P.fallback[k] && P.best[A.blk0 + k].error > P.full_limit) {
+        } else if (plan_hard(P, pc, k, g)) {
             // a hard block: the full search from 2 partitions on (all trial
             // modes, 8 slots; later counts see the fallback flag)
             P.fallback[k] = 1;
@@ -2670,15 +2703,11 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
             n = 4;
         } else if (P.fallback[k]) {
             n = 4;
-        } else {
-            float pp = 0.0f;
-            bool first = false;
-            for (int j = 0; j < P.kmax && j < 4; ++j)
-                if (ti[j] == pc - 1) { pp = tp[j]; first = j == 0; }
-            if ((first || pp >= P.count_below) && P.seeds == 0) {
+        } else if (plan_keeps_count(P, pc, k)) {
+            if (P.seeds == 0) {
                 // the compat positions k_partition wrote
                 n = P.keep_pb < 4 ? rank_positions(T, P, pc, k, pb_part) : 4;
-            } else if (first || pp >= P.count_below) {
+            } else {
                 for (int j = 0; j < P.kmax && j < P.seeds && n < 4; ++j) {
                     const int seed = ti[(pc - 1) * P.kmax + j];
                     if (seed < 0 || seed >= kPartitionings) break;
@@ -3051,8 +3080,8 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         if (pc > 1) {
             if (prof) prof->begin(K_PARTITION, s);
             hipLaunchKernelGGL(k_partition, dim3((A.nb + kPartBlocks - 1) / kPartBlocks), dim3(64),
-                               partition_lds_bytes(T), s, A, T, pc, B.pb_part, plan ? plan->best : nullptr,
-                               plan ? plan->error_limit : 0.0f);
+                               partition_lds_bytes(T), s, A, T, pc, B.pb_part, plan ? *plan : SearchPlan{},
+                               plan ? 1 : 0);
             if (plan) {
                 hipLaunchKernelGGL(k_plan, dim3(blocks_for(A.nb)), dim3(64), 0, s, A, T, pc, B.pb_part, *plan,
                                    plan->pb_count + (pc - 2));
