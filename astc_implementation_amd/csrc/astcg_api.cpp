@@ -413,15 +413,22 @@ astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpS
     return ASTCG_OK;
 }
 
-// Blocks per chunk: keeps the per-chunk working set near 8 GiB (a 4096^2
-// image at 4x4 takes two chunks).
+// Blocks per chunk: keeps the per-chunk working set near 16 GiB, so a 4096^2
+// image takes one chunk at every footprint (14.0 GiB at 4x4, less at larger
+// footprints).  Every partition count of a chunk ends with a few long
+// refinement lanes that leave most of the GPU idle, so fewer, larger chunks
+// pay that tail fewer times.  A requested size is capped so that the element
+// indices of the per-PB arrays ([colour-table row or trial mode][PB]) stay
+// below 2^31.
 int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested) {
     size_t offs[12];
     const size_t fixed = chunk_bytes_per_block(f, offs, 0);
     const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) - fixed) >> 16;
-    int chunk = (int)std::max<size_t>(1024, ((size_t(8) << 30) - fixed) / per_block);
-    if (requested) chunk = (int)requested;
-    return std::max(1, std::min<int>(chunk, (int)nblocks));
+    const size_t max_chunk = (((size_t)1 << 31) - 1) / (4 * (size_t)std::max(kColorTabRows, f.nbm) + 8 * 4);
+    size_t chunk = std::max<size_t>(1024, ((size_t(16) << 30) - fixed) / per_block);
+    if (requested) chunk = requested;
+    chunk = std::min<size_t>(chunk, max_chunk);
+    return (int)std::max<size_t>(1, std::min<size_t>(chunk, nblocks));
 }
 
 // Lays the chunk buffers out for footprint F and `chunk` blocks, growing the
