@@ -1501,22 +1501,94 @@ __device__ bool try_rgb_bc(const float* color0, const float* color1, int* out0, 
     for (int i = 0; i < 4; ++i) { out0[i] = c1i[i]; out1[i] = c0i[i]; }
     return true;
 }
-// quantize_rgb, pass14:715-746
-__device__ void quantize_rgb(const float* color0, const float* color1, int* out0, int* out1, int q) {
-    float c0[4], c1[4];
-    for (int i = 0; i < 4; ++i) { c0[i] = color0[i]; c1[i] = color1[i]; }
-    int c0i[4], c1i[4];
-    for (;;) {
-        int c0q[4], c1q[4];
-        for (int i = 0; i < 4; ++i) c0q[i] = max(wf2i(wround(c0[i])), 0);
-        quant3_flt(q, c0q, c0, c0i);
-        for (int i = 0; i < 4; ++i) c0[i] = c0[i] - 0.2f;
-        for (int i = 0; i < 4; ++i) c1q[i] = min(wf2i(wround(c1[i])), 255);
-        quant3_flt(q, c1q, c1, c1i);
-        for (int i = 0; i < 4; ++i) c1[i] = c1[i] + 0.2f;
-        if ((c0i[0] + c0i[1] + c0i[2]) <= (c1i[0] + c1i[1] + c1i[2])) break;
+// quantize_rgb, pass14:715-746.  The reference steps the endpoints apart,
+// c0 down and c1 up by 0.2 per step (float subtractions / additions), until
+// the quantised colour sums are ordered, and returns the quantised endpoints
+// of that step.  For inverted endpoints that takes hundreds of steps, each
+// with six table lookups.  Both sums are monotone in the step count (every
+// float step moves a component strictly, the rounded index 2*round(c) +
+// [c - round(c) >= -0.1] follows it, and the table is non-decreasing in its
+// index), so the first ordered step is found by doubling the step count and
+// then bisecting; the endpoints at a step are recomputed by repeating the
+// same float operations from the last unordered step, so the values, and the
+// result, are those of the sequential loop.
+__device__ __forceinline__ void rgb_steps(float* c0, float* c1, int n) {
+    for (int s = 0; s < n; ++s)
+        for (int i = 0; i < 3; ++i) {
+            c0[i] = c0[i] - 0.2f;
+            c1[i] = c1[i] + 0.2f;
+        }
+}
+__device__ bool rgb_ordered(const float* c0, const float* c1, int q, int* c0i, int* c1i) {
+    int c0q[4], c1q[4];
+    const float f0[4] = {c0[0], c0[1], c0[2], 0.0f}, f1[4] = {c1[0], c1[1], c1[2], 0.0f};
+    for (int i = 0; i < 3; ++i) {
+        c0q[i] = max(wf2i(wround(c0[i])), 0);
+        c1q[i] = min(wf2i(wround(c1[i])), 255);
     }
-    for (int i = 0; i < 4; ++i) { out0[i] = c0i[i]; out1[i] = c1i[i]; }
+    c0q[3] = c1q[3] = 0;
+    quant3_flt(q, c0q, f0, c0i);
+    quant3_flt(q, c1q, f1, c1i);
+    return (c0i[0] + c0i[1] + c0i[2]) <= (c1i[0] + c1i[1] + c1i[2]);
+}
+// kGallop = false keeps the plain step loop: the search costs registers
+// (one wave per SIMD less for the 1-partition refinement, where the long
+// loops are rare), and the 2..4-partition refinement, where they make the
+// launch tail, is at its register budget either way.
+template <bool kGallop>
+__device__ void quantize_rgb(const float* color0, const float* color1, int* out0, int* out1, int q) {
+    // the first steps one by one, as the reference takes them (most endpoints
+    // are ordered at once or after a few steps)
+    if (!kGallop) {
+        float c0[4], c1[4];
+        for (int i = 0; i < 4; ++i) { c0[i] = color0[i]; c1[i] = color1[i]; }
+        int c0i[4], c1i[4];
+        for (;;) {
+            int c0q[4], c1q[4];
+            for (int i = 0; i < 4; ++i) c0q[i] = max(wf2i(wround(c0[i])), 0);
+            quant3_flt(q, c0q, c0, c0i);
+            for (int i = 0; i < 4; ++i) c0[i] = c0[i] - 0.2f;
+            for (int i = 0; i < 4; ++i) c1q[i] = min(wf2i(wround(c1[i])), 255);
+            quant3_flt(q, c1q, c1, c1i);
+            for (int i = 0; i < 4; ++i) c1[i] = c1[i] + 0.2f;
+            if ((c0i[0] + c0i[1] + c0i[2]) <= (c1i[0] + c1i[1] + c1i[2])) break;
+        }
+        for (int i = 0; i < 4; ++i) { out0[i] = c0i[i]; out1[i] = c1i[i]; }
+        return;
+    }
+    float l0[3] = {color0[0], color0[1], color0[2]}, l1[3] = {color1[0], color1[1], color1[2]};
+    for (int s = 0;; ++s) {
+        if (rgb_ordered(l0, l1, q, out0, out1)) return;
+        if (s == 7) break;
+        rgb_steps(l0, l1, 1);
+    }
+    // gallop: the state l (step lo) is unordered; probe lo + span, doubling span
+    float m0[3], m1[3];
+    int span = 1;
+    for (;;) {
+        for (int i = 0; i < 3; ++i) { m0[i] = l0[i]; m1[i] = l1[i]; }
+        rgb_steps(m0, m1, span);
+        if (rgb_ordered(m0, m1, q, out0, out1)) break;
+        for (int i = 0; i < 3; ++i) { l0[i] = m0[i]; l1[i] = m1[i]; }
+        if (span >= (1 << 14)) break;  // unreachable for finite endpoints (ordered within 1300 steps)
+        span *= 2;
+    }
+    // bisect: lo unordered, lo + span ordered; then the quantised endpoints
+    // of the first ordered step (lo + 1)
+    int lo = 0, hi = span;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        for (int i = 0; i < 3; ++i) { m0[i] = l0[i]; m1[i] = l1[i]; }
+        rgb_steps(m0, m1, mid - lo);
+        if (rgb_ordered(m0, m1, q, out0, out1)) {
+            hi = mid;
+        } else {
+            lo = mid;
+            for (int i = 0; i < 3; ++i) { l0[i] = m0[i]; l1[i] = m1[i]; }
+        }
+    }
+    rgb_steps(l0, l1, 1);
+    (void)rgb_ordered(l0, l1, q, out0, out1);
 }
 // try_quantize_alpha_delta, pass14:748-797
 __device__ bool try_alpha_delta(const float* color0, const float* color1, int* out0, int* out1, int q) {
@@ -1581,6 +1653,7 @@ __device__ bool try_la_delta(const float* c0, const float* c1, uint32_t* out, in
 }
 
 // pack_color_endpoints_helper, pass14:1018-1221
+template <bool kGallop>
 __device__ int pack_color_endpoints(const float* color0, const float* color1, const float* rgbs, int format, int q,
                                     uint32_t* packed) {
     float c0[4], c1[4];
@@ -1625,7 +1698,7 @@ __device__ int pack_color_endpoints(const float* color0, const float* color1, co
                 rgba_unpack(t0, t1, u0, u1);
                 take(enc_error(c0, c1, u0, u1), alpha ? FMT_RGBA : FMT_RGB);
             }
-            quantize_rgb(c0, c1, t0, t1, q);
+            quantize_rgb<kGallop>(c0, c1, t0, t1, q);
             if (alpha) {
                 t0[3] = quant1_flt(q, wf2i(wround(c0[3])), c0[3]);
                 t1[3] = quant1_flt(q, wf2i(wround(c1[3])), c1[3]);
@@ -1773,12 +1846,13 @@ struct PackOut {
     uint2 v;  // eight 0..255 values, little-endian bytes
 };
 
+template <bool kGallop>
 __device__ __noinline__ PackOut pack_partition(float4 e0, float4 e1, float4 rgbs, int format, int q) {
     const float a[4] = {e0.x, e0.y, e0.z, e0.w}, b[4] = {e1.x, e1.y, e1.z, e1.w};
     const float r[4] = {rgbs.x, rgbs.y, rgbs.z, rgbs.w};
     uint32_t v[8];
     PackOut o;
-    o.format = pack_color_endpoints(a, b, r, format, q, v);
+    o.format = pack_color_endpoints<kGallop>(a, b, r, format, q, v);
     o.v.x = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
     o.v.y = v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24);
     return o;
@@ -1988,7 +2062,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             if (p >= pc) continue;
-            const PackOut o = pack_partition(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), ql);
+            const PackOut o = pack_partition<(PC > 1)>(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), ql);
             fn[p] = o.format;
             pk[p] = o.v;
             if (p > 0) all_same = all_same && (o.format == fn[0]);
@@ -2001,7 +2075,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
                 if (p >= pc || !same_mod) continue;
-                const PackOut o = pack_partition(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), qlm);
+                const PackOut o = pack_partition<(PC > 1)>(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), qlm);
                 fm[p] = o.format;
                 pm[p] = o.v;
                 if (p > 0 && o.format != fm[0]) same_mod = false;
