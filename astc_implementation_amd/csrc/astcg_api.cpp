@@ -500,11 +500,17 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     hipError_t err = grow(&e->d_plan, &e->plan_cap, L.size);
     if (err != hipSuccess) return hip_fail(e, err, "plan buffers");
     uint8_t* pb = static_cast<uint8_t*>(e->d_plan);
-    // inference workspace for one sub-chunk
-    const int H = P.hidden, NO = P.nout, sub = std::min(nb, kInferBlocks);
+    // inference workspace for one sub-chunk.  The fused kernel needs only the
+    // features (2 * nin bytes per block) and runs over the whole chunk in one
+    // launch, so its workgroups overlap their serial layer-by-layer latency;
+    // the separate GEMM / GELU / top-k launches keep fp32 intermediates of the
+    // 3-head output layer (about 16 KB per block) and run in sub-chunks.
+    const bool fused = P.fw[0] && lp.seeds == 0;
+    const int H = P.hidden, NO = P.nout, sub = fused ? nb : std::min(nb, kInferBlocks);
+    const int usub = fused ? 0 : sub;  // rows of the unfused intermediates
     Lay W;
-    const size_t o_x = W.add(2 * (size_t)sub * P.nin), o_z = W.add(4 * (size_t)sub * H), o_h = W.add(2 * (size_t)sub * H);
-    const size_t o_z2 = W.add(4 * (size_t)sub * H), o_h2 = W.add(2 * (size_t)sub * H), o_l = W.add(4 * (size_t)sub * NO);
+    const size_t o_x = W.add(2 * (size_t)sub * P.nin), o_z = W.add(4 * (size_t)usub * H), o_h = W.add(2 * (size_t)usub * H);
+    const size_t o_z2 = W.add(4 * (size_t)usub * H), o_h2 = W.add(2 * (size_t)usub * H), o_l = W.add(4 * (size_t)usub * NO);
     err = grow(&e->d_pwork, &e->pwork_cap, W.size);
     if (err != hipSuccess) return hip_fail(e, err, "predictor workspace");
     if (reserve_only) return ASTCG_OK;  // buffers sized, nothing enqueued
@@ -529,7 +535,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
                                                  static_cast<uint8_t*>(X) + 2 * (size_t)(lo - g0) * P.nin, P.nin, s);
             if (st != ASTCG_OK) return fail(e, st, "predictor features");
         }
-        if (P.fw[0] && lp.seeds == 0) {
+        if (fused) {
             // one fused kernel: hidden layers, count and mode heads, top-k
             astcg_status st = astcg_mlp_infer(n, X, P.nin, P.nin, P.fw[0], P.bias[0], P.fw[1], P.bias[1], P.fw[2], P.fb3,
                                               4 + P.nmodes, P.nmodes, kmax, 4, (int)std::min<uint32_t>(lp.modes, P.nmodes),
