@@ -2667,34 +2667,63 @@ __global__ void __launch_bounds__(64) k_fb_reset(unsigned long long* fb_count) {
     if (threadIdx.x == 0) *fb_count = 0ull;
 }
 
+// One lane per block for the decisions; the blocks' mode-head entries are
+// first staged through LDS 16 at a time with contiguous loads (a block's
+// top-k record is 5 * kmax entries, so lane-per-block loads would touch one
+// cache line per lane).  The kept probability is still summed per block in
+// rank order.
 __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, int nbm, uint32_t* mask, uint8_t* slots) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.nb) return;
-    const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
-    const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
+    __shared__ int s_idx[64 * 17];    // [block][16 entries], rows padded to 17 words
+    __shared__ float s_prob[64 * 17];
+    const int base = blockIdx.x * 64, lane = threadIdx.x, k = base + lane;
+    const bool live = k < A.nb;
+    const int nblk = min(64, A.nb - base);
     // confidence: the probability mass of the trial modes the block keeps
     // (the top-k kernel fills min(modes, nbm) entries of the mode head)
     const int nkeep = min(min(P.kmax, P.modes), nbm);
+    auto stage = [&](int j0, int nj, bool probs) {
+        __syncthreads();
+        for (int e = lane; e < 64 * 16; e += 64) {
+            const int b = e >> 4, j = e & 15;
+            if (b < nblk && j < nj) {
+                const size_t o = (size_t)(base + b) * 5 * P.kmax + 4 * P.kmax + j0 + j;
+                s_idx[b * 17 + j] = P.topk_idx[o];
+                if (probs) s_prob[b * 17 + j] = P.topk_prob[o];
+            }
+        }
+        __syncthreads();
+    };
     float kept = 0.0f;
-    for (int j = 0; j < nkeep; ++j)
-        if (ti[4 * P.kmax + j] >= 0) kept += tp[4 * P.kmax + j];
-    const bool fb = kept < P.fallback_below;
-    P.fallback[k] = fb ? 1 : 0;
+    for (int j0 = 0; j0 < nkeep; j0 += 16) {
+        const int nj = min(16, nkeep - j0);
+        stage(j0, nj, true);
+        if (live)
+            for (int j = 0; j < nj; ++j)
+                if (s_idx[lane * 17 + j] >= 0) kept += s_prob[lane * 17 + j];
+    }
+    const bool fb = live && kept < P.fallback_below;
     // one atomic per wave for the fallback statistics
     const unsigned long long wave_fb = __ballot(fb);
-    if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && wave_fb)
-        atomicAdd(P.fb_count, (unsigned long long)__popcll(wave_fb));
+    if (lane == 0 && wave_fb) atomicAdd(P.fb_count, (unsigned long long)__popcll(wave_fb));
     uint32_t* m = mask + (size_t)k * P.mask_words;
-    for (int w = 0; w < P.mask_words; ++w) {
-        const int lo = 32 * w;
-        m[w] = !fb ? 0u : (nbm - lo >= 32 ? 0xFFFFFFFFu : (nbm > lo ? (1u << (nbm - lo)) - 1u : 0u));
-    }
-    if (!fb)
-        for (int j = 0; j < nkeep; ++j) {
-            const int md = ti[4 * P.kmax + j];
-            if (md >= 0 && md < nbm) m[md >> 5] |= 1u << (md & 31);
+    if (live) {
+        P.fallback[k] = fb ? 1 : 0;
+        for (int w = 0; w < P.mask_words; ++w) {
+            const int lo = 32 * w;
+            m[w] = !fb ? 0u : (nbm - lo >= 32 ? 0xFFFFFFFFu : (nbm > lo ? (1u << (nbm - lo)) - 1u : 0u));
         }
-    slots[k] = fb ? 8 : (uint8_t)P.nslots;
+        slots[k] = fb ? 8 : (uint8_t)P.nslots;
+    }
+    if (__ballot(live && !fb) == 0ull) return;  // every block of the wave falls back
+    for (int j0 = 0; j0 < nkeep; j0 += 16) {
+        const int nj = min(16, nkeep - j0);
+        stage(j0, nj, false);
+        if (live && !fb)
+            for (int j = 0; j < nj; ++j) {
+                const int md = s_idx[lane * 17 + j];
+                if (md >= 0 && md < nbm) m[md >> 5] |= 1u << (md & 31);
+            }
+    }
 }
 
 __device__ __forceinline__ float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
