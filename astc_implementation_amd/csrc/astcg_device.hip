@@ -2237,12 +2237,15 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
 __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, const int16_t* pb_part,
                                              const CandOut* cand, const float* cand_err, SymBest* best,
                                              LabelBufs lab) {
+    // one lane per block picks; the wave then copies its blocks' chosen
+    // 96-byte packed / weight runs together (consecutive lanes on consecutive
+    // dwords instead of one 96-byte stride per lane)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.nb) return;
+    const bool live = k < A.nb;
     const int npos = pc == 1 ? 1 : 4;
     float cur_err = 0.0f;
     int cur_pb = -1, cur_slot = -1;
-    for (int c = 0; c < npos; ++c) {
+    for (int c = 0; c < (live ? npos : 0); ++c) {
         const int pb = (pc > 1 && A.pb_inv) ? A.pb_inv[k * 4 + c] : k * npos + c;
         if (pb < 0) {  // position not searched (learned policy)
             if (lab.cand_seed) {
@@ -2265,34 +2268,46 @@ __global__ void __launch_bounds__(64) k_pick(ImageArgs A, DevTables T, int pc, c
         }
         if (cur_pb < 0 || be < cur_err) { cur_err = be; cur_pb = pb; cur_slot = bs; }
     }
-    SymBest& sb = best[A.blk0 + k];
-    // optional record of each partition count's best partitioning (labels
-    // for the candidate predictor)
-    if (lab.best_seed && pc > 1)
-        lab.best_seed[(size_t)(pc - 2) * A.num_blocks + A.blk0 + k] =
-            cur_slot >= 0 ? (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[pb_slot_of(A, pc, cur_pb)]]
-                          : (int16_t)-1;
-    if (pc == 1) {
-        sb.error = 1e30f;
-        sb.raw_mode = 0xFFFF;
-        sb.pc = 0;
-        sb.pad = 0;  // single plane
+    int src = -1;  // candidate whose packed / weight run this block takes
+    if (live) {
+        SymBest& sb = best[A.blk0 + k];
+        // optional record of each partition count's best partitioning
+        // (labels for the candidate predictor)
+        if (lab.best_seed && pc > 1)
+            lab.best_seed[(size_t)(pc - 2) * A.num_blocks + A.blk0 + k] =
+                cur_slot >= 0
+                    ? (int16_t)T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[pb_slot_of(A, pc, cur_pb)]]
+                    : (int16_t)-1;
+        if (pc == 1) {
+            sb.error = 1e30f;
+            sb.raw_mode = 0xFFFF;
+            sb.pc = 0;
+            sb.pad = 0;  // single plane
+        }
+        if (cur_slot >= 0 && cur_err < sb.error) {
+            src = cur_pb * 8 + cur_slot;
+            const CandOut& co = cand[src];
+            sb.error = cur_err;
+            sb.raw_mode = T.bm[co.bm].raw_mode;
+            sb.pc = (uint8_t)pc;
+            int kb, cb, slot;
+            pb_decode(A, pc, cur_pb, &kb, &cb, &slot);
+            sb.seed = pc == 1 ? 0 : T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[slot]];
+            sb.matched = co.matched;
+            sb.quant = co.quant;
+            sb.formats = co.formats;
+        }
     }
-    if (cur_slot >= 0 && cur_err < sb.error) {
-        const CandOut& co = cand[cur_pb * 8 + cur_slot];
-        sb.error = cur_err;
-        sb.raw_mode = T.bm[co.bm].raw_mode;
-        sb.pc = (uint8_t)pc;
-        int kb, cb, slot;
-        pb_decode(A, pc, cur_pb, &kb, &cb, &slot);
-        sb.seed = pc == 1 ? 0 : T.part_seed[(size_t)(pc - 2) * kPartitionings + pb_part[slot]];
-        sb.matched = co.matched;
-        sb.quant = co.quant;
-        sb.formats = co.formats;
-        // 96 bytes as 24 dwords (both records keep the arrays 4-byte aligned)
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(co.packed);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(sb.packed);
-        for (int i = 0; i < 24; ++i) dst[i] = src[i];
+    // 64 blocks x 96 bytes as 24 dwords each (both records keep the arrays
+    // 4-byte aligned); lane l of round i copies dword (64 i + l) % 24 of block
+    // (64 i + l) / 24 of the wave
+    const int wbase = blockIdx.x * blockDim.x;
+    for (int i = 0; i < 24; ++i) {
+        const int e = i * 64 + (int)threadIdx.x, b = e / 24, d = e - b * 24;
+        const int so = __shfl(src, b);
+        if (so >= 0)
+            reinterpret_cast<uint32_t*>(best[A.blk0 + wbase + b].packed)[d] =
+                reinterpret_cast<const uint32_t*>(cand[so].packed)[d];
     }
 }
 
