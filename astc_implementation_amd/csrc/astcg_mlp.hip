@@ -414,13 +414,28 @@ __global__ void __launch_bounds__(256) k_infer(int rows, const bf16* __restrict_
     se += __shfl_xor(se, 2);
     se += __shfl_xor(se, 4);
     if (!live) return;
-    for (int j = sub; j < nmodes; j += 8) {
-        const float v = m[j];
-        int rk = 0;
-        for (int i = 0; i < nmodes && rk < kmode; ++i) rk += (m[i] > v) || (m[i] == v && i < j);
-        if (rk < kmode) {
-            out_idx[o + 4 * kmax + rk] = j;
-            if (out_prob) out_prob[o + 4 * kmax + rk] = expf(v - mx) / se;
+    // selection, one rank per round: the row's 8 lanes each take the best of
+    // their remaining columns (value descending, lowest column first on ties),
+    // the group keeps the best of the eight, and its owner removes it
+    float* mw = sL + r * (n3p + 1) + 4;  // the row's mode logits (lane sub owns columns sub + 8c)
+    const int nsel = min(kmode, nmodes);
+    for (int rk = 0; rk < nsel; ++rk) {
+        float bv = -INFINITY;
+        int bj = 0x7FFFFFFF;
+        for (int j = sub; j < nmodes; j += 8) {
+            const float v = mw[j];
+            if (v > bv) { bv = v; bj = j; }
+        }
+#pragma unroll
+        for (int msk = 1; msk < 8; msk <<= 1) {
+            const float ov = __shfl_xor(bv, msk);
+            const int oj = __shfl_xor(bj, msk);
+            if (ov > bv || (ov == bv && oj < bj)) { bv = ov; bj = oj; }
+        }
+        if ((bj & 7) == sub) mw[bj] = -INFINITY;
+        if (sub == 0) {
+            out_idx[o + 4 * kmax + rk] = bj;
+            if (out_prob) out_prob[o + 4 * kmax + rk] = expf(bv - mx) / se;
         }
     }
     for (int j = nmodes + sub; j < kmode; j += 8) {
