@@ -4,9 +4,9 @@ fixed order as the oracle).
 
 The 2..4-partition refinement finds quantize_rgb's first ordered step by
 doubling and bisection where the oracle steps one by one
-(code pass14:715-746); the all-partition images here take that search
-1 700 to 27 000 times per image (counted with an instrumented build), so
-these tests pin it to the step loop."""
+(shaders/pass14_pack_color_endpoints.wgsl:715-746); the all-partition
+images here take that search 1 700 to 27 000 times per image (counted with
+an instrumented build), so these tests pin it to the step loop."""
 import functools
 
 import numpy as np
