@@ -154,12 +154,26 @@ __device__ __forceinline__ uint32_t texel_rgba8(const BlockLoc& l, const DevTabl
     return *reinterpret_cast<const uint32_t*>(l.rgba + ((size_t)y * l.width + x) * 4);
 }
 
+// (u8 / 255) * 65536 with IEEE float division semantics, without a divide or
+// a table: the product with the rounded reciprocal is corrected by one fused
+// residual step (Markstein), which is exact for all 256 inputs, and the
+// scaling by 2^16 is exact.
+__device__ __forceinline__ float u8_search_value(uint32_t v) {
+    const float x = (float)v;
+    const float r = 1.0f / 255.0f;
+    const float q = x * r;
+    const float e = __builtin_fmaf(-q, 255.0f, x);
+    return __builtin_fmaf(e, r * 65536.0f, q * 65536.0f);
+}
+
+// a texel's channels as search values (arithmetic, not a table: four fewer
+// dependent loads per texel)
 __device__ __forceinline__ void texel_px(const BlockLoc& l, const DevTables& T, int i, float* o) {
     const uint32_t v = texel_rgba8(l, T, i);
-    o[0] = c_qt.pixel_scale[v & 0xFF];
-    o[1] = c_qt.pixel_scale[(v >> 8) & 0xFF];
-    o[2] = c_qt.pixel_scale[(v >> 16) & 0xFF];
-    o[3] = c_qt.pixel_scale[v >> 24];
+    o[0] = u8_search_value(v & 0xFF);
+    o[1] = u8_search_value((v >> 8) & 0xFF);
+    o[2] = u8_search_value((v >> 16) & 0xFF);
+    o[3] = u8_search_value(v >> 24);
 }
 
 struct Pb {
@@ -1829,18 +1843,6 @@ size_t refine_lds_bytes(const DevTables& T, int wstride, int pc) {
     return bytes;
 }
 
-// (u8 / 255) * 65536 with IEEE float division semantics, without a divide or
-// a table: the product with the rounded reciprocal is corrected by one fused
-// residual step (Markstein), which is exact for all 256 inputs, and the
-// scaling by 2^16 is exact.
-__device__ __forceinline__ float u8_search_value(uint32_t v) {
-    const float x = (float)v;
-    const float r = 1.0f / 255.0f;
-    const float q = x * r;
-    const float e = __builtin_fmaf(-q, 255.0f, x);
-    return __builtin_fmaf(e, r * 65536.0f, q * 65536.0f);
-}
-
 struct PackOut {
     int format;
     uint2 v;  // eight 0..255 values, little-endian bytes
@@ -2640,7 +2642,7 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
                 for (int j = 0; j < 4; ++j) {
                     const int wv = wt[j == c ? 1 : 0];
                     const int col = (eq0[j] * 257 * (64 - wv) + eq1[j] * 257 * wv + 32) >> 6;
-                    const float df = wmin(fabsf(c_qt.pixel_scale[(v >> (8 * j)) & 0xFFu] - (float)col), 1e15f);
+                    const float df = wmin(fabsf(u8_search_value((v >> (8 * j)) & 0xFFu) - (float)col), 1e15f);
                     dd[j] = df * df;
                 }
                 total += wmin(dot4(dd, kCW), kErrDefault);
