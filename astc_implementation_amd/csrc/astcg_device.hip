@@ -1892,7 +1892,7 @@ struct Pass13Sums {
 // Specialised per partition count, so per-partition state only occupies
 // registers for the partitions that exist.
 template <bool kStage, int PC>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PC == 1 ? 6 : 4, 8)))
 k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                                const Ideal* ideal, const int16_t* top, const Ccr* ccr,
                                                const uint8_t* qw_all, const int* work, const int* work_total,
@@ -2089,6 +2089,7 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
             }
         }
         const int final_quant = matched ? qlm : ql;
+        int he0[4] = {0, 0, 0, 0}, he1[4] = {0, 0, 0, 0};  // 1 partition: decoded endpoints (x 257)
         // pass15_unpack_color_endpoints.wgsl:307-391
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -2097,9 +2098,36 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
             for (int i = 0; i < 8; ++i) v[i] = pk_byte(pk[p], i);
             int o0[4], o1[4];
             unpack_color_endpoints(fn[p], v, o0, o1);
-            s_ue0[p * 64 + lane] = make_uchar4(o0[0], o0[1], o0[2], o0[3]);
-            s_ue1[p * 64 + lane] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
+            if (PC == 1) {
+                for (int c = 0; c < 4; ++c) {
+                    he0[c] = (int)(uint8_t)o0[c] * 257;
+                    he1[c] = (int)(uint8_t)o1[c] * 257;
+                }
+            } else {
+                s_ue0[p * 64 + lane] = make_uchar4(o0[0], o0[1], o0[2], o0[3]);
+                s_ue1[p * 64 + lane] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
+            }
         }
+        // the endpoints of texel t's partition as the texel loops use them:
+        // decoded values (x 257) and the interpolation base and step per
+        // channel; with one partition they stay in registers
+        auto ep_at = [&](int t, int* e0, int* e1, float* bs, float* of) {
+            int4 u0, u1;
+            if (PC == 1) {
+                u0 = make_int4(he0[0], he0[1], he0[2], he0[3]);
+                u1 = make_int4(he1[0], he1[1], he1[2], he1[3]);
+            } else {
+                const int p = part_of(t);
+                u0 = load_ue(s_ue0[p * 64 + lane]);
+                u1 = load_ue(s_ue1[p * 64 + lane]);
+            }
+            e0[0] = u0.x; e0[1] = u0.y; e0[2] = u0.z; e0[3] = u0.w;
+            e1[0] = u1.x; e1[1] = u1.y; e1[2] = u1.z; e1[3] = u1.w;
+            for (int j = 0; j < 4; ++j) {
+                bs[j] = (float)e0[j];
+                of[j] = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+            }
+        };
         // pass17 (before pass16 on the first iteration), then pass16 + pass17
         // Once pass16 leaves every weight where it was, the next iteration
         // recomputes the same endpoints from the same weights (pass13 keeps a
@@ -2123,14 +2151,14 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                         const uint32_t pn = c_qt.prev_next[q][uqw];
                         const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
                         const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
-                        const int p = part_of(t);
-                        const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
-                        const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
+                        int e0[4], e1[4];
+                        float bs[4], of[4];
+                        ep_at(t, e0, e1, bs, of);
                         const float4 px4 = px_of(t);
                         const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                         float s0[4], s1[4], s2[4];
                         for (int j = 0; j < 4; ++j) {
-                            const float base = (float)e0[j], offs = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+                            const float base = bs[j], offs = of[j];
                             const float diff = (base + offs * wb) - px[j];
                             const float dfd = diff + offs * dd, dfu = diff + offs * udf;
                             s0[j] = diff * diff; s1[j] = dfd * dfd; s2[j] = dfu * dfu;
@@ -2168,13 +2196,13 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                 if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
                             }
                             const float wdd = ddn * twb, wud = dup * twb;
-                            const int p = part_of(t);
-                            const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
-                            const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
+                            int e0[4], e1[4];
+                            float bs[4], of[4];
+                            ep_at(t, e0, e1, bs, of);
                             const float4 px4 = px_of(t);
                             const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                             for (int j = 0; j < 4; ++j) {
-                                const float base = (float)e0[j], offs = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+                                const float base = bs[j], offs = of[j];
                                 const float diff = (base + offs * wbt) - px[j];
                                 const float dd = diff + offs * wdd, du = diff + offs * wud;
                                 ebv[j] += diff * diff;
@@ -2194,9 +2222,9 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
             // pass17_compute_final_error.wgsl:149-231
             for (int i = 0; i < tc && !fused; ++i) {
                 const float w = weight_of(i);
-                const int p = part_of(i);
-                const int4 u0 = load_ue(s_ue0[p * 64 + lane]), u1 = load_ue(s_ue1[p * 64 + lane]);
-                const int e0[4] = {u0.x, u0.y, u0.z, u0.w}, e1[4] = {u1.x, u1.y, u1.z, u1.w};
+                int e0[4], e1[4];
+                float bs[4], of[4];
+                ep_at(i, e0, e1, bs, of);
                 const float4 px4 = px_of(i);
                 const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                 const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
