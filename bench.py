@@ -44,7 +44,7 @@ def measured_traffic(workload, kernel):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3; 1 for the batch workloads)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 5; 1 for the batch workloads)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["image", "batch64", "batch256", "small"], default="image",
                     help="image: one 4096x4096 image (the headline); batch64: 64 textures of 128..2048 per side, "
@@ -279,7 +279,7 @@ def main():
     if a.workload != "image":
         bench_batch(a)
         return
-    a.steps = a.steps or 3
+    a.steps = a.steps or 5
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
     import astc_implementation_amd as A
