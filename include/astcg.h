@@ -110,7 +110,7 @@ void astcg_default_learned_params(astcg_learned_params* p);
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */
     uint32_t policy;          /* astcg_policy */
-    uint32_t chunk_blocks;    /* blocks per device chunk, 0 = automatic */
+    uint32_t chunk_blocks;    /* blocks per device chunk, 0 = automatic (a working set near 16 GiB) */
     uint32_t flags;           /* ASTCG_FLAG_* */
     uint32_t reserved[4];
 } astcg_params;
