@@ -1040,6 +1040,50 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, 
     // pass02_decimated_weights.wgsl:132-270
     if (tc == wc) {
         for (int i = 0; i < wc; ++i) dw[i * 64] = wts_of(i);
+    } else if (tc <= wstride) {
+        // The lane's rows hold a whole block: cache the ideal weights in dw,
+        // form the initial estimates in aux, then each texel's infill minus
+        // its ideal weight once (the refinement step reads the initial
+        // estimates only, so a texel's infill is the same for every weight
+        // it touches), refine each weight in place in aux and copy the
+        // results to dw.  Same float operations in the same order as the
+        // loop below, without recomputing infills per weight-texel pair.
+        for (int t = 0; t < tc; ++t) dw[t * 64] = wts_of(t);
+        for (int w = 0; w < wc; ++w) {
+            const int n = w2tc[w];
+            float v = 0.0f;
+            if (n != 0) {
+                const int off = w2to[w];
+                float ws = 0.0f, tcon = 1e-10f;
+                for (int jx = 0; jx < n; ++jx) {
+                    const int t = w2ti[off + jx];
+                    const float cw = w2tw[off + jx] * wes_of(t);
+                    ws += dw[t * 64] * cw;
+                    tcon += cw;
+                }
+                v = ws / tcon;
+            }
+            aux[w * 64] = v;  // initial estimate
+        }
+        for (int t = 0; t < tc; ++t) {
+            float infill = 0.0f;
+            const int nt = t2wc[t];
+            for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
+            dw[t * 64] = infill - dw[t * 64];
+        }
+        for (int w = 0; w < wc; ++w) {
+            float a = 1e-10f, bacc = 0.0f;
+            const int n = w2tc[w], off = w2to[w];
+            for (int jx = 0; jx < n; ++jx) {
+                const int t = w2ti[off + jx];
+                const float contrib = w2tw[off + jx];
+                const float scale = wes_of(t) * contrib;
+                a += contrib * scale;
+                bacc += dw[t * 64] * scale;
+            }
+            aux[w * 64] = aux[w * 64] + wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
+        }
+        for (int w = 0; w < wc; ++w) dw[w * 64] = aux[w * 64];
     } else {
         for (int w = 0; w < wc; ++w) {
             const int n = w2tc[w];
