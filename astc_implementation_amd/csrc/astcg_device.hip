@@ -289,7 +289,7 @@ __device__ __forceinline__ uint64_t grp_or_u64(uint64_t v) {
 
 // pass001-003: texel -> cluster after the reference's 4 k-means iterations;
 // lane `sub` of the group owns texels sub, sub + 16, ...
-__device__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, uint8_t* asg) {
+__device__ __forceinline__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, uint8_t* asg) {
     float centers[4][4] = {};
     ld4(kpx, 145897 % tc, centers[0]);
     float dist[kPartMaxR];
@@ -377,7 +377,7 @@ __device__ void kmeans_assign_grp(const float4* kpx, int tc, int pc, int sub, ui
 
 // pass004: minimum-over-permutations coverage mismatch for one partitioning;
 // lanes OR their share of the 64 k-means texels into the group's bitmaps
-__device__ uint32_t mismatch_count_grp(const DevTables& T, const uint8_t* asg, int pc, int entry, int sub) {
+__device__ __forceinline__ uint32_t mismatch_count_grp(const DevTables& T, const uint8_t* asg, int pc, int entry, int sub) {
     uint64_t a[4] = {0, 0, 0, 0};
     for (int l = sub; l < 64; l += kPartLanes) {
         const int p = asg[T.kmeans_texels[l]];
@@ -578,8 +578,13 @@ __device__ bool plan_uses_positions(const SearchPlan& P, int pc, int k, int g) {
     return P.seeds == 0 && plan_keeps_count(P, pc, k);
 }
 
-__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int pc, int16_t* pb_part, SearchPlan P,
+// Specialised per partition count (launched once per count), so the
+// per-partition selects of the k-means and line-fit loops only cover the
+// partitions that exist.
+template <int PC>
+__global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int16_t* pb_part, SearchPlan P,
                                                   int use_plan) {
+    constexpr int pc = PC;
     extern __shared__ __align__(16) uint8_t smem[];
     const int tc = T.tc, lane = threadIdx.x, grp = lane / kPartLanes, sub = lane % kPartLanes;
     const int stride = tc + 1;  // float4 units; the pad spreads the 4 blocks over LDS banks
@@ -3271,9 +3276,15 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         ImageArgs A = A0;
         if (pc > 1) {
             if (prof) prof->begin(K_PARTITION, s);
-            hipLaunchKernelGGL(k_partition, dim3((A.nb + kPartBlocks - 1) / kPartBlocks), dim3(64),
-                               partition_lds_bytes(T), s, A, T, pc, B.pb_part, plan ? *plan : SearchPlan{},
-                               plan ? 1 : 0);
+            {
+                const dim3 pgrid((A.nb + kPartBlocks - 1) / kPartBlocks);
+                const size_t plds = partition_lds_bytes(T);
+                const SearchPlan pplan = plan ? *plan : SearchPlan{};
+                const int use = plan ? 1 : 0;
+                if (pc == 2) hipLaunchKernelGGL(k_partition<2>, pgrid, dim3(64), plds, s, A, T, B.pb_part, pplan, use);
+                else if (pc == 3) hipLaunchKernelGGL(k_partition<3>, pgrid, dim3(64), plds, s, A, T, B.pb_part, pplan, use);
+                else hipLaunchKernelGGL(k_partition<4>, pgrid, dim3(64), plds, s, A, T, B.pb_part, pplan, use);
+            }
             if (plan) {
                 hipLaunchKernelGGL(k_plan, dim3(blocks_for(A.nb)), dim3(64), 0, s, A, T, pc, B.pb_part, *plan,
                                    plan->pb_count + (pc - 2));
