@@ -675,8 +675,11 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int1
 // quant level by quant level.
 __device__ __constant__ int kColorLevels[17] = {5, 7, 9, 11, 15, 19, 23, 31, 39, 47, 63, 79, 95, 127, 159, 191, 255};
 
-__global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
+// Specialised per partition count (launched once per count).
+template <int PC>
+__global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                                     Ideal* ideal_out, float* wts_out, float* ct_err, uint16_t* ct_fmt) {
+    constexpr int pc = PC;
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     if (pb >= pb_limit(A, pc, npb)) return;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
@@ -3294,8 +3297,21 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             }
             if (prof) prof->end(K_PARTITION, s);
         }
-        ASTCG_LAUNCH(K_IDEAL_COLOR, k_ideal_color, blocks_for(npb), A, T, pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err,
-                     B.ct_fmt);
+        {
+            if (prof) prof->begin(K_IDEAL_COLOR, s);
+            const dim3 igrid(blocks_for(npb));
+#define ASTCG_IDEAL(PCV)                                                                                          \
+    hipLaunchKernelGGL(k_ideal_color<PCV>, igrid, dim3(threads), 0, s, A, T, B.pb_part, npb, B.ideal, B.wts, B.ct_err, \
+                       B.ct_fmt)
+            switch (pc) {
+                case 1: ASTCG_IDEAL(1); break;
+                case 2: ASTCG_IDEAL(2); break;
+                case 3: ASTCG_IDEAL(3); break;
+                default: ASTCG_IDEAL(4); break;
+            }
+#undef ASTCG_IDEAL
+            if (prof) prof->end(K_IDEAL_COLOR, s);
+        }
         {
             if (prof) prof->begin(K_DM_BM, s);
             for (int g = 0; g < kDmGroups; ++g) {
