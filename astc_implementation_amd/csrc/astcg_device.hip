@@ -970,12 +970,14 @@ unsigned dm_bm_grid(int npb, int ndm_launch) {
     return (unsigned)(((groups + kXcds - 1) / kXcds) * kXcds * ndm_launch);
 }
 
-__global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, int pc, const int16_t* pb_part, int npb,
+template <int PC>
+__global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
                                               const Ideal* ideal, const float* wts_all, const float* ct_err,
                                               const uint16_t* ct_fmt, Ccr* ccr,
                                               uint8_t* qw_out, int wstride,
                                               const uint32_t* mode_mask, int mask_words, int dm_first,
                                               int dm_count) {
+    constexpr int pc = PC;
     extern __shared__ __align__(16) uint8_t smem[];
     const int ndm = T.ndm, tc = T.tc, lane = threadIdx.x;
     const int wg = blockIdx.x;
@@ -3318,9 +3320,17 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                 const int first = T.dm_group_start[g], count = T.dm_group_start[g + 1] - first;
                 if (count <= 0) continue;
                 const int w = T.dm_group_w[g];
-                hipLaunchKernelGGL(k_dm_bm, dim3(dm_bm_grid(npb, count)), dim3(64), dm_bm_lds_bytes(T, w, pc), s, A, T,
-                                   pc, B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw, w, mode_mask,
-                                   mask_words, first, count);
+#define ASTCG_DM_BM(PCV)                                                                                          \
+    hipLaunchKernelGGL(k_dm_bm<PCV>, dim3(dm_bm_grid(npb, count)), dim3(64), dm_bm_lds_bytes(T, w, pc), s, A, T, \
+                       B.pb_part, npb, B.ideal, B.wts, B.ct_err, B.ct_fmt, B.ccr, B.qw, w, mode_mask, mask_words, first,  \
+                       count)
+                switch (pc) {
+                    case 1: ASTCG_DM_BM(1); break;
+                    case 2: ASTCG_DM_BM(2); break;
+                    case 3: ASTCG_DM_BM(3); break;
+                    default: ASTCG_DM_BM(4); break;
+                }
+#undef ASTCG_DM_BM
             }
             if (prof) prof->end(K_DM_BM, s);
         }
