@@ -254,3 +254,35 @@ def test_partitioned_slots_default_and_valid(setup):
     dec = enc.decode(4, 4, 512, 512, outs[2])
     assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
     assert p_plain - A.psnr(dec, img) < 0.5
+
+
+def test_count_below_prunes_counts_by_the_count_head(setup):
+    """count_below: a block searches a partition count only when the count
+    head ranks it first or gives it at least count_below.  The counts a block
+    still searches see exactly the candidates they see without pruning (the
+    k-means kernel and the plan skip the same blocks), and with count_below
+    above every probability a block searches at most its top-ranked count.
+    Early out, fallback and the hard-block search are off, so nothing else
+    decides which counts are searched."""
+    enc, img, params, hidden, _ = setup
+    base = dict(fallback_below=0.0, early_out=0.0, full_search_above=0.0)
+
+    def run(cb):
+        enc.set_predictor(4, 4, params, hidden=hidden, count_below=cb, **base)
+        enc.configure(512, 512, 4, 4, labels=True, policy=A.POLICY_LEARNED)
+        enc.encode(img)
+        return enc.candidates()
+
+    s0, e0 = run(0.0)
+    searched0 = (e0 < 1e29).any(axis=2)  # (block, count 2..4)
+    assert searched0.all()
+    counts = []
+    for cb in (0.3, 1.01):
+        s, e = run(cb)
+        searched = (e < 1e29).any(axis=2)
+        counts.append(int(searched.sum()))
+        assert 0 < searched.sum() < searched0.sum()
+        assert np.array_equal(e[searched], e0[searched])
+        assert np.array_equal(s[searched], s0[searched])
+    assert (searched.sum(axis=1) <= 1).all()
+    assert counts[1] <= counts[0]
