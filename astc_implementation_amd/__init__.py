@@ -17,6 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libastcg.so")
 
 OK = 0
+ERR_OUT_OF_MEMORY = 6
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "invalid footprint", 3: "buffer too small",
     4: "not configured", 5: "device error", 6: "out of memory", 7: "i/o error", 8: "unsupported",

@@ -86,6 +86,7 @@ struct astcg_encoder {
     astcg_params params{};
     const FpState* cur = nullptr;
     int chunk_blocks = 0;
+    int chunk_limit = 0;  // automatic chunk size cap after an allocation failure (0 = none)
     uint8_t* d_rgba = nullptr;  // host-buffer staging
     uint8_t* d_out = nullptr;
     size_t rgba_cap = 0, out_cap = 0;
@@ -164,6 +165,7 @@ astcg_status fail(astcg_encoder* e, astcg_status s, const std::string& msg) {
 }
 
 astcg_status hip_fail(astcg_encoder* e, hipError_t err, const char* what) {
+    (void)hipGetLastError();  // reported through the status; later calls (ours or the caller's) start clean
     return fail(e, err == hipErrorOutOfMemory ? ASTCG_ERR_OUT_OF_MEMORY : ASTCG_ERR_DEVICE,
                 std::string(what) + ": " + hipGetErrorString(err));
 }
@@ -216,6 +218,7 @@ hipError_t grow(void** p, size_t* cap, size_t bytes) {
     *cap = 0;
     hipError_t err = hipMalloc(p, std::max<size_t>(bytes, 256));
     if (err == hipSuccess) *cap = std::max<size_t>(bytes, 256);
+    else (void)hipGetLastError();  // reported through the return value, not left for later calls
     return err;
 }
 
@@ -439,7 +442,11 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
     size_t offs[12];
     const size_t total = chunk_bytes_per_block(*F.fp, offs, chunk);
     hipError_t err = grow(&e->d_chunk, &e->chunk_cap, total);
-    if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
+    if (err != hipSuccess) {
+        e->layout_fp = nullptr;  // the old buffers are gone
+        e->layout_chunk = 0;
+        return hip_fail(e, err, "chunk buffers");
+    }
     uint8_t* b = static_cast<uint8_t*>(e->d_chunk);
     ChunkBuffers& B = e->B;
     B.pb_part = reinterpret_cast<int16_t*>(b + offs[0]);
@@ -474,6 +481,27 @@ LabelBufs label_bufs(void* base, size_t nb) {
     l.best_seed = reinterpret_cast<int16_t*>(l.cand_err + 12 * nb);
     l.cand_seed = l.best_seed + 3 * nb;
     return l;
+}
+
+// Lays the chunk buffers out for `nblocks` blocks of footprint F and returns
+// the chunk size used.  The automatic size (requested == 0) halves after an
+// allocation failure (another process may hold device memory) down to 4096
+// blocks, and the encoder remembers the smaller size; results do not depend
+// on the chunk size.  A requested size is used as given.
+astcg_status layout_fitting(astcg_encoder* e, const FpState& F, uint32_t nblocks, uint32_t requested, hipStream_t s,
+                            int* chunk_out) {
+    int chunk = chunk_size(*F.fp, nblocks, requested);
+    if (!requested && e->chunk_limit > 0) chunk = std::min(chunk, e->chunk_limit);
+    for (;;) {
+        const astcg_status st = layout_chunk(e, F, chunk, s);
+        if (st == ASTCG_OK) {
+            *chunk_out = chunk;
+            return ASTCG_OK;
+        }
+        if (st != ASTCG_ERR_OUT_OF_MEMORY || requested || chunk <= 4096) return st;
+        chunk = std::max(4096, chunk / 2);
+        e->chunk_limit = chunk;
+    }
 }
 
 constexpr int kInferBlocks = 32768;  // predictor inference sub-chunk
@@ -612,8 +640,8 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
 astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
                         uint32_t flags, hipStream_t s, const LabelBufs* labels = nullptr, const PredState* pred = nullptr,
                         const std::vector<TexDesc>* texs = nullptr) {
-    const int chunk = chunk_size(*F.fp, (uint32_t)A.num_blocks, chunk_req);
-    astcg_status st = layout_chunk(e, F, chunk, s);
+    int chunk = 0;
+    astcg_status st = layout_fitting(e, F, (uint32_t)A.num_blocks, chunk_req, s, &chunk);
     if (st != ASTCG_OK) return st;
     hipError_t err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)A.num_blocks);
     if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
@@ -788,8 +816,7 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->params = p;
     e->cur = F;
     // allocate the working set now so that encode_device allocates nothing
-    e->chunk_blocks = chunk_size(*F->fp, e->nblocks, p.chunk_blocks);
-    st = layout_chunk(e, *F, e->chunk_blocks, e->stream);
+    st = layout_fitting(e, *F, e->nblocks, p.chunk_blocks, e->stream, &e->chunk_blocks);
     if (st != ASTCG_OK) return st;
     err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)e->nblocks);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);

@@ -3261,6 +3261,7 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan,
                         bool dual_plane) {
+    (void)hipGetLastError();  // the final check reports this call's launches only
     const LabelBufs lab = labels ? *labels : LabelBufs{nullptr, nullptr, nullptr};
     const int threads = 64;
     auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
@@ -3370,6 +3371,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
 
 hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
                          hipStream_t s) {
+    (void)hipGetLastError();  // the final check reports this call's launches only
     const int blocks_x = (width + T.bx - 1) / T.bx;
     const int nblocks = blocks_x * ((height + T.by - 1) / T.by);
     hipLaunchKernelGGL(k_decode, dim3((nblocks + 63) / 64), dim3(64), 0, s, T, blocks, width, height, blocks_x, nblocks,
@@ -3378,12 +3380,14 @@ hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, i
 }
 
 hipError_t launch_fb_reset(unsigned long long* fb_count, hipStream_t s) {
+    (void)hipGetLastError();  // the final check reports this call's launches only
     hipLaunchKernelGGL(k_fb_reset, dim3(1), dim3(64), 0, s, fb_count);
     return hipGetLastError();
 }
 
 hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof,
                        bool void_extent) {
+    (void)hipGetLastError();  // the final check reports this call's launches only
     const int threads = 64;
     ASTCG_LAUNCH(K_PACK, k_pack, (A.num_blocks + threads - 1) / threads, T, A, best, void_extent ? 1 : 0);
     return hipGetLastError();

@@ -449,6 +449,7 @@ extern "C" {
 
 astcg_status astcg_mlp_gemm(int m, int n, int k, const void* a, int lda, int trans_a, const void* b, int ldb,
                             int trans_b, float* c, int ldc, const float* bias, int accumulate, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (m <= 0 || n <= 0 || k <= 0 || !a || !b || !c) return ASTCG_ERR_INVALID_ARGUMENT;
     const dim3 grid((n + kBN - 1) / kBN, (m + kBM - 1) / kBM);
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -462,6 +463,7 @@ astcg_status astcg_mlp_gemm(int m, int n, int k, const void* a, int lda, int tra
 }
 
 astcg_status astcg_mlp_gelu(const float* z, void* h, long long n, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!z || !h || n < 0) return ASTCG_ERR_INVALID_ARGUMENT;
     if (n) hipLaunchKernelGGL(k_gelu_fwd, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), z,
                               static_cast<bf16*>(h), n);
@@ -470,6 +472,7 @@ astcg_status astcg_mlp_gelu(const float* z, void* h, long long n, void* stream) 
 
 astcg_status astcg_mlp_gelu_backward(const float* dh, const float* z, void* dz, float* dz32, long long n,
                                      void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!dh || !z || !dz || n < 0) return ASTCG_ERR_INVALID_ARGUMENT;
     if (n) hipLaunchKernelGGL(k_gelu_bwd, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), dh, z,
                               static_cast<bf16*>(dz), dz32, n);
@@ -477,6 +480,7 @@ astcg_status astcg_mlp_gelu_backward(const float* dh, const float* z, void* dz, 
 }
 
 astcg_status astcg_mlp_to_bf16(const float* x, void* y, long long n, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!x || !y || n < 0) return ASTCG_ERR_INVALID_ARGUMENT;
     if (n) hipLaunchKernelGGL(k_to_bf16, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x,
                               static_cast<bf16*>(y), n);
@@ -484,6 +488,7 @@ astcg_status astcg_mlp_to_bf16(const float* x, void* y, long long n, void* strea
 }
 
 astcg_status astcg_mlp_colsum(const float* x, int rows, int cols, float* out, int accumulate, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!x || !out || rows < 0 || cols <= 0) return ASTCG_ERR_INVALID_ARGUMENT;
     hipLaunchKernelGGL(k_colsum, dim3(grid1(cols, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), x, rows, cols,
                        out, accumulate);
@@ -493,6 +498,7 @@ astcg_status astcg_mlp_colsum(const float* x, int rows, int cols, float* out, in
 astcg_status astcg_mlp_softmax_xent(const float* logits, int rows, int ld, int nheads, const int* head_off,
                                     const int* head_len, const int* labels, float scale, void* dlogits,
                                     float* dlogits32, float* loss, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!logits || rows < 0 || nheads <= 0 || !head_off || !head_len || !labels || !dlogits)
         return ASTCG_ERR_INVALID_ARGUMENT;
     if (rows)
@@ -504,6 +510,7 @@ astcg_status astcg_mlp_softmax_xent(const float* logits, int rows, int ld, int n
 astcg_status astcg_mlp_rank_xent(const float* logits, int rows, int ld, int ngroups, int ncand, const int* cols,
                                  const int* target, float scale, void* dlogits, float* dlogits32, float* loss,
                                  void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!logits || rows < 0 || ngroups <= 0 || ncand <= 0 || ncand > 64 || !cols || !target || !dlogits)
         return ASTCG_ERR_INVALID_ARGUMENT;
     if (rows)
@@ -514,6 +521,7 @@ astcg_status astcg_mlp_rank_xent(const float* logits, int rows, int ld, int ngro
 
 astcg_status astcg_mlp_adam(float* w, const float* g, float* m, float* v, void* w16, long long n, float lr,
                             float beta1, float beta2, float eps, int step, float weight_decay, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!w || !g || !m || !v || n < 0 || step < 1) return ASTCG_ERR_INVALID_ARGUMENT;
     float c1 = 1.0f, c2 = 1.0f;
     for (int i = 0; i < step && i < 100000; ++i) { c1 *= beta1; c2 *= beta2; }
@@ -527,6 +535,7 @@ astcg_status astcg_mlp_adam(float* w, const float* g, float* m, float* v, void* 
 
 astcg_status astcg_mlp_features(const uint8_t* d_rgba, int width, int height, int block_x, int block_y,
                                 int first_block, int num_blocks, void* x, int ldx, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!d_rgba || !x || width <= 0 || height <= 0 || block_x < 4 || block_y < 4 || num_blocks < 0 ||
         ldx < 3 * block_x * block_y)
         return ASTCG_ERR_INVALID_ARGUMENT;
@@ -539,12 +548,14 @@ astcg_status astcg_mlp_features(const uint8_t* d_rgba, int width, int height, in
 
 astcg_status astcg_mlp_topk(const float* logits, int rows, int ld, int nheads, const int* head_off,
                             const int* head_len, int k, int* out_idx, float* out_prob, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     return astcg_mlp_topk_heads(logits, rows, ld, nheads, head_off, head_len, k, nullptr, out_idx, out_prob, stream);
 }
 
 astcg_status astcg_mlp_topk_heads(const float* logits, int rows, int ld, int nheads, const int* head_off,
                                   const int* head_len, int kmax, const int* head_k, int* out_idx, float* out_prob,
                                   void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     if (!logits || rows < 0 || nheads <= 0 || !head_off || !head_len || kmax <= 0 || !out_idx)
         return ASTCG_ERR_INVALID_ARGUMENT;
     if (rows)
@@ -557,6 +568,7 @@ astcg_status astcg_mlp_topk_heads(const float* logits, int rows, int ld, int nhe
 astcg_status astcg_mlp_infer(int rows, const void* x, int ldx, int nin, const void* w1t, const float* b1,
                              const void* w2t, const float* b2, const void* w3t, const float* b3, int n3, int nmodes,
                              int kmax, int kpc, int kmode, int* out_idx, float* out_prob, void* h2_out, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
     const int kin = (nin + 63) / 64 * 64, n3p = (n3 + 15) / 16 * 16;
     if (!x || !w1t || !b1 || !w2t || !b2 || !w3t || !b3 || !out_idx || rows < 0 || nin <= 0 || kin > 256 ||
         ldx < nin || n3 < 4 + nmodes || nmodes <= 0 || n3p > 256 || kpc < 0 || kpc > 4 || kpc > kmax || kmode < 0 ||

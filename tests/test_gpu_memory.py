@@ -1,0 +1,50 @@
+"""The automatic chunk size when device memory is short: the whole image is
+one chunk by default (about 14 KB of search state per 4x4 block); when that
+does not fit, the encoder halves the chunk until it does, and the blocks do
+not depend on the chunk size."""
+import numpy as np
+import pytest
+
+import astc_implementation_amd as A
+from astc_implementation_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_automatic_chunk_halves_when_device_memory_is_short():
+    import torch
+    img = synth.gradient_image(4096, 2048, 5)  # 524 288 blocks: a 7 GiB chunk at 4x4
+    enc = A.Encoder(0)
+    enc.configure(4096, 2048, 4, 4)
+    ref = enc.encode(img)
+    enc.close()
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    hold = torch.empty(free - (5 << 30), dtype=torch.uint8, device="cuda")  # leave 5 GiB
+    try:
+        enc = A.Encoder(0)
+        enc.configure(4096, 2048, 4, 4)
+        out = enc.encode(img)
+        enc.close()
+    finally:
+        del hold
+        torch.cuda.empty_cache()
+    assert np.array_equal(out, ref)
+
+
+def test_requested_chunk_that_does_not_fit_fails_cleanly():
+    """An explicit chunk size is used as given: too large for the free
+    memory, configure reports out of memory instead of shrinking it."""
+    import torch
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    hold = torch.empty(free - (2 << 30), dtype=torch.uint8, device="cuda")  # leave 2 GiB
+    try:
+        enc = A.Encoder(0)
+        with pytest.raises(A.AstcError) as ei:
+            enc.configure(4096, 2048, 4, 4, chunk_blocks=524288)
+        assert ei.value.status == A.ERR_OUT_OF_MEMORY
+        enc.close()
+    finally:
+        del hold
+        torch.cuda.empty_cache()
