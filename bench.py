@@ -35,10 +35,10 @@ def measured_traffic(workload, kernel):
         with open(TRAFFIC_FILE) as f:
             entry = json.load(f).get(workload, {}).get(kernel)
     except (OSError, ValueError):
-        return None, None
+        return None, None, None
     if not entry:
-        return None, None
-    return entry["bytes_per_launch"], entry["source"]
+        return None, None, None
+    return entry["bytes_per_launch"], entry["source"], entry.get("valu_issue_pct_est")
 
 
 def parse():
@@ -330,7 +330,7 @@ def main():
     texels_per_launch = texels * a.steps / max(launches, 1)
     achieved = bytes_per_texel * texels_per_launch / (avg_ms / 1e3) / 1e9
     workload_key = "%dx%d seed %d %dx%d %s" % (w, h, SEED, bx, by, head)
-    traffic, traffic_src = measured_traffic(workload_key, dom)
+    traffic, traffic_src, valu_busy = measured_traffic(workload_key, dom)
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -365,7 +365,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches,
                      "algorithmic_bytes_per_launch": bytes_per_texel * texels_per_launch,
-                     "traffic_source": traffic_src},
+                     "traffic_source": traffic_src,
+                     # the bound that matters for this search: the kernel's wave64
+                     # VALU instructions x 4 cycles against the 1024 SIMDs' issue
+                     # capacity over its duration (an estimate, same profile)
+                     "valu_issue_pct_est": valu_busy},
         "kernels_ms_per_step": {k: v["total_ms"] / a.steps for k, v in ktimes.items()},
     }
     if not a.no_cpu:
