@@ -1003,6 +1003,10 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
             s_tpart[t * kDmPbs + l] = T.part_of_texel[((size_t)(pc - 2) * kPartitionings + pb_part[slot]) * tc + t];
         }
     }
+    // the weight unquantisation rows (quant levels 0..11) for pass07's
+    // per-lane lookups, from LDS instead of constant memory
+    __shared__ uint8_t s_q2u[12 * 32];
+    for (int e = lane; e < 12 * 32; e += 64) s_q2u[e] = c_qt.quant_to_unquant[e >> 5][e & 31];
     __syncthreads();
     const int pb = pb0 + lane;
     if (pb >= npb_live) return;
@@ -1260,7 +1264,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
                 const float ix1 = ix * qlm1;
                 const uint32_t wl = wf2u(ix1);
                 const uint32_t wh = min(wl + 1u, qmax);
-                const int il = c_qt.quant_to_unquant[q][wl], ih = c_qt.quant_to_unquant[q][wh];
+                const int il = s_q2u[q * 32 + wl], ih = s_q2u[q * 32 + wh];
                 const float fl = (float)il, fh = (float)ih;
                 const bool use_high = (fl + fh) < (128.0f * ix);
                 qwo[(size_t)i * npb] = i < keep ? (uint8_t)(use_high ? ih : il) : 0;
