@@ -94,6 +94,13 @@ struct astcg_encoder {
     TexDesc* d_tex = nullptr;
     size_t tex_cap = 0;
     std::vector<TexDesc> h_tex;
+    // pinned staging of the texture table: the upload is enqueued without a
+    // host wait; the next batch waits only for that copy (tex_copied) before
+    // it rewrites the staging buffer
+    TexDesc* h_tex_pinned = nullptr;
+    size_t tex_pinned_cap = 0;
+    hipEvent_t tex_copied = nullptr;
+    bool tex_copy_pending = false;
     std::map<int, PredState> preds;  // keyed like fps
     void* d_pwork = nullptr;         // predictor inference workspace
     size_t pwork_cap = 0;
@@ -179,6 +186,13 @@ void release_all(astcg_encoder* e) {
     if (e->d_rgba) (void)hipFree(e->d_rgba);
     if (e->d_out) (void)hipFree(e->d_out);
     if (e->d_tex) (void)hipFree(e->d_tex);
+    if (e->tex_copied) (void)hipEventSynchronize(e->tex_copied);
+    if (e->h_tex_pinned) (void)hipHostFree(e->h_tex_pinned);
+    if (e->tex_copied) (void)hipEventDestroy(e->tex_copied);
+    e->h_tex_pinned = nullptr;
+    e->tex_pinned_cap = 0;
+    e->tex_copied = nullptr;
+    e->tex_copy_pending = false;
     if (e->d_pcseed) (void)hipFree(e->d_pcseed);
     for (auto& kv : e->preds) free_pred(kv.second);
     e->preds.clear();
@@ -423,10 +437,12 @@ astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpS
 // pay that tail fewer times.  A requested size is capped so that the element
 // indices of the per-PB arrays ([colour-table row or trial mode][PB]) stay
 // below 2^31.
-int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested) {
+size_t plan_bytes(const Footprint& f, const PredState* P, int nb);
+
+int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested, const PredState* pred) {
     size_t offs[12];
-    const size_t fixed = chunk_bytes_per_block(f, offs, 0);
-    const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) - fixed) >> 16;
+    const size_t fixed = chunk_bytes_per_block(f, offs, 0) + plan_bytes(f, pred, 0);
+    const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) + plan_bytes(f, pred, 1 << 16) - fixed) >> 16;
     const size_t max_chunk = (((size_t)1 << 31) - 1) / (4 * (size_t)std::max(kColorTabRows, f.nbm) + 8 * 4);
     size_t chunk = std::max<size_t>(1024, ((size_t(16) << 30) - fixed) / per_block);
     if (requested) chunk = requested;
@@ -483,17 +499,28 @@ LabelBufs label_bufs(void* base, size_t nb) {
     return l;
 }
 
-// Lays the chunk buffers out for `nblocks` blocks of footprint F and returns
-// the chunk size used.  The automatic size (requested == 0) halves after an
-// allocation failure (another process may hold device memory) down to 4096
-// blocks, and the encoder remembers the smaller size; results do not depend
-// on the chunk size.  A requested size is used as given.
+astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, const ImageArgs& A,
+                        const std::vector<TexDesc>* texs, hipStream_t s, SearchPlan* plan, bool reserve_only = false);
+
+// Lays the chunk buffers out for `nblocks` blocks of footprint F (and, with a
+// predictor, sizes the LEARNED policy's plan buffers and inference workspace
+// for a chunk) and returns the chunk size used.  The automatic size
+// (requested == 0) halves after an allocation failure (another process may
+// hold device memory) down to 4096 blocks, and the encoder keeps the smaller
+// size until the next configure or batched call; results do not depend on
+// the chunk size.  A requested size is used as given.
 astcg_status layout_fitting(astcg_encoder* e, const FpState& F, uint32_t nblocks, uint32_t requested, hipStream_t s,
-                            int* chunk_out) {
-    int chunk = chunk_size(*F.fp, nblocks, requested);
+                            int* chunk_out, const PredState* pred = nullptr) {
+    int chunk = chunk_size(*F.fp, nblocks, requested, pred);
     if (!requested && e->chunk_limit > 0) chunk = std::min(chunk, e->chunk_limit);
     for (;;) {
-        const astcg_status st = layout_chunk(e, F, chunk, s);
+        astcg_status st = layout_chunk(e, F, chunk, s);
+        if (st == ASTCG_OK && pred) {
+            ImageArgs A1{};
+            A1.nb = chunk;
+            SearchPlan unused;
+            st = plan_chunk(e, F, *pred, A1, nullptr, s, &unused, true);
+        }
         if (st == ASTCG_OK) {
             *chunk_out = chunk;
             return ASTCG_OK;
@@ -507,41 +534,82 @@ astcg_status layout_fitting(astcg_encoder* e, const FpState& F, uint32_t nblocks
 constexpr int kInferBlocks = 32768;  // predictor inference sub-chunk
 typedef const uint8_t* bfloat16_ptr;    // bf16 weights addressed bytewise
 
-// LEARNED policy: runs the predictor over the chunk's blocks and lays out the
-// search plan (top-k of every head per block, then the plan kernels).
-astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, const ImageArgs& A,
-                        const std::vector<TexDesc>& texs, hipStream_t s, SearchPlan* plan, bool reserve_only = false) {
+// Plan buffers and inference workspace of the LEARNED policy for a chunk of
+// nb blocks: offsets into the plan allocation (o_*) and the workspace (w_*).
+struct PlanLayout {
+    int kmax = 4, words = 1, sub = 0;
+    bool rank = false, prune_refine = false, fused = false;
+    size_t o_idx = 0, o_prob = 0, o_mask = 0, o_slots = 0, o_fb = 0, o_list = 0, o_inv = 0, o_cnt = 0, o_h2all = 0,
+           o_est = 0, plan_size = 0;
+    size_t w_x = 0, w_z = 0, w_h = 0, w_z2 = 0, w_h2 = 0, w_l = 0, work_size = 0;
+};
+
+PlanLayout plan_layout(const Footprint& f, const PredState& P, int nb) {
     const astcg_learned_params& lp = P.lp;
-    const int kmax = (int)std::max<uint32_t>(std::max<uint32_t>(lp.seeds, lp.modes), 4);
-    const int nb = A.nb, words = (F.fp->nbm + 31) / 32;
-    // plan buffers
+    PlanLayout Y;
+    Y.kmax = (int)std::max<uint32_t>(std::max<uint32_t>(lp.seeds, lp.modes), 4);
+    Y.words = (f.nbm + 31) / 32;
     struct Lay { size_t size = 0; size_t add(size_t b) { size = (size + 255) & ~size_t(255); size_t o = size; size += b; return o; } } L;
-    const size_t o_idx = L.add(sizeof(int) * (size_t)nb * 5 * kmax), o_prob = L.add(sizeof(float) * (size_t)nb * 5 * kmax);
-    const size_t o_mask = L.add(sizeof(uint32_t) * (size_t)nb * words), o_slots = L.add((size_t)nb);
-    const size_t o_fb = L.add((size_t)nb), o_list = L.add(sizeof(int) * (size_t)nb * 4);
-    const size_t o_inv = L.add(sizeof(int) * (size_t)nb * 4), o_cnt = L.add(sizeof(int) * 4);
+    Y.o_idx = L.add(sizeof(int) * (size_t)nb * 5 * Y.kmax);
+    Y.o_prob = L.add(sizeof(float) * (size_t)nb * 5 * Y.kmax);
+    Y.o_mask = L.add(sizeof(uint32_t) * (size_t)nb * Y.words);
+    Y.o_slots = L.add((size_t)nb);
+    Y.o_fb = L.add((size_t)nb);
+    Y.o_list = L.add(sizeof(int) * (size_t)nb * 4);
+    Y.o_inv = L.add(sizeof(int) * (size_t)nb * 4);
+    Y.o_cnt = L.add(sizeof(int) * 4);
     // ranking the compat positions needs every block's second hidden layer
-    const bool rank = P.w3_seed != nullptr;
-    const size_t o_h2all = rank ? L.add(2 * (size_t)nb * P.hidden) : 0;
-    const bool prune_refine = lp.refine_partitionings >= 1 && lp.refine_partitionings < 4;
-    const size_t o_est = prune_refine ? L.add(sizeof(float) * (size_t)nb * 4) : 0;
-    hipError_t err = grow(&e->d_plan, &e->plan_cap, L.size);
-    if (err != hipSuccess) return hip_fail(e, err, "plan buffers");
-    uint8_t* pb = static_cast<uint8_t*>(e->d_plan);
+    Y.rank = P.w3_seed != nullptr;
+    Y.o_h2all = Y.rank ? L.add(2 * (size_t)nb * P.hidden) : 0;
+    Y.prune_refine = lp.refine_partitionings >= 1 && lp.refine_partitionings < 4;
+    Y.o_est = Y.prune_refine ? L.add(sizeof(float) * (size_t)nb * 4) : 0;
+    Y.plan_size = L.size;
     // inference workspace for one sub-chunk.  The fused kernel needs only the
     // features (2 * nin bytes per block) and runs over the whole chunk in one
     // launch, so its workgroups overlap their serial layer-by-layer latency;
     // the separate GEMM / GELU / top-k launches keep fp32 intermediates of the
     // 3-head output layer (about 16 KB per block) and run in sub-chunks.
-    const bool fused = P.fw[0] && lp.seeds == 0;
-    const int H = P.hidden, NO = P.nout, sub = fused ? nb : std::min(nb, kInferBlocks);
-    const int usub = fused ? 0 : sub;  // rows of the unfused intermediates
+    Y.fused = P.fw[0] && lp.seeds == 0;
+    const int H = P.hidden, NO = P.nout;
+    Y.sub = Y.fused ? nb : std::min(nb, kInferBlocks);
+    const int usub = Y.fused ? 0 : Y.sub;  // rows of the unfused intermediates
     Lay W;
-    const size_t o_x = W.add(2 * (size_t)sub * P.nin), o_z = W.add(4 * (size_t)usub * H), o_h = W.add(2 * (size_t)usub * H);
-    const size_t o_z2 = W.add(4 * (size_t)usub * H), o_h2 = W.add(2 * (size_t)usub * H), o_l = W.add(4 * (size_t)usub * NO);
-    err = grow(&e->d_pwork, &e->pwork_cap, W.size);
+    Y.w_x = W.add(2 * (size_t)Y.sub * P.nin);
+    Y.w_z = W.add(4 * (size_t)usub * H);
+    Y.w_h = W.add(2 * (size_t)usub * H);
+    Y.w_z2 = W.add(4 * (size_t)usub * H);
+    Y.w_h2 = W.add(2 * (size_t)usub * H);
+    Y.w_l = W.add(4 * (size_t)usub * NO);
+    Y.work_size = W.size;
+    return Y;
+}
+
+// LEARNED policy: runs the predictor over the chunk's blocks and lays out the
+// search plan (top-k of every head per block, then the plan kernels).  With
+// reserve_only the buffers are only sized for A.nb blocks.
+size_t plan_bytes(const Footprint& f, const PredState* P, int nb) {
+    if (!P) return 0;
+    const PlanLayout Y = plan_layout(f, *P, nb);
+    return Y.plan_size + Y.work_size;
+}
+
+astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, const ImageArgs& A,
+                        const std::vector<TexDesc>* texs, hipStream_t s, SearchPlan* plan, bool reserve_only) {
+    const astcg_learned_params& lp = P.lp;
+    const int nb = A.nb;
+    const PlanLayout Y = plan_layout(*F.fp, P, nb);
+    const int kmax = Y.kmax, words = Y.words;
+    hipError_t err = grow(&e->d_plan, &e->plan_cap, Y.plan_size);
+    if (err != hipSuccess) return hip_fail(e, err, "plan buffers");
+    err = grow(&e->d_pwork, &e->pwork_cap, Y.work_size);
     if (err != hipSuccess) return hip_fail(e, err, "predictor workspace");
     if (reserve_only) return ASTCG_OK;  // buffers sized, nothing enqueued
+    uint8_t* pb = static_cast<uint8_t*>(e->d_plan);
+    const size_t o_idx = Y.o_idx, o_prob = Y.o_prob, o_mask = Y.o_mask, o_slots = Y.o_slots, o_fb = Y.o_fb,
+                 o_list = Y.o_list, o_inv = Y.o_inv, o_cnt = Y.o_cnt, o_h2all = Y.o_h2all, o_est = Y.o_est;
+    const size_t o_x = Y.w_x, o_z = Y.w_z, o_h = Y.w_h, o_z2 = Y.w_z2, o_h2 = Y.w_h2, o_l = Y.w_l;
+    const bool rank = Y.rank, prune_refine = Y.prune_refine, fused = Y.fused;
+    const int H = P.hidden, NO = P.nout, sub = Y.sub;
     uint8_t* wb = static_cast<uint8_t*>(e->d_pwork);
     void* X = wb + o_x;
     float* Z1 = reinterpret_cast<float*>(wb + o_z);
@@ -555,7 +623,7 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
         const int n = std::min(sub, nb - b0);
         const int g0 = A.blk0 + b0;
         // features: one call per texture overlapping the blocks g0 .. g0 + n - 1
-        for (const TexDesc& t : texs) {
+        for (const TexDesc& t : *texs) {
             const int lo = std::max(g0, t.first_block), hi = std::min(g0 + n, t.first_block + t.num_blocks);
             if (lo >= hi) continue;
             astcg_status st = astcg_mlp_features(t.rgba, t.width, t.height, (int)F.fp->bx, (int)F.fp->by,
@@ -641,7 +709,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
                         uint32_t flags, hipStream_t s, const LabelBufs* labels = nullptr, const PredState* pred = nullptr,
                         const std::vector<TexDesc>* texs = nullptr) {
     int chunk = 0;
-    astcg_status st = layout_fitting(e, F, (uint32_t)A.num_blocks, chunk_req, s, &chunk);
+    astcg_status st = layout_fitting(e, F, (uint32_t)A.num_blocks, chunk_req, s, &chunk, pred);
     if (st != ASTCG_OK) return st;
     hipError_t err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)A.num_blocks);
     if (err != hipSuccess) return hip_fail(e, err, "symbolic block buffer");
@@ -656,14 +724,8 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
     const bool graph = (flags & ASTCG_FLAG_GRAPH) && s != nullptr && !(flags & ASTCG_FLAG_PROFILE);
     std::vector<uint8_t> key;
     if (graph) {
-        if (pred) {  // size the plan buffers now: no allocation may happen while capturing
-            ImageArgs A1 = A;
-            A1.blk0 = 0;
-            A1.nb = std::min(chunk, A.num_blocks);
-            SearchPlan tmp;
-            st = plan_chunk(e, F, *pred, A1, *texs, s, &tmp, true);
-            if (st != ASTCG_OK) return st;
-        }
+        // layout_fitting has sized every buffer (plan buffers included): no
+        // allocation happens while capturing
         auto put = [&key](const void* p, size_t n) {
             const uint8_t* b = static_cast<const uint8_t*>(p);
             key.insert(key.end(), b, b + n);
@@ -695,7 +757,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
         A.blk0 = blk0;
         A.nb = std::min(chunk, A.num_blocks - blk0);
         SearchPlan plan;
-        if (pred) st = plan_chunk(e, F, *pred, A, *texs, s, &plan);
+        if (pred) st = plan_chunk(e, F, *pred, A, texs, s, &plan);
         if (st != ASTCG_OK) break;
         err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, labels, pred ? &plan : nullptr,
                            (flags & ASTCG_FLAG_DUAL_PLANE) != 0);
@@ -815,8 +877,14 @@ astcg_status astcg_configure(astcg_encoder* e, uint32_t width, uint32_t height, 
     e->nblocks = (uint32_t)nb;
     e->params = p;
     e->cur = F;
-    // allocate the working set now so that encode_device allocates nothing
-    st = layout_fitting(e, *F, e->nblocks, p.chunk_blocks, e->stream, &e->chunk_blocks);
+    // allocate the working set now (plan buffers included under the LEARNED
+    // policy) so that encode_device allocates nothing; a chunk cap left by an
+    // earlier memory shortage is dropped, the full automatic size is tried
+    // first
+    e->chunk_limit = 0;
+    const PredState* pred = nullptr;
+    if (p.policy == ASTCG_POLICY_LEARNED) pred = &e->preds.find((int)(bx * 16 + by))->second;
+    st = layout_fitting(e, *F, e->nblocks, p.chunk_blocks, e->stream, &e->chunk_blocks, pred);
     if (st != ASTCG_OK) return st;
     err = grow(reinterpret_cast<void**>(&e->d_best), &e->best_cap, sizeof(SymBest) * (size_t)e->nblocks);
     if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
@@ -952,11 +1020,26 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
         if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
     }
     if (!e->h_tex.empty()) {
-        hipError_t err = grow(reinterpret_cast<void**>(&e->d_tex), &e->tex_cap, sizeof(TexDesc) * e->h_tex.size());
-        if (err == hipSuccess)
-            err = hipMemcpyAsync(e->d_tex, e->h_tex.data(), sizeof(TexDesc) * e->h_tex.size(), hipMemcpyHostToDevice, s);
-        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        const size_t bytes = sizeof(TexDesc) * e->h_tex.size();
+        hipError_t err = grow(reinterpret_cast<void**>(&e->d_tex), &e->tex_cap, bytes);
+        // the previous batch's table copy must have left the staging buffer
+        if (err == hipSuccess && e->tex_copy_pending) err = hipEventSynchronize(e->tex_copied);
+        if (err == hipSuccess) e->tex_copy_pending = false;
+        if (err == hipSuccess && e->tex_pinned_cap < bytes) {
+            if (e->h_tex_pinned) (void)hipHostFree(e->h_tex_pinned);
+            e->h_tex_pinned = nullptr;
+            e->tex_pinned_cap = 0;
+            err = hipHostMalloc(reinterpret_cast<void**>(&e->h_tex_pinned), bytes, hipHostMallocDefault);
+            if (err == hipSuccess) e->tex_pinned_cap = bytes;
+        }
+        if (err == hipSuccess && !e->tex_copied) err = hipEventCreateWithFlags(&e->tex_copied, hipEventDisableTiming);
+        if (err == hipSuccess) {
+            std::memcpy(e->h_tex_pinned, e->h_tex.data(), bytes);
+            err = hipMemcpyAsync(e->d_tex, e->h_tex_pinned, bytes, hipMemcpyHostToDevice, s);
+        }
+        if (err == hipSuccess) err = hipEventRecord(e->tex_copied, s);
         if (err != hipSuccess) return hip_fail(e, err, "texture table");
+        e->tex_copy_pending = true;
     }
     for (size_t gi = 0; gi < keys.size(); ++gi) {
         const FpState* F = nullptr;
@@ -995,8 +1078,10 @@ astcg_status astcg_encode_batch_device(astcg_encoder* e, const astcg_texture* te
     }
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
-    // the search buffers may be laid out for another footprint afterwards
+    // the search buffers may be laid out for another footprint afterwards;
+    // a chunk cap of an earlier memory shortage is dropped
     e->layout_fp = nullptr;
+    e->chunk_limit = 0;
     st = batch_device(e, textures, count, p, static_cast<hipStream_t>(stream));
     e->layout_fp = nullptr;
     return st;
@@ -1035,6 +1120,7 @@ astcg_status astcg_encode_batch(astcg_encoder* e, const astcg_texture* textures,
         oo += ((size_t)nbs[i] * 16 + 255) & ~size_t(255);
     }
     e->layout_fp = nullptr;
+    e->chunk_limit = 0;
     st = batch_device(e, dev.data(), count, p, e->stream);
     e->layout_fp = nullptr;
     if (st != ASTCG_OK) return st;
@@ -1151,6 +1237,8 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     size_t need = 0;
     for (size_t z : sizes) need += z;
     if (count != need) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "predictor weight count mismatch");
+    for (size_t i = 0; i < count; ++i)
+        if (!std::isfinite(weights[i])) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "predictor weights must be finite");
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
     const int key = (int)(bx * 16 + by);

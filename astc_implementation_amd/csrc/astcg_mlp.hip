@@ -432,10 +432,12 @@ __global__ void __launch_bounds__(256) k_infer(int rows, const bf16* __restrict_
             const int oj = __shfl_xor(bj, msk);
             if (ov > bv || (ov == bv && oj < bj)) { bv = ov; bj = oj; }
         }
-        if ((bj & 7) == sub) mw[bj] = -INFINITY;
+        // no column left above -inf (or NaN logits): the rank stays empty
+        const bool found = bj < nmodes;
+        if (found && (bj & 7) == sub) mw[bj] = -INFINITY;
         if (sub == 0) {
-            out_idx[o + 4 * kmax + rk] = bj;
-            if (out_prob) out_prob[o + 4 * kmax + rk] = expf(bv - mx) / se;
+            out_idx[o + 4 * kmax + rk] = found ? bj : -1;
+            if (out_prob) out_prob[o + 4 * kmax + rk] = found ? expf(bv - mx) / se : 0.0f;
         }
     }
     for (int j = nmodes + sub; j < kmode; j += 8) {

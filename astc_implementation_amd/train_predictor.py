@@ -138,7 +138,7 @@ def main():
     ap.add_argument("--rank-loss", type=int, default=0,
                     help="also train the seed heads to rank the 4 compat positions of each partition count")
     ap.add_argument("--out", default=None)
-    # pruning saved with the weights (tuned with eval_predictor)
+    # pruning saved with the weights (tuned on a held-out image)
     ap.add_argument("--seeds", type=int, default=0)
     ap.add_argument("--modes", type=int, default=16)
     ap.add_argument("--slots", type=int, default=5)

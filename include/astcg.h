@@ -179,7 +179,11 @@ astcg_status astcg_encode_batch(astcg_encoder* enc, const astcg_texture* texture
                                 const astcg_params* params);
 
 /* Same with `rgba` / `out` in device memory of the encoder's device; work is
- * enqueued on `stream` and the call returns without synchronising. */
+ * enqueued on `stream` and the call returns without waiting for it (a call
+ * that must grow the encoder's device buffers first waits for the device;
+ * calls with the same or smaller batches reuse them).  Successive calls on
+ * one encoder must use one stream or order their streams: they share the
+ * encoder's buffers. */
 astcg_status astcg_encode_batch_device(astcg_encoder* enc, const astcg_texture* textures, uint32_t count,
                                        const astcg_params* params, void* stream);
 

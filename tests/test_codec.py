@@ -139,10 +139,15 @@ def test_random_symbolic_round_trip(fp):
         lv = sorted(set(O.table(2)[(q - 4) * 512:(q - 3) * 512]))
         for i in range(32):
             sb.packed_color_values[i] = lv[codes[i] % len(lv)]
+        # weights: random legal unquantised values (0..64) of the mode's weight quant level
+        wx, wy, wq = _mode_grid_and_quant(sb.block_mode_index)
+        wlev = O.table(0)[sum(LEVELS[:wq]):sum(LEVELS[:wq + 1])]
+        wcodes = rng.integers(0, len(wlev), wx * wy)
+        for i in range(64):
+            sb.quantized_weights[i] = wlev[wcodes[i]] if i < wx * wy else 0
+        assert any(sb.quantized_weights[i] for i in range(wx * wy)) or wx * wy < 4
         blk = A.symbolic_to_physical(fp[0], fp[1], sb)
         back = A.physical_to_symbolic(fp[0], fp[1], blk)
-        wlev = None
-        # weights: all zero is legal for any weight quant; use the decoder's view
         pc = sb.partition_count
         assert back.block_mode_index == sb.block_mode_index
         assert back.partition_count == pc
@@ -150,10 +155,10 @@ def test_random_symbolic_round_trip(fp):
         for p in range(pc):
             n = 2 * (sb.partition_formats[p] >> 2) + 2
             assert list(back.packed_color_values)[p * 8:p * 8 + n] == list(sb.packed_color_values)[p * 8:p * 8 + n]
+        assert list(back.quantized_weights)[:wx * wy] == list(sb.quantized_weights)[:wx * wy]
         # same bits from the oracle's packer
         o = O.OrSymbolic.from_buffer_copy(bytes(sb))
         assert O.symbolic_to_physical(fp[0], fp[1], o) == blk
-        del wlev
         done += 1
 
 
@@ -202,6 +207,28 @@ def _ldr_endpoints(cem, v):
             return [cl(x) for x in blue(c1)], [cl(x) for x in blue(c0)]
         return [cl(x) for x in c0], [cl(x) for x in c1]
     raise ValueError(cem)
+
+
+def _mode_grid_and_quant(m):
+    """(grid width, grid height, weight quant level) of a 1-plane 2D block
+    mode, decoded from its bits (ASTC specification block mode layout;
+    quant level = (R - 2) + 6 H in the QUANT_2..QUANT_32 order)."""
+    A_ = (m >> 5) & 3
+    if m & 3:
+        R = ((m >> 4) & 1) | ((m & 3) << 1)
+        B = (m >> 7) & 3
+        k = (m >> 2) & 3
+        wx, wy = [(B + 4, A_ + 2), (B + 8, A_ + 2), (A_ + 2, B + 8),
+                  ((B & 1) + 2, A_ + 2) if (m & 0x100) else (A_ + 2, (B & 1) + 6)][k]
+        H = (m >> 9) & 1
+    else:
+        R = ((m >> 4) & 1) | (((m >> 2) & 3) << 1)
+        B = (m >> 9) & 3
+        k = (m >> 7) & 3
+        wx, wy = [(12, A_ + 2), (A_ + 2, 12), (A_ + 6, B + 6),
+                  (6, 10) if ((m >> 5) & 3) == 0 else (10, 6)][k]
+        H = 0 if k == 2 else (m >> 9) & 1
+    return wx, wy, (R - 2) + 6 * H
 
 
 def _grid(bx, by, wx, wy):
@@ -260,25 +287,7 @@ def test_reported_error_matches_reconstruction(oracle_blocks, fp):
             e0, e1 = _ldr_endpoints(cem, list(back.packed_color_values)[p * 8:p * 8 + n])
             eps.append(([c * 257 for c in e0], [c * 257 for c in e1]))
             vi += n
-        # find the grid: the number of non-padding weights is wx*wy; try the
-        # grids of this footprint whose ISE length matches the block
-        wx = wy = None
-        for cand_wx in range(2, bx + 1):
-            for cand_wy in range(2, by + 1):
-                pass
-        # decode the mode bits directly (ASTC 2D block mode layout)
-        m = mode
-        A_ = (m >> 5) & 3
-        if m & 3:
-            B = (m >> 7) & 3
-            k = (m >> 2) & 3
-            wx, wy = [(B + 4, A_ + 2), (B + 8, A_ + 2), (A_ + 2, B + 8),
-                      ((B & 1) + 2, A_ + 2) if (m & 0x100) else (A_ + 2, (B & 1) + 6)][k]
-        else:
-            B = (m >> 9) & 3
-            k = (m >> 7) & 3
-            wx, wy = [(12, A_ + 2), (A_ + 2, 12), (A_ + 6, B + 6),
-                      (6, 10) if ((m >> 5) & 3) == 0 else (10, 6)][k]
+        wx, wy, _ = _mode_grid_and_quant(mode)
         grid = _grid(bx, by, wx, wy)
         qw = list(back.quantized_weights)
         by_i, bx_i = divmod(i, bxn)

@@ -48,3 +48,30 @@ def test_requested_chunk_that_does_not_fit_fails_cleanly():
     finally:
         del hold
         torch.cuda.empty_cache()
+
+
+def test_learned_policy_chunk_halves_with_its_plan_buffers():
+    """The LEARNED policy's plan buffers and inference workspace count in the
+    chunk budget: with device memory short, configure shrinks the chunk
+    (plan included) instead of failing, and the blocks equal those of the
+    unconstrained learned encode."""
+    import torch
+    img = synth.gradient_image(4096, 2048, 5)
+    enc = A.Encoder(0)
+    enc.load_predictor(4, 4)
+    enc.configure(4096, 2048, 4, 4, policy=A.POLICY_LEARNED)
+    ref = enc.encode(img)
+    enc.close()
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    hold = torch.empty(free - (5 << 30), dtype=torch.uint8, device="cuda")  # leave 5 GiB
+    try:
+        enc = A.Encoder(0)
+        enc.load_predictor(4, 4)
+        enc.configure(4096, 2048, 4, 4, policy=A.POLICY_LEARNED)
+        out = enc.encode(img)
+        enc.close()
+    finally:
+        del hold
+        torch.cuda.empty_cache()
+    assert np.array_equal(out, ref)

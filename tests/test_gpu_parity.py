@@ -75,7 +75,8 @@ def _throughput_image(alpha):
     return synth.alpha_image(4096, 4096, 2) if alpha else synth.gradient_image(4096, 4096, 1)
 
 
-@pytest.mark.parametrize("bx,by,alpha", [(4, 4, False), (6, 6, False), (8, 8, False), (4, 4, True)])
+@pytest.mark.parametrize("bx,by,alpha", [(4, 4, False), (6, 6, False), (8, 8, False), (4, 4, True), (6, 6, True),
+                                           (8, 8, True)])
 def test_full_size_sampled_batches(encoder, bx, by, alpha):
     """4096x4096 at full size: whole reference batches of 920 blocks (first,
     middle and the partial last one) equal the oracle's encoding of those
