@@ -132,6 +132,7 @@ def library():
         "astcg_configure": (i32, [vp, u32, u32, u32, u32, vp]),
         "astcg_encode": (i32, [vp, vp, sz, vp, sz]),
         "astcg_encode_device": (i32, [vp, vp, vp, vp]),
+        "astcg_encode_device_noalloc": (i32, [vp, vp, vp, vp]),
         "astcg_encode_batch": (i32, [vp, vp, u32, vp]),
         "astcg_decode_device": (i32, [vp, u32, u32, u32, u32, vp, vp, vp]),
         "astcg_get_labels": (i32, [vp, vp, sz]),
@@ -228,6 +229,12 @@ class Encoder:
         """Enqueues an encode of device buffers (integer addresses) on a HIP stream."""
         _check(library().astcg_encode_device(self._h, ctypes.c_void_p(d_rgba), ctypes.c_void_p(d_out),
                                              ctypes.c_void_p(stream or 0)), self._h)
+
+    def encode_device_noalloc(self, d_rgba, d_out, stream=None):
+        """Same encode as encode_device with no device allocation: the working
+        memory is the library's fixed per-GPU scratch array."""
+        _check(library().astcg_encode_device_noalloc(self._h, ctypes.c_void_p(d_rgba), ctypes.c_void_p(d_out),
+                                                     ctypes.c_void_p(stream or 0)), self._h)
 
     def encode_batch(self, images, footprints, max_partitions=4, chunk_blocks=0, profile=False,
                      policy=POLICY_COMPAT, void_extent=False, dual_plane=False):

@@ -15,6 +15,7 @@
 #include <map>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,20 @@
 #include "astcg_tables.h"
 
 using namespace astcg;
+
+// Working memory of astcg_encode_device_noalloc: a fixed device array (each
+// GPU the library runs on holds its own copy until the process ends).  Host
+// code never passes `scratch` itself to a kernel or a copy: its name is only
+// a host-side stand-in; the device address comes from hipGetSymbolAddress,
+// taken once per device in astcg_create and kept in g_scratch.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};  // device address of `scratch` per device
+}  // namespace
 
 // Device tables of one footprint.
 struct FpState {
@@ -111,6 +126,7 @@ struct astcg_encoder {
     void* d_pcseed = nullptr;  // label buffers (LabelBufs) with ASTCG_FLAG_LABELS
     size_t pcseed_cap = 0;
     struct EventProfiler* prof = nullptr;
+    void* scratch = nullptr;  // this device's static scratch (astcg_encode_device_noalloc)
     // ASTCG_FLAG_GRAPH: instantiated launch chains, keyed by everything the
     // chain's launches depend on (most recently used last)
     std::vector<std::pair<std::vector<uint8_t>, hipGraphExec_t>> graphs;
@@ -450,21 +466,10 @@ int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested, const P
     return (int)std::max<size_t>(1, std::min<size_t>(chunk, nblocks));
 }
 
-// Lays the chunk buffers out for footprint F and `chunk` blocks, growing the
-// allocation if needed; the per-mode counters are cleared whenever the
-// layout changes (k_bins clears them after every use otherwise).
-astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStream_t s) {
-    if (e->layout_fp == &F && e->layout_chunk == chunk) return ASTCG_OK;
-    size_t offs[12];
-    const size_t total = chunk_bytes_per_block(*F.fp, offs, chunk);
-    hipError_t err = grow(&e->d_chunk, &e->chunk_cap, total);
-    if (err != hipSuccess) {
-        e->layout_fp = nullptr;  // the old buffers are gone
-        e->layout_chunk = 0;
-        return hip_fail(e, err, "chunk buffers");
-    }
-    uint8_t* b = static_cast<uint8_t*>(e->d_chunk);
-    ChunkBuffers& B = e->B;
+// The chunk buffers of footprint F at `base`, with the offsets of
+// chunk_bytes_per_block.
+ChunkBuffers chunk_buffers_at(uint8_t* b, const FpState& F, const size_t* offs) {
+    ChunkBuffers B;
     B.pb_part = reinterpret_cast<int16_t*>(b + offs[0]);
     B.ideal = reinterpret_cast<Ideal*>(b + offs[1]);
     B.wts = reinterpret_cast<float*>(b + offs[2]);
@@ -480,6 +485,24 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
     B.work_total = B.bin_cur + F.fp->nbm;
     B.work = reinterpret_cast<int*>(b + offs[9]);
     B.wstride = F.wstride;
+    return B;
+}
+
+// Lays the chunk buffers out for footprint F and `chunk` blocks, growing the
+// allocation if needed; the per-mode counters are cleared whenever the
+// layout changes (k_bins clears them after every use otherwise).
+astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStream_t s) {
+    if (e->layout_fp == &F && e->layout_chunk == chunk) return ASTCG_OK;
+    size_t offs[12];
+    const size_t total = chunk_bytes_per_block(*F.fp, offs, chunk);
+    hipError_t err = grow(&e->d_chunk, &e->chunk_cap, total);
+    if (err != hipSuccess) {
+        e->layout_fp = nullptr;  // the old buffers are gone
+        e->layout_chunk = 0;
+        return hip_fail(e, err, "chunk buffers");
+    }
+    ChunkBuffers& B = e->B;
+    B = chunk_buffers_at(static_cast<uint8_t*>(e->d_chunk), F, offs);
     err = hipMemsetAsync(B.bin_count, 0, sizeof(int) * (2 * (size_t)F.fp->nbm + 1), s);
     if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
     e->layout_fp = &F;
@@ -829,7 +852,18 @@ astcg_status astcg_create(int hip_device, astcg_encoder** out) {
     }
     upload_quant_tables(e->stream);
     err = hipStreamSynchronize(e->stream);
+    if (err == hipSuccess && hip_device < kMaxDevices) {
+        // the device address of this device's static scratch, taken once
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch[hip_device]) {
+            void* p = nullptr;
+            err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (err == hipSuccess) g_scratch[hip_device] = p;
+        }
+        e->scratch = g_scratch[hip_device];
+    }
     if (err != hipSuccess) {
+        (void)hipGetLastError();
         (void)hipStreamDestroy(e->stream);
         delete e;
         return ASTCG_ERR_DEVICE;
@@ -932,6 +966,76 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
     }
     return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, e->params.flags, s,
                       labels, pred);
+}
+
+astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba, uint8_t* d_out, void* stream) {
+    if (!e || !d_rgba || !d_out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "encode before configure");
+    if (!e->scratch) return fail(e, ASTCG_ERR_DEVICE, "no static scratch on this device");
+    if (e->params.policy != ASTCG_POLICY_COMPAT)
+        return fail(e, ASTCG_ERR_UNSUPPORTED, "the no-allocation encode runs the COMPAT policy only");
+    if (e->params.flags & (ASTCG_FLAG_PROFILE | ASTCG_FLAG_LABELS | ASTCG_FLAG_GRAPH))
+        return fail(e, ASTCG_ERR_UNSUPPORTED,
+                    "the no-allocation encode takes no PROFILE, LABELS or GRAPH flag (they need allocations)");
+    const FpState& F = *e->cur;
+    const Footprint& f = *F.fp;
+    // chunk: as many blocks as fit in the scratch with their SymBest records
+    size_t offs[12];
+    const size_t fixed = chunk_bytes_per_block(f, offs, 0);
+    const size_t per_block = ((chunk_bytes_per_block(f, offs, 1 << 16) - fixed) >> 16) + sizeof(SymBest);
+    // exact fit: the sub-arrays' 256-byte alignment and the SymBest run
+    // after them included
+    auto need = [&](size_t n) {
+        return ((chunk_bytes_per_block(f, offs, (int)n) + 255) & ~size_t(255)) + n * sizeof(SymBest);
+    };
+    size_t fit = kStaticScratchBytes > fixed ? (kStaticScratchBytes - fixed) / per_block + 1 : 0;
+    while (fit > 0 && need(fit) > kStaticScratchBytes) --fit;
+    if (e->params.chunk_blocks > fit)
+        return fail(e, ASTCG_ERR_INVALID_ARGUMENT,
+                    "chunk_blocks " + std::to_string(e->params.chunk_blocks) + " exceeds what the " +
+                        std::to_string(kStaticScratchBytes >> 20) + " MiB static scratch holds at this footprint (" +
+                        std::to_string(fit) + " blocks)");
+    const int chunk = (int)std::min<size_t>(e->params.chunk_blocks ? e->params.chunk_blocks : fit,
+                                            std::min<size_t>(e->nblocks, (size_t)chunk_size(f, e->nblocks, 0, nullptr)));
+    if (chunk < 1) return fail(e, ASTCG_ERR_UNSUPPORTED, "footprint working set exceeds the static scratch");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    uint8_t* base = static_cast<uint8_t*>(e->scratch);
+    if (need((size_t)chunk) > kStaticScratchBytes) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "scratch layout");
+    const size_t used = chunk_bytes_per_block(f, offs, chunk);
+    const ChunkBuffers B = chunk_buffers_at(base, F, offs);
+    SymBest* best = reinterpret_cast<SymBest*>(base + ((used + 255) & ~size_t(255)));
+    // the scratch keeps the previous call's contents: the per-mode counters
+    // must start at zero (the search clears them again after each use)
+    err = launch_zero_ints(B.bin_count, 2 * f.nbm + 1, s);
+    if (err != hipSuccess) return hip_fail(e, err, "scratch clear");
+    // each chunk is encoded as its own launch range: the texture's block
+    // numbering starts at -blk0, so the chunk's blocks 0..n-1 are the image's
+    // blocks blk0..blk0+n-1 (in-batch indices and output positions are the
+    // texture's), and the chunk's SymBest records are packed before the next
+    // chunk reuses them
+    for (uint32_t blk0 = 0; blk0 < e->nblocks; blk0 += (uint32_t)chunk) {
+        const int n = (int)std::min<uint32_t>((uint32_t)chunk, e->nblocks - blk0);
+        ImageArgs A{};
+        A.t0.rgba = d_rgba;
+        A.t0.out = d_out;
+        A.t0.width = (int)e->width;
+        A.t0.height = (int)e->height;
+        A.t0.blocks_x = (int)e->blocks_x;
+        A.t0.first_block = -(int)blk0;
+        A.t0.num_blocks = (int)e->nblocks;
+        A.ntex = 1;
+        A.num_blocks = n;
+        A.blk0 = 0;
+        A.nb = n;
+        err = launch_chunk(F.T, A, B, best, (int)e->params.max_partitions, s, nullptr, nullptr, nullptr,
+                           (e->params.flags & ASTCG_FLAG_DUAL_PLANE) != 0);
+        if (err != hipSuccess) return hip_fail(e, err, "search kernels");
+        err = launch_pack(F.T, A, best, s, nullptr, (e->params.flags & ASTCG_FLAG_VOID_EXTENT) != 0);
+        if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
+    }
+    return ASTCG_OK;
 }
 
 astcg_status astcg_encode(astcg_encoder* e, const uint8_t* rgba, size_t rgba_len, uint8_t* out, size_t out_len) {

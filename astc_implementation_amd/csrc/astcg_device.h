@@ -221,6 +221,8 @@ hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, i
 // Zeroes the fallback counter of the learned policy (a kernel, so that it
 // can be part of a captured graph).
 hipError_t launch_fb_reset(unsigned long long* fb_count, hipStream_t s);
+// Zeroes n ints with a kernel (graph-capturable, unlike a memset node).
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 
 hipError_t launch_pack(const DevTables& T, const ImageArgs& A, const SymBest* best, hipStream_t s, Profiler* prof,
                        bool void_extent);

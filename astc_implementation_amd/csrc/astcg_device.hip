@@ -2769,6 +2769,12 @@ __global__ void __launch_bounds__(64) k_plan_zero(int* counts) {
     if (threadIdx.x < 3) counts[threadIdx.x] = 0;
 }
 
+// Zeroes n ints (a kernel rather than a memset, so that it can sit in a
+// captured graph).
+__global__ void __launch_bounds__(256) k_zero_ints(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 __global__ void __launch_bounds__(64) k_fb_reset(unsigned long long* fb_count) {
     if (threadIdx.x == 0) *fb_count = 0ull;
 }
@@ -3383,6 +3389,12 @@ hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, i
     const int nblocks = blocks_x * ((height + T.by - 1) / T.by);
     hipLaunchKernelGGL(k_decode, dim3((nblocks + 63) / 64), dim3(64), 0, s, T, blocks, width, height, blocks_x, nblocks,
                        rgba);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
+    (void)hipGetLastError();  // the final check reports this call's launches only
+    if (n > 0) hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)((n + 255) / 256 < 64 ? (n + 255) / 256 : 64)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

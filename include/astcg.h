@@ -157,6 +157,20 @@ astcg_status astcg_encode(astcg_encoder* enc, const uint8_t* rgba, size_t rgba_l
  * keeps its scratch buffers until the next configure/destroy. */
 astcg_status astcg_encode_device(astcg_encoder* enc, const uint8_t* d_rgba, uint8_t* d_out, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they encode (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same encode as
+ * astcg_encode_device (identical blocks) that calls no hipMalloc / hipFree
+ * or allocating library routine.  Its working memory is one fixed 64 MiB
+ * device array per GPU, held until the process ends; the image is searched
+ * in chunks that fit it.  astcg_configure (which allocates, once) must have
+ * been called; the COMPAT policy only, flags VOID_EXTENT and DUAL_PLANE
+ * allowed, PROFILE / LABELS / GRAPH rejected; an explicit chunk_blocks the
+ * array cannot hold is rejected with ASTCG_ERR_INVALID_ARGUMENT.  All calls
+ * on one GPU share the array and keep its last contents: calls that overlap
+ * on one GPU (other streams, other threads, other encoders) race. */
+astcg_status astcg_encode_device_noalloc(astcg_encoder* enc, const uint8_t* d_rgba, uint8_t* d_out, void* stream);
+
 /* One texture of a batch.  `rgba` is row-major RGBA8 (width*height*4 bytes),
  * `out` receives ceil(width/block_x)*ceil(height/block_y) 16-byte blocks in
  * row-major block order; out_len is checked against that size. */
