@@ -1244,3 +1244,403 @@ int oracle_encode(const uint8_t* rgba, int width, int height, int bx, int by, co
     if (stats) *stats = st;
     return 0;
 }
+
+/* ====================================================================== */
+/* Stage-level exports for the oracle's own unit tests (tests/           */
+/* test_oracle_pins.py): one pass at a time on explicit inputs, so that   */
+/* each pass can be checked against closed forms, float64 recomputations */
+/* and brute force.  Texel values are in the search's scale (u8/255 *     */
+/* 65536); tp gives each texel's partition.                               */
+/* ====================================================================== */
+
+static int dbg_block(const or_desc* d, const float* px, const uint8_t* tp, int pc, or_block* b) {
+    memset(b, 0, sizeof(*b));
+    float amin = 1e38f, amax = 0.0f;
+    int gray = 1;
+    for (int i = 0; i < d->texel_count; i++) {
+        for (int c = 0; c < 4; c++) b->px[i][c] = px[i * 4 + c];
+        b->tp[i] = tp ? tp[i] : 0;
+        if (b->tp[i] >= pc) return -1;
+        b->ppc[b->tp[i]]++;
+        amax = fmaxf(amax, b->px[i][3]);
+        amin = fminf(amin, b->px[i][3]);
+        gray = gray && (b->px[i][0] == b->px[i][1] && b->px[i][0] == b->px[i][2]);
+    }
+    b->grayscale = gray;
+    b->constant_alpha = amax == amin;
+    return 0;
+}
+
+static const or_dm* dbg_dm(const or_desc* d, int m) {
+    if (m < 0 || m >= d->ndm) return NULL;
+    return &d->dm[d->valid_dm[m]];
+}
+
+/* pass01: out = avg[16], dir[16], ep0[16], ep1[16], weights[tc], wes[tc],
+ * is_const_wes, min_weight_cutoff.  Returns the texel count. */
+int oracle_debug_pass01(int bx, int by, const float* px, const uint8_t* tp, int pc, float* out) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || dbg_block(d, px, tp, pc, &b)) return -1;
+    or_ideal o;
+    pass01(d, &b, &o);
+    int tc = d->texel_count, n = 0;
+    for (int p = 0; p < 4; p++) for (int c = 0; c < 4; c++) out[n++] = o.part[p].avg[c];
+    for (int p = 0; p < 4; p++) for (int c = 0; c < 4; c++) out[n++] = o.part[p].dir[c];
+    for (int p = 0; p < 4; p++) for (int c = 0; c < 4; c++) out[n++] = o.part[p].ep0[c];
+    for (int p = 0; p < 4; p++) for (int c = 0; c < 4; c++) out[n++] = o.part[p].ep1[c];
+    for (int i = 0; i < tc; i++) out[n++] = o.weights[i];
+    for (int i = 0; i < tc; i++) out[n++] = o.wes[i];
+    out[n++] = (float)o.is_const_wes;
+    out[n++] = o.min_weight_cutoff;
+    return tc;
+}
+
+static void dbg_ideal(const or_desc* d, const float* weights, const float* wes, int is_const, or_ideal* o) {
+    memset(o, 0, sizeof(*o));
+    for (int i = 0; i < d->texel_count; i++) { o->weights[i] = weights[i]; o->wes[i] = wes[i]; }
+    o->is_const_wes = is_const;
+}
+
+/* pass02 for trial decimation mode m: decimated weights; returns the weight count. */
+int oracle_debug_pass02(int bx, int by, int m, const float* weights, const float* wes, int is_const, float* dw) {
+    or_desc* d = or_desc_get(bx, by);
+    const or_dm* di = d ? dbg_dm(d, m) : NULL;
+    if (!di) return -1;
+    or_ideal o;
+    dbg_ideal(d, weights, wes, is_const, &o);
+    pass02(d, di, &o, dw);
+    return di->weight_count;
+}
+
+/* pass03: 16 angular offsets of trial decimation mode m. */
+int oracle_debug_pass03(int bx, int by, int m, const float* dw, float* ao) {
+    or_desc* d = or_desc_get(bx, by);
+    const or_dm* di = d ? dbg_dm(d, m) : NULL;
+    if (!di) return -1;
+    pass03(d, di, dw, ao);
+    return 16;
+}
+
+/* pass04: per angular step {lowest, span, error, cut_low, cut_high};
+ * returns max_angular_steps (its max_quant_steps in info[0], its
+ * max_quant_level in info[1]). */
+int oracle_debug_pass04(int bx, int by, int m, const float* dw, const float* ao, float* hl5, int* info) {
+    or_desc* d = or_desc_get(bx, by);
+    const or_dm* di = d ? dbg_dm(d, m) : NULL;
+    if (!di) return -1;
+    or_hl hl[16];
+    pass04(di, dw, ao, hl);
+    for (int s = 0; s < di->max_angular_steps; s++) {
+        hl5[s * 5 + 0] = hl[s].lowest;
+        hl5[s * 5 + 1] = (float)hl[s].span;
+        hl5[s * 5 + 2] = hl[s].error;
+        hl5[s * 5 + 3] = hl[s].cut_low;
+        hl5[s * 5 + 4] = hl[s].cut_high;
+    }
+    if (info) { info[0] = di->max_quant_steps; info[1] = di->max_quant_level; }
+    return di->max_angular_steps;
+}
+
+/* pass05: low/high weight values per quant level 0..max_quant_level from
+ * pass04's records (same layout as oracle_debug_pass04). */
+int oracle_debug_pass05(int bx, int by, int m, const float* hl5, const float* ao, float* low, float* high) {
+    or_desc* d = or_desc_get(bx, by);
+    const or_dm* di = d ? dbg_dm(d, m) : NULL;
+    if (!di) return -1;
+    or_hl hl[16];
+    for (int s = 0; s < di->max_angular_steps; s++) {
+        hl[s].lowest = hl5[s * 5 + 0];
+        hl[s].span = (int)hl5[s * 5 + 1];
+        hl[s].error = hl5[s * 5 + 2];
+        hl[s].cut_low = hl5[s * 5 + 3];
+        hl[s].cut_high = hl5[s * 5 + 4];
+    }
+    pass05(di, hl, ao, low, high);
+    return di->max_quant_level;
+}
+
+/* Trial block mode k: {decimation trial index, weight quant, weight bits,
+ * weight count, raw mode}. */
+int oracle_debug_trial_mode(int bx, int by, int k, int* info) {
+    or_desc* d = or_desc_get(bx, by);
+    if (!d || k < 0 || k >= d->nbm) return -1;
+    const or_bm* bm = &d->bm[d->valid_bm[k]];
+    info[0] = d->valid_bm_dml[k];
+    info[1] = bm->quant_mode;
+    info[2] = bm->weight_bits;
+    info[3] = d->dm[bm->decimation_mode].weight_count;
+    info[4] = bm->mode_index;
+    return d->nbm;
+}
+
+/* pass07 for trial block mode k (dw: its decimation mode's weights):
+ * error, remaining bit count, quantised weights (0..64, whole groups of four
+ * kept).  Returns the weight count. */
+int oracle_debug_pass07(int bx, int by, int k, int pc, const float* weights, const float* wes, int is_const,
+                        float cutoff, const float* dw, float low, float high, float* error, int* bitcount,
+                        uint8_t* qw) {
+    or_desc* d = or_desc_get(bx, by);
+    if (!d || k < 0 || k >= d->nbm || pc < 1 || pc > 4) return -1;
+    const or_bm* bm = &d->bm[d->valid_bm[k]];
+    or_ideal o;
+    dbg_ideal(d, weights, wes, is_const, &o);
+    o.min_weight_cutoff = cutoff;
+    or_qres r;
+    pass07(d, bm, &d->dm[bm->decimation_mode], &o, dw, low, high, pc, &r);
+    *error = r.error;
+    *bitcount = r.bitcount;
+    memcpy(qw, r.qw, OR_MAX_WEIGHTS);
+    return d->dm[bm->decimation_mode].weight_count;
+}
+
+/* pass08 (after pass01 on the same block): per partition {rgb_scale_error,
+ * rgb_luma_error, luminance_error, alpha_drop_error, can_offset,
+ * can_blue_contract}. */
+int oracle_debug_pass08(int bx, int by, const float* px, const uint8_t* tp, int pc, float* out) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || dbg_block(d, px, tp, pc, &b)) return -1;
+    or_ideal o;
+    pass01(d, &b, &o);
+    or_ece e[4];
+    pass08(d, &b, &o, pc, e);
+    for (int p = 0; p < pc; p++) {
+        out[p * 6 + 0] = e[p].rgb_scale_error;
+        out[p * 6 + 1] = e[p].rgb_luma_error;
+        out[p * 6 + 2] = e[p].luminance_error;
+        out[p * 6 + 3] = e[p].alpha_drop_error;
+        out[p * 6 + 4] = (float)e[p].can_offset;
+        out[p * 6 + 5] = (float)e[p].can_blue_contract;
+    }
+    return pc;
+}
+
+/* pass09 from explicit inputs: endpoints ep0/ep1 [4][4], texel counts per
+ * partition, pass08 records (6 floats per partition as above).  Out:
+ * err[4][21][4], fmt[4][21][4]. */
+int oracle_debug_pass09(int pc, const float* ep0, const float* ep1, const int* counts, const float* ece6, float* err,
+                        int* fmt) {
+    if (pc < 1 || pc > 4) return -1;
+    or_block b;
+    memset(&b, 0, sizeof(b));
+    or_ideal o;
+    memset(&o, 0, sizeof(o));
+    or_ece e[4];
+    for (int p = 0; p < pc; p++) {
+        b.ppc[p] = counts[p];
+        for (int c = 0; c < 4; c++) { o.part[p].ep0[c] = ep0[p * 4 + c]; o.part[p].ep1[c] = ep1[p * 4 + c]; }
+        e[p].rgb_scale_error = ece6[p * 6 + 0];
+        e[p].rgb_luma_error = ece6[p * 6 + 1];
+        e[p].luminance_error = ece6[p * 6 + 2];
+        e[p].alpha_drop_error = ece6[p * 6 + 3];
+        e[p].can_offset = ece6[p * 6 + 4] != 0.0f;
+        e[p].can_blue_contract = ece6[p * 6 + 5] != 0.0f;
+    }
+    float cerr[4][21][4];
+    int cfmt[4][21][4];
+    memset(cerr, 0, sizeof(cerr));
+    memset(cfmt, 0, sizeof(cfmt));
+    pass09(&b, &o, e, pc, cerr, cfmt);
+    memcpy(err, cerr, sizeof(cerr));
+    memcpy(fmt, cfmt, sizeof(cfmt));
+    return pc;
+}
+
+/* pass10 from pass09's tables: comb_err[21][13], comb_fmt[21][13][4]. */
+int oracle_debug_pass10(int pc, const float* err, const int* fmt, float* comb_err, int* comb_fmt) {
+    if (pc < 2 || pc > 4) return -1;
+    float cerr[4][21][4];
+    int cfmt[4][21][4];
+    memcpy(cerr, err, sizeof(cerr));
+    memcpy(cfmt, fmt, sizeof(cfmt));
+    static __thread or_comb comb[21][13];
+    pass10(pc, cerr, cfmt, comb);
+    for (int q = 0; q < 21; q++)
+        for (int k = 0; k < 13; k++) {
+            comb_err[q * 13 + k] = comb[q][k].error;
+            for (int p = 0; p < 4; p++) comb_fmt[(q * 13 + k) * 4 + p] = comb[q][k].formats[p];
+        }
+    return 0;
+}
+
+/* pass11 from explicit inputs: weight error and remaining bits of a block
+ * mode, pass09 tables (1 partition) or pass10 tables (2..4).  Out: total
+ * error, colour quant level, its modified level, formats[4]. */
+int oracle_debug_pass11(int pc, float weight_error, int bitcount, const float* err, const int* fmt,
+                        const float* comb_err, const int* comb_fmt, float* total, int* ql, int* ql_mod, int* formats) {
+    if (pc < 1 || pc > 4) return -1;
+    float cerr[4][21][4];
+    int cfmt[4][21][4];
+    memcpy(cerr, err, sizeof(cerr));
+    memcpy(cfmt, fmt, sizeof(cfmt));
+    static __thread or_comb comb[21][13];
+    for (int q = 0; q < 21; q++)
+        for (int k = 0; k < 13; k++) {
+            comb[q][k].error = comb_err ? comb_err[q * 13 + k] : OR_ERR_DEFAULT;
+            for (int p = 0; p < 4; p++) comb[q][k].formats[p] = comb_fmt ? comb_fmt[(q * 13 + k) * 4 + p] : 0;
+        }
+    or_qres qr;
+    memset(&qr, 0, sizeof(qr));
+    qr.error = weight_error;
+    qr.bitcount = bitcount;
+    or_ccr c;
+    memset(&c, 0, sizeof(c));
+    pass11(pc, &qr, cerr, cfmt, comb, &c);
+    *total = c.total_error;
+    *ql = c.ql;
+    *ql_mod = c.ql_mod;
+    for (int p = 0; p < 4; p++) formats[p] = c.formats[p];
+    return 0;
+}
+
+/* pass12: indices of the 8 best of n errors (insertion, strict <), and
+ * their errors. */
+int oracle_debug_pass12(const float* errors, int n, int* idx, float* top) {
+    float te[8];
+    int ti[8];
+    for (int i = 0; i < 8; i++) { te[i] = OR_ERR_DEFAULT; ti[i] = 0; }
+    for (int k = 0; k < n; k++) {
+        float e = errors[k];
+        if (e < te[7]) {
+            te[7] = e; ti[7] = k;
+            for (int j = 7; j > 0; j--)
+                if (te[j] < te[j - 1]) {
+                    float fe = te[j]; te[j] = te[j - 1]; te[j - 1] = fe;
+                    int fi = ti[j]; ti[j] = ti[j - 1]; ti[j - 1] = fi;
+                }
+        }
+    }
+    for (int i = 0; i < 8; i++) { idx[i] = ti[i]; top[i] = te[i]; }
+    return 0;
+}
+
+/* pass13 for trial block mode k of a block with quantised weights qw
+ * (0..64): endpoints in/out ep[2][4][4] (ep0 then ep1), averages avg[4][4]
+ * (from pass01), rgbs[4][4] out. */
+int oracle_debug_pass13(int bx, int by, int k, const float* px, const uint8_t* tp, int pc, const float* avg,
+                        float* ep, const uint8_t* qw, float* rgbs) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || k < 0 || k >= d->nbm || dbg_block(d, px, tp, pc, &b)) return -1;
+    or_cand c;
+    memset(&c, 0, sizeof(c));
+    c.bm_packed = d->valid_bm[k];
+    for (int p = 0; p < 4; p++)
+        for (int i = 0; i < 4; i++) {
+            c.part[p].avg[i] = avg[p * 4 + i];
+            c.part[p].ep0[i] = ep[p * 4 + i];
+            c.part[p].ep1[i] = ep[16 + p * 4 + i];
+        }
+    for (int i = 0; i < OR_MAX_WEIGHTS; i++) c.qw[i] = qw[i];
+    float r[4][4];
+    memset(r, 0, sizeof(r));
+    pass13(d, &b, pc, &c, r);
+    for (int p = 0; p < 4; p++)
+        for (int i = 0; i < 4; i++) {
+            ep[p * 4 + i] = c.part[p].ep0[i];
+            ep[16 + p * 4 + i] = c.part[p].ep1[i];
+            rgbs[p * 4 + i] = r[p][i];
+        }
+    return 0;
+}
+
+/* pass14's per-partition endpoint packer (pack_color_endpoints): packed
+ * values (0..255 unquantised), returns the format it chose. */
+int oracle_debug_pack_endpoints(const float* ep0, const float* ep1, const float* rgbs, int format, int q,
+                                uint32_t* packed) {
+    return or_pack_color_endpoints(ep0, ep1, rgbs, format, q, packed);
+}
+
+/* pass15's unpacker: endpoints 0..255 (before the x257 of pass15). */
+int oracle_debug_unpack_endpoints(int format, const uint32_t* packed, int* e0, int* e1) {
+    or_unpack_color_endpoints(format, packed, e0, e1);
+    return 0;
+}
+
+/* pass16 for trial block mode k: qw (0..64) realigned in place against the
+ * unpacked endpoints ue0/ue1 [4][4] (x257 scale). */
+int oracle_debug_pass16(int bx, int by, int k, const float* px, const uint8_t* tp, int pc, const int* ue0,
+                        const int* ue1, uint8_t* qw) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || k < 0 || k >= d->nbm || dbg_block(d, px, tp, pc, &b)) return -1;
+    or_cand c;
+    memset(&c, 0, sizeof(c));
+    c.bm_packed = d->valid_bm[k];
+    for (int i = 0; i < OR_MAX_WEIGHTS; i++) c.qw[i] = qw[i];
+    int u0[4][4], u1[4][4];
+    memcpy(u0, ue0, sizeof(u0));
+    memcpy(u1, ue1, sizeof(u1));
+    pass16(d, &b, pc, &c, u0, u1);
+    for (int i = 0; i < OR_MAX_WEIGHTS; i++) qw[i] = (uint8_t)c.qw[i];
+    return 0;
+}
+
+/* pass17 for trial block mode k: the block error of weights qw and
+ * endpoints ue0/ue1 (x257 scale). */
+float oracle_debug_pass17(int bx, int by, int k, const float* px, const uint8_t* tp, int pc, const int* ue0,
+                          const int* ue1, const uint8_t* qw) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || k < 0 || k >= d->nbm || dbg_block(d, px, tp, pc, &b)) return -1.0f;
+    or_cand c;
+    memset(&c, 0, sizeof(c));
+    c.bm_packed = d->valid_bm[k];
+    for (int i = 0; i < OR_MAX_WEIGHTS; i++) c.qw[i] = qw[i];
+    int u0[4][4], u1[4][4];
+    memcpy(u0, ue0, sizeof(u0));
+    memcpy(u1, ue1, sizeof(u1));
+    return pass17(d, &b, pc, &c, u0, u1);
+}
+
+/* Full per-partitioned-block search (passes 01-18) on an explicit block:
+ * the chosen symbolic block.  pb_in_batch places it in the reference's
+ * batch (pass05's out-of-bounds emulation depends on it). */
+int oracle_debug_search_pb(int bx, int by, const float* px, const uint8_t* tp, int pc, int pb_in_batch,
+                           or_symbolic* out) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || dbg_block(d, px, tp, pc, &b)) return -1;
+    search_pb(d, &b, pc, pb_in_batch, out, NULL);
+    return 0;
+}
+
+/* passes 001-003: k-means assignment of the block's texels to pc clusters. */
+int oracle_debug_kmeans(int bx, int by, const float* px, int pc, uint8_t* assign) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || pc < 2 || pc > 4 || dbg_block(d, px, NULL, 1, &b)) return -1;
+    kmeans(d, &b, pc, assign);
+    return d->texel_count;
+}
+
+/* pass004: mismatch of an assignment against packed partitioning pidx. */
+int oracle_debug_mismatch(int bx, int by, const uint8_t* assign, int pc, int pidx) {
+    or_desc* d = or_desc_get(bx, by);
+    if (!d || pc < 2 || pc > 4 || pidx < 0 || pidx >= OR_PARTITIONINGS) return -1;
+    return (int)mismatch(d, assign, pc, pidx);
+}
+
+/* pass005: the first 128 entries of the partial bitonic ordering when only
+ * entry 0 of the nsel selected partitionings has a non-zero score. */
+int oracle_debug_ordering(int nsel, uint32_t score0, uint32_t* ord128) {
+    ordering(nsel, score0, ord128);
+    return 128;
+}
+
+/* pass006 for packed partitioning `entry` of count pc: {uncorrelated,
+ * same-chroma} line-fit errors. */
+int oracle_debug_eval_candidate(int bx, int by, const float* px, int pc, int entry, float* eu, float* es) {
+    or_desc* d = or_desc_get(bx, by);
+    or_block b;
+    if (!d || pc < 2 || pc > 4 || entry < 0 || entry >= OR_PARTITIONINGS || dbg_block(d, px, NULL, 1, &b)) return -1;
+    eval_candidate(d, &b, &d->part[(pc - 2) * OR_PARTITIONINGS + entry], pc, eu, es);
+    return 0;
+}
+
+/* pass007: the 4 positions chosen from 128 {uncorrelated, same-chroma} errors. */
+int oracle_debug_select_positions(const float* eu, const float* es, int* pos) {
+    select_positions(eu, es, pos);
+    return 0;
+}
