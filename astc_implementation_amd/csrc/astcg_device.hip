@@ -2574,6 +2574,58 @@ constexpr int kDualMaxTc = 64;
 
 size_t dual_lds_bytes(const DevTables& T) { return (size_t)2 * T.tc * 64 * sizeof(float) + 2 * 32 * 64; }
 
+// Weight of texel t as the ASTC decoder infills it: (sum of grid weight x
+// contribution in sixteenths + 8) >> 4 (contributions are multiples of 1/16).
+__device__ __forceinline__ int infill_int(const DevTables& T, int dm, const uint8_t* qw, int stride, int t) {
+    const int tc = T.tc;
+    if (T.dm[dm].weight_count == tc) return qw[t * stride];
+    const int n = T.t2w_count[(size_t)dm * tc + t];
+    int s = 0;
+    for (int j = 0; j < n; ++j)
+        s += qw[T.t2w_idx[((size_t)dm * tc + t) * 4 + j] * stride] *
+             (int)(T.t2w_w[((size_t)dm * tc + t) * 4 + j] * 16.0f + 0.5f);
+    return (s + 8) >> 4;
+}
+
+// 8-bit value of a decoded texel channel: the 16-bit interpolation of the
+// x257 endpoints, rounded to 8 bits.
+__device__ __forceinline__ int decoded_u8(int e0, int e1, int w) {
+    const int v16 = (e0 * 257 * (64 - w) + e1 * 257 * w + 32) >> 6;
+    return (v16 * 255 + 32767) / 65535;
+}
+
+// Channel-weighted squared error, in 8-bit units, of a single-plane symbolic
+// block as the decoder reconstructs it (integer weight infill, decoded
+// endpoints, 8-bit rounding) against the block's texels.
+__device__ float decoded_block_error(const BlockLoc& loc, const DevTables& T, const SymBest& sb) {
+    const int tc = T.tc, pc = sb.pc;
+    if (pc < 1 || sb.raw_mode >= 2048 || T.raw_to_bm[sb.raw_mode] < 0) return kErrDefault;
+    const int dm = T.bm[T.raw_to_bm[sb.raw_mode]].dm;
+    int e0[4][4], e1[4][4];
+    for (int p = 0; p < pc; ++p) {
+        uint32_t v[8];
+        for (int i = 0; i < 8; ++i) v[i] = sb.packed[p * 8 + i];
+        unpack_color_endpoints(fmt_of(sb.formats, p), v, e0[p], e1[p]);
+    }
+    const uint8_t* pot = nullptr;
+    if (pc > 1) {
+        const int e = T.seed_to_packed[(pc - 2) * kPartitionings + sb.seed];
+        pot = T.part_of_texel + ((size_t)(pc - 2) * kPartitionings + e) * tc;
+    }
+    float total = 0.0f;
+    for (int t = 0; t < tc; ++t) {
+        const int p = pot ? pot[t] : 0, w = infill_int(T, dm, sb.qw, 1, t);
+        const uint32_t v = texel_rgba8(loc, T, t);
+        float dd[4];
+        for (int j = 0; j < 4; ++j) {
+            const float df = (float)(decoded_u8(e0[p][j], e1[p][j], w) - (int)((v >> (8 * j)) & 0xFFu));
+            dd[j] = df * df;
+        }
+        total += dot4(dd, kCW);
+    }
+    return total;
+}
+
 __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* best) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x, k = blockIdx.x * 64 + lane, tc = T.tc;
@@ -2583,7 +2635,9 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
     const int g = A.blk0 + k;
     const BlockLoc loc = block_loc(A, T, g);
     SymBest& sb = best[g];
-    float best_err = sb.error;
+    // candidates compete with the searched block in what the decoder
+    // reconstructs (8-bit texels), so a replaced block never decodes worse
+    float best_err = decoded_block_error(loc, T, sb);
     float* w1s = s_w + lane;
     float* w2s = s_w + tc * 64 + lane;
     uint8_t* q1s = s_q + lane;
@@ -2676,14 +2730,11 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
                     qs[w * 64] = c_qt.quant_to_unquant[q][rank];
                 }
             }
-            // infilled weight (0..64) of texel t on a plane, as pass17 forms it
-            auto infill = [&](const uint8_t* qs, int t) -> int {
-                if (undecimated) return qs[t * 64];
-                float w = 0.0f;
-                const int n = t2wc[t];
-                for (int j = 0; j < n; ++j) w += ((float)qs[t2wi[t * 4 + j] * 64] / 64.0f) * t2ww[t * 4 + j];
-                return wf2i(wround(w * 64.0f));
-            };
+            // infilled weight (0..64) of texel t on a plane, as the decoder forms it
+            auto infill = [&](const uint8_t* qs, int t) -> int { return infill_int(T, md.dm, qs, 64, t); };
+            (void)t2wc;
+            (void)t2wi;
+            (void)t2ww;
             // least-squares endpoints per channel for the quantised weights
             float a00[2] = {0.0f, 0.0f}, a01[2] = {0.0f, 0.0f}, a11[2] = {0.0f, 0.0f};
             float b0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, b1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -2722,23 +2773,29 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
             const bool swap = eq1[0] + eq1[1] + eq1[2] < eq0[0] + eq0[1] + eq0[2];
             if (swap)
                 for (int j = 0; j < 4; ++j) { const int x = eq0[j]; eq0[j] = eq1[j]; eq1[j] = x; }
-            float total = 0.0f;
+            // decoded error (8-bit units) with an early exit, and the
+            // search's error (pass17 scale) of the same reconstruction
+            float total = 0.0f, total16 = 0.0f;
             for (int t = 0; t < tc && total < best_err; ++t) {
                 int wt[2] = {infill(q1s, t), infill(q2s, t)};
                 if (swap) { wt[0] = 64 - wt[0]; wt[1] = 64 - wt[1]; }
                 const uint32_t v = texel_rgba8(loc, T, t);
-                float dd[4];
+                float dd[4], dd16[4];
                 for (int j = 0; j < 4; ++j) {
                     const int wv = wt[j == c ? 1 : 0];
+                    const uint32_t x = (v >> (8 * j)) & 0xFFu;
                     const int col = (eq0[j] * 257 * (64 - wv) + eq1[j] * 257 * wv + 32) >> 6;
-                    const float df = wmin(fabsf(u8_search_value((v >> (8 * j)) & 0xFFu) - (float)col), 1e15f);
+                    const float df16 = wmin(fabsf(u8_search_value(x) - (float)col), 1e15f);
+                    dd16[j] = df16 * df16;
+                    const float df = (float)((col * 255 + 32767) / 65535 - (int)x);
                     dd[j] = df * df;
                 }
-                total += wmin(dot4(dd, kCW), kErrDefault);
+                total += dot4(dd, kCW);
+                total16 += wmin(dot4(dd16, kCW), kErrDefault);
             }
             if (!(total < best_err)) continue;
             best_err = total;
-            sb.error = total;
+            sb.error = total16;
             sb.raw_mode = md.raw_mode;
             sb.pc = 1;
             sb.seed = 0;

@@ -1,0 +1,102 @@
+"""The product's LDR decoder against an independent decoder written in the
+tests from the ASTC specification (tests/astc_spec.py: block-mode layout,
+ISE trit / quint procedures, weight and colour unquantisation, the
+partition hash, weight infill, endpoint modes, interpolation), texel for
+texel.  The decoded PSNR the benchmark reports, and every quality bound the
+tests assert, go through the product decoder, so this pins them to the
+specification.  It also pins the oracle's partition tables and quantisation
+tables at the level of individual texels and values."""
+import numpy as np
+import pytest
+
+import astc_implementation_amd as A
+from astc_implementation_amd import synth
+import astc_spec as S
+import oracle_lib as O
+
+ERR = np.array(S.ERROR_COLOUR, np.uint8)
+
+
+@pytest.mark.parametrize("fp", [(4, 4), (5, 5), (6, 6), (8, 8), (10, 6), (12, 12)])
+def test_partition_tables_are_the_spec_hash(fp):
+    bx, by = fp
+    small = bx * by < 31
+    for pc in (2, 3, 4):
+        for e in range(1024):
+            pot, meta = O.partition_entry(bx, by, pc, e)
+            seed = meta["seed"]
+            ref = [S.select_partition(seed, t % bx, t // bx, pc, small) for t in range(bx * by)]
+            assert list(pot) == ref, (fp, pc, e, seed)
+
+
+def test_unquantisation_tables_are_the_spec_procedures():
+    t = O.table(0)
+    for q in range(12):
+        b, tr, qn = S.ise_shape(q)
+        n = S.LEVELS[q]
+        if tr or qn:
+            vals = [S.weight_unquant(q, d, m) for d in range(3 if tr else 5) for m in range(1 << b)]
+        else:
+            vals = [S.weight_unquant(q, None, m) for m in range(n)]
+        off = sum(S.LEVELS[:q])
+        assert sorted(vals) == sorted(t[off:off + n]), q
+    for q in range(4, 21):
+        b, tr, qn = S.ise_shape(q)
+        if tr or qn:
+            vals = {S.color_unquant(q, d, m) for d in range(3 if tr else 5) for m in range(1 << b)}
+        else:
+            vals = {S.color_unquant(q, None, m) for m in range(S.LEVELS[q])}
+        assert vals == set(O.table(2)[(q - 4) * 512:(q - 3) * 512]), q
+
+
+@pytest.mark.parametrize("kind", ["gradient", "alpha", "normal", "flat"])
+def test_host_decoder_equals_spec_decoder_on_encoder_output(kind):
+    gen = {"gradient": synth.gradient_image, "alpha": synth.alpha_image, "normal": synth.normal_map_image,
+           "flat": synth.flat_noise_image}[kind]
+    img = gen(40, 28, 5)
+    for bx, by in A.FOOTPRINTS:
+        blocks, _, _ = O.encode(img, bx, by, 4, threads=8)
+        ours = A.decode_image(bx, by, 40, 28, blocks)
+        spec = S.decode_image(bx, by, 40, 28, blocks)
+        bad = np.argwhere((ours != spec).any(axis=-1))
+        assert bad.size == 0, ((bx, by), bad[:5].tolist())
+
+
+@pytest.mark.parametrize("fp", [(4, 4), (5, 4), (6, 6), (8, 8), (12, 12)])
+def test_host_decoder_equals_spec_decoder_on_random_blocks(fp):
+    """Random 128-bit strings: reserved modes, oversized grids, HDR endpoint
+    modes and data that does not fit decode to the error colour in both;
+    legal ones to the same texels."""
+    bx, by = fp
+    rng = np.random.default_rng(bx * 16 + by)
+    # encoder blocks with a few flipped bits: mostly legal, other modes,
+    # seeds, endpoint modes and quant levels than the encoder picks
+    seeds, _, _ = O.encode(synth.gradient_image(8 * bx, 4 * by, 3), bx, by, 4, threads=8)
+    seeds = seeds.reshape(-1, 16)
+    legal = 0
+    for i in range(800):
+        if i % 2:
+            blk = rng.integers(0, 256, 16).astype(np.uint8)
+        else:
+            blk = seeds[rng.integers(0, len(seeds))].copy()
+            for bit in rng.integers(0, 128, rng.integers(1, 9)):
+                blk[bit // 8] ^= np.uint8(1 << (bit % 8))
+        ours = A.decode_image(bx, by, bx, by, blk)
+        spec = np.array(S.decode_block(bx, by, bytes(blk)), np.uint8).reshape(by, bx, 4)
+        assert np.array_equal(ours, spec), blk.tobytes().hex()
+        legal += not (ours == ERR).all()
+    assert legal > 100
+
+
+def test_void_extent_blocks_decode_to_their_colour():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        col = rng.integers(0, 65536, 4)
+        blk = bytearray(16)
+        v = 0x1FC | (0x3 << 10) | (0x1FFF << 12) | (0x1FFF << 25) | (0x1FFF << 38) | (0x1FFF << 51)
+        v |= sum(int(c) << (64 + 16 * i) for i, c in enumerate(col))
+        blk[:] = v.to_bytes(16, "little")
+        ours = A.decode_image(6, 6, 6, 6, np.frombuffer(bytes(blk), np.uint8))
+        spec = np.array(S.decode_block(6, 6, bytes(blk)), np.uint8).reshape(6, 6, 4)
+        assert np.array_equal(ours, spec)
+        assert np.array_equal(ours[0, 0], np.rint(col * 255 / 65535).astype(np.uint8))
