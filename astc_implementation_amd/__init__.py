@@ -94,12 +94,13 @@ class LearnedParams(ctypes.Structure):
         ("full_search_above", ctypes.c_float),
         ("rounds", ctypes.c_uint32),
         ("partitioned_slots", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("candidates", ctypes.c_uint32),
     ]
 
 
 POLICY_COMPAT = 0
 POLICY_LEARNED = 1
+POLICY_EXHAUSTIVE = 2
 
 
 class KernelTime(ctypes.Structure):

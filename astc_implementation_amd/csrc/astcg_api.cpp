@@ -730,7 +730,8 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
 // LEARNED policy (texs: the textures of A, for the predictor's inputs).
 astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
                         uint32_t flags, hipStream_t s, const LabelBufs* labels = nullptr, const PredState* pred = nullptr,
-                        const std::vector<TexDesc>* texs = nullptr) {
+                        const std::vector<TexDesc>* texs = nullptr, bool exhaustive = false) {
+    if (pred && pred->lp.candidates == 1) exhaustive = true;
     int chunk = 0;
     astcg_status st = layout_fitting(e, F, (uint32_t)A.num_blocks, chunk_req, s, &chunk, pred);
     if (st != ASTCG_OK) return st;
@@ -757,7 +758,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
                               labels ? labels->best_seed : nullptr, labels ? labels->cand_seed : nullptr,
                               labels ? labels->cand_err : nullptr};
         put(ptrs, sizeof(ptrs));
-        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk};
+        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk, exhaustive ? 1 : 0};
         put(ints, sizeof(ints));
         put(&A, sizeof(A));
         if (pred) put(&pred->lp, sizeof(pred->lp));
@@ -783,7 +784,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
         if (pred) st = plan_chunk(e, F, *pred, A, texs, s, &plan);
         if (st != ASTCG_OK) break;
         err = launch_chunk(F.T, A, e->B, e->d_best, max_pc, s, e->prof, labels, pred ? &plan : nullptr,
-                           (flags & ASTCG_FLAG_DUAL_PLANE) != 0);
+                           (flags & ASTCG_FLAG_DUAL_PLANE) != 0, exhaustive);
         if (err != hipSuccess) st = hip_fail(e, err, "search kernels");
     }
     if (st == ASTCG_OK) {
@@ -818,7 +819,7 @@ astcg_status check_params(astcg_encoder* e, const astcg_params* params, astcg_pa
     if (params) *p = *params;
     if (p->max_partitions < 1 || p->max_partitions > 4)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "max_partitions must be 1..4");
-    if (p->policy != ASTCG_POLICY_COMPAT && p->policy != ASTCG_POLICY_LEARNED)
+    if (p->policy != ASTCG_POLICY_COMPAT && p->policy != ASTCG_POLICY_LEARNED && p->policy != ASTCG_POLICY_EXHAUSTIVE)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "unknown partition policy");
     if ((p->flags & ASTCG_FLAG_PROFILE) && !e->prof) e->prof = new EventProfiler();
     return ASTCG_OK;
@@ -965,15 +966,15 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
         if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
     }
     return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, e->params.flags, s,
-                      labels, pred);
+                      labels, pred, nullptr, e->params.policy == ASTCG_POLICY_EXHAUSTIVE);
 }
 
 astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba, uint8_t* d_out, void* stream) {
     if (!e || !d_rgba || !d_out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
     if (!e->configured) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "encode before configure");
     if (!e->scratch) return fail(e, ASTCG_ERR_DEVICE, "no static scratch on this device");
-    if (e->params.policy != ASTCG_POLICY_COMPAT)
-        return fail(e, ASTCG_ERR_UNSUPPORTED, "the no-allocation encode runs the COMPAT policy only");
+    if (e->params.policy == ASTCG_POLICY_LEARNED)
+        return fail(e, ASTCG_ERR_UNSUPPORTED, "the no-allocation encode runs the COMPAT and EXHAUSTIVE policies only");
     if (e->params.flags & (ASTCG_FLAG_PROFILE | ASTCG_FLAG_LABELS | ASTCG_FLAG_GRAPH))
         return fail(e, ASTCG_ERR_UNSUPPORTED,
                     "the no-allocation encode takes no PROFILE, LABELS or GRAPH flag (they need allocations)");
@@ -1030,7 +1031,7 @@ astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba
         A.blk0 = 0;
         A.nb = n;
         err = launch_chunk(F.T, A, B, best, (int)e->params.max_partitions, s, nullptr, nullptr, nullptr,
-                           (e->params.flags & ASTCG_FLAG_DUAL_PLANE) != 0);
+                           (e->params.flags & ASTCG_FLAG_DUAL_PLANE) != 0, e->params.policy == ASTCG_POLICY_EXHAUSTIVE);
         if (err != hipSuccess) return hip_fail(e, err, "search kernels");
         err = launch_pack(F.T, A, best, s, nullptr, (e->params.flags & ASTCG_FLAG_VOID_EXTENT) != 0);
         if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
@@ -1161,7 +1162,8 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
             if (it == e->preds.end()) return fail(e, ASTCG_ERR_NOT_CONFIGURED, "no predictor for a batch footprint");
             pred = &it->second;
         }
-        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, p.flags, s, nullptr, pred, &texs);
+        st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, p.flags, s, nullptr, pred, &texs,
+                        p.policy == ASTCG_POLICY_EXHAUSTIVE);
         if (st != ASTCG_OK) return st;
     }
     return ASTCG_OK;
@@ -1331,7 +1333,8 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     if (lp) q = *lp;
     if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
         q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f) || q.rounds > 6 ||
-        q.partitioned_slots > 8)
+        q.partitioned_slots > 8 || q.candidates > 1 || (q.candidates == 1 && (q.seeds != 0 || (q.partitionings >= 1 &&
+                                                                                               q.partitionings < 4))))
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");

@@ -671,6 +671,343 @@ __global__ void __launch_bounds__(64) k_partition(ImageArgs A, DevTables T, int1
     for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)pos[c];
 }
 
+// ---- exhaustive partition candidates (code/best_partitionings.cpp:498-737) ----
+// The CPU partition search the reference leaves commented out
+// (code/compression_pipeline.cpp:363-389), run over every selected
+// partitioning: weighted-random k-means seeding and three k-means passes,
+// coverage mismatch of every selected partitioning, a stable counting sort by
+// mismatch, the uncorrelated / same-chroma line fits of each partitioning in
+// that order, the 4 best of each kept by the reference's insertion, then
+// interleaved and de-duplicated.  One lane per block: every step is a
+// per-block sequential loop whose float operations keep the reference's
+// order.  The block's RGBA8 texels and k-means assignment sit in LDS
+// ([texel][lane]); the mismatch counts and the ordering go to a per-block
+// scratch column (the mode-error buffer of the same partition count, which is
+// written only later), [row][block] so that the lanes' accesses coalesce.
+__device__ __constant__ float kKmeansCutoffs[9] = {0.626220f, 0.932770f, 0.275454f, 0.318558f, 0.240113f,
+                                                   0.009190f, 0.347661f, 0.731960f, 0.156391f};
+
+__device__ __forceinline__ float ex_dist(const float* a, const float* b) {
+    const float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2], d3 = a[3] - b[3];
+    return (((d0 * d0 * kCW[0]) + (d1 * d1 * kCW[1])) + (d2 * d2 * kCW[2])) + (d3 * d3 * kCW[3]);
+}
+
+__device__ __forceinline__ void ex_normalize(const float* v, float* o) {
+    float n = ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3];
+    if (n != 0.0f) {
+        n = sqrtf(n);
+        for (int c = 0; c < 4; ++c) o[c] = v[c] / n;
+    } else {
+        for (int c = 0; c < 4; ++c) o[c] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ void ex_insert(float e, int id, float* be, int* bi) {
+    if (e >= be[3]) return;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (e > be[i]) continue;
+        for (int j = 3; j > i; --j) { be[j] = be[j - 1]; bi[j] = bi[j - 1]; }
+        be[i] = e;
+        bi[i] = id;
+        break;
+    }
+}
+
+size_t partition_ex_lds_bytes(const DevTables& T) { return (size_t)T.tc * 64 * 5 + 64 * 64 * 2; }
+
+template <int PC>
+__global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, int16_t* pb_part, uint8_t* scratch,
+                                                     SearchPlan P, int use_plan) {
+    constexpr int pc = PC;
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tc = T.tc, lane = threadIdx.x, k = blockIdx.x * 64 + lane, nb = A.nb;
+    uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);             // [tc][64] RGBA8
+    uint8_t* s_part = smem + (size_t)tc * 64 * 4;                      // [tc][64] k-means cluster
+    uint16_t* s_hist = reinterpret_cast<uint16_t*>(s_part + (size_t)tc * 64);  // [64][64] mismatch histogram
+    const int g = A.blk0 + k;
+    const bool live = k < nb && (!use_plan || plan_uses_positions(P, pc, k, g));
+    if (!live) {
+        if (k < nb)
+            for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)c;
+        return;  // no barriers below
+    }
+    const BlockLoc loc = block_loc(A, T, g);
+    uint32_t amin = 0xFFu, amax = 0u;
+    for (int t = 0; t < tc; ++t) {
+        const uint32_t v = texel_rgba8(loc, T, t);
+        s_tex[t * 64 + lane] = v;
+        amin = min(amin, v >> 24);
+        amax = max(amax, v >> 24);
+    }
+    auto PX = [&](int t, float* o) __attribute__((always_inline)) {
+        const uint32_t v = s_tex[t * 64 + lane];
+        o[0] = u8_search_value(v & 0xFF);
+        o[1] = u8_search_value((v >> 8) & 0xFF);
+        o[2] = u8_search_value((v >> 16) & 0xFF);
+        o[3] = u8_search_value(v >> 24);
+    };
+    // --- k-means: weighted-random seeding, then assign / update / assign /
+    // update / assign (best_partitionings.cpp:26-194, 419-430).  The running
+    // minimum distances are recomputed from the chosen centres (a min is exact)
+    float centers[4][4];
+    {
+        float px[4];
+        PX(145897 % tc, px);
+        for (int c = 0; c < 4; ++c) centers[0][c] = px[c];
+    }
+    int ci = 3 * (pc - 2);
+    for (int n = 1; n < pc; ++n) {
+        auto dmin = [&](int t) __attribute__((always_inline)) {
+            float px[4];
+            PX(t, px);
+            float d = ex_dist(px, centers[0]);
+            for (int j = 1; j < n; ++j) {
+                const float dj = ex_dist(px, centers[j]);
+                d = dj < d ? dj : d;
+            }
+            return d;
+        };
+        float dsum = 0.0f;
+        for (int t = 0; t < tc; ++t) dsum += dmin(t);
+        const float cut = dsum * kKmeansCutoffs[ci++];
+        float summa = 0.0f;
+        int next = 0;
+        for (; next < tc; ++next) {
+            summa += dmin(next);
+            if (summa >= cut) break;
+        }
+        next = min(next, tc - 1);
+        float px[4];
+        PX(next, px);
+        for (int c = 0; c < 4; ++c) centers[n][c] = px[c];
+    }
+    for (int it = 0; it < 3; ++it) {
+        if (it > 0) {
+            float sum[4][4] = {};
+            int cnt[4] = {0, 0, 0, 0};
+            for (int t = 0; t < tc; ++t) {
+                float px[4];
+                PX(t, px);
+                const int p = s_part[t * 64 + lane];
+#pragma unroll
+                for (int q = 0; q < pc; ++q)
+                    if (p == q) {
+                        for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
+                        cnt[q]++;
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < pc; ++q) {
+                const float sc = cnt[q] ? 1.0f / (float)cnt[q] : 0.0f;
+                for (int c = 0; c < 4; ++c) centers[q][c] = cnt[q] ? sum[q][c] * sc : 0.0f;
+            }
+        }
+        int cnt[4] = {0, 0, 0, 0};
+        for (int t = 0; t < tc; ++t) {
+            float px[4];
+            PX(t, px);
+            float bd = 3.402823466e38f;
+            int bp = 0;
+#pragma unroll
+            for (int q = 0; q < pc; ++q) {
+                const float d = ex_dist(px, centers[q]);
+                if (d < bd) { bd = d; bp = q; }
+            }
+            s_part[t * 64 + lane] = (uint8_t)bp;
+#pragma unroll
+            for (int q = 0; q < pc; ++q) cnt[q] += bp == q;
+        }
+        bool problem;
+        do {
+            problem = false;
+#pragma unroll
+            for (int q = 0; q < pc; ++q)
+                if (cnt[q] == 0) {
+                    problem = true;
+                    const int old = s_part[q * 64 + lane];
+#pragma unroll
+                    for (int r = 0; r < pc; ++r) cnt[r] -= r == old;
+                    cnt[q]++;
+                    s_part[q * 64 + lane] = (uint8_t)q;
+                }
+        } while (problem);
+    }
+    // --- mismatch of every selected partitioning, stable counting sort
+    // (best_partitionings.cpp:325-397); the histogram's running sum stops
+    // at the k-means texel count as in the reference, and ordering slots it
+    // leaves unwritten read partitioning 0
+    const int nk = min(tc, 64), nsel = T.count_selected[pc - 1];
+    uint64_t a[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nk; ++i) {
+        const int p = s_part[T.kmeans_texels[i] * 64 + lane];
+#pragma unroll
+        for (int q = 0; q < pc; ++q) a[q] |= p == q ? (1ull << i) : 0ull;
+    }
+    uint8_t* mis = scratch;                                               // [nsel][nb]
+    uint16_t* order = reinterpret_cast<uint16_t*>(scratch + (((size_t)nsel * nb + 255) & ~size_t(255)));  // [nsel][nb]
+    for (int m = 0; m < 64; ++m) s_hist[m * 64 + lane] = 0;
+    for (int i = 0; i < nsel; ++i) {
+        const uint64_t* bb = T.coverage + ((size_t)(pc - 2) * kPartitionings + i) * 4;
+        auto Pp = [&](int x, int y) { return __popcll(a[x] ^ bb[y]); };
+        int v;
+        if (pc == 2) {
+            v = min(Pp(0, 0) + Pp(1, 1), Pp(0, 1) + Pp(1, 0));
+        } else if (pc == 3) {
+            const int v0 = min(Pp(1, 1) + Pp(2, 2), Pp(1, 2) + Pp(2, 1)) + Pp(0, 0);
+            const int v1 = min(Pp(1, 0) + Pp(2, 2), Pp(1, 2) + Pp(2, 0)) + Pp(0, 1);
+            const int v2 = min(Pp(1, 0) + Pp(2, 1), Pp(1, 1) + Pp(2, 0)) + Pp(0, 2);
+            v = min(v0, min(v1, v2));
+        } else {
+            const int mx23 = min(Pp(2, 2) + Pp(3, 3), Pp(2, 3) + Pp(3, 2));
+            const int mx13 = min(Pp(2, 1) + Pp(3, 3), Pp(2, 3) + Pp(3, 1));
+            const int mx12 = min(Pp(2, 1) + Pp(3, 2), Pp(2, 2) + Pp(3, 1));
+            const int mx03 = min(Pp(2, 0) + Pp(3, 3), Pp(2, 3) + Pp(3, 0));
+            const int mx02 = min(Pp(2, 0) + Pp(3, 2), Pp(2, 2) + Pp(3, 0));
+            const int mx01 = min(Pp(2, 1) + Pp(3, 0), Pp(2, 0) + Pp(3, 1));
+            const int v0 = Pp(0, 0) + min(Pp(1, 1) + mx23, min(Pp(1, 2) + mx13, Pp(1, 3) + mx12));
+            const int v1 = Pp(0, 1) + min(Pp(1, 0) + mx23, min(Pp(1, 2) + mx03, Pp(1, 3) + mx02));
+            const int v2 = Pp(0, 2) + min(Pp(1, 1) + mx03, min(Pp(1, 0) + mx13, Pp(1, 3) + mx01));
+            const int v3 = Pp(0, 3) + min(Pp(1, 1) + mx02, min(Pp(1, 2) + mx01, Pp(1, 0) + mx12));
+            v = min(v0, min(v1, min(v2, v3)));
+        }
+        const int m = v / 2;  // < 64 (half of at most 2 x 64 differing bits)
+        mis[(size_t)i * nb + k] = (uint8_t)m;
+        s_hist[m * 64 + lane]++;
+        order[(size_t)i * nb + k] = 0;
+    }
+    {
+        uint16_t sum = 0;
+        for (int m = 0; m < nk; ++m) {
+            const uint16_t c = s_hist[m * 64 + lane];
+            s_hist[m * 64 + lane] = sum;
+            sum = (uint16_t)(sum + c);
+        }
+    }
+    for (int i = 0; i < nsel; ++i) {
+        const int m = mis[(size_t)i * nb + k];
+        const int pos = s_hist[m * 64 + lane];
+        s_hist[m * 64 + lane] = (uint16_t)(pos + 1);
+        if (pos < nsel) order[(size_t)pos * nb + k] = (uint16_t)i;
+    }
+    // --- line fits in mismatch order (find_best_partition_candidates,
+    // compute_avgs_and_dirs_{4_comp,3_comp_rgb}, compute_error_squared_{rgba,rgb})
+    const bool rgba = amin != amax;
+    float wie = 0.055f;
+    if (tc <= 20) wie = 0.03f;
+    else if (tc <= 31) wie = 0.04f;
+    else if (tc <= 41) wie = 0.05f;
+    wie = wie * wie;
+    float ube[4] = {1e30f, 1e30f, 1e30f, 1e30f}, sbe[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
+    for (int s = 0; s < nsel; ++s) {
+        const int i = order[(size_t)s * nb + k];
+        const size_t row = (size_t)(pc - 2) * kPartitionings + i;
+        const uint8_t* pot = T.part_of_texel + row * tc;
+        const uint8_t* cnts = T.part_counts + row * 4;
+        float sum[PC][4] = {};
+        for (int t = 0; t < tc; ++t) {
+            float px[4];
+            PX(t, px);
+            const int p = pot[t];
+#pragma unroll
+            for (int q = 0; q < PC; ++q)
+                if (p == q)
+                    for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
+        }
+        float ue = 0.0f, se = 0.0f, len[PC], bdir[PC][4], bavg[PC][4];
+#pragma unroll
+        for (int p = 0; p < PC; ++p) {
+            float avg[4];
+            for (int c = 0; c < 4; ++c) avg[c] = sum[p][c] / (float)cnts[p];
+            float sx[4] = {}, sy[4] = {}, sz[4] = {}, sw[4] = {};
+            for (int t = 0; t < tc; ++t) {
+                if (pot[t] != p) continue;
+                float px[4], d[4];
+                PX(t, px);
+                for (int c = 0; c < 4; ++c) d[c] = px[c] - avg[c];
+                if (d[0] > 0) for (int c = 0; c < 4; ++c) sx[c] += d[c];
+                if (d[1] > 0) for (int c = 0; c < 4; ++c) sy[c] += d[c];
+                if (d[2] > 0) for (int c = 0; c < 4; ++c) sz[c] += d[c];
+                if (rgba && d[3] > 0) for (int c = 0; c < 4; ++c) sw[c] += d[c];
+            }
+            float best[4];
+            for (int c = 0; c < 4; ++c) best[c] = sx[c];
+            if (rgba) {
+                float bs = dot4(sx, sx), pr = dot4(sy, sy);
+                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sy[c]; }
+                pr = dot4(sz, sz);
+                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sz[c]; }
+                pr = dot4(sw, sw);
+                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sw[c]; }
+            } else {
+                float bs = dot3(sx, sx), pr = dot3(sy, sy);
+                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sy[c]; }
+                pr = dot3(sz, sz);
+                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sz[c]; }
+                best[3] = 0.0f;
+            }
+            ex_normalize(avg, bavg[p]);
+            ex_normalize(best, bdir[p]);
+            const float ud = rgba ? dot4(avg, bdir[p]) : dot3(avg, bdir[p]);
+            float amod[4];
+            for (int c = 0; c < 4; ++c) amod[c] = avg[c] - bdir[p][c] * ud;
+            float lo = 1e10f, hi = -1e10f;
+            for (int t = 0; t < tc; ++t) {
+                if (pot[t] != p) continue;
+                float px[4];
+                PX(t, px);
+                float up = (px[0] * bdir[p][0] + px[1] * bdir[p][1]) + px[2] * bdir[p][2];
+                float sp = (px[0] * bavg[p][0] + px[1] * bavg[p][1]) + px[2] * bavg[p][2];
+                if (rgba) {
+                    up = up + px[3] * bdir[p][3];
+                    sp = sp + px[3] * bavg[p][3];
+                }
+                lo = up < lo ? up : lo;
+                hi = up > hi ? up : hi;
+                float err = 0.0f, serr = 0.0f;
+                for (int c = 0; c < (rgba ? 4 : 3); ++c) {
+                    const float dd = (amod[c] - px[c]) + (up * bdir[p][c]);
+                    const float ds = sp * bavg[p][c] - px[c];
+                    err = c ? err + kCW[c] * dd * dd : kCW[c] * dd * dd;
+                    serr = c ? serr + kCW[c] * ds * ds : kCW[c] * ds * ds;
+                }
+                ue += err;
+                se += serr;
+            }
+            const float l = hi - lo;
+            len[p] = l > 1e-7f ? l : 1e-7f;
+        }
+#pragma unroll
+        for (int p = 0; p < PC; ++p) {
+            const float ew = (float)cnts[p] * wie;
+            float uv[4], sv[4];
+            for (int c = 0; c < 4; ++c) { uv[c] = bdir[p][c] * len[p]; sv[c] = bavg[p][c] * len[p]; }
+            ue += (rgba ? dot4(uv, uv) : dot3(uv, uv)) * ew;
+            se += (rgba ? dot4(sv, sv) : dot3(sv, sv)) * ew;
+        }
+        ex_insert(ue, i, ube, ubi);
+        ex_insert(se, i, sbe, sbi);
+    }
+    int pos[4] = {-1, -1, -1, -1}, emitted = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int idx = (j & 1) ? sbi[j >> 1] : ubi[j >> 1];
+        bool dup = emitted >= 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dup |= q < emitted && pos[q] == idx;
+        if (!dup) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q == emitted) pos[q] = idx;
+            ++emitted;
+        }
+    }
+    // a missing candidate (fewer than 4 distinct) repeats the first one: a
+    // duplicate search never beats the first under the strict < of the picks
+    for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)(pos[c] >= 0 ? pos[c] : pos[0]);
+}
+
 // ---- pass01 + pass08 + pass09 + pass10 --------------------------------------
 // One lane per partitioned block.  Per-partition accumulators are updated
 // through a compile-time partition index (texel partition compared against
@@ -3330,7 +3667,7 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
 
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan,
-                        bool dual_plane) {
+                        bool dual_plane, bool exhaustive) {
     (void)hipGetLastError();  // the final check reports this call's launches only
     const LabelBufs lab = labels ? *labels : LabelBufs{nullptr, nullptr, nullptr};
     const int threads = 64;
@@ -3352,7 +3689,17 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         ImageArgs A = A0;
         if (pc > 1) {
             if (prof) prof->begin(K_PARTITION, s);
-            {
+            if (exhaustive) {
+                // the mode-error buffer of this count is free until k_dm_bm
+                const dim3 pgrid((A.nb + 63) / 64);
+                const size_t plds = partition_ex_lds_bytes(T);
+                const SearchPlan pplan = plan ? *plan : SearchPlan{};
+                const int use = plan ? 1 : 0;
+                uint8_t* scr = reinterpret_cast<uint8_t*>(B.ccr);
+                if (pc == 2) hipLaunchKernelGGL(k_partition_ex<2>, pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, pplan, use);
+                else if (pc == 3) hipLaunchKernelGGL(k_partition_ex<3>, pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, pplan, use);
+                else hipLaunchKernelGGL(k_partition_ex<4>, pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, pplan, use);
+            } else {
                 const dim3 pgrid((A.nb + kPartBlocks - 1) / kPartBlocks);
                 const size_t plds = partition_lds_bytes(T);
                 const SearchPlan pplan = plan ? *plan : SearchPlan{};

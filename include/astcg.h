@@ -42,14 +42,19 @@ typedef struct astcg_encoder astcg_encoder;
 
 /* Candidate policy.  COMPAT reproduces the reference's search (packed table
  * positions chosen by passes 001-007 including their indexing quirks, all
- * trial modes, 8 refined candidates).  LEARNED prunes it with a trained
- * candidate predictor (astcg_set_predictor): no counterpart in the
- * reference. */
-typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1 } astcg_policy;
+ * trial modes, 8 refined candidates).  EXHAUSTIVE replaces passes 001-007 by
+ * the partition search the reference keeps but never calls
+ * (code/best_partitionings.cpp:498-737, caller commented out at
+ * code/compression_pipeline.cpp:363-389), run over every selected
+ * partitioning; the rest of the search is the same.  LEARNED prunes either
+ * with a trained candidate predictor (astcg_set_predictor): no counterpart
+ * in the reference. */
+typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1, ASTCG_POLICY_EXHAUSTIVE = 2 } astcg_policy;
 
 /* Pruning of the LEARNED policy.  A block whose kept trial modes (its
  * `modes` most likely) hold less than fallback_below of the predicted
- * probability gets the full COMPAT search; any other block searches only the
+ * probability gets the full search (COMPAT, or EXHAUSTIVE with
+ * candidates = 1); any other block searches only the
  * partition counts that are its most likely count or
  * at least count_below probable (with the COMPAT positions when seeds is 0,
  * else its `seeds` most likely partitionings of each such count), only its
@@ -78,11 +83,12 @@ typedef struct {
                                times the reference's error limit skips the remaining counts; 0 = off */
     float full_search_above; /* > 0: a block whose best error after one partition is above this multiple
                                 of the error limit searches 2..4 partitions with every trial mode and 8
-                                slots (the COMPAT search); 0 = off */
+                                slots (the full search); 0 = off */
     uint32_t rounds;        /* refinement rounds per candidate (pass13-17), 1..6, 0 = 6 as the reference */
     uint32_t partitioned_slots; /* candidates refined per partitioned block for 2..4 partitions
                                    (1..8, 0 = slots); fallback blocks keep 8 */
-    uint32_t reserved;
+    uint32_t candidates;    /* partition candidates of the searched counts (and of fallback / hard
+                               blocks): 0 = the COMPAT positions, 1 = the EXHAUSTIVE search */
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);

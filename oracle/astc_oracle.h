@@ -64,6 +64,8 @@ typedef struct {
     int threads;          /* OpenMP threads, 0 = 1 */
     int first_block;      /* encode only blocks [first_block, first_block+block_count) */
     int block_count;      /* 0 = all */
+    int policy;           /* partition candidates: 0 = the live passes 001-007 (compat),
+                             2 = the exhaustive CPU search of best_partitionings.cpp */
 } or_params;
 
 typedef struct {
@@ -102,6 +104,12 @@ int oracle_partition_entry(int bx, int by, int pc, int idx, uint8_t* part_of_tex
 
 /* Decimation table of trial decimation mode m (see or_tables.c). */
 int oracle_decimation_entry(int bx, int by, int m, int32_t* out, int cap);
+
+/* The exhaustive partition candidates (code/best_partitionings.cpp:498-737)
+ * of block block_index for partition count pc: 4 packed table entries
+ * (-1 where fewer were found). */
+int oracle_debug_exhaustive_candidates(const uint8_t* rgba, int width, int height, int bx, int by,
+                                       int block_index, int partition_count, int32_t* entries_out);
 
 /* Stage-level hooks used by unit tests (single partitioned block). */
 int oracle_debug_kmeans_candidates(const uint8_t* rgba, int width, int height, int bx, int by,

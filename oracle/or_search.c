@@ -1167,6 +1167,276 @@ static void compat_positions(const or_desc* d, const uint8_t* rgba, int w, int h
     select_positions(eu, es, pos);
 }
 
+/* ---------- exhaustive partition candidates: code/best_partitionings.cpp ---------- */
+
+/* dot_product_squared_difference, best_partitionings.cpp:7-21 */
+static float ex_dist(const float* a, const float* b) {
+    float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2], d3 = a[3] - b[3];
+    return (((d0 * d0 * CW[0]) + (d1 * d1 * CW[1])) + (d2 * d2 * CW[2])) + (d3 * d3 * CW[3]);
+}
+
+/* kmeans_init, best_partitionings.cpp:26-99: first centre texel 145897 % n,
+ * then weighted-random picks by cumulative distance against fixed cutoffs */
+static void ex_kmeans_init(const or_block* b, int tc, int pc, float centers[4][4]) {
+    static const float cutoffs[9] = {0.626220f, 0.932770f, 0.275454f, 0.318558f, 0.240113f,
+                                     0.009190f, 0.347661f, 0.731960f, 0.156391f};
+    float dist[OR_MAX_TEXELS];
+    int n = 0;
+    memcpy(centers[n++], b->px[145897 % tc], sizeof(float) * 4);
+    float dsum = 0.0f;
+    for (int i = 0; i < tc; i++) { dist[i] = ex_dist(b->px[i], centers[0]); dsum += dist[i]; }
+    int ci = 3 * (pc - 2);
+    for (;;) {
+        float summa = 0.0f, cut = dsum * cutoffs[ci++];
+        int next = 0;
+        for (; next < tc; next++) { summa += dist[next]; if (summa >= cut) break; }
+        if (next > tc - 1) next = tc - 1;
+        memcpy(centers[n++], b->px[next], sizeof(float) * 4);
+        if (n >= pc) break;
+        dsum = 0.0f;
+        for (int i = 0; i < tc; i++) {
+            float d = ex_dist(b->px[i], centers[n - 1]);
+            dist[i] = d < dist[i] ? d : dist[i];
+            dsum += dist[i];
+        }
+    }
+}
+
+/* kmeans_assign, best_partitionings.cpp:101-151 (with its empty-cluster fix) */
+static void ex_kmeans_assign(const or_block* b, int tc, int pc, float centers[4][4], uint8_t* part) {
+    int cnt[4] = {0, 0, 0, 0};
+    for (int i = 0; i < tc; i++) {
+        float bd = 3.402823466e38f;
+        int bp = 0;
+        for (int j = 0; j < pc; j++) {
+            float d = ex_dist(b->px[i], centers[j]);
+            if (d < bd) { bd = d; bp = j; }
+        }
+        part[i] = (uint8_t)bp;
+        cnt[bp]++;
+    }
+    int problem;
+    do {
+        problem = 0;
+        for (int i = 0; i < pc; i++)
+            if (cnt[i] == 0) {
+                problem = 1;
+                cnt[part[i]]--;
+                cnt[i]++;
+                part[i] = (uint8_t)i;
+            }
+    } while (problem);
+}
+
+/* kmeans_update, best_partitionings.cpp:153-194 */
+static void ex_kmeans_update(const or_block* b, int tc, int pc, float centers[4][4], const uint8_t* part) {
+    float sum[4][4] = {{0}};
+    int cnt[4] = {0, 0, 0, 0};
+    for (int i = 0; i < tc; i++) {
+        for (int c = 0; c < 4; c++) sum[part[i]][c] += b->px[i][c];
+        cnt[part[i]]++;
+    }
+    for (int i = 0; i < pc; i++) {
+        if (cnt[i] == 0) { for (int c = 0; c < 4; c++) centers[i][c] = 0.0f; continue; }
+        float sc = 1.0f / (float)cnt[i];
+        for (int c = 0; c < 4; c++) centers[i][c] = sum[i][c] * sc;
+    }
+}
+
+/* compute_kmeans_partition_ordering + get_partition_ordering_by_mismatch_bits,
+ * best_partitionings.cpp:364-397, 399-447: 3 k-means passes, bitmaps over
+ * the first min(texels, 64) k-means texels, mismatch per selected
+ * partitioning, stable counting sort by mismatch.  Returns the count. */
+static int ex_ordering(const or_desc* d, const or_block* b, int pc, uint16_t* order) {
+    int tc = d->texel_count;
+    float centers[4][4];
+    uint8_t part[OR_MAX_TEXELS];
+    for (int it = 0; it < 3; it++) {
+        if (it == 0) ex_kmeans_init(b, tc, pc, centers);
+        else ex_kmeans_update(b, tc, pc, centers, part);
+        ex_kmeans_assign(b, tc, pc, centers, part);
+    }
+    int nk = tc < 64 ? tc : 64;
+    uint64_t a[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nk; i++) a[part[d->kmeans_texels[i]]] |= 1ull << i;
+    int nsel = d->count_selected[pc - 1];
+    uint8_t mis[OR_PARTITIONINGS];
+    for (int i = 0; i < nsel; i++) {
+        const uint64_t* bb = d->coverage[pc - 2][i];
+#define P(x, y) popc64(a[x] ^ bb[y])
+#define MIN(x, y) ((x) < (y) ? (x) : (y))
+        int v;
+        if (pc == 2) v = MIN(P(0, 0) + P(1, 1), P(0, 1) + P(1, 0));
+        else if (pc == 3) {
+            int v0 = MIN(P(1, 1) + P(2, 2), P(1, 2) + P(2, 1)) + P(0, 0);
+            int v1 = MIN(P(1, 0) + P(2, 2), P(1, 2) + P(2, 0)) + P(0, 1);
+            int v2 = MIN(P(1, 0) + P(2, 1), P(1, 1) + P(2, 0)) + P(0, 2);
+            v = MIN(v0, MIN(v1, v2));
+        } else {
+            int mx23 = MIN(P(2, 2) + P(3, 3), P(2, 3) + P(3, 2)), mx13 = MIN(P(2, 1) + P(3, 3), P(2, 3) + P(3, 1));
+            int mx12 = MIN(P(2, 1) + P(3, 2), P(2, 2) + P(3, 1)), mx03 = MIN(P(2, 0) + P(3, 3), P(2, 3) + P(3, 0));
+            int mx02 = MIN(P(2, 0) + P(3, 2), P(2, 2) + P(3, 0)), mx01 = MIN(P(2, 1) + P(3, 0), P(2, 0) + P(3, 1));
+            int v0 = P(0, 0) + MIN(P(1, 1) + mx23, MIN(P(1, 2) + mx13, P(1, 3) + mx12));
+            int v1 = P(0, 1) + MIN(P(1, 0) + mx23, MIN(P(1, 2) + mx03, P(1, 3) + mx02));
+            int v2 = P(0, 2) + MIN(P(1, 1) + mx03, MIN(P(1, 0) + mx13, P(1, 3) + mx01));
+            int v3 = P(0, 3) + MIN(P(1, 1) + mx02, MIN(P(1, 2) + mx01, P(1, 0) + mx12));
+            v = MIN(v0, MIN(v1, MIN(v2, v3)));
+        }
+#undef P
+#undef MIN
+        mis[i] = (uint8_t)(v / 2);
+    }
+    uint16_t msc[64];
+    memset(msc, 0, sizeof(msc));
+    for (int i = 0; i < nsel; i++) msc[mis[i]]++;
+    uint16_t sum = 0;
+    for (int i = 0; i < nk; i++) { uint16_t c = msc[i]; msc[i] = sum; sum = (uint16_t)(sum + c); }
+    /* a mismatch equal to the texel count is past the running sum, so its
+     * entries can land past the sequence: those writes are dropped (the
+     * reference writes past its array there) */
+    for (int i = 0; i < nsel; i++) {
+        int pos = msc[mis[i]]++;
+        if (pos < nsel) order[pos] = (uint16_t)i;
+    }
+    return nsel;
+}
+
+/* normalize, averages_and_directions.cpp:11-22 (zero vector -> zero) */
+static void ex_normalize(const float* v, float* o) {
+    float n = ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + v[3] * v[3];
+    if (n != 0.0f) { n = sqrtf(n); for (int c = 0; c < 4; c++) o[c] = v[c] / n; }
+    else for (int c = 0; c < 4; c++) o[c] = 0.0f;
+}
+
+/* insert_result, best_partitionings.cpp:460-496 */
+static void ex_insert(float e, int id, float* be, int* bi) {
+    if (e >= be[3]) return;
+    for (int i = 0; i < 4; i++) {
+        if (e > be[i]) continue;
+        for (int j = 3; j > i; j--) { be[j] = be[j - 1]; bi[j] = bi[j - 1]; }
+        be[i] = e;
+        bi[i] = id;
+        break;
+    }
+}
+
+/* find_best_partition_candidates, best_partitionings.cpp:498-737, over every
+ * selected partitioning (search limit = all), 4 requested candidates, with
+ * the line fits of averages_and_directions.cpp (4 components when alpha
+ * varies, RGB otherwise).  Writes 4 packed table entries (-1 = none). */
+static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, int* pos) {
+    int tc = d->texel_count;
+    float wie = 0.055f;
+    if (tc <= 20) wie = 0.03f;
+    else if (tc <= 31) wie = 0.04f;
+    else if (tc <= 41) wie = 0.05f;
+    wie = wie * wie;
+    /* positions the counting sort leaves unwritten (its running sum stops
+     * before the largest possible mismatch count) read 0 here; the
+     * reference reads an uninitialised stack array there */
+    uint16_t seq[OR_PARTITIONINGS];
+    memset(seq, 0, sizeof(seq));
+    int n = ex_ordering(d, b, pc, seq);
+    int req = n < 4 ? n : 4;
+    float ube[4], sbe[4];
+    int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) ube[i] = sbe[i] = 1e30f;
+    int rgba = !b->constant_alpha;
+    for (int s = 0; s < n; s++) {
+        const or_pinfo* pi = &d->part[(pc - 2) * OR_PARTITIONINGS + seq[s]];
+        float sum[4][4] = {{0}};
+        for (int i = 0; i < tc; i++)
+            for (int c = 0; c < 4; c++) sum[pi->partition_of_texel[i]][c] += b->px[i][c];
+        float avg[4][4], bdir[4][4], bavg[4][4], amod[4][4], len[4];
+        for (int p = 0; p < pc; p++) {
+            for (int c = 0; c < 4; c++) avg[p][c] = sum[p][c] / (float)pi->texel_count[p];
+            float sx[4] = {0}, sy[4] = {0}, sz[4] = {0}, sw[4] = {0};
+            for (int i = 0; i < tc; i++) {
+                if (pi->partition_of_texel[i] != p) continue;
+                float t[4];
+                for (int c = 0; c < 4; c++) t[c] = b->px[i][c] - avg[p][c];
+                if (t[0] > 0) for (int c = 0; c < 4; c++) sx[c] += t[c];
+                if (t[1] > 0) for (int c = 0; c < 4; c++) sy[c] += t[c];
+                if (t[2] > 0) for (int c = 0; c < 4; c++) sz[c] += t[c];
+                if (rgba && t[3] > 0) for (int c = 0; c < 4; c++) sw[c] += t[c];
+            }
+            float* best = sx;
+            float bs, pr;
+            if (rgba) {
+                bs = dot4(sx, sx);
+                pr = dot4(sy, sy); if (pr > bs) { bs = pr; best = sy; }
+                pr = dot4(sz, sz); if (pr > bs) { bs = pr; best = sz; }
+                pr = dot4(sw, sw); if (pr > bs) { bs = pr; best = sw; }
+            } else {
+                bs = dot3(sx, sx);
+                pr = dot3(sy, sy); if (pr > bs) { bs = pr; best = sy; }
+                pr = dot3(sz, sz); if (pr > bs) { bs = pr; best = sz; }
+            }
+            float dir[4] = {best[0], best[1], best[2], rgba ? best[3] : 0.0f};
+            ex_normalize(avg[p], bavg[p]);
+            ex_normalize(dir, bdir[p]);
+            float ud = rgba ? dot4(avg[p], bdir[p]) : dot3(avg[p], bdir[p]);
+            for (int c = 0; c < 4; c++) amod[p][c] = avg[p][c] - bdir[p][c] * ud;
+        }
+        float ue = 0.0f, se = 0.0f;
+        int nch = rgba ? 4 : 3;
+        for (int p = 0; p < pc; p++) {
+            float lo = 1e10f, hi = -1e10f;
+            for (int i = 0; i < tc; i++) {
+                if (pi->partition_of_texel[i] != p) continue;
+                const float* px = b->px[i];
+                float up = px[0] * bdir[p][0] + px[1] * bdir[p][1] + px[2] * bdir[p][2];
+                if (rgba) up = up + px[3] * bdir[p][3];
+                lo = up < lo ? up : lo;
+                hi = up > hi ? up : hi;
+                float err = 0.0f, sp = px[0] * bavg[p][0] + px[1] * bavg[p][1] + px[2] * bavg[p][2];
+                if (rgba) sp = sp + px[3] * bavg[p][3];
+                float serr = 0.0f;
+                for (int c = 0; c < nch; c++) {
+                    float dd = (amod[p][c] - px[c]) + (up * bdir[p][c]);
+                    float ds = sp * bavg[p][c] - px[c];
+                    err = c ? err + CW[c] * dd * dd : CW[c] * dd * dd;
+                    serr = c ? serr + CW[c] * ds * ds : CW[c] * ds * ds;
+                }
+                ue += err;
+                se += serr;
+            }
+            float l = hi - lo;
+            len[p] = l > 1e-7f ? l : 1e-7f;
+        }
+        for (int p = 0; p < pc; p++) {
+            float ew = (float)pi->texel_count[p] * wie;
+            float uv[4], sv[4];
+            for (int c = 0; c < 4; c++) { uv[c] = bdir[p][c] * len[p]; sv[c] = bavg[p][c] * len[p]; }
+            ue += (rgba ? dot4(uv, uv) : dot3(uv, uv)) * ew;
+            se += (rgba ? dot4(sv, sv) : dot3(sv, sv)) * ew;
+        }
+        ex_insert(ue, seq[s], ube, ubi);
+        ex_insert(se, seq[s], sbe, sbi);
+    }
+    int inter[8], emitted = 0;
+    for (int i = 0; i < req; i++) { inter[2 * i] = ubi[i]; inter[2 * i + 1] = sbi[i]; }
+    for (int i = 0; i < 4; i++) pos[i] = -1;
+    for (int i = 0; i < 2 * req && emitted < req; i++) {
+        int dup = 0;
+        for (int j = 0; j < emitted; j++) if (pos[j] == inter[i]) dup = 1;
+        if (!dup) pos[emitted++] = inter[i];
+    }
+}
+
+int oracle_debug_exhaustive_candidates(const uint8_t* rgba, int width, int height, int bx, int by, int block_index,
+                                       int partition_count, int32_t* entries_out) {
+    or_desc* d = or_desc_get(bx, by);
+    if (!d || partition_count < 2 || partition_count > 4) return -1;
+    int bxn = (width + bx - 1) / bx;
+    or_block blk;
+    load_block(d, rgba, width, height, block_index % bxn, block_index / bxn, &blk);
+    int pos[4];
+    exhaustive_positions(d, &blk, partition_count, pos);
+    for (int i = 0; i < 4; i++) entries_out[i] = pos[i];
+    return 0;
+}
+
 int oracle_debug_kmeans_candidates(const uint8_t* rgba, int width, int height, int bx, int by, int block_index,
                                    int partition_count, int batch_size, int32_t* positions_out) {
     or_desc* d = or_desc_get(bx, by);
@@ -1189,6 +1459,7 @@ int oracle_encode(const uint8_t* rgba, int width, int height, int bx, int by, co
     int maxp = prm && prm->max_partitions > 0 ? prm->max_partitions : 4;
     int batch = prm && prm->batch_size > 0 ? prm->batch_size : REF_BATCH;
     int threads = prm && prm->threads > 0 ? prm->threads : 1;
+    int policy = prm ? prm->policy : 0;
     int bxn = (width + bx - 1) / bx, byn = (height + by - 1) / by;
     int nb = bxn * byn;
     int first = prm ? prm->first_block : 0;
@@ -1215,7 +1486,11 @@ int oracle_encode(const uint8_t* rgba, int width, int height, int bx, int by, co
             int have = 0;
             int pos[4] = {0, -1, -1, -1};
             int npos = 1;
-            if (pc > 1) { compat_positions(d, rgba, width, height, bxn, base, B, j, pc, pos); npos = 4; }
+            if (pc > 1) {
+                if (policy == 2) exhaustive_positions(d, &blk, pc, pos);
+                else compat_positions(d, rgba, width, height, bxn, base, B, j, pc, pos);
+                npos = 4;
+            }
             for (int c = 0; c < npos; c++) {
                 if (pos[c] < 0) continue;
                 or_block pbk = blk;

@@ -32,7 +32,10 @@ class OrSymbolic(ctypes.Structure):
 
 class OrParams(ctypes.Structure):
     _fields_ = [("max_partitions", ctypes.c_int), ("batch_size", ctypes.c_int), ("threads", ctypes.c_int),
-                ("first_block", ctypes.c_int), ("block_count", ctypes.c_int)]
+                ("first_block", ctypes.c_int), ("block_count", ctypes.c_int), ("policy", ctypes.c_int)]
+
+POLICY_COMPAT = 0
+POLICY_EXHAUSTIVE = 2
 
 
 class OrStats(ctypes.Structure):
@@ -70,6 +73,8 @@ def lib():
         L.oracle_partition_entry.restype = i32
         L.oracle_debug_kmeans_candidates.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, vp]
         L.oracle_debug_kmeans_candidates.restype = i32
+        L.oracle_debug_exhaustive_candidates.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp]
+        L.oracle_debug_exhaustive_candidates.restype = i32
         L.oracle_decimation_entry.argtypes = [i32, i32, i32, vp, i32]
         L.oracle_decimation_entry.restype = i32
         _lib = L
@@ -77,7 +82,7 @@ def lib():
 
 
 def encode(rgba, bx, by, max_partitions=4, threads=1, first_block=0, block_count=0, batch_size=920,
-           want_symbolic=True):
+           want_symbolic=True, policy=POLICY_COMPAT):
     """Returns (blocks uint8[n*16], symbolic list or None, stats)."""
     rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
     h, w = rgba.shape[:2]
@@ -86,7 +91,7 @@ def encode(rgba, bx, by, max_partitions=4, threads=1, first_block=0, block_count
     out = np.zeros(n * 16, np.uint8)
     sym = (OrSymbolic * n)() if want_symbolic else None
     st = OrStats()
-    p = OrParams(max_partitions, batch_size, threads, first_block, block_count)
+    p = OrParams(max_partitions, batch_size, threads, first_block, block_count, policy)
     r = lib().oracle_encode(rgba.ctypes.data, w, h, bx, by, ctypes.byref(p), out.ctypes.data,
                             ctypes.cast(sym, ctypes.c_void_p) if sym is not None else None, ctypes.byref(st))
     if r != 0:
@@ -139,6 +144,16 @@ def compat_positions(rgba, bx, by, block_index, partition_count, batch_size=920)
                                              batch_size, pos)
     if r != 0:
         raise RuntimeError("oracle_debug_kmeans_candidates failed")
+    return list(pos)
+
+
+def exhaustive_candidates(rgba, bx, by, block_index, partition_count):
+    """The 4 packed table entries of the exhaustive partition search."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    pos = (ctypes.c_int32 * 4)()
+    if lib().oracle_debug_exhaustive_candidates(rgba.ctypes.data, w, h, bx, by, block_index, partition_count, pos):
+        raise RuntimeError("oracle_debug_exhaustive_candidates failed")
     return list(pos)
 
 

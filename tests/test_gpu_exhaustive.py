@@ -1,0 +1,64 @@
+"""The EXHAUSTIVE partition policy (the CPU partition search of
+code/best_partitionings.cpp:498-737, which the reference never calls, run
+over every selected partitioning) on the GPU: bit-exact with the oracle's
+restatement of it on the same images, also through the batched and the
+no-allocation entry points; it finds better partitionings than the live
+passes 001-007, so the decoded PSNR is at least the COMPAT search's."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import astc_implementation_amd as A
+from astc_implementation_amd import synth
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def encoder():
+    enc = A.Encoder(0)
+    yield enc
+    enc.close()
+
+
+@pytest.mark.parametrize("bx,by", [(4, 4), (5, 4), (6, 6), (8, 8), (10, 10), (12, 12)])
+@pytest.mark.parametrize("kind", ["gradient", "alpha"])
+def test_exhaustive_bit_exact(encoder, bx, by, kind):
+    img = (synth.gradient_image if kind == "gradient" else synth.alpha_image)(96, 80, 13)
+    encoder.configure(96, 80, bx, by, policy=A.POLICY_EXHAUSTIVE)
+    gpu = encoder.encode(img)
+    ref, _, _ = O.encode(img, bx, by, 4, threads=16, want_symbolic=False, policy=O.POLICY_EXHAUSTIVE)
+    bad = np.nonzero(np.any(gpu.reshape(-1, 16) != ref.reshape(-1, 16), axis=1))[0]
+    assert bad.size == 0, "blocks differ: %s" % bad[:10].tolist()
+    encoder.configure(96, 80, bx, by)
+    compat = encoder.encode(img)
+    assert A.psnr(A.decode_image(bx, by, 96, 80, gpu), img) >= A.psnr(A.decode_image(bx, by, 96, 80, compat), img)
+
+
+def test_exhaustive_512_config0_image(encoder):
+    img = synth.gradient_image(512, 512, 0)
+    encoder.configure(512, 512, 4, 4, policy=A.POLICY_EXHAUSTIVE)
+    gpu = encoder.encode(img)
+    ref, _, _ = O.encode(img, 4, 4, 4, threads=16, want_symbolic=False, policy=O.POLICY_EXHAUSTIVE)
+    assert np.array_equal(gpu, ref)
+
+
+def test_exhaustive_batch_and_noalloc(encoder):
+    import torch
+    imgs = [synth.gradient_image(70, 50, 3), synth.normal_map_image(45, 66, 4), synth.flat_noise_image(33, 29, 5)]
+    fps = [(4, 4), (6, 6), (8, 8)]
+    outs = encoder.encode_batch(imgs, fps, policy=A.POLICY_EXHAUSTIVE)
+    for img, (bx, by), out in zip(imgs, fps):
+        h, w = img.shape[:2]
+        ref, _, _ = O.encode(img, bx, by, 4, threads=16, want_symbolic=False, policy=O.POLICY_EXHAUSTIVE)
+        assert np.array_equal(out, ref)
+        encoder.configure(w, h, bx, by, policy=A.POLICY_EXHAUSTIVE, chunk_blocks=37)
+        d_in = torch.from_numpy(img.reshape(-1).copy()).cuda()
+        d_out = torch.zeros(ref.size, dtype=torch.uint8, device="cuda")
+        st = A.library().astcg_encode_device_noalloc(encoder._h, ctypes.c_void_p(d_in.data_ptr()),
+                                                    ctypes.c_void_p(d_out.data_ptr()), None)
+        assert st == A.OK
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out.cpu().numpy(), ref)

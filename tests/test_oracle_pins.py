@@ -1194,3 +1194,46 @@ def test_positions_are_interleaved_stable_top4_without_duplicates():
                 out.append(int(v))
         out += [-1] * (4 - len(out))
         assert O.select_positions(eu, es) == out
+
+
+# ---------------------------------------------------------------- exhaustive partition candidates
+@pytest.mark.parametrize("fp", [(4, 4), (6, 6), (8, 8)])
+@pytest.mark.parametrize("pc", [2, 3, 4])
+def test_exhaustive_search_finds_the_partitioning_a_block_is_made_of(fp, pc):
+    """A block painted from one selected partitioning (a distinct, well
+    separated colour per partition): k-means recovers it, its mismatch is 0
+    and both of its line fits are exact, so it is the first candidate."""
+    bx, by = fp
+    rng = np.random.default_rng(300 + pc + bx)
+    cols = np.array([[10, 10, 10, 255], [240, 20, 20, 255], [20, 240, 20, 255], [20, 20, 240, 255]], np.uint8)
+    nsel = O.footprint_info(bx, by)["selected_p%d" % pc]
+    for e in rng.integers(0, nsel, 12):
+        pot, meta = O.partition_entry(bx, by, pc, int(e))
+        img = cols[np.array(pot)].reshape(by, bx, 4)
+        got = O.exhaustive_candidates(img, bx, by, 0, pc)
+        assert got[0] == e, (fp, pc, e, got)
+        assert len(set(got)) == 4
+
+
+@pytest.mark.parametrize("alpha", [False, True])
+@pytest.mark.parametrize("pc", [2, 3, 4])
+def test_exhaustive_search_with_noise_and_alpha(alpha, pc):
+    """Blocks painted from one selected partitioning with per-partition
+    colours (and alphas, which switches the line fits to four components)
+    plus texel noise well below the colour separation: that partitioning
+    stays the first candidate.  (Beyond such clear cases the ranking is
+    decided by the reference's float32 line fits, whose cancellations make
+    a float64 ranking differ.)"""
+    bx = by = 6
+    rng = np.random.default_rng(320 + pc + 10 * alpha)
+    nsel = O.footprint_info(bx, by)["selected_p%d" % pc]
+    for e in rng.integers(0, nsel, 10):
+        pot, _ = O.partition_entry(bx, by, pc, int(e))
+        cols = rng.permutation(np.array([[20, 20, 20], [230, 30, 30], [30, 230, 30], [30, 30, 230]]))[:pc]
+        img = np.zeros((by * bx, 4))
+        img[:, :3] = cols[np.array(pot)] + rng.normal(0, 2.0, (bx * by, 3))
+        img[:, 3] = (np.array([255, 120, 60, 200])[np.array(pot)] if alpha else 255) + \
+            (rng.normal(0, 2.0, bx * by) if alpha else 0)
+        img = np.clip(np.rint(img), 0, 255).astype(np.uint8).reshape(by, bx, 4)
+        got = O.exhaustive_candidates(img, bx, by, 0, pc)
+        assert got[0] == e, (pc, e, got)
