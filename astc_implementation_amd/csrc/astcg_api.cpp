@@ -730,8 +730,10 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
 // LEARNED policy (texs: the textures of A, for the predictor's inputs).
 astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max_pc, uint32_t chunk_req,
                         uint32_t flags, hipStream_t s, const LabelBufs* labels = nullptr, const PredState* pred = nullptr,
-                        const std::vector<TexDesc>* texs = nullptr, bool exhaustive = false) {
-    if (pred && pred->lp.candidates == 1) exhaustive = true;
+                        const std::vector<TexDesc>* texs = nullptr, int exhaustive = 0) {
+    // the learned policy's candidates: 1 = every selected partitioning, N >= 2
+    // = the first N in k-means mismatch order (partition_search_limit)
+    if (pred && pred->lp.candidates >= 1) exhaustive = pred->lp.candidates == 1 ? kPartitionings : (int)pred->lp.candidates;
     int chunk = 0;
     astcg_status st = layout_fitting(e, F, (uint32_t)A.num_blocks, chunk_req, s, &chunk, pred);
     if (st != ASTCG_OK) return st;
@@ -758,7 +760,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
                               labels ? labels->best_seed : nullptr, labels ? labels->cand_seed : nullptr,
                               labels ? labels->cand_err : nullptr};
         put(ptrs, sizeof(ptrs));
-        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk, exhaustive ? 1 : 0};
+        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk, exhaustive};
         put(ints, sizeof(ints));
         put(&A, sizeof(A));
         if (pred) put(&pred->lp, sizeof(pred->lp));
@@ -966,7 +968,7 @@ astcg_status astcg_encode_device(astcg_encoder* e, const uint8_t* d_rgba, uint8_
         if (err != hipSuccess) return hip_fail(e, err, "fallback counter");
     }
     return run_encode(e, *e->cur, A, (int)e->params.max_partitions, e->params.chunk_blocks, e->params.flags, s,
-                      labels, pred, nullptr, e->params.policy == ASTCG_POLICY_EXHAUSTIVE);
+                      labels, pred, nullptr, e->params.policy == ASTCG_POLICY_EXHAUSTIVE ? kPartitionings : 0);
 }
 
 astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba, uint8_t* d_out, void* stream) {
@@ -1031,7 +1033,8 @@ astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba
         A.blk0 = 0;
         A.nb = n;
         err = launch_chunk(F.T, A, B, best, (int)e->params.max_partitions, s, nullptr, nullptr, nullptr,
-                           (e->params.flags & ASTCG_FLAG_DUAL_PLANE) != 0, e->params.policy == ASTCG_POLICY_EXHAUSTIVE);
+                           (e->params.flags & ASTCG_FLAG_DUAL_PLANE) != 0,
+                           e->params.policy == ASTCG_POLICY_EXHAUSTIVE ? kPartitionings : 0);
         if (err != hipSuccess) return hip_fail(e, err, "search kernels");
         err = launch_pack(F.T, A, best, s, nullptr, (e->params.flags & ASTCG_FLAG_VOID_EXTENT) != 0);
         if (err != hipSuccess) return hip_fail(e, err, "pack kernel");
@@ -1163,7 +1166,7 @@ astcg_status batch_device(astcg_encoder* e, const astcg_texture* tex, uint32_t c
             pred = &it->second;
         }
         st = run_encode(e, *F, A, (int)p.max_partitions, p.chunk_blocks, p.flags, s, nullptr, pred, &texs,
-                        p.policy == ASTCG_POLICY_EXHAUSTIVE);
+                        p.policy == ASTCG_POLICY_EXHAUSTIVE ? kPartitionings : 0);
         if (st != ASTCG_OK) return st;
     }
     return ASTCG_OK;
@@ -1333,8 +1336,8 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     if (lp) q = *lp;
     if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
         q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f) || q.rounds > 6 ||
-        q.partitioned_slots > 8 || q.candidates > 1 || (q.candidates == 1 && (q.seeds != 0 || (q.partitionings >= 1 &&
-                                                                                               q.partitionings < 4))))
+        q.partitioned_slots > 8 || q.candidates > (uint32_t)kPartitionings ||
+        (q.candidates >= 1 && (q.seeds != 0 || (q.partitionings >= 1 && q.partitionings < 4))))
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");

@@ -212,12 +212,13 @@ struct Profiler {
 void upload_quant_tables(hipStream_t s);
 // pc_seed (optional, [3][A.num_blocks]): seed of each partition count's best
 // partitioning per block, -1 when that count produced no candidate.
-// exhaustive: partition candidates from the exhaustive search of every
-// selected partitioning (k_partition_ex) instead of the reference's live
-// passes 001-007 (k_partition).
+// exhaustive > 0: partition candidates from the exhaustive search
+// (k_partition_ex) of the first `exhaustive` partitionings in k-means
+// mismatch order (a value >= 1024 covers every selected partitioning)
+// instead of the reference's live passes 001-007 (k_partition).
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels = nullptr,
-                        const SearchPlan* plan = nullptr, bool dual_plane = false, bool exhaustive = false);
+                        const SearchPlan* plan = nullptr, bool dual_plane = false, int exhaustive = 0);
 // ASTC LDR decode of a width x height image's blocks (row-major) to RGBA8.
 hipError_t launch_decode(const DevTables& T, const uint8_t* blocks, int width, int height, uint8_t* rgba,
                          hipStream_t s);

@@ -702,50 +702,83 @@ __device__ __forceinline__ void ex_normalize(const float* v, float* o) {
     }
 }
 
+// insert_result (best_partitionings.cpp:460-496) into a sorted list of 4:
+// the new entry goes before the first entry it does not exceed; written with
+// compile-time indices only (the lists stay in registers)
 __device__ __forceinline__ void ex_insert(float e, int id, float* be, int* bi) {
     if (e >= be[3]) return;
+    int at = 4;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (e > be[i]) continue;
-        for (int j = 3; j > i; --j) { be[j] = be[j - 1]; bi[j] = bi[j - 1]; }
-        be[i] = e;
-        bi[i] = id;
-        break;
-    }
+    for (int i = 3; i >= 0; --i)
+        if (!(e > be[i])) at = i;
+#pragma unroll
+    for (int j = 3; j > 0; --j)
+        if (j > at) { be[j] = be[j - 1]; bi[j] = bi[j - 1]; }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j == at) { be[j] = e; bi[j] = id; }
 }
 
-size_t partition_ex_lds_bytes(const DevTables& T) { return (size_t)T.tc * 64 * 5 + 64 * 64 * 2; }
+// Learned policy: the chunk blocks whose partition candidates count pc
+// needs (plan_uses_positions), compacted into list[0 .. *count) so that the
+// exhaustive search's waves hold only such blocks.
+__global__ void __launch_bounds__(64) k_need_list(ImageArgs A, SearchPlan P, int pc, int* list, int* count) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const bool need = k < A.nb && plan_uses_positions(P, pc, k, A.blk0 + k);
+    const unsigned long long m = __ballot(need);
+    int base = 0;
+    if (threadIdx.x == 0 && m) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0);
+    if (need) list[base + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = k;
+}
 
-template <int PC>
+// Texels staged as converted floats up to 4x4 (LDS per workgroup stays
+// small enough for 6 workgroups per CU), as RGBA8 words above.
+constexpr int kExF4MaxTc = 16;
+
+size_t partition_ex_lds_bytes(const DevTables& T) {
+    const size_t per_texel = T.tc <= kExF4MaxTc ? 16 : 4;
+    return (size_t)T.tc * 64 * (per_texel + 1);
+}
+
+template <int PC, bool kF4>
 __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, int16_t* pb_part, uint8_t* scratch,
-                                                     SearchPlan P, int use_plan) {
+                                                     const int* list, const int* list_count, int search_limit) {
     constexpr int pc = PC;
+    constexpr int kExWords = kF4 ? 1 : (kMaxTexels + 63) / 64;  // 64-texel words of a partition's texel mask
     extern __shared__ __align__(16) uint8_t smem[];
-    const int tc = T.tc, lane = threadIdx.x, k = blockIdx.x * 64 + lane, nb = A.nb;
-    uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);             // [tc][64] RGBA8
-    uint8_t* s_part = smem + (size_t)tc * 64 * 4;                      // [tc][64] k-means cluster
-    uint16_t* s_hist = reinterpret_cast<uint16_t*>(s_part + (size_t)tc * 64);  // [64][64] mismatch histogram
+    const int tc = T.tc, lane = threadIdx.x, nb = A.nb;
+    // scratch column of this lane (blocks of a list use their list slot)
+    const int col = blockIdx.x * 64 + lane;
+    const int k = list ? (col < *list_count ? list[col] : nb) : col;
+    uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);             // [tc][64] RGBA8 (!kF4)
+    float4* s_f4 = reinterpret_cast<float4*>(smem);                    // [tc][64] search values (kF4)
+    uint8_t* s_part = smem + (size_t)tc * 64 * (kF4 ? 16 : 4);         // [tc][64] k-means cluster
     const int g = A.blk0 + k;
-    const bool live = k < nb && (!use_plan || plan_uses_positions(P, pc, k, g));
-    if (!live) {
-        if (k < nb)
-            for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)c;
-        return;  // no barriers below
-    }
+    if (k >= nb) return;  // no barriers below
     const BlockLoc loc = block_loc(A, T, g);
     uint32_t amin = 0xFFu, amax = 0u;
     for (int t = 0; t < tc; ++t) {
         const uint32_t v = texel_rgba8(loc, T, t);
-        s_tex[t * 64 + lane] = v;
+        if (kF4)
+            s_f4[t * 64 + lane] = make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
+                                              u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
+        else
+            s_tex[t * 64 + lane] = v;
         amin = min(amin, v >> 24);
         amax = max(amax, v >> 24);
     }
     auto PX = [&](int t, float* o) __attribute__((always_inline)) {
-        const uint32_t v = s_tex[t * 64 + lane];
-        o[0] = u8_search_value(v & 0xFF);
-        o[1] = u8_search_value((v >> 8) & 0xFF);
-        o[2] = u8_search_value((v >> 16) & 0xFF);
-        o[3] = u8_search_value(v >> 24);
+        if (kF4) {
+            const float4 f = s_f4[t * 64 + lane];
+            o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
+        } else {
+            const uint32_t v = s_tex[t * 64 + lane];
+            o[0] = u8_search_value(v & 0xFF);
+            o[1] = u8_search_value((v >> 8) & 0xFF);
+            o[2] = u8_search_value((v >> 16) & 0xFF);
+            o[3] = u8_search_value(v >> 24);
+        }
     };
     // --- k-means: weighted-random seeding, then assign / update / assign /
     // update / assign (best_partitionings.cpp:26-194, 419-430).  The running
@@ -756,12 +789,13 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
         PX(145897 % tc, px);
         for (int c = 0; c < 4; ++c) centers[0][c] = px[c];
     }
-    int ci = 3 * (pc - 2);
+#pragma unroll
     for (int n = 1; n < pc; ++n) {
         auto dmin = [&](int t) __attribute__((always_inline)) {
             float px[4];
             PX(t, px);
             float d = ex_dist(px, centers[0]);
+#pragma unroll
             for (int j = 1; j < n; ++j) {
                 const float dj = ex_dist(px, centers[j]);
                 d = dj < d ? dj : d;
@@ -770,7 +804,7 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
         };
         float dsum = 0.0f;
         for (int t = 0; t < tc; ++t) dsum += dmin(t);
-        const float cut = dsum * kKmeansCutoffs[ci++];
+        const float cut = dsum * kKmeansCutoffs[3 * (pc - 2) + n - 1];
         float summa = 0.0f;
         int next = 0;
         for (; next < tc; ++next) {
@@ -782,6 +816,9 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
         PX(next, px);
         for (int c = 0; c < 4; ++c) centers[n][c] = px[c];
     }
+#pragma unroll
+    for (int n = pc; n < 4; ++n)
+        for (int c = 0; c < 4; ++c) centers[n][c] = 0.0f;
     for (int it = 0; it < 3; ++it) {
         if (it > 0) {
             float sum[4][4] = {};
@@ -844,9 +881,15 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
 #pragma unroll
         for (int q = 0; q < pc; ++q) a[q] |= p == q ? (1ull << i) : 0ull;
     }
-    uint8_t* mis = scratch;                                               // [nsel][nb]
-    uint16_t* order = reinterpret_cast<uint16_t*>(scratch + (((size_t)nsel * nb + 255) & ~size_t(255)));  // [nsel][nb]
-    for (int m = 0; m < 64; ++m) s_hist[m * 64 + lane] = 0;
+    // scratch columns: mismatch [nsel][nb] u8, ordering [nsel][nb] u16,
+    // mismatch histogram [64][nb] u16
+    const size_t o_order = ((size_t)nsel * nb + 255) & ~size_t(255);
+    const size_t o_hist = o_order + (((size_t)nsel * nb * 2 + 255) & ~size_t(255));
+    uint8_t* mis = scratch;
+    uint16_t* order = reinterpret_cast<uint16_t*>(scratch + o_order);
+    uint16_t* hist = reinterpret_cast<uint16_t*>(scratch + o_hist);
+#define s_hist_at(m) hist[(size_t)(m) * nb + col]
+    for (int m = 0; m < 64; ++m) s_hist_at(m) = 0;
     for (int i = 0; i < nsel; ++i) {
         const uint64_t* bb = T.coverage + ((size_t)(pc - 2) * kPartitionings + i) * 4;
         auto Pp = [&](int x, int y) { return __popcll(a[x] ^ bb[y]); };
@@ -872,24 +915,25 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
             v = min(v0, min(v1, min(v2, v3)));
         }
         const int m = v / 2;  // < 64 (half of at most 2 x 64 differing bits)
-        mis[(size_t)i * nb + k] = (uint8_t)m;
-        s_hist[m * 64 + lane]++;
-        order[(size_t)i * nb + k] = 0;
+        mis[(size_t)i * nb + col] = (uint8_t)m;
+        s_hist_at(m)++;
+        order[(size_t)i * nb + col] = 0;
     }
     {
         uint16_t sum = 0;
         for (int m = 0; m < nk; ++m) {
-            const uint16_t c = s_hist[m * 64 + lane];
-            s_hist[m * 64 + lane] = sum;
+            const uint16_t c = s_hist_at(m);
+            s_hist_at(m) = sum;
             sum = (uint16_t)(sum + c);
         }
     }
     for (int i = 0; i < nsel; ++i) {
-        const int m = mis[(size_t)i * nb + k];
-        const int pos = s_hist[m * 64 + lane];
-        s_hist[m * 64 + lane] = (uint16_t)(pos + 1);
-        if (pos < nsel) order[(size_t)pos * nb + k] = (uint16_t)i;
+        const int m = mis[(size_t)i * nb + col];
+        const int pos = s_hist_at(m);
+        s_hist_at(m) = (uint16_t)(pos + 1);
+        if (pos < nsel) order[(size_t)pos * nb + col] = (uint16_t)i;
     }
+#undef s_hist_at
     // --- line fits in mismatch order (find_best_partition_candidates,
     // compute_avgs_and_dirs_{4_comp,3_comp_rgb}, compute_error_squared_{rgba,rgb})
     const bool rgba = amin != amax;
@@ -900,20 +944,32 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
     wie = wie * wie;
     float ube[4] = {1e30f, 1e30f, 1e30f, 1e30f}, sbe[4] = {1e30f, 1e30f, 1e30f, 1e30f};
     int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
-    for (int s = 0; s < nsel; ++s) {
-        const int i = order[(size_t)s * nb + k];
+    // partition_search_limit (best_partitionings.cpp:520): the line fits
+    // cover the first search_limit partitionings of the mismatch order
+    const int nsearch = min(nsel, search_limit);
+    const int req = min(nsearch, 4);
+    for (int s = 0; s < nsearch; ++s) {
+        const int i = order[(size_t)s * nb + col];
         const size_t row = (size_t)(pc - 2) * kPartitionings + i;
         const uint8_t* pot = T.part_of_texel + row * tc;
         const uint8_t* cnts = T.part_counts + row * 4;
+        // per-partition sums in texel order, and each partition's texels as
+        // bit masks: the later passes visit only a partition's own texels,
+        // in increasing order like the reference's texels_of_partition
         float sum[PC][4] = {};
+        uint64_t mk[PC][kExWords] = {};
         for (int t = 0; t < tc; ++t) {
             float px[4];
             PX(t, px);
             const int p = pot[t];
+            const uint64_t bit = 1ull << (t & 63);
 #pragma unroll
             for (int q = 0; q < PC; ++q)
-                if (p == q)
+                if (p == q) {
                     for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
+#pragma unroll
+                    for (int wd = 0; wd < kExWords; ++wd) mk[q][wd] |= (t >> 6) == wd ? bit : 0ull;
+                }
         }
         float ue = 0.0f, se = 0.0f, len[PC], bdir[PC][4], bavg[PC][4];
 #pragma unroll
@@ -921,16 +977,18 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
             float avg[4];
             for (int c = 0; c < 4; ++c) avg[c] = sum[p][c] / (float)cnts[p];
             float sx[4] = {}, sy[4] = {}, sz[4] = {}, sw[4] = {};
-            for (int t = 0; t < tc; ++t) {
-                if (pot[t] != p) continue;
-                float px[4], d[4];
-                PX(t, px);
-                for (int c = 0; c < 4; ++c) d[c] = px[c] - avg[c];
-                if (d[0] > 0) for (int c = 0; c < 4; ++c) sx[c] += d[c];
-                if (d[1] > 0) for (int c = 0; c < 4; ++c) sy[c] += d[c];
-                if (d[2] > 0) for (int c = 0; c < 4; ++c) sz[c] += d[c];
-                if (rgba && d[3] > 0) for (int c = 0; c < 4; ++c) sw[c] += d[c];
-            }
+#pragma unroll
+            for (int wd = 0; wd < kExWords; ++wd)
+                for (uint64_t m = mk[p][wd]; m; m &= m - 1) {
+                    const int t = wd * 64 + __builtin_ctzll(m);
+                    float px[4], d[4];
+                    PX(t, px);
+                    for (int c = 0; c < 4; ++c) d[c] = px[c] - avg[c];
+                    if (d[0] > 0) for (int c = 0; c < 4; ++c) sx[c] += d[c];
+                    if (d[1] > 0) for (int c = 0; c < 4; ++c) sy[c] += d[c];
+                    if (d[2] > 0) for (int c = 0; c < 4; ++c) sz[c] += d[c];
+                    if (rgba && d[3] > 0) for (int c = 0; c < 4; ++c) sw[c] += d[c];
+                }
             float best[4];
             for (int c = 0; c < 4; ++c) best[c] = sx[c];
             if (rgba) {
@@ -953,28 +1011,30 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
             float amod[4];
             for (int c = 0; c < 4; ++c) amod[c] = avg[c] - bdir[p][c] * ud;
             float lo = 1e10f, hi = -1e10f;
-            for (int t = 0; t < tc; ++t) {
-                if (pot[t] != p) continue;
-                float px[4];
-                PX(t, px);
-                float up = (px[0] * bdir[p][0] + px[1] * bdir[p][1]) + px[2] * bdir[p][2];
-                float sp = (px[0] * bavg[p][0] + px[1] * bavg[p][1]) + px[2] * bavg[p][2];
-                if (rgba) {
-                    up = up + px[3] * bdir[p][3];
-                    sp = sp + px[3] * bavg[p][3];
+#pragma unroll
+            for (int wd = 0; wd < kExWords; ++wd)
+                for (uint64_t m = mk[p][wd]; m; m &= m - 1) {
+                    const int t = wd * 64 + __builtin_ctzll(m);
+                    float px[4];
+                    PX(t, px);
+                    float up = (px[0] * bdir[p][0] + px[1] * bdir[p][1]) + px[2] * bdir[p][2];
+                    float sp = (px[0] * bavg[p][0] + px[1] * bavg[p][1]) + px[2] * bavg[p][2];
+                    if (rgba) {
+                        up = up + px[3] * bdir[p][3];
+                        sp = sp + px[3] * bavg[p][3];
+                    }
+                    lo = up < lo ? up : lo;
+                    hi = up > hi ? up : hi;
+                    float err = 0.0f, serr = 0.0f;
+                    for (int c = 0; c < (rgba ? 4 : 3); ++c) {
+                        const float dd = (amod[c] - px[c]) + (up * bdir[p][c]);
+                        const float ds = sp * bavg[p][c] - px[c];
+                        err = c ? err + kCW[c] * dd * dd : kCW[c] * dd * dd;
+                        serr = c ? serr + kCW[c] * ds * ds : kCW[c] * ds * ds;
+                    }
+                    ue += err;
+                    se += serr;
                 }
-                lo = up < lo ? up : lo;
-                hi = up > hi ? up : hi;
-                float err = 0.0f, serr = 0.0f;
-                for (int c = 0; c < (rgba ? 4 : 3); ++c) {
-                    const float dd = (amod[c] - px[c]) + (up * bdir[p][c]);
-                    const float ds = sp * bavg[p][c] - px[c];
-                    err = c ? err + kCW[c] * dd * dd : kCW[c] * dd * dd;
-                    serr = c ? serr + kCW[c] * ds * ds : kCW[c] * ds * ds;
-                }
-                ue += err;
-                se += serr;
-            }
             const float l = hi - lo;
             len[p] = l > 1e-7f ? l : 1e-7f;
         }
@@ -993,7 +1053,7 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int idx = (j & 1) ? sbi[j >> 1] : ubi[j >> 1];
-        bool dup = emitted >= 4;
+        bool dup = emitted >= req || (j >> 1) >= req;
 #pragma unroll
         for (int q = 0; q < 4; ++q) dup |= q < emitted && pos[q] == idx;
         if (!dup) {
@@ -2912,15 +2972,20 @@ constexpr int kDualMaxTc = 64;
 size_t dual_lds_bytes(const DevTables& T) { return (size_t)2 * T.tc * 64 * sizeof(float) + 2 * 32 * 64; }
 
 // Weight of texel t as the ASTC decoder infills it: (sum of grid weight x
-// contribution in sixteenths + 8) >> 4 (contributions are multiples of 1/16).
-__device__ __forceinline__ int infill_int(const DevTables& T, int dm, const uint8_t* qw, int stride, int t) {
+// contribution in sixteenths + 8) >> 4 (contributions are multiples of 1/16);
+// mirror: of the grid weights 64 - q (the rounding does not commute with
+// the mirroring, so it is applied before the infill, as the packed block
+// holds it).
+__device__ __forceinline__ int infill_int(const DevTables& T, int dm, const uint8_t* qw, int stride, int t,
+                                          bool mirror = false) {
     const int tc = T.tc;
-    if (T.dm[dm].weight_count == tc) return qw[t * stride];
+    if (T.dm[dm].weight_count == tc) return mirror ? 64 - qw[t * stride] : qw[t * stride];
     const int n = T.t2w_count[(size_t)dm * tc + t];
     int s = 0;
-    for (int j = 0; j < n; ++j)
-        s += qw[T.t2w_idx[((size_t)dm * tc + t) * 4 + j] * stride] *
-             (int)(T.t2w_w[((size_t)dm * tc + t) * 4 + j] * 16.0f + 0.5f);
+    for (int j = 0; j < n; ++j) {
+        const int q = qw[T.t2w_idx[((size_t)dm * tc + t) * 4 + j] * stride];
+        s += (mirror ? 64 - q : q) * (int)(T.t2w_w[((size_t)dm * tc + t) * 4 + j] * 16.0f + 0.5f);
+    }
     return (s + 8) >> 4;
 }
 
@@ -2931,12 +2996,25 @@ __device__ __forceinline__ int decoded_u8(int e0, int e1, int w) {
     return (v16 * 255 + 32767) / 65535;
 }
 
-// Channel-weighted squared error, in 8-bit units, of a single-plane symbolic
-// block as the decoder reconstructs it (integer weight infill, decoded
-// endpoints, 8-bit rounding) against the block's texels.
-__device__ float decoded_block_error(const BlockLoc& loc, const DevTables& T, const SymBest& sb) {
+// Channel weights of the error metric (astc.h:60-63: 0.675, 1.3275, 0.2475,
+// 1) times 4000, so that decoded 8-bit errors compare exactly in integers.
+__device__ __constant__ uint32_t kCW4000[4] = {2700u, 5310u, 990u, 4000u};
+
+__device__ __forceinline__ uint64_t texel_err_u8(const int* dec, uint32_t v) {
+    uint64_t e = 0;
+    for (int j = 0; j < 4; ++j) {
+        const int d = dec[j] - (int)((v >> (8 * j)) & 0xFFu);
+        e += (uint64_t)kCW4000[j] * (uint64_t)(d * d);
+    }
+    return e;
+}
+
+// Channel-weighted squared error (x4000, exact), in 8-bit units, of a
+// single-plane symbolic block as the decoder reconstructs it (integer weight
+// infill, decoded endpoints, 8-bit rounding) against the block's texels.
+__device__ uint64_t decoded_block_error(const BlockLoc& loc, const DevTables& T, const SymBest& sb) {
     const int tc = T.tc, pc = sb.pc;
-    if (pc < 1 || sb.raw_mode >= 2048 || T.raw_to_bm[sb.raw_mode] < 0) return kErrDefault;
+    if (pc < 1 || sb.raw_mode >= 2048 || T.raw_to_bm[sb.raw_mode] < 0) return ~0ull;
     const int dm = T.bm[T.raw_to_bm[sb.raw_mode]].dm;
     int e0[4][4], e1[4][4];
     for (int p = 0; p < pc; ++p) {
@@ -2949,16 +3027,12 @@ __device__ float decoded_block_error(const BlockLoc& loc, const DevTables& T, co
         const int e = T.seed_to_packed[(pc - 2) * kPartitionings + sb.seed];
         pot = T.part_of_texel + ((size_t)(pc - 2) * kPartitionings + e) * tc;
     }
-    float total = 0.0f;
+    uint64_t total = 0;
     for (int t = 0; t < tc; ++t) {
         const int p = pot ? pot[t] : 0, w = infill_int(T, dm, sb.qw, 1, t);
-        const uint32_t v = texel_rgba8(loc, T, t);
-        float dd[4];
-        for (int j = 0; j < 4; ++j) {
-            const float df = (float)(decoded_u8(e0[p][j], e1[p][j], w) - (int)((v >> (8 * j)) & 0xFFu));
-            dd[j] = df * df;
-        }
-        total += dot4(dd, kCW);
+        int dec[4];
+        for (int j = 0; j < 4; ++j) dec[j] = decoded_u8(e0[p][j], e1[p][j], w);
+        total += texel_err_u8(dec, texel_rgba8(loc, T, t));
     }
     return total;
 }
@@ -2974,7 +3048,7 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
     SymBest& sb = best[g];
     // candidates compete with the searched block in what the decoder
     // reconstructs (8-bit texels), so a replaced block never decodes worse
-    float best_err = decoded_block_error(loc, T, sb);
+    uint64_t best_err = decoded_block_error(loc, T, sb);
     float* w1s = s_w + lane;
     float* w2s = s_w + tc * 64 + lane;
     uint8_t* q1s = s_q + lane;
@@ -3068,7 +3142,9 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
                 }
             }
             // infilled weight (0..64) of texel t on a plane, as the decoder forms it
-            auto infill = [&](const uint8_t* qs, int t) -> int { return infill_int(T, md.dm, qs, 64, t); };
+            auto infill = [&](const uint8_t* qs, int t, bool mirror = false) -> int {
+                return infill_int(T, md.dm, qs, 64, t, mirror);
+            };
             (void)t2wc;
             (void)t2wi;
             (void)t2ww;
@@ -3110,24 +3186,25 @@ __global__ void __launch_bounds__(64) k_dual(ImageArgs A, DevTables T, SymBest* 
             const bool swap = eq1[0] + eq1[1] + eq1[2] < eq0[0] + eq0[1] + eq0[2];
             if (swap)
                 for (int j = 0; j < 4; ++j) { const int x = eq0[j]; eq0[j] = eq1[j]; eq1[j] = x; }
-            // decoded error (8-bit units) with an early exit, and the
-            // search's error (pass17 scale) of the same reconstruction
-            float total = 0.0f, total16 = 0.0f;
+            // decoded error (8-bit units, exact integers) with an early
+            // exit, and the search's error (pass17 scale) of the same
+            // reconstruction
+            uint64_t total = 0;
+            float total16 = 0.0f;
             for (int t = 0; t < tc && total < best_err; ++t) {
-                int wt[2] = {infill(q1s, t), infill(q2s, t)};
-                if (swap) { wt[0] = 64 - wt[0]; wt[1] = 64 - wt[1]; }
+                const int wt[2] = {infill(q1s, t, swap), infill(q2s, t, swap)};
                 const uint32_t v = texel_rgba8(loc, T, t);
-                float dd[4], dd16[4];
+                float dd16[4];
+                int dec[4];
                 for (int j = 0; j < 4; ++j) {
                     const int wv = wt[j == c ? 1 : 0];
                     const uint32_t x = (v >> (8 * j)) & 0xFFu;
                     const int col = (eq0[j] * 257 * (64 - wv) + eq1[j] * 257 * wv + 32) >> 6;
                     const float df16 = wmin(fabsf(u8_search_value(x) - (float)col), 1e15f);
                     dd16[j] = df16 * df16;
-                    const float df = (float)((col * 255 + 32767) / 65535 - (int)x);
-                    dd[j] = df * df;
+                    dec[j] = (col * 255 + 32767) / 65535;
                 }
-                total += dot4(dd, kCW);
+                total += texel_err_u8(dec, v);
                 total16 += wmin(dot4(dd16, kCW), kErrDefault);
             }
             if (!(total < best_err)) continue;
@@ -3667,7 +3744,7 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
 
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan,
-                        bool dual_plane, bool exhaustive) {
+                        bool dual_plane, int exhaustive) {
     (void)hipGetLastError();  // the final check reports this call's launches only
     const LabelBufs lab = labels ? *labels : LabelBufs{nullptr, nullptr, nullptr};
     const int threads = 64;
@@ -3690,15 +3767,29 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
         if (pc > 1) {
             if (prof) prof->begin(K_PARTITION, s);
             if (exhaustive) {
-                // the mode-error buffer of this count is free until k_dm_bm
+                // the mode-error buffer of this count is free until k_dm_bm;
+                // under the learned policy only the blocks that search this
+                // count run, compacted into a list (the plan's partitioned-block
+                // list, filled by k_plan only afterwards)
                 const dim3 pgrid((A.nb + 63) / 64);
                 const size_t plds = partition_ex_lds_bytes(T);
-                const SearchPlan pplan = plan ? *plan : SearchPlan{};
-                const int use = plan ? 1 : 0;
                 uint8_t* scr = reinterpret_cast<uint8_t*>(B.ccr);
-                if (pc == 2) hipLaunchKernelGGL(k_partition_ex<2>, pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, pplan, use);
-                else if (pc == 3) hipLaunchKernelGGL(k_partition_ex<3>, pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, pplan, use);
-                else hipLaunchKernelGGL(k_partition_ex<4>, pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, pplan, use);
+                int* list = nullptr;
+                int* list_count = nullptr;
+                if (plan) {
+                    list = plan->pb_list;
+                    list_count = plan->pb_count + 3;
+                    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, s, list_count, 1);
+                    hipLaunchKernelGGL(k_need_list, pgrid, dim3(64), 0, s, A, *plan, pc, list, list_count);
+                }
+#define ASTCG_PEX(PCV, F4)                                                                                      \
+    hipLaunchKernelGGL((k_partition_ex<PCV, F4>), pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, list, list_count, \
+                       exhaustive)
+                const bool f4 = T.tc <= kExF4MaxTc;
+                if (pc == 2) { if (f4) ASTCG_PEX(2, true); else ASTCG_PEX(2, false); }
+                else if (pc == 3) { if (f4) ASTCG_PEX(3, true); else ASTCG_PEX(3, false); }
+                else { if (f4) ASTCG_PEX(4, true); else ASTCG_PEX(4, false); }
+#undef ASTCG_PEX
             } else {
                 const dim3 pgrid((A.nb + kPartBlocks - 1) / kPartBlocks);
                 const size_t plds = partition_lds_bytes(T);

@@ -88,7 +88,9 @@ typedef struct {
     uint32_t partitioned_slots; /* candidates refined per partitioned block for 2..4 partitions
                                    (1..8, 0 = slots); fallback blocks keep 8 */
     uint32_t candidates;    /* partition candidates of the searched counts (and of fallback / hard
-                               blocks): 0 = the COMPAT positions, 1 = the EXHAUSTIVE search */
+                               blocks): 0 = the COMPAT positions, 1 = the EXHAUSTIVE search over every
+                               selected partitioning, N = 2..1024: that search over the first N in k-means
+                               mismatch order (partition_search_limit, code/best_partitionings.cpp:520) */
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);

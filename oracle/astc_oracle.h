@@ -66,6 +66,7 @@ typedef struct {
     int block_count;      /* 0 = all */
     int policy;           /* partition candidates: 0 = the live passes 001-007 (compat),
                              2 = the exhaustive CPU search of best_partitionings.cpp */
+    int search_limit;     /* policy 2: partitionings line-fitted, in mismatch order (0 = all) */
 } or_params;
 
 typedef struct {

@@ -1324,7 +1324,7 @@ static void ex_insert(float e, int id, float* be, int* bi) {
  * selected partitioning (search limit = all), 4 requested candidates, with
  * the line fits of averages_and_directions.cpp (4 components when alpha
  * varies, RGB otherwise).  Writes 4 packed table entries (-1 = none). */
-static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, int* pos) {
+static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, int limit, int* pos) {
     int tc = d->texel_count;
     float wie = 0.055f;
     if (tc <= 20) wie = 0.03f;
@@ -1337,6 +1337,7 @@ static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, in
     uint16_t seq[OR_PARTITIONINGS];
     memset(seq, 0, sizeof(seq));
     int n = ex_ordering(d, b, pc, seq);
+    if (limit > 0 && limit < n) n = limit;  /* partition_search_limit, best_partitionings.cpp:520 */
     int req = n < 4 ? n : 4;
     float ube[4], sbe[4];
     int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
@@ -1432,7 +1433,7 @@ int oracle_debug_exhaustive_candidates(const uint8_t* rgba, int width, int heigh
     or_block blk;
     load_block(d, rgba, width, height, block_index % bxn, block_index / bxn, &blk);
     int pos[4];
-    exhaustive_positions(d, &blk, partition_count, pos);
+    exhaustive_positions(d, &blk, partition_count, 0, pos);
     for (int i = 0; i < 4; i++) entries_out[i] = pos[i];
     return 0;
 }
@@ -1460,6 +1461,7 @@ int oracle_encode(const uint8_t* rgba, int width, int height, int bx, int by, co
     int batch = prm && prm->batch_size > 0 ? prm->batch_size : REF_BATCH;
     int threads = prm && prm->threads > 0 ? prm->threads : 1;
     int policy = prm ? prm->policy : 0;
+    int search_limit = prm ? prm->search_limit : 0;
     int bxn = (width + bx - 1) / bx, byn = (height + by - 1) / by;
     int nb = bxn * byn;
     int first = prm ? prm->first_block : 0;
@@ -1487,7 +1489,7 @@ int oracle_encode(const uint8_t* rgba, int width, int height, int bx, int by, co
             int pos[4] = {0, -1, -1, -1};
             int npos = 1;
             if (pc > 1) {
-                if (policy == 2) exhaustive_positions(d, &blk, pc, pos);
+                if (policy == 2) exhaustive_positions(d, &blk, pc, search_limit, pos);
                 else compat_positions(d, rgba, width, height, bxn, base, B, j, pc, pos);
                 npos = 4;
             }

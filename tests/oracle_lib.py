@@ -32,7 +32,8 @@ class OrSymbolic(ctypes.Structure):
 
 class OrParams(ctypes.Structure):
     _fields_ = [("max_partitions", ctypes.c_int), ("batch_size", ctypes.c_int), ("threads", ctypes.c_int),
-                ("first_block", ctypes.c_int), ("block_count", ctypes.c_int), ("policy", ctypes.c_int)]
+                ("first_block", ctypes.c_int), ("block_count", ctypes.c_int), ("policy", ctypes.c_int),
+                ("search_limit", ctypes.c_int)]
 
 POLICY_COMPAT = 0
 POLICY_EXHAUSTIVE = 2
@@ -82,7 +83,7 @@ def lib():
 
 
 def encode(rgba, bx, by, max_partitions=4, threads=1, first_block=0, block_count=0, batch_size=920,
-           want_symbolic=True, policy=POLICY_COMPAT):
+           want_symbolic=True, policy=POLICY_COMPAT, search_limit=0):
     """Returns (blocks uint8[n*16], symbolic list or None, stats)."""
     rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
     h, w = rgba.shape[:2]
@@ -91,7 +92,7 @@ def encode(rgba, bx, by, max_partitions=4, threads=1, first_block=0, block_count
     out = np.zeros(n * 16, np.uint8)
     sym = (OrSymbolic * n)() if want_symbolic else None
     st = OrStats()
-    p = OrParams(max_partitions, batch_size, threads, first_block, block_count, policy)
+    p = OrParams(max_partitions, batch_size, threads, first_block, block_count, policy, search_limit)
     r = lib().oracle_encode(rgba.ctypes.data, w, h, bx, by, ctypes.byref(p), out.ctypes.data,
                             ctypes.cast(sym, ctypes.c_void_p) if sym is not None else None, ctypes.byref(st))
     if r != 0:

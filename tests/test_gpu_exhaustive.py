@@ -50,7 +50,7 @@ def test_exhaustive_batch_and_noalloc(encoder):
     imgs = [synth.gradient_image(70, 50, 3), synth.normal_map_image(45, 66, 4), synth.flat_noise_image(33, 29, 5)]
     fps = [(4, 4), (6, 6), (8, 8)]
     outs = encoder.encode_batch(imgs, fps, policy=A.POLICY_EXHAUSTIVE)
-    for img, (bx, by), out in zip(imgs, fps):
+    for img, (bx, by), out in zip(imgs, fps, outs):
         h, w = img.shape[:2]
         ref, _, _ = O.encode(img, bx, by, 4, threads=16, want_symbolic=False, policy=O.POLICY_EXHAUSTIVE)
         assert np.array_equal(out, ref)
@@ -62,3 +62,38 @@ def test_exhaustive_batch_and_noalloc(encoder):
         assert st == A.OK
         torch.cuda.synchronize()
         assert np.array_equal(d_out.cpu().numpy(), ref)
+
+
+def test_learned_policy_on_exhaustive_candidates(encoder):
+    """LEARNED with candidates = 1 takes its partition candidates from the
+    exhaustive search: with every block falling back it reproduces the
+    EXHAUSTIVE encode bit for bit (the compacted block list of the
+    exhaustive kernel covers exactly the blocks the plan searches)."""
+    img = synth.gradient_image(160, 120, 21)
+    encoder.configure(160, 120, 4, 4, policy=A.POLICY_EXHAUSTIVE)
+    ref = encoder.encode(img)
+    encoder.load_predictor(4, 4, fallback_below=2.0, candidates=1, early_out=0.0, count_below=0.0,
+                           full_search_above=0.0)
+    encoder.configure(160, 120, 4, 4, policy=A.POLICY_LEARNED, chunk_blocks=500)
+    assert np.array_equal(encoder.encode(img), ref)
+    assert encoder.fallback_rate() == 1.0
+    encoder.load_predictor(4, 4, candidates=1)
+    encoder.configure(160, 120, 4, 4, policy=A.POLICY_LEARNED)
+    out = encoder.encode(img)
+    dec = A.decode_image(4, 4, 160, 120, out)
+    assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
+    assert A.psnr(A.decode_image(4, 4, 160, 120, ref), img) - A.psnr(dec, img) < 0.5
+
+
+def test_learned_exhaustive_search_limit_matches_oracle(encoder):
+    """candidates = N >= 2: the exhaustive search line-fits only the first N
+    partitionings of the k-means mismatch order (the reference's
+    partition_search_limit); with every block falling back and no pruning
+    the blocks equal the oracle's EXHAUSTIVE encode with that limit."""
+    img = synth.alpha_image(96, 72, 17)
+    ref, _, _ = O.encode(img, 4, 4, 4, threads=16, want_symbolic=False, policy=O.POLICY_EXHAUSTIVE,
+                         search_limit=100)
+    encoder.load_predictor(4, 4, fallback_below=2.0, candidates=100, early_out=0.0, count_below=0.0,
+                           full_search_above=0.0)
+    encoder.configure(96, 72, 4, 4, policy=A.POLICY_LEARNED)
+    assert np.array_equal(encoder.encode(img), ref)

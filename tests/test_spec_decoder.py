@@ -100,3 +100,15 @@ def test_void_extent_blocks_decode_to_their_colour():
         spec = np.array(S.decode_block(6, 6, bytes(blk)), np.uint8).reshape(6, 6, 4)
         assert np.array_equal(ours, spec)
         assert np.array_equal(ours[0, 0], np.rint(col * 255 / 65535).astype(np.uint8))
+
+
+def test_decimation_tables_are_the_spec_infill():
+    """Every trial weight grid of every footprint: the oracle's texel ->
+    (weight, contribution) table is the specification's bilinear infill."""
+    import test_oracle_pins as T
+    for bx, by in A.FOOTPRINTS:
+        for m, (wx, wy, wc) in T.trial_dm_of(bx, by).items():
+            tc, wc2, wx2, wy2, texels = O.decimation_entry(bx, by, m)
+            assert (wx2, wy2) == (wx, wy)
+            ref = [sorted(x) for x in S.grid(bx, by, wx, wy)]
+            assert [sorted((a, b) for a, b in t if b) for t in texels] == ref, ((bx, by), m)
