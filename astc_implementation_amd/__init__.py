@@ -95,6 +95,10 @@ class LearnedParams(ctypes.Structure):
         ("rounds", ctypes.c_uint32),
         ("partitioned_slots", ctypes.c_uint32),
         ("candidates", ctypes.c_uint32),
+        ("count_above", ctypes.c_float),
+        ("full_search_modes", ctypes.c_uint32),
+        ("mismatch_candidates", ctypes.c_uint32),
+        ("fallback_above", ctypes.c_float),
     ]
 
 
@@ -303,7 +307,8 @@ class Encoder:
         for footprint bx x by; keyword arguments override the pruning
         defaults (seeds, modes, slots, fallback_below, count_below,
         partitionings, refine_partitionings, early_out, full_search_above,
-        rounds, partitioned_slots)."""
+        rounds, partitioned_slots, candidates, count_above, full_search_modes,
+        mismatch_candidates, fallback_above)."""
         flat = np.concatenate([np.asarray(w, dtype=np.float32).reshape(-1) for w in params])
         lp = LearnedParams()
         library().astcg_default_learned_params(ctypes.byref(lp))
@@ -379,6 +384,16 @@ def load_predictor_file(path):
         prune.update(rounds=int(z["prune_n"]))
     if "prune_s" in z.files:
         prune.update(partitioned_slots=int(z["prune_s"]))
+    if "prune_c" in z.files:
+        prune.update(candidates=int(z["prune_c"]))
+    if "prune_a" in z.files:
+        prune.update(count_above=float(z["prune_a"]))
+    if "prune_m" in z.files:
+        prune.update(full_search_modes=int(z["prune_m"]))
+    if "prune_k" in z.files:
+        prune.update(mismatch_candidates=int(z["prune_k"]))
+    if "prune_l" in z.files:
+        prune.update(fallback_above=float(z["prune_l"]))
     return params, int(z["hidden"]), prune
 
 

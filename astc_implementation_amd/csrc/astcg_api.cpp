@@ -561,7 +561,7 @@ typedef const uint8_t* bfloat16_ptr;    // bf16 weights addressed bytewise
 // nb blocks: offsets into the plan allocation (o_*) and the workspace (w_*).
 struct PlanLayout {
     int kmax = 4, words = 1, sub = 0;
-    bool rank = false, prune_refine = false, fused = false;
+    bool rank = false, prune_refine = false, fused = false, seed_topk = false;
     size_t o_idx = 0, o_prob = 0, o_mask = 0, o_slots = 0, o_fb = 0, o_list = 0, o_inv = 0, o_cnt = 0, o_h2all = 0,
            o_est = 0, plan_size = 0;
     size_t w_x = 0, w_z = 0, w_h = 0, w_z2 = 0, w_h2 = 0, w_l = 0, work_size = 0;
@@ -580,8 +580,9 @@ PlanLayout plan_layout(const Footprint& f, const PredState& P, int nb) {
     Y.o_fb = L.add((size_t)nb);
     Y.o_list = L.add(sizeof(int) * (size_t)nb * 4);
     Y.o_inv = L.add(sizeof(int) * (size_t)nb * 4);
-    Y.o_cnt = L.add(sizeof(int) * 4);
-    // ranking the compat positions needs every block's second hidden layer
+    Y.o_cnt = L.add(sizeof(int) * 5);  // 3 list counts, the two need-list counts
+    // ranking the compat positions, and the seed heads' top-k of the fused
+    // path, read every block's second hidden layer
     Y.rank = P.w3_seed != nullptr;
     Y.o_h2all = Y.rank ? L.add(2 * (size_t)nb * P.hidden) : 0;
     Y.prune_refine = lp.refine_partitionings >= 1 && lp.refine_partitionings < 4;
@@ -592,7 +593,8 @@ PlanLayout plan_layout(const Footprint& f, const PredState& P, int nb) {
     // launch, so its workgroups overlap their serial layer-by-layer latency;
     // the separate GEMM / GELU / top-k launches keep fp32 intermediates of the
     // 3-head output layer (about 16 KB per block) and run in sub-chunks.
-    Y.fused = P.fw[0] && lp.seeds == 0;
+    Y.fused = P.fw[0] && (lp.seeds == 0 || P.w3_seed);
+    Y.seed_topk = Y.fused && lp.seeds > 0;
     const int H = P.hidden, NO = P.nout;
     Y.sub = Y.fused ? nb : std::min(nb, kInferBlocks);
     const int usub = Y.fused ? 0 : Y.sub;  // rows of the unfused intermediates
@@ -703,7 +705,8 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
     pl.count_below = lp.count_below;
     pl.fallback = pb + o_fb;
     pl.fb_count = e->d_fb;
-    pl.keep_pb = rank ? (int)lp.partitionings : 4;
+    pl.keep_pb = (lp.seeds == 0 && lp.partitionings >= 1 && lp.partitionings < 4) ? (int)lp.partitionings : 4;
+    pl.seed_topk = Y.seed_topk ? 1 : 0;
     pl.hidden = H;
     pl.h2 = rank ? reinterpret_cast<const uint16_t*>(pb + o_h2all) : nullptr;
     pl.w3_seed = static_cast<const uint16_t*>(P.w3_seed);
@@ -717,6 +720,10 @@ astcg_status plan_chunk(astcg_encoder* e, const FpState& F, const PredState& P, 
         const float limit = powf(0.1f, db * 0.1f) * 65535.0f * 65535.0f * texels * weights_sum;
         pl.error_limit = lp.early_out > 0.0f ? lp.early_out * limit : 0.0f;
         pl.full_limit = lp.full_search_above > 0.0f ? lp.full_search_above * limit : 0.0f;
+        pl.count_limit = lp.count_above > 0.0f ? lp.count_above * limit : 0.0f;
+        pl.hard_keeps_modes = lp.full_search_modes == 1 ? 1 : 0;
+        pl.mis_candidates = lp.seeds > 4 ? (int)lp.mismatch_candidates : 0;
+        pl.line_limit = lp.fallback_above > 0.0f ? lp.fallback_above * limit : 0.0f;
     }
     pl.best = e->d_best;
     pl.iterations = lp.rounds ? (int)lp.rounds : 6;
@@ -1334,10 +1341,10 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
     astcg_learned_params q;
     astcg_default_learned_params(&q);
     if (lp) q = *lp;
-    if (q.seeds > 4 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
-        q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f) || q.rounds > 6 ||
+    if (q.seeds > 64 || q.modes < 1 || q.modes > 64 || q.slots < 1 || q.slots > 8 || q.partitionings > 4 ||
+        q.refine_partitionings > 4 || !(q.early_out >= 0.0f) || !(q.full_search_above >= 0.0f) || !(q.count_above >= 0.0f) || q.full_search_modes > 1 || q.mismatch_candidates > (uint32_t)kPartitionings || !(q.fallback_above >= 0.0f) || q.rounds > 6 ||
         q.partitioned_slots > 8 || q.candidates > (uint32_t)kPartitionings ||
-        (q.candidates >= 1 && (q.seeds != 0 || (q.partitionings >= 1 && q.partitionings < 4))))
+        ((q.candidates >= 1 || q.seeds != 0) && q.partitionings >= 1 && q.partitionings < 4))
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "learned params out of range");
     if (q.partitionings >= 1 && q.partitionings < 4 && hidden % 8 != 0)
         return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ranking partitionings needs a hidden width divisible by 8");
@@ -1393,9 +1400,11 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
                           4, sk, sk, sk, (int)std::min<uint32_t>(q.modes, (uint32_t)f->nbm)};
     if (err == hipSuccess) err = hipMalloc(&P.d_head, sizeof(head));
     if (err == hipSuccess) err = hipMemcpy(P.d_head, head, sizeof(head), hipMemcpyHostToDevice);
-    if (err == hipSuccess && q.seeds == 0 && q.partitionings >= 1 && q.partitionings < 4) {
+    const bool fused_ok = H == 256 && nin <= 256 && 4 + f->nbm <= 256;
+    if (err == hipSuccess && ((q.seeds == 0 && q.partitionings >= 1 && q.partitionings < 4) || (q.seeds > 0 && fused_ok))) {
         // seed-head columns of W3 transposed ([3 * 1024][H], one contiguous
-        // row per seed) for ranking the compat positions
+        // row per seed) for ranking the compat positions, or for the seed
+        // heads' top-k of the blocks that search a count from their seeds
         const float* w3 = weights + sizes[0] + sizes[1] + sizes[2] + sizes[3];
         std::vector<float> t((size_t)3 * 1024 * H);
         for (int c = 0; c < 3 * 1024; ++c)
@@ -1406,7 +1415,7 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
             err = hipErrorUnknown;
         if (err == hipSuccess) err = hipDeviceSynchronize();
     }
-    if (err == hipSuccess && H == 256 && nin <= 256 && 4 + f->nbm <= 256) {
+    if (err == hipSuccess && fused_ok) {
         // fused-inference copies: W1^T [256][kin], W2^T, and the count and
         // mode columns of W3 as rows of [n3p][256] (zero padding)
         const int kin = (nin + 63) / 64 * 64, n3p = (4 + f->nbm + 15) / 16 * 16;

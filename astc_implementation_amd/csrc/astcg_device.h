@@ -113,6 +113,9 @@ struct SearchPlan {
     const uint16_t* h2;         // [nb][hidden]
     const uint16_t* w3_seed;    // [3][1024][hidden] (seed-head columns, transposed)
     const float* b3_seed;       // [3][1024]
+    // the seed heads' top-k is computed per partition count for the blocks
+    // that search it from their seeds (from h2), not with the other heads
+    int seed_topk;
     // refinement pruning (keep_refine < 4): each partitioned block's best
     // estimated candidate error
     int keep_refine;
@@ -123,6 +126,12 @@ struct SearchPlan {
     // blocks whose best error after one partition is above full_limit get
     // the full search for 2..4 partitions (0 = off)
     float full_limit;
+    // blocks whose best error so far is above count_limit search every
+    // partition count (0 = off)
+    float count_limit;
+    int hard_keeps_modes;       // hard blocks keep their predicted modes and slots
+    int mis_candidates;         // seeded blocks also line-fit this many of the mismatch order first
+    float line_limit;           // blocks whose best-line error is above this fall back (0 = off)
     int iterations;             // refinement rounds per candidate (1..6)
     int part_slots;             // slots of non-fallback blocks for 2..4 partitions (1..8)
     const SymBest* best;        // running best per block (all blocks of the launch chain)

@@ -13,6 +13,7 @@
 // sequentially in texel/weight order, which fixes the summation order.
 // The file must be compiled with -ffp-contract=off.
 #include "astcg_device.h"
+#include "../../include/astcg_predictor.h"
 
 namespace astcg {
 
@@ -563,8 +564,10 @@ __device__ __forceinline__ bool plan_hard(const SearchPlan& P, int pc, int k, in
     return pc == 2 && P.full_limit > 0.0f && !P.fallback[k] && P.best[g].error > P.full_limit;
 }
 
-__device__ bool plan_keeps_count(const SearchPlan& P, int pc, int k) {
-    // pc is the count head's most likely count, or at least count_below probable
+__device__ bool plan_keeps_count(const SearchPlan& P, int pc, int k, int g) {
+    // pc is the count head's most likely count, or at least count_below
+    // probable, or the block's best error so far is above count_limit
+    if (P.count_limit > 0.0f && P.best[g].error > P.count_limit) return true;
     const int* ti = P.topk_idx + (size_t)k * 5 * P.kmax;
     const float* tp = P.topk_prob + (size_t)k * 5 * P.kmax;
     float pp = 0.0f;
@@ -578,7 +581,14 @@ __device__ bool plan_keeps_count(const SearchPlan& P, int pc, int k) {
 __device__ bool plan_uses_positions(const SearchPlan& P, int pc, int k, int g) {
     if (plan_early_out(P, g)) return false;
     if (P.fallback[k] || plan_hard(P, pc, k, g)) return true;
-    return P.seeds == 0 && plan_keeps_count(P, pc, k);
+    return P.seeds == 0 && plan_keeps_count(P, pc, k, g);
+}
+
+// Whether block k searches count pc from its predicted seeds (with more than
+// 4 seeds kept per count, ranked by their line fits).
+__device__ bool plan_uses_seeds(const SearchPlan& P, int pc, int k, int g) {
+    if (P.seeds <= 0 || plan_early_out(P, g) || P.fallback[k] || plan_hard(P, pc, k, g)) return false;
+    return plan_keeps_count(P, pc, k, g);
 }
 
 // Specialised per partition count (launched once per count), so the
@@ -719,18 +729,134 @@ __device__ __forceinline__ void ex_insert(float e, int id, float* be, int* bi) {
         if (j == at) { be[j] = e; bi[j] = id; }
 }
 
+// The uncorrelated / same-chroma line fits of one partitioning (packed entry
+// i of count pc) of a block (find_best_partition_candidates,
+// compute_avgs_and_dirs_{4_comp,3_comp_rgb}, compute_error_squared_{rgba,rgb},
+// best_partitionings.cpp:520-737): errors *ue_out, *se_out.  PX(t, px)
+// gives texel t's search values.
+template <int PC, int kExWords, class PXF>
+__device__ __forceinline__ void ex_line_fit(const PXF& PX, const DevTables& T, int pc, int i, bool rgba, float wie,
+                                            float* ue_out, float* se_out) {
+    const int tc = T.tc;
+    const size_t row = (size_t)(pc - 2) * kPartitionings + i;
+    const uint8_t* pot = T.part_of_texel + row * tc;
+    const uint8_t* cnts = T.part_counts + row * 4;
+    // per-partition sums in texel order, and each partition's texels as
+    // bit masks: the later passes visit only a partition's own texels,
+    // in increasing order like the reference's texels_of_partition
+    float sum[PC][4] = {};
+    uint64_t mk[PC][kExWords] = {};
+    for (int t = 0; t < tc; ++t) {
+        float px[4];
+        PX(t, px);
+        const int p = pot[t];
+        const uint64_t bit = 1ull << (t & 63);
+#pragma unroll
+        for (int q = 0; q < PC; ++q)
+            if (p == q) {
+                for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
+#pragma unroll
+                for (int wd = 0; wd < kExWords; ++wd) mk[q][wd] |= (t >> 6) == wd ? bit : 0ull;
+            }
+    }
+    float ue = 0.0f, se = 0.0f, len[PC], bdir[PC][4], bavg[PC][4];
+#pragma unroll
+    for (int p = 0; p < PC; ++p) {
+        float avg[4];
+        for (int c = 0; c < 4; ++c) avg[c] = sum[p][c] / (float)cnts[p];
+        float sx[4] = {}, sy[4] = {}, sz[4] = {}, sw[4] = {};
+#pragma unroll
+        for (int wd = 0; wd < kExWords; ++wd)
+            for (uint64_t m = mk[p][wd]; m; m &= m - 1) {
+                const int t = wd * 64 + __builtin_ctzll(m);
+                float px[4], d[4];
+                PX(t, px);
+                for (int c = 0; c < 4; ++c) d[c] = px[c] - avg[c];
+                if (d[0] > 0) for (int c = 0; c < 4; ++c) sx[c] += d[c];
+                if (d[1] > 0) for (int c = 0; c < 4; ++c) sy[c] += d[c];
+                if (d[2] > 0) for (int c = 0; c < 4; ++c) sz[c] += d[c];
+                if (rgba && d[3] > 0) for (int c = 0; c < 4; ++c) sw[c] += d[c];
+            }
+        float best[4];
+        for (int c = 0; c < 4; ++c) best[c] = sx[c];
+        if (rgba) {
+            float bs = dot4(sx, sx), pr = dot4(sy, sy);
+            if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sy[c]; }
+            pr = dot4(sz, sz);
+            if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sz[c]; }
+            pr = dot4(sw, sw);
+            if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sw[c]; }
+        } else {
+            float bs = dot3(sx, sx), pr = dot3(sy, sy);
+            if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sy[c]; }
+            pr = dot3(sz, sz);
+            if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sz[c]; }
+            best[3] = 0.0f;
+        }
+        ex_normalize(avg, bavg[p]);
+        ex_normalize(best, bdir[p]);
+        const float ud = rgba ? dot4(avg, bdir[p]) : dot3(avg, bdir[p]);
+        float amod[4];
+        for (int c = 0; c < 4; ++c) amod[c] = avg[c] - bdir[p][c] * ud;
+        float lo = 1e10f, hi = -1e10f;
+#pragma unroll
+        for (int wd = 0; wd < kExWords; ++wd)
+            for (uint64_t m = mk[p][wd]; m; m &= m - 1) {
+                const int t = wd * 64 + __builtin_ctzll(m);
+                float px[4];
+                PX(t, px);
+                float up = (px[0] * bdir[p][0] + px[1] * bdir[p][1]) + px[2] * bdir[p][2];
+                float sp = (px[0] * bavg[p][0] + px[1] * bavg[p][1]) + px[2] * bavg[p][2];
+                if (rgba) {
+                    up = up + px[3] * bdir[p][3];
+                    sp = sp + px[3] * bavg[p][3];
+                }
+                lo = up < lo ? up : lo;
+                hi = up > hi ? up : hi;
+                float err = 0.0f, serr = 0.0f;
+                for (int c = 0; c < (rgba ? 4 : 3); ++c) {
+                    const float dd = (amod[c] - px[c]) + (up * bdir[p][c]);
+                    const float ds = sp * bavg[p][c] - px[c];
+                    err = c ? err + kCW[c] * dd * dd : kCW[c] * dd * dd;
+                    serr = c ? serr + kCW[c] * ds * ds : kCW[c] * ds * ds;
+                }
+                ue += err;
+                se += serr;
+            }
+        const float l = hi - lo;
+        len[p] = l > 1e-7f ? l : 1e-7f;
+    }
+#pragma unroll
+    for (int p = 0; p < PC; ++p) {
+        const float ew = (float)cnts[p] * wie;
+        float uv[4], sv[4];
+        for (int c = 0; c < 4; ++c) { uv[c] = bdir[p][c] * len[p]; sv[c] = bavg[p][c] * len[p]; }
+        ue += (rgba ? dot4(uv, uv) : dot3(uv, uv)) * ew;
+        se += (rgba ? dot4(sv, sv) : dot3(sv, sv)) * ew;
+    }
+    *ue_out = ue;
+    *se_out = se;
+}
+
 // Learned policy: the chunk blocks whose partition candidates count pc
-// needs (plan_uses_positions), compacted into list[0 .. *count) so that the
-// exhaustive search's waves hold only such blocks.
-__global__ void __launch_bounds__(64) k_need_list(ImageArgs A, SearchPlan P, int pc, int* list, int* count) {
+// needs (plan_uses_positions, or with `seeded` plan_uses_seeds), compacted
+// into list[0 .. *count) so that the exhaustive search's waves hold only
+// such blocks.
+__global__ void __launch_bounds__(64) k_need_list(ImageArgs A, SearchPlan P, int pc, int seeded, int* list,
+                                                  int* count) {
     const int k = blockIdx.x * 64 + threadIdx.x;
-    const bool need = k < A.nb && plan_uses_positions(P, pc, k, A.blk0 + k);
+    const bool need = k < A.nb && (seeded ? plan_uses_seeds(P, pc, k, A.blk0 + k) : plan_uses_positions(P, pc, k, A.blk0 + k));
     const unsigned long long m = __ballot(need);
     int base = 0;
     if (threadIdx.x == 0 && m) base = atomicAdd(count, __popcll(m));
     base = __shfl(base, 0);
     if (need) list[base + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = k;
 }
+
+// Exhaustive lists longer than this run on k_partition_ex (a lane per
+// block), whose throughput per block is higher once its waves fill the
+// chip; shorter ones on k_partition_exw (a wave per block).
+constexpr int kExwMaxList = 65536;
 
 // Texels staged as converted floats up to 4x4 (LDS per workgroup stays
 // small enough for 6 workgroups per CU), as RGBA8 words above.
@@ -741,15 +867,24 @@ size_t partition_ex_lds_bytes(const DevTables& T) {
     return (size_t)T.tc * 64 * (per_texel + 1);
 }
 
-template <int PC, bool kF4>
+// kSeeded: the learned policy's variant.  The line fits cover the first
+// P.mis_candidates partitionings of the mismatch order (the k-means and
+// ordering are skipped when that is 0), then the block's P.seeds most likely
+// seeds of this count from the predictor's seed head (in rank order; seeds
+// that are not selected partitionings or were already fitted are skipped),
+// and the 4 best are kept the same way.
+template <int PC, bool kF4, bool kSeeded>
 __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, int16_t* pb_part, uint8_t* scratch,
-                                                     const int* list, const int* list_count, int search_limit) {
+                                                     const int* list, const int* list_count, int search_limit,
+                                                     SearchPlan P) {
     constexpr int pc = PC;
     constexpr int kExWords = kF4 ? 1 : (kMaxTexels + 63) / 64;  // 64-texel words of a partition's texel mask
     extern __shared__ __align__(16) uint8_t smem[];
     const int tc = T.tc, lane = threadIdx.x, nb = A.nb;
-    // scratch column of this lane (blocks of a list use their list slot)
+    // scratch column of this lane (blocks of a list use their list slot;
+    // short exhaustive lists are k_partition_exw's)
     const int col = blockIdx.x * 64 + lane;
+    if (list && !kSeeded && *list_count <= kExwMaxList) return;
     const int k = list ? (col < *list_count ? list[col] : nb) : col;
     uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);             // [tc][64] RGBA8 (!kF4)
     float4* s_f4 = reinterpret_cast<float4*>(smem);                    // [tc][64] search values (kF4)
@@ -780,6 +915,13 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
             o[3] = u8_search_value(v >> 24);
         }
     };
+    const int nsel = T.count_selected[pc - 1];
+    uint16_t* order = nullptr;
+    // seeded: the first mis_n partitionings of the mismatch order are
+    // line-fitted before the seeds (0 = the seeds alone)
+    const int mis_n = kSeeded ? min(P.mis_candidates, nsel) : 0;
+    uint8_t* mis = scratch;
+    if (!kSeeded || mis_n > 0) {
     // --- k-means: weighted-random seeding, then assign / update / assign /
     // update / assign (best_partitionings.cpp:26-194, 419-430).  The running
     // minimum distances are recomputed from the chosen centres (a min is exact)
@@ -874,7 +1016,7 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
     // (best_partitionings.cpp:325-397); the histogram's running sum stops
     // at the k-means texel count as in the reference, and ordering slots it
     // leaves unwritten read partitioning 0
-    const int nk = min(tc, 64), nsel = T.count_selected[pc - 1];
+    const int nk = min(tc, 64);
     uint64_t a[4] = {0, 0, 0, 0};
     for (int i = 0; i < nk; ++i) {
         const int p = s_part[T.kmeans_texels[i] * 64 + lane];
@@ -885,8 +1027,7 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
     // mismatch histogram [64][nb] u16
     const size_t o_order = ((size_t)nsel * nb + 255) & ~size_t(255);
     const size_t o_hist = o_order + (((size_t)nsel * nb * 2 + 255) & ~size_t(255));
-    uint8_t* mis = scratch;
-    uint16_t* order = reinterpret_cast<uint16_t*>(scratch + o_order);
+    order = reinterpret_cast<uint16_t*>(scratch + o_order);
     uint16_t* hist = reinterpret_cast<uint16_t*>(scratch + o_hist);
 #define s_hist_at(m) hist[(size_t)(m) * nb + col]
     for (int m = 0; m < 64; ++m) s_hist_at(m) = 0;
@@ -932,8 +1073,12 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
         const int pos = s_hist_at(m);
         s_hist_at(m) = (uint16_t)(pos + 1);
         if (pos < nsel) order[(size_t)pos * nb + col] = (uint16_t)i;
+        // seeded: mark the partitionings line-fitted from the order, so a
+        // seed among them is not fitted twice
+        if (kSeeded && pos < mis_n) mis[(size_t)i * nb + col] = (uint8_t)(m | 0x80);
     }
 #undef s_hist_at
+    }  // !kSeeded || mis_n > 0
     // --- line fits in mismatch order (find_best_partition_candidates,
     // compute_avgs_and_dirs_{4_comp,3_comp_rgb}, compute_error_squared_{rgba,rgb})
     const bool rgba = amin != amax;
@@ -946,109 +1091,27 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
     int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
     // partition_search_limit (best_partitionings.cpp:520): the line fits
     // cover the first search_limit partitionings of the mismatch order
-    const int nsearch = min(nsel, search_limit);
-    const int req = min(nsearch, 4);
+    const int* seeds = kSeeded ? P.topk_idx + (size_t)k * 5 * P.kmax + (size_t)(pc - 1) * P.kmax : nullptr;
+    const int nsearch = kSeeded ? mis_n + min(P.seeds, P.kmax) : min(nsel, search_limit);
+    int nvalid = 0;
     for (int s = 0; s < nsearch; ++s) {
-        const int i = order[(size_t)s * nb + col];
-        const size_t row = (size_t)(pc - 2) * kPartitionings + i;
-        const uint8_t* pot = T.part_of_texel + row * tc;
-        const uint8_t* cnts = T.part_counts + row * 4;
-        // per-partition sums in texel order, and each partition's texels as
-        // bit masks: the later passes visit only a partition's own texels,
-        // in increasing order like the reference's texels_of_partition
-        float sum[PC][4] = {};
-        uint64_t mk[PC][kExWords] = {};
-        for (int t = 0; t < tc; ++t) {
-            float px[4];
-            PX(t, px);
-            const int p = pot[t];
-            const uint64_t bit = 1ull << (t & 63);
-#pragma unroll
-            for (int q = 0; q < PC; ++q)
-                if (p == q) {
-                    for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
-#pragma unroll
-                    for (int wd = 0; wd < kExWords; ++wd) mk[q][wd] |= (t >> 6) == wd ? bit : 0ull;
-                }
+        int i;
+        if (!kSeeded || s < mis_n) {
+            i = order[(size_t)s * nb + col];
+        } else {
+            const int seed = seeds[s - mis_n];
+            if (seed < 0 || seed >= kPartitionings) break;  // the head has no more columns
+            i = T.seed_to_packed[(pc - 2) * kPartitionings + seed];
+            if (i >= nsel) continue;  // not a selected partitioning of this count
+            if (mis_n > 0 && (mis[(size_t)i * nb + col] & 0x80)) continue;  // fitted from the order
         }
-        float ue = 0.0f, se = 0.0f, len[PC], bdir[PC][4], bavg[PC][4];
-#pragma unroll
-        for (int p = 0; p < PC; ++p) {
-            float avg[4];
-            for (int c = 0; c < 4; ++c) avg[c] = sum[p][c] / (float)cnts[p];
-            float sx[4] = {}, sy[4] = {}, sz[4] = {}, sw[4] = {};
-#pragma unroll
-            for (int wd = 0; wd < kExWords; ++wd)
-                for (uint64_t m = mk[p][wd]; m; m &= m - 1) {
-                    const int t = wd * 64 + __builtin_ctzll(m);
-                    float px[4], d[4];
-                    PX(t, px);
-                    for (int c = 0; c < 4; ++c) d[c] = px[c] - avg[c];
-                    if (d[0] > 0) for (int c = 0; c < 4; ++c) sx[c] += d[c];
-                    if (d[1] > 0) for (int c = 0; c < 4; ++c) sy[c] += d[c];
-                    if (d[2] > 0) for (int c = 0; c < 4; ++c) sz[c] += d[c];
-                    if (rgba && d[3] > 0) for (int c = 0; c < 4; ++c) sw[c] += d[c];
-                }
-            float best[4];
-            for (int c = 0; c < 4; ++c) best[c] = sx[c];
-            if (rgba) {
-                float bs = dot4(sx, sx), pr = dot4(sy, sy);
-                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sy[c]; }
-                pr = dot4(sz, sz);
-                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sz[c]; }
-                pr = dot4(sw, sw);
-                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sw[c]; }
-            } else {
-                float bs = dot3(sx, sx), pr = dot3(sy, sy);
-                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sy[c]; }
-                pr = dot3(sz, sz);
-                if (pr > bs) { bs = pr; for (int c = 0; c < 4; ++c) best[c] = sz[c]; }
-                best[3] = 0.0f;
-            }
-            ex_normalize(avg, bavg[p]);
-            ex_normalize(best, bdir[p]);
-            const float ud = rgba ? dot4(avg, bdir[p]) : dot3(avg, bdir[p]);
-            float amod[4];
-            for (int c = 0; c < 4; ++c) amod[c] = avg[c] - bdir[p][c] * ud;
-            float lo = 1e10f, hi = -1e10f;
-#pragma unroll
-            for (int wd = 0; wd < kExWords; ++wd)
-                for (uint64_t m = mk[p][wd]; m; m &= m - 1) {
-                    const int t = wd * 64 + __builtin_ctzll(m);
-                    float px[4];
-                    PX(t, px);
-                    float up = (px[0] * bdir[p][0] + px[1] * bdir[p][1]) + px[2] * bdir[p][2];
-                    float sp = (px[0] * bavg[p][0] + px[1] * bavg[p][1]) + px[2] * bavg[p][2];
-                    if (rgba) {
-                        up = up + px[3] * bdir[p][3];
-                        sp = sp + px[3] * bavg[p][3];
-                    }
-                    lo = up < lo ? up : lo;
-                    hi = up > hi ? up : hi;
-                    float err = 0.0f, serr = 0.0f;
-                    for (int c = 0; c < (rgba ? 4 : 3); ++c) {
-                        const float dd = (amod[c] - px[c]) + (up * bdir[p][c]);
-                        const float ds = sp * bavg[p][c] - px[c];
-                        err = c ? err + kCW[c] * dd * dd : kCW[c] * dd * dd;
-                        serr = c ? serr + kCW[c] * ds * ds : kCW[c] * ds * ds;
-                    }
-                    ue += err;
-                    se += serr;
-                }
-            const float l = hi - lo;
-            len[p] = l > 1e-7f ? l : 1e-7f;
-        }
-#pragma unroll
-        for (int p = 0; p < PC; ++p) {
-            const float ew = (float)cnts[p] * wie;
-            float uv[4], sv[4];
-            for (int c = 0; c < 4; ++c) { uv[c] = bdir[p][c] * len[p]; sv[c] = bavg[p][c] * len[p]; }
-            ue += (rgba ? dot4(uv, uv) : dot3(uv, uv)) * ew;
-            se += (rgba ? dot4(sv, sv) : dot3(sv, sv)) * ew;
-        }
+        ++nvalid;
+        float ue, se;
+        ex_line_fit<PC, kExWords>(PX, T, pc, i, rgba, wie, &ue, &se);
         ex_insert(ue, i, ube, ubi);
         ex_insert(se, i, sbe, sbi);
     }
+    const int req = min(nvalid, 4);
     int pos[4] = {-1, -1, -1, -1}, emitted = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -1064,8 +1127,9 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
         }
     }
     // a missing candidate (fewer than 4 distinct) repeats the first one: a
-    // duplicate search never beats the first under the strict < of the picks
-    for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)(pos[c] >= 0 ? pos[c] : pos[0]);
+    // duplicate search never beats the first under the strict < of the
+    // picks.  Seeded: -1 (k_plan lists only the candidates found)
+    for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)(kSeeded || pos[c] >= 0 ? pos[c] : pos[0]);
 }
 
 // ---- pass01 + pass08 + pass09 + pass10 --------------------------------------
@@ -3250,12 +3314,75 @@ __global__ void __launch_bounds__(64) k_fb_reset(unsigned long long* fb_count) {
     if (threadIdx.x == 0) *fb_count = 0ull;
 }
 
+// The weighted squared distance of a block's texels to their best line (the
+// average plus the direction of the largest positive-deviation sum, as the
+// partition search's line fits choose it), over RGBA when alpha varies, RGB
+// otherwise: what one partition cannot encode whatever its endpoints.  A
+// decision heuristic of the learned policy, no counterpart in the reference.
+__device__ float block_line_error(const ImageArgs& A, const DevTables& T, int g) {
+    const BlockLoc loc = block_loc(A, T, g);
+    const int tc = T.tc;
+    float avg[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t amin = 0xFFu, amax = 0u;
+    for (int t = 0; t < tc; ++t) {
+        float px[4];
+        texel_px(loc, T, t, px);
+        for (int c = 0; c < 4; ++c) avg[c] += px[c];
+        const uint32_t a = texel_rgba8(loc, T, t) >> 24;
+        amin = min(amin, a);
+        amax = max(amax, a);
+    }
+    for (int c = 0; c < 4; ++c) avg[c] *= 1.0f / (float)tc;
+    const int nch = amin != amax ? 4 : 3;
+    float sx[4] = {}, sy[4] = {}, sz[4] = {}, sw[4] = {};
+    for (int t = 0; t < tc; ++t) {
+        float px[4], d[4];
+        texel_px(loc, T, t, px);
+        for (int c = 0; c < 4; ++c) d[c] = px[c] - avg[c];
+        if (d[0] > 0) for (int c = 0; c < 4; ++c) sx[c] += d[c];
+        if (d[1] > 0) for (int c = 0; c < 4; ++c) sy[c] += d[c];
+        if (d[2] > 0) for (int c = 0; c < 4; ++c) sz[c] += d[c];
+        if (nch == 4 && d[3] > 0) for (int c = 0; c < 4; ++c) sw[c] += d[c];
+    }
+    float dir[4], best = -1.0f;
+    auto pick = [&](const float* v) {
+        float n = 0.0f;
+        for (int c = 0; c < nch; ++c) n += v[c] * v[c];
+        if (n > best) {
+            best = n;
+            for (int c = 0; c < 4; ++c) dir[c] = c < nch ? v[c] : 0.0f;
+        }
+    };
+    pick(sx);
+    pick(sy);
+    pick(sz);
+    if (nch == 4) pick(sw);
+    const float inv = best > 0.0f ? 1.0f / sqrtf(best) : 0.0f;
+    for (int c = 0; c < 4; ++c) dir[c] *= inv;
+    float err = 0.0f;
+    for (int t = 0; t < tc; ++t) {
+        float px[4], d[4];
+        texel_px(loc, T, t, px);
+        float u = 0.0f;
+        for (int c = 0; c < nch; ++c) {
+            d[c] = px[c] - avg[c];
+            u += d[c] * dir[c];
+        }
+        for (int c = 0; c < nch; ++c) {
+            const float r = d[c] - u * dir[c];
+            err += kCW[c] * r * r;
+        }
+    }
+    return err;
+}
+
 // One lane per block for the decisions; the blocks' mode-head entries are
 // first staged through LDS 16 at a time with contiguous loads (a block's
 // top-k record is 5 * kmax entries, so lane-per-block loads would touch one
 // cache line per lane).  The kept probability is still summed per block in
 // rank order.
-__global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, int nbm, uint32_t* mask, uint8_t* slots) {
+__global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, DevTables T, SearchPlan P, int nbm, uint32_t* mask,
+                                                   uint8_t* slots) {
     __shared__ int s_idx[64 * 17];    // [block][16 entries], rows padded to 17 words
     __shared__ float s_prob[64 * 17];
     const int base = blockIdx.x * 64, lane = threadIdx.x, k = base + lane;
@@ -3284,7 +3411,10 @@ __global__ void __launch_bounds__(64) k_plan_modes(ImageArgs A, SearchPlan P, in
             for (int j = 0; j < nj; ++j)
                 if (s_idx[lane * 17 + j] >= 0) kept += s_prob[lane * 17 + j];
     }
-    const bool fb = live && kept < P.fallback_below;
+    // blocks the pruning should not touch: low confidence, or (line_limit)
+    // more error off their best line than the error limit allows
+    const bool fb = live && (kept < P.fallback_below ||
+                             (P.line_limit > 0.0f && block_line_error(A, T, A.blk0 + k) > P.line_limit));
     // one atomic per wave for the fallback statistics
     const unsigned long long wave_fb = __ballot(fb);
     if (lane == 0 && wave_fb) atomicAdd(P.fb_count, (unsigned long long)__popcll(wave_fb));
@@ -3376,23 +3506,30 @@ __global__ void __launch_bounds__(64) k_plan(ImageArgs A, DevTables T, int pc, i
         if (plan_early_out(P, g)) {
             n = 0;  // early out: the block's best encoding so far is good enough
         } else if (plan_hard(P, pc, k, g)) {
-            // a hard block: the full search from 2 partitions on (all trial
-            // modes, 8 slots; later counts see the fallback flag)
+            // a hard block: the full search from 2 partitions on (every
+            // count, the full partition candidates, and unless
+            // hard_keeps_modes all trial modes and 8 slots; later counts see
+            // the fallback flag)
             P.fallback[k] = 1;
             atomicAdd(P.fb_count, 1ull);
-            uint32_t* m = const_cast<uint32_t*>(P.mode_mask) + (size_t)k * P.mask_words;
-            for (int w = 0; w < P.mask_words; ++w) {
-                const int lo = 32 * w;
-                m[w] = T.nbm - lo >= 32 ? 0xFFFFFFFFu : (T.nbm > lo ? (1u << (T.nbm - lo)) - 1u : 0u);
+            if (!P.hard_keeps_modes) {
+                uint32_t* m = const_cast<uint32_t*>(P.mode_mask) + (size_t)k * P.mask_words;
+                for (int w = 0; w < P.mask_words; ++w) {
+                    const int lo = 32 * w;
+                    m[w] = T.nbm - lo >= 32 ? 0xFFFFFFFFu : (T.nbm > lo ? (1u << (T.nbm - lo)) - 1u : 0u);
+                }
+                const_cast<uint8_t*>(P.slots)[k] = 8;
             }
-            const_cast<uint8_t*>(P.slots)[k] = 8;
             n = 4;
         } else if (P.fallback[k]) {
             n = 4;
-        } else if (plan_keeps_count(P, pc, k)) {
+        } else if (plan_keeps_count(P, pc, k, g)) {
             if (P.seeds == 0) {
                 // the compat positions k_partition wrote
                 n = P.keep_pb < 4 ? rank_positions(T, P, pc, k, pb_part) : 4;
+            } else if (P.seeds > 4) {
+                // the best of its seeds by line fit (k_partition_ex), -1 after the last
+                while (n < 4 && pb_part[k * 4 + n] >= 0) ++n;
             } else {
                 for (int j = 0; j < P.kmax && j < P.seeds && n < 4; ++j) {
                     const int seed = ti[(pc - 1) * P.kmax + j];
@@ -3742,6 +3879,286 @@ void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, con
     else launch_refine_pc<false>(pc, grid, lds, s, A, T, B, npb, iterations);
 }
 
+// ---- exhaustive partition candidates of a short block list ----------------
+// The same search as k_partition_ex (bit for bit: every float operation in
+// the same order), one wave per listed block instead of one lane: the lanes
+// share the block's texels in LDS and split the texel-parallel steps (k-means
+// assignment, the k-means texels' cluster masks) and the partitioning-parallel
+// ones (mismatch counts, the counting sort's placement, the line fits); the
+// sequential steps (k-means seeding and centre sums, the prefix of the
+// histogram, the top-4 insertion in mismatch order) run on every lane alike.
+// The workgroups loop over the list, so a list of a few hundred blocks (the
+// learned policy's hard and fallback blocks) fills the chip.
+constexpr int kExwGrid = 4096;
+
+size_t partition_exw_lds_bytes(const DevTables& T) {
+    return (size_t)T.tc * 16 + (((size_t)T.tc + 15) & ~size_t(15)) + kPartitionings * (1 + 4 + 4 + 4) + 2 * 64 * 4;
+}
+
+template <int PC, int kW>
+__global__ void __launch_bounds__(64) k_partition_exw(ImageArgs A, DevTables T, int16_t* pb_part, const int* list,
+                                                      const int* list_count, int search_limit) {
+    constexpr int pc = PC;
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tc = T.tc, lane = threadIdx.x;
+    float4* s_px = reinterpret_cast<float4*>(smem);                                    // [tc]
+    uint8_t* s_part = smem + (size_t)tc * 16;                                          // [tc] k-means cluster
+    uint8_t* s_mis = s_part + ((tc + 15) & ~15);                                       // [1024] mismatch / 2
+    int* s_order = reinterpret_cast<int*>(s_mis + kPartitionings);                    // [1024]
+    float* s_eu = reinterpret_cast<float*>(s_order + kPartitionings);                 // [1024]
+    float* s_es = s_eu + kPartitionings;                                               // [1024]
+    int* s_start = reinterpret_cast<int*>(s_es + kPartitionings);                     // [64]
+    int* s_hist = s_start + 64;                                                        // [64]
+    const int n = *list_count, nsel = T.count_selected[pc - 1], nk = min(tc, 64);
+    if (n > kExwMaxList) return;  // k_partition_ex takes this list
+    auto PX = [&](int t, float* o) __attribute__((always_inline)) {
+        const float4 f = s_px[t];
+        o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
+    };
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const int k = list[li];
+        const BlockLoc loc = block_loc(A, T, A.blk0 + k);
+        uint32_t amin = 0xFFu, amax = 0u;
+        for (int t = lane; t < tc; t += 64) {
+            const uint32_t v = texel_rgba8(loc, T, t);
+            s_px[t] = make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
+                                  u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
+            amin = min(amin, v >> 24);
+            amax = max(amax, v >> 24);
+        }
+        for (int m = 32; m > 0; m >>= 1) {
+            amin = min(amin, (uint32_t)__shfl_xor((int)amin, m));
+            amax = max(amax, (uint32_t)__shfl_xor((int)amax, m));
+        }
+        if (lane < 64) s_hist[lane] = 0;
+        __syncthreads();
+        // --- k-means (best_partitionings.cpp:26-194, 419-430): seeding and
+        // centre sums in texel order on every lane, assignment per texel
+        float centers[4][4];
+        {
+            float px[4];
+            PX(145897 % tc, px);
+            for (int c = 0; c < 4; ++c) centers[0][c] = px[c];
+        }
+#pragma unroll
+        for (int q = 1; q < pc; ++q) {
+            auto dmin = [&](int t) __attribute__((always_inline)) {
+                float px[4];
+                PX(t, px);
+                float d = ex_dist(px, centers[0]);
+#pragma unroll
+                for (int j = 1; j < q; ++j) {
+                    const float dj = ex_dist(px, centers[j]);
+                    d = dj < d ? dj : d;
+                }
+                return d;
+            };
+            float dsum = 0.0f;
+            for (int t = 0; t < tc; ++t) dsum += dmin(t);
+            const float cut = dsum * kKmeansCutoffs[3 * (pc - 2) + q - 1];
+            float summa = 0.0f;
+            int next = 0;
+            for (; next < tc; ++next) {
+                summa += dmin(next);
+                if (summa >= cut) break;
+            }
+            next = min(next, tc - 1);
+            float px[4];
+            PX(next, px);
+            for (int c = 0; c < 4; ++c) centers[q][c] = px[c];
+        }
+#pragma unroll
+        for (int q = pc; q < 4; ++q)
+            for (int c = 0; c < 4; ++c) centers[q][c] = 0.0f;
+        for (int it = 0; it < 3; ++it) {
+            if (it > 0) {
+                float sum[4][4] = {};
+                int cnt[4] = {0, 0, 0, 0};
+                for (int t = 0; t < tc; ++t) {
+                    float px[4];
+                    PX(t, px);
+                    const int p = s_part[t];
+#pragma unroll
+                    for (int q = 0; q < pc; ++q)
+                        if (p == q) {
+                            for (int c = 0; c < 4; ++c) sum[q][c] += px[c];
+                            cnt[q]++;
+                        }
+                }
+#pragma unroll
+                for (int q = 0; q < pc; ++q) {
+                    const float sc = cnt[q] ? 1.0f / (float)cnt[q] : 0.0f;
+                    for (int c = 0; c < 4; ++c) centers[q][c] = cnt[q] ? sum[q][c] * sc : 0.0f;
+                }
+                __syncthreads();  // every lane has read s_part
+            }
+            int cnt[4] = {0, 0, 0, 0};
+            for (int t0 = 0; t0 < tc; t0 += 64) {
+                const int t = t0 + lane;
+                int bp = -1;
+                if (t < tc) {
+                    float px[4];
+                    PX(t, px);
+                    float bd = 3.402823466e38f;
+                    bp = 0;
+#pragma unroll
+                    for (int q = 0; q < pc; ++q) {
+                        const float d = ex_dist(px, centers[q]);
+                        if (d < bd) { bd = d; bp = q; }
+                    }
+                    s_part[t] = (uint8_t)bp;
+                }
+#pragma unroll
+                for (int q = 0; q < pc; ++q) cnt[q] += __popcll(__ballot(bp == q));
+            }
+            __syncthreads();
+            bool problem;
+            do {
+                problem = false;
+#pragma unroll
+                for (int q = 0; q < pc; ++q)
+                    if (cnt[q] == 0) {
+                        problem = true;
+                        const int old = s_part[q];
+#pragma unroll
+                        for (int r = 0; r < pc; ++r) cnt[r] -= r == old;
+                        cnt[q]++;
+                        __syncthreads();  // every lane has read s_part[q]
+                        if (lane == 0) s_part[q] = (uint8_t)q;
+                        __syncthreads();
+                    }
+            } while (problem);
+        }
+        // --- mismatch of every selected partitioning, counting sort
+        // (best_partitionings.cpp:325-397)
+        uint64_t a[4] = {0, 0, 0, 0};
+        {
+            const int p = lane < nk ? s_part[T.kmeans_texels[lane]] : -1;
+#pragma unroll
+            for (int q = 0; q < pc; ++q) a[q] = __ballot(p == q);
+        }
+        for (int i = lane; i < nsel; i += 64) {
+            const uint64_t* bb = T.coverage + ((size_t)(pc - 2) * kPartitionings + i) * 4;
+            auto Pp = [&](int x, int y) { return __popcll(a[x] ^ bb[y]); };
+            int v;
+            if (pc == 2) {
+                v = min(Pp(0, 0) + Pp(1, 1), Pp(0, 1) + Pp(1, 0));
+            } else if (pc == 3) {
+                const int v0 = min(Pp(1, 1) + Pp(2, 2), Pp(1, 2) + Pp(2, 1)) + Pp(0, 0);
+                const int v1 = min(Pp(1, 0) + Pp(2, 2), Pp(1, 2) + Pp(2, 0)) + Pp(0, 1);
+                const int v2 = min(Pp(1, 0) + Pp(2, 1), Pp(1, 1) + Pp(2, 0)) + Pp(0, 2);
+                v = min(v0, min(v1, v2));
+            } else {
+                const int mx23 = min(Pp(2, 2) + Pp(3, 3), Pp(2, 3) + Pp(3, 2));
+                const int mx13 = min(Pp(2, 1) + Pp(3, 3), Pp(2, 3) + Pp(3, 1));
+                const int mx12 = min(Pp(2, 1) + Pp(3, 2), Pp(2, 2) + Pp(3, 1));
+                const int mx03 = min(Pp(2, 0) + Pp(3, 3), Pp(2, 3) + Pp(3, 0));
+                const int mx02 = min(Pp(2, 0) + Pp(3, 2), Pp(2, 2) + Pp(3, 0));
+                const int mx01 = min(Pp(2, 1) + Pp(3, 0), Pp(2, 0) + Pp(3, 1));
+                const int v0 = Pp(0, 0) + min(Pp(1, 1) + mx23, min(Pp(1, 2) + mx13, Pp(1, 3) + mx12));
+                const int v1 = Pp(0, 1) + min(Pp(1, 0) + mx23, min(Pp(1, 2) + mx03, Pp(1, 3) + mx02));
+                const int v2 = Pp(0, 2) + min(Pp(1, 1) + mx03, min(Pp(1, 0) + mx13, Pp(1, 3) + mx01));
+                const int v3 = Pp(0, 3) + min(Pp(1, 1) + mx02, min(Pp(1, 2) + mx01, Pp(1, 0) + mx12));
+                v = min(v0, min(v1, min(v2, v3)));
+            }
+            const int m = v / 2;  // < 64
+            s_mis[i] = (uint8_t)m;
+            atomicAdd(&s_hist[m], 1);
+            s_order[i] = 0;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            // the running sum stops at the k-means texel count as in the
+            // reference; buckets above keep their count as the start
+            int sum = 0;
+            for (int m = 0; m < 64; ++m) {
+                const int c = s_hist[m];
+                s_start[m] = m < nk ? sum : c;
+                if (m < nk) sum += c;
+            }
+        }
+        __syncthreads();
+        // stable placement, 64 partitionings at a time in index order: a
+        // lane's slot is its bucket's start plus the lanes of the same
+        // bucket before it; a slot written twice keeps the later index
+        for (int i0 = 0; i0 < nsel; i0 += 64) {
+            const int i = i0 + lane;
+            const bool valid = i < nsel;
+            const int m = valid ? s_mis[i] : 0;
+            uint64_t same = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+                const uint64_t bm = __ballot(valid && ((m >> b) & 1));
+                same &= ((m >> b) & 1) ? bm : ~bm;
+            }
+            const int rank = __popcll(same & ((1ull << lane) - 1ull));
+            const int pos = valid ? s_start[m] + rank : 0;
+            __syncthreads();
+            if (valid && rank == __popcll(same) - 1) s_start[m] += __popcll(same);
+            if (valid && pos < nsel) atomicMax(&s_order[pos], i);
+            __syncthreads();
+        }
+        // --- line fits of the first search_limit partitionings in that order
+        const bool rgba = amin != amax;
+        float wie = 0.055f;
+        if (tc <= 20) wie = 0.03f;
+        else if (tc <= 31) wie = 0.04f;
+        else if (tc <= 41) wie = 0.05f;
+        wie = wie * wie;
+        const int nsearch = min(nsel, search_limit);
+        for (int sidx = lane; sidx < nsearch; sidx += 64) {
+            float ue, se;
+            ex_line_fit<PC, kW>(PX, T, pc, s_order[sidx], rgba, wie, &ue, &se);
+            s_eu[sidx] = ue;
+            s_es[sidx] = se;
+        }
+        __syncthreads();
+        // --- insert_result in mismatch order (every lane alike)
+        float ube[4] = {1e30f, 1e30f, 1e30f, 1e30f}, sbe[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+        int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
+        for (int sidx = 0; sidx < nsearch; ++sidx) {
+            const int i = s_order[sidx];
+            ex_insert(s_eu[sidx], i, ube, ubi);
+            ex_insert(s_es[sidx], i, sbe, sbi);
+        }
+        const int req = min(nsearch, 4);
+        int pos[4] = {-1, -1, -1, -1}, emitted = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int idx = (j & 1) ? sbi[j >> 1] : ubi[j >> 1];
+            bool dup = emitted >= req || (j >> 1) >= req;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dup |= q < emitted && pos[q] == idx;
+            if (!dup) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q == emitted) pos[q] = idx;
+                ++emitted;
+            }
+        }
+        if (lane == 0)
+            for (int c = 0; c < 4; ++c) pb_part[k * 4 + c] = (int16_t)(pos[c] >= 0 ? pos[c] : pos[0]);
+        __syncthreads();  // the next block reuses the LDS
+    }
+}
+
+void launch_partition_ex(bool seeded, int pc, const DevTables& T, dim3 grid, size_t lds, hipStream_t s,
+                         const ImageArgs& A, int16_t* pb_part, uint8_t* scratch, const int* list, const int* list_count,
+                         int search_limit, const SearchPlan& P) {
+#define ASTCG_PEX(PCV, F4, SD)                                                                                       \
+    hipLaunchKernelGGL((k_partition_ex<PCV, F4, SD>), grid, dim3(64), lds, s, A, T, pb_part, scratch, list, list_count, \
+                       search_limit, P)
+#define ASTCG_PEX_PC(SD)                                                 \
+    if (pc == 2) { if (f4) ASTCG_PEX(2, true, SD); else ASTCG_PEX(2, false, SD); } \
+    else if (pc == 3) { if (f4) ASTCG_PEX(3, true, SD); else ASTCG_PEX(3, false, SD); } \
+    else { if (f4) ASTCG_PEX(4, true, SD); else ASTCG_PEX(4, false, SD); }
+    const bool f4 = T.tc <= kExF4MaxTc;
+    if (seeded) { ASTCG_PEX_PC(true) }
+    else { ASTCG_PEX_PC(false) }
+#undef ASTCG_PEX_PC
+#undef ASTCG_PEX
+}
+
 hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan,
                         bool dual_plane, int exhaustive) {
@@ -3754,7 +4171,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
     int mask_words = 0;
     if (plan) {
         hipLaunchKernelGGL(k_plan_zero, dim3(1), dim3(64), 0, s, plan->pb_count);
-        hipLaunchKernelGGL(k_plan_modes, dim3(blocks_for(A0.nb)), dim3(64), 0, s, A0, *plan, T.nbm,
+        hipLaunchKernelGGL(k_plan_modes, dim3(blocks_for(A0.nb)), dim3(64), 0, s, A0, T, *plan, T.nbm,
                            const_cast<uint32_t*>(plan->mode_mask), const_cast<uint8_t*>(plan->slots));
         mode_mask = plan->mode_mask;
         slots = plan->slots;
@@ -3780,16 +4197,29 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                     list = plan->pb_list;
                     list_count = plan->pb_count + 3;
                     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, s, list_count, 1);
-                    hipLaunchKernelGGL(k_need_list, pgrid, dim3(64), 0, s, A, *plan, pc, list, list_count);
+                    hipLaunchKernelGGL(k_need_list, pgrid, dim3(64), 0, s, A, *plan, pc, 0, list, list_count);
                 }
-#define ASTCG_PEX(PCV, F4)                                                                                      \
-    hipLaunchKernelGGL((k_partition_ex<PCV, F4>), pgrid, dim3(64), plds, s, A, T, B.pb_part, scr, list, list_count, \
-                       exhaustive)
-                const bool f4 = T.tc <= kExF4MaxTc;
-                if (pc == 2) { if (f4) ASTCG_PEX(2, true); else ASTCG_PEX(2, false); }
-                else if (pc == 3) { if (f4) ASTCG_PEX(3, true); else ASTCG_PEX(3, false); }
-                else { if (f4) ASTCG_PEX(4, true); else ASTCG_PEX(4, false); }
-#undef ASTCG_PEX
+                if (plan) {
+                    // hard and fallback blocks: a wave per block for a short
+                    // list, a lane per block for a long one (each kernel
+                    // reads the list length and only one of them works)
+                    launch_partition_ex(false, pc, T, pgrid, plds, s, A, B.pb_part, scr, list, list_count, exhaustive,
+                                        SearchPlan{});
+                    const dim3 wgrid((unsigned)std::min(A.nb, kExwGrid));
+                    const size_t wlds = partition_exw_lds_bytes(T);
+#define ASTCG_PEXW(PCV)                                                                                              \
+    if (T.tc <= 64) hipLaunchKernelGGL((k_partition_exw<PCV, 1>), wgrid, dim3(64), wlds, s, A, T, B.pb_part, list,  \
+                                       list_count, exhaustive);                                                     \
+    else hipLaunchKernelGGL((k_partition_exw<PCV, (kMaxTexels + 63) / 64>), wgrid, dim3(64), wlds, s, A, T,         \
+                            B.pb_part, list, list_count, exhaustive)
+                    if (pc == 2) { ASTCG_PEXW(2); }
+                    else if (pc == 3) { ASTCG_PEXW(3); }
+                    else { ASTCG_PEXW(4); }
+#undef ASTCG_PEXW
+                } else {
+                    launch_partition_ex(false, pc, T, pgrid, plds, s, A, B.pb_part, scr, list, list_count, exhaustive,
+                                        SearchPlan{});
+                }
             } else {
                 const dim3 pgrid((A.nb + kPartBlocks - 1) / kPartBlocks);
                 const size_t plds = partition_lds_bytes(T);
@@ -3798,6 +4228,24 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                 if (pc == 2) hipLaunchKernelGGL(k_partition<2>, pgrid, dim3(64), plds, s, A, T, B.pb_part, pplan, use);
                 else if (pc == 3) hipLaunchKernelGGL(k_partition<3>, pgrid, dim3(64), plds, s, A, T, B.pb_part, pplan, use);
                 else hipLaunchKernelGGL(k_partition<4>, pgrid, dim3(64), plds, s, A, T, B.pb_part, pplan, use);
+            }
+            if (plan && plan->seeds > 0 && (plan->seed_topk || plan->seeds > 4)) {
+                // blocks searching this count from their predicted seeds (a
+                // second list after the first): the seed head's top-k of
+                // those blocks, then (more than 4 seeds) their line fits rank them
+                const dim3 pgrid((A.nb + 63) / 64);
+                int* list = plan->pb_list + A.nb;
+                int* list_count = plan->pb_count + 4;
+                hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, s, list_count, 1);
+                hipLaunchKernelGGL(k_need_list, pgrid, dim3(64), 0, s, A, *plan, pc, 1, list, list_count);
+                if (plan->seed_topk &&
+                    astcg_mlp_seed_topk(list, list_count, A.nb, plan->h2, plan->w3_seed + (size_t)(pc - 2) * 1024 * plan->hidden,
+                                        plan->b3_seed + (pc - 2) * 1024, plan->seeds, const_cast<int*>(plan->topk_idx),
+                                        5 * plan->kmax, (pc - 1) * plan->kmax, s) != ASTCG_OK)
+                    return hipErrorLaunchFailure;
+                if (plan->seeds > 4)
+                    launch_partition_ex(true, pc, T, pgrid, partition_ex_lds_bytes(T), s, A, B.pb_part,
+                                        reinterpret_cast<uint8_t*>(B.ccr), list, list_count, 0, *plan);
             }
             if (plan) {
                 hipLaunchKernelGGL(k_plan, dim3(blocks_for(A.nb)), dim3(64), 0, s, A, T, pc, B.pb_part, *plan,

@@ -445,6 +445,157 @@ __global__ void __launch_bounds__(256) k_infer(int rows, const bf16* __restrict_
         if (out_prob) out_prob[o + 4 * kmax + j] = 0.0f;
     }
 }
+
+// Seed-head top-k of the listed rows for one partition count: the logits
+// h2 W3 + b3 of the head's 1024 columns (the same matrix instructions in the
+// same K order as the GEMM, so the same values as the separate kernels),
+// then the k largest (value descending, lowest column first on ties) by
+// repeated arg-max, one wave per row; the selected column is removed from
+// its owner lane's registers after each round.  16 rows per 256-thread
+// workgroup (one 16-row tile of the matrix instruction, so each 512 KB head
+// is read once per 16 rows); each wave computes 256 of the columns.
+constexpr int kSR = 16, kSeedCols = 1024, kSL = kSeedCols + 1;
+constexpr int kSeedCand = 192;  // logits at or above the threshold ranked directly
+
+// Orders one wave's LDS accesses (the waves of k_seed_topk handle different
+// numbers of rows, so a workgroup barrier cannot be used there).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) k_seed_topk(const int* __restrict__ list, const int* __restrict__ list_count,
+                                                    int rows, const bf16* __restrict__ h2,
+                                                    const bf16* __restrict__ W3t, const float* __restrict__ b3, int k,
+                                                    int* __restrict__ out_idx, int out_stride, int out_off) {
+    __shared__ __align__(16) bf16 sA[16 * kFS];
+    __shared__ float sL[kSR * kSL];
+    __shared__ float sM[4 * 64], sCV[4 * kSeedCand];
+    __shared__ int sCI[4 * kSeedCand];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int n = list ? min(*list_count, rows) : rows;
+    const int r0 = blockIdx.x * kSR;
+    if (r0 >= n) return;  // the whole workgroup
+    for (int i = t; i < 16 * (kFH / 8); i += 256) {
+        const int r = i / (kFH / 8), c = 8 * (i % (kFH / 8));
+        bf16x8 v;
+        for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
+        if (r < kSR && r0 + r < n) {
+            const int row = list ? list[r0 + r] : r0 + r;
+            v = *reinterpret_cast<const bf16x8*>(h2 + (size_t)row * kFH + c);
+        }
+        *reinterpret_cast<bf16x8*>(sA + r * kFS + c) = v;
+    }
+    __syncthreads();
+    {
+        const int lr = lane & 15, lk = 8 * (lane >> 4);
+        f32x4 acc[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < kFH / 32; ++ks) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(sA + lr * kFS + 32 * ks + lk);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const bf16x8 w =
+                    *reinterpret_cast<const bf16x8*>(W3t + (size_t)(256 * wave + 16 * j + lr) * kFH + 32 * ks + lk);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[j], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int col = 256 * wave + 16 * j + lr;
+            const float bias = b3[col];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 4 * (lane >> 4) + r;
+                if (row < kSR) sL[row * kSL + col] = acc[j][r] + bias;
+            }
+        }
+    }
+    __syncthreads();
+    float* s_max = sM + 64 * wave;     // [64] the lanes' largest logits
+    float* s_cv = sCV + kSeedCand * wave;  // [kSeedCand] candidate values
+    int* s_ci = sCI + kSeedCand * wave;    // [kSeedCand] candidate columns
+    for (int rr = wave; rr < kSR && r0 + rr < n; rr += 4) {
+        const int row = list ? list[r0 + rr] : r0 + rr;
+        float v[kSeedCols / 64];
+#pragma unroll
+        for (int m = 0; m < kSeedCols / 64; ++m) v[m] = sL[rr * kSL + lane + 64 * m];
+        // Every one of the k largest is at least theta, the k-th largest of
+        // the 64 lanes' maxima (64 distinct logits).  When at most kSeedCand
+        // logits reach theta, they are ranked directly (value descending,
+        // lowest column first); otherwise (many ties) the repeated arg-max
+        // below does it.
+        float mv = v[0];
+#pragma unroll
+        for (int m = 1; m < kSeedCols / 64; ++m) mv = fmaxf(mv, v[m]);
+        s_max[lane] = mv;
+        wave_sync();
+        int rank = 0;
+        for (int l = 0; l < 64; ++l) {
+            const float o = s_max[l];
+            rank += (o > mv) || (o == mv && l < lane);
+        }
+        wave_sync();
+        if (rank == k - 1) s_cv[0] = mv;
+        wave_sync();
+        const float theta = s_cv[0];
+        int mine = 0;
+#pragma unroll
+        for (int m = 0; m < kSeedCols / 64; ++m) mine += v[m] >= theta;
+        int before = mine;  // inclusive prefix over the lanes
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(before, d);
+            if (lane >= d) before += o;
+        }
+        const int total = __shfl(before, 63);
+        wave_sync();
+        if (k <= 64 && total <= kSeedCand && theta == theta) {
+            int slot = before - mine;
+#pragma unroll
+            for (int m = 0; m < kSeedCols / 64; ++m)
+                if (v[m] >= theta) {
+                    s_cv[slot] = v[m];
+                    s_ci[slot] = lane + 64 * m;
+                    ++slot;
+                }
+            wave_sync();
+            for (int q = lane; q < total; q += 64) {
+                const float cv = s_cv[q];
+                const int ci = s_ci[q];
+                int r = 0;
+                for (int j = 0; j < total; ++j) {
+                    const float o = s_cv[j];
+                    r += (o > cv) || (o == cv && s_ci[j] < ci);
+                }
+                if (r < k) out_idx[(size_t)row * out_stride + out_off + r] = ci;
+            }
+            wave_sync();
+            continue;
+        }
+        for (int j = 0; j < k; ++j) {
+            float bv = -3.4e38f;
+            int bi = 0x7FFFFFFF;
+#pragma unroll
+            for (int m = 0; m < kSeedCols / 64; ++m)
+                if (v[m] > bv) { bv = v[m]; bi = lane + 64 * m; }
+#pragma unroll
+            for (int msk = 32; msk > 0; msk >>= 1) {
+                const float ov = __shfl_xor(bv, msk);
+                const int oi = __shfl_xor(bi, msk);
+                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            }
+            const bool found = bi < kSeedCols;
+            if (found && (bi & 63) == lane) {
+#pragma unroll
+                for (int m = 0; m < kSeedCols / 64; ++m)
+                    if (bi == lane + 64 * m) v[m] = -INFINITY;
+            }
+            if (lane == 0) out_idx[(size_t)row * out_stride + out_off + j] = found ? bi : -1;
+        }
+    }
+}
 }  // namespace
 
 extern "C" {
@@ -593,6 +744,19 @@ astcg_status astcg_mlp_infer(int rows, const void* x, int ldx, int nin, const vo
         default: ASTCG_INFER(256); break;
     }
 #undef ASTCG_INFER
+    return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
+}
+
+astcg_status astcg_mlp_seed_topk(const int* list, const int* list_count, int rows, const void* h2, const void* w3t,
+                                 const float* b3, int k, int* out_idx, int out_stride, int out_off, void* stream) {
+    (void)hipGetLastError();  // a stale error of an earlier call is not this call's
+    if (!h2 || !w3t || !b3 || !out_idx || rows < 0 || k < 0 || k > kSeedCols || out_off < 0 ||
+        out_stride < out_off + k || (list_count && !list) || (list && !list_count))
+        return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!rows || !k) return ASTCG_OK;
+    hipLaunchKernelGGL(k_seed_topk, dim3(grid1(rows, kSR)), dim3(256), 0, static_cast<hipStream_t>(stream), list,
+                       list_count, rows, static_cast<const bf16*>(h2), static_cast<const bf16*>(w3t), b3, k, out_idx,
+                       out_stride, out_off);
     return hipGetLastError() == hipSuccess ? ASTCG_OK : ASTCG_ERR_DEVICE;
 }
 }  // extern "C"

@@ -39,6 +39,7 @@ def _sig(lib):
         "astcg_mlp_features": [vp, i32, i32, i32, i32, i32, i32, vp, i32, vp],
         "astcg_mlp_topk": [vp, i32, i32, i32, vp, vp, i32, vp, vp, vp],
         "astcg_mlp_infer": [i32, vp, i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp],
+        "astcg_mlp_seed_topk": [vp, vp, i32, vp, vp, vp, i32, vp, i32, i32, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

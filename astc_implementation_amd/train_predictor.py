@@ -1,7 +1,9 @@
 """Trains the candidate predictor of the LEARNED policy on the GPU.
 
-Labels come from the encoder's own plain (exhaustive compat) search on
-synthetic images that are not part of any benchmark (seeds >= 10000): half
+Labels come from the encoder's own plain search on synthetic images that
+are not part of any benchmark (seeds 10000..19999): by default the
+EXHAUSTIVE partition search (every selected partitioning ranked by its line
+fits, code/best_partitionings.cpp:498-737), or the COMPAT search; half
 gradient + noise + rectangles images, a quarter normal maps, a quarter flat
 colour + noise.  Inputs are the blocks' normalised RGB texels.  Every
 forward / backward / Adam step runs in the library's HIP kernels
@@ -48,17 +50,18 @@ def rank_targets(seeds, errors):
     return cols.astype(np.int32), tgt
 
 
-def collect(P, enc, n_blocks, size=1024, log=print):
-    """Encodes training images with the plain search until n_blocks blocks
-    are labelled; returns device tensors X (bf16 features), Y (int32 head
+def collect(P, enc, n_blocks, size=1024, log=print, policy=None):
+    """Encodes training images with the plain search (`policy`, default
+    EXHAUSTIVE) until n_blocks blocks are labelled; returns device tensors X (bf16 features), Y (int32 head
     labels: pc - 1, best seed for 2 / 3 / 4 partitions, trial mode) and the
     position-ranking labels R (logit columns, (n, 12)) and T (winners, (n, 3))."""
     import torch
+    policy = A.POLICY_EXHAUSTIVE if policy is None else policy
     xs, ys, rs, ts, total, i = [], [], [], [], 0, 0
     while total < n_blocks:
         img = training_image(i, size)
         h, w = img.shape[:2]
-        enc.configure(w, h, P.bx, P.by, labels=True)
+        enc.configure(w, h, P.bx, P.by, labels=True, policy=policy)
         d_img = torch.from_numpy(img.reshape(-1).copy()).cuda()
         d_out = torch.empty(enc.num_blocks * 16, dtype=torch.uint8, device="cuda")
         enc.encode_device(d_img.data_ptr(), d_out.data_ptr(), torch.cuda.current_stream().cuda_stream)
@@ -138,6 +141,8 @@ def main():
     ap.add_argument("--rank-loss", type=int, default=0,
                     help="also train the seed heads to rank the 4 compat positions of each partition count")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--policy", choices=["exhaustive", "compat"], default="exhaustive",
+                    help="the plain search whose outcome labels the blocks")
     # pruning saved with the weights (tuned on a held-out image)
     ap.add_argument("--seeds", type=int, default=0)
     ap.add_argument("--modes", type=int, default=16)
@@ -150,6 +155,11 @@ def main():
     ap.add_argument("--full-search-above", type=float, default=0.0)
     ap.add_argument("--rounds", type=int, default=0)
     ap.add_argument("--partitioned-slots", type=int, default=0)
+    ap.add_argument("--candidates", type=int, default=1)
+    ap.add_argument("--count-above", type=float, default=0.0)
+    ap.add_argument("--full-search-modes", type=int, default=0)
+    ap.add_argument("--mismatch-candidates", type=int, default=0)
+    ap.add_argument("--fallback-above", type=float, default=0.0)
     a = ap.parse_args()
     bx, by = (int(v) for v in a.block.split("x"))
     import torch
@@ -157,7 +167,8 @@ def main():
     P = Predictor((bx, by), seed=0)
     enc = A.Encoder(0)
     t0 = time.time()
-    X, Y, R, T = collect(P, enc, a.blocks)
+    X, Y, R, T = collect(P, enc, a.blocks,
+                         policy=A.POLICY_EXHAUSTIVE if a.policy == "exhaustive" else A.POLICY_COMPAT)
     torch.cuda.synchronize()
     t1 = time.time()
     n_val = min(65536, X.shape[0] // 10)
@@ -176,7 +187,9 @@ def main():
                       count_below=a.count_below, partitionings=a.partitionings,
                       refine_partitionings=a.refine_partitionings, early_out=a.early_out,
                       full_search_above=a.full_search_above, rounds=a.rounds,
-                      partitioned_slots=a.partitioned_slots))
+                      partitioned_slots=a.partitioned_slots, candidates=a.candidates,
+                      count_above=a.count_above, full_search_modes=a.full_search_modes,
+                      mismatch_candidates=a.mismatch_candidates, fallback_above=a.fallback_above))
     enc.close()
 
 
@@ -193,7 +206,12 @@ def save(path, P, prune):
                         prune_e=np.float32(prune.get("early_out", 0.0)),
                         prune_h=np.float32(prune.get("full_search_above", 0.0)),
                         prune_n=np.int32(prune.get("rounds", 0)),
-                        prune_s=np.int32(prune.get("partitioned_slots", 0)))
+                        prune_s=np.int32(prune.get("partitioned_slots", 0)),
+                        prune_c=np.int32(prune.get("candidates", 0)),
+                        prune_a=np.float32(prune.get("count_above", 0.0)),
+                        prune_m=np.int32(prune.get("full_search_modes", 0)),
+                        prune_k=np.int32(prune.get("mismatch_candidates", 0)),
+                        prune_l=np.float32(prune.get("fallback_above", 0.0)))
 
 
 def load_params(path):

@@ -1,12 +1,20 @@
 """Throughput benchmark: Mtexel/s of the ASTC encoder on one MI355X, 4096x4096
 RGBA8 synthetic image (gradients + value noise + rectangles, seed 1), 4x4
-blocks (the headline) and 6x6 blocks, partition counts 1..4, for the plain
-(compat) search and the learned (predictor-pruned) search on the same image.
+blocks (the headline) and 6x6 blocks, partition counts 1..4, for
+
+* the plain EXHAUSTIVE search (every selected partitioning ranked by its line
+  fits, code/best_partitionings.cpp:498-737): the search the predictor is
+  trained on and judged against;
+* the plain COMPAT search (the reference's live passes 001-007);
+* the LEARNED search (the shipped predictor and pruning settings);
+* an ablation without the network: the learned policy's error-limit early
+  out with every block on the full exhaustive search (all trial modes, 8
+  slots, every partition count), i.e. what the non-learned parts give alone.
 
 One step = one full encode of the image from RGBA8 resident in HBM to the
 128-bit block stream in HBM.  The learned encoder's rate is the headline when
-its decoded PSNR is within 0.05 dB of the plain search's; both rates and the
-PSNR difference are reported.  Prints one JSON line; the CPU baseline is the C
+its decoded PSNR is within 0.05 dB of the exhaustive search's; every rate and
+PSNR difference is reported.  Prints one JSON line; the CPU baseline is the C
 oracle timed on a bounded sample of the same workload on this host.
 """
 import argparse
@@ -31,6 +39,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def measured_traffic(workload, kernel):
+    """(HBM bytes per launch, source summary, VALU counters) of a kernel from
+    the committed profile summaries."""
     try:
         with open(TRAFFIC_FILE) as f:
             entry = json.load(f).get(workload, {}).get(kernel)
@@ -38,7 +48,8 @@ def measured_traffic(workload, kernel):
         return None, None, None
     if not entry:
         return None, None, None
-    return entry["bytes_per_launch"], entry["source"], entry.get("valu_issue_pct_est")
+    valu = {k: entry.get(k) for k in ("valu_busy_pct", "valu_lane_util_pct", "lds_bank_conflict_pct")}
+    return entry.get("bytes_per_launch"), entry.get("source"), valu if any(v is not None for v in valu.values()) else None
 
 
 def parse():
@@ -55,8 +66,11 @@ def parse():
     ap.add_argument("--size", type=int, default=WIDTH)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--policy", choices=["plain", "learned", "both"], default="both",
-                    help="search policies to time; the learned one needs predictor_<bx>x<by>.npz")
+    ap.add_argument("--policy", choices=["exhaustive", "compat", "learned", "ablation", "all", "plain", "both"],
+                    default="all",
+                    help="search policies to time (all = exhaustive, compat, learned and the ablation without "
+                         "the network; the learned ones need predictor_<bx>x<by>.npz); for the batch workloads "
+                         "plain = exhaustive, both = exhaustive and learned")
     ap.add_argument("--graph", action="store_true",
                     help="batch workloads: capture the batched call's launch chain in a HIP graph and replay it")
     ap.add_argument("--also", default="6x6",
@@ -96,11 +110,11 @@ METRIC = "Mtexel/s at 4096\u00b2 RGBA8 4x4/6x6, learned vs plain at \u0394PSNR\u
 MAX_DPSNR = 0.05
 
 
-def measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, policy, copy_probe_mib=0):
+def measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, policy, copy_probe_mib=0, profile=True):
     """One policy: warm-up, then `steps` timed encodes of the resident image
     (HIP events on the encoder's stream); returns timings, kernel times and
     the decoded PSNR of the last encode."""
-    enc.configure(w, h, bx, by, max_partitions=4, profile=True, policy=policy)
+    enc.configure(w, h, bx, by, max_partitions=4, profile=profile, policy=policy)
     dev = d_in.device
     d_out = torch.zeros(enc.num_blocks * 16, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -127,7 +141,7 @@ def measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, policy, copy_probe
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[steps]) / steps
     step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
-    ktimes = enc.kernel_times()
+    ktimes = enc.kernel_times() if profile else {}
     blocks = d_out.cpu().numpy()
     img = d_in.cpu().numpy().reshape(h, w, 4)
     dec = enc.decode(bx, by, w, h, blocks)
@@ -155,15 +169,23 @@ def bench_batch(a):
     import astc_implementation_amd as A
     from astc_implementation_amd import synth
     if a.workload == "batch64":
+        # the reference's plain method over a batch: the COMPAT search
         spec = synth.batch_spec(64, 128, 2048, [(4, 4), (6, 6), (8, 8)], 1000, 2)
-        policies = ["plain"]
+        policies = ["compat"]
     else:
+        # learned against the plain search it is trained on and judged by
         spec = synth.batch_spec(256, 64, 4096, [(4, 4), (6, 6)], 20000, 4)
-        policies = ["plain", "learned"] if a.policy == "both" else [a.policy]
+        policies = {"all": ["exhaustive", "compat", "learned"], "both": ["exhaustive", "learned"],
+                    "plain": ["exhaustive"]}.get(a.policy, [a.policy])
     steps = a.steps or 1
+
+    def progress(msg):
+        print("bench.py %s: %s" % (a.workload, msg), file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
     imgs = synth.make_textures(spec, workers=min(16, os.cpu_count() or 1))
     gen_s = time.perf_counter() - t0
+    progress("%d textures generated in %.1f s" % (len(spec), gen_s))
     dev = torch.device("cuda:0")
     d_in = [torch.from_numpy(im.reshape(-1)).to(dev) for im in imgs]
     texels = sum(w * h for _, w, h, _, _ in spec)
@@ -175,12 +197,14 @@ def bench_batch(a):
     stream = torch.cuda.current_stream(dev)
     res = {}
     for pol in policies:
-        policy = A.POLICY_LEARNED if pol == "learned" else A.POLICY_COMPAT
+        policy = {"learned": A.POLICY_LEARNED, "exhaustive": A.POLICY_EXHAUSTIVE}.get(pol, A.POLICY_COMPAT)
         if pol == "learned":
             for fp in sorted({e[3] for e in spec}):
                 enc.load_predictor(*fp)
         for _ in range(a.warmup):
             enc.encode_batch_device(textures, stream=stream.cuda_stream, policy=policy, graph=a.graph)
+            torch.cuda.synchronize()
+            progress("%s warm-up call done" % pol)
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -198,6 +222,7 @@ def bench_batch(a):
             psnrs.append(99.0 if mse == 0 else 10.0 * np.log10(255.0 * 255.0 / mse))
         res[pol] = {"ms": ms, "value": texels / (ms / 1e3) / 1e6, "psnr": np.array(psnrs),
                     "fallback": enc.fallback_rate() if pol == "learned" else None}
+        progress("%s: %.1f ms per step, %.1f Mtexel/s" % (pol, ms, res[pol]["value"]))
     head = "learned" if "learned" in res else policies[0]
     r = res[head]
     cfg = {"workload": "%d textures, sides %d..%d, footprints %s, content 2/4 gradient+noise+rectangles, "
@@ -209,13 +234,23 @@ def bench_batch(a):
     for pol, v in res.items():
         cfg["%s_mtexel_s" % pol] = v["value"]
         cfg["%s_mean_psnr_db" % pol] = round(float(v["psnr"].mean()), 4)
-    if "plain" in res and "learned" in res:
-        d = res["plain"]["psnr"] - res["learned"]["psnr"]
-        cfg.update({"dpsnr_db_mean": round(float(d.mean()), 4), "dpsnr_db_max": round(float(d.max()), 4),
-                    "textures_over_0.05_db": int((d > MAX_DPSNR).sum()),
-                    "learned_fallback_rate": res["learned"]["fallback"],
+    if "learned" in res:
+        cfg.update({"learned_fallback_rate": res["learned"]["fallback"],
                     "learned_pruning": {"%dx%d" % fp: A.load_predictor_file(A.predictor_path(*fp))[2]
                                         for fp in sorted({e[3] for e in spec})}})
+        for base in ("exhaustive", "compat"):
+            if base not in res:
+                continue
+            # per-texture decoded PSNR loss of the learned search against the plain one
+            d = res[base]["psnr"] - res["learned"]["psnr"]
+            sfx = "" if base == "exhaustive" else "_vs_compat"
+            cfg.update({"dpsnr_db_mean" + sfx: round(float(d.mean()), 4), "dpsnr_db_max" + sfx: round(float(d.max()), 4),
+                        "textures_over_0.05_db" + sfx: int((d > MAX_DPSNR).sum()),
+                        "speedup_vs_" + base: round(res["learned"]["value"] / res[base]["value"], 3)})
+            if base == "exhaustive":
+                cfg["worst_textures"] = [{"index": int(i), "kind": spec[i][0], "width": spec[i][1],
+                                          "height": spec[i][2], "block": "%dx%d" % spec[i][3],
+                                          "dpsnr_db": round(float(d[i]), 4)} for i in np.argsort(-d)[:8]]
     line = {"metric": METRIC, "value": r["value"], "unit": "Mtexel/s", "n_gpus": 1, "steps": steps,
             "warmup": a.warmup, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": cfg}
@@ -290,35 +325,57 @@ def main():
     enc = A.Encoder(0)
     d_in = torch.from_numpy(img.reshape(-1)).to(torch.device("cuda:0"))
     pruning = {}
+    want = {"all": ("exhaustive", "compat", "learned", "ablation"), "plain": ("exhaustive",),
+            "both": ("exhaustive", "learned")}.get(a.policy, (a.policy,))
 
     def run_policies(bx, by, steps, warmup, probe):
         runs = {}
-        if a.policy in ("plain", "both"):
-            runs["plain"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_COMPAT, probe)
-        if a.policy in ("learned", "both") and os.path.exists(A.predictor_path(bx, by)):
-            enc.load_predictor(bx, by)
-            pruning["%dx%d" % (bx, by)] = A.load_predictor_file(A.predictor_path(bx, by))[2]
-            runs["learned"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_LEARNED,
-                                      probe if "plain" not in runs else 0)
-        dpsnr = None
-        if "plain" in runs and "learned" in runs:
-            dpsnr = runs["plain"]["psnr"] - runs["learned"]["psnr"]
-        return runs, dpsnr
+        if "exhaustive" in want:
+            runs["exhaustive"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_EXHAUSTIVE, probe)
+            probe = 0
+        if "compat" in want:
+            runs["compat"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_COMPAT, probe)
+            probe = 0
+        if os.path.exists(A.predictor_path(bx, by)):
+            if "learned" in want:
+                enc.load_predictor(bx, by)
+                pruning["%dx%d" % (bx, by)] = A.load_predictor_file(A.predictor_path(bx, by))[2]
+                runs["learned"] = measure(enc, A, torch, d_in, w, h, bx, by, steps, warmup, A.POLICY_LEARNED, probe)
+            if "ablation" in want:
+                # the same early out, every block on the full exhaustive search:
+                # the network's outputs are computed and ignored
+                enc.load_predictor(bx, by, fallback_below=2.0)
+                runs["ablation"] = measure(enc, A, torch, d_in, w, h, bx, by, 1, 1, A.POLICY_LEARNED, 0)
+                enc.load_predictor(bx, by)
+        return runs
 
-    runs, dpsnr = run_policies(bx, by, a.steps, a.warmup, a.copy_probe_mib)
+    def summary(runs):
+        out = {}
+        for k, v in runs.items():
+            out["%s_mtexel_s" % k] = v["value"]
+            out["psnr_%s_db" % k] = round(v["psnr"], 4)
+            out["psnr_weighted_%s_db" % k] = round(v["psnr_weighted"], 4)
+        for base in ("exhaustive", "compat"):
+            if base not in runs:
+                continue
+            for k in ("learned", "ablation"):
+                if k in runs:
+                    out["dpsnr_%s_vs_%s_db" % (k, base)] = round(runs[base]["psnr"] - runs[k]["psnr"], 4)
+                    out["speedup_%s_vs_%s" % (k, base)] = round(runs[k]["value"] / runs[base]["value"], 3)
+        if "learned" in runs:
+            out["learned_fallback_rate"] = runs["learned"]["fallback"]
+        return out
+
+    runs = run_policies(bx, by, a.steps, a.warmup, a.copy_probe_mib)
     others = {}
     for fp in [f for f in a.also.split(",") if f and f != a.block]:
         obx, oby = (int(v) for v in fp.split("x"))
-        oruns, od = run_policies(obx, oby, 2, 1, 0)
-        others[fp] = {"plain_mtexel_s": oruns["plain"]["value"] if "plain" in oruns else None,
-                      "learned_mtexel_s": oruns["learned"]["value"] if "learned" in oruns else None,
-                      "psnr_plain_db": round(oruns["plain"]["psnr"], 4) if "plain" in oruns else None,
-                      "psnr_learned_db": round(oruns["learned"]["psnr"], 4) if "learned" in oruns else None,
-                      "dpsnr_db": round(od, 4) if od is not None else None,
-                      "learned_fallback_rate": oruns["learned"]["fallback"] if "learned" in oruns else None}
+        others[fp] = summary(run_policies(obx, oby, 2, 1, 0))
     # the learned encoder is the headline when it stays within the quality
-    # bound against the plain search on the same image
-    head = "learned" if "learned" in runs and (dpsnr is None or dpsnr <= MAX_DPSNR) else "plain"
+    # bound against the exhaustive search on the same image
+    plain = "exhaustive" if "exhaustive" in runs else ("compat" if "compat" in runs else None)
+    dpsnr = runs[plain]["psnr"] - runs["learned"]["psnr"] if plain and "learned" in runs else None
+    head = "learned" if "learned" in runs and (dpsnr is None or dpsnr <= MAX_DPSNR) else (plain or "learned")
     r = runs[head]
     ktimes = r["ktimes"]
     texels = w * h
@@ -330,7 +387,34 @@ def main():
     texels_per_launch = texels * a.steps / max(launches, 1)
     achieved = bytes_per_texel * texels_per_launch / (avg_ms / 1e3) / 1e9
     workload_key = "%dx%d seed %d %dx%d %s" % (w, h, SEED, bx, by, head)
-    traffic, traffic_src, valu_busy = measured_traffic(workload_key, dom)
+    traffic, traffic_src, valu = measured_traffic(workload_key, dom)
+    cfg = {"workload": "%s search, %dx%d RGBA8 gradient+noise+rectangles seed %d, %dx%d blocks, partitions 1-4"
+                       % (head, w, h, SEED, bx, by),
+           "blocks": ((w + bx - 1) // bx) * ((h + by - 1) // by),
+           "policy": head,
+           "plain_policy": plain,
+           "dpsnr_db": round(dpsnr, 4) if dpsnr is not None else None,
+           "median_ms_per_step": r["median_ms"],
+           "end_to_end_mtexel_s": end_to_end(enc, A, img, bx, by, {"learned": A.POLICY_LEARNED,
+                                                                   "exhaustive": A.POLICY_EXHAUSTIVE}
+                                             .get(head, A.POLICY_COMPAT)),
+           "learned_pruning": pruning,
+           "other_footprints": others}
+    cfg.update(summary(runs))
+    roof = {"bound": "valu", "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": traffic,
+            "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches,
+            "algorithmic_bytes_per_launch": bytes_per_texel * texels_per_launch,
+            # the contract's HBM figure: algorithmic bytes over the live average
+            # launch duration against the 8 TB/s peak
+            "hbm": {"achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS},
+            "traffic_source": traffic_src}
+    if valu:
+        # the search kernels are VALU-bound: the counter-measured VALU busy
+        # share of the kernel's duration (its profile's SQ counters)
+        roof.update({"achieved": valu.get("valu_busy_pct"), "peak": 100.0, "unit": "% VALU busy",
+                     "frac": (valu.get("valu_busy_pct") or 0.0) / 100.0,
+                     "valu_lane_utilisation_pct": valu.get("valu_lane_util_pct"),
+                     "lds_bank_conflict_pct": valu.get("lds_bank_conflict_pct")})
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -344,32 +428,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "%s search, %dx%d RGBA8 gradient+noise+rectangles seed %d, %dx%d blocks, partitions 1-4"
-                               % (head, w, h, SEED, bx, by),
-                   "blocks": ((w + bx - 1) // bx) * ((h + by - 1) // by),
-                   "policy": head,
-                   "plain_mtexel_s": runs["plain"]["value"] if "plain" in runs else None,
-                   "learned_mtexel_s": runs["learned"]["value"] if "learned" in runs else None,
-                   "psnr_plain_db": round(runs["plain"]["psnr"], 4) if "plain" in runs else None,
-                   "psnr_learned_db": round(runs["learned"]["psnr"], 4) if "learned" in runs else None,
-                   "dpsnr_db": round(dpsnr, 4) if dpsnr is not None else None,
-                   "psnr_weighted_plain_db": round(runs["plain"]["psnr_weighted"], 4) if "plain" in runs else None,
-                   "psnr_weighted_learned_db": round(runs["learned"]["psnr_weighted"], 4) if "learned" in runs else None,
-                   "median_ms_per_step": r["median_ms"],
-                   "end_to_end_mtexel_s": end_to_end(enc, A, img, bx, by, A.POLICY_LEARNED if head == "learned"
-                                                     else A.POLICY_COMPAT),
-                   "learned_fallback_rate": runs["learned"]["fallback"] if "learned" in runs else None,
-                   "learned_pruning": pruning,
-                   "other_footprints": others},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": dom, "kernel_avg_ms": avg_ms, "kernel_launches": launches,
-                     "algorithmic_bytes_per_launch": bytes_per_texel * texels_per_launch,
-                     "traffic_source": traffic_src,
-                     # the bound that matters for this search: the kernel's wave64
-                     # VALU instructions x 4 cycles against the 1024 SIMDs' issue
-                     # capacity over its duration (an estimate, same profile)
-                     "valu_issue_pct_est": valu_busy},
+        "config": cfg,
+        "roofline": roof,
         "kernels_ms_per_step": {k: v["total_ms"] / a.steps for k, v in ktimes.items()},
     }
     if not a.no_cpu:

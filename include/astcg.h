@@ -56,8 +56,8 @@ typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1, ASTCG_POLICY_E
  * probability gets the full search (COMPAT, or EXHAUSTIVE with
  * candidates = 1); any other block searches only the
  * partition counts that are its most likely count or
- * at least count_below probable (with the COMPAT positions when seeds is 0,
- * else its `seeds` most likely partitionings of each such count), only its
+ * at least count_below probable (with the `candidates` search when seeds is
+ * 0, else from its `seeds` most likely partitionings of each such count), only its
  * `modes` most likely trial modes, and refines `slots` candidates per
  * partitioned block instead of 8.  With the COMPAT positions, `partitionings`
  * (1..3) keeps only that many of the 4 positions of each searched count: the
@@ -71,7 +71,9 @@ typedef enum { ASTCG_POLICY_COMPAT = 0, ASTCG_POLICY_LEARNED = 1, ASTCG_POLICY_E
  * best encoding so far is below early_out times that limit, its remaining
  * partition counts are not searched. */
 typedef struct {
-    uint32_t seeds;         /* 0 (COMPAT positions, default) or 1..4 */
+    uint32_t seeds;         /* 0 (the `candidates` search, default), 1..4: the predictor's most likely seeds of
+                               each searched count, 5..64: those seeds ranked by their line fits (as in the
+                               EXHAUSTIVE search) and the 4 best searched */
     uint32_t modes;         /* 1..64, default 16 */
     uint32_t slots;         /* 1..8, default 5 */
     float fallback_below;   /* default 0.5 */
@@ -91,6 +93,18 @@ typedef struct {
                                blocks): 0 = the COMPAT positions, 1 = the EXHAUSTIVE search over every
                                selected partitioning, N = 2..1024: that search over the first N in k-means
                                mismatch order (partition_search_limit, code/best_partitionings.cpp:520) */
+    float count_above;      /* > 0: a block whose best error so far is above this multiple of the error limit
+                               searches every remaining partition count, whatever the count head predicts;
+                               0 = off */
+    uint32_t full_search_modes; /* hard blocks (full_search_above): 0 = every trial mode and 8 slots (default),
+                                   1 = their predicted modes and slots (the full search then widens only the
+                                   partition counts and candidates) */
+    uint32_t mismatch_candidates; /* seeds 5..64: the line fits first cover this many partitionings of the
+                                     k-means mismatch order (as the EXHAUSTIVE search visits them), then the
+                                     seeds not among them; 0 = the seeds alone (default) */
+    float fallback_above;   /* > 0: a block whose texels lie further from their best line (weighted squared
+                               distance) than this multiple of the error limit gets the full search from one
+                               partition on, like a low-confidence block; 0 = off */
 } astcg_learned_params;
 
 void astcg_default_learned_params(astcg_learned_params* p);

@@ -97,6 +97,16 @@ astcg_status astcg_mlp_infer(int rows, const void* x, int ldx, int nin, const vo
                              const void* w2t, const float* b2, const void* w3t, const float* b3, int n3, int nmodes,
                              int kmax, int kpc, int kmode, int* out_idx, float* out_prob, void* h2_out, void* stream);
 
+/* Seed-head top-k for one partition count, hidden width 256: for row r of
+ * the list (list[0 .. min(*list_count, rows)), or rows 0 .. rows - 1 when
+ * list is NULL), the k <= 1024 largest logits h2[row] W3 + b3 of the head's
+ * 1024 columns (value descending, lowest column first on ties, -1 past the
+ * head's end) to out_idx[row * out_stride + out_off + j], j < k; the same
+ * values as astcg_mlp_gemm + astcg_mlp_topk_heads.  h2: bf16 [row][256];
+ * w3t: the head's transposed bf16 weights [1024][256]; b3: [1024]. */
+astcg_status astcg_mlp_seed_topk(const int* list, const int* list_count, int rows, const void* h2, const void* w3t,
+                                 const float* b3, int k, int* out_idx, int out_stride, int out_off, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

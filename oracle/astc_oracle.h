@@ -65,8 +65,12 @@ typedef struct {
     int first_block;      /* encode only blocks [first_block, first_block+block_count) */
     int block_count;      /* 0 = all */
     int policy;           /* partition candidates: 0 = the live passes 001-007 (compat),
-                             2 = the exhaustive CPU search of best_partitionings.cpp */
+                             2 = the exhaustive CPU search of best_partitionings.cpp,
+                             3 = seed lists (seeded_positions) */
     int search_limit;     /* policy 2: partitionings line-fitted, in mismatch order (0 = all) */
+    const int32_t* seed_list; /* policy 3: every block's partition seeds for 2, 3, 4 partitions, */
+    int seed_list_len;        /* seed_list_len each, best first (the learned policy's seed heads) */
+    int mismatch_candidates;  /* policy 3, more than 4 seeds: mismatch-order partitionings fitted first */
 } or_params;
 
 typedef struct {
@@ -111,6 +115,12 @@ int oracle_decimation_entry(int bx, int by, int m, int32_t* out, int cap);
  * (-1 where fewer were found). */
 int oracle_debug_exhaustive_candidates(const uint8_t* rgba, int width, int height, int bx, int by,
                                        int block_index, int partition_count, int32_t* entries_out);
+/* The 4 packed table entries (-1 = none) of the seeded candidates
+ * (seeded_positions) of one block for the seed list seeds[0 .. nseeds),
+ * after mismatch_candidates partitionings of the mismatch order. */
+int oracle_debug_seeded_candidates(const uint8_t* rgba, int width, int height, int bx, int by, int block_index,
+                                   int partition_count, const int32_t* seeds, int nseeds, int mismatch_candidates,
+                                   int32_t* entries_out);
 
 /* Stage-level hooks used by unit tests (single partitioned block). */
 int oracle_debug_kmeans_candidates(const uint8_t* rgba, int width, int height, int bx, int by,

@@ -1324,20 +1324,17 @@ static void ex_insert(float e, int id, float* be, int* bi) {
  * selected partitioning (search limit = all), 4 requested candidates, with
  * the line fits of averages_and_directions.cpp (4 components when alpha
  * varies, RGB otherwise).  Writes 4 packed table entries (-1 = none). */
-static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, int limit, int* pos) {
+/* The line fits of find_best_partition_candidates (best_partitionings.cpp:
+ * 520-737) over the packed table entries seq[0 .. n), in that order, and
+ * the 4 best of each fit kept by insert_result, interleaved and
+ * de-duplicated into pos (-1 = none). */
+static void line_fit_positions(const or_desc* d, const or_block* b, int pc, const uint16_t* seq, int n, int* pos) {
     int tc = d->texel_count;
     float wie = 0.055f;
     if (tc <= 20) wie = 0.03f;
     else if (tc <= 31) wie = 0.04f;
     else if (tc <= 41) wie = 0.05f;
     wie = wie * wie;
-    /* positions the counting sort leaves unwritten (its running sum stops
-     * before the largest possible mismatch count) read 0 here; the
-     * reference reads an uninitialised stack array there */
-    uint16_t seq[OR_PARTITIONINGS];
-    memset(seq, 0, sizeof(seq));
-    int n = ex_ordering(d, b, pc, seq);
-    if (limit > 0 && limit < n) n = limit;  /* partition_search_limit, best_partitionings.cpp:520 */
     int req = n < 4 ? n : 4;
     float ube[4], sbe[4];
     int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
@@ -1425,6 +1422,55 @@ static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, in
     }
 }
 
+static void exhaustive_positions(const or_desc* d, const or_block* b, int pc, int limit, int* pos) {
+    /* positions the counting sort leaves unwritten (its running sum stops
+     * before the largest possible mismatch count) read 0 here; the
+     * reference reads an uninitialised stack array there */
+    uint16_t seq[OR_PARTITIONINGS];
+    memset(seq, 0, sizeof(seq));
+    int n = ex_ordering(d, b, pc, seq);
+    if (limit > 0 && limit < n) n = limit;  /* partition_search_limit, best_partitionings.cpp:520 */
+    line_fit_positions(d, b, pc, seq, n, pos);
+}
+
+/* Candidates from a list of partition seeds (a predictor's most likely
+ * seeds, best first): the seeds that are selected partitionings of this
+ * count, in list order; more than 4 seeds are ranked by the line fits of
+ * line_fit_positions (after the first mis_n partitionings of the mismatch
+ * order, seeds among those skipped), up to 4 are taken as they are.  No
+ * counterpart in the reference: the learned policy's candidates. */
+static void seeded_positions(const or_desc* d, const or_block* b, int pc, const int32_t* seeds, int nseeds,
+                             int mis_n, int* pos) {
+    uint16_t seq[2 * OR_PARTITIONINGS];
+    int n = 0;
+    if (nseeds > 4 && mis_n > 0) {
+        /* the first mis_n partitionings of the exhaustive search's mismatch
+         * order come first (every slot of that order is written: a
+         * mismatch count stays below the k-means texel count) */
+        for (int i = 0; i < OR_PARTITIONINGS; i++) seq[i] = 0;
+        int nsel = ex_ordering(d, b, pc, seq);
+        n = mis_n < nsel ? mis_n : nsel;
+    }
+    int n_order = n;
+    for (int j = 0; j < nseeds; j++) {
+        int seed = seeds[j];
+        if (seed < 0 || seed >= OR_PARTITIONINGS) break;
+        for (int e = 0; e < d->count_selected[pc - 1]; e++)
+            if (d->part[(pc - 2) * OR_PARTITIONINGS + e].partition_index == seed) {
+                int dup = 0;
+                for (int t = 0; t < n_order; t++) dup |= seq[t] == e;
+                if (!dup) seq[n++] = (uint16_t)e;
+                break;
+            }
+    }
+    for (int i = 0; i < 4; i++) pos[i] = -1;
+    if (nseeds > 4) {
+        line_fit_positions(d, b, pc, seq, n, pos);
+    } else {
+        for (int i = 0; i < n && i < 4; i++) pos[i] = seq[i];
+    }
+}
+
 int oracle_debug_exhaustive_candidates(const uint8_t* rgba, int width, int height, int bx, int by, int block_index,
                                        int partition_count, int32_t* entries_out) {
     or_desc* d = or_desc_get(bx, by);
@@ -1434,6 +1480,20 @@ int oracle_debug_exhaustive_candidates(const uint8_t* rgba, int width, int heigh
     load_block(d, rgba, width, height, block_index % bxn, block_index / bxn, &blk);
     int pos[4];
     exhaustive_positions(d, &blk, partition_count, 0, pos);
+    for (int i = 0; i < 4; i++) entries_out[i] = pos[i];
+    return 0;
+}
+
+int oracle_debug_seeded_candidates(const uint8_t* rgba, int width, int height, int bx, int by, int block_index,
+                                   int partition_count, const int32_t* seeds, int nseeds, int mismatch_candidates,
+                                   int32_t* entries_out) {
+    or_desc* d = or_desc_get(bx, by);
+    if (!d || partition_count < 2 || partition_count > 4 || nseeds < 0 || nseeds > OR_PARTITIONINGS) return -1;
+    int bxn = (width + bx - 1) / bx;
+    or_block blk;
+    load_block(d, rgba, width, height, block_index % bxn, block_index / bxn, &blk);
+    int pos[4];
+    seeded_positions(d, &blk, partition_count, seeds, nseeds, mismatch_candidates, pos);
     for (int i = 0; i < 4; i++) entries_out[i] = pos[i];
     return 0;
 }
@@ -1490,6 +1550,9 @@ int oracle_encode(const uint8_t* rgba, int width, int height, int bx, int by, co
             int npos = 1;
             if (pc > 1) {
                 if (policy == 2) exhaustive_positions(d, &blk, pc, search_limit, pos);
+                else if (policy == 3)
+                    seeded_positions(d, &blk, pc, prm->seed_list + (pc - 2) * prm->seed_list_len, prm->seed_list_len,
+                                     prm->mismatch_candidates, pos);
                 else compat_positions(d, rgba, width, height, bxn, base, B, j, pc, pos);
                 npos = 4;
             }

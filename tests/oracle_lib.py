@@ -33,10 +33,12 @@ class OrSymbolic(ctypes.Structure):
 class OrParams(ctypes.Structure):
     _fields_ = [("max_partitions", ctypes.c_int), ("batch_size", ctypes.c_int), ("threads", ctypes.c_int),
                 ("first_block", ctypes.c_int), ("block_count", ctypes.c_int), ("policy", ctypes.c_int),
-                ("search_limit", ctypes.c_int)]
+                ("search_limit", ctypes.c_int), ("seed_list", ctypes.c_void_p), ("seed_list_len", ctypes.c_int),
+                ("mismatch_candidates", ctypes.c_int)]
 
 POLICY_COMPAT = 0
 POLICY_EXHAUSTIVE = 2
+POLICY_SEEDED = 3
 
 
 class OrStats(ctypes.Structure):
@@ -76,6 +78,8 @@ def lib():
         L.oracle_debug_kmeans_candidates.restype = i32
         L.oracle_debug_exhaustive_candidates.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp]
         L.oracle_debug_exhaustive_candidates.restype = i32
+        L.oracle_debug_seeded_candidates.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp, i32, i32, vp]
+        L.oracle_debug_seeded_candidates.restype = i32
         L.oracle_decimation_entry.argtypes = [i32, i32, i32, vp, i32]
         L.oracle_decimation_entry.restype = i32
         _lib = L
@@ -83,8 +87,10 @@ def lib():
 
 
 def encode(rgba, bx, by, max_partitions=4, threads=1, first_block=0, block_count=0, batch_size=920,
-           want_symbolic=True, policy=POLICY_COMPAT, search_limit=0):
-    """Returns (blocks uint8[n*16], symbolic list or None, stats)."""
+           want_symbolic=True, policy=POLICY_COMPAT, search_limit=0, seed_lists=None, mismatch_candidates=0):
+    """Returns (blocks uint8[n*16], symbolic list or None, stats).  seed_lists
+    (POLICY_SEEDED): (3, n) partition seeds for 2, 3, 4 partitions, best
+    first, the same for every block."""
     rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
     h, w = rgba.shape[:2]
     nb_total = ((w + bx - 1) // bx) * ((h + by - 1) // by)
@@ -92,7 +98,10 @@ def encode(rgba, bx, by, max_partitions=4, threads=1, first_block=0, block_count
     out = np.zeros(n * 16, np.uint8)
     sym = (OrSymbolic * n)() if want_symbolic else None
     st = OrStats()
-    p = OrParams(max_partitions, batch_size, threads, first_block, block_count, policy, search_limit)
+    seeds = None if seed_lists is None else np.ascontiguousarray(seed_lists, dtype=np.int32)
+    p = OrParams(max_partitions, batch_size, threads, first_block, block_count, policy, search_limit,
+                 seeds.ctypes.data if seeds is not None else None, seeds.shape[1] if seeds is not None else 0,
+                 mismatch_candidates)
     r = lib().oracle_encode(rgba.ctypes.data, w, h, bx, by, ctypes.byref(p), out.ctypes.data,
                             ctypes.cast(sym, ctypes.c_void_p) if sym is not None else None, ctypes.byref(st))
     if r != 0:
@@ -157,6 +166,19 @@ def exhaustive_candidates(rgba, bx, by, block_index, partition_count):
         raise RuntimeError("oracle_debug_exhaustive_candidates failed")
     return list(pos)
 
+
+def seeded_candidates(rgba, bx, by, block_index, partition_count, seeds, mismatch_candidates=0):
+    """The 4 packed table entries (-1 = none) of the seeded candidates for
+    the seed list `seeds` (best first), after `mismatch_candidates`
+    partitionings of the mismatch order when more than 4 seeds are given."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = rgba.shape[:2]
+    sd = np.ascontiguousarray(seeds, dtype=np.int32)
+    pos = np.zeros(4, np.int32)
+    if lib().oracle_debug_seeded_candidates(rgba.ctypes.data, w, h, bx, by, block_index, partition_count,
+                                            sd.ctypes.data, sd.size, mismatch_candidates, pos.ctypes.data):
+        raise RuntimeError("oracle_debug_seeded_candidates failed")
+    return [int(v) for v in pos]
 
 def decimation_entry(bx, by, m):
     """(texel_count, weight_count, wx, wy, [(weight index, 16*contribution) per texel])"""

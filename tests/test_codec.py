@@ -290,8 +290,8 @@ def test_predictor_files_carry_their_pruning():
         assert [p.shape for p in params] == [(nin, hidden), (hidden,), (hidden, hidden), (hidden,),
                                              (hidden, 4 + 3 * 1024 + nmodes), (4 + 3 * 1024 + nmodes,)]
         assert all(p.dtype == np.float32 and np.isfinite(p).all() for p in params)
-        assert prune["seeds"] == 0 and 1 <= prune["modes"] <= 64 and 1 <= prune["slots"] <= 8
-        assert prune["early_out"] > 0
+        assert 5 <= prune["seeds"] <= 64 and 1 <= prune["modes"] <= 64 and 1 <= prune["slots"] <= 8
+        assert prune["early_out"] > 0 and prune["candidates"] == 1  # hard blocks: the exhaustive search
 
 
 def test_batch_spec_is_reproducible():

@@ -64,25 +64,28 @@ def test_exhaustive_batch_and_noalloc(encoder):
         assert np.array_equal(d_out.cpu().numpy(), ref)
 
 
-def test_learned_policy_on_exhaustive_candidates(encoder):
+@pytest.mark.parametrize("w,h,chunk", [(160, 120, 500), (1088, 1024, 0)])
+def test_learned_policy_on_exhaustive_candidates(encoder, w, h, chunk):
     """LEARNED with candidates = 1 takes its partition candidates from the
     exhaustive search: with every block falling back it reproduces the
     EXHAUSTIVE encode bit for bit (the compacted block list of the
-    exhaustive kernel covers exactly the blocks the plan searches)."""
-    img = synth.gradient_image(160, 120, 21)
-    encoder.configure(160, 120, 4, 4, policy=A.POLICY_EXHAUSTIVE)
+    exhaustive kernel covers exactly the blocks the plan searches; 1200
+    blocks run a wave per block, 69632 blocks, above the 65536-block cut,
+    a lane per block)."""
+    img = synth.gradient_image(w, h, 21)
+    encoder.configure(w, h, 4, 4, policy=A.POLICY_EXHAUSTIVE)
     ref = encoder.encode(img)
     encoder.load_predictor(4, 4, fallback_below=2.0, candidates=1, early_out=0.0, count_below=0.0,
                            full_search_above=0.0)
-    encoder.configure(160, 120, 4, 4, policy=A.POLICY_LEARNED, chunk_blocks=500)
+    encoder.configure(w, h, 4, 4, policy=A.POLICY_LEARNED, chunk_blocks=chunk)
     assert np.array_equal(encoder.encode(img), ref)
     assert encoder.fallback_rate() == 1.0
     encoder.load_predictor(4, 4, candidates=1)
-    encoder.configure(160, 120, 4, 4, policy=A.POLICY_LEARNED)
+    encoder.configure(w, h, 4, 4, policy=A.POLICY_LEARNED)
     out = encoder.encode(img)
-    dec = A.decode_image(4, 4, 160, 120, out)
+    dec = A.decode_image(4, 4, w, h, out)
     assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
-    assert A.psnr(A.decode_image(4, 4, 160, 120, ref), img) - A.psnr(dec, img) < 0.5
+    assert A.psnr(A.decode_image(4, 4, w, h, ref), img) - A.psnr(dec, img) < 0.5
 
 
 def test_learned_exhaustive_search_limit_matches_oracle(encoder):

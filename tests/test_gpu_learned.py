@@ -1,7 +1,8 @@
-"""The trained candidate predictor (astc_implementation_amd/predictor_4x4.npz)
-in the LEARNED policy: what the network gains over the same pruning without
-it, and its quality bound against the plain search on an image it was not
-trained on (training images use seeds >= 10000)."""
+"""The trained candidate predictor (astc_implementation_amd/predictor_4x4.npz,
+trained on the EXHAUSTIVE search's outcomes) in the LEARNED policy: what the
+network gains over the same pruning without it, its quality bound against
+the plain searches on an image it was not trained on (training images use
+seeds 10000..19999), and the effect of each pruning setting."""
 import os
 
 import numpy as np
@@ -35,18 +36,28 @@ def _learned_psnr(enc, img, params, hidden, **lp):
     return A.psnr(enc.decode(4, 4, 512, 512, blocks), img), enc.fallback_rate()
 
 
-def test_trained_network_beats_uninformed_pruning(setup):
-    """Same pruning (16 of 53 modes, 5 refined candidates, no fallback), with
-    the trained network vs an all-zero network (every logit equal, so the
-    first 16 trial modes are kept): the network must keep less than half of
-    the PSNR loss the uninformed pruning has against the plain search (on
-    this image: 0.027 dB vs 0.096 dB)."""
-    enc, img, params, hidden, p_plain = setup
-    p_net, _ = _learned_psnr(enc, img, params, hidden, fallback_below=0.0)
+@pytest.fixture(scope="module")
+def p_exhaustive(setup):
+    enc, img, _, _, _ = setup
+    enc.configure(512, 512, 4, 4, policy=A.POLICY_EXHAUSTIVE)
+    return A.psnr(enc.decode(4, 4, 512, 512, enc.encode(img)), img)
+
+
+@pytest.mark.parametrize("prune", [dict(modes=16, candidates=1), dict(modes=53, seeds=32, candidates=1)])
+def test_trained_network_beats_uninformed_pruning(setup, p_exhaustive, prune):
+    """Same pruning against the exhaustive search (every partition count, no
+    fallback, no early out, 8 slots): 16 of the 53 trial modes with the
+    exhaustive partition candidates, or every mode with 32 seeds per count
+    ranked by their line fits.  The trained network against an all-zero one
+    (every logit equal: the first 16 trial modes, the 32 lowest seeds) must
+    keep less than half of the uninformed pruning's PSNR loss."""
+    enc, img, params, hidden, _ = setup
+    lp = dict(fallback_below=0.0, count_below=0.0, slots=8, **prune)
+    p_net, _ = _learned_psnr(enc, img, params, hidden, **lp)
     zero = [np.zeros_like(w) for w in params]
-    p_zero, _ = _learned_psnr(enc, img, zero, hidden, fallback_below=0.0)
-    assert p_net > p_zero, (p_plain, p_net, p_zero)
-    assert p_plain - p_net < 0.5 * (p_plain - p_zero), (p_plain, p_net, p_zero)
+    p_zero, _ = _learned_psnr(enc, img, zero, hidden, **lp)
+    assert p_net > p_zero, (p_exhaustive, p_net, p_zero)
+    assert p_exhaustive - p_net < 0.5 * (p_exhaustive - p_zero), (p_exhaustive, p_net, p_zero)
 
 
 def test_default_learned_policy_quality(setup):
@@ -109,9 +120,10 @@ def test_ranked_partitionings_keep_the_seed_heads_best(setup):
         bad = np.nonzero(l_one < l_plain - 0.05)[0]
         assert bad.size == 0, (c, bad[:8], l_one[bad[:8]], l_plain[bad[:8]])
         same += int((s_one == s_plain).sum())
-    # the seed head ranks the plain search's winner first far more often
-    # than chance (1 in 4; measured 0.46 on this image)
-    assert same > 0.35 * 3 * len(rows), same
+    # the seed head ranks the plain search's winner first more often than
+    # chance (1 in 4; measured 0.35 on this image with the head trained on
+    # the exhaustive search's winners)
+    assert same > 0.3 * 3 * len(rows), same
     dec = enc.decode(4, 4, 512, 512, blocks)
     assert A.psnr(dec, img) > 40.0
 
@@ -186,12 +198,12 @@ def test_early_out_skips_counts_below_the_error_limit(setup):
 def test_shipped_predictor_settings(bx, by):
     """The committed predictors with the pruning saved in their files
     (Encoder.load_predictor), on an image outside their training set: the
-    decoded PSNR stays within 0.1 dB of the plain search, every block is
-    a legal encoding and the early out skips partitioned searches for part
-    of the blocks."""
+    decoded PSNR stays within 0.05 dB of the exhaustive search, every block
+    is a legal encoding and the early out skips partitioned searches for
+    part of the blocks."""
     img = synth.gradient_image(384, 320, 7)
     enc = A.Encoder(0)
-    enc.configure(384, 320, bx, by)
+    enc.configure(384, 320, bx, by, policy=A.POLICY_EXHAUSTIVE)
     plain = enc.encode(img)
     p_plain = A.psnr(enc.decode(bx, by, 384, 320, plain), img)
     enc.load_predictor(bx, by)
@@ -199,7 +211,7 @@ def test_shipped_predictor_settings(bx, by):
     learned = enc.encode(img)
     dec = enc.decode(bx, by, 384, 320, learned)
     assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
-    assert p_plain - A.psnr(dec, img) <= 0.1, (p_plain, A.psnr(dec, img))
+    assert p_plain - A.psnr(dec, img) <= 0.05, (p_plain, A.psnr(dec, img))
     _, errors = enc.candidates()
     skipped = (errors[:, 0, :] >= 1e29).all(axis=1)
     assert 0 < skipped.mean() < 1
@@ -209,8 +221,9 @@ def test_shipped_predictor_settings(bx, by):
 def test_full_search_above_marks_hard_blocks(setup):
     """full_search_above: blocks whose one-partition error is above the
     multiple of the error limit are searched like the plain search from two
-    partitions on (counted as fallback); a multiple no block reaches changes
-    nothing."""
+    partitions on (counted as fallback), which lowers the search's own
+    objective (the summed weighted block errors); a multiple no block
+    reaches changes nothing."""
     enc, img, params, hidden, _ = setup
     limit = _reference_error_limit(4, 4)
     base = dict(fallback_below=0.0, modes=8, slots=4)
@@ -226,7 +239,12 @@ def test_full_search_above_marks_hard_blocks(setup):
     out = enc.encode(img)
     rate = enc.fallback_rate()
     assert 0.0 < rate < 1.0
-    assert A.psnr(enc.decode(4, 4, 512, 512, out), img) >= A.psnr(enc.decode(4, 4, 512, 512, ref), img)
+    err_full = sum(b.errorval for b in enc.symbolic())
+    enc.set_predictor(4, 4, params, hidden=hidden, **base)
+    enc.configure(512, 512, 4, 4, policy=A.POLICY_LEARNED)
+    enc.encode(img)
+    assert err_full < sum(b.errorval for b in enc.symbolic())
+    assert not np.array_equal(out, ref)
 
 
 def test_rounds_six_is_the_default_and_fewer_rounds_stay_valid(setup):

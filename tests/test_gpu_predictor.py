@@ -108,6 +108,76 @@ def test_forward_backward_match_reference(block, B):
         _close(g_gpu, g_ref)
 
 
+class _RoundBf16(torch.autograd.Function):
+    """bf16 rounding in the forward pass, identity in the backward pass (the
+    kernels differentiate through their bf16 copies as if they were the fp32
+    master weights and activations)."""
+
+    @staticmethod
+    def forward(ctx, x):
+        return x.to(torch.bfloat16).float()
+
+    @staticmethod
+    def backward(ctx, g):
+        return g
+
+
+@pytest.mark.parametrize("block,B,with_rank", [((4, 4), 33, False), ((6, 6), 48, False), ((4, 4), 40, True)])
+def test_backward_matches_autograd(block, B, with_rank):
+    """The hand-written backward kernels against torch.autograd's gradients
+    of the training objective: the mean over rows of the summed softmax
+    cross-entropies of the labelled heads (plus, with ranking, of the
+    softmax over each partition count's candidate seed logits), built in fp32
+    on the CPU from the same bf16 operands as the forward kernels, with
+    autograd deriving every gradient (GELU's included)."""
+    R = _RoundBf16.apply
+    P = Predictor(block, seed=11)
+    g = torch.Generator().manual_seed(7 * B)
+    x = torch.rand((B, P.nin), generator=g)
+    labels = torch.stack([torch.randint(0, ln, (B,), generator=g) for ln in P.head_len], 1).int()
+    labels[::5, 2] = -1
+    labels[2::7, 4] = -1
+    rank = None
+    if with_rank:
+        seeds = torch.randint(0, 1024, (B, 3, 4), generator=g)
+        cols = (4 + torch.arange(3)[None, :, None] * 1024 + seeds).reshape(B, 12).int()
+        cols[::6, 5] = -1  # an absent candidate
+        tgt = torch.randint(0, 4, (B, 3), generator=g).int()
+        tgt[1::5, 0] = -1  # a count without a winner
+        tgt[::6, 1] = 0
+        rank = (cols.cuda(), tgt.cuda())
+    logits_gpu = P.forward(x.to(torch.bfloat16).cuda())
+    loss_gpu, grads = P.backward(logits_gpu, labels.cuda(), rank)
+    torch.cuda.synchronize()
+    W = [w.detach().cpu().clone().requires_grad_(True) for w in P.params]
+    W1, b1, W2, b2, W3, b3 = W
+    gelu = torch.nn.functional.gelu
+    h1 = R(gelu(R(x) @ R(W1) + b1))
+    h2 = R(gelu(h1 @ R(W2) + b2))
+    logits = h2 @ R(W3) + b3
+    row = torch.zeros(B)
+    for hi, (off, ln) in enumerate(zip(P.head_off, P.head_len)):
+        lab = labels[:, hi].long()
+        on = lab >= 0
+        ce = torch.nn.functional.cross_entropy(logits[:, off:off + ln], lab.clamp(min=0), reduction="none")
+        row = row + torch.where(on, ce, torch.zeros_like(ce))
+    if with_rank:
+        c = cols.reshape(B, 3, 4).long()
+        t = tgt.long()
+        for r in range(B):
+            for q in range(3):
+                if t[r, q] < 0:
+                    continue
+                keep = c[r, q] >= 0
+                sel = logits[r, c[r, q][keep]]
+                pos = int(keep[:t[r, q]].sum())
+                row[r] = row[r] + torch.logsumexp(sel, 0) - sel[pos]
+    (row.sum() / B).backward()
+    _close(loss_gpu, row.detach())
+    for g_gpu, w in zip(grads, W):
+        _close(g_gpu, w.grad)
+
+
 def test_adam_matches_reference():
     P = Predictor((4, 4), seed=5)
     g = torch.Generator().manual_seed(9)
@@ -302,6 +372,51 @@ def test_fused_inference_matches_unfused(block):
     _close(out_prob[:, 0, :4], prob_ref[:, 0, :4], 1e-5)
     _close(out_prob[:, 4, :kmode], prob_ref[:, 4, :kmode], 1e-5)
     assert (out_idx[:, 1:4].cpu() == -7).all()  # the seed heads are not written
+
+
+@pytest.mark.parametrize("k", [1, 8, 33, 48, 64])
+def test_seed_topk_matches_unfused(k):
+    """astcg_mlp_seed_topk (one partition count's seed-head logits of the
+    listed rows from h2, then their top-k) against the GEMM over every
+    output column + top-k of the seed head: same matrix instructions in the
+    same K order, so the same indices (two identical seed columns: the
+    lower first); unlisted rows and other heads are not written."""
+    P = Predictor((4, 4), seed=13)
+    g = torch.Generator().manual_seed(14 + k)
+    with torch.no_grad():
+        for w in P.params:
+            w.add_(torch.randn(w.shape, generator=g).cuda() * 0.05)
+        for pc in (2, 3, 4):
+            c0 = 4 + (pc - 2) * 1024
+            P.params[4][:, c0 + 9] = P.params[4][:, c0 + 7]
+            P.params[5][c0 + 9] = P.params[5][c0 + 7] + 1.0
+            P.params[5][c0 + 7] += 1.0
+    P.refresh_bf16()
+    rows = 300
+    x = torch.rand((rows, P.nin), generator=g).to(torch.bfloat16).cuda()
+    logits = P.forward(x)
+    h2 = P.act[4]
+    idx_ref, _ = P.topk(logits, k)
+    W3, b3 = P.params[4], P.params[5]
+    listed = torch.tensor(sorted(torch.randperm(rows, generator=g)[:117].tolist()), dtype=torch.int32).cuda()
+    count = torch.tensor([listed.numel()], dtype=torch.int32).cuda()
+    sel = listed.long().cpu()
+    mask = torch.ones(rows, dtype=torch.bool)
+    mask[sel] = False
+    for pc in (2, 3, 4):
+        cols = slice(4 + (pc - 2) * 1024, 4 + (pc - 1) * 1024)
+        w3t = W3[:, cols].t().contiguous().to(torch.bfloat16)
+        b3s = b3[cols].contiguous()
+        kmax = k + 3
+        out = torch.full((rows, 5, kmax), -7, dtype=torch.int32, device="cuda")
+        A._check(P.lib.astcg_mlp_seed_topk(listed.data_ptr(), count.data_ptr(), rows, h2.data_ptr(), w3t.data_ptr(),
+                                           b3s.data_ptr(), k, out.data_ptr(), 5 * kmax, (pc - 1) * kmax, None))
+        torch.cuda.synchronize()
+        got = out.cpu()
+        assert torch.equal(got[sel, pc - 1, :k], idx_ref[:, pc - 1, :k].cpu()[sel])
+        assert (got[mask] == -7).all()
+        others = [h for h in range(5) if h != pc - 1]
+        assert (got[sel][:, others] == -7).all() and (got[sel, pc - 1, k:] == -7).all()
 
 
 def test_fused_inference_argument_limits():

@@ -1237,3 +1237,104 @@ def test_exhaustive_search_with_noise_and_alpha(alpha, pc):
         img = np.clip(np.rint(img), 0, 255).astype(np.uint8).reshape(by, bx, 4)
         got = O.exhaustive_candidates(img, bx, by, 0, pc)
         assert got[0] == e, (pc, e, got)
+
+
+# ---------------------------------------------------------------- seeded candidates (learned policy)
+def _seed_table(bx, by, pc):
+    """seed -> packed entry of the selected partitionings, and the seeds of
+    the entries that are not selected."""
+    nsel = O.footprint_info(bx, by)["selected_p%d" % pc]
+    sel = {O.partition_entry(bx, by, pc, e)[1]["seed"]: e for e in range(nsel)}
+    unsel = [O.partition_entry(bx, by, pc, e)[1]["seed"] for e in range(nsel, 1024)]
+    return sel, unsel
+
+
+@pytest.mark.parametrize("fp", [(4, 4), (6, 6)])
+@pytest.mark.parametrize("pc", [2, 3, 4])
+def test_seeded_candidates_up_to_four_are_taken_as_listed(fp, pc):
+    """Up to 4 seeds: the selected ones in list order, unselected seeds
+    skipped, a negative seed ends the list, -1 padding."""
+    bx, by = fp
+    sel, unsel = _seed_table(bx, by, pc)
+    rng = np.random.default_rng(400 + pc + bx)
+    img = rng.integers(0, 256, (by, bx, 4), dtype=np.uint8)
+    seeds_sel = [int(v) for v in rng.permutation(sorted(sel))[:4]]
+    lists = [[seeds_sel[0], unsel[0], seeds_sel[1], seeds_sel[2]] if unsel else seeds_sel,
+             [seeds_sel[3], -1, seeds_sel[0]], [unsel[-1]] if unsel else [-1], seeds_sel[:2]]
+    for lst in lists:
+        exp = []
+        for sd in lst:
+            if sd < 0:
+                break
+            if sd in sel:
+                exp.append(sel[sd])
+        exp += [-1] * (4 - len(exp))
+        assert O.seeded_candidates(img, bx, by, 0, pc, lst) == exp, (lst, exp)
+
+
+@pytest.mark.parametrize("fp", [(4, 4), (6, 6)])
+@pytest.mark.parametrize("pc", [2, 3, 4])
+def test_seeded_candidates_rank_the_painted_partitioning_first(fp, pc):
+    """More than 4 seeds are ranked by the line fits: a block painted from
+    one selected partitioning puts that partitioning first wherever its seed
+    sits in a list of 12 (unselected seeds included), and every candidate
+    is one of the listed selected partitionings."""
+    bx, by = fp
+    sel, unsel = _seed_table(bx, by, pc)
+    rng = np.random.default_rng(420 + pc + bx)
+    cols = np.array([[10, 10, 10, 255], [240, 20, 20, 255], [20, 240, 20, 255], [20, 20, 240, 255]], np.uint8)
+    keys = sorted(sel)
+    for it in range(8):
+        lst = [int(v) for v in rng.permutation(keys)[:12]]
+        if unsel:
+            lst[int(rng.integers(0, 12))] = unsel[it % len(unsel)]
+        target = lst[it % 12] if lst[it % 12] in sel else lst[(it + 1) % 12]
+        pot, _ = O.partition_entry(bx, by, pc, sel[target])
+        img = cols[np.array(pot)].reshape(by, bx, 4)
+        got = O.seeded_candidates(img, bx, by, 0, pc, lst)
+        assert got[0] == sel[target], (pc, lst, target, got)
+        listed = {sel[v] for v in lst if v in sel}
+        assert all(g in listed for g in got if g >= 0) and len({g for g in got if g >= 0}) == 4
+
+
+@pytest.mark.parametrize("pc", [2, 3, 4])
+def test_seeded_candidates_over_every_selected_seed_equal_the_exhaustive_search(pc):
+    """With every selected partitioning's seed listed (in packed order), the
+    line fits see the same partitionings as the exhaustive search (which
+    visits them in mismatch order): on noisy blocks without tied errors both
+    keep the same 4 candidates."""
+    bx = by = 4
+    sel, unsel = _seed_table(bx, by, pc)
+    lst = sorted(sel, key=lambda sd: sel[sd]) + unsel[:5]
+    rng = np.random.default_rng(440 + pc)
+    for it in range(6):
+        img = np.clip(rng.normal(128, 50, (by, bx, 4)), 0, 255).astype(np.uint8)
+        if it % 2:
+            img[..., 3] = 255
+        assert O.seeded_candidates(img, bx, by, 0, pc, lst) == O.exhaustive_candidates(img, bx, by, 0, pc)
+
+
+@pytest.mark.parametrize("pc", [2, 3, 4])
+def test_seeded_candidates_after_the_mismatch_order(pc):
+    """More than 4 seeds after the first N partitionings of the mismatch
+    order: with N covering every selected partitioning the line fits visit
+    exactly the exhaustive search's sequence (the seeds are all among them
+    and skipped), so the candidates equal the exhaustive search's even where
+    errors tie (a constant block); a block painted from one of the seeds'
+    partitionings puts it first, fitted once."""
+    bx = by = 4
+    sel, unsel = _seed_table(bx, by, pc)
+    rng = np.random.default_rng(460 + pc)
+    nsel = len(sel)
+    lst = [int(v) for v in rng.permutation(sorted(sel))[:12]]
+    for it in range(5):
+        img = np.clip(rng.normal(128, 50, (by, bx, 4)), 0, 255).astype(np.uint8)
+        if it == 4:
+            img[:] = 77  # every line fit ties
+        assert O.seeded_candidates(img, bx, by, 0, pc, lst, nsel) == O.exhaustive_candidates(img, bx, by, 0, pc)
+    cols = np.array([[10, 10, 10, 255], [240, 20, 20, 255], [20, 240, 20, 255], [20, 20, 240, 255]], np.uint8)
+    pot, _ = O.partition_entry(bx, by, pc, sel[lst[5]])
+    img = cols[np.array(pot)].reshape(by, bx, 4)
+    got = O.seeded_candidates(img, bx, by, 0, pc, lst, 2)
+    assert got[0] == sel[lst[5]], got
+    assert len({g for g in got if g >= 0}) == 4
