@@ -74,8 +74,8 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="batch workloads: capture the batched call's launch chain in a HIP graph and replay it")
     ap.add_argument("--also", default="6x6",
-                    help="comma-separated further footprints timed for both policies and reported in config "
-                         "(1 warm-up, 2 steps each; empty = none)")
+                    help="comma-separated further footprints timed for the same policies and reported in config "
+                         "(1 warm-up, 2 steps each; 'none' or empty = none)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one streaming kernel over this many MiB (read once, written once) before "
                          "the encode: a known-size reference dispatch for the FETCH_SIZE / WRITE_SIZE counters")
@@ -368,7 +368,7 @@ def main():
 
     runs = run_policies(bx, by, a.steps, a.warmup, a.copy_probe_mib)
     others = {}
-    for fp in [f for f in a.also.split(",") if f and f != a.block]:
+    for fp in [f for f in a.also.split(",") if f and f != "none" and f != a.block]:
         obx, oby = (int(v) for v in fp.split("x"))
         others[fp] = summary(run_policies(obx, oby, 2, 1, 0))
     # the learned encoder is the headline when it stays within the quality
