@@ -275,13 +275,14 @@ hipError_t upload_tables(FpState& F) {
             if (f.bm[k].dm == m) list.push_back((uint16_t)k);
     }
     off[f.ndm] = (uint16_t)list.size();
-    // packed per-block row of quantised weights: each trial mode keeps only
-    // its own weight count
+    // quantised weights of a trial mode: a region of npb rows, one row of
+    // the mode's weight count rounded up to 4 per partitioned block, so the
+    // rows are written and read as whole dwords
     std::vector<uint16_t> qw_off(std::max(f.nbm, 1), 0);
     int qw_total = 0;
     for (int k = 0; k < f.nbm; ++k) {
         qw_off[k] = (uint16_t)qw_total;
-        qw_total += f.dm[f.bm[k].dm].weight_count;
+        qw_total += (f.dm[f.bm[k].dm].weight_count + 3) & ~3;
     }
     std::vector<uint16_t> ordering(3 * 2 * 128);
     for (int p = 0; p < 3; ++p)
@@ -418,7 +419,7 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[10] = L.add<uint16_t>(npb * kColorTabRows);
     offs[4] = L.add<Ccr>(npb * f.nbm);
     int qw_total = 0;
-    for (int k = 0; k < f.nbm; ++k) qw_total += f.dm[f.bm[k].dm].weight_count;
+    for (int k = 0; k < f.nbm; ++k) qw_total += (f.dm[f.bm[k].dm].weight_count + 3) & ~3;
     offs[5] = L.add<uint8_t>(npb * (size_t)((qw_total + 15) & ~15));
     offs[6] = L.add<int16_t>(npb * 8);
     offs[7] = L.add<CandOut>(npb * 8);

@@ -37,7 +37,7 @@ struct DevTables {
     const uint8_t* raw_wc;        // [2048]
     const uint8_t* raw_quant;     // [2048]
     const uint16_t* seed_to_packed; // [3][1024] partition seed -> packed table entry
-    const uint16_t* qw_off;       // [nbm] offset of a trial mode's weights in a block's packed weight row
+    const uint16_t* qw_off;       // [nbm] start of a trial mode's weight region, in units of npb bytes
     int qw_stride;                // bytes of packed quantised weights per partitioned block
     const DualMode* dual;         // [ndual] dual-plane modes of the optional dual-plane search
     int ndual;
@@ -197,7 +197,7 @@ struct ChunkBuffers {
     float* ct_err;      // [17][13][npb of the launch]
     uint16_t* ct_fmt;   // [17][13][npb of the launch]
     Ccr* ccr;           // [nbm][npb of the launch]
-    uint8_t* qw;        // [qw_stride][npb of the launch]: row qw_off[mode] + weight
+    uint8_t* qw;        // per trial mode a region at qw_off[mode] * npb: [npb][weight count rounded up to 4]
     int16_t* top;       // [4 * max_nb][8]
     CandOut* cand;      // [4 * max_nb][8]
     float* cand_err;    // [4 * max_nb][8] each slot's CandOut::error again, packed for k_pick

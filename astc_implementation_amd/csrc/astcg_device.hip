@@ -1700,7 +1700,9 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
         const BmInfo bmi = T.bm[k];
         Ccr out;
         out.ql = -1; out.ql_mod = -1; out.formats = 0;
-        uint8_t* qwo = qw_out + (size_t)T.qw_off[k] * npb + pb;  // [weight row][PB]: coalesced
+        // the mode's region, this PB's row (the weight count rounded up to 4)
+        const int wcp = (di.weight_count + 3) & ~3;
+        uint32_t* qwo = reinterpret_cast<uint32_t*>(qw_out + (size_t)T.qw_off[k] * npb + (size_t)pb * wcp);
         const int q = bmi.quant;
         const int bc = kFreeBits[pc - 1] - bmi.weight_bits;
         float we;
@@ -1723,6 +1725,7 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
             const uint32_t qmax = wf2u(qlm1);
             // only whole groups of four weights reach the output (pass07:322-331)
             const int keep = (wc / 4) * 4;
+            uint32_t word = 0;
             for (int i = 0; i < wc; ++i) {
                 const float ix = wclamp(dw[i * 64] * scale - slb, 0.0f, 1.0f);
                 const float ix1 = ix * qlm1;
@@ -1731,7 +1734,11 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
                 const int il = s_q2u[q * 32 + wl], ih = s_q2u[q * 32 + wh];
                 const float fl = (float)il, fh = (float)ih;
                 const bool use_high = (fl + fh) < (128.0f * ix);
-                qwo[(size_t)i * npb] = i < keep ? (uint8_t)(use_high ? ih : il) : 0;
+                word |= (uint32_t)(i < keep ? (use_high ? ih : il) : 0) << (8 * (i & 3));
+                if ((i & 3) == 3 || i == wc - 1) {
+                    qwo[i >> 2] = word;
+                    word = 0;
+                }
                 aux[i * 64] = (use_high ? fh : fl) * rscale + lo;
             }
             float err = 0.0f;
@@ -2485,8 +2492,13 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
     const int ql = cr.ql, qlm = cr.ql_mod;
     const uint16_t in_formats = cr.formats;
     {
-        const uint8_t* qsrc = qw_all + (size_t)T.qw_off[k] * npb + pb;
-        for (int i = 0; i < wc; ++i) qw[i * 64] = qsrc[(size_t)i * npb];
+        // this PB's row of the mode's region, read as dwords
+        const uint32_t* qsrc = reinterpret_cast<const uint32_t*>(qw_all + (size_t)T.qw_off[k] * npb +
+                                                                 (size_t)pb * ((wc + 3) & ~3));
+        for (int i = 0; i < wc; i += 4) {
+            const uint32_t v = qsrc[i >> 2];
+            for (int j = 0; j < 4 && i + j < wc; ++j) qw[(i + j) * 64] = (uint8_t)(v >> (8 * j));
+        }
     }
     const float lsw = (kCW[0] + kCW[1]) + kCW[2];
     const float cw[4] = {kCW[0], kCW[1], kCW[2], kCW[3]};
