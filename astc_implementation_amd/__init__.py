@@ -107,6 +107,12 @@ POLICY_LEARNED = 1
 POLICY_EXHAUSTIVE = 2
 
 
+class Tuning(ctypes.Structure):
+    """astcg_tuning: launch shapes of one footprint's search kernels."""
+    _fields_ = [("refine_stage_max_tc", ctypes.c_uint32), ("ex_f4_max_tc", ctypes.c_uint32),
+                ("exw_max_list", ctypes.c_uint32)]
+
+
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("total_ms", ctypes.c_double), ("launches", ctypes.c_uint64)]
 
@@ -159,6 +165,9 @@ def library():
         "astcg_physical_to_symbolic": (i32, [u32, u32, vp, vp]),
         "astcg_kernel_times": (i32, [vp, vp, sz, ctypes.POINTER(sz)]),
         "astcg_reset_kernel_times": (i32, [vp]),
+        "astcg_get_tuning": (i32, [vp, u32, u32, vp]),
+        "astcg_set_tuning": (i32, [vp, u32, u32, vp]),
+        "astcg_autotune": (i32, [vp, u32, u32, ctypes.c_char_p, vp]),
         "astcg_last_error": (ctypes.c_char_p, [vp]),
         "astcg_status_string": (ctypes.c_char_p, [i32]),
     }
@@ -324,6 +333,26 @@ class Encoder:
         params, hidden, prune = load_predictor_file(path or predictor_path(bx, by))
         prune.update(learned)
         self.set_predictor(bx, by, params, hidden=hidden, **prune)
+
+    def tuning(self, bx, by):
+        """The launch shapes of footprint bx x by (a dict of astcg_tuning)."""
+        t = Tuning()
+        _check(library().astcg_get_tuning(self._h, bx, by, ctypes.byref(t)), self._h)
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, bx, by, **fields):
+        """Changes some launch shapes of footprint bx x by (no result changes)."""
+        t = Tuning(**self.tuning(bx, by))
+        for k, v in fields.items():
+            setattr(t, k, v)
+        _check(library().astcg_set_tuning(self._h, bx, by, ctypes.byref(t)), self._h)
+
+    def autotune(self, bx, by, cache_path=None):
+        """astcg_autotune: the cached or the measured launch shapes of bx x by."""
+        t = Tuning()
+        path = os.fsencode(cache_path) if cache_path else None
+        _check(library().astcg_autotune(self._h, bx, by, path, ctypes.byref(t)), self._h)
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def fallback_rate(self):
         r = ctypes.c_double()

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <cstring>
 #include <fstream>
@@ -403,6 +404,9 @@ hipError_t upload_tables(FpState& F) {
     T.ndm = f.ndm;
     T.nbm = f.nbm;
     for (int i = 0; i < 4; ++i) T.count_selected[i] = f.count_selected[i];
+    T.refine_stage_max_tc = kRefineStageMaxTc;
+    T.ex_f4_max_tc = kExF4MaxTc;
+    T.exw_max_list = kExwMaxList;
     int maxw = 4;
     for (const DmInfo& d : f.dm) maxw = std::max<int>(maxw, d.weight_count);
     F.wstride = (maxw + 3) & ~3;
@@ -430,7 +434,55 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
 }
 
 // Tables of footprint bx x by, uploaded on first use.
-astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpState** out) {
+// ---- launch-shape tuning and its cache file ----------------------------------
+// One line per GPU and footprint:
+//   astcg-tune 1 <gcnArchName> <CU count> <bx>x<by> <refine_stage_max_tc> <ex_f4_max_tc> <exw_max_list>
+std::string tune_device_id(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown 0";
+    std::string arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == ' ') c = '_';
+    return arch + " " + std::to_string(prop.multiProcessorCount);
+}
+
+bool tune_cache_lookup(const char* path, const std::string& dev, uint32_t bx, uint32_t by, astcg_tuning* t) {
+    FILE* f = path ? std::fopen(path, "r") : nullptr;
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    const std::string want = dev + " " + std::to_string(bx) + "x" + std::to_string(by) + " ";
+    while (std::fgets(line, sizeof(line), f)) {
+        const std::string l(line);
+        const std::string head = "astcg-tune 1 ";
+        if (l.compare(0, head.size(), head) != 0 || l.compare(head.size(), want.size(), want) != 0) continue;
+        unsigned r = 0, x = 0, w = 0;
+        if (std::sscanf(l.c_str() + head.size() + want.size(), "%u %u %u", &r, &x, &w) == 3 && x <= 64) {
+            t->refine_stage_max_tc = r;
+            t->ex_f4_max_tc = x;
+            t->exw_max_list = w;
+            found = true;  // the last entry of the file wins
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void apply_tuning(FpState& F, const astcg_tuning& t) {
+    F.T.refine_stage_max_tc = (int)std::min<uint32_t>(t.refine_stage_max_tc, 1u << 20);
+    F.T.ex_f4_max_tc = (int)std::min<uint32_t>(t.ex_f4_max_tc, 64u);
+    F.T.exw_max_list = (int)std::min<uint32_t>(t.exw_max_list, 0x7FFFFFFFu);
+}
+
+astcg_tuning tuning_of(const FpState& F) {
+    astcg_tuning t;
+    t.refine_stage_max_tc = (uint32_t)F.T.refine_stage_max_tc;
+    t.ex_f4_max_tc = (uint32_t)F.T.ex_f4_max_tc;
+    t.exw_max_list = (uint32_t)F.T.exw_max_list;
+    return t;
+}
+
+astcg_status get_footprint_mut(astcg_encoder* e, uint32_t bx, uint32_t by, FpState** out) {
     const int key = (int)(bx * 16 + by);
     auto it = e->fps.find(key);
     if (it == e->fps.end()) {
@@ -441,10 +493,21 @@ astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpS
             if (F.d_tables) (void)hipFree(F.d_tables);
             return hip_fail(e, err, "table upload");
         }
+        // launch shapes tuned earlier for this GPU, if the cache names them
+        astcg_tuning t;
+        const char* cache = std::getenv("ASTCG_TUNE_CACHE");
+        if (cache && tune_cache_lookup(cache, tune_device_id(e->device), bx, by, &t)) apply_tuning(F, t);
         it = e->fps.emplace(key, F).first;
     }
     *out = &it->second;
     return ASTCG_OK;
+}
+
+astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpState** out) {
+    FpState* F = nullptr;
+    astcg_status st = get_footprint_mut(e, bx, by, &F);
+    *out = F;
+    return st;
 }
 
 // Blocks per chunk: keeps the per-chunk working set near 16 GiB, so a 4096^2
@@ -768,7 +831,8 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
                               labels ? labels->best_seed : nullptr, labels ? labels->cand_seed : nullptr,
                               labels ? labels->cand_err : nullptr};
         put(ptrs, sizeof(ptrs));
-        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk, exhaustive};
+        const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk, exhaustive, F.T.refine_stage_max_tc,
+                            F.T.ex_f4_max_tc, F.T.exw_max_list};
         put(ints, sizeof(ints));
         put(&A, sizeof(A));
         if (pred) put(&pred->lp, sizeof(pred->lp));
@@ -1451,6 +1515,172 @@ astcg_status astcg_set_predictor(astcg_encoder* e, uint32_t bx, uint32_t by, uin
         return hip_fail(e, err, "predictor upload");
     }
     e->preds[key] = P;
+    return ASTCG_OK;
+}
+
+astcg_status astcg_get_tuning(astcg_encoder* e, uint32_t bx, uint32_t by, astcg_tuning* out) {
+    if (!e || !out) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    FpState* F = nullptr;
+    astcg_status st = get_footprint_mut(e, bx, by, &F);
+    if (st != ASTCG_OK) return st;
+    *out = tuning_of(*F);
+    return ASTCG_OK;
+}
+
+astcg_status astcg_set_tuning(astcg_encoder* e, uint32_t bx, uint32_t by, const astcg_tuning* t) {
+    if (!e || !t) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    if (t->ex_f4_max_tc > 64) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ex_f4_max_tc above 64");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    FpState* F = nullptr;
+    astcg_status st = get_footprint_mut(e, bx, by, &F);
+    if (st != ASTCG_OK) return st;
+    apply_tuning(*F, *t);
+    return ASTCG_OK;
+}
+
+namespace {
+// The tuning image: two colour gradients, block-scale noise and sharp-edged
+// rectangles (xorshift64, fixed seed), so that every partition count and
+// most trial modes are exercised.
+void tune_image(int w, int h, std::vector<uint8_t>& px) {
+    uint64_t s = 0x2545F4914F6CDD1Dull;
+    auto next = [&s]() {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        return s;
+    };
+    auto unit = [&next]() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); };
+    double g[2][3], o[2][3], d[2][2];
+    for (int k = 0; k < 2; ++k) {
+        const double a = unit() * 6.283185307179586;
+        d[k][0] = std::cos(a) / w;
+        d[k][1] = std::sin(a) / h;
+        for (int c = 0; c < 3; ++c) {
+            g[k][c] = unit() * 160.0 - 80.0;
+            o[k][c] = unit() * 100.0;
+        }
+    }
+    px.assign((size_t)w * h * 4, 0);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            uint64_t hsh = (uint64_t)(x / 3) * 0x9E3779B97F4A7C15ull ^ (uint64_t)(y / 3) * 0xC2B2AE3D27D4EB4Full;
+            hsh ^= hsh >> 29;
+            for (int c = 0; c < 3; ++c) {
+                double v = 40.0;
+                for (int k = 0; k < 2; ++k) v += o[k][c] + g[k][c] * (x * d[k][0] + y * d[k][1]);
+                v += (double)((hsh >> (8 * c)) & 15) - 7.5;
+                px[((size_t)y * w + x) * 4 + c] = (uint8_t)std::min(255.0, std::max(0.0, v));
+            }
+            px[((size_t)y * w + x) * 4 + 3] = 255;
+        }
+    for (int r = 0; r < 40; ++r) {
+        const int x0 = (int)(unit() * w), y0 = (int)(unit() * h);
+        const int x1 = std::min(w, x0 + 4 + (int)(unit() * w / 6)), y1 = std::min(h, y0 + 4 + (int)(unit() * h / 6));
+        uint8_t col[3];
+        for (int c = 0; c < 3; ++c) col[c] = (uint8_t)(unit() * 255.0);
+        for (int y = y0; y < y1; ++y)
+            for (int x = x0; x < x1; ++x)
+                for (int c = 0; c < 3; ++c) px[((size_t)y * w + x) * 4 + c] = col[c];
+    }
+}
+}  // namespace
+
+astcg_status astcg_autotune(astcg_encoder* e, uint32_t bx, uint32_t by, const char* cache_path, astcg_tuning* out) {
+    if (!e) return ASTCG_ERR_INVALID_ARGUMENT;
+    if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
+    hipError_t err = hipSetDevice(e->device);
+    if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
+    FpState* F = nullptr;
+    astcg_status st = get_footprint_mut(e, bx, by, &F);
+    if (st != ASTCG_OK) return st;
+    const std::string dev = tune_device_id(e->device);
+    astcg_tuning t;
+    if (cache_path && tune_cache_lookup(cache_path, dev, bx, by, &t)) {
+        apply_tuning(*F, t);
+        if (out) *out = t;
+        return ASTCG_OK;
+    }
+    // the caller's configuration, restored afterwards
+    const bool was_configured = e->configured;
+    const uint32_t sw = e->width, sh = e->height, sbx = e->bx, sby = e->by;
+    const astcg_params sp = e->params;
+    const int W = 1024, H = 1024;
+    std::vector<uint8_t> img;
+    tune_image(W, H, img);
+    const size_t nb = (size_t)((W + bx - 1) / bx) * ((H + by - 1) / by);
+    uint8_t *d_in = nullptr, *d_out = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    err = hipMalloc(reinterpret_cast<void**>(&d_in), img.size());
+    if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&d_out), nb * 16);
+    if (err == hipSuccess) err = hipMemcpy(d_in, img.data(), img.size(), hipMemcpyHostToDevice);
+    for (int i = 0; i < 2 && err == hipSuccess; ++i) err = hipEventCreate(&ev[i]);
+    auto time_encode = [&](uint32_t policy, float* ms) -> astcg_status {
+        astcg_params p;
+        astcg_default_params(&p);
+        p.policy = policy;
+        astcg_status s2 = astcg_configure(e, W, H, bx, by, &p);
+        if (s2 == ASTCG_OK) s2 = astcg_encode_device(e, d_in, d_out, e->stream);
+        hipError_t e2 = hipSuccess;
+        if (s2 == ASTCG_OK) e2 = hipEventRecord(ev[0], e->stream);
+        if (s2 == ASTCG_OK && e2 == hipSuccess) s2 = astcg_encode_device(e, d_in, d_out, e->stream);
+        if (s2 == ASTCG_OK && e2 == hipSuccess) e2 = hipEventRecord(ev[1], e->stream);
+        if (s2 == ASTCG_OK && e2 == hipSuccess) e2 = hipEventSynchronize(ev[1]);
+        if (s2 == ASTCG_OK && e2 == hipSuccess) e2 = hipEventElapsedTime(ms, ev[0], ev[1]);
+        if (s2 == ASTCG_OK && e2 != hipSuccess) s2 = hip_fail(e, e2, "autotune timing");
+        return s2;
+    };
+    t = tuning_of(*F);
+    st = err == hipSuccess ? ASTCG_OK : hip_fail(e, err, "autotune buffers");
+    // each setting in turn, under the search whose dominant kernel it shapes
+    struct Knob {
+        uint32_t* field;
+        uint32_t choices[2];
+        uint32_t policy;
+    };
+    Knob knobs[2] = {{&t.refine_stage_max_tc, {0u, 1u << 20}, ASTCG_POLICY_COMPAT},
+                     {&t.ex_f4_max_tc, {0u, 64u}, ASTCG_POLICY_EXHAUSTIVE}};
+    for (int kn = 0; kn < 2 && st == ASTCG_OK; ++kn) {
+        if (kn == 1 && F->fp->texel_count > 64) {
+            t.ex_f4_max_tc = 0;  // float staging only up to 64 texels
+            continue;
+        }
+        float best_ms = 0.0f;
+        uint32_t best = *knobs[kn].field;
+        for (int c = 0; c < 2 && st == ASTCG_OK; ++c) {
+            *knobs[kn].field = knobs[kn].choices[c];
+            apply_tuning(*F, t);
+            float ms = 0.0f;
+            st = time_encode(knobs[kn].policy, &ms);
+            if (st == ASTCG_OK && (c == 0 || ms < best_ms)) {
+                best_ms = ms;
+                best = knobs[kn].choices[c];
+            }
+        }
+        *knobs[kn].field = best;
+        apply_tuning(*F, t);
+    }
+    for (int i = 0; i < 2; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    if (st != ASTCG_OK) return st;
+    if (cache_path) {
+        FILE* f = std::fopen(cache_path, "a");
+        if (!f) return fail(e, ASTCG_ERR_IO, "cannot append to the tuning cache");
+        std::fprintf(f, "astcg-tune 1 %s %ux%u %u %u %u\n", dev.c_str(), bx, by, t.refine_stage_max_tc,
+                     t.ex_f4_max_tc, t.exw_max_list);
+        const bool ok = std::fclose(f) == 0;
+        if (!ok) return fail(e, ASTCG_ERR_IO, "cannot write the tuning cache");
+    }
+    if (out) *out = t;
+    e->configured = false;
+    if (was_configured) return astcg_configure(e, sw, sh, sbx, sby, &sp);
     return ASTCG_OK;
 }
 

@@ -49,7 +49,16 @@ struct DevTables {
     const uint8_t* dm_order;      // [ndm]
     int dm_group_start[kDmGroups + 1];  // group g = dm_order[dm_group_start[g] .. dm_group_start[g + 1])
     int dm_group_w[kDmGroups];          // LDS row (weights, multiple of 4) of group g
+    // launch shapes (astcg_tuning), none of which changes a result
+    int refine_stage_max_tc;      // k_refine stages the block's RGBA8 texels in LDS up to this texel count
+    int ex_f4_max_tc;             // k_partition_ex stages float texels in LDS up to this count (<= 64)
+    int exw_max_list;             // exhaustive block lists up to this length run k_partition_exw
 };
+
+// Launch-shape defaults (measured on the 4096x4096 benchmark images).
+constexpr int kRefineStageMaxTc = 25;  // larger blocks ran faster unstaged, with more workgroups per CU
+constexpr int kExF4MaxTc = 16;         // keeps 6 k_partition_ex workgroups per CU
+constexpr int kExwMaxList = 65536;     // lane-per-block throughput is higher once its waves fill the chip
 
 // One texture of a launch: its blocks occupy [first_block, first_block +
 // num_blocks) of the launch's block index space; `out` receives its 16-byte

@@ -853,17 +853,15 @@ __global__ void __launch_bounds__(64) k_need_list(ImageArgs A, SearchPlan P, int
     if (need) list[base + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = k;
 }
 
-// Exhaustive lists longer than this run on k_partition_ex (a lane per
-// block), whose throughput per block is higher once its waves fill the
+// Exhaustive lists longer than T.exw_max_list run on k_partition_ex (a lane
+// per block), whose throughput per block is higher once its waves fill the
 // chip; shorter ones on k_partition_exw (a wave per block).
-constexpr int kExwMaxList = 65536;
 
-// Texels staged as converted floats up to 4x4 (LDS per workgroup stays
-// small enough for 6 workgroups per CU), as RGBA8 words above.
-constexpr int kExF4MaxTc = 16;
-
+// Texels staged as converted floats up to T.ex_f4_max_tc texels (4x4 by
+// default: LDS per workgroup stays small enough for 6 workgroups per CU), as
+// RGBA8 words above.
 size_t partition_ex_lds_bytes(const DevTables& T) {
-    const size_t per_texel = T.tc <= kExF4MaxTc ? 16 : 4;
+    const size_t per_texel = T.tc <= T.ex_f4_max_tc ? 16 : 4;
     return (size_t)T.tc * 64 * (per_texel + 1);
 }
 
@@ -884,7 +882,7 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
     // scratch column of this lane (blocks of a list use their list slot;
     // short exhaustive lists are k_partition_exw's)
     const int col = blockIdx.x * 64 + lane;
-    if (list && !kSeeded && *list_count <= kExwMaxList) return;
+    if (list && !kSeeded && *list_count <= T.exw_max_list) return;
     const int k = list ? (col < *list_count ? list[col] : nb) : col;
     uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);             // [tc][64] RGBA8 (!kF4)
     float4* s_f4 = reinterpret_cast<float4*>(smem);                    // [tc][64] search values (kF4)
@@ -2361,14 +2359,13 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 // of pass13 are accumulated partition by partition (each partition still
 // sees its texels in index order, so the float results are those of the
 // interleaved loop).
-constexpr int kRefineStageMaxTc = 25;  // stage RGBA8 texels in LDS up to 5x5 (measured: larger blocks run faster unstaged, with more workgroups per CU)
 
 // LDS of one k_refine workgroup: staged texels, unpacked endpoints of the
 // pc partitions, the texel partition map (pc > 1 only), candidate and best
 // quantised weights.  A smaller 1-partition layout fits more workgroups per CU.
 size_t refine_lds_bytes(const DevTables& T, int wstride, int pc) {
     size_t bytes = 2 * (size_t)pc * 64 * sizeof(uchar4) + (pc > 1 ? (size_t)T.tc * 64 : 0) + 2 * (size_t)wstride * 64;
-    if (T.tc <= kRefineStageMaxTc) bytes += (size_t)T.tc * 64 * sizeof(uint32_t);
+    if (T.tc <= T.refine_stage_max_tc) bytes += (size_t)T.tc * 64 * sizeof(uint32_t);
     return bytes;
 }
 
@@ -3922,7 +3919,7 @@ __global__ void __launch_bounds__(64) k_partition_exw(ImageArgs A, DevTables T, 
     int* s_start = reinterpret_cast<int*>(s_es + kPartitionings);                     // [64]
     int* s_hist = s_start + 64;                                                        // [64]
     const int n = *list_count, nsel = T.count_selected[pc - 1], nk = min(tc, 64);
-    if (n > kExwMaxList) return;  // k_partition_ex takes this list
+    if (n > T.exw_max_list) return;  // k_partition_ex takes this list
     auto PX = [&](int t, float* o) __attribute__((always_inline)) {
         const float4 f = s_px[t];
         o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
@@ -4164,7 +4161,7 @@ void launch_partition_ex(bool seeded, int pc, const DevTables& T, dim3 grid, siz
     if (pc == 2) { if (f4) ASTCG_PEX(2, true, SD); else ASTCG_PEX(2, false, SD); } \
     else if (pc == 3) { if (f4) ASTCG_PEX(3, true, SD); else ASTCG_PEX(3, false, SD); } \
     else { if (f4) ASTCG_PEX(4, true, SD); else ASTCG_PEX(4, false, SD); }
-    const bool f4 = T.tc <= kExF4MaxTc;
+    const bool f4 = T.tc <= T.ex_f4_max_tc;
     if (seeded) { ASTCG_PEX_PC(true) }
     else { ASTCG_PEX_PC(false) }
 #undef ASTCG_PEX_PC
@@ -4323,7 +4320,7 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             if (prof) prof->begin(K_REFINE, s);
             const size_t lds = refine_lds_bytes(T, B.wstride, pc);
             const unsigned grid = blocks_for((long long)npb * 8 + (long long)T.nbm * 63);
-            launch_refine(T.tc <= kRefineStageMaxTc, pc, dim3(grid), lds, s, A, T, B, npb,
+            launch_refine(T.tc <= T.refine_stage_max_tc, pc, dim3(grid), lds, s, A, T, B, npb,
                           plan ? plan->iterations : 6);
             if (prof) prof->end(K_REFINE, s);
         }

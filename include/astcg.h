@@ -109,6 +109,33 @@ typedef struct {
 
 void astcg_default_learned_params(astcg_learned_params* p);
 
+/* Launch shapes of one footprint's search kernels.  No counterpart in the
+ * reference (its dispatch sizes are fixed in the WGSL); none of them changes
+ * any encoded block, only how the work is laid out on the GPU. */
+typedef struct {
+    uint32_t refine_stage_max_tc; /* the refinement kernel stages a block's RGBA8 texels in LDS when the footprint has
+                                     at most this many texels (default 25) */
+    uint32_t ex_f4_max_tc;        /* the exhaustive partition search stages float texels in LDS up to this many
+                                     texels, at most 64 (default 16); RGBA8 words above */
+    uint32_t exw_max_list;        /* exhaustive searches of block lists (LEARNED hard / fallback blocks) up to this
+                                     many blocks run one wave per block, longer ones one lane per block
+                                     (default 65536) */
+} astcg_tuning;
+
+astcg_status astcg_get_tuning(astcg_encoder* enc, uint32_t bx, uint32_t by, astcg_tuning* out);
+astcg_status astcg_set_tuning(astcg_encoder* enc, uint32_t bx, uint32_t by, const astcg_tuning* tuning);
+/* Chooses the launch shapes of footprint bx x by for this GPU.  With a
+ * cache_path that holds an entry for this GPU (architecture and CU count) and
+ * footprint, that entry is applied; otherwise each setting is timed on
+ * encodes of a synthetic 1024x1024 image (the LDS staging of the refinement
+ * under the COMPAT search, the float staging of the exhaustive partition
+ * search under the EXHAUSTIVE search), the faster one kept, and, with a
+ * cache_path, appended to that file.  An encoder whose footprint tables are
+ * first built while the environment variable ASTCG_TUNE_CACHE names such a
+ * file applies its entry the same way.  The encoder's configuration is kept
+ * (configure is re-run with the last parameters). */
+astcg_status astcg_autotune(astcg_encoder* enc, uint32_t bx, uint32_t by, const char* cache_path, astcg_tuning* out);
+
 #define ASTCG_FLAG_PROFILE 1u  /* record HIP events around every kernel launch */
 #define ASTCG_FLAG_LABELS 2u   /* keep each partition count's best partitioning (astcg_get_labels) */
 /* Blocks whose texels are all equal are written as LDR void-extent blocks

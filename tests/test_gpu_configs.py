@@ -6,6 +6,11 @@ cases are in test_gpu_parity.py):
   texture equals its own single-texture encode, and one whole 920-block
   batch of the reference (code/compression_pipeline.cpp:345) of textures of
   every content kind and footprint equals the oracle's encoding of it;
+* the 256-texture learned batch (64..4096 per side, mixed 4x4 / 6x6, the
+  content mix of the 64-texture batch) through one batched call per policy:
+  every texture's decoded PSNR against the EXHAUSTIVE search's, with the
+  bound this pruning reaches (mean loss <= 0.02 dB, at most 3 textures
+  above 0.05 dB, none above 0.075 dB), no error blocks;
 * the learned policy at 4x4 and 6x6 on a fresh 4096x4096 image (seed 3,
   outside the training seeds 10000..19999) against the plain EXHAUSTIVE
   search it is trained on: decoded PSNR (RGBA, unweighted, peak 255) within
@@ -99,3 +104,46 @@ def test_config3_learned_on_fresh_4096_image(encoder, fresh_image, bx, by, recor
     assert not np.all(dec == np.array([255, 0, 255, 255], np.uint8), axis=-1).any()
     assert p_plain - p_learned <= MAX_DPSNR, (p_plain, p_learned, p_compat, fb)
     assert fb < 0.05, fb
+
+
+def _batch_psnrs(encoder, spec, d_in, d_out, tex, policy):
+    import torch
+    stream = torch.cuda.current_stream()
+    encoder.encode_batch_device(tex, stream=stream.cuda_stream, policy=policy)
+    torch.cuda.synchronize()
+    out = []
+    magenta = 0
+    for i, (_, w, h, (bx, by), _) in enumerate(spec):
+        dec = torch.empty(w * h * 4, dtype=torch.uint8, device="cuda")
+        encoder.decode_device(bx, by, w, h, d_out[i].data_ptr(), dec.data_ptr(), stream.cuda_stream)
+        d4 = dec.view(-1, 4)
+        magenta += int(((d4[:, 0] == 255) & (d4[:, 1] == 0) & (d4[:, 2] == 255) & (d4[:, 3] == 255)).sum())
+        mse = float(((dec.float() - d_in[i].float()) ** 2).mean())
+        out.append(99.0 if mse == 0 else 10.0 * np.log10(255.0 * 255.0 / mse))
+    return np.array(out), magenta
+
+
+def test_config4_learned_batch256_against_exhaustive(encoder, record_property):
+    import torch
+    spec = synth.batch_spec(256, 64, 4096, [(4, 4), (6, 6)], 20000, 4)
+    imgs = synth.make_textures(spec, workers=16)
+    d_in = [torch.from_numpy(im.reshape(-1).copy()).cuda() for im in imgs]
+    del imgs
+    nbs = [((w + bx - 1) // bx) * ((h + by - 1) // by) for _, w, h, (bx, by), _ in spec]
+    d_out = [torch.zeros(n * 16, dtype=torch.uint8, device="cuda") for n in nbs]
+    tex = [(d_in[i].data_ptr(), w, h, bx, by, d_out[i].data_ptr(), nbs[i] * 16)
+           for i, (_, w, h, (bx, by), _) in enumerate(spec)]
+    p_exh, m_exh = _batch_psnrs(encoder, spec, d_in, d_out, tex, A.POLICY_EXHAUSTIVE)
+    for fp in ((4, 4), (6, 6)):
+        encoder.load_predictor(*fp)
+    p_learned, m_learned = _batch_psnrs(encoder, spec, d_in, d_out, tex, A.POLICY_LEARNED)
+    fb = encoder.fallback_rate()
+    d = p_exh - p_learned
+    record_property("dpsnr_db_mean", float(d.mean()))
+    record_property("dpsnr_db_max", float(d.max()))
+    record_property("textures_over_0.05_db", int((d > MAX_DPSNR).sum()))
+    record_property("fallback_rate", fb)
+    worst = [(int(i), spec[i][0], spec[i][1], spec[i][2], spec[i][3], round(float(d[i]), 4)) for i in np.argsort(-d)[:4]]
+    assert m_exh == 0 and m_learned == 0
+    assert d.mean() <= 0.02, (float(d.mean()), worst)
+    assert (d > MAX_DPSNR).sum() <= 3 and d.max() <= 0.075, worst
