@@ -110,7 +110,8 @@ POLICY_EXHAUSTIVE = 2
 class Tuning(ctypes.Structure):
     """astcg_tuning: launch shapes of one footprint's search kernels."""
     _fields_ = [("refine_stage_max_tc", ctypes.c_uint32), ("ex_f4_max_tc", ctypes.c_uint32),
-                ("exw_max_list", ctypes.c_uint32)]
+                ("exw_max_list", ctypes.c_uint32), ("refine_queue", ctypes.c_uint32),
+                ("refine_classes", ctypes.c_uint32)]
 
 
 class KernelTime(ctypes.Structure):

@@ -407,6 +407,12 @@ hipError_t upload_tables(FpState& F) {
     T.refine_stage_max_tc = kRefineStageMaxTc;
     T.ex_f4_max_tc = kExF4MaxTc;
     T.exw_max_list = kExwMaxList;
+    T.refine_queue = kRefineQueue;
+    // grouping by format class pays where the refinement stages its texels
+    // (4x4 4096^2 images: -10 % refinement time); unstaged, the classes split
+    // neighbouring blocks' candidates apart and their texel reads lose cache
+    // locality (6x6: +4 %)
+    T.refine_classes = f.texel_count <= kRefineStageMaxTc ? 1 : 0;
     int maxw = 4;
     for (const DmInfo& d : f.dm) maxw = std::max<int>(maxw, d.weight_count);
     F.wstride = (maxw + 3) & ~3;
@@ -427,8 +433,9 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[5] = L.add<uint8_t>(npb * (size_t)((qw_total + 15) & ~15));
     offs[6] = L.add<int16_t>(npb * 8);
     offs[7] = L.add<CandOut>(npb * 8);
-    offs[8] = L.add<int>(2 * (size_t)f.nbm + 1);
-    offs[9] = L.add<int>(npb * 8 + (size_t)f.nbm * 64);
+    const size_t bins = (size_t)f.nbm * kRefineClasses;
+    offs[8] = L.add<int>(2 * bins + 2);
+    offs[9] = L.add<int>(npb * 8 + bins * 64);
     offs[11] = L.add<float>(npb * 8);
     return L.size;
 }
@@ -436,7 +443,9 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
 // Tables of footprint bx x by, uploaded on first use.
 // ---- launch-shape tuning and its cache file ----------------------------------
 // One line per GPU and footprint:
-//   astcg-tune 1 <gcnArchName> <CU count> <bx>x<by> <refine_stage_max_tc> <ex_f4_max_tc> <exw_max_list>
+//   astcg-tune 2 <gcnArchName> <CU count> <bx>x<by> <refine_stage_max_tc> <ex_f4_max_tc> <exw_max_list>
+//                <refine_queue> <refine_classes>
+// (lines of format 1, without the last two fields, keep their defaults)
 std::string tune_device_id(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown 0";
@@ -454,13 +463,20 @@ bool tune_cache_lookup(const char* path, const std::string& dev, uint32_t bx, ui
     const std::string want = dev + " " + std::to_string(bx) + "x" + std::to_string(by) + " ";
     while (std::fgets(line, sizeof(line), f)) {
         const std::string l(line);
-        const std::string head = "astcg-tune 1 ";
-        if (l.compare(0, head.size(), head) != 0 || l.compare(head.size(), want.size(), want) != 0) continue;
-        unsigned r = 0, x = 0, w = 0;
-        if (std::sscanf(l.c_str() + head.size() + want.size(), "%u %u %u", &r, &x, &w) == 3 && x <= 64) {
+        int version = 0;
+        for (int v = 1; v <= 2; ++v) {
+            const std::string head = "astcg-tune " + std::to_string(v) + " ";
+            if (l.compare(0, head.size(), head) == 0 && l.compare(head.size(), want.size(), want) == 0) version = v;
+        }
+        if (!version) continue;
+        unsigned r = 0, x = 0, w = 0, q = t->refine_queue, c = t->refine_classes;
+        const int n = std::sscanf(l.c_str() + 13 + want.size(), "%u %u %u %u %u", &r, &x, &w, &q, &c);
+        if (n == (version == 1 ? 3 : 5) && x <= 64 && q <= 64 && c <= 1) {
             t->refine_stage_max_tc = r;
             t->ex_f4_max_tc = x;
             t->exw_max_list = w;
+            t->refine_queue = q;
+            t->refine_classes = c;
             found = true;  // the last entry of the file wins
         }
     }
@@ -472,6 +488,8 @@ void apply_tuning(FpState& F, const astcg_tuning& t) {
     F.T.refine_stage_max_tc = (int)std::min<uint32_t>(t.refine_stage_max_tc, 1u << 20);
     F.T.ex_f4_max_tc = (int)std::min<uint32_t>(t.ex_f4_max_tc, 64u);
     F.T.exw_max_list = (int)std::min<uint32_t>(t.exw_max_list, 0x7FFFFFFFu);
+    F.T.refine_queue = (int)std::min<uint32_t>(t.refine_queue, 64u);
+    F.T.refine_classes = t.refine_classes ? 1 : 0;
 }
 
 astcg_tuning tuning_of(const FpState& F) {
@@ -479,6 +497,8 @@ astcg_tuning tuning_of(const FpState& F) {
     t.refine_stage_max_tc = (uint32_t)F.T.refine_stage_max_tc;
     t.ex_f4_max_tc = (uint32_t)F.T.ex_f4_max_tc;
     t.exw_max_list = (uint32_t)F.T.exw_max_list;
+    t.refine_queue = (uint32_t)F.T.refine_queue;
+    t.refine_classes = (uint32_t)F.T.refine_classes;
     return t;
 }
 
@@ -545,8 +565,9 @@ ChunkBuffers chunk_buffers_at(uint8_t* b, const FpState& F, const size_t* offs) 
     B.cand = reinterpret_cast<CandOut*>(b + offs[7]);
     B.cand_err = reinterpret_cast<float*>(b + offs[11]);
     B.bin_count = reinterpret_cast<int*>(b + offs[8]);
-    B.bin_cur = B.bin_count + F.fp->nbm;
-    B.work_total = B.bin_cur + F.fp->nbm;
+    B.bin_cur = B.bin_count + (size_t)F.fp->nbm * kRefineClasses;
+    B.work_total = B.bin_cur + (size_t)F.fp->nbm * kRefineClasses;
+    B.tile_next = B.work_total + 1;
     B.work = reinterpret_cast<int*>(b + offs[9]);
     B.wstride = F.wstride;
     return B;
@@ -567,7 +588,7 @@ astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStre
     }
     ChunkBuffers& B = e->B;
     B = chunk_buffers_at(static_cast<uint8_t*>(e->d_chunk), F, offs);
-    err = hipMemsetAsync(B.bin_count, 0, sizeof(int) * (2 * (size_t)F.fp->nbm + 1), s);
+    err = hipMemsetAsync(B.bin_count, 0, sizeof(int) * (2 * (size_t)F.fp->nbm * kRefineClasses + 2), s);
     if (err != hipSuccess) return hip_fail(e, err, "chunk buffers");
     e->layout_fp = &F;
     e->layout_chunk = chunk;
@@ -832,7 +853,7 @@ astcg_status run_encode(astcg_encoder* e, const FpState& F, ImageArgs A, int max
                               labels ? labels->cand_err : nullptr};
         put(ptrs, sizeof(ptrs));
         const int ints[] = {max_pc, chunk, (int)flags, e->layout_chunk, exhaustive, F.T.refine_stage_max_tc,
-                            F.T.ex_f4_max_tc, F.T.exw_max_list};
+                            F.T.ex_f4_max_tc, F.T.exw_max_list, F.T.refine_queue, F.T.refine_classes};
         put(ints, sizeof(ints));
         put(&A, sizeof(A));
         if (pred) put(&pred->lp, sizeof(pred->lp));
@@ -1083,7 +1104,7 @@ astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba
     SymBest* best = reinterpret_cast<SymBest*>(base + ((used + 255) & ~size_t(255)));
     // the scratch keeps the previous call's contents: the per-mode counters
     // must start at zero (the search clears them again after each use)
-    err = launch_zero_ints(B.bin_count, 2 * f.nbm + 1, s);
+    err = launch_zero_ints(B.bin_count, 2 * f.nbm * kRefineClasses + 2, s);
     if (err != hipSuccess) return hip_fail(e, err, "scratch clear");
     // each chunk is encoded as its own launch range: the texture's block
     // numbering starts at -blk0, so the chunk's blocks 0..n-1 are the image's
@@ -1534,6 +1555,8 @@ astcg_status astcg_set_tuning(astcg_encoder* e, uint32_t bx, uint32_t by, const 
     if (!e || !t) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "null pointer");
     if (!valid_footprint((int)bx, (int)by)) return fail(e, ASTCG_ERR_INVALID_FOOTPRINT, "not a 2D ASTC footprint");
     if (t->ex_f4_max_tc > 64) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "ex_f4_max_tc above 64");
+    if (t->refine_queue > 64) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "refine_queue above 64");
+    if (t->refine_classes > 1) return fail(e, ASTCG_ERR_INVALID_ARGUMENT, "refine_classes must be 0 or 1");
     hipError_t err = hipSetDevice(e->device);
     if (err != hipSuccess) return hip_fail(e, err, "hipSetDevice");
     FpState* F = nullptr;
@@ -1640,19 +1663,22 @@ astcg_status astcg_autotune(astcg_encoder* e, uint32_t bx, uint32_t by, const ch
     // each setting in turn, under the search whose dominant kernel it shapes
     struct Knob {
         uint32_t* field;
-        uint32_t choices[2];
+        int nchoices;
+        uint32_t choices[4];
         uint32_t policy;
     };
-    Knob knobs[2] = {{&t.refine_stage_max_tc, {0u, 1u << 20}, ASTCG_POLICY_COMPAT},
-                     {&t.ex_f4_max_tc, {0u, 64u}, ASTCG_POLICY_EXHAUSTIVE}};
-    for (int kn = 0; kn < 2 && st == ASTCG_OK; ++kn) {
-        if (kn == 1 && F->fp->texel_count > 64) {
+    Knob knobs[4] = {{&t.refine_stage_max_tc, 2, {0u, 1u << 20, 0u, 0u}, ASTCG_POLICY_COMPAT},
+                     {&t.refine_classes, 2, {0u, 1u, 0u, 0u}, ASTCG_POLICY_COMPAT},
+                     {&t.refine_queue, 3, {0u, 16u, 64u, 0u}, ASTCG_POLICY_COMPAT},
+                     {&t.ex_f4_max_tc, 2, {0u, 64u, 0u, 0u}, ASTCG_POLICY_EXHAUSTIVE}};
+    for (int kn = 0; kn < 4 && st == ASTCG_OK; ++kn) {
+        if (knobs[kn].field == &t.ex_f4_max_tc && F->fp->texel_count > 64) {
             t.ex_f4_max_tc = 0;  // float staging only up to 64 texels
             continue;
         }
         float best_ms = 0.0f;
         uint32_t best = *knobs[kn].field;
-        for (int c = 0; c < 2 && st == ASTCG_OK; ++c) {
+        for (int c = 0; c < knobs[kn].nchoices && st == ASTCG_OK; ++c) {
             *knobs[kn].field = knobs[kn].choices[c];
             apply_tuning(*F, t);
             float ms = 0.0f;
@@ -1673,8 +1699,8 @@ astcg_status astcg_autotune(astcg_encoder* e, uint32_t bx, uint32_t by, const ch
     if (cache_path) {
         FILE* f = std::fopen(cache_path, "a");
         if (!f) return fail(e, ASTCG_ERR_IO, "cannot append to the tuning cache");
-        std::fprintf(f, "astcg-tune 1 %s %ux%u %u %u %u\n", dev.c_str(), bx, by, t.refine_stage_max_tc,
-                     t.ex_f4_max_tc, t.exw_max_list);
+        std::fprintf(f, "astcg-tune 2 %s %ux%u %u %u %u %u %u\n", dev.c_str(), bx, by, t.refine_stage_max_tc,
+                     t.ex_f4_max_tc, t.exw_max_list, t.refine_queue, t.refine_classes);
         const bool ok = std::fclose(f) == 0;
         if (!ok) return fail(e, ASTCG_ERR_IO, "cannot write the tuning cache");
     }

@@ -53,12 +53,16 @@ struct DevTables {
     int refine_stage_max_tc;      // k_refine stages the block's RGBA8 texels in LDS up to this texel count
     int ex_f4_max_tc;             // k_partition_ex stages float texels in LDS up to this count (<= 64)
     int exw_max_list;             // exhaustive block lists up to this length run k_partition_exw
+    int refine_queue;             // 0: k_refine, a workgroup per work-list tile; n >= 1: the persistent
+                                  // k_refine_q, whose waves refill once n of their lanes are free
+    int refine_classes;           // 1: the work list is bucketed by trial mode and format class; 0: by mode
 };
 
 // Launch-shape defaults (measured on the 4096x4096 benchmark images).
 constexpr int kRefineStageMaxTc = 25;  // larger blocks ran faster unstaged, with more workgroups per CU
 constexpr int kExF4MaxTc = 16;         // keeps 6 k_partition_ex workgroups per CU
 constexpr int kExwMaxList = 65536;     // lane-per-block throughput is higher once its waves fill the chip
+constexpr int kRefineQueue = 0;        // the persistent refinement measured slower on the 4096^2 images
 
 // One texture of a launch: its blocks occupy [first_block, first_block +
 // num_blocks) of the launch's block index space; `out` receives its 16-byte
@@ -170,6 +174,13 @@ struct Ccr {
     uint16_t formats;  // 4 x 4 bits
 };
 
+// The refinement's work list is bucketed by (trial block mode, class): the
+// class groups candidates whose endpoint packing (pass14) takes the same
+// format path, so the lanes of a wave branch alike.  Bucket key = mode *
+// kRefineClasses + class.
+constexpr int kRefineClassBits = 4;
+constexpr int kRefineClasses = 1 << kRefineClassBits;
+
 // Best refined candidate per (partitioned block, candidate slot).
 struct CandOut {
     float error;
@@ -207,13 +218,14 @@ struct ChunkBuffers {
     uint16_t* ct_fmt;   // [17][13][npb of the launch]
     Ccr* ccr;           // [nbm][npb of the launch]
     uint8_t* qw;        // per trial mode a region at qw_off[mode] * npb: [npb][weight count rounded up to 4]
-    int16_t* top;       // [4 * max_nb][8]
+    int16_t* top;       // [4 * max_nb][8] bucket key (trial mode * kRefineClasses + class) of each slot, -1 = none
     CandOut* cand;      // [4 * max_nb][8]
     float* cand_err;    // [4 * max_nb][8] each slot's CandOut::error again, packed for k_pick
-    int* bin_count;     // [nbm] candidates per trial mode (zero between launches)
-    int* bin_cur;       // [nbm] next free work-list position per mode
+    int* bin_count;     // [nbm * kRefineClasses] candidates per bucket (zero between launches)
+    int* bin_cur;       // [nbm * kRefineClasses] next free work-list position per bucket
     int* work_total;    // [1] work-list length including padding
-    int* work;          // [4 * max_nb * 8 + nbm * 64] (partitioned block * 8 + slot), bucketed by mode
+    int* tile_next;     // [1] next work-list tile of the persistent refinement's queue (zeroed by k_bins)
+    int* work;          // [4 * max_nb * 8 + nbm * kRefineClasses * 64] (partitioned block * 8 + slot), bucketed
     int wstride;        // LDS row of quantised weights: max weight count of the trial grids, multiple of 4
 };
 

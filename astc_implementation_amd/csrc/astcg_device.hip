@@ -6,12 +6,17 @@
 //   k_dm_bm       passes 02-05 then 06, 07, 11 for every block mode of the
 //                 decimation mode (one lane per partitioned block x decimation mode)
 //   k_top         pass 12 (one lane per partitioned block)
-//   k_refine      passes 13-17 x 6 (one lane per partitioned block x candidate)
+//   k_refine      passes 13-17 x 6 (one lane per partitioned block x candidate);
+//                 k_refine_q is its persistent work-queue form
 //   k_pick        pass 18 + the host-side argmin of compression_pipeline.cpp:499-519
 //   k_pack        symbolic_to_physical (physical_compression.cpp:843-1003)
 // Every reduction the reference performs with float atomics is evaluated here
 // sequentially in texel/weight order, which fixes the summation order.
 // The file must be compiled with -ffp-contract=off.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "astcg_device.h"
 #include "../../include/astcg_predictor.h"
 
@@ -1825,12 +1830,37 @@ __device__ __forceinline__ int pb_est_rank(const ImageArgs& A, int pc, int pb, c
 // Top-8 trial modes per partitioned block, plus the per-mode candidate counts
 // that size the refinement work list (LDS histogram, one global add per
 // workgroup and mode).  Slots without a candidate get the default error.
+// Work-list class of a candidate: pass14 packs each partition's endpoints
+// along the path of its requested format (luminance, luminance + alpha, RGB
+// scale, RGB scale + alpha, RGB, RGBA); the RGB paths try the delta and
+// blue-contraction encodings only below colour quant levels 19 / 20, and a
+// block whose partitions request different formats walks several paths.
+__device__ __forceinline__ int refine_class(const Ccr& c, int pc) {
+    const int f0 = c.formats & 0xF;
+    bool uniform = true;
+    for (int p = 1; p < pc; ++p) uniform = uniform && fmt_of(c.formats, p) == f0;
+    int fam;
+    switch (f0) {
+        case FMT_LUMINANCE_ALPHA: case FMT_LUMINANCE_ALPHA_DELTA: fam = 1; break;
+        case FMT_RGB_SCALE: fam = 2; break;
+        case FMT_RGB_SCALE_ALPHA: fam = 3; break;
+        case FMT_RGB: case FMT_RGB_DELTA: fam = 4; break;
+        case FMT_RGBA: case FMT_RGBA_DELTA: fam = 5; break;
+        default: fam = 0; break;
+    }
+    if (!uniform) return 10 + fam;  // 10..15
+    if (fam < 4) return fam;        // 0..3
+    const int qc = c.ql <= 18 ? 0 : (c.ql == 19 ? 1 : 2);
+    return 4 + 3 * (fam - 4) + qc;  // 4..9
+}
+
 __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr, int16_t* top,
                                             int* bin_count, CandOut* cand, float* cand_err, const uint32_t* mode_mask,
                                             int mask_words, const uint8_t* slots, const float* pb_est, int keep_refine,
-                                            const uint8_t* fallback) {
-    extern __shared__ int s_hist[];  // [nbm]
-    for (int i = threadIdx.x; i < nbm; i += 64) s_hist[i] = 0;
+                                            const uint8_t* fallback, int classes) {
+    extern __shared__ int s_hist[];  // [nbm * kRefineClasses]
+    const int nbins = nbm * kRefineClasses;
+    for (int i = threadIdx.x; i < nbins; i += 64) s_hist[i] = 0;
     __syncthreads();
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     if (pb < pb_limit(A, pc, npb)) {
@@ -1864,7 +1894,8 @@ __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nb
             }
         }
         for (int i = 0; i < 8; ++i) {
-            const int v = (te[i] < kErrDefault && i < nslots) ? ti[i] : -1;
+            int v = (te[i] < kErrDefault && i < nslots) ? ti[i] : -1;
+            if (v >= 0) v = (v << kRefineClassBits) | (classes ? refine_class(ccr[(size_t)v * npb + pb], pc) : 0);
             top[pb * 8 + i] = (int16_t)v;
             if (v >= 0) atomicAdd(&s_hist[v], 1);
             else {  // pass12 produced no candidate for this slot
@@ -1874,40 +1905,51 @@ __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nb
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nbm; i += 64)
+    for (int i = threadIdx.x; i < nbins; i += 64)
         if (s_hist[i]) atomicAdd(&bin_count[i], s_hist[i]);
 }
 
-// Work-list layout: bucket k starts at a multiple of 64; padding entries are
-// -1.  Resets the counts for the next launch.
-__global__ void __launch_bounds__(64) k_bins(int nbm, int* bin_count, int* bin_cur, int* work, int* work_total) {
-    extern __shared__ int s_off[];  // [nbm + 1]
+// Work-list layout: each bucket (trial mode, class) starts at a multiple of
+// 64; padding entries are -1.  Each lane lays out a contiguous range of
+// buckets.  Resets the counts for the next launch.
+__global__ void __launch_bounds__(64) k_bins(int nbins, int* bin_count, int* bin_cur, int* work, int* work_total,
+                                             int* tile_next) {
+    __shared__ int s_off[64];
+    const int per = (nbins + 63) / 64;
+    const int lo = min(nbins, (int)threadIdx.x * per), hi = min(nbins, lo + per);
+    int sum = 0;
+    for (int b = lo; b < hi; ++b) sum += (bin_count[b] + 63) & ~63;
+    s_off[threadIdx.x] = sum;
+    __syncthreads();
     if (threadIdx.x == 0) {
         int run = 0;
-        for (int k = 0; k < nbm; ++k) {
-            s_off[k] = run;
-            run += (bin_count[k] + 63) & ~63;
+        for (int i = 0; i < 64; ++i) {
+            const int v = s_off[i];
+            s_off[i] = run;
+            run += v;
         }
-        s_off[nbm] = run;
         *work_total = run;
+        *tile_next = 0;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < nbm; k += 64) {
-        const int c = bin_count[k], o = s_off[k];
-        bin_cur[k] = o;
-        for (int i = o + c; i < s_off[k + 1]; ++i) work[i] = -1;
-        bin_count[k] = 0;
+    int run = s_off[threadIdx.x];
+    for (int b = lo; b < hi; ++b) {
+        const int c = bin_count[b], padded = (c + 63) & ~63;
+        bin_cur[b] = run;
+        for (int i = run + c; i < run + padded; ++i) work[i] = -1;
+        bin_count[b] = 0;
+        run += padded;
     }
 }
 
 // Fills the buckets: positions are reserved per workgroup and mode, so the
 // order inside a bucket is arbitrary; every task writes its own output slot.
-__global__ void __launch_bounds__(64) k_scatter(ImageArgs A, int pc, int npb, int nbm, const int16_t* top, int* bin_cur,
-                                                int* work) {
-    extern __shared__ int s_sc[];  // [2][nbm]
+__global__ void __launch_bounds__(64) k_scatter(ImageArgs A, int pc, int npb, int nbins, const int16_t* top,
+                                                int* bin_cur, int* work) {
+    extern __shared__ int s_sc[];  // [2][nbins]
     int* s_cnt = s_sc;
-    int* s_base = s_sc + nbm;
-    for (int i = threadIdx.x; i < nbm; i += 64) s_cnt[i] = 0;
+    int* s_base = s_sc + nbins;
+    for (int i = threadIdx.x; i < nbins; i += 64) s_cnt[i] = 0;
     __syncthreads();
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     int kk[8], loc[8];
@@ -1916,7 +1958,7 @@ __global__ void __launch_bounds__(64) k_scatter(ImageArgs A, int pc, int npb, in
         loc[s] = kk[s] >= 0 ? atomicAdd(&s_cnt[kk[s]], 1) : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nbm; i += 64) {
+    for (int i = threadIdx.x; i < nbins; i += 64) {
         const int c = s_cnt[i];
         s_base[i] = c ? atomicAdd(&bin_cur[i], c) : 0;
     }
@@ -2351,9 +2393,10 @@ __device__ void unpack_color_endpoints(int format, const uint32_t* v, int* o0, i
 
 // ---- passes 13-17, refinement loop of compression_pipeline.cpp:473-484 ---------
 // One lane per (partitioned block, candidate slot).  The candidates are
-// first bucketed by trial block mode (k_top counts, k_bins lays out buckets
-// padded to whole workgroups, k_scatter fills them), so a workgroup's lanes
-// share the mode: loop bounds and decimation-table reads are uniform.  Each
+// first bucketed by trial block mode and format class (k_top counts, k_bins
+// lays out buckets padded to whole workgroups, k_scatter fills them), so a
+// workgroup's lanes share the mode (loop bounds and decimation-table reads
+// are uniform) and mostly take the same endpoint packing path.  Each
 // lane keeps its block's texels (small footprints), partition map, quantised
 // weights and unpacked endpoints in its own LDS rows; the per-partition sums
 // of pass13 are accumulated partition by partition (each partition still
@@ -2390,6 +2433,86 @@ __device__ __forceinline__ uint32_t pk_byte(uint2 v, int i) { return ((i < 4 ? v
 
 __device__ __forceinline__ int4 load_ue(const uchar4 u) { return make_int4(u.x * 257, u.y * 257, u.z * 257, u.w * 257); }
 
+// ---- passes 13-17: refinement of one candidate ------------------------------
+// A candidate is a partitioned block x slot refined with one trial block mode
+// k.  Every lane of a wave refines a candidate of the same mode, so the
+// mode's tables are wave-uniform; the candidate's state lives in registers
+// and in the lane's LDS rows ([index][lane]).
+
+struct RefineMode {
+    int k;
+    int q;              // weight quant level
+    int wc;             // weight count
+    bool undecimated;
+    const uint8_t* t2wc;
+    const uint8_t* t2wi;
+    const float* t2ww;
+    const uint8_t* w2tc;
+    const uint16_t* w2to;
+    const uint8_t* w2ti;
+};
+
+__device__ __forceinline__ RefineMode refine_mode(const DevTables& T, int k) {
+    const BmInfo bmi = T.bm[k];
+    const DmInfo di = T.dm[bmi.dm];
+    RefineMode m;
+    m.k = k;
+    m.q = bmi.quant;
+    m.wc = di.weight_count;
+    m.undecimated = T.tc == m.wc;
+    m.t2wc = T.t2w_count + (size_t)bmi.dm * T.tc;
+    m.t2wi = T.t2w_idx + (size_t)bmi.dm * T.tc * 4;
+    m.t2ww = T.t2w_w + (size_t)bmi.dm * T.tc * 4;
+    m.w2tc = T.w2t_count + bmi.dm * 64;
+    m.w2to = T.w2t_off + bmi.dm * 64;
+    m.w2ti = T.w2t_idx + di.w2t_base;
+    return m;
+}
+
+// The lane's LDS rows, each pointer already offset by the lane (element i of
+// a row is at [i * 64]).
+struct RefineRows {
+    uint32_t* tex;  // [tc] RGBA8 texels (staged kernels)
+    uchar4* ue0;    // [pc] unpacked endpoints
+    uchar4* ue1;
+    uint8_t* tp;    // [tc] texel partitions (pc > 1)
+    uint8_t* qw;    // [wstride] quantised weights of the current round
+    uint8_t* bqw;   // [wstride] quantised weights of the best round
+};
+
+template <bool kStage, int PC>
+__device__ __forceinline__ RefineRows refine_rows(uint8_t* smem, const DevTables& T, int wstride, int lane) {
+    const int tc = T.tc;
+    uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);
+    uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(uint32_t) : 0));
+    uchar4* s_ue1 = s_ue0 + PC * 64;
+    uint8_t* s_tp = reinterpret_cast<uint8_t*>(s_ue1 + PC * 64);
+    uint8_t* s_qw = s_tp + (PC > 1 ? (size_t)tc * 64 : 0);
+    uint8_t* s_bqw = s_qw + (size_t)wstride * 64;
+    RefineRows r;
+    r.tex = s_tex + lane;
+    r.ue0 = s_ue0 + lane;
+    r.ue1 = s_ue1 + lane;
+    r.tp = s_tp + lane;
+    r.qw = s_qw + lane;
+    r.bqw = s_bqw + lane;
+    return r;
+}
+
+struct RefineCand {
+    int task;           // partitioned block * 8 + slot, -1 = none
+    int ql, qlm;        // colour quant level and its mod variant (pass11)
+    uint16_t in_formats;
+    int ppc[4];         // texels per partition
+    BlockLoc loc;       // texel origin (unstaged kernels read the image)
+    float4 ep0[4], ep1[4];
+    float sdirs[4][3];  // pass13's scaled direction: depends on the average only
+    float best_err;
+    int best_matched, best_quant;
+    uint16_t best_formats;
+    uint2 best_pk[4];
+};
+
 // pass13's sums over one partition's texels (weights, their complements,
 // colour and scalar projections), taken in texel order.
 struct Pass13Sums {
@@ -2415,94 +2538,84 @@ struct Pass13Sums {
     }
 };
 
-// Specialised per partition count, so per-partition state only occupies
-// registers for the partitions that exist.
+// Loads candidate `task` of mode M into the lane's rows and registers.
 template <bool kStage, int PC>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PC == 1 ? 6 : 4, 8)))
-k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
-                                               const Ideal* ideal, const int16_t* top, const Ccr* ccr,
-                                               const uint8_t* qw_all, const int* work, const int* work_total,
-                                               CandOut* cand_out, float* cand_err, int wstride, int iterations) {
-    extern __shared__ __align__(16) uint8_t smem[];
+__device__ __forceinline__ void refine_init(const ImageArgs& A, const DevTables& T, const int16_t* pb_part, int npb,
+                                            const Ideal* ideal, const Ccr* ccr, const uint8_t* qw_all,
+                                            const RefineMode& M, const RefineRows& R, int task, RefineCand& C) {
     constexpr int pc = PC;
-    const int tc = T.tc, lane = threadIdx.x;
-    const int base = blockIdx.x * 64;
-    if (base >= *work_total) return;
-    // every lane of the workgroup refines a candidate of the same trial block
-    // mode (the work list is bucketed by mode, buckets padded to 64 lanes)
-    const int task = work[base + lane];
-    const int k = __builtin_amdgcn_readfirstlane((int)top[__builtin_amdgcn_readfirstlane(work[base])]);
-    if (task < 0) return;
-    // per-lane LDS rows, laid out [index][lane]
-    uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);
-    uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(uint32_t) : 0));
-    uchar4* s_ue1 = s_ue0 + pc * 64;
-    uint8_t* s_tp = reinterpret_cast<uint8_t*>(s_ue1 + pc * 64);
-    uint8_t* s_qw = s_tp + (pc > 1 ? (size_t)tc * 64 : 0);
-    uint8_t* s_bqw = s_qw + (size_t)wstride * 64;
+    const int tc = T.tc;
     const int pb = task >> 3;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
     for (int i = 0; i < tc; ++i) {
-        if (kStage) s_tex[i * 64 + lane] = texel_rgba8(b.loc, T, i);
-        if (pc > 1) s_tp[i * 64 + lane] = (uint8_t)texel_part(b, i);
+        if (kStage) R.tex[i * 64] = texel_rgba8(b.loc, T, i);
+        if (pc > 1) R.tp[i * 64] = (uint8_t)texel_part(b, i);
     }
+    C.task = task;
+    C.loc = b.loc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) C.ppc[i] = b.ppc[i];
+    const Ideal& I = ideal[pb];
+    const Ccr cr = ccr[(size_t)M.k * npb + pb];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        C.ep0[p] = make_float4(I.ep0[p][0], I.ep0[p][1], I.ep0[p][2], I.ep0[p][3]);
+        C.ep1[p] = make_float4(I.ep1[p][0], I.ep1[p][1], I.ep1[p][2], I.ep1[p][3]);
+        const float size = (float)C.ppc[p];
+        float q3[3];
+        for (int j = 0; j < 3; ++j) q3[j] = ((I.avg[p][j] * size) * kCW[j]) / wmax(kCW[j] * size, 1e-17f);
+        normalize3(q3, C.sdirs[p]);
+    }
+    C.ql = cr.ql;
+    C.qlm = cr.ql_mod;
+    C.in_formats = cr.formats;
+    {
+        // this PB's row of the mode's region, read as dwords
+        const uint32_t* qsrc = reinterpret_cast<const uint32_t*>(qw_all + (size_t)T.qw_off[M.k] * npb +
+                                                                 (size_t)pb * ((M.wc + 3) & ~3));
+        for (int i = 0; i < M.wc; i += 4) {
+            const uint32_t v = qsrc[i >> 2];
+            for (int j = 0; j < 4 && i + j < M.wc; ++j) R.qw[(i + j) * 64] = (uint8_t)(v >> (8 * j));
+        }
+    }
+    C.best_err = kErrDefault;
+    C.best_matched = 0;
+    C.best_quant = 0;
+    C.best_formats = 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) C.best_pk[p] = make_uint2(0, 0);
+}
+
+// One refinement round (pass13, pass14, pass15, then pass17 on the first
+// round and pass16 + pass17); returns whether pass16 moved a weight.  Once
+// pass16 leaves every weight where it was, the next round would recompute
+// the same endpoints from the same weights (pass13 keeps a previous endpoint
+// only where it kept it this time), so it would repeat this round's error and
+// could not replace the best candidate: the caller stops there.
+template <bool kStage, int PC>
+__device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMode& M, const RefineRows& R, bool first,
+                                             RefineCand& C) {
+    constexpr int pc = PC;
+    const int tc = T.tc;
     auto px_of = [&](int i) -> float4 {
         if (kStage) {
-            const uint32_t v = s_tex[i * 64 + lane];
+            const uint32_t v = R.tex[i * 64];
             return make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
                                u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
         }
         float px[4];
-        texel_px(b.loc, T, i, px);
+        texel_px(C.loc, T, i, px);
         return make_float4(px[0], px[1], px[2], px[3]);
     };
-    const uint8_t* btp = s_tp + lane;  // btp[i * 64], pc > 1 only
-    auto part_of = [&](int i) -> int { return pc > 1 ? (int)btp[i * 64] : 0; };
-    uint8_t* qw = s_qw + lane;
-    uint8_t* bqw = s_bqw + lane;
-    int ppc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) ppc[i] = b.ppc[i];
-    const BmInfo bmi = T.bm[k];
-    const DmInfo di = T.dm[bmi.dm];
-    const int wc = di.weight_count;
-    const uint8_t* t2wc = T.t2w_count + (size_t)bmi.dm * tc;
-    const uint8_t* t2wi = T.t2w_idx + (size_t)bmi.dm * tc * 4;
-    const float* t2ww = T.t2w_w + (size_t)bmi.dm * tc * 4;
-    const uint8_t* w2tc = T.w2t_count + bmi.dm * 64;
-    const uint16_t* w2to = T.w2t_off + bmi.dm * 64;
-    const uint8_t* w2ti = T.w2t_idx + di.w2t_base;
-    const bool undecimated = tc == wc;
-    const Ideal& I = ideal[pb];
-    const Ccr cr = ccr[(size_t)k * npb + pb];
-    float4 ep0[4], ep1[4];
-    float sdirs[4][3];  // pass13's scaled direction: depends on the average only
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        ep0[p] = make_float4(I.ep0[p][0], I.ep0[p][1], I.ep0[p][2], I.ep0[p][3]);
-        ep1[p] = make_float4(I.ep1[p][0], I.ep1[p][1], I.ep1[p][2], I.ep1[p][3]);
-        const float size = (float)ppc[p];
-        float q3[3];
-        for (int j = 0; j < 3; ++j) q3[j] = ((I.avg[p][j] * size) * kCW[j]) / wmax(kCW[j] * size, 1e-17f);
-        normalize3(q3, sdirs[p]);
-    }
-    const int ql = cr.ql, qlm = cr.ql_mod;
-    const uint16_t in_formats = cr.formats;
-    {
-        // this PB's row of the mode's region, read as dwords
-        const uint32_t* qsrc = reinterpret_cast<const uint32_t*>(qw_all + (size_t)T.qw_off[k] * npb +
-                                                                 (size_t)pb * ((wc + 3) & ~3));
-        for (int i = 0; i < wc; i += 4) {
-            const uint32_t v = qsrc[i >> 2];
-            for (int j = 0; j < 4 && i + j < wc; ++j) qw[(i + j) * 64] = (uint8_t)(v >> (8 * j));
-        }
-    }
+    auto part_of = [&](int i) -> int { return pc > 1 ? (int)R.tp[i * 64] : 0; };
+    uint8_t* qw = R.qw;
+    const int wc = M.wc;
+    const bool undecimated = M.undecimated;
+    const uint8_t* t2wc = M.t2wc;
+    const uint8_t* t2wi = M.t2wi;
+    const float* t2ww = M.t2ww;
     const float lsw = (kCW[0] + kCW[1]) + kCW[2];
     const float cw[4] = {kCW[0], kCW[1], kCW[2], kCW[3]};
-    float best_err = kErrDefault;
-    int best_matched = 0, best_quant = 0;
-    uint16_t best_formats = 0;
-    uint2 best_pk[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
     auto weight_of = [&](int i) -> float {
         if (undecimated) return (float)qw[i * 64] / 64.0f;
         float w = 0.0f;
@@ -2510,288 +2623,407 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb,
         for (int j = 0; j < n; ++j) w += ((float)qw[t2wi[i * 4 + j] * 64] / 64.0f) * t2ww[i * 4 + j];
         return w;
     };
-    for (int a = 0; a < iterations; ++a) {  // 6 in the reference (pass13-17 rounds)
-        // pass13_recompute_ideal_endpoints.wgsl:182-393
-        // Per-partition sums over the partition's texels in texel order.  With
-        // two partitions one walk over the texels fills both sets (each set
-        // still receives its texels in index order), instead of one divergent
-        // walk per partition.
-        Pass13Sums sums2[2];
+    // pass13_recompute_ideal_endpoints.wgsl:182-393
+    // Per-partition sums over the partition's texels in texel order.  With
+    // two partitions one walk over the texels fills both sets (each set
+    // still receives its texels in index order), instead of one divergent
+    // walk per partition.
+    Pass13Sums sums2[2];
+    if (pc == 2) {
+        for (int i = 0; i < tc; ++i) {
+            const int p = R.tp[i * 64];
+            const float w = weight_of(i);
+            const float4 px4 = px_of(i);
+            const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
+            const float* sd = p ? C.sdirs[1] : C.sdirs[0];
+            const float sc = dot3(rgba, sd);
+            if (p) sums2[1].add(w, rgba, sc, lsw);
+            else sums2[0].add(w, rgba, sc, lsw);
+        }
+    }
+    float4 rgbs[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        rgbs[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (p >= pc) continue;
+        const float size = (float)C.ppc[p];
+        const float* sdir = C.sdirs[p];
+        Pass13Sums acc;
         if (pc == 2) {
+            acc = sums2[p & 1];
+        } else {
             for (int i = 0; i < tc; ++i) {
-                const int p = btp[i * 64];
+                if (pc > 1 && R.tp[i * 64] != p) continue;
                 const float w = weight_of(i);
                 const float4 px4 = px_of(i);
                 const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
-                const float* sd = p ? sdirs[1] : sdirs[0];
-                const float sc = dot3(rgba, sd);
-                if (p) sums2[1].add(w, rgba, sc, lsw);
-                else sums2[0].add(w, rgba, sc, lsw);
+                acc.add(w, rgba, dot3(rgba, sdir), lsw);
             }
         }
-        float4 rgbs[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            rgbs[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (p >= pc) continue;
-            const float size = (float)ppc[p];
-            const float* sdir = sdirs[p];
-            Pass13Sums acc;
-            if (pc == 2) {
-                acc = sums2[p & 1];
-            } else {
-                for (int i = 0; i < tc; ++i) {
-                    if (pc > 1 && btp[i * 64] != p) continue;
-                    const float w = weight_of(i);
-                    const float4 px4 = px_of(i);
-                    const float rgba[4] = {px4.x, px4.y, px4.z, px4.w};
-                    acc.add(w, rgba, dot3(rgba, sdir), lsw);
-                }
-            }
-            const float wmin1 = acc.wmin1, wmax1 = acc.wmax1, ls = acc.ls, ms = acc.ms, rs = acc.rs;
-            const float smin = acc.smin, smax = acc.smax, sv0 = acc.sv0, sv1 = acc.sv1;
-            const float* cx = acc.cx;
-            const float* cy = acc.cy;
-            float colx[4], coly[4], lv[4], mv[4], rv[4];
+        const float wmin1 = acc.wmin1, wmax1 = acc.wmax1, ls = acc.ls, ms = acc.ms, rs = acc.rs;
+        const float smin = acc.smin, smax = acc.smax, sv0 = acc.sv0, sv1 = acc.sv1;
+        const float* cx = acc.cx;
+        const float* cy = acc.cy;
+        float colx[4], coly[4], lv[4], mv[4], rv[4];
+        for (int j = 0; j < 4; ++j) {
+            colx[j] = cx[j] * cw[j]; coly[j] = cy[j] * cw[j];
+            lv[j] = ls * cw[j]; mv[j] = ms * cw[j]; rv[j] = rs * cw[j];
+        }
+        const float lx = ls * lsw, ly = ms * lsw, lz = rs * lsw;
+        const float scalediv = wclamp(smin / wmax(smax, 1e-10f), 0.0f, 1.0f);
+        rgbs[p] = make_float4(sdir[0] * smax, sdir[1] * smax, sdir[2] * smax, scalediv);
+        float e0v[4] = {C.ep0[p].x, C.ep0[p].y, C.ep0[p].z, C.ep0[p].w};
+        float e1v[4] = {C.ep1[p].x, C.ep1[p].y, C.ep1[p].z, C.ep1[p].w};
+        if (wmin1 >= wmax1 * 0.999f) {
             for (int j = 0; j < 4; ++j) {
-                colx[j] = cx[j] * cw[j]; coly[j] = cy[j] * cw[j];
-                lv[j] = ls * cw[j]; mv[j] = ms * cw[j]; rv[j] = rs * cw[j];
+                const float ac = (colx[j] + coly[j]) / wmax(cw[j] * size, 1e-17f);
+                if (ac == ac) { e0v[j] = ac; e1v[j] = ac; }
             }
-            const float lx = ls * lsw, ly = ms * lsw, lz = rs * lsw;
-            const float scalediv = wclamp(smin / wmax(smax, 1e-10f), 0.0f, 1.0f);
-            rgbs[p] = make_float4(sdir[0] * smax, sdir[1] * smax, sdir[2] * smax, scalediv);
-            float e0v[4] = {ep0[p].x, ep0[p].y, ep0[p].z, ep0[p].w};
-            float e1v[4] = {ep1[p].x, ep1[p].y, ep1[p].z, ep1[p].w};
-            if (wmin1 >= wmax1 * 0.999f) {
-                for (int j = 0; j < 4; ++j) {
-                    const float ac = (colx[j] + coly[j]) / wmax(cw[j] * size, 1e-17f);
-                    if (ac == ac) { e0v[j] = ac; e1v[j] = ac; }
-                }
-                rgbs[p].w = 1.0f;
-            } else {
-                for (int j = 0; j < 4; ++j) {
-                    const float det = (lv[j] * rv[j]) - (mv[j] * mv[j]);
-                    const float rdet = 1.0f / det;
-                    const float mss = ((lv[j] * lv[j]) + 2.0f * (mv[j] * mv[j])) + (rv[j] * rv[j]);
-                    const float e0 = (rv[j] * colx[j] - mv[j] * coly[j]) * rdet;
-                    const float e1 = (lv[j] * coly[j] - mv[j] * colx[j]) * rdet;
-                    if (fabsf(det) > (mss * 1e-4f) && e0 == e0 && e1 == e1) { e0v[j] = e0; e1v[j] = e1; }
-                }
-                const float lsdet = (lx * lz) - (ly * ly);
-                const float lsrdet = 1.0f / lsdet;
-                const float lsmss = ((lx * lx) + 2.0f * (ly * ly)) + (lz * lz);
-                const float se0 = (lz * sv0 - ly * sv1) * lsrdet;
-                const float se1 = (lx * sv1 - ly * sv0) * lsrdet;
-                if ((fabsf(lsdet) > (lsmss * 1e-4f)) && se0 == se0 && se1 == se1 && se0 < se1)
-                    rgbs[p] = make_float4(sdir[0] * se1, sdir[1] * se1, sdir[2] * se1, se0 / se1);
-            }
-            ep0[p] = make_float4(e0v[0], e0v[1], e0v[2], e0v[3]);
-            ep1[p] = make_float4(e1v[0], e1v[1], e1v[2], e1v[3]);
-        }
-        // pass14_pack_color_endpoints.wgsl:1226-1304
-        int fn[4] = {0, 0, 0, 0};
-        uint2 pk[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
-        bool all_same = ql != qlm;
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            if (p >= pc) continue;
-            const PackOut o = pack_partition<(PC > 1)>(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), ql);
-            fn[p] = o.format;
-            pk[p] = o.v;
-            if (p > 0) all_same = all_same && (o.format == fn[0]);
-        }
-        bool matched = false;
-        if (pc >= 2 && all_same) {
-            int fm[4] = {0, 0, 0, 0};
-            uint2 pm[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
-            bool same_mod = true;
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                if (p >= pc || !same_mod) continue;
-                const PackOut o = pack_partition<(PC > 1)>(ep0[p], ep1[p], rgbs[p], fmt_of(in_formats, p), qlm);
-                fm[p] = o.format;
-                pm[p] = o.v;
-                if (p > 0 && o.format != fm[0]) same_mod = false;
-            }
-            if (same_mod) {
-                matched = true;
-#pragma unroll
-                for (int p = 0; p < 4; ++p) { fn[p] = fm[p]; pk[p] = pm[p]; }
-            }
-        }
-        const int final_quant = matched ? qlm : ql;
-        int he0[4] = {0, 0, 0, 0}, he1[4] = {0, 0, 0, 0};  // 1 partition: decoded endpoints (x 257)
-        // pass15_unpack_color_endpoints.wgsl:307-391
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            if (p >= pc) continue;
-            uint32_t v[8];
-            for (int i = 0; i < 8; ++i) v[i] = pk_byte(pk[p], i);
-            int o0[4], o1[4];
-            unpack_color_endpoints(fn[p], v, o0, o1);
-            if (PC == 1) {
-                for (int c = 0; c < 4; ++c) {
-                    he0[c] = (int)(uint8_t)o0[c] * 257;
-                    he1[c] = (int)(uint8_t)o1[c] * 257;
-                }
-            } else {
-                s_ue0[p * 64 + lane] = make_uchar4(o0[0], o0[1], o0[2], o0[3]);
-                s_ue1[p * 64 + lane] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
-            }
-        }
-        // the endpoints of texel t's partition as the texel loops use them:
-        // decoded values (x 257) and the interpolation base and step per
-        // channel; with one partition they stay in registers
-        auto ep_at = [&](int t, int* e0, int* e1, float* bs, float* of) {
-            int4 u0, u1;
-            if (PC == 1) {
-                u0 = make_int4(he0[0], he0[1], he0[2], he0[3]);
-                u1 = make_int4(he1[0], he1[1], he1[2], he1[3]);
-            } else {
-                const int p = part_of(t);
-                u0 = load_ue(s_ue0[p * 64 + lane]);
-                u1 = load_ue(s_ue1[p * 64 + lane]);
-            }
-            e0[0] = u0.x; e0[1] = u0.y; e0[2] = u0.z; e0[3] = u0.w;
-            e1[0] = u1.x; e1[1] = u1.y; e1[2] = u1.z; e1[3] = u1.w;
+            rgbs[p].w = 1.0f;
+        } else {
             for (int j = 0; j < 4; ++j) {
-                bs[j] = (float)e0[j];
-                of[j] = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+                const float det = (lv[j] * rv[j]) - (mv[j] * mv[j]);
+                const float rdet = 1.0f / det;
+                const float mss = ((lv[j] * lv[j]) + 2.0f * (mv[j] * mv[j])) + (rv[j] * rv[j]);
+                const float e0 = (rv[j] * colx[j] - mv[j] * coly[j]) * rdet;
+                const float e1 = (lv[j] * coly[j] - mv[j] * colx[j]) * rdet;
+                if (fabsf(det) > (mss * 1e-4f) && e0 == e0 && e1 == e1) { e0v[j] = e0; e1v[j] = e1; }
             }
-        };
-        // pass17 (before pass16 on the first iteration), then pass16 + pass17
-        // Once pass16 leaves every weight where it was, the next iteration
-        // recomputes the same endpoints from the same weights (pass13 keeps a
-        // previous endpoint only where it kept it this time), so it repeats
-        // this iteration's error and cannot replace the best candidate: the
-        // remaining iterations are skipped.
-        bool moved = false;
-        for (int phase = (a == 0 ? 0 : 1); phase < 2; ++phase) {
-            float total = 0.0f;  // pass17's error sum
-            bool fused = false;  // pass17 already summed inside pass16's texel loop
-            if (phase == 1) {
-                // pass16_realign_weights.wgsl:211-367
-                const int q = bmi.quant;
-                if (undecimated) {
-                    // every texel owns its weight, so texel t's final error
-                    // (pass17) depends only on its own realigned weight: both
-                    // passes share one walk over the texels, in texel order
-                    fused = true;
-                    for (int t = 0; t < tc; ++t) {
-                        const int uqw = qw[t * 64];
-                        const uint32_t pn = c_qt.prev_next[q][uqw];
-                        const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
-                        const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
+            const float lsdet = (lx * lz) - (ly * ly);
+            const float lsrdet = 1.0f / lsdet;
+            const float lsmss = ((lx * lx) + 2.0f * (ly * ly)) + (lz * lz);
+            const float se0 = (lz * sv0 - ly * sv1) * lsrdet;
+            const float se1 = (lx * sv1 - ly * sv0) * lsrdet;
+            if ((fabsf(lsdet) > (lsmss * 1e-4f)) && se0 == se0 && se1 == se1 && se0 < se1)
+                rgbs[p] = make_float4(sdir[0] * se1, sdir[1] * se1, sdir[2] * se1, se0 / se1);
+        }
+        C.ep0[p] = make_float4(e0v[0], e0v[1], e0v[2], e0v[3]);
+        C.ep1[p] = make_float4(e1v[0], e1v[1], e1v[2], e1v[3]);
+    }
+    // pass14_pack_color_endpoints.wgsl:1226-1304
+    int fn[4] = {0, 0, 0, 0};
+    uint2 pk[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
+    bool all_same = C.ql != C.qlm;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= pc) continue;
+        const PackOut o = pack_partition<(PC > 1)>(C.ep0[p], C.ep1[p], rgbs[p], fmt_of(C.in_formats, p), C.ql);
+        fn[p] = o.format;
+        pk[p] = o.v;
+        if (p > 0) all_same = all_same && (o.format == fn[0]);
+    }
+    bool matched = false;
+    if (pc >= 2 && all_same) {
+        int fm[4] = {0, 0, 0, 0};
+        uint2 pm[4] = {make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0), make_uint2(0, 0)};
+        bool same_mod = true;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (p >= pc || !same_mod) continue;
+            const PackOut o = pack_partition<(PC > 1)>(C.ep0[p], C.ep1[p], rgbs[p], fmt_of(C.in_formats, p), C.qlm);
+            fm[p] = o.format;
+            pm[p] = o.v;
+            if (p > 0 && o.format != fm[0]) same_mod = false;
+        }
+        if (same_mod) {
+            matched = true;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) { fn[p] = fm[p]; pk[p] = pm[p]; }
+        }
+    }
+    const int final_quant = matched ? C.qlm : C.ql;
+    int he0[4] = {0, 0, 0, 0}, he1[4] = {0, 0, 0, 0};  // 1 partition: decoded endpoints (x 257)
+    // pass15_unpack_color_endpoints.wgsl:307-391
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= pc) continue;
+        uint32_t v[8];
+        for (int i = 0; i < 8; ++i) v[i] = pk_byte(pk[p], i);
+        int o0[4], o1[4];
+        unpack_color_endpoints(fn[p], v, o0, o1);
+        if (PC == 1) {
+            for (int c = 0; c < 4; ++c) {
+                he0[c] = (int)(uint8_t)o0[c] * 257;
+                he1[c] = (int)(uint8_t)o1[c] * 257;
+            }
+        } else {
+            R.ue0[p * 64] = make_uchar4(o0[0], o0[1], o0[2], o0[3]);
+            R.ue1[p * 64] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
+        }
+    }
+    // the endpoints of texel t's partition as the texel loops use them:
+    // decoded values (x 257) and the interpolation base and step per
+    // channel; with one partition they stay in registers
+    auto ep_at = [&](int t, int* e0, int* e1, float* bs, float* of) {
+        int4 u0, u1;
+        if (PC == 1) {
+            u0 = make_int4(he0[0], he0[1], he0[2], he0[3]);
+            u1 = make_int4(he1[0], he1[1], he1[2], he1[3]);
+        } else {
+            const int p = part_of(t);
+            u0 = load_ue(R.ue0[p * 64]);
+            u1 = load_ue(R.ue1[p * 64]);
+        }
+        e0[0] = u0.x; e0[1] = u0.y; e0[2] = u0.z; e0[3] = u0.w;
+        e1[0] = u1.x; e1[1] = u1.y; e1[2] = u1.z; e1[3] = u1.w;
+        for (int j = 0; j < 4; ++j) {
+            bs[j] = (float)e0[j];
+            of[j] = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+        }
+    };
+    // pass17 (before pass16 on the first round), then pass16 + pass17
+    bool moved = false;
+    for (int phase = (first ? 0 : 1); phase < 2; ++phase) {
+        float total = 0.0f;  // pass17's error sum
+        bool fused = false;  // pass17 already summed inside pass16's texel loop
+        if (phase == 1) {
+            // pass16_realign_weights.wgsl:211-367
+            const int q = M.q;
+            if (undecimated) {
+                // every texel owns its weight, so texel t's final error
+                // (pass17) depends only on its own realigned weight: both
+                // passes share one walk over the texels, in texel order
+                fused = true;
+                for (int t = 0; t < tc; ++t) {
+                    const int uqw = qw[t * 64];
+                    const uint32_t pn = c_qt.prev_next[q][uqw];
+                    const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
+                    const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
+                    int e0[4], e1[4];
+                    float bs[4], of[4];
+                    ep_at(t, e0, e1, bs, of);
+                    const float4 px4 = px_of(t);
+                    const float px[4] = {px4.x, px4.y, px4.z, px4.w};
+                    float s0[4], s1[4], s2[4];
+                    for (int j = 0; j < 4; ++j) {
+                        const float base = bs[j], offs = of[j];
+                        const float diff = (base + offs * wb) - px[j];
+                        const float dfd = diff + offs * dd, dfu = diff + offs * udf;
+                        s0[j] = diff * diff; s1[j] = dfd * dfd; s2[j] = dfu * dfu;
+                    }
+                    const float eb = dot4(s0, cw), ed = dot4(s1, cw), eu = dot4(s2, cw);
+                    int nq = uqw;
+                    if ((eu < eb) && (eu < ed) && (uqw < 64)) nq = up;
+                    else if ((ed < eb) && (uqw > 0)) nq = down;
+                    moved |= nq != uqw;
+                    qw[t * 64] = (uint8_t)nq;
+                    // pass17_compute_final_error.wgsl:149-231 for this texel
+                    float dfe[4];
+                    for (int j = 0; j < 4; ++j) {
+                        const int col = (e0[j] * (64 - nq) + e1[j] * nq + 32) >> 6;
+                        const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
+                        dfe[j] = df * df;
+                    }
+                    total += wmin(dot4(dfe, cw), kErrDefault);
+                }
+            } else {
+                const uint8_t* w2tc = M.w2tc;
+                const uint16_t* w2to = M.w2to;
+                const uint8_t* w2ti = M.w2ti;
+                for (int we = 0; we < wc; ++we) {
+                    const int uqw = qw[we * 64];
+                    const uint32_t pn = c_qt.prev_next[q][uqw];
+                    const float down = (float)(pn & 0xFF), up = (float)((pn >> 8) & 0xFF), wbase = (float)uqw;
+                    const float ddn = down - wbase, dup = up - wbase;
+                    float ebv[4] = {0, 0, 0, 0}, edv[4] = {0, 0, 0, 0}, euv[4] = {0, 0, 0, 0};
+                    const int n = w2tc[we], woff = w2to[we];
+                    for (int te = 0; te < n; ++te) {
+                        const int t = w2ti[woff + te];
+                        float wbt = 0.0f, twb = 0.0f;
+                        bool found = false;
+                        for (int j = 0; j < t2wc[t]; ++j) {
+                            const int wi = t2wi[t * 4 + j];
+                            wbt += (float)qw[wi * 64] * t2ww[t * 4 + j];
+                            if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
+                        }
+                        const float wdd = ddn * twb, wud = dup * twb;
                         int e0[4], e1[4];
                         float bs[4], of[4];
                         ep_at(t, e0, e1, bs, of);
                         const float4 px4 = px_of(t);
                         const float px[4] = {px4.x, px4.y, px4.z, px4.w};
-                        float s0[4], s1[4], s2[4];
                         for (int j = 0; j < 4; ++j) {
                             const float base = bs[j], offs = of[j];
-                            const float diff = (base + offs * wb) - px[j];
-                            const float dfd = diff + offs * dd, dfu = diff + offs * udf;
-                            s0[j] = diff * diff; s1[j] = dfd * dfd; s2[j] = dfu * dfu;
+                            const float diff = (base + offs * wbt) - px[j];
+                            const float dd = diff + offs * wdd, du = diff + offs * wud;
+                            ebv[j] += diff * diff;
+                            edv[j] += dd * dd;
+                            euv[j] += du * du;
                         }
-                        const float eb = dot4(s0, cw), ed = dot4(s1, cw), eu = dot4(s2, cw);
-                        int nq = uqw;
-                        if ((eu < eb) && (eu < ed) && (uqw < 64)) nq = up;
-                        else if ((ed < eb) && (uqw > 0)) nq = down;
-                        moved |= nq != uqw;
-                        qw[t * 64] = (uint8_t)nq;
-                        // pass17_compute_final_error.wgsl:149-231 for this texel
-                        float dfe[4];
-                        for (int j = 0; j < 4; ++j) {
-                            const int col = (e0[j] * (64 - nq) + e1[j] * nq + 32) >> 6;
-                            const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
-                            dfe[j] = df * df;
-                        }
-                        total += wmin(dot4(dfe, cw), kErrDefault);
                     }
-                } else {
-                    for (int we = 0; we < wc; ++we) {
-                        const int uqw = qw[we * 64];
-                        const uint32_t pn = c_qt.prev_next[q][uqw];
-                        const float down = (float)(pn & 0xFF), up = (float)((pn >> 8) & 0xFF), wbase = (float)uqw;
-                        const float ddn = down - wbase, dup = up - wbase;
-                        float ebv[4] = {0, 0, 0, 0}, edv[4] = {0, 0, 0, 0}, euv[4] = {0, 0, 0, 0};
-                        const int n = w2tc[we], woff = w2to[we];
-                        for (int te = 0; te < n; ++te) {
-                            const int t = w2ti[woff + te];
-                            float wbt = 0.0f, twb = 0.0f;
-                            bool found = false;
-                            for (int j = 0; j < t2wc[t]; ++j) {
-                                const int wi = t2wi[t * 4 + j];
-                                wbt += (float)qw[wi * 64] * t2ww[t * 4 + j];
-                                if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
-                            }
-                            const float wdd = ddn * twb, wud = dup * twb;
-                            int e0[4], e1[4];
-                            float bs[4], of[4];
-                            ep_at(t, e0, e1, bs, of);
-                            const float4 px4 = px_of(t);
-                            const float px[4] = {px4.x, px4.y, px4.z, px4.w};
-                            for (int j = 0; j < 4; ++j) {
-                                const float base = bs[j], offs = of[j];
-                                const float diff = (base + offs * wbt) - px[j];
-                                const float dd = diff + offs * wdd, du = diff + offs * wud;
-                                ebv[j] += diff * diff;
-                                edv[j] += dd * dd;
-                                euv[j] += du * du;
-                            }
-                        }
-                        const float eb = dot4(ebv, cw), ed = dot4(edv, cw), eu = dot4(euv, cw);
-                        int nq = uqw;
-                        if ((eu < eb) && (eu < ed) && (uqw < 64)) nq = (int)up;
-                        else if ((ed < eb) && (uqw > 0)) nq = (int)down;
-                        moved |= nq != uqw;
-                        qw[we * 64] = (uint8_t)nq;
-                    }
+                    const float eb = dot4(ebv, cw), ed = dot4(edv, cw), eu = dot4(euv, cw);
+                    int nq = uqw;
+                    if ((eu < eb) && (eu < ed) && (uqw < 64)) nq = (int)up;
+                    else if ((ed < eb) && (uqw > 0)) nq = (int)down;
+                    moved |= nq != uqw;
+                    qw[we * 64] = (uint8_t)nq;
                 }
-            }
-            // pass17_compute_final_error.wgsl:149-231
-            for (int i = 0; i < tc && !fused; ++i) {
-                const float w = weight_of(i);
-                int e0[4], e1[4];
-                float bs[4], of[4];
-                ep_at(i, e0, e1, bs, of);
-                const float4 px4 = px_of(i);
-                const float px[4] = {px4.x, px4.y, px4.z, px4.w};
-                const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
-                float dd[4];
-                for (int j = 0; j < 4; ++j) {
-                    const int col = (e0[j] * w0 + e1[j] * w1 + 32) >> 6;
-                    const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
-                    dd[j] = df * df;
-                }
-                total += wmin(dot4(dd, cw), kErrDefault);
-            }
-            if (total < best_err) {
-                best_err = total;
-                best_matched = matched ? 1 : 0;
-                best_quant = final_quant;
-                best_formats = pack_fmt4(fn);
-#pragma unroll
-                for (int p = 0; p < 4; ++p) best_pk[p] = pk[p];
-                for (int i = 0; i < wc; ++i) bqw[i * 64] = qw[i * 64];
             }
         }
-        if (!moved) break;
+        // pass17_compute_final_error.wgsl:149-231
+        for (int i = 0; i < tc && !fused; ++i) {
+            const float w = weight_of(i);
+            int e0[4], e1[4];
+            float bs[4], of[4];
+            ep_at(i, e0, e1, bs, of);
+            const float4 px4 = px_of(i);
+            const float px[4] = {px4.x, px4.y, px4.z, px4.w};
+            const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
+            float dd[4];
+            for (int j = 0; j < 4; ++j) {
+                const int col = (e0[j] * w0 + e1[j] * w1 + 32) >> 6;
+                const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
+                dd[j] = df * df;
+            }
+            total += wmin(dot4(dd, cw), kErrDefault);
+        }
+        if (total < C.best_err) {
+            C.best_err = total;
+            C.best_matched = matched ? 1 : 0;
+            C.best_quant = final_quant;
+            C.best_formats = pack_fmt4(fn);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) C.best_pk[p] = pk[p];
+            for (int i = 0; i < wc; ++i) R.bqw[i * 64] = qw[i * 64];
+        }
     }
+    return moved;
+}
+
+// Writes the candidate's best round to its output slot.
+__device__ __forceinline__ void refine_store(const RefineMode& M, const RefineRows& R, const RefineCand& C,
+                                             CandOut* cand_out, float* cand_err) {
     CandOut best;
-    best.error = best_err;
-    best.bm = (uint16_t)k;
-    best.matched = (uint8_t)best_matched;
-    best.quant = (uint8_t)best_quant;
-    best.formats = best_formats;
+    best.error = C.best_err;
+    best.bm = (uint16_t)M.k;
+    best.matched = (uint8_t)C.best_matched;
+    best.quant = (uint8_t)C.best_quant;
+    best.formats = C.best_formats;
     best.pad = 0;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
-        for (int i = 0; i < 8; ++i) best.packed[p * 8 + i] = (uint8_t)pk_byte(best_pk[p], i);
-    for (int i = 0; i < 64; ++i) best.qw[i] = i < wc ? bqw[i * 64] : 0;
-    cand_out[task] = best;
-    cand_err[task] = best.error;
+        for (int i = 0; i < 8; ++i) best.packed[p * 8 + i] = (uint8_t)pk_byte(C.best_pk[p], i);
+    for (int i = 0; i < 64; ++i) best.qw[i] = i < M.wc ? R.bqw[i * 64] : 0;
+    cand_out[C.task] = best;
+    cand_err[C.task] = best.error;
+}
+
+// One candidate per lane: a workgroup takes one 64-entry tile of the
+// mode-bucketed work list (buckets padded to 64 lanes).  Specialised per
+// partition count, so per-partition state only occupies registers for the
+// partitions that exist.
+template <bool kStage, int PC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PC == 1 ? 6 : 4, 8)))
+k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb, const Ideal* ideal, const int16_t* top,
+         const Ccr* ccr, const uint8_t* qw_all, const int* work, const int* work_total, CandOut* cand_out,
+         float* cand_err, int wstride, int iterations) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = threadIdx.x;
+    const int base = blockIdx.x * 64;
+    if (base >= *work_total) return;
+    const int task = work[base + lane];
+    const int k = __builtin_amdgcn_readfirstlane((int)top[__builtin_amdgcn_readfirstlane(work[base])]) >>
+                  kRefineClassBits;
+    if (task < 0) return;
+    const RefineRows R = refine_rows<kStage, PC>(smem, T, wstride, lane);
+    const RefineMode M = refine_mode(T, k);
+    RefineCand C;
+    refine_init<kStage, PC>(A, T, pb_part, npb, ideal, ccr, qw_all, M, R, task, C);
+    for (int a = 0; a < iterations; ++a)  // 6 in the reference (pass13-17 rounds)
+        if (!refine_round<kStage, PC>(T, M, R, a == 0, C)) break;
+    refine_store(M, R, C, cand_out, cand_err);
+}
+
+// Persistent form: a grid of resident waves that take the work list's
+// 64-entry tiles from a queue (tile_next, zeroed by k_bins) until it is
+// empty.  A lane that finishes its candidate (pass16 moved no weight, or the
+// last round) takes the next entry of the wave's current tile, so the lanes
+// of a wave do not wait for its slowest candidate; a tile of another mode is
+// held until every lane has finished (the wave's tables are one mode's).
+// Free lanes are refilled once at least refill_min of them are free (each
+// refill runs the candidate set-up with the other lanes masked off).  Every
+// candidate's arithmetic is that of k_refine, and it writes the same output
+// slot, so the results are identical.
+template <bool kStage, int PC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PC == 1 ? 6 : 4, 8)))
+k_refine_q(ImageArgs A, DevTables T, const int16_t* pb_part, int npb, const Ideal* ideal, const int16_t* top,
+           const Ccr* ccr, const uint8_t* qw_all, const int* work, const int* work_total, int* tile_next,
+           CandOut* cand_out, float* cand_err, int wstride, int iterations, int refill_min) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = threadIdx.x;
+    const int ntiles = __builtin_amdgcn_readfirstlane(*work_total) >> 6;
+    const RefineRows R = refine_rows<kStage, PC>(smem, T, wstride, lane);
+    RefineMode M{};
+    int k = -1;                     // the wave's mode, -1 before its first tile
+    int pool = 0, pool_end = 0;     // entries of the current tile not handed out yet
+    int pending = -1;               // a tile of another mode, started after the drain
+    bool more = true;               // the queue may hold tiles
+    RefineCand C;
+    C.task = -1;
+    int a = 0;                      // the lane's round
+    for (;;) {
+        uint64_t fm = __ballot(C.task < 0);
+        int nf = __popcll(fm);
+        if (nf >= refill_min || nf == 64) {
+            while (nf > 0) {
+                if (pool >= pool_end) {
+                    if (pending >= 0) {
+                        if (nf < 64) break;  // lanes of the current mode still run
+                        k = __builtin_amdgcn_readfirstlane((int)top[work[pending * 64]]) >> kRefineClassBits;
+                        M = refine_mode(T, k);
+                        pool = pending * 64;
+                        pool_end = pool + 64;
+                        pending = -1;
+                    } else if (more) {
+                        int t = 0;
+                        if (lane == 0) t = atomicAdd(tile_next, 1);
+                        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+                        if (t >= ntiles) {
+                            more = false;
+                            continue;
+                        }
+                        const int m = __builtin_amdgcn_readfirstlane((int)top[work[t * 64]]) >> kRefineClassBits;
+                        if (k >= 0 && m != k) {
+                            pending = t;
+                            continue;
+                        }
+                        if (k < 0) {
+                            k = m;
+                            M = refine_mode(T, k);
+                        }
+                        pool = t * 64;
+                        pool_end = pool + 64;
+                    } else {
+                        break;
+                    }
+                }
+                // free lanes take the tile's next entries in lane order;
+                // padding (-1) ends a bucket's last tile
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0u));
+                const int avail = pool_end - pool;
+                const bool mine = C.task < 0 && rank < avail;
+                const int tk = mine ? work[pool + rank] : -1;
+                pool += nf < avail ? nf : avail;
+                if (__ballot(mine && tk < 0)) pool = pool_end;
+                if (tk >= 0) {
+                    refine_init<kStage, PC>(A, T, pb_part, npb, ideal, ccr, qw_all, M, R, tk, C);
+                    a = 0;
+                }
+                fm = __ballot(C.task < 0);
+                nf = __popcll(fm);
+            }
+        }
+        if (nf == 64) break;  // nothing left: a full refill found no work
+        if (C.task >= 0) {
+            const bool moved = refine_round<kStage, PC>(T, M, R, a == 0, C);
+            ++a;
+            if (!moved || a >= iterations) {
+                refine_store(M, R, C, cand_out, cand_err);
+                C.task = -1;
+            }
+        }
+    }
 }
 
 // ---- pass18 + host argmin (compression_pipeline.cpp:499-519) --------------------
@@ -3867,12 +4099,52 @@ const char* kernel_name(int id) {
         if (prof) prof->end(id, s);                                                \
     } while (0)
 
+// Workgroups of a persistent kernel: as many as the device holds at once
+// (per kernel and LDS size, looked up once per device).
+unsigned resident_workgroups(const void* kernel, size_t lds) {
+    struct Entry { const void* k; size_t lds; int dev; unsigned n; };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const Entry& e : cache)
+        if (e.k == kernel && e.lds == lds && e.dev == dev) return e.n;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, lds) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        return 1024;  // not cached: the queue is correct at any grid size
+    }
+    cache.push_back(Entry{kernel, lds, dev, (unsigned)(cus * per_cu)});
+    return (unsigned)(cus * per_cu);
+}
+
 template <bool kStage>
-void launch_refine_pc(int pc, dim3 grid, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
+void launch_refine_pc(int pc, unsigned tiles, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
                       const ChunkBuffers& B, int npb, int iterations) {
+    if (T.refine_queue > 0) {
+        const int refill = T.refine_queue < 64 ? T.refine_queue : 64;
+#define ASTCG_REFINE_Q(PCV)                                                                                        \
+    do {                                                                                                          \
+        const unsigned n = resident_workgroups(reinterpret_cast<const void*>(&k_refine_q<kStage, PCV>), lds);     \
+        const unsigned grid = std::max(1u, std::min(tiles, n));                                                   \
+        hipLaunchKernelGGL((k_refine_q<kStage, PCV>), dim3(grid), dim3(64), lds, s, A, T, B.pb_part, npb, B.ideal, \
+                           B.top, B.ccr, B.qw, B.work, B.work_total, B.tile_next, B.cand, B.cand_err, B.wstride,  \
+                           iterations, refill);                                                                  \
+    } while (0)
+        switch (pc) {
+            case 1: ASTCG_REFINE_Q(1); break;
+            case 2: ASTCG_REFINE_Q(2); break;
+            case 3: ASTCG_REFINE_Q(3); break;
+            default: ASTCG_REFINE_Q(4); break;
+        }
+#undef ASTCG_REFINE_Q
+        return;
+    }
 #define ASTCG_REFINE(PCV)                                                                                         \
-    hipLaunchKernelGGL((k_refine<kStage, PCV>), grid, dim3(64), lds, s, A, T, B.pb_part, npb, B.ideal, B.top, B.ccr, \
-                       B.qw, B.work, B.work_total, B.cand, B.cand_err, B.wstride, iterations)
+    hipLaunchKernelGGL((k_refine<kStage, PCV>), dim3(tiles), dim3(64), lds, s, A, T, B.pb_part, npb, B.ideal, B.top, \
+                       B.ccr, B.qw, B.work, B.work_total, B.cand, B.cand_err, B.wstride, iterations)
     switch (pc) {
         case 1: ASTCG_REFINE(1); break;
         case 2: ASTCG_REFINE(2); break;
@@ -3882,10 +4154,12 @@ void launch_refine_pc(int pc, dim3 grid, size_t lds, hipStream_t s, const ImageA
 #undef ASTCG_REFINE
 }
 
-void launch_refine(bool stage, int pc, dim3 grid, size_t lds, hipStream_t s, const ImageArgs& A, const DevTables& T,
-                   const ChunkBuffers& B, int npb, int iterations) {
-    if (stage) launch_refine_pc<true>(pc, grid, lds, s, A, T, B, npb, iterations);
-    else launch_refine_pc<false>(pc, grid, lds, s, A, T, B, npb, iterations);
+// tiles: an upper bound of the work list's 64-entry tiles (the list length
+// is only known on the device)
+void launch_refine(bool stage, int pc, unsigned tiles, size_t lds, hipStream_t s, const ImageArgs& A,
+                   const DevTables& T, const ChunkBuffers& B, int npb, int iterations) {
+    if (stage) launch_refine_pc<true>(pc, tiles, lds, s, A, T, B, npb, iterations);
+    else launch_refine_pc<false>(pc, tiles, lds, s, A, T, B, npb, iterations);
 }
 
 // ---- exhaustive partition candidates of a short block list ----------------
@@ -4306,21 +4580,23 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
             if (pb_est)
                 hipLaunchKernelGGL(k_pb_est, dim3(blocks_for(npb)), dim3(64), 0, s, A, pc, npb, T.nbm, B.ccr,
                                    mode_mask, mask_words, pb_est);
-            const size_t hist = (size_t)T.nbm * sizeof(int);
+            const int nbins = T.nbm * kRefineClasses;
+            const size_t hist = (size_t)nbins * sizeof(int);
             hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, A, pc, npb, T.nbm, B.ccr, B.top,
                                B.bin_count, B.cand, B.cand_err, mode_mask, mask_words, slots, pb_est,
-                               plan ? plan->keep_refine : 4, plan ? plan->fallback : nullptr);
-            hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), hist + sizeof(int), s, T.nbm, B.bin_count, B.bin_cur,
-                               B.work, B.work_total);
-            hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, A, pc, npb, T.nbm, B.top,
+                               plan ? plan->keep_refine : 4, plan ? plan->fallback : nullptr,
+                               T.refine_classes);
+            hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), 0, s, nbins, B.bin_count, B.bin_cur, B.work, B.work_total,
+                               B.tile_next);
+            hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, A, pc, npb, nbins, B.top,
                                B.bin_cur, B.work);
             if (prof) prof->end(K_TOP, s);
         }
         {
             if (prof) prof->begin(K_REFINE, s);
             const size_t lds = refine_lds_bytes(T, B.wstride, pc);
-            const unsigned grid = blocks_for((long long)npb * 8 + (long long)T.nbm * 63);
-            launch_refine(T.tc <= T.refine_stage_max_tc, pc, dim3(grid), lds, s, A, T, B, npb,
+            const unsigned tiles = blocks_for((long long)npb * 8 + (long long)T.nbm * kRefineClasses * 63);
+            launch_refine(T.tc <= T.refine_stage_max_tc, pc, tiles, lds, s, A, T, B, npb,
                           plan ? plan->iterations : 6);
             if (prof) prof->end(K_REFINE, s);
         }

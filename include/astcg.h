@@ -120,6 +120,12 @@ typedef struct {
     uint32_t exw_max_list;        /* exhaustive searches of block lists (LEARNED hard / fallback blocks) up to this
                                      many blocks run one wave per block, longer ones one lane per block
                                      (default 65536) */
+    uint32_t refine_queue;        /* 0: the refinement runs a workgroup per 64 candidates of one block mode (default);
+                                     n >= 1: a persistent work-queue kernel whose waves take those 64-candidate
+                                     tiles from a queue and give a lane the next candidate of its tile once n lanes
+                                     (at most 64) have finished theirs */
+    uint32_t refine_classes;      /* 1: the refinement's candidates are grouped by block mode and endpoint format
+                                     class, 0: by block mode only (default 1 where the refinement stages texels) */
 } astcg_tuning;
 
 astcg_status astcg_get_tuning(astcg_encoder* enc, uint32_t bx, uint32_t by, astcg_tuning* out);
@@ -128,9 +134,9 @@ astcg_status astcg_set_tuning(astcg_encoder* enc, uint32_t bx, uint32_t by, cons
  * cache_path that holds an entry for this GPU (architecture and CU count) and
  * footprint, that entry is applied; otherwise each setting is timed on
  * encodes of a synthetic 1024x1024 image (the LDS staging of the refinement
- * under the COMPAT search, the float staging of the exhaustive partition
- * search under the EXHAUSTIVE search), the faster one kept, and, with a
- * cache_path, appended to that file.  An encoder whose footprint tables are
+ * and its work queue under the COMPAT search, the float staging of the
+ * exhaustive partition search under the EXHAUSTIVE search), the fastest one
+ * kept, and, with a cache_path, appended to that file.  An encoder whose footprint tables are
  * first built while the environment variable ASTCG_TUNE_CACHE names such a
  * file applies its entry the same way.  The encoder's configuration is kept
  * (configure is re-run with the last parameters). */

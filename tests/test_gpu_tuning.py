@@ -1,9 +1,10 @@
 """Launch-shape tuning (astcg_tuning, astcg_autotune): every setting leaves
-the encoded blocks byte-identical (texel staging of the refinement, float
-staging of the exhaustive partition search, the wave- / lane-per-block
-choice for exhaustive block lists), and the autotuner's on-disk cache is
-written, read back by another encoder (also through ASTCG_TUNE_CACHE), and
-gives the same blocks."""
+the encoded blocks byte-identical (texel staging of the refinement, the
+refinement's persistent work queue and its refill threshold against the
+workgroup-per-tile kernel, float staging of the exhaustive partition search,
+the wave- / lane-per-block choice for exhaustive block lists), and the
+autotuner's on-disk cache is written, read back by another encoder (also
+through ASTCG_TUNE_CACHE), and gives the same blocks."""
 import os
 import subprocess
 import sys
@@ -32,9 +33,12 @@ def test_every_tuning_gives_the_same_blocks(bx, by):
     try:
         base = {p: _encode(enc, img, bx, by, p) for p in (A.POLICY_COMPAT, A.POLICY_EXHAUSTIVE)}
         default = enc.tuning(bx, by)
-        assert default == {"refine_stage_max_tc": 25, "ex_f4_max_tc": 16, "exw_max_list": 65536}
+        assert default == {"refine_stage_max_tc": 25, "ex_f4_max_tc": 16, "exw_max_list": 65536, "refine_queue": 0,
+                           "refine_classes": int(bx * by <= 25)}
         for t in ({"refine_stage_max_tc": 0}, {"refine_stage_max_tc": 144}, {"ex_f4_max_tc": 0},
-                  {"ex_f4_max_tc": 64}):
+                  {"ex_f4_max_tc": 64}, {"refine_queue": 16}, {"refine_queue": 1}, {"refine_queue": 64},
+                  {"refine_queue": 16, "refine_stage_max_tc": 0}, {"refine_queue": 7, "refine_stage_max_tc": 0},
+                  {"refine_classes": 1 - int(bx * by <= 25)}, {"refine_classes": 0, "refine_queue": 16}):
             enc.set_tuning(bx, by, **default)
             enc.set_tuning(bx, by, **t)
             for p, ref in base.items():
@@ -76,7 +80,7 @@ def test_autotune_cache_round_trip(tmp_path):
     finally:
         enc.close()
     lines = open(cache).read().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("astcg-tune 1 ") and " 6x6 " in lines[0]
+    assert len(lines) == 1 and lines[0].startswith("astcg-tune 2 ") and " 6x6 " in lines[0]
     enc2 = A.Encoder(0)
     try:
         assert enc2.autotune(6, 6, cache) == tuned  # read back, not timed again
@@ -92,3 +96,27 @@ def test_autotune_cache_round_trip(tmp_path):
                          cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().splitlines()[-1] == str(sorted(tuned.items()))
+
+
+def test_learned_and_batch_under_both_refinement_kernels():
+    """The learned policy (compacted partitioned-block lists) and a batch of
+    uneven textures of two footprints give the same blocks through the
+    persistent refinement and the workgroup-per-tile kernel, with and without
+    the format classes."""
+    img = synth.gradient_image(264, 200, 37)
+    texs = [synth.gradient_image(w, h, 40 + i) for i, (w, h) in enumerate([(70, 45), (33, 90), (128, 17)])]
+    foot = [(4, 4), (6, 6), (4, 4)]
+    enc = A.Encoder(0)
+    try:
+        outs = []
+        for q, c in ((0, 1), (16, 1), (3, 0), (0, 0)):
+            for bx, by in ((4, 4), (6, 6)):
+                enc.set_tuning(bx, by, refine_queue=q, refine_classes=c)
+            learned = _encode(enc, img, 4, 4, A.POLICY_LEARNED)
+            batch = enc.encode_batch(texs, foot)
+            outs.append((learned, batch))
+        for learned, batch in outs[1:]:
+            assert np.array_equal(learned, outs[0][0])
+            assert all(np.array_equal(a, b) for a, b in zip(batch, outs[0][1]))
+    finally:
+        enc.close()
