@@ -530,9 +530,10 @@ astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpS
     return st;
 }
 
-// Blocks per chunk: keeps the per-chunk working set near 16 GiB, so a 4096^2
+// Blocks per chunk: keeps the per-chunk working set near 20 GiB, so a 4096^2
 // image takes one chunk at every footprint (14.0 GiB at 4x4, less at larger
-// footprints).  Every partition count of a chunk ends with a few long
+// footprints; up to 17 GiB at 4x4 with the learned policy's plan buffers for
+// 64 predicted seeds).  Every partition count of a chunk ends with a few long
 // refinement lanes that leave most of the GPU idle, so fewer, larger chunks
 // pay that tail fewer times.  A requested size is capped so that the element
 // indices of the per-PB arrays ([colour-table row or trial mode][PB]) stay
@@ -544,7 +545,7 @@ int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested, const P
     const size_t fixed = chunk_bytes_per_block(f, offs, 0) + plan_bytes(f, pred, 0);
     const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) + plan_bytes(f, pred, 1 << 16) - fixed) >> 16;
     const size_t max_chunk = (((size_t)1 << 31) - 1) / (4 * (size_t)std::max(kColorTabRows, f.nbm) + 8 * 4);
-    size_t chunk = std::max<size_t>(1024, ((size_t(16) << 30) - fixed) / per_block);
+    size_t chunk = std::max<size_t>(1024, ((size_t(20) << 30) - fixed) / per_block);
     if (requested) chunk = requested;
     chunk = std::min<size_t>(chunk, max_chunk);
     return (int)std::max<size_t>(1, std::min<size_t>(chunk, nblocks));

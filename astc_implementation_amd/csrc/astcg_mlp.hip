@@ -449,13 +449,16 @@ __global__ void __launch_bounds__(256) k_infer(int rows, const bf16* __restrict_
 // Seed-head top-k of the listed rows for one partition count: the logits
 // h2 W3 + b3 of the head's 1024 columns (the same matrix instructions in the
 // same K order as the GEMM, so the same values as the separate kernels),
-// then the k largest (value descending, lowest column first on ties) by
-// repeated arg-max, one wave per row; the selected column is removed from
-// its owner lane's registers after each round.  16 rows per 256-thread
-// workgroup (one 16-row tile of the matrix instruction, so each 512 KB head
-// is read once per 16 rows); each wave computes 256 of the columns.
+// then the k largest (value descending, lowest column first on ties), one
+// wave per row.  16 rows per 256-thread workgroup (one 16-row tile of the
+// matrix instruction, so each 512 KB head is read once per 16 rows); each
+// wave computes 256 of the columns.  A row's logits sit in LDS until the
+// wave has them in registers; that LDS row then holds the selection's
+// histogram and candidate list, so two workgroups fit a CU.
 constexpr int kSR = 16, kSeedCols = 1024, kSL = kSeedCols + 1;
-constexpr int kSeedCand = 192;  // logits at or above the threshold ranked directly
+constexpr int kSeedBuckets = 256;  // histogram of a row's logits over [min, max]
+constexpr int kSeedCand = 192;     // logits in the selected buckets ranked directly
+static_assert(kSeedBuckets + 2 * kSeedCand <= kSL, "selection scratch fits a logit row");
 
 // Orders one wave's LDS accesses (the waves of k_seed_topk handle different
 // numbers of rows, so a workgroup barrier cannot be used there).
@@ -471,8 +474,6 @@ __global__ void __launch_bounds__(256) k_seed_topk(const int* __restrict__ list,
                                                     int* __restrict__ out_idx, int out_stride, int out_off) {
     __shared__ __align__(16) bf16 sA[16 * kFS];
     __shared__ float sL[kSR * kSL];
-    __shared__ float sM[4 * 64], sCV[4 * kSeedCand];
-    __shared__ int sCI[4 * kSeedCand];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int n = list ? min(*list_count, rows) : rows;
     const int r0 = blockIdx.x * kSR;
@@ -514,48 +515,75 @@ __global__ void __launch_bounds__(256) k_seed_topk(const int* __restrict__ list,
         }
     }
     __syncthreads();
-    float* s_max = sM + 64 * wave;     // [64] the lanes' largest logits
-    float* s_cv = sCV + kSeedCand * wave;  // [kSeedCand] candidate values
-    int* s_ci = sCI + kSeedCand * wave;    // [kSeedCand] candidate columns
     for (int rr = wave; rr < kSR && r0 + rr < n; rr += 4) {
         const int row = list ? list[r0 + rr] : r0 + rr;
         float v[kSeedCols / 64];
 #pragma unroll
         for (int m = 0; m < kSeedCols / 64; ++m) v[m] = sL[rr * kSL + lane + 64 * m];
-        // Every one of the k largest is at least theta, the k-th largest of
-        // the 64 lanes' maxima (64 distinct logits).  When at most kSeedCand
-        // logits reach theta, they are ranked directly (value descending,
-        // lowest column first); otherwise (many ties) the repeated arg-max
-        // below does it.
-        float mv = v[0];
+        // Bucket threshold: the row's logits are histogrammed into 256
+        // buckets over [min, max] (a monotone map, so a larger logit never
+        // falls into a lower bucket); B is the highest bucket with at least k
+        // logits in buckets >= B, so those buckets hold the k largest.  When
+        // at most kSeedCand logits are there, they are ranked directly (value
+        // descending, lowest column first).  Otherwise (heavy ties, or a
+        // non-finite logit) the repeated arg-max below selects them.
+        float lo = v[0], hi = v[0];
+        bool finite = true;
 #pragma unroll
-        for (int m = 1; m < kSeedCols / 64; ++m) mv = fmaxf(mv, v[m]);
-        s_max[lane] = mv;
+        for (int m = 0; m < kSeedCols / 64; ++m) {
+            lo = fminf(lo, v[m]);
+            hi = fmaxf(hi, v[m]);
+            finite = finite && isfinite(v[m]);
+        }
+#pragma unroll
+        for (int msk = 32; msk > 0; msk >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, msk));
+            hi = fmaxf(hi, __shfl_xor(hi, msk));
+        }
+        finite = __ballot(!finite) == 0;
+        int* s_h = reinterpret_cast<int*>(sL + rr * kSL);  // [kSeedBuckets], the row's logits are in v now
+        float* s_cv = sL + rr * kSL + kSeedBuckets;       // [kSeedCand] candidate values
+        int* s_ci = reinterpret_cast<int*>(s_cv + kSeedCand);  // [kSeedCand] candidate columns
         wave_sync();
-        int rank = 0;
-        for (int l = 0; l < 64; ++l) {
-            const float o = s_max[l];
-            rank += (o > mv) || (o == mv && l < lane);
+        for (int i = lane; i < kSeedBuckets; i += 64) s_h[i] = 0;
+        wave_sync();
+        const float scale = (finite && hi > lo) ? (float)(kSeedBuckets - 1) / (hi - lo) : 0.0f;
+        int bk[kSeedCols / 64];
+#pragma unroll
+        for (int m = 0; m < kSeedCols / 64; ++m) {
+            bk[m] = min(kSeedBuckets - 1, max(0, (int)((v[m] - lo) * scale)));
+            atomicAdd(&s_h[bk[m]], 1);
         }
         wave_sync();
-        if (rank == k - 1) s_cv[0] = mv;
-        wave_sync();
-        const float theta = s_cv[0];
+        // suffix counts: lane l holds buckets 4l .. 4l + 3
+        const int h0 = s_h[4 * lane], h1 = s_h[4 * lane + 1], h2c = s_h[4 * lane + 2], h3 = s_h[4 * lane + 3];
+        int suf = h0 + h1 + h2c + h3;  // inclusive suffix over lanes lane .. 63
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_down(suf, d);
+            if (lane + d < 64) suf += o;
+        }
+        const int s3 = suf - h0 - h1 - h2c, s2 = s3 + h2c, s1 = s2 + h1;
+        int bsel = -1;
+        if (s3 >= k) bsel = 4 * lane + 3;
+        else if (s2 >= k) bsel = 4 * lane + 2;
+        else if (s1 >= k) bsel = 4 * lane + 1;
+        else if (suf >= k) bsel = 4 * lane;
+#pragma unroll
+        for (int msk = 32; msk > 0; msk >>= 1) bsel = max(bsel, __shfl_xor(bsel, msk));
         int mine = 0;
 #pragma unroll
-        for (int m = 0; m < kSeedCols / 64; ++m) mine += v[m] >= theta;
+        for (int m = 0; m < kSeedCols / 64; ++m) mine += bk[m] >= bsel;
         int before = mine;  // inclusive prefix over the lanes
         for (int d = 1; d < 64; d <<= 1) {
             const int o = __shfl_up(before, d);
             if (lane >= d) before += o;
         }
         const int total = __shfl(before, 63);
-        wave_sync();
-        if (k <= 64 && total <= kSeedCand && theta == theta) {
+        if (finite && scale > 0.0f && total <= kSeedCand) {
             int slot = before - mine;
 #pragma unroll
             for (int m = 0; m < kSeedCols / 64; ++m)
-                if (v[m] >= theta) {
+                if (bk[m] >= bsel) {
                     s_cv[slot] = v[m];
                     s_ci[slot] = lane + 64 * m;
                     ++slot;

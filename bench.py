@@ -409,10 +409,19 @@ def main():
             "hbm": {"achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS},
             "traffic_source": traffic_src}
     if valu:
-        # the search kernels are VALU-bound: the counter-measured VALU busy
-        # share of the kernel's duration (its profile's SQ counters)
-        roof.update({"achieved": valu.get("valu_busy_pct"), "peak": 100.0, "unit": "% VALU busy",
-                     "frac": (valu.get("valu_busy_pct") or 0.0) / 100.0,
+        # the search kernels are VALU-bound.  From the kernel's profile (SQ
+        # counters): VALUBusy, the share of its time the SIMDs issue VALU
+        # instructions (a counter ratio that reads slightly above 100 when
+        # the SIMDs never idle), and VALUUtilization, the share of the 64
+        # lanes active per issued instruction.  Their product (busy capped at
+        # 100 %) is the share of the peak VALU lane throughput doing useful
+        # work: the kernel's VALU roofline fraction.
+        busy = min(valu.get("valu_busy_pct") or 0.0, 100.0) / 100.0
+        lanes = (valu.get("valu_lane_util_pct") or 0.0) / 100.0
+        roof.update({"achieved": round(100.0 * busy * lanes, 1), "peak": 100.0,
+                     "unit": "% of peak VALU lane throughput (VALU busy x lane utilisation)",
+                     "frac": round(busy * lanes, 3),
+                     "valu_busy_pct": valu.get("valu_busy_pct"),
                      "valu_lane_utilisation_pct": valu.get("valu_lane_util_pct"),
                      "lds_bank_conflict_pct": valu.get("lds_bank_conflict_pct")})
     line = {

@@ -165,7 +165,7 @@ astcg_status astcg_autotune(astcg_encoder* enc, uint32_t bx, uint32_t by, const 
 typedef struct {
     uint32_t max_partitions;  /* 1..4 (reference: 4) */
     uint32_t policy;          /* astcg_policy */
-    uint32_t chunk_blocks;    /* blocks per device chunk, 0 = automatic (a working set near 16 GiB) */
+    uint32_t chunk_blocks;    /* blocks per device chunk, 0 = automatic (a working set near 20 GiB) */
     uint32_t flags;           /* ASTCG_FLAG_* */
     uint32_t reserved[4];
 } astcg_params;

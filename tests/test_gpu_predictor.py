@@ -374,13 +374,14 @@ def test_fused_inference_matches_unfused(block):
     assert (out_idx[:, 1:4].cpu() == -7).all()  # the seed heads are not written
 
 
-@pytest.mark.parametrize("k", [1, 8, 33, 48, 64])
+@pytest.mark.parametrize("k", [1, 8, 33, 48, 64, 100, 200])
 def test_seed_topk_matches_unfused(k):
     """astcg_mlp_seed_topk (one partition count's seed-head logits of the
     listed rows from h2, then their top-k) against the GEMM over every
     output column + top-k of the seed head: same matrix instructions in the
     same K order, so the same indices (two identical seed columns: the
-    lower first); unlisted rows and other heads are not written."""
+    lower first); unlisted rows and other heads are not written.  k up to
+    192 takes the histogram-threshold selection, 200 the repeated arg-max."""
     P = Predictor((4, 4), seed=13)
     g = torch.Generator().manual_seed(14 + k)
     with torch.no_grad():
