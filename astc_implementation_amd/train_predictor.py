@@ -160,8 +160,23 @@ def main():
     ap.add_argument("--full-search-modes", type=int, default=0)
     ap.add_argument("--mismatch-candidates", type=int, default=0)
     ap.add_argument("--fallback-above", type=float, default=0.0)
+    ap.add_argument("--retune", default=None, metavar="NPZ",
+                    help="no training: write NPZ's weights again (to --out, default NPZ itself) with the pruning "
+                         "settings given here")
     a = ap.parse_args()
     bx, by = (int(v) for v in a.block.split("x"))
+    prune = dict(seeds=a.seeds, modes=a.modes, slots=a.slots, fallback_below=a.fallback_below,
+                 count_below=a.count_below, partitionings=a.partitionings,
+                 refine_partitionings=a.refine_partitionings, early_out=a.early_out,
+                 full_search_above=a.full_search_above, rounds=a.rounds,
+                 partitioned_slots=a.partitioned_slots, candidates=a.candidates,
+                 count_above=a.count_above, full_search_modes=a.full_search_modes,
+                 mismatch_candidates=a.mismatch_candidates, fallback_above=a.fallback_above)
+    if a.retune:
+        params, hidden = load_params(a.retune)
+        save_arrays(a.out or a.retune, params, hidden, (bx, by), prune)
+        print(json.dumps({"block": a.block, "retuned": a.out or a.retune, "pruning": prune}))
+        return
     import torch
     torch.manual_seed(0)
     P = Predictor((bx, by), seed=0)
@@ -183,13 +198,7 @@ def main():
     print(json.dumps({"block": a.block, "train_blocks": int(X.shape[0] - n_val), "label_s": round(t1 - t0, 1),
                       "train_s": round(t2 - t1, 1), "val_recall_topk": rec}))
     out = a.out or os.path.join(HERE, "predictor_%dx%d.npz" % (bx, by))
-    save(out, P, dict(seeds=a.seeds, modes=a.modes, slots=a.slots, fallback_below=a.fallback_below,
-                      count_below=a.count_below, partitionings=a.partitionings,
-                      refine_partitionings=a.refine_partitionings, early_out=a.early_out,
-                      full_search_above=a.full_search_above, rounds=a.rounds,
-                      partitioned_slots=a.partitioned_slots, candidates=a.candidates,
-                      count_above=a.count_above, full_search_modes=a.full_search_modes,
-                      mismatch_candidates=a.mismatch_candidates, fallback_above=a.fallback_above))
+    save(out, P, prune)
     enc.close()
 
 
@@ -197,8 +206,14 @@ def save(path, P, prune):
     """Weights as fp16 plus the pruning settings (LearnedParams fields) to
     install with them."""
     st = P.state()
-    np.savez_compressed(path, **{"p%d" % i: w.astype(np.float16) for i, w in enumerate(st["params"])},
-                        hidden=np.int32(P.hidden), block=np.array([P.bx, P.by], np.int32),
+    save_arrays(path, st["params"], P.hidden, (P.bx, P.by), prune)
+
+
+def save_arrays(path, params, hidden, block, prune):
+    """The six parameter arrays (stored as fp16: a saved file read back and
+    written again keeps its weights bit for bit) and the pruning settings."""
+    np.savez_compressed(path, **{"p%d" % i: np.asarray(w).astype(np.float16) for i, w in enumerate(params)},
+                        hidden=np.int32(hidden), block=np.array(block, np.int32),
                         prune_i=np.array([prune["seeds"], prune["modes"], prune["slots"]], np.int32),
                         prune_f=np.array([prune["fallback_below"], prune["count_below"]], np.float32),
                         prune_p=np.int32(prune.get("partitionings", 0)),
