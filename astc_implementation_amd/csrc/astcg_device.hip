@@ -160,19 +160,17 @@ __device__ __forceinline__ uint32_t texel_rgba8(const BlockLoc& l, const DevTabl
     return *reinterpret_cast<const uint32_t*>(l.rgba + ((size_t)y * l.width + x) * 4);
 }
 
-// (u8 / 255) * 65536 with IEEE float division semantics, without a divide or
-// a table: the product with the rounded reciprocal is corrected by one fused
-// residual step (Markstein), which is exact for all 256 inputs, and the
-// scaling by 2^16 is exact.
-// The same operations with the 2^16 scaling folded in (exact: scaling by a
-// power of two commutes with rounding here): q' = x * (r * 2^16) = q * 2^16,
-// and fma(-q', 255 * 2^-16, x) = fma(-q, 255, x); one multiply fewer.
+// (u8 / 255) * 65536 as IEEE float division and scaling round it, without a
+// divide or a table: 65536 / 255 = 257 + 1 / 255, and 257 * u8 is an exact
+// float, so one fused multiply-add with the rounded reciprocal of 255,
+// fma(u8, RN(1/255), 257 * u8), rounds the exact 257 u8 + u8 / 255 once.
+// Its error term u8 * (RN(1/255) - 1/255) is below 2^-33 * 255, which moves
+// none of the 256 values across a rounding boundary (checked exhaustively in
+// exact rational arithmetic; the parity tests compare with the oracle's
+// division).
 __device__ __forceinline__ float u8_search_value(uint32_t v) {
     const float x = (float)v;
-    const float r16 = (1.0f / 255.0f) * 65536.0f;
-    const float q16 = x * r16;
-    const float e = __builtin_fmaf(-q16, 255.0f / 65536.0f, x);
-    return __builtin_fmaf(e, r16, q16);
+    return __builtin_fmaf(x, 1.0f / 255.0f, x * 257.0f);
 }
 
 // a texel's channels as search values (arithmetic, not a table: four fewer
