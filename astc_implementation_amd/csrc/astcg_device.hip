@@ -1828,6 +1828,11 @@ __device__ __forceinline__ int pb_est_rank(const ImageArgs& A, int pc, int pb, c
 // Top-8 trial modes per partitioned block, plus the per-mode candidate counts
 // that size the refinement work list (LDS histogram, one global add per
 // workgroup and mode).  Slots without a candidate get the default error.
+// k_top and k_scatter: partitioned blocks per workgroup (a workgroup clears
+// and flushes one LDS histogram over all buckets, so larger workgroups do
+// that once for more blocks)
+constexpr int kTopThreads = 256;
+
 // Work-list class of a candidate: pass14 packs each partition's endpoints
 // along the path of its requested format (luminance, luminance + alpha, RGB
 // scale, RGB scale + alpha, RGB, RGBA); the RGB paths try the delta and
@@ -1852,13 +1857,13 @@ __device__ __forceinline__ int refine_class(const Ccr& c, int pc) {
     return 4 + 3 * (fam - 4) + qc;  // 4..9
 }
 
-__global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr, int16_t* top,
+__global__ void __launch_bounds__(kTopThreads) k_top(ImageArgs A, int pc, int npb, int nbm, const Ccr* ccr, int16_t* top,
                                             int* bin_count, CandOut* cand, float* cand_err, const uint32_t* mode_mask,
                                             int mask_words, const uint8_t* slots, const float* pb_est, int keep_refine,
                                             const uint8_t* fallback, int classes) {
     extern __shared__ int s_hist[];  // [nbm * kRefineClasses]
     const int nbins = nbm * kRefineClasses;
-    for (int i = threadIdx.x; i < nbins; i += 64) s_hist[i] = 0;
+    for (int i = threadIdx.x; i < nbins; i += kTopThreads) s_hist[i] = 0;
     __syncthreads();
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     if (pb < pb_limit(A, pc, npb)) {
@@ -1903,7 +1908,7 @@ __global__ void __launch_bounds__(64) k_top(ImageArgs A, int pc, int npb, int nb
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nbins; i += 64)
+    for (int i = threadIdx.x; i < nbins; i += kTopThreads)
         if (s_hist[i]) atomicAdd(&bin_count[i], s_hist[i]);
 }
 
@@ -1940,14 +1945,19 @@ __global__ void __launch_bounds__(64) k_bins(int nbins, int* bin_count, int* bin
     }
 }
 
-// Fills the buckets: positions are reserved per workgroup and mode, so the
-// order inside a bucket is arbitrary; every task writes its own output slot.
-__global__ void __launch_bounds__(64) k_scatter(ImageArgs A, int pc, int npb, int nbins, const int16_t* top,
+// Fills the buckets: positions are reserved per workgroup and bucket, so the
+// order inside a bucket is arbitrary (every task writes its own output
+// slot) but keeps each workgroup's partitioned blocks together.  Where the
+// refinement stages texels, 256 partitioned blocks per workgroup (a tile of
+// the refinement then reads the candidate state of fewer, neighbouring
+// blocks: 4x4 COMPAT refinement 171 -> 164 ms); unstaged, 64 (its texel
+// reads lose locality with the wider ranges: 6x6 learned 67 -> 71 ms).
+__global__ void __launch_bounds__(kTopThreads) k_scatter(ImageArgs A, int pc, int npb, int nbins, const int16_t* top,
                                                 int* bin_cur, int* work) {
     extern __shared__ int s_sc[];  // [2][nbins]
     int* s_cnt = s_sc;
     int* s_base = s_sc + nbins;
-    for (int i = threadIdx.x; i < nbins; i += 64) s_cnt[i] = 0;
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_cnt[i] = 0;
     __syncthreads();
     const int pb = blockIdx.x * blockDim.x + threadIdx.x;
     int kk[8], loc[8];
@@ -1956,7 +1966,7 @@ __global__ void __launch_bounds__(64) k_scatter(ImageArgs A, int pc, int npb, in
         loc[s] = kk[s] >= 0 ? atomicAdd(&s_cnt[kk[s]], 1) : 0;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nbins; i += 64) {
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
         const int c = s_cnt[i];
         s_base[i] = c ? atomicAdd(&bin_cur[i], c) : 0;
     }
@@ -4580,13 +4590,16 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                                    mode_mask, mask_words, pb_est);
             const int nbins = T.nbm * kRefineClasses;
             const size_t hist = (size_t)nbins * sizeof(int);
-            hipLaunchKernelGGL(k_top, dim3(blocks_for(npb)), dim3(64), hist, s, A, pc, npb, T.nbm, B.ccr, B.top,
+            const unsigned tgrid = (unsigned)((npb + kTopThreads - 1) / kTopThreads);
+            hipLaunchKernelGGL(k_top, dim3(tgrid), dim3(kTopThreads), hist, s, A, pc, npb, T.nbm, B.ccr, B.top,
                                B.bin_count, B.cand, B.cand_err, mode_mask, mask_words, slots, pb_est,
                                plan ? plan->keep_refine : 4, plan ? plan->fallback : nullptr,
                                T.refine_classes);
             hipLaunchKernelGGL(k_bins, dim3(1), dim3(64), 0, s, nbins, B.bin_count, B.bin_cur, B.work, B.work_total,
                                B.tile_next);
-            hipLaunchKernelGGL(k_scatter, dim3(blocks_for(npb)), dim3(64), 2 * hist, s, A, pc, npb, nbins, B.top,
+            const int sthreads = T.tc <= T.refine_stage_max_tc ? kTopThreads : 64;
+            hipLaunchKernelGGL(k_scatter, dim3((unsigned)((npb + sthreads - 1) / sthreads)), dim3(sthreads), 2 * hist,
+                               s, A, pc, npb, nbins, B.top,
                                B.bin_cur, B.work);
             if (prof) prof->end(K_TOP, s);
         }
