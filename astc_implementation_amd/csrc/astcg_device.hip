@@ -885,7 +885,7 @@ __global__ void __launch_bounds__(64) k_partition_ex(ImageArgs A, DevTables T, i
     // scratch column of this lane (blocks of a list use their list slot;
     // short exhaustive lists are k_partition_exw's)
     const int col = blockIdx.x * 64 + lane;
-    if (list && !kSeeded && *list_count <= T.exw_max_list) return;
+    if (list && *list_count <= T.exw_max_list && (!kSeeded || P.mis_candidates == 0)) return;  // wave-per-block kernels
     const int k = list ? (col < *list_count ? list[col] : nb) : col;
     uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);             // [tc][64] RGBA8 (!kF4)
     float4* s_f4 = reinterpret_cast<float4*>(smem);                    // [tc][64] search values (kF4)
@@ -4433,6 +4433,106 @@ __global__ void __launch_bounds__(64) k_partition_exw(ImageArgs A, DevTables T, 
     }
 }
 
+// ---- the learned policy's seeded candidates of a short block list ---------
+// The seeded search of k_partition_ex (its `seeds` predicted seeds of the
+// count, ranked by the uncorrelated / same-chroma line fits), one wave per
+// listed block instead of one lane: lane s fits the block's s-th seed (the
+// block's texels in LDS, read by every lane alike), then every lane runs the
+// top-4 insertion over the seeds in their listed order, so the candidates are
+// those of the lane-per-block kernel bit for bit.  A lane per block keeps 48
+// line fits in one lane's chain, which bounds the latency of a small image;
+// the workgroups loop over the list like k_partition_exw's.  Only without a
+// mismatch-order prefix (mismatch_candidates 0); the lane kernel does that.
+size_t partition_exs_lds_bytes(const DevTables& T) { return (size_t)T.tc * 16 + 3 * 64 * 4; }
+
+template <int PC, int kW>
+__global__ void __launch_bounds__(64) k_partition_exs(ImageArgs A, DevTables T, int16_t* pb_part, const int* list,
+                                                      const int* list_count, SearchPlan P) {
+    constexpr int pc = PC;
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tc = T.tc, lane = threadIdx.x;
+    float4* s_px = reinterpret_cast<float4*>(smem);               // [tc]
+    float* s_eu = reinterpret_cast<float*>(s_px + tc);           // [64] per seed slot
+    float* s_es = s_eu + 64;                                      // [64]
+    int* s_i = reinterpret_cast<int*>(s_es + 64);                 // [64] packed entry, -1 = not fitted
+    const int n = *list_count, nsel = T.count_selected[pc - 1];
+    if (n > T.exw_max_list || P.mis_candidates > 0) return;  // k_partition_ex takes this list
+    const int nseeds = min(min(P.seeds, P.kmax), 64);
+    auto PX = [&](int t, float* o) __attribute__((always_inline)) {
+        const float4 f = s_px[t];
+        o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
+    };
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const int k = list[li];
+        const BlockLoc loc = block_loc(A, T, A.blk0 + k);
+        uint32_t amin = 0xFFu, amax = 0u;
+        for (int t = lane; t < tc; t += 64) {
+            const uint32_t v = texel_rgba8(loc, T, t);
+            s_px[t] = make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
+                                  u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
+            amin = min(amin, v >> 24);
+            amax = max(amax, v >> 24);
+        }
+        for (int m = 32; m > 0; m >>= 1) {
+            amin = min(amin, (uint32_t)__shfl_xor((int)amin, m));
+            amax = max(amax, (uint32_t)__shfl_xor((int)amax, m));
+        }
+        __syncthreads();
+        const bool rgba = amin != amax;
+        float wie = 0.055f;
+        if (tc <= 20) wie = 0.03f;
+        else if (tc <= 31) wie = 0.04f;
+        else if (tc <= 41) wie = 0.05f;
+        wie = wie * wie;
+        // the list ends at its first seed outside the head; seeds that are
+        // not selected partitionings of this count are skipped
+        const int* seeds = P.topk_idx + (size_t)k * 5 * P.kmax + (size_t)(pc - 1) * P.kmax;
+        const int seed = lane < nseeds ? seeds[lane] : -1;
+        const uint64_t ends = __ballot(lane < nseeds && (seed < 0 || seed >= kPartitionings));
+        const int nlist = ends ? (__ffsll((unsigned long long)ends) - 1) : nseeds;
+        int i = -1;
+        if (lane < nlist) {
+            i = T.seed_to_packed[(pc - 2) * kPartitionings + seed];
+            if (i >= nsel) i = -1;
+        }
+        if (i >= 0) {
+            float ue, se;
+            ex_line_fit<PC, kW>(PX, T, pc, i, rgba, wie, &ue, &se);
+            s_eu[lane] = ue;
+            s_es[lane] = se;
+        }
+        s_i[lane] = i;
+        __syncthreads();
+        float ube[4] = {1e30f, 1e30f, 1e30f, 1e30f}, sbe[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+        int ubi[4] = {0, 0, 0, 0}, sbi[4] = {0, 0, 0, 0};
+        int nvalid = 0;
+        for (int q = 0; q < nlist; ++q) {
+            const int iq = s_i[q];
+            if (iq < 0) continue;
+            ++nvalid;
+            ex_insert(s_eu[q], iq, ube, ubi);
+            ex_insert(s_es[q], iq, sbe, sbi);
+        }
+        const int req = min(nvalid, 4);
+        int pos[4] = {-1, -1, -1, -1}, emitted = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int idx = (j & 1) ? sbi[j >> 1] : ubi[j >> 1];
+            bool dup = emitted >= req || (j >> 1) >= req;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dup |= q < emitted && pos[q] == idx;
+            if (!dup) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q == emitted) pos[q] = idx;
+                ++emitted;
+            }
+        }
+        if (lane < 4) pb_part[k * 4 + lane] = (int16_t)(lane == 0 ? pos[0] : lane == 1 ? pos[1] : lane == 2 ? pos[2] : pos[3]);
+        __syncthreads();  // the next block reuses the LDS
+    }
+}
+
 void launch_partition_ex(bool seeded, int pc, const DevTables& T, dim3 grid, size_t lds, hipStream_t s,
                          const ImageArgs& A, int16_t* pb_part, uint8_t* scratch, const int* list, const int* list_count,
                          int search_limit, const SearchPlan& P) {
@@ -4534,9 +4634,23 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuff
                                         plan->b3_seed + (pc - 2) * 1024, plan->seeds, const_cast<int*>(plan->topk_idx),
                                         5 * plan->kmax, (pc - 1) * plan->kmax, s) != ASTCG_OK)
                     return hipErrorLaunchFailure;
-                if (plan->seeds > 4)
+                if (plan->seeds > 4) {
+                    // a lane per block for a long list, a wave per block for a
+                    // short one (each kernel reads the list length)
                     launch_partition_ex(true, pc, T, pgrid, partition_ex_lds_bytes(T), s, A, B.pb_part,
                                         reinterpret_cast<uint8_t*>(B.ccr), list, list_count, 0, *plan);
+                    const dim3 sgrid((unsigned)std::min(A.nb, kExwGrid));
+                    const size_t slds = partition_exs_lds_bytes(T);
+#define ASTCG_PEXS(PCV)                                                                                              \
+    if (T.tc <= 64) hipLaunchKernelGGL((k_partition_exs<PCV, 1>), sgrid, dim3(64), slds, s, A, T, B.pb_part, list,  \
+                                       list_count, *plan);                                                          \
+    else hipLaunchKernelGGL((k_partition_exs<PCV, (kMaxTexels + 63) / 64>), sgrid, dim3(64), slds, s, A, T,         \
+                            B.pb_part, list, list_count, *plan)
+                    if (pc == 2) { ASTCG_PEXS(2); }
+                    else if (pc == 3) { ASTCG_PEXS(3); }
+                    else { ASTCG_PEXS(4); }
+#undef ASTCG_PEXS
+                }
             }
             if (plan) {
                 hipLaunchKernelGGL(k_plan, dim3(blocks_for(A.nb)), dim3(64), 0, s, A, T, pc, B.pb_part, *plan,

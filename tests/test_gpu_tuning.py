@@ -66,6 +66,24 @@ def test_exhaustive_list_kernels_agree():
         enc.close()
 
 
+def test_seeded_list_kernels_agree():
+    """The learned policy's seeded candidates (48 seeds ranked by line fits)
+    through a wave per listed block (lists up to exw_max_list) or a lane per
+    block give the same blocks."""
+    img = synth.gradient_image(320, 256, 36)
+    enc = A.Encoder(0)
+    try:
+        for bx, by in ((4, 4), (6, 6)):
+            enc.set_tuning(bx, by, exw_max_list=1 << 20)
+            wave = _encode(enc, img, bx, by, A.POLICY_LEARNED, seeds=48)
+            enc.set_tuning(bx, by, exw_max_list=0)
+            lane = _encode(enc, img, bx, by, A.POLICY_LEARNED, seeds=48)
+            enc.set_tuning(bx, by, exw_max_list=65536)
+            assert np.array_equal(wave, lane), (bx, by)
+    finally:
+        enc.close()
+
+
 def test_autotune_cache_round_trip(tmp_path):
     cache = str(tmp_path / "tune.txt")
     img = synth.gradient_image(160, 128, 35)
