@@ -52,7 +52,7 @@ struct DevTables {
     // launch shapes (astcg_tuning), none of which changes a result
     int refine_stage_max_tc;      // k_refine stages the block's RGBA8 texels in LDS up to this texel count
     int ex_f4_max_tc;             // k_partition_ex stages float texels in LDS up to this count (<= 64)
-    int exw_max_list;             // exhaustive block lists up to this length run k_partition_exw
+    int exw_max_list;             // block lists up to this length run k_partition_exw / k_partition_exs
     int refine_queue;             // 0: k_refine, a workgroup per work-list tile; n >= 1: the persistent
                                   // k_refine_q, whose waves refill once n of their lanes are free
     int refine_classes;           // 1: the work list is bucketed by trial mode and format class; 0: by mode

@@ -117,8 +117,9 @@ typedef struct {
                                      at most this many texels (default 25) */
     uint32_t ex_f4_max_tc;        /* the exhaustive partition search stages float texels in LDS up to this many
                                      texels, at most 64 (default 16); RGBA8 words above */
-    uint32_t exw_max_list;        /* exhaustive searches of block lists (LEARNED hard / fallback blocks) up to this
-                                     many blocks run one wave per block, longer ones one lane per block
+    uint32_t exw_max_list;        /* partition searches of block lists (LEARNED: the exhaustive search of hard /
+                                     fallback blocks and the seeded search of listed blocks) up to this many
+                                     blocks run one wave per block, longer ones one lane per block
                                      (default 65536) */
     uint32_t refine_queue;        /* 0: the refinement runs a workgroup per 64 candidates of one block mode (default);
                                      n >= 1: a persistent work-queue kernel whose waves take those 64-candidate
