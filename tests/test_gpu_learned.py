@@ -61,10 +61,32 @@ def test_trained_network_beats_uninformed_pruning(setup, p_exhaustive, prune):
 
 
 def test_default_learned_policy_quality(setup):
+    """The library's default learned parameters (16 modes, 5 slots, the
+    COMPAT positions, no seeds) stay within 0.1 dB of the COMPAT search they
+    prune."""
     enc, img, params, hidden, p_plain = setup
     p_net, fb = _learned_psnr(enc, img, params, hidden)
     assert p_plain - p_net <= 0.1, (p_plain, p_net)
     assert fb < 0.05
+
+
+@pytest.mark.parametrize("bx", [4, 6])
+def test_shipped_settings_within_bound_of_exhaustive(bx, record_property):
+    """The shipped predictors with the pruning saved in their files, against
+    the EXHAUSTIVE search on a 512x512 image none of them was trained or
+    tuned on: within 0.05 dB (decoded RGBA PSNR)."""
+    img = synth.gradient_image(512, 512, 3)
+    enc = A.Encoder(0)
+    try:
+        enc.configure(512, 512, bx, bx, policy=A.POLICY_EXHAUSTIVE)
+        p_ex = A.psnr(enc.decode(bx, bx, 512, 512, enc.encode(img)), img)
+        enc.load_predictor(bx, bx)
+        enc.configure(512, 512, bx, bx, policy=A.POLICY_LEARNED)
+        p_learned = A.psnr(enc.decode(bx, bx, 512, 512, enc.encode(img)), img)
+        record_property("dpsnr_db", p_ex - p_learned)
+        assert p_ex - p_learned <= 0.05, (bx, p_ex, p_learned)
+    finally:
+        enc.close()
 
 
 def _bf16(a):
