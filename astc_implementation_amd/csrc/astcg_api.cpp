@@ -406,7 +406,10 @@ hipError_t upload_tables(FpState& F) {
     for (int i = 0; i < 4; ++i) T.count_selected[i] = f.count_selected[i];
     T.refine_stage_max_tc = kRefineStageMaxTc;
     T.ex_f4_max_tc = kExF4MaxTc;
-    T.exw_max_list = kExwMaxList;
+    // from 36 texels per block a wave per listed block keeps up with a lane
+    // per block on long lists too (6x6 learned 4096^2: -1 %); at 4x4 the
+    // lane kernel wins once its waves fill the chip (+2 % without the cut)
+    T.exw_max_list = f.texel_count >= kExwAllListsTc ? kExwAllLists : kExwMaxList;
     T.refine_queue = kRefineQueue;
     // grouping by format class pays where the refinement stages its texels
     // (4x4 4096^2 images: -10 % refinement time); unstaged, the classes split

@@ -62,6 +62,8 @@ struct DevTables {
 constexpr int kRefineStageMaxTc = 25;  // larger blocks ran faster unstaged, with more workgroups per CU
 constexpr int kExF4MaxTc = 16;         // keeps 6 k_partition_ex workgroups per CU
 constexpr int kExwMaxList = 65536;     // lane-per-block throughput is higher once its waves fill the chip
+constexpr int kExwAllListsTc = 36;     // from this many texels per block every list runs a wave per block
+constexpr int kExwAllLists = 1 << 20;
 constexpr int kRefineQueue = 0;        // the persistent refinement measured slower on the 4096^2 images
 
 // One texture of a launch: its blocks occupy [first_block, first_block +

@@ -120,7 +120,7 @@ typedef struct {
     uint32_t exw_max_list;        /* partition searches of block lists (LEARNED: the exhaustive search of hard /
                                      fallback blocks and the seeded search of listed blocks) up to this many
                                      blocks run one wave per block, longer ones one lane per block
-                                     (default 65536) */
+                                     (default 65536; 1048576 for footprints of 36 texels or more) */
     uint32_t refine_queue;        /* 0: the refinement runs a workgroup per 64 candidates of one block mode (default);
                                      n >= 1: a persistent work-queue kernel whose waves take those 64-candidate
                                      tiles from a queue and give a lane the next candidate of its tile once n lanes

@@ -33,7 +33,8 @@ def test_every_tuning_gives_the_same_blocks(bx, by):
     try:
         base = {p: _encode(enc, img, bx, by, p) for p in (A.POLICY_COMPAT, A.POLICY_EXHAUSTIVE)}
         default = enc.tuning(bx, by)
-        assert default == {"refine_stage_max_tc": 25, "ex_f4_max_tc": 16, "exw_max_list": 65536, "refine_queue": 0,
+        assert default == {"refine_stage_max_tc": 25, "ex_f4_max_tc": 16,
+                           "exw_max_list": 1 << 20 if bx * by >= 36 else 65536, "refine_queue": 0,
                            "refine_classes": int(bx * by <= 25)}
         for t in ({"refine_stage_max_tc": 0}, {"refine_stage_max_tc": 144}, {"ex_f4_max_tc": 0},
                   {"ex_f4_max_tc": 64}, {"refine_queue": 16}, {"refine_queue": 1}, {"refine_queue": 64},
@@ -74,12 +75,14 @@ def test_seeded_list_kernels_agree():
     enc = A.Encoder(0)
     try:
         for bx, by in ((4, 4), (6, 6)):
+            default = enc.tuning(bx, by)["exw_max_list"]
             enc.set_tuning(bx, by, exw_max_list=1 << 20)
             wave = _encode(enc, img, bx, by, A.POLICY_LEARNED, seeds=48)
             enc.set_tuning(bx, by, exw_max_list=0)
             lane = _encode(enc, img, bx, by, A.POLICY_LEARNED, seeds=48)
-            enc.set_tuning(bx, by, exw_max_list=65536)
+            enc.set_tuning(bx, by, exw_max_list=default)
             assert np.array_equal(wave, lane), (bx, by)
+            assert np.array_equal(wave, _encode(enc, img, bx, by, A.POLICY_LEARNED, seeds=48)), (bx, by)
     finally:
         enc.close()
 
