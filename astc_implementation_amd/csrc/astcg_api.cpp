@@ -440,6 +440,7 @@ size_t chunk_bytes_per_block(const Footprint& f, size_t* offs, int nb) {
     offs[8] = L.add<int>(2 * bins + 2);
     offs[9] = L.add<int>(npb * 8 + bins * 64);
     offs[11] = L.add<float>(npb * 8);
+    offs[12] = L.add<uint32_t>((size_t)nb * f.texel_count);
     return L.size;
 }
 
@@ -544,7 +545,7 @@ astcg_status get_footprint(astcg_encoder* e, uint32_t bx, uint32_t by, const FpS
 size_t plan_bytes(const Footprint& f, const PredState* P, int nb);
 
 int chunk_size(const Footprint& f, uint32_t nblocks, uint32_t requested, const PredState* pred) {
-    size_t offs[12];
+    size_t offs[13];
     const size_t fixed = chunk_bytes_per_block(f, offs, 0) + plan_bytes(f, pred, 0);
     const size_t per_block = (chunk_bytes_per_block(f, offs, 1 << 16) + plan_bytes(f, pred, 1 << 16) - fixed) >> 16;
     const size_t max_chunk = (((size_t)1 << 31) - 1) / (4 * (size_t)std::max(kColorTabRows, f.nbm) + 8 * 4);
@@ -573,6 +574,7 @@ ChunkBuffers chunk_buffers_at(uint8_t* b, const FpState& F, const size_t* offs) 
     B.work_total = B.bin_cur + (size_t)F.fp->nbm * kRefineClasses;
     B.tile_next = B.work_total + 1;
     B.work = reinterpret_cast<int*>(b + offs[9]);
+    B.texels = reinterpret_cast<uint32_t*>(b + offs[12]);
     B.wstride = F.wstride;
     return B;
 }
@@ -582,7 +584,7 @@ ChunkBuffers chunk_buffers_at(uint8_t* b, const FpState& F, const size_t* offs) 
 // layout changes (k_bins clears them after every use otherwise).
 astcg_status layout_chunk(astcg_encoder* e, const FpState& F, int chunk, hipStream_t s) {
     if (e->layout_fp == &F && e->layout_chunk == chunk) return ASTCG_OK;
-    size_t offs[12];
+    size_t offs[13];
     const size_t total = chunk_bytes_per_block(*F.fp, offs, chunk);
     hipError_t err = grow(&e->d_chunk, &e->chunk_cap, total);
     if (err != hipSuccess) {
@@ -1080,7 +1082,7 @@ astcg_status astcg_encode_device_noalloc(astcg_encoder* e, const uint8_t* d_rgba
     const FpState& F = *e->cur;
     const Footprint& f = *F.fp;
     // chunk: as many blocks as fit in the scratch with their SymBest records
-    size_t offs[12];
+    size_t offs[13];
     const size_t fixed = chunk_bytes_per_block(f, offs, 0);
     const size_t per_block = ((chunk_bytes_per_block(f, offs, 1 << 16) - fixed) >> 16) + sizeof(SymBest);
     // exact fit: the sub-arrays' 256-byte alignment and the SymBest run

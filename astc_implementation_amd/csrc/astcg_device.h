@@ -91,6 +91,9 @@ struct ImageArgs {
     const int* pb_list;
     const int* pb_inv;        // [nb * 4] list index of each position, -1 = not searched
     const int* pb_count;      // entries in pb_list
+    // [nb][texel count] RGBA8 texels of the chunk's blocks, edge-clamped,
+    // one contiguous row per block (k_block_texels; set by launch_chunk)
+    const uint32_t* blk_texels;
 };
 
 // Search outcome recorded for training the candidate predictor (all over
@@ -228,6 +231,7 @@ struct ChunkBuffers {
     int* work_total;    // [1] work-list length including padding
     int* tile_next;     // [1] next work-list tile of the persistent refinement's queue (zeroed by k_bins)
     int* work;          // [4 * max_nb * 8 + nbm * kRefineClasses * 64] (partitioned block * 8 + slot), bucketed
+    uint32_t* texels;   // [max_nb][texel count] the chunk's block texels (ImageArgs::blk_texels)
     int wstride;        // LDS row of quantised weights: max weight count of the trial grids, multiple of 4
 };
 

@@ -183,6 +183,18 @@ __device__ __forceinline__ void texel_px(const BlockLoc& l, const DevTables& T, 
     o[3] = u8_search_value(v >> 24);
 }
 
+// The chunk's block texels as one contiguous row per block ([block][texel],
+// edge-clamped), so the kernels that walk a partitioned block's texels many
+// times read a few cache lines with no coordinate arithmetic.  A thread per
+// texel.
+__global__ void __launch_bounds__(256) k_block_texels(ImageArgs A, DevTables T, uint32_t* out) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int tc = T.tc;
+    if (e >= (size_t)A.nb * tc) return;
+    const int k = (int)(e / tc), t = (int)(e - (size_t)k * tc);
+    out[e] = texel_rgba8(block_loc(A, T, A.blk0 + k), T, t);
+}
+
 struct Pb {
     int g;          // block index of the launch
     BlockLoc loc;
@@ -2512,7 +2524,8 @@ struct RefineCand {
     int ql, qlm;        // colour quant level and its mod variant (pass11)
     uint16_t in_formats;
     int ppc[4];         // texels per partition
-    BlockLoc loc;       // texel origin (unstaged kernels read the image)
+    BlockLoc loc;         // texel origin in the image (unstaged 1-partition kernels read it in every pass)
+    const uint32_t* tex;  // the block's texel row (unstaged partitioned kernels read it in every pass)
     float4 ep0[4], ep1[4];
     float sdirs[4][3];  // pass13's scaled direction: depends on the average only
     float best_err;
@@ -2556,11 +2569,12 @@ __device__ __forceinline__ void refine_init(const ImageArgs& A, const DevTables&
     const int pb = task >> 3;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
     for (int i = 0; i < tc; ++i) {
-        if (kStage) R.tex[i * 64] = texel_rgba8(b.loc, T, i);
+        if (kStage) R.tex[i * 64] = A.blk_texels[(size_t)(b.g - A.blk0) * tc + i];
         if (pc > 1) R.tp[i * 64] = (uint8_t)texel_part(b, i);
     }
     C.task = task;
     C.loc = b.loc;
+    C.tex = A.blk_texels + (size_t)(b.g - A.blk0) * tc;
 #pragma unroll
     for (int i = 0; i < 4; ++i) C.ppc[i] = b.ppc[i];
     const Ideal& I = ideal[pb];
@@ -2611,9 +2625,11 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
             return make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
                                u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
         }
-        float px[4];
-        texel_px(C.loc, T, i, px);
-        return make_float4(px[0], px[1], px[2], px[3]);
+        // one partition: neighbouring lanes' blocks share the image's cache
+        // lines (measured faster than the block rows); partitioned: the rows
+        const uint32_t v = PC == 1 ? texel_rgba8(C.loc, T, i) : C.tex[i];
+        return make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
+                           u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
     };
     auto part_of = [&](int i) -> int { return pc > 1 ? (int)R.tp[i * 64] : 0; };
     uint8_t* qw = R.qw;
@@ -4550,10 +4566,16 @@ void launch_partition_ex(bool seeded, int pc, const DevTables& T, dim3 grid, siz
 #undef ASTCG_PEX
 }
 
-hipError_t launch_chunk(const DevTables& T, const ImageArgs& A0, const ChunkBuffers& B, SymBest* best, int max_pc,
+hipError_t launch_chunk(const DevTables& T, const ImageArgs& A_in, const ChunkBuffers& B, SymBest* best, int max_pc,
                         hipStream_t s, Profiler* prof, const LabelBufs* labels, const SearchPlan* plan,
                         bool dual_plane, int exhaustive) {
     (void)hipGetLastError();  // the final check reports this call's launches only
+    ImageArgs A0 = A_in;
+    A0.blk_texels = B.texels;
+    {
+        const size_t n = (size_t)A0.nb * T.tc;
+        hipLaunchKernelGGL(k_block_texels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A0, T, B.texels);
+    }
     const LabelBufs lab = labels ? *labels : LabelBufs{nullptr, nullptr, nullptr};
     const int threads = 64;
     auto blocks_for = [&](long long n) { return (unsigned)((n + threads - 1) / threads); };
