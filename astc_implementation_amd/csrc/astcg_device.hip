@@ -1161,6 +1161,20 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     if (pb >= pb_limit(A, pc, npb)) return;
     const Pb b = make_pb(A, T, pb_part, pc, pb);
     const int tc = T.tc;
+    // partitioned blocks: the block's texel row (its 4 positions sit in
+    // neighbouring lanes); one partition reads the image like its neighbours
+    const uint32_t* row = A.blk_texels + (size_t)(b.g - A.blk0) * tc;
+    auto px_at = [&](int i, float* o) {
+        if (PC == 1) {
+            texel_px(b.loc, T, i, o);
+        } else {
+            const uint32_t v = row[i];
+            o[0] = u8_search_value(v & 0xFF);
+            o[1] = u8_search_value((v >> 8) & 0xFF);
+            o[2] = u8_search_value((v >> 16) & 0xFF);
+            o[3] = u8_search_value(v >> 24);
+        }
+    };
     // pass01_ideal_endpoints_and_weights.wgsl:63-207
     const float error_weight = (((kCW[0] + kCW[1]) + kCW[2]) + kCW[3]) / 4.0f;
     float sum[4][4] = {}, count[4] = {};
@@ -1168,7 +1182,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     float amin = 1e38f, amax = 0.0f;
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(b.loc, T, i, px);
+        px_at(i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -1187,7 +1201,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     float dv[4][4] = {};
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(b.loc, T, i, px);
+        px_at(i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -1199,7 +1213,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     for (int p = 0; p < 4; ++p) { normalize4(dv[p], dir[p]); minp[p] = 1e10f; maxp[p] = -1e10f; }
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(b.loc, T, i, px);
+        px_at(i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -1227,7 +1241,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     }
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(b.loc, T, i, px);
+        px_at(i, px);
         const int p = texel_part(b, i);
         float w = 0.0f;
 #pragma unroll
@@ -1256,7 +1270,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     float sxp[4][4] = {}, syp[4][4] = {}, szp[4][4] = {};
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(b.loc, T, i, px);
+        px_at(i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -1293,7 +1307,7 @@ __global__ void __launch_bounds__(64) k_ideal_color(ImageArgs A, DevTables T, co
     float acc[4][5] = {};
     for (int i = 0; i < tc; ++i) {
         float px[4];
-        texel_px(b.loc, T, i, px);
+        px_at(i, px);
         const int p = texel_part(b, i);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
