@@ -1590,37 +1590,44 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
         }
         for (int w = 0; w < wc; ++w) dw[w * 64] = aux[w * 64];
     } else {
-        for (int w = 0; w < wc; ++w) {
-            const int n = w2tc[w];
-            float v = 0.0f;
-            if (n != 0) {
-                const int off = w2to[w];
-                float ws = 0.0f, tcon = 1e-10f;
-                for (int jx = 0; jx < n; ++jx) {
-                    const int t = w2ti[off + jx];
-                    const float cw = w2tw[off + jx] * wes_of(t);
-                    ws += wts_of(t) * cw;
-                    tcon += cw;
-                }
-                v = ws / tcon;
+        // The block does not fit the lane's rows, so the per-weight sums are
+        // taken in one walk over the texels, each texel adding its terms to
+        // the weights it touches: a weight's texels are listed in increasing
+        // texel order and both tables hold the same contribution, so every
+        // sum takes the same terms in the same order as a walk over the
+        // weight's own texels, with one ideal-weight load and one infill per
+        // texel instead of one per weight-texel pair.
+        // initial estimates: sums in aux, contribution totals in dw
+        for (int w = 0; w < wc; ++w) { aux[w * 64] = 0.0f; dw[w * 64] = 1e-10f; }
+        for (int t = 0; t < tc; ++t) {
+            const int nt = t2wc[t];
+            const float wt = wts_of(t), es = wes_of(t);
+            for (int k = 0; k < nt; ++k) {
+                const int w = t2wi[t * 4 + k];
+                const float cw = t2ww[t * 4 + k] * es;
+                aux[w * 64] += wt * cw;
+                dw[w * 64] += cw;
             }
-            aux[w * 64] = v;  // initial estimate
         }
-        // one refinement step per weight against the infill of the initial estimate
+        for (int w = 0; w < wc; ++w) aux[w * 64] = w2tc[w] != 0 ? aux[w * 64] / dw[w * 64] : 0.0f;
+        // one refinement step per weight against the infill of the initial
+        // estimate: the gradient sums in dw, then each weight's curvature
+        for (int w = 0; w < wc; ++w) dw[w * 64] = 0.0f;
+        for (int t = 0; t < tc; ++t) {
+            const int nt = t2wc[t];
+            float infill = 0.0f;
+            for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
+            const float r = infill - wts_of(t), es = wes_of(t);
+            for (int k = 0; k < nt; ++k) dw[t2wi[t * 4 + k] * 64] += r * (es * t2ww[t * 4 + k]);
+        }
         for (int w = 0; w < wc; ++w) {
-            float a = 1e-10f, bacc = 0.0f;
+            float a = 1e-10f;
             const int n = w2tc[w], off = w2to[w];
             for (int jx = 0; jx < n; ++jx) {
-                const int t = w2ti[off + jx];
                 const float contrib = w2tw[off + jx];
-                const float scale = wes_of(t) * contrib;
-                float infill = 0.0f;
-                const int nt = t2wc[t];
-                for (int k = 0; k < nt; ++k) infill += aux[t2wi[t * 4 + k] * 64] * t2ww[t * 4 + k];
-                a += contrib * scale;
-                bacc += (infill - wts_of(t)) * scale;
+                a += contrib * (wes_of(w2ti[off + jx]) * contrib);
             }
-            dw[w * 64] = aux[w * 64] + wclamp((bacc * -16.0f) / a, -0.25f, 0.25f);
+            dw[w * 64] = aux[w * 64] + wclamp((dw[w * 64] * -16.0f) / a, -0.25f, 0.25f);
         }
     }
     // passes 03-05 (pass03_compute_angular_offsets.wgsl:65-116,
