@@ -1760,6 +1760,13 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
             // only whole groups of four weights reach the output (pass07:322-331)
             const int keep = (wc / 4) * 4;
             uint32_t word = 0;
+            // an undecimated grid's weight i is texel i's whole infill (one
+            // contribution of 1.0), and dw still holds the ideal weights, so
+            // texel i's error is summed here, in the same order, with the
+            // same value: (0 + q * 1) - w differs from q - w at most in the
+            // sign of a zero, which squaring removes
+            const bool undecimated = tc == wc;
+            float err = 0.0f;
             for (int i = 0; i < wc; ++i) {
                 const float ix = wclamp(dw[i * 64] * scale - slb, 0.0f, 1.0f);
                 const float ix1 = ix * qlm1;
@@ -1773,10 +1780,15 @@ __global__ void __launch_bounds__(64) k_dm_bm(ImageArgs A, DevTables T, const in
                     qwo[i >> 2] = word;
                     word = 0;
                 }
-                aux[i * 64] = (use_high ? fh : fl) * rscale + lo;
+                const float qv = (use_high ? fh : fl) * rscale + lo;
+                if (undecimated) {
+                    const float diff = qv - dw[i * 64];
+                    err += diff * diff * wes_of(i);
+                } else {
+                    aux[i * 64] = qv;
+                }
             }
-            float err = 0.0f;
-            for (int i = 0; i < tc; ++i) {
+            for (int i = 0; i < tc && !undecimated; ++i) {
                 float v = 0.0f;
                 for (int jx = 0; jx < t2wc[i]; ++jx) v += aux[t2wi[i * 4 + jx] * 64] * t2ww[i * 4 + jx];
                 const float diff = v - wts_of(i);
