@@ -2808,25 +2808,60 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
             R.ue1[p * 64] = make_uchar4(o1[0], o1[1], o1[2], o1[3]);
         }
     }
-    // the endpoints of texel t's partition as the texel loops use them:
-    // decoded values (x 257) and the interpolation base and step per
-    // channel; with one partition they stay in registers
-    auto ep_at = [&](int t, int* e0, int* e1, float* bs, float* of) {
-        int4 u0, u1;
-        if (PC == 1) {
-            u0 = make_int4(he0[0], he0[1], he0[2], he0[3]);
-            u1 = make_int4(he1[0], he1[1], he1[2], he1[3]);
+    // the endpoints of texel t's partition as the texel loops use them: the
+    // interpolation base (decoded endpoint 0, x 257) and step per channel;
+    // with one or two partitions they are formed once per round and stay
+    // in registers, with more they are decoded per texel from LDS
+    float pbs[2][4], pof[2][4];
+    if (PC <= 2) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            if (p >= pc) continue;
+            int4 u0, u1;
+            if (PC == 1) {
+                u0 = make_int4(he0[0], he0[1], he0[2], he0[3]);
+                u1 = make_int4(he1[0], he1[1], he1[2], he1[3]);
+            } else {
+                u0 = load_ue(R.ue0[p * 64]);
+                u1 = load_ue(R.ue1[p * 64]);
+            }
+            const int a0[4] = {u0.x, u0.y, u0.z, u0.w}, a1[4] = {u1.x, u1.y, u1.z, u1.w};
+            for (int j = 0; j < 4; ++j) {
+                pbs[p][j] = (float)a0[j];
+                pof[p][j] = (float)(a1[j] - a0[j]) * (1.0f / 64.0f);
+            }
+        }
+    }
+    // (e0, e1: the decoded endpoints, kept as integers where they are
+    // decoded per texel)
+    auto ep_at = [&](int t, float* bs, float* of, int* e0, int* e1) {
+        if (PC <= 2) {
+            const bool p1 = PC == 2 && part_of(t) != 0;
+            for (int j = 0; j < 4; ++j) {
+                bs[j] = p1 ? pbs[1][j] : pbs[0][j];
+                of[j] = p1 ? pof[1][j] : pof[0][j];
+            }
         } else {
             const int p = part_of(t);
-            u0 = load_ue(R.ue0[p * 64]);
-            u1 = load_ue(R.ue1[p * 64]);
+            const int4 u0 = load_ue(R.ue0[p * 64]), u1 = load_ue(R.ue1[p * 64]);
+            e0[0] = u0.x; e0[1] = u0.y; e0[2] = u0.z; e0[3] = u0.w;
+            e1[0] = u1.x; e1[1] = u1.y; e1[2] = u1.z; e1[3] = u1.w;
+            for (int j = 0; j < 4; ++j) {
+                bs[j] = (float)e0[j];
+                of[j] = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+            }
         }
-        e0[0] = u0.x; e0[1] = u0.y; e0[2] = u0.z; e0[3] = u0.w;
-        e1[0] = u1.x; e1[1] = u1.y; e1[2] = u1.z; e1[3] = u1.w;
-        for (int j = 0; j < 4; ++j) {
-            bs[j] = (float)e0[j];
-            of[j] = (float)(e1[j] - e0[j]) * (1.0f / 64.0f);
+    };
+    // pass17's reconstruction (e0 (64 - w) + e1 w + 32) >> 6 of channel j.
+    // From registers it runs in float on e0 = bs, e1 = bs + 64 of: every term
+    // is an integer below 2^24 (e <= 65535, w <= 64) and / 64 is a power of
+    // two, so each step is exact and the floor is the shift.
+    auto recon = [&](const float* bs, const float* of, const int* e0, const int* e1, int j, int w0, int w1) {
+        if (PC <= 2) {
+            const float e1f = bs[j] + of[j] * 64.0f;
+            return floorf(((bs[j] * (float)w0 + e1f * (float)w1) + 32.0f) * (1.0f / 64.0f));
         }
+        return (float)((e0[j] * w0 + e1[j] * w1 + 32) >> 6);
     };
     // pass17 (before pass16 on the first round), then pass16 + pass17
     bool moved = false;
@@ -2846,9 +2881,9 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
                     const uint32_t pn = c_qt.prev_next[q][uqw];
                     const int down = (int)(pn & 0xFF), up = (int)((pn >> 8) & 0xFF);
                     const float wb = (float)uqw, dd = (float)(down - uqw), udf = (float)(up - uqw);
-                    int e0[4], e1[4];
                     float bs[4], of[4];
-                    ep_at(t, e0, e1, bs, of);
+                    int e0[4], e1[4];
+                    ep_at(t, bs, of, e0, e1);
                     const float4 px4 = px_of(t);
                     const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                     float s0[4], s1[4], s2[4];
@@ -2867,8 +2902,8 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
                     // pass17_compute_final_error.wgsl:149-231 for this texel
                     float dfe[4];
                     for (int j = 0; j < 4; ++j) {
-                        const int col = (e0[j] * (64 - nq) + e1[j] * nq + 32) >> 6;
-                        const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
+                        const float col = recon(bs, of, e0, e1, j, 64 - nq, nq);
+                        const float df = wmin(fabsf(px[j] - col), 1e15f);
                         dfe[j] = df * df;
                     }
                     total += wmin(dot4(dfe, cw), kErrDefault);
@@ -2894,9 +2929,9 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
                             if (!found && wi == we) { twb = t2ww[t * 4 + j]; found = true; }
                         }
                         const float wdd = ddn * twb, wud = dup * twb;
-                        int e0[4], e1[4];
                         float bs[4], of[4];
-                        ep_at(t, e0, e1, bs, of);
+                        int e0[4], e1[4];
+                        ep_at(t, bs, of, e0, e1);
                         const float4 px4 = px_of(t);
                         const float px[4] = {px4.x, px4.y, px4.z, px4.w};
                         for (int j = 0; j < 4; ++j) {
@@ -2920,16 +2955,16 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
         // pass17_compute_final_error.wgsl:149-231
         for (int i = 0; i < tc && !fused; ++i) {
             const float w = weight_of(i);
-            int e0[4], e1[4];
             float bs[4], of[4];
-            ep_at(i, e0, e1, bs, of);
+            int e0[4], e1[4];
+            ep_at(i, bs, of, e0, e1);
             const float4 px4 = px_of(i);
             const float px[4] = {px4.x, px4.y, px4.z, px4.w};
             const int w1 = wf2i(wround(w * 64.0f)), w0 = 64 - w1;
             float dd[4];
             for (int j = 0; j < 4; ++j) {
-                const int col = (e0[j] * w0 + e1[j] * w1 + 32) >> 6;
-                const float df = wmin(fabsf(px[j] - (float)col), 1e15f);
+                const float col = recon(bs, of, e0, e1, j, w0, w1);
+                const float df = wmin(fabsf(px[j] - col), 1e15f);
                 dd[j] = df * df;
             }
             total += wmin(dot4(dd, cw), kErrDefault);
