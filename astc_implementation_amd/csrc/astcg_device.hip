@@ -857,15 +857,29 @@ __device__ __forceinline__ void ex_line_fit(const PXF& PX, const DevTables& T, i
 // needs (plan_uses_positions, or with `seeded` plan_uses_seeds), compacted
 // into list[0 .. *count) so that the exhaustive search's waves hold only
 // such blocks.
-__global__ void __launch_bounds__(64) k_need_list(ImageArgs A, SearchPlan P, int pc, int seeded, int* list,
-                                                  int* count) {
-    const int k = blockIdx.x * 64 + threadIdx.x;
+// One counter update per 256 blocks (the waves' counts summed in LDS): the
+// list's order is free, since every kernel that reads it handles each
+// listed block on its own.
+constexpr int kNeedListThreads = 256;
+__global__ void __launch_bounds__(kNeedListThreads) k_need_list(ImageArgs A, SearchPlan P, int pc, int seeded, int* list,
+                                                                int* count) {
+    __shared__ int s_cnt[kNeedListThreads / 64];
+    __shared__ int s_base;
+    const int k = blockIdx.x * kNeedListThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool need = k < A.nb && (seeded ? plan_uses_seeds(P, pc, k, A.blk0 + k) : plan_uses_positions(P, pc, k, A.blk0 + k));
     const unsigned long long m = __ballot(need);
-    int base = 0;
-    if (threadIdx.x == 0 && m) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, 0);
-    if (need) list[base + __popcll(m & ((1ull << threadIdx.x) - 1ull))] = k;
+    if (lane == 0) s_cnt[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < kNeedListThreads / 64; ++w) total += s_cnt[w];
+        s_base = total ? atomicAdd(count, total) : 0;
+    }
+    __syncthreads();
+    int base = s_base;
+    for (int w = 0; w < wave; ++w) base += s_cnt[w];
+    if (need) list[base + __popcll(m & ((1ull << lane) - 1ull))] = k;
 }
 
 // Exhaustive lists longer than T.exw_max_list run on k_partition_ex (a lane
@@ -4678,7 +4692,8 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A_in, const ChunkBu
                     list = plan->pb_list;
                     list_count = plan->pb_count + 3;
                     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, s, list_count, 1);
-                    hipLaunchKernelGGL(k_need_list, pgrid, dim3(64), 0, s, A, *plan, pc, 0, list, list_count);
+                    hipLaunchKernelGGL(k_need_list, dim3((A.nb + kNeedListThreads - 1) / kNeedListThreads), dim3(kNeedListThreads), 0, s, A,
+                                       *plan, pc, 0, list, list_count);
                 }
                 if (plan) {
                     // hard and fallback blocks: a wave per block for a short
@@ -4718,7 +4733,8 @@ hipError_t launch_chunk(const DevTables& T, const ImageArgs& A_in, const ChunkBu
                 int* list = plan->pb_list + A.nb;
                 int* list_count = plan->pb_count + 4;
                 hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, s, list_count, 1);
-                hipLaunchKernelGGL(k_need_list, pgrid, dim3(64), 0, s, A, *plan, pc, 1, list, list_count);
+                hipLaunchKernelGGL(k_need_list, dim3((A.nb + kNeedListThreads - 1) / kNeedListThreads), dim3(kNeedListThreads), 0, s, A,
+                                   *plan, pc, 1, list, list_count);
                 if (plan->seed_topk &&
                     astcg_mlp_seed_topk(list, list_count, A.nb, plan->h2, plan->w3_seed + (size_t)(pc - 2) * 1024 * plan->hidden,
                                         plan->b3_seed + (pc - 2) * 1024, plan->seeds, const_cast<int*>(plan->topk_idx),
