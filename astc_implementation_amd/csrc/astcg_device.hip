@@ -2545,10 +2545,12 @@ struct RefineRows {
     uint8_t* tp;    // [tc] texel partitions (pc > 1)
     uint8_t* qw;    // [wstride] quantised weights of the current round
     uint8_t* bqw;   // [wstride] quantised weights of the best round
+    const float* u8f;  // [256] texel channel byte -> search value (u8_search_value), shared by the lanes
 };
 
 template <bool kStage, int PC>
-__device__ __forceinline__ RefineRows refine_rows(uint8_t* smem, const DevTables& T, int wstride, int lane) {
+__device__ __forceinline__ RefineRows refine_rows(uint8_t* smem, const DevTables& T, int wstride, int lane,
+                                                  const float* u8f) {
     const int tc = T.tc;
     uint32_t* s_tex = reinterpret_cast<uint32_t*>(smem);
     uchar4* s_ue0 = reinterpret_cast<uchar4*>(smem + (kStage ? (size_t)tc * 64 * sizeof(uint32_t) : 0));
@@ -2563,6 +2565,7 @@ __device__ __forceinline__ RefineRows refine_rows(uint8_t* smem, const DevTables
     r.tp = s_tp + lane;
     r.qw = s_qw + lane;
     r.bqw = s_bqw + lane;
+    r.u8f = u8f;
     return r;
 }
 
@@ -2666,17 +2669,13 @@ __device__ __forceinline__ bool refine_round(const DevTables& T, const RefineMod
                                              RefineCand& C) {
     constexpr int pc = PC;
     const int tc = T.tc;
+    // a texel's channels as search values, through the workgroup's table
+    // (four LDS reads instead of the conversion's arithmetic)
     auto px_of = [&](int i) -> float4 {
-        if (kStage) {
-            const uint32_t v = R.tex[i * 64];
-            return make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
-                               u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
-        }
-        // one partition: neighbouring lanes' blocks share the image's cache
-        // lines (measured faster than the block rows); partitioned: the rows
-        const uint32_t v = PC == 1 ? texel_rgba8(C.loc, T, i) : C.tex[i];
-        return make_float4(u8_search_value(v & 0xFF), u8_search_value((v >> 8) & 0xFF),
-                           u8_search_value((v >> 16) & 0xFF), u8_search_value(v >> 24));
+        // one partition unstaged: neighbouring lanes' blocks share the
+        // image's cache lines (measured faster than the block rows)
+        const uint32_t v = kStage ? R.tex[i * 64] : PC == 1 ? texel_rgba8(C.loc, T, i) : C.tex[i];
+        return make_float4(R.u8f[v & 0xFF], R.u8f[(v >> 8) & 0xFF], R.u8f[(v >> 16) & 0xFF], R.u8f[v >> 24]);
     };
     auto part_of = [&](int i) -> int { return pc > 1 ? (int)R.tp[i * 64] : 0; };
     uint8_t* qw = R.qw;
@@ -3027,11 +3026,14 @@ k_refine(ImageArgs A, DevTables T, const int16_t* pb_part, int npb, const Ideal*
     const int lane = threadIdx.x;
     const int base = blockIdx.x * 64;
     if (base >= *work_total) return;
+    __shared__ float s_u8f[256];
+    for (int e = lane; e < 256; e += 64) s_u8f[e] = u8_search_value((uint32_t)e);
+    __syncthreads();
     const int task = work[base + lane];
     const int k = __builtin_amdgcn_readfirstlane((int)top[__builtin_amdgcn_readfirstlane(work[base])]) >>
                   kRefineClassBits;
     if (task < 0) return;
-    const RefineRows R = refine_rows<kStage, PC>(smem, T, wstride, lane);
+    const RefineRows R = refine_rows<kStage, PC>(smem, T, wstride, lane, s_u8f);
     const RefineMode M = refine_mode(T, k);
     RefineCand C;
     refine_init<kStage, PC>(A, T, pb_part, npb, ideal, ccr, qw_all, M, R, task, C);
@@ -3058,7 +3060,10 @@ k_refine_q(ImageArgs A, DevTables T, const int16_t* pb_part, int npb, const Idea
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const int ntiles = __builtin_amdgcn_readfirstlane(*work_total) >> 6;
-    const RefineRows R = refine_rows<kStage, PC>(smem, T, wstride, lane);
+    __shared__ float s_u8f[256];
+    for (int e = lane; e < 256; e += 64) s_u8f[e] = u8_search_value((uint32_t)e);
+    __syncthreads();
+    const RefineRows R = refine_rows<kStage, PC>(smem, T, wstride, lane, s_u8f);
     RefineMode M{};
     int k = -1;                     // the wave's mode, -1 before its first tile
     int pool = 0, pool_end = 0;     // entries of the current tile not handed out yet
